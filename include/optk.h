@@ -1,0 +1,344 @@
+/*
+ * optk.h -- C ABI of the MI355X sequential ray tracer (liboptk.so).
+ *
+ * Plain pointers and sizes only.  All ray buffers are caller-owned device
+ * memory laid out structure-of-arrays (one contiguous array per component);
+ * the library owns only compiled system descriptors.  Every entry point
+ * returns 0 on success or a negative OPTK_E* code, never throws, and leaves a
+ * message for optk_last_error() (thread-local).  Work is enqueued on the
+ * caller's HIP stream; no entry point synchronises the device except
+ * optk_system_create/destroy (host<->device copies of the descriptor).
+ *
+ * Units: lengths mm, wavelengths nm, angles in radians inside records (the
+ * host layer converts degrees), attenuation 1/mm.
+ *
+ * Reference interface each entry point replaces (sun-data/optika):
+ *   optk_trace       optika/propagators.py:19-74 (propagate_rays /
+ *                    accumulate_rays over AbstractSurface.propagate_rays,
+ *                    optika/surfaces.py:123-198)
+ *   optk_trace_grid  optika/systems/_sequential.py:791-988 (input rays from
+ *                    wavelength/field/pupil, raytrace, rayfunction)
+ *   optk_eval        the per-operator protocols: sags (optika/sags/_abc.py:47-122),
+ *                    apertures (optika/apertures/_apertures.py:68-102),
+ *                    rulings (optika/rulings/_rulings.py:24-221),
+ *                    materials (optika/materials/_materials.py:28-78,
+ *                    _snells_law.py:13-366, _multilayers.py:33-532)
+ *   optk_collect     optika/sensors/_sensors.py:92-171 (sensor binning)
+ */
+#ifndef OPTK_H
+#define OPTK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OPTK_ABI_VERSION 1
+
+/* return codes */
+#define OPTK_OK 0
+#define OPTK_EINVAL -1  /* bad argument */
+#define OPTK_EDEVICE -2 /* HIP runtime error */
+#define OPTK_ENOMEM -3  /* device allocation failed */
+
+/* precision of a trace */
+#define OPTK_F64 0 /* float64 throughout: the drop-in form of the reference */
+#define OPTK_F32 1 /* float32 compensated form (float I/O) */
+
+#define OPTK_MAX_ORDERS 8
+#define OPTK_MAX_POLY 8
+
+/* kinds -------------------------------------------------------------------*/
+enum {
+  OPTK_XF_IDENTITY = 0, /* no transformation */
+  OPTK_XF_TRANSLATE = 1,
+  OPTK_XF_AFFINE = 2
+};
+enum {
+  OPTK_SAG_NONE = 0,
+  OPTK_SAG_SPHERE = 1,
+  OPTK_SAG_CONIC = 2,
+  OPTK_SAG_PARABOLA = 3,
+  OPTK_SAG_CYLINDER = 4,
+  OPTK_SAG_TOROID = 5
+};
+enum {
+  OPTK_MATERIAL_VACUUM = 0,
+  OPTK_MATERIAL_MIRROR = 1,
+  OPTK_MATERIAL_MEASURED_MIRROR = 2,
+  OPTK_MATERIAL_GLASS = 3,
+  OPTK_MATERIAL_MULTILAYER_MIRROR = 4,
+  OPTK_MATERIAL_MULTILAYER_FILM = 5
+};
+enum {
+  OPTK_RULINGS_NONE = 0,
+  OPTK_RULINGS_IDEAL = 1,
+  OPTK_RULINGS_MEASURED = 2,
+  OPTK_RULINGS_SINUSOIDAL = 3,
+  OPTK_RULINGS_SQUARE = 4,
+  OPTK_RULINGS_SAWTOOTH = 5,
+  OPTK_RULINGS_TRIANGULAR = 6,
+  OPTK_RULINGS_RECTANGULAR = 7
+};
+enum { OPTK_SPACING_CONSTANT = 0, OPTK_SPACING_POLY1D = 1, OPTK_SPACING_HOLOGRAPHIC = 2 };
+enum {
+  OPTK_APERTURE_NONE = 0,
+  OPTK_APERTURE_CIRCLE = 1,
+  OPTK_APERTURE_SECTOR = 2,
+  OPTK_APERTURE_ELLIPSE = 3,
+  OPTK_APERTURE_RECTANGLE = 4,
+  OPTK_APERTURE_POLYGON = 5
+};
+enum { OPTK_AP_ACTIVE = 1, OPTK_AP_INVERTED = 2, OPTK_AP_DIRECTION = 4 };
+enum {
+  OPTK_PROFILE_NONE = 0,
+  OPTK_PROFILE_ERF = 1,
+  OPTK_PROFILE_EXPONENTIAL = 2,
+  OPTK_PROFILE_LINEAR = 3,
+  OPTK_PROFILE_SINUSOIDAL = 4
+};
+enum { OPTK_BLOCK_SEQUENCE = 0, OPTK_BLOCK_PERIODIC = 1 };
+
+/* records -----------------------------------------------------------------*/
+
+/* p_parent = m * p_local + t (m row-major).  Inverse applied as m^T (p - t).
+ * kind == OPTK_XF_TRANSLATE adds t only (m ignored). */
+typedef struct optk_affine {
+  double m[9];
+  double t[3];
+  int32_t kind;
+  int32_t pad;
+} optk_affine;
+
+/* One optical surface: AbstractSurface (optika/surfaces.py:282-395). */
+typedef struct optk_surface {
+  optk_affine transform;      /* surface frame -> system frame */
+
+  /* sag */
+  int32_t sag_kind;
+  int32_t pad0;
+  double sag_p[4];            /* sphere: R | conic: R, k | parabola: R, -1, f |
+                                 cylinder: R | toroid: R, R_rotation */
+  optk_affine sag_transform;  /* sag frame -> surface frame */
+
+  /* material */
+  int32_t material_kind;
+  int32_t is_mirror;
+  double material_p[6];       /* glass: b1, b2, b3, c1, c2, c3 (c in nm^2) */
+  int32_t material_table;     /* measured mirror: index into tables[] (wavelength
+                                 row then efficiency row), -1 if none */
+  int32_t material_table_len;
+  int32_t stack;              /* multilayer: index into stacks[], -1 if none */
+  int32_t pad1;
+
+  /* rulings */
+  int32_t rulings_kind;
+  int32_t spacing_kind;
+  int32_t num_orders;         /* 1, or the length of the diffraction-order axis */
+  int32_t num_terms;          /* poly1d terms */
+  int32_t orders[OPTK_MAX_ORDERS];
+  double rulings_p[2];        /* depth (nm), duty ratio */
+  double spacing_constant;    /* constant spacing */
+  double wavelength_scale;    /* nm -> spacing length unit */
+  double spacing_normal[3];   /* ruling-plane normal (constant, poly1d) */
+  double poly_coeff[OPTK_MAX_POLY];
+  int32_t poly_power[OPTK_MAX_POLY];
+  double holo_x1[3];
+  double holo_x2[3];
+  double holo_wavelength;
+  int32_t holo_diverging1;
+  int32_t holo_diverging2;
+  optk_affine spacing_transform; /* poly1d argument transformation */
+  int32_t rulings_table;      /* measured rulings: index into tables[], -1 if none */
+  int32_t rulings_table_len;
+
+  /* aperture */
+  int32_t aperture_kind;
+  int32_t aperture_flags;     /* OPTK_AP_* */
+  double aperture_p[4];       /* circle: r | sector: r, a0, a1 (rad) |
+                                 ellipse: rx, ry | rectangle: hx, hy */
+  int32_t polygon_offset;     /* polygon: index into tables[] (x row, y row) */
+  int32_t polygon_count;
+  optk_affine aperture_transform;
+} optk_surface;
+
+/* One homogeneous layer: Layer (optika/materials/_layers.py:134-277). */
+typedef struct optk_layer {
+  int32_t chemical;           /* index into chemicals[], -1 = vacuum (n = 1) */
+  int32_t profile_kind;       /* interface roughness profile, OPTK_PROFILE_* */
+  double thickness;           /* nm */
+  double profile_width;       /* nm */
+} optk_layer;
+
+/* A run of layers: a LayerSequence, or a PeriodicLayerSequence
+ * (optika/materials/_layers.py:473-499, 611-645). */
+typedef struct optk_block {
+  int32_t kind;               /* OPTK_BLOCK_* */
+  int32_t first_layer;        /* index into layers[] */
+  int32_t num_layers;
+  int32_t num_periods;
+} optk_block;
+
+#define OPTK_MAX_BLOCKS 8
+
+/* A multilayer coating: blocks applied in order, then the substrate
+ * (thickness forced to zero, optika/materials/_multilayers.py:186-222). */
+typedef struct optk_stack {
+  int32_t num_blocks;
+  int32_t substrate_layer;    /* index into layers[], -1 = vacuum */
+  optk_block blocks[OPTK_MAX_BLOCKS];
+} optk_stack;
+
+/* An n/k table: rows (wavelength in Angstrom, n, k) stored as three
+ * consecutive rows of `count` doubles starting at tables[offset]. */
+typedef struct optk_chemical {
+  int32_t offset;
+  int32_t count;
+} optk_chemical;
+
+/* The whole system.  Host pointers; optk_system_create copies them. */
+typedef struct optk_system_desc {
+  const optk_surface *surfaces;
+  int32_t num_surfaces;
+  int32_t num_paths;          /* length of the diffraction-order axis (>= 1) */
+  const optk_layer *layers;
+  int32_t num_layers;
+  int32_t num_stacks;
+  const optk_stack *stacks;
+  const optk_chemical *chemicals;
+  int32_t num_chemicals;
+  int32_t pad;
+  const double *tables;
+  int64_t num_tables;         /* doubles in tables[] */
+  optk_affine pre[2];         /* applied to input rays in order: the object's
+                                 transformation (forward) and the system's
+                                 (inverse), or identity */
+  int32_t pre_is_inverse[2];  /* 1: apply pre[i] as an inverse */
+  optk_affine post;           /* applied (inverse) to final rays: the sensor's
+                                 transformation for rayfunction, or identity */
+} optk_system_desc;
+
+typedef struct optk_system optk_system; /* opaque, device-resident */
+
+/* Ray buffers: one device array per component.  For input, a NULL field
+ * takes the matching *_default; for output, a NULL field is not written.
+ * Element type is double (OPTK_F64) or float (OPTK_F32); the mask is uint8. */
+typedef struct optk_rays {
+  void *position[3];
+  void *direction[3];
+  void *wavelength;
+  void *intensity;
+  void *attenuation;
+  void *index_refraction;
+  uint8_t *unvignetted;
+  double wavelength_default;
+  double intensity_default;
+  double attenuation_default;
+  double index_default;
+  int32_t unvignetted_default;
+  int32_t pad;
+} optk_rays;
+
+/* Input grid of SequentialSystem (ObjectVectorArray): field and pupil are
+ * angles in degrees or positions in mm depending on object_at_infinity. */
+typedef struct optk_grid {
+  const void *field[2];
+  const void *pupil[2];
+  const void *wavelength;
+  const void *intensity;
+  double field_default[2];
+  double pupil_default[2];
+  double wavelength_default;
+  double intensity_default;
+  int32_t object_at_infinity;
+  int32_t pad;
+} optk_grid;
+
+/* flags for optk_trace / optk_trace_grid */
+#define OPTK_TRACE_APPLY_POST 1 /* apply desc.post (rayfunction) */
+
+int32_t optk_abi_version(void);
+const char *optk_last_error(void);
+/* sizeof of each record, so bindings can check their layout */
+int64_t optk_sizeof(const char *record_name);
+
+int32_t optk_system_create(const optk_system_desc *desc, optk_system **out);
+int32_t optk_system_destroy(optk_system *sys);
+
+/* Trace n input rays through every surface in one fused pass.
+ * out fields have num_paths * n elements (path-major: path p of ray i at
+ * p * n + i).  accum (may be NULL) receives the rays after every surface:
+ * num_surfaces * num_paths * n elements, surface-major. */
+int32_t optk_trace(const optk_system *sys, int32_t precision, const optk_rays *in,
+                   const optk_rays *out, const optk_rays *accum, int64_t n,
+                   int32_t flags, void *hip_stream);
+
+/* As optk_trace, with input rays generated from the grid on the device
+ * (optika/systems/_sequential.py:791-828). */
+int32_t optk_trace_grid(const optk_system *sys, int32_t precision, const optk_grid *in,
+                        const optk_rays *out, const optk_rays *accum, int64_t n,
+                        int32_t flags, void *hip_stream);
+
+/* Per-operator evaluation on surface `surface` of `sys` (float64 only).
+ * in[]/out[] are device arrays of n doubles; the layout per op is: */
+enum {
+  OPTK_OP_SAG_VALUE = 0,     /* in: px py pz              -> out: z          */
+  OPTK_OP_SAG_NORMAL = 1,    /* in: px py pz              -> out: nx ny nz   */
+  OPTK_OP_SAG_INTERCEPT = 2, /* in: px py pz dx dy dz attenuation intensity
+                                -> out: px py pz dx dy dz intensity (sag.propagate_rays) */
+  OPTK_OP_APERTURE = 3,      /* in: px py pz              -> out_mask        */
+  OPTK_OP_SNELL = 4,         /* in: dx dy dz n1 n2 nx ny nz, iparam: is_mirror -> out: bx by bz */
+  OPTK_OP_MULTILAYER = 5,    /* in: wavelength cos_re cos_im n_re n_im
+                                -> out: Rs Rp Ts Tp rs_re rs_im rp_re rp_im ts_re ts_im tp_re tp_im */
+  OPTK_OP_SPACING = 6,       /* in: px py pz nx ny nz     -> out: kx ky kz   */
+  OPTK_OP_INCIDENT = 7,      /* in: wavelength dx dy dz n nx ny nz kx ky kz (ruling vector),
+                                iparam: path -> out: ex ey ez */
+  OPTK_OP_RULINGS_EFF = 8,   /* in: wavelength dx dy dz px py pz nx ny nz, iparam: path -> out: eff */
+  OPTK_OP_MATERIAL_EFF = 9,  /* in: wavelength dx dy dz n attenuation nx ny nz -> out: eff */
+  OPTK_OP_INDEX = 10,        /* in: wavelength n                   -> out: n2 */
+  OPTK_OP_SNELL_SCALAR = 11  /* in: cos_re cos_im n1_re n1_im n2_re n2_im -> out: re im */
+};
+typedef struct optk_eval_args {
+  const double *in[12];
+  double *out[12];
+  uint8_t *out_mask;
+  int32_t surface;
+  int32_t iparam;
+  int32_t stack;              /* OPTK_OP_MULTILAYER: index into stacks[] */
+  int32_t pad;
+} optk_eval_args;
+
+int32_t optk_eval(const optk_system *sys, int32_t op, const optk_eval_args *args,
+                  int64_t n, void *hip_stream);
+
+/* Sensor binning (AbstractImagingSensor.collect): weighted 3-D histogram
+ * over (wavelength edges, x edges, y edges) with x/y edges
+ * numpy.linspace(lo, hi, num + 1), right edge of the last bin inclusive.
+ * Rays with unvignetted == 0 or outside every bin are skipped.
+ * image and moment are float64 device arrays of n_wavelength * nx * ny,
+ * accumulated into (not cleared).  moment gets intensity * d_z. */
+typedef struct optk_collect_args {
+  const void *position[2];
+  const void *direction_z;
+  const void *wavelength;
+  const void *intensity;
+  const uint8_t *unvignetted;
+  const double *wavelength_edges; /* device, n_wavelength + 1 ascending */
+  int32_t n_wavelength;
+  int32_t nx;
+  int32_t ny;
+  int32_t precision;              /* element type of the ray arrays */
+  double x_lo, x_hi, y_lo, y_hi;
+  double *image;
+  double *moment;
+} optk_collect_args;
+
+int32_t optk_collect(const optk_collect_args *args, int64_t n, void *hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPTK_H */

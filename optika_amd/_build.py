@@ -1,0 +1,85 @@
+"""Build liboptk.so (gfx950) in-tree with hipcc.
+
+The library links against the HIP runtime that torch ships (its
+``libamdhip64.so`` carries the soname ``libamdhip64.so.7``), so a process
+that imports torch first shares one HIP runtime with it.
+"""
+
+from __future__ import annotations
+
+import os
+import pathlib
+import shutil
+import subprocess
+import sys
+
+HERE = pathlib.Path(__file__).resolve().parent
+ROOT = HERE.parent
+CSRC = HERE / "csrc"
+LIB = HERE / "liboptk.so"
+
+ARCH = os.environ.get("OPTK_OFFLOAD_ARCH", "gfx950")
+
+COMMON = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-mcode-object-version=5",
+    # packed FP32 instructions are kept out of every kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics",
+    f"-I{ROOT / 'include'}",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and pathlib.Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _torch_lib() -> str:
+    import importlib.util
+
+    spec = importlib.util.find_spec("torch")
+    if spec is None or spec.origin is None:
+        raise RuntimeError("torch is required to locate its HIP runtime")
+    return str(pathlib.Path(spec.origin).parent / "lib")
+
+
+def sources() -> list[pathlib.Path]:
+    return [CSRC / "optk_lib.hip"]
+
+
+def build(verbose: bool = True) -> pathlib.Path:
+    """Compile every HIP source into ``optika_amd/liboptk.so``."""
+    tl = _torch_lib()
+    objs = []
+    tmpdir = HERE / "_build"
+    tmpdir.mkdir(exist_ok=True)
+    for src in sources():
+        obj = tmpdir / (src.stem + ".o")
+        # the float64 trace keeps the reference's rounding sequence: no FMA
+        flags = ["-ffp-contract=off"]
+        cmd = [_hipcc(), *COMMON, *flags, "-c", str(src), "-o", str(obj)]
+        cmd = [c for c in cmd if c != "-shared"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(str(obj))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp),
+           f"-L{tl}", f"-Wl,-rpath,{tl}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,84 @@
+"""Sensor binning on the device (optk_collect).
+
+``AbstractImagingSensor.collect`` (``optika/sensors/_sensors.py:92-171``):
+a weighted histogram of rays over ``(wavelength edges, x edges, y edges)``
+with ``num_pixel + 1`` linear pixel edges spanning the sensor aperture,
+weights ``intensity`` for unvignetted rays, plus the flux-weighted mean
+refracted cosine per pixel.  The ideal sensor material does not refract, so
+the cosine is ``-d . n = d_z`` for the sensor normal ``(0, 0, -1)``.
+
+Accumulation uses float64 atomics in HBM; counts and bin assignment are
+exact, weighted sums differ from a sequential sum by rounding only.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib, _trace
+from .vectors import as_vector2
+
+__all__ = ["collect", "histogram"]
+
+
+def histogram(rays, wavelength_edges, x_range, y_range, num_x, num_y, where=None):
+    """Raw device histogram: returns ``(image, moment)`` float64 tensors of shape
+    ``(n_wavelength, num_x, num_y)``."""
+    dev = _trace.device()
+    n = rays.num
+    f32 = isinstance(rays.position.x, torch.Tensor) and rays.position.x.dtype == torch.float32
+    dt = torch.float32 if f32 else torch.float64
+
+    def full(v):
+        if isinstance(v, torch.Tensor) and v.ndim > 0:
+            return v.reshape(-1).to(device=dev, dtype=dt).contiguous()
+        return torch.full((n,), float(v), dtype=dt, device=dev)
+
+    px, py = full(rays.position.x), full(rays.position.y)
+    dz = full(rays.direction.z)
+    wl = full(rays.wavelength)
+    inten = full(rays.intensity)
+    u = rays.unvignetted
+    if isinstance(u, torch.Tensor) and u.ndim > 0:
+        mask = u.reshape(-1).to(device=dev, dtype=torch.uint8).contiguous()
+    else:
+        mask = torch.full((n,), 1 if bool(u) else 0, dtype=torch.uint8, device=dev)
+    if where is not None and not (np.ndim(where) == 0 and bool(where)):
+        w = torch.as_tensor(where, device=dev).reshape(-1).to(torch.uint8)
+        mask = mask & w.expand(n).contiguous()
+    edges = torch.as_tensor(np.asarray(wavelength_edges, dtype=np.float64), device=dev)
+    nw = edges.numel() - 1
+    if nw < 1 or not bool(torch.all(edges[1:] > edges[:-1])):
+        raise ValueError("wavelength edges must hold at least two ascending values")
+    image = torch.zeros((nw, num_x, num_y), dtype=torch.float64, device=dev)
+    moment = torch.zeros_like(image)
+    a = _lib.CollectArgs()
+    a.position[0], a.position[1] = px.data_ptr(), py.data_ptr()
+    a.direction_z = dz.data_ptr()
+    a.wavelength = wl.data_ptr()
+    a.intensity = inten.data_ptr()
+    a.unvignetted = mask.data_ptr()
+    a.wavelength_edges = edges.data_ptr()
+    a.n_wavelength, a.nx, a.ny = nw, int(num_x), int(num_y)
+    a.precision = _lib.F32 if f32 else _lib.F64
+    a.x_lo, a.x_hi = float(x_range[0]), float(x_range[1])
+    a.y_lo, a.y_hi = float(y_range[0]), float(y_range[1])
+    a.image, a.moment = image.data_ptr(), moment.data_ptr()
+    _lib.check(_lib.lib().optk_collect(ctypes.byref(a), n, _trace.stream_handle()))
+    return image, moment
+
+
+def collect(sensor, rays, wavelength, where=True):
+    """``(image, cosine)`` of ``AbstractImagingSensor.collect``."""
+    w = as_vector2(sensor.width_pixel)
+    npx = as_vector2(sensor.num_pixel)
+    hx = w.x * npx.x / 2
+    hy = w.y * npx.y / 2
+    image, moment = histogram(rays, wavelength, (-hx, hx), (-hy, hy), int(npx.x), int(npx.y),
+                              where=None if (np.ndim(where) == 0 and bool(where)) else where)
+    cosine = torch.where(image > 0, moment / torch.where(image > 0, image, 1.0),
+                         torch.ones_like(image))
+    return image, cosine

@@ -1,0 +1,237 @@
+"""ctypes binding of liboptk.so (the C ABI declared in include/optk.h).
+
+The structures below mirror the C records field for field; their sizes are
+checked against ``optk_sizeof`` when the library is loaded.  torch is
+imported first so that the library binds to the HIP runtime torch already
+loaded (one HIP runtime per process).
+
+There is no fallback: if the library is missing or fails to load, every
+trace entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+
+import torch  # noqa: F401  (must precede loading liboptk.so)
+
+__all__ = ["lib", "check", "OptkError", "Affine", "Surface", "Layer", "Block", "Stack",
+           "Chemical", "SystemDesc", "Rays", "Grid", "EvalArgs", "CollectArgs"]
+
+_LIB_PATH = pathlib.Path(os.environ.get(
+    "OPTK_LIBRARY", pathlib.Path(__file__).resolve().parent / "liboptk.so"))
+
+MAX_ORDERS = 8
+MAX_POLY = 8
+MAX_BLOCKS = 8
+
+F64 = 0
+F32 = 1
+TRACE_APPLY_POST = 1
+
+OP_SAG_VALUE = 0
+OP_SAG_NORMAL = 1
+OP_SAG_INTERCEPT = 2
+OP_APERTURE = 3
+OP_SNELL = 4
+OP_MULTILAYER = 5
+OP_SPACING = 6
+OP_INCIDENT = 7
+OP_RULINGS_EFF = 8
+OP_MATERIAL_EFF = 9
+OP_INDEX = 10
+OP_SNELL_SCALAR = 11
+
+c_double = ctypes.c_double
+c_int32 = ctypes.c_int32
+c_int64 = ctypes.c_int64
+c_void_p = ctypes.c_void_p
+
+
+class Affine(ctypes.Structure):
+    _fields_ = [("m", c_double * 9), ("t", c_double * 3), ("kind", c_int32), ("pad", c_int32)]
+
+
+class Surface(ctypes.Structure):
+    _fields_ = [
+        ("transform", Affine),
+        ("sag_kind", c_int32), ("pad0", c_int32),
+        ("sag_p", c_double * 4),
+        ("sag_transform", Affine),
+        ("material_kind", c_int32), ("is_mirror", c_int32),
+        ("material_p", c_double * 6),
+        ("material_table", c_int32), ("material_table_len", c_int32),
+        ("stack", c_int32), ("pad1", c_int32),
+        ("rulings_kind", c_int32), ("spacing_kind", c_int32),
+        ("num_orders", c_int32), ("num_terms", c_int32),
+        ("orders", c_int32 * MAX_ORDERS),
+        ("rulings_p", c_double * 2),
+        ("spacing_constant", c_double),
+        ("wavelength_scale", c_double),
+        ("spacing_normal", c_double * 3),
+        ("poly_coeff", c_double * MAX_POLY),
+        ("poly_power", c_int32 * MAX_POLY),
+        ("holo_x1", c_double * 3),
+        ("holo_x2", c_double * 3),
+        ("holo_wavelength", c_double),
+        ("holo_diverging1", c_int32), ("holo_diverging2", c_int32),
+        ("spacing_transform", Affine),
+        ("rulings_table", c_int32), ("rulings_table_len", c_int32),
+        ("aperture_kind", c_int32), ("aperture_flags", c_int32),
+        ("aperture_p", c_double * 4),
+        ("polygon_offset", c_int32), ("polygon_count", c_int32),
+        ("aperture_transform", Affine),
+    ]
+
+
+class Layer(ctypes.Structure):
+    _fields_ = [("chemical", c_int32), ("profile_kind", c_int32),
+                ("thickness", c_double), ("profile_width", c_double)]
+
+
+class Block(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("first_layer", c_int32),
+                ("num_layers", c_int32), ("num_periods", c_int32)]
+
+
+class Stack(ctypes.Structure):
+    _fields_ = [("num_blocks", c_int32), ("substrate_layer", c_int32),
+                ("blocks", Block * MAX_BLOCKS)]
+
+
+class Chemical(ctypes.Structure):
+    _fields_ = [("offset", c_int32), ("count", c_int32)]
+
+
+class SystemDesc(ctypes.Structure):
+    _fields_ = [
+        ("surfaces", ctypes.POINTER(Surface)),
+        ("num_surfaces", c_int32), ("num_paths", c_int32),
+        ("layers", ctypes.POINTER(Layer)),
+        ("num_layers", c_int32), ("num_stacks", c_int32),
+        ("stacks", ctypes.POINTER(Stack)),
+        ("chemicals", ctypes.POINTER(Chemical)),
+        ("num_chemicals", c_int32), ("pad", c_int32),
+        ("tables", ctypes.POINTER(c_double)),
+        ("num_tables", c_int64),
+        ("pre", Affine * 2),
+        ("pre_is_inverse", c_int32 * 2),
+        ("post", Affine),
+    ]
+
+
+class Rays(ctypes.Structure):
+    _fields_ = [
+        ("position", c_void_p * 3),
+        ("direction", c_void_p * 3),
+        ("wavelength", c_void_p),
+        ("intensity", c_void_p),
+        ("attenuation", c_void_p),
+        ("index_refraction", c_void_p),
+        ("unvignetted", c_void_p),
+        ("wavelength_default", c_double),
+        ("intensity_default", c_double),
+        ("attenuation_default", c_double),
+        ("index_default", c_double),
+        ("unvignetted_default", c_int32), ("pad", c_int32),
+    ]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [
+        ("field", c_void_p * 2),
+        ("pupil", c_void_p * 2),
+        ("wavelength", c_void_p),
+        ("intensity", c_void_p),
+        ("field_default", c_double * 2),
+        ("pupil_default", c_double * 2),
+        ("wavelength_default", c_double),
+        ("intensity_default", c_double),
+        ("object_at_infinity", c_int32), ("pad", c_int32),
+    ]
+
+
+class EvalArgs(ctypes.Structure):
+    _fields_ = [
+        ("in_", c_void_p * 12),
+        ("out", c_void_p * 12),
+        ("out_mask", c_void_p),
+        ("surface", c_int32), ("iparam", c_int32),
+        ("stack", c_int32), ("pad", c_int32),
+    ]
+
+
+class CollectArgs(ctypes.Structure):
+    _fields_ = [
+        ("position", c_void_p * 2),
+        ("direction_z", c_void_p),
+        ("wavelength", c_void_p),
+        ("intensity", c_void_p),
+        ("unvignetted", c_void_p),
+        ("wavelength_edges", c_void_p),
+        ("n_wavelength", c_int32), ("nx", c_int32), ("ny", c_int32), ("precision", c_int32),
+        ("x_lo", c_double), ("x_hi", c_double), ("y_lo", c_double), ("y_hi", c_double),
+        ("image", c_void_p),
+        ("moment", c_void_p),
+    ]
+
+
+class OptkError(RuntimeError):
+    """An error reported by liboptk.so."""
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load liboptk.so once, declare its prototypes and check record sizes."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        raise OptkError(
+            f"{_LIB_PATH} is missing: build it with `python -m optika_amd._build` "
+            "(the trace runs only on the GPU; there is no CPU fallback)"
+        )
+    L = ctypes.CDLL(str(_LIB_PATH))
+    L.optk_abi_version.restype = c_int32
+    L.optk_last_error.restype = ctypes.c_char_p
+    L.optk_sizeof.restype = c_int64
+    L.optk_sizeof.argtypes = [ctypes.c_char_p]
+    L.optk_system_create.restype = c_int32
+    L.optk_system_create.argtypes = [ctypes.POINTER(SystemDesc), ctypes.POINTER(c_void_p)]
+    L.optk_system_destroy.restype = c_int32
+    L.optk_system_destroy.argtypes = [c_void_p]
+    L.optk_trace.restype = c_int32
+    L.optk_trace.argtypes = [c_void_p, c_int32, ctypes.POINTER(Rays), ctypes.POINTER(Rays),
+                             ctypes.POINTER(Rays), c_int64, c_int32, c_void_p]
+    L.optk_trace_grid.restype = c_int32
+    L.optk_trace_grid.argtypes = [c_void_p, c_int32, ctypes.POINTER(Grid), ctypes.POINTER(Rays),
+                                  ctypes.POINTER(Rays), c_int64, c_int32, c_void_p]
+    L.optk_eval.restype = c_int32
+    L.optk_eval.argtypes = [c_void_p, c_int32, ctypes.POINTER(EvalArgs), c_int64, c_void_p]
+    L.optk_collect.restype = c_int32
+    L.optk_collect.argtypes = [ctypes.POINTER(CollectArgs), c_int64, c_void_p]
+    if L.optk_abi_version() != 1:
+        raise OptkError(f"liboptk ABI {L.optk_abi_version()} != 1")
+    for name, cls in [("optk_affine", Affine), ("optk_surface", Surface), ("optk_layer", Layer),
+                      ("optk_block", Block), ("optk_stack", Stack), ("optk_chemical", Chemical),
+                      ("optk_system_desc", SystemDesc), ("optk_rays", Rays), ("optk_grid", Grid),
+                      ("optk_eval_args", EvalArgs), ("optk_collect_args", CollectArgs)]:
+        size = L.optk_sizeof(name.encode())
+        if size != ctypes.sizeof(cls):
+            raise OptkError(f"record {name}: C size {size} != binding size {ctypes.sizeof(cls)}")
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().optk_last_error().decode(errors="replace")
+        raise OptkError(f"liboptk error {rc}: {msg}")
+
+
+def library_path() -> pathlib.Path:
+    return _LIB_PATH

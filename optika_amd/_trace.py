@@ -1,0 +1,277 @@
+"""Device dispatch of the fused trace (optk_trace / optk_trace_grid).
+
+PyTorch supplies device memory and the current stream; every ray field is a
+contiguous 1-D tensor and goes to the kernel as a raw pointer.  There is no
+host fallback: without a GPU and liboptk.so these functions raise.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from collections import OrderedDict
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from . import _compile, _lib
+from .rays import RayFunctionArray, RayVectorArray
+from .vectors import Cartesian3dVectorArray, ObjectVectorArray
+
+__all__ = ["trace_sequence", "trace_system", "device", "compiled"]
+
+_PRECISIONS = {"f64": _lib.F64, "float64": _lib.F64, "f32": _lib.F32, "float32": _lib.F32}
+
+_COMPILED: "OrderedDict[bytes, _compile.CompiledSystem]" = OrderedDict()
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise _lib.OptkError(
+            "optika_amd traces on the GPU only: no HIP device is visible "
+            "(there is no CPU fallback)"
+        )
+    _lib.lib()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def compiled(surfaces: Sequence, pre=(None, None), pre_is_inverse=(0, 0), post=None):
+    """Compile (or reuse) the device records of a surface sequence."""
+    rec = _compile.build_records(surfaces, pre, pre_is_inverse, post)
+    key = rec.key()
+    cs = _COMPILED.get(key)
+    if cs is None:
+        device()
+        cs = _compile.CompiledSystem(rec)
+        _COMPILED[key] = cs
+        while len(_COMPILED) > 32:
+            _COMPILED.popitem(last=False)
+    else:
+        _COMPILED.move_to_end(key)
+    return cs
+
+
+def _precision(precision: str) -> int:
+    try:
+        return _PRECISIONS[precision]
+    except KeyError:
+        raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}, got {precision!r}")
+
+
+def _as_tensor(v, n, dev, dtype=torch.float64) -> None | torch.Tensor:
+    """A contiguous 1-D device tensor of length n, or None for a scalar."""
+    if isinstance(v, torch.Tensor) and v.ndim > 0:
+        t = v.reshape(-1).to(device=dev, dtype=dtype).contiguous()
+    elif isinstance(v, np.ndarray) and v.ndim > 0:
+        t = torch.as_tensor(np.ascontiguousarray(v.reshape(-1))).to(device=dev, dtype=dtype)
+    else:
+        return None
+    if t.numel() != n:
+        raise ValueError(f"field has {t.numel()} elements, expected {n}")
+    return t
+
+
+def _scalar(v) -> float:
+    if isinstance(v, torch.Tensor):
+        return float(v.item())
+    return float(v)
+
+
+def _numel(v) -> int | None:
+    if isinstance(v, (torch.Tensor, np.ndarray)) and v.ndim > 0:
+        return int(np.prod(v.shape))
+    return None
+
+
+def _ptr(t: None | torch.Tensor):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class _Out:
+    """Output tensors of one trace and their ``optk_rays`` record."""
+
+    def __init__(self, total: int, dev, dtype, fields: set[str]):
+        mk = lambda: torch.empty(total, dtype=dtype, device=dev)  # noqa: E731
+        self.t = {}
+        self.t["position"] = [mk(), mk(), mk()]
+        self.t["direction"] = [mk(), mk(), mk()]
+        for f in ("wavelength", "intensity", "attenuation", "index_refraction"):
+            if f in fields:
+                self.t[f] = mk()
+        self.t["unvignetted"] = torch.empty(total, dtype=torch.uint8, device=dev)
+        r = _lib.Rays()
+        for k in range(3):
+            r.position[k] = self.t["position"][k].data_ptr()
+            r.direction[k] = self.t["direction"][k].data_ptr()
+        for f in ("wavelength", "intensity", "attenuation", "index_refraction"):
+            if f in self.t:
+                setattr(r, f, self.t[f].data_ptr())
+        r.unvignetted = self.t["unvignetted"].data_ptr()
+        self.rec = r
+
+
+def _make_rays(out: _Out, shape, scalars: dict) -> RayVectorArray:
+    def shp(t):
+        return t.view(shape)
+
+    def field(name):
+        if name in out.t:
+            return shp(out.t[name])
+        return scalars[name]
+
+    return RayVectorArray(
+        wavelength=field("wavelength"),
+        position=Cartesian3dVectorArray(*(shp(t) for t in out.t["position"])),
+        direction=Cartesian3dVectorArray(*(shp(t) for t in out.t["direction"])),
+        intensity=field("intensity"),
+        attenuation=field("attenuation"),
+        index_refraction=field("index_refraction"),
+        unvignetted=shp(out.t["unvignetted"]).bool(),
+    )
+
+
+def _shape(num_paths: int, n: int, num_surfaces: int | None = None):
+    shape = (n,) if num_paths == 1 else (num_paths, n)
+    if num_surfaces is not None:
+        shape = (num_surfaces,) + shape
+    return shape
+
+
+def trace_sequence(surfaces: Sequence, rays: RayVectorArray, accumulate: bool = False,
+                   precision: str = "f64") -> RayVectorArray:
+    """``propagate_rays`` / ``accumulate_rays`` over ``surfaces``."""
+    prec = _precision(precision)
+    if prec != _lib.F64:
+        from . import _trace32
+
+        return _trace32.trace_sequence_f32(surfaces, rays, accumulate)
+    dev = device()
+    cs = compiled(surfaces)
+    n = rays.num
+    P = cs.num_paths
+
+    keep = []
+    rin = _lib.Rays()
+    for k, c in enumerate(rays.position):
+        t = _as_tensor(c, n, dev)
+        if t is None:
+            t = torch.full((n,), _scalar(c), dtype=torch.float64, device=dev)
+        keep.append(t)
+        rin.position[k] = t.data_ptr()
+    for k, c in enumerate(rays.direction):
+        t = _as_tensor(c, n, dev)
+        if t is None:
+            t = torch.full((n,), _scalar(c), dtype=torch.float64, device=dev)
+        keep.append(t)
+        rin.direction[k] = t.data_ptr()
+    scalars_in = {}
+    for f, dflt in (("wavelength", "wavelength_default"), ("intensity", "intensity_default"),
+                    ("attenuation", "attenuation_default"), ("index_refraction", "index_default")):
+        v = getattr(rays, f)
+        t = _as_tensor(v, n, dev)
+        if t is None:
+            setattr(rin, dflt, _scalar(v))
+            scalars_in[f] = _scalar(v)
+        else:
+            keep.append(t)
+            setattr(rin, f, t.data_ptr())
+    u = rays.unvignetted
+    tu = _as_tensor(u, n, dev, torch.uint8)
+    if tu is None:
+        rin.unvignetted_default = 1 if bool(u if not isinstance(u, torch.Tensor) else u.item()) else 0
+    else:
+        keep.append(tu)
+        rin.unvignetted = tu.data_ptr()
+
+    fields = {"wavelength", "intensity", "attenuation", "index_refraction"}
+    out = _Out(P * n, dev, torch.float64, fields)
+    acc = _Out(cs.num_surfaces * P * n, dev, torch.float64, fields) if accumulate else None
+    L = _lib.lib()
+    _lib.check(L.optk_trace(cs.handle, _lib.F64, ctypes.byref(rin), ctypes.byref(out.rec),
+                            ctypes.byref(acc.rec) if acc else None, n, 0, stream_handle()))
+    if accumulate:
+        return _make_rays(acc, _shape(P, n, cs.num_surfaces), {})
+    return _make_rays(out, _shape(P, n), {})
+
+
+def _grid_fields(grid: ObjectVectorArray, intensity):
+    vals = {
+        "fx": grid.field.x, "fy": grid.field.y,
+        "px": grid.pupil.x, "py": grid.pupil.y,
+        "wavelength": grid.wavelength,
+        "intensity": 1.0 if intensity is None else intensity,
+    }
+    sizes = {s for s in (_numel(v) for v in vals.values()) if s is not None}
+    if len(sizes) > 1:
+        raise ValueError(
+            f"grid fields must be scalars or 1-D arrays of one common length, got sizes {sorted(sizes)}"
+        )
+    n = sizes.pop() if sizes else 1
+    return vals, n
+
+
+def trace_system(system, grid: ObjectVectorArray, intensity=None, accumulate: bool = False,
+                 sensor_frame: bool = False, precision: str = "f64") -> RayFunctionArray:
+    """``SequentialSystem.raytrace`` / ``rayfunction`` in one launch."""
+    prec = _precision(precision)
+    surfaces = system.surfaces_all
+    obj_t = system.object.transformation if system.object is not None else None
+    post = system.sensor.transformation if (sensor_frame and system.sensor is not None) else None
+    if prec != _lib.F64:
+        from . import _trace32
+
+        return _trace32.trace_system_f32(system, grid, intensity, accumulate, sensor_frame)
+    dev = device()
+    cs = compiled(surfaces, pre=(obj_t, system.transformation), pre_is_inverse=(0, 1), post=post)
+    vals, n = _grid_fields(grid, intensity)
+    P = cs.num_paths
+
+    g = _lib.Grid()
+    keep = []
+
+    def put(name, arr_field, idx, dflt_field):
+        t = _as_tensor(vals[name], n, dev)
+        if t is None:
+            if idx is None:
+                setattr(g, dflt_field, _scalar(vals[name]))
+            else:
+                getattr(g, dflt_field)[idx] = _scalar(vals[name])
+        else:
+            keep.append(t)
+            if idx is None:
+                setattr(g, arr_field, t.data_ptr())
+            else:
+                getattr(g, arr_field)[idx] = t.data_ptr()
+        return t
+
+    put("fx", "field", 0, "field_default")
+    put("fy", "field", 1, "field_default")
+    put("px", "pupil", 0, "pupil_default")
+    put("py", "pupil", 1, "pupil_default")
+    tw = put("wavelength", "wavelength", None, "wavelength_default")
+    put("intensity", "intensity", None, "intensity_default")
+    g.object_at_infinity = 1 if system.object_is_at_infinity else 0
+
+    invariant = _compile.invariant_fields(surfaces)
+    fields = {"intensity"} if invariant else {"wavelength", "intensity", "attenuation",
+                                               "index_refraction"}
+    out = _Out(P * n, dev, torch.float64, fields)
+    acc = (_Out(cs.num_surfaces * P * n, dev, torch.float64,
+                {"wavelength", "intensity", "attenuation", "index_refraction"})
+           if accumulate else None)
+    flags = _lib.TRACE_APPLY_POST if post is not None else 0
+    L = _lib.lib()
+    _lib.check(L.optk_trace_grid(cs.handle, _lib.F64, ctypes.byref(g), ctypes.byref(out.rec),
+                                 ctypes.byref(acc.rec) if acc else None, n, flags,
+                                 stream_handle()))
+    if accumulate:
+        outputs = _make_rays(acc, _shape(P, n, cs.num_surfaces), {})
+    else:
+        wl = _scalar(vals["wavelength"]) if tw is None else (tw if P == 1 else tw.expand(P, n))
+        outputs = _make_rays(out, _shape(P, n), {
+            "wavelength": wl, "attenuation": 0.0, "index_refraction": 1.0})
+    return RayFunctionArray(inputs=grid, outputs=outputs)

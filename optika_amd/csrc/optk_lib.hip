@@ -1,0 +1,678 @@
+// optk_lib.hip -- gfx950 kernels and the C ABI of liboptk.so (float64 trace,
+// per-operator evaluation, sensor binning).  See include/optk.h.
+//
+// Compiled with -ffp-contract=off: the float64 trace keeps the reference's
+// rounding sequence (no fused multiply-adds).
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "optk_surface.h"
+
+using namespace optk;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int32_t fail(int32_t code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define OPTK_HIP_CHECK(expr)                                                         \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(OPTK_EDEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+struct optk_system {
+  optk_surface *surfaces = nullptr;
+  optk_layer *layers = nullptr;
+  optk_stack *stacks = nullptr;
+  optk_chemical *chemicals = nullptr;
+  double *tables = nullptr;
+  int32_t num_surfaces = 0;
+  int32_t num_paths = 1;
+  int32_t num_layers = 0;
+  int32_t num_stacks = 0;
+  int32_t num_chemicals = 0;
+  int64_t num_tables = 0;
+  optk_affine pre[2];
+  int32_t pre_is_inverse[2];
+  optk_affine post;
+  std::vector<optk_surface> host_surfaces;
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+struct RaysIn {
+  const double *p[3];
+  const double *d[3];
+  const double *wavelength, *intensity, *attenuation, *index;
+  const uint8_t *unvignetted;
+  double wavelength_default, intensity_default, attenuation_default, index_default;
+  int32_t unvignetted_default;
+};
+
+struct RaysOut {
+  double *p[3];
+  double *d[3];
+  double *wavelength, *intensity, *attenuation, *index;
+  uint8_t *unvignetted;
+};
+
+struct GridIn {
+  const double *field[2];
+  const double *pupil[2];
+  const double *wavelength, *intensity;
+  double field_default[2], pupil_default[2], wavelength_default, intensity_default;
+  int32_t object_at_infinity;
+};
+
+struct TraceParams {
+  TraceSpec spec;
+  int32_t has_accum;
+  int64_t n;
+  RaysOut out;
+  RaysOut accum;
+};
+
+__device__ __forceinline__ double ld(const double *p, int64_t i, double dflt) {
+  return p ? p[i] : dflt;
+}
+
+__device__ __forceinline__ void store(const RaysOut &o, int64_t j, const Ray<double> &r) {
+  if (o.p[0]) o.p[0][j] = r.px;
+  if (o.p[1]) o.p[1][j] = r.py;
+  if (o.p[2]) o.p[2][j] = r.pz;
+  if (o.d[0]) o.d[0][j] = r.dx;
+  if (o.d[1]) o.d[1][j] = r.dy;
+  if (o.d[2]) o.d[2][j] = r.dz;
+  if (o.wavelength) o.wavelength[j] = r.wavelength;
+  if (o.intensity) o.intensity[j] = r.intensity;
+  if (o.attenuation) o.attenuation[j] = r.attenuation;
+  if (o.index) o.index[j] = r.index;
+  if (o.unvignetted) o.unvignetted[j] = r.unvignetted ? 1 : 0;
+}
+
+// every diffraction-order path of one ray, through every surface
+__device__ __forceinline__ void trace_one(const TraceParams &P, int64_t i, const Ray<double> &r0) {
+  const SystemView &sys = P.spec.sys;
+  const int64_t n = P.n;
+  for (int path = 0; path < sys.num_paths; ++path) {
+    Ray<double> r = r0;
+    for (int k = 0; k < sys.num_surfaces; ++k) {
+      propagate_surface(sys.surfaces[k], sys, path, r);
+      if (P.has_accum) store(P.accum, ((int64_t)k * sys.num_paths + path) * n + i, r);
+    }
+    apply_post(P.spec, r);
+    store(P.out, (int64_t)path * n + i, r);
+  }
+}
+
+__global__ __launch_bounds__(256) void trace_rays_f64(const TraceParams P, const RaysIn in) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  Ray<double> r;
+  r.px = in.p[0][i];
+  r.py = in.p[1][i];
+  r.pz = in.p[2][i];
+  r.dx = in.d[0][i];
+  r.dy = in.d[1][i];
+  r.dz = in.d[2][i];
+  r.wavelength = ld(in.wavelength, i, in.wavelength_default);
+  r.intensity = ld(in.intensity, i, in.intensity_default);
+  r.attenuation = ld(in.attenuation, i, in.attenuation_default);
+  r.index = ld(in.index, i, in.index_default);
+  r.unvignetted = in.unvignetted ? (in.unvignetted[i] != 0) : (in.unvignetted_default != 0);
+  apply_pre(P.spec, r);
+  trace_one(P, i, r);
+}
+
+__global__ __launch_bounds__(256) void trace_grid_f64(const TraceParams P, const GridIn g) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  Ray<double> r = grid_ray(ld(g.field[0], i, g.field_default[0]), ld(g.field[1], i, g.field_default[1]),
+                           ld(g.pupil[0], i, g.pupil_default[0]), ld(g.pupil[1], i, g.pupil_default[1]),
+                           ld(g.wavelength, i, g.wavelength_default),
+                           ld(g.intensity, i, g.intensity_default), g.object_at_infinity);
+  apply_pre(P.spec, r);
+  trace_one(P, i, r);
+}
+
+// per-operator evaluation --------------------------------------------------
+
+struct EvalParams {
+  SystemView sys;
+  int32_t op, surface, iparam, stack;
+  int64_t n;
+  const double *in[12];
+  double *out[12];
+  uint8_t *out_mask;
+};
+
+__global__ __launch_bounds__(256) void eval_f64(const EvalParams E) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= E.n) return;
+  const optk_surface &s = E.sys.surfaces[E.surface];
+  const double *const *in = E.in;
+  double *const *out = E.out;
+  switch (E.op) {
+  case OPTK_OP_SAG_VALUE:
+    out[0][i] = sag_z(s, in[0][i], in[1][i], in[2][i]);
+    break;
+  case OPTK_OP_SAG_NORMAL: {
+    V3<double> v = sag_normal(s, in[0][i], in[1][i], in[2][i]);
+    out[0][i] = v.x;
+    out[1][i] = v.y;
+    out[2][i] = v.z;
+    break;
+  }
+  case OPTK_OP_SAG_INTERCEPT: {
+    Ray<double> r = {};
+    r.px = in[0][i];
+    r.py = in[1][i];
+    r.pz = in[2][i];
+    r.dx = in[3][i];
+    r.dy = in[4][i];
+    r.dz = in[5][i];
+    r.attenuation = in[6][i];
+    r.intensity = in[7][i];
+    sag_propagate(s, r);
+    out[0][i] = r.px;
+    out[1][i] = r.py;
+    out[2][i] = r.pz;
+    out[3][i] = r.dx;
+    out[4][i] = r.dy;
+    out[5][i] = r.dz;
+    out[6][i] = r.intensity;
+    break;
+  }
+  case OPTK_OP_APERTURE:
+    E.out_mask[i] = aperture_contains(s, E.sys.sv.tables, in[0][i], in[1][i], in[2][i]) ? 1 : 0;
+    break;
+  case OPTK_OP_SNELL: {
+    V3<double> b = snells_law({in[0][i], in[1][i], in[2][i]}, in[3][i], in[4][i],
+                              {in[5][i], in[6][i], in[7][i]}, E.iparam != 0);
+    out[0][i] = b.x;
+    out[1][i] = b.y;
+    out[2][i] = b.z;
+    break;
+  }
+  case OPTK_OP_MULTILAYER: {
+    MultilayerOut co;
+    MultilayerRT rt = multilayer_efficiency(E.sys.sv, E.stack, in[0][i], C<double>{in[1][i], in[2][i]},
+                                            C<double>{in[3][i], in[4][i]}, true, &co);
+    out[0][i] = rt.R[0];
+    out[1][i] = rt.R[1];
+    out[2][i] = rt.T[0];
+    out[3][i] = rt.T[1];
+    out[4][i] = co.r[0].re;
+    out[5][i] = co.r[0].im;
+    out[6][i] = co.r[1].re;
+    out[7][i] = co.r[1].im;
+    out[8][i] = co.t[0].re;
+    out[9][i] = co.t[0].im;
+    out[10][i] = co.t[1].re;
+    out[11][i] = co.t[1].im;
+    break;
+  }
+  case OPTK_OP_SPACING: {
+    V3<double> k = spacing_vector(s, {in[0][i], in[1][i], in[2][i]}, {in[3][i], in[4][i], in[5][i]});
+    out[0][i] = k.x;
+    out[1][i] = k.y;
+    out[2][i] = k.z;
+    break;
+  }
+  case OPTK_OP_INCIDENT: {
+    V3<double> nrm = {in[5][i], in[6][i], in[7][i]};
+    V3<double> kappa = {in[8][i], in[9][i], in[10][i]};
+    V3<double> e = incident_effective(s, path_order(s, E.iparam), in[0][i],
+                                      {in[1][i], in[2][i], in[3][i]}, in[4][i], nrm, kappa);
+    out[0][i] = e.x;
+    out[1][i] = e.y;
+    out[2][i] = e.z;
+    break;
+  }
+  case OPTK_OP_RULINGS_EFF:
+    out[0][i] = rulings_efficiency(s, path_order(s, E.iparam), in[0][i], {in[1][i], in[2][i], in[3][i]},
+                                   {in[4][i], in[5][i], in[6][i]}, {in[7][i], in[8][i], in[9][i]},
+                                   E.sys.sv.tables);
+    break;
+  case OPTK_OP_MATERIAL_EFF:
+    out[0][i] = material_efficiency(s, E.sys.sv, in[0][i], {in[1][i], in[2][i], in[3][i]}, in[4][i],
+                                    in[5][i], {in[6][i], in[7][i], in[8][i]});
+    break;
+  case OPTK_OP_INDEX:
+    out[0][i] = material_index(s, in[0][i], in[1][i]);
+    break;
+  case OPTK_OP_SNELL_SCALAR: {
+    C<double> c = snell_scalar(C<double>{in[0][i], in[1][i]}, C<double>{in[2][i], in[3][i]},
+                               C<double>{in[4][i], in[5][i]});
+    out[0][i] = c.re;
+    out[1][i] = c.im;
+    break;
+  }
+  }
+}
+
+// sensor binning --------------------------------------------------------------
+
+struct CollectParams {
+  const double *px, *py, *dz, *wavelength, *intensity;
+  const float *pxf, *pyf, *dzf, *wavelengthf, *intensityf;
+  const uint8_t *unvignetted;
+  const double *wedges;
+  int32_t nw, nx, ny;
+  double x_lo, x_hi, y_lo, y_hi;
+  double *image, *moment;
+  int64_t n;
+};
+
+// Bin index of x among numpy.linspace(lo, hi, num + 1) edges with
+// numpy.histogramdd's rule (right edge of the last bin inclusive); -1 if out.
+__device__ __forceinline__ int linspace_bin(double x, double lo, double hi, int num) {
+  if (!(x >= lo) || !(x <= hi)) return -1;
+  double step = (hi - lo) / (double)num;
+  int k = (int)floor((x - lo) / step);
+  k = k < 0 ? 0 : (k > num ? num : k);
+  // edges: e_j = j * step + lo for j < num, e_num = hi
+  auto edge = [&](int j) { return j >= num ? hi : (double)j * step + lo; };
+  while (k < num && edge(k + 1) <= x) ++k;
+  while (k > 0 && edge(k) > x) --k;
+  // k = number of edges <= x, minus one
+  if (x == hi) return num - 1;
+  return k < num ? k : -1;
+}
+
+__device__ __forceinline__ int edges_bin(double x, const double *e, int num) {
+  if (!(x >= e[0]) || !(x <= e[num])) return -1;
+  if (x == e[num]) return num - 1;
+  int lo = 0, hi = num;  // e[lo] <= x < e[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (x < e[mid])
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void collect_kernel(const CollectParams C) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.n) return;
+  if (C.unvignetted && !C.unvignetted[i]) return;
+  double x, y, dz, w, inten;
+  if (C.px) {
+    x = C.px[i];
+    y = C.py[i];
+    dz = C.dz[i];
+    w = C.wavelength[i];
+    inten = C.intensity[i];
+  } else {
+    x = C.pxf[i];
+    y = C.pyf[i];
+    dz = C.dzf[i];
+    w = C.wavelengthf[i];
+    inten = C.intensityf[i];
+  }
+  int bw = edges_bin(w, C.wedges, C.nw);
+  int bx = linspace_bin(x, C.x_lo, C.x_hi, C.nx);
+  int by = linspace_bin(y, C.y_lo, C.y_hi, C.ny);
+  if (bw < 0 || bx < 0 || by < 0) return;
+  int64_t idx = ((int64_t)bw * C.nx + bx) * C.ny + by;
+  unsafeAtomicAdd(&C.image[idx], inten);
+  unsafeAtomicAdd(&C.moment[idx], inten * dz);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+static SystemView view_of(const optk_system *s) {
+  SystemView v;
+  v.surfaces = s->surfaces;
+  v.num_surfaces = s->num_surfaces;
+  v.num_paths = s->num_paths;
+  v.sv.layers = s->layers;
+  v.sv.stacks = s->stacks;
+  v.sv.chemicals = s->chemicals;
+  v.sv.tables = s->tables;
+  return v;
+}
+
+template <typename T> static int32_t upload(T **dst, const T *src, int64_t count) {
+  *dst = nullptr;
+  if (count <= 0) return OPTK_OK;
+  if (!src) return fail(OPTK_EINVAL, "null host array for a non-empty record list");
+  hipError_t e = hipMalloc((void **)dst, sizeof(T) * count);
+  if (e != hipSuccess) return fail(OPTK_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  return OPTK_OK;
+}
+
+static bool valid_affine(const optk_affine &a) {
+  return a.kind >= OPTK_XF_IDENTITY && a.kind <= OPTK_XF_AFFINE;
+}
+
+// Check every index a kernel will follow, so no kernel can read outside the
+// uploaded records.
+static int32_t validate(const optk_system_desc *d) {
+  if (d->num_surfaces < 0 || d->num_surfaces > 4096) return fail(OPTK_EINVAL, "num_surfaces out of range");
+  if (d->num_paths < 1 || d->num_paths > OPTK_MAX_ORDERS) return fail(OPTK_EINVAL, "num_paths out of range");
+  for (int c = 0; c < d->num_chemicals; ++c) {
+    const optk_chemical &ch = d->chemicals[c];
+    if (ch.count < 2 || ch.offset < 0 || (int64_t)ch.offset + 3LL * ch.count > d->num_tables)
+      return fail(OPTK_EINVAL, "chemical table out of range");
+  }
+  for (int l = 0; l < d->num_layers; ++l) {
+    const optk_layer &L = d->layers[l];
+    if (L.chemical < -1 || L.chemical >= d->num_chemicals) return fail(OPTK_EINVAL, "layer chemical index out of range");
+  }
+  for (int k = 0; k < d->num_stacks; ++k) {
+    const optk_stack &st = d->stacks[k];
+    if (st.num_blocks < 0 || st.num_blocks > OPTK_MAX_BLOCKS) return fail(OPTK_EINVAL, "stack block count out of range");
+    if (st.substrate_layer < -1 || st.substrate_layer >= d->num_layers) return fail(OPTK_EINVAL, "substrate index out of range");
+    for (int b = 0; b < st.num_blocks; ++b) {
+      const optk_block &bl = st.blocks[b];
+      if (bl.first_layer < 0 || bl.num_layers < 0 || bl.first_layer + bl.num_layers > d->num_layers)
+        return fail(OPTK_EINVAL, "stack block layers out of range");
+      if (bl.kind == OPTK_BLOCK_PERIODIC && bl.num_periods < 1) return fail(OPTK_EINVAL, "num_periods must be >= 1");
+    }
+  }
+  for (int i = 0; i < d->num_surfaces; ++i) {
+    const optk_surface &s = d->surfaces[i];
+    if (!valid_affine(s.transform) || !valid_affine(s.sag_transform) || !valid_affine(s.spacing_transform) ||
+        !valid_affine(s.aperture_transform))
+      return fail(OPTK_EINVAL, "bad transformation kind");
+    if (s.sag_kind < OPTK_SAG_NONE || s.sag_kind > OPTK_SAG_TOROID) return fail(OPTK_EINVAL, "bad sag kind");
+    if (s.material_kind < OPTK_MATERIAL_VACUUM || s.material_kind > OPTK_MATERIAL_MULTILAYER_FILM)
+      return fail(OPTK_EINVAL, "bad material kind");
+    if (s.material_kind == OPTK_MATERIAL_MEASURED_MIRROR &&
+        (s.material_table < 0 || s.material_table_len < 2 ||
+         (int64_t)s.material_table + 2LL * s.material_table_len > d->num_tables))
+      return fail(OPTK_EINVAL, "measured-mirror table out of range");
+    if ((s.material_kind == OPTK_MATERIAL_MULTILAYER_MIRROR || s.material_kind == OPTK_MATERIAL_MULTILAYER_FILM) &&
+        (s.stack < 0 || s.stack >= d->num_stacks))
+      return fail(OPTK_EINVAL, "multilayer stack index out of range");
+    if (s.rulings_kind < OPTK_RULINGS_NONE || s.rulings_kind > OPTK_RULINGS_RECTANGULAR)
+      return fail(OPTK_EINVAL, "bad rulings kind");
+    if (s.rulings_kind != OPTK_RULINGS_NONE) {
+      if (s.num_orders != 1 && s.num_orders != d->num_paths) return fail(OPTK_EINVAL, "diffraction-order count mismatch");
+      if (s.spacing_kind < 0 || s.spacing_kind > OPTK_SPACING_HOLOGRAPHIC) return fail(OPTK_EINVAL, "bad spacing kind");
+      if (s.num_terms < 0 || s.num_terms > OPTK_MAX_POLY) return fail(OPTK_EINVAL, "bad polynomial term count");
+      if (s.rulings_kind == OPTK_RULINGS_MEASURED &&
+          (s.rulings_table < 0 || s.rulings_table_len < 2 ||
+           (int64_t)s.rulings_table + 2LL * s.rulings_table_len > d->num_tables))
+        return fail(OPTK_EINVAL, "measured-rulings table out of range");
+    }
+    if (s.aperture_kind < OPTK_APERTURE_NONE || s.aperture_kind > OPTK_APERTURE_POLYGON)
+      return fail(OPTK_EINVAL, "bad aperture kind");
+    if (s.aperture_kind == OPTK_APERTURE_POLYGON &&
+        (s.polygon_count < 3 || s.polygon_offset < 0 ||
+         (int64_t)s.polygon_offset + 2LL * s.polygon_count > d->num_tables))
+      return fail(OPTK_EINVAL, "polygon table out of range");
+  }
+  for (int k = 0; k < 2; ++k)
+    if (!valid_affine(d->pre[k])) return fail(OPTK_EINVAL, "bad pre transformation");
+  if (!valid_affine(d->post)) return fail(OPTK_EINVAL, "bad post transformation");
+  return OPTK_OK;
+}
+
+static dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+extern "C" {
+
+int32_t optk_abi_version(void) { return OPTK_ABI_VERSION; }
+
+const char *optk_last_error(void) { return g_last_error.c_str(); }
+
+int64_t optk_sizeof(const char *name) {
+  if (!name) return -1;
+  std::string s(name);
+  if (s == "optk_affine") return sizeof(optk_affine);
+  if (s == "optk_surface") return sizeof(optk_surface);
+  if (s == "optk_layer") return sizeof(optk_layer);
+  if (s == "optk_block") return sizeof(optk_block);
+  if (s == "optk_stack") return sizeof(optk_stack);
+  if (s == "optk_chemical") return sizeof(optk_chemical);
+  if (s == "optk_system_desc") return sizeof(optk_system_desc);
+  if (s == "optk_rays") return sizeof(optk_rays);
+  if (s == "optk_grid") return sizeof(optk_grid);
+  if (s == "optk_eval_args") return sizeof(optk_eval_args);
+  if (s == "optk_collect_args") return sizeof(optk_collect_args);
+  return -1;
+}
+
+int32_t optk_system_destroy(optk_system *sys) {
+  if (!sys) return OPTK_OK;
+  (void)hipFree(sys->surfaces);
+  (void)hipFree(sys->layers);
+  (void)hipFree(sys->stacks);
+  (void)hipFree(sys->chemicals);
+  (void)hipFree(sys->tables);
+  delete sys;
+  return OPTK_OK;
+}
+
+int32_t optk_system_create(const optk_system_desc *desc, optk_system **out) {
+  if (!desc || !out) return fail(OPTK_EINVAL, "null argument");
+  *out = nullptr;
+  if ((desc->num_surfaces > 0 && !desc->surfaces) || (desc->num_layers > 0 && !desc->layers) ||
+      (desc->num_stacks > 0 && !desc->stacks) || (desc->num_chemicals > 0 && !desc->chemicals) ||
+      (desc->num_tables > 0 && !desc->tables))
+    return fail(OPTK_EINVAL, "null record array");
+  int32_t rc = validate(desc);
+  if (rc) return rc;
+  optk_system *s = new optk_system();
+  s->num_surfaces = desc->num_surfaces;
+  s->num_paths = desc->num_paths;
+  s->num_layers = desc->num_layers;
+  s->num_stacks = desc->num_stacks;
+  s->num_chemicals = desc->num_chemicals;
+  s->num_tables = desc->num_tables;
+  s->pre[0] = desc->pre[0];
+  s->pre[1] = desc->pre[1];
+  s->pre_is_inverse[0] = desc->pre_is_inverse[0];
+  s->pre_is_inverse[1] = desc->pre_is_inverse[1];
+  s->post = desc->post;
+  s->host_surfaces.assign(desc->surfaces, desc->surfaces + desc->num_surfaces);
+  if ((rc = upload(&s->surfaces, desc->surfaces, desc->num_surfaces)) ||
+      (rc = upload(&s->layers, desc->layers, desc->num_layers)) ||
+      (rc = upload(&s->stacks, desc->stacks, desc->num_stacks)) ||
+      (rc = upload(&s->chemicals, desc->chemicals, desc->num_chemicals)) ||
+      (rc = upload(&s->tables, desc->tables, desc->num_tables))) {
+    optk_system_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return OPTK_OK;
+}
+
+static void fill_out(RaysOut &o, const optk_rays *r) {
+  std::memset(&o, 0, sizeof(o));
+  if (!r) return;
+  for (int k = 0; k < 3; ++k) {
+    o.p[k] = (double *)r->position[k];
+    o.d[k] = (double *)r->direction[k];
+  }
+  o.wavelength = (double *)r->wavelength;
+  o.intensity = (double *)r->intensity;
+  o.attenuation = (double *)r->attenuation;
+  o.index = (double *)r->index_refraction;
+  o.unvignetted = r->unvignetted;
+}
+
+static int32_t prepare(const optk_system *sys, int32_t precision, const optk_rays *out,
+                       const optk_rays *accum, int64_t n, int32_t flags, TraceParams &P) {
+  if (!sys) return fail(OPTK_EINVAL, "null system");
+  if (!out) return fail(OPTK_EINVAL, "null output rays");
+  if (n < 0) return fail(OPTK_EINVAL, "negative ray count");
+  if (precision != OPTK_F64) return fail(OPTK_EINVAL, "this entry point supports OPTK_F64 only");
+  std::memset(&P, 0, sizeof(P));
+  P.spec.sys = view_of(sys);
+  P.spec.pre[0] = sys->pre[0];
+  P.spec.pre[1] = sys->pre[1];
+  P.spec.pre_is_inverse[0] = sys->pre_is_inverse[0];
+  P.spec.pre_is_inverse[1] = sys->pre_is_inverse[1];
+  P.spec.post = sys->post;
+  P.spec.apply_post = (flags & OPTK_TRACE_APPLY_POST) ? 1 : 0;
+  P.n = n;
+  fill_out(P.out, out);
+  P.has_accum = accum ? 1 : 0;
+  fill_out(P.accum, accum);
+  return OPTK_OK;
+}
+
+int32_t optk_trace(const optk_system *sys, int32_t precision, const optk_rays *in,
+                   const optk_rays *out, const optk_rays *accum, int64_t n, int32_t flags,
+                   void *hip_stream) {
+  TraceParams P;
+  int32_t rc = prepare(sys, precision, out, accum, n, flags, P);
+  if (rc) return rc;
+  if (!in) return fail(OPTK_EINVAL, "null input rays");
+  for (int k = 0; k < 3; ++k)
+    if (!in->position[k] || !in->direction[k]) return fail(OPTK_EINVAL, "input position and direction are required");
+  if (n == 0) return OPTK_OK;
+  RaysIn ri;
+  for (int k = 0; k < 3; ++k) {
+    ri.p[k] = (const double *)in->position[k];
+    ri.d[k] = (const double *)in->direction[k];
+  }
+  ri.wavelength = (const double *)in->wavelength;
+  ri.intensity = (const double *)in->intensity;
+  ri.attenuation = (const double *)in->attenuation;
+  ri.index = (const double *)in->index_refraction;
+  ri.unvignetted = in->unvignetted;
+  ri.wavelength_default = in->wavelength_default;
+  ri.intensity_default = in->intensity_default;
+  ri.attenuation_default = in->attenuation_default;
+  ri.index_default = in->index_default;
+  ri.unvignetted_default = in->unvignetted_default;
+  hipLaunchKernelGGL(trace_rays_f64, grid_for(n), dim3(256), 0, (hipStream_t)hip_stream, P, ri);
+  OPTK_HIP_CHECK(hipGetLastError());
+  return OPTK_OK;
+}
+
+int32_t optk_trace_grid(const optk_system *sys, int32_t precision, const optk_grid *in,
+                        const optk_rays *out, const optk_rays *accum, int64_t n, int32_t flags,
+                        void *hip_stream) {
+  TraceParams P;
+  int32_t rc = prepare(sys, precision, out, accum, n, flags, P);
+  if (rc) return rc;
+  if (!in) return fail(OPTK_EINVAL, "null input grid");
+  if (n == 0) return OPTK_OK;
+  GridIn g;
+  for (int k = 0; k < 2; ++k) {
+    g.field[k] = (const double *)in->field[k];
+    g.pupil[k] = (const double *)in->pupil[k];
+    g.field_default[k] = in->field_default[k];
+    g.pupil_default[k] = in->pupil_default[k];
+  }
+  g.wavelength = (const double *)in->wavelength;
+  g.intensity = (const double *)in->intensity;
+  g.wavelength_default = in->wavelength_default;
+  g.intensity_default = in->intensity_default;
+  g.object_at_infinity = in->object_at_infinity;
+  hipLaunchKernelGGL(trace_grid_f64, grid_for(n), dim3(256), 0, (hipStream_t)hip_stream, P, g);
+  OPTK_HIP_CHECK(hipGetLastError());
+  return OPTK_OK;
+}
+
+int32_t optk_eval(const optk_system *sys, int32_t op, const optk_eval_args *args, int64_t n,
+                  void *hip_stream) {
+  if (!sys || !args) return fail(OPTK_EINVAL, "null argument");
+  if (op < OPTK_OP_SAG_VALUE || op > OPTK_OP_SNELL_SCALAR) return fail(OPTK_EINVAL, "unknown op");
+  if (n < 0) return fail(OPTK_EINVAL, "negative count");
+  bool needs_surface = !(op == OPTK_OP_SNELL || op == OPTK_OP_MULTILAYER || op == OPTK_OP_SNELL_SCALAR);
+  if (needs_surface && (args->surface < 0 || args->surface >= sys->num_surfaces))
+    return fail(OPTK_EINVAL, "surface index out of range");
+  if (op == OPTK_OP_MULTILAYER && (args->stack < 0 || args->stack >= sys->num_stacks))
+    return fail(OPTK_EINVAL, "stack index out of range");
+  if ((op == OPTK_OP_INCIDENT || op == OPTK_OP_RULINGS_EFF)) {
+    const optk_surface &s = sys->host_surfaces[args->surface];
+    if (s.rulings_kind == OPTK_RULINGS_NONE) return fail(OPTK_EINVAL, "surface has no rulings");
+    if (args->iparam < 0 || args->iparam >= (s.num_orders > 1 ? s.num_orders : 1))
+      return fail(OPTK_EINVAL, "path index out of range");
+  }
+  static const int n_in[] = {3, 3, 8, 3, 8, 5, 6, 11, 10, 9, 2, 6};
+  static const int n_out[] = {1, 3, 7, 0, 3, 12, 3, 3, 1, 1, 1, 2};
+  for (int k = 0; k < n_in[op]; ++k)
+    if (!args->in[k]) return fail(OPTK_EINVAL, "missing input array");
+  for (int k = 0; k < n_out[op]; ++k)
+    if (!args->out[k]) return fail(OPTK_EINVAL, "missing output array");
+  if (op == OPTK_OP_APERTURE && !args->out_mask) return fail(OPTK_EINVAL, "missing output mask");
+  if (n == 0) return OPTK_OK;
+  EvalParams E;
+  std::memset(&E, 0, sizeof(E));
+  E.sys = view_of(sys);
+  E.op = op;
+  E.surface = args->surface;
+  E.iparam = args->iparam;
+  E.stack = args->stack;
+  E.n = n;
+  for (int k = 0; k < 12; ++k) {
+    E.in[k] = args->in[k];
+    E.out[k] = args->out[k];
+  }
+  E.out_mask = args->out_mask;
+  hipLaunchKernelGGL(eval_f64, grid_for(n), dim3(256), 0, (hipStream_t)hip_stream, E);
+  OPTK_HIP_CHECK(hipGetLastError());
+  return OPTK_OK;
+}
+
+int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
+  if (!a) return fail(OPTK_EINVAL, "null argument");
+  if (n < 0) return fail(OPTK_EINVAL, "negative count");
+  if (a->n_wavelength < 1 || a->nx < 1 || a->ny < 1) return fail(OPTK_EINVAL, "empty histogram");
+  if (!a->position[0] || !a->position[1] || !a->direction_z || !a->wavelength || !a->intensity ||
+      !a->wavelength_edges || !a->image || !a->moment)
+    return fail(OPTK_EINVAL, "missing array");
+  if (!(a->x_hi > a->x_lo) || !(a->y_hi > a->y_lo)) return fail(OPTK_EINVAL, "empty pixel range");
+  if (a->precision != OPTK_F64 && a->precision != OPTK_F32) return fail(OPTK_EINVAL, "bad precision");
+  if (n == 0) return OPTK_OK;
+  CollectParams C;
+  std::memset(&C, 0, sizeof(C));
+  if (a->precision == OPTK_F64) {
+    C.px = (const double *)a->position[0];
+    C.py = (const double *)a->position[1];
+    C.dz = (const double *)a->direction_z;
+    C.wavelength = (const double *)a->wavelength;
+    C.intensity = (const double *)a->intensity;
+  } else {
+    C.pxf = (const float *)a->position[0];
+    C.pyf = (const float *)a->position[1];
+    C.dzf = (const float *)a->direction_z;
+    C.wavelengthf = (const float *)a->wavelength;
+    C.intensityf = (const float *)a->intensity;
+  }
+  C.unvignetted = a->unvignetted;
+  C.wedges = a->wavelength_edges;
+  C.nw = a->n_wavelength;
+  C.nx = a->nx;
+  C.ny = a->ny;
+  C.x_lo = a->x_lo;
+  C.x_hi = a->x_hi;
+  C.y_lo = a->y_lo;
+  C.y_hi = a->y_hi;
+  C.image = a->image;
+  C.moment = a->moment;
+  C.n = n;
+  hipLaunchKernelGGL(collect_kernel, grid_for(n), dim3(256), 0, (hipStream_t)hip_stream, C);
+  OPTK_HIP_CHECK(hipGetLastError());
+  return OPTK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,489 @@
+// optk_physics.h -- per-ray physics of the sequential trace, shared by the
+// gfx950 kernels (hipcc) and a host build used only by the CPU test harness.
+//
+// One thread owns one ray.  Everything below is straight-line per-ray code
+// over a uniform surface record: branches on record fields are uniform
+// across a wavefront (the compiler keeps the record in scalar registers),
+// branches on ray data become selects.
+//
+// Float64 parity with the reference's numpy arithmetic: expressions keep the
+// reference's operation order, the library is compiled with
+// -ffp-contract=off, and complex arithmetic follows numpy's loops
+// (componentwise products, Smith's division) so that the float64 kernel
+// agrees with a numpy float64 evaluation to rounding (correctly rounded
+// +,-,*,/,sqrt; transcendental functions to within an ulp).
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#include "../../include/optk.h"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define OPTK_HD __host__ __device__ __forceinline__
+#else
+#define OPTK_HD inline
+#endif
+
+namespace optk {
+
+// ---------------------------------------------------------------------------
+// real helpers
+// ---------------------------------------------------------------------------
+
+template <typename T> OPTK_HD T sign_of(T x) {  // numpy.sign
+  return x > T(0) ? T(1) : (x < T(0) ? T(-1) : (x == T(0) ? T(0) : x));
+}
+
+template <typename T> struct V3 {
+  T x, y, z;
+};
+
+template <typename T> OPTK_HD T dot3(const V3<T> &a, const V3<T> &b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <typename T> OPTK_HD T len3(const V3<T> &a) {
+  return sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+}
+template <typename T> OPTK_HD V3<T> cross3(const V3<T> &a, const V3<T> &b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <typename T> OPTK_HD V3<T> div3(const V3<T> &a, T s) {
+  return {a.x / s, a.y / s, a.z / s};
+}
+
+// ---------------------------------------------------------------------------
+// complex arithmetic with numpy's semantics
+// ---------------------------------------------------------------------------
+
+template <typename T> struct C {
+  T re, im;
+};
+
+template <typename T> OPTK_HD C<T> cadd(C<T> a, C<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T> OPTK_HD C<T> csub(C<T> a, C<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T> OPTK_HD C<T> cmul(C<T> a, C<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// real * complex (numpy promotes the real operand; the zero imaginary part
+// contributes exact zeros for finite values)
+template <typename T> OPTK_HD C<T> cscale(T a, C<T> b) { return {a * b.re, a * b.im}; }
+template <typename T> OPTK_HD C<T> cconj(C<T> a) { return {a.re, -a.im}; }
+
+// Smith's division, as numpy's complex divide loop.
+template <typename T> OPTK_HD C<T> cdiv(C<T> a, C<T> b) {
+  T br_abs = fabs(b.re), bi_abs = fabs(b.im);
+  if (br_abs >= bi_abs) {
+    if (br_abs == T(0) && bi_abs == T(0)) {
+      return {a.re / br_abs, a.im / br_abs};
+    }
+    T rat = b.im / b.re;
+    T scl = T(1) / (b.re + b.im * rat);
+    return {(a.re + a.im * rat) * scl, (a.im - a.re * rat) * scl};
+  }
+  T rat = b.re / b.im;
+  T scl = T(1) / (b.im + b.re * rat);
+  return {(a.re * rat + a.im) * scl, (a.im * rat - a.re) * scl};
+}
+
+template <typename T> OPTK_HD T cabs(C<T> a) { return hypot(a.re, a.im); }
+
+// Principal square root (C99 csqrt algorithm: hypot-based, cancellation-free
+// branch selection; the extreme-magnitude rescaling branches are omitted,
+// the trace never reaches them).
+template <typename T> OPTK_HD C<T> csqrt(C<T> z) {
+  T x = z.re, y = z.im;
+  if (y == T(0)) {
+    if (x < T(0)) return {T(0), copysign(sqrt(-x), y)};
+    return {fabs(sqrt(x)), copysign(T(0), y)};
+  }
+  if (x == T(0)) {
+    T r = sqrt(T(0.5) * fabs(y));
+    return {r, copysign(r, y)};
+  }
+  T d = hypot(x, y);
+  T r, s;
+  if (x > T(0)) {
+    r = sqrt(T(0.5) * (d + x));
+    s = T(0.5) * (y / r);
+  } else {
+    s = sqrt(T(0.5) * (d - x));
+    r = fabs(T(0.5) * (y / s));
+  }
+  return {r, copysign(s, y)};
+}
+
+// exp(x + i y) = e^x (cos y + i sin y)
+template <typename T> OPTK_HD C<T> cexp(C<T> z) {
+  T e = exp(z.re);
+  T s, c;
+#if defined(__HIP_DEVICE_COMPILE__)
+  sincos(z.im, &s, &c);
+#else
+  s = sin(z.im);
+  c = cos(z.im);
+#endif
+  return {e * c, e * s};
+}
+
+template <typename T> struct M2 {
+  C<T> a, b, c, d;  // [[a, b], [c, d]]
+};
+
+template <typename T> OPTK_HD M2<T> mmul(const M2<T> &x, const M2<T> &y) {
+  return {cadd(cmul(x.a, y.a), cmul(x.b, y.c)), cadd(cmul(x.a, y.b), cmul(x.b, y.d)),
+          cadd(cmul(x.c, y.a), cmul(x.d, y.c)), cadd(cmul(x.c, y.b), cmul(x.d, y.d))};
+}
+template <typename T> OPTK_HD M2<T> midentity() {
+  return {{T(1), T(0)}, {T(0), T(0)}, {T(0), T(0)}, {T(1), T(0)}};
+}
+template <typename T> OPTK_HD M2<T> mselect(bool cond, const M2<T> &x, const M2<T> &y) {
+  return cond ? x : y;
+}
+
+// ---------------------------------------------------------------------------
+// transformations
+// ---------------------------------------------------------------------------
+
+template <typename T> OPTK_HD void xf_apply_pos(const optk_affine &a, T &x, T &y, T &z) {
+  if (a.kind == OPTK_XF_IDENTITY) return;
+  if (a.kind == OPTK_XF_AFFINE) {
+    T X = T(a.m[0]) * x + T(a.m[1]) * y + T(a.m[2]) * z;
+    T Y = T(a.m[3]) * x + T(a.m[4]) * y + T(a.m[5]) * z;
+    T Z = T(a.m[6]) * x + T(a.m[7]) * y + T(a.m[8]) * z;
+    x = X;
+    y = Y;
+    z = Z;
+  }
+  x = x + T(a.t[0]);
+  y = y + T(a.t[1]);
+  z = z + T(a.t[2]);
+}
+template <typename T> OPTK_HD void xf_apply_dir(const optk_affine &a, T &x, T &y, T &z) {
+  if (a.kind != OPTK_XF_AFFINE) return;
+  T X = T(a.m[0]) * x + T(a.m[1]) * y + T(a.m[2]) * z;
+  T Y = T(a.m[3]) * x + T(a.m[4]) * y + T(a.m[5]) * z;
+  T Z = T(a.m[6]) * x + T(a.m[7]) * y + T(a.m[8]) * z;
+  x = X;
+  y = Y;
+  z = Z;
+}
+template <typename T> OPTK_HD void xf_inverse_pos(const optk_affine &a, T &x, T &y, T &z) {
+  if (a.kind == OPTK_XF_IDENTITY) return;
+  x = x - T(a.t[0]);
+  y = y - T(a.t[1]);
+  z = z - T(a.t[2]);
+  if (a.kind == OPTK_XF_AFFINE) {
+    T X = T(a.m[0]) * x + T(a.m[3]) * y + T(a.m[6]) * z;
+    T Y = T(a.m[1]) * x + T(a.m[4]) * y + T(a.m[7]) * z;
+    T Z = T(a.m[2]) * x + T(a.m[5]) * y + T(a.m[8]) * z;
+    x = X;
+    y = Y;
+    z = Z;
+  }
+}
+template <typename T> OPTK_HD void xf_inverse_dir(const optk_affine &a, T &x, T &y, T &z) {
+  if (a.kind != OPTK_XF_AFFINE) return;
+  T X = T(a.m[0]) * x + T(a.m[3]) * y + T(a.m[6]) * z;
+  T Y = T(a.m[1]) * x + T(a.m[4]) * y + T(a.m[7]) * z;
+  T Z = T(a.m[2]) * x + T(a.m[5]) * y + T(a.m[8]) * z;
+  x = X;
+  y = Y;
+  z = Z;
+}
+
+// ---------------------------------------------------------------------------
+// ray state
+// ---------------------------------------------------------------------------
+
+template <typename T> struct Ray {
+  T px, py, pz;
+  T dx, dy, dz;
+  T wavelength, intensity, attenuation, index;
+  bool unvignetted;
+};
+
+// ---------------------------------------------------------------------------
+// 1-D linear interpolation with numpy.interp's rules (clamped ends, exact
+// knots), over a table of `count` ascending abscissae.
+// ---------------------------------------------------------------------------
+
+OPTK_HD int interp_bracket(const double *xp, int count, double x) {
+  // index j with xp[j] <= x < xp[j+1]; assumes xp[0] <= x < xp[count-1]
+  int lo = 0, hi = count - 1;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (x < xp[mid])
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return lo;
+}
+
+OPTK_HD double interp_at(const double *xp, const double *fp, int count, double x, int j) {
+  if (x == xp[j]) return fp[j];
+  double slope = (fp[j + 1] - fp[j]) / (xp[j + 1] - xp[j]);
+  double r = slope * (x - xp[j]) + fp[j];
+  if (r != r) {
+    r = slope * (x - xp[j + 1]) + fp[j + 1];
+    if (r != r && fp[j] == fp[j + 1]) r = fp[j];
+  }
+  return r;
+}
+
+OPTK_HD double interp1(const double *xp, const double *fp, int count, double x) {
+  if (!(x >= xp[0])) return (x != x) ? x : fp[0];  // left clamp (NaN passes)
+  if (x >= xp[count - 1]) return fp[count - 1];
+  int j = interp_bracket(xp, count, x);
+  return interp_at(xp, fp, count, x, j);
+}
+
+// n + i k of a chemical at wavelength (nm); the table is in Angstrom.
+// numpy.interp of a complex fp interpolates both parts with one reciprocal
+// of the knot spacing (numpy's complex interp loop); the same is done here.
+OPTK_HD C<double> chemical_n(const optk_chemical &ch, const double *tables, double wavelength_nm) {
+  const double *w = tables + ch.offset;
+  const double *n = w + ch.count;
+  const double *k = n + ch.count;
+  double x = wavelength_nm * 10.0;
+  int cnt = ch.count;
+  if (!(x >= w[0])) {
+    if (x != x) return {x, x};
+    return {n[0], k[0]};
+  }
+  if (x >= w[cnt - 1]) return {n[cnt - 1], k[cnt - 1]};
+  int j = interp_bracket(w, cnt, x);
+  if (x == w[j]) return {n[j], k[j]};
+  double inv_dx = 1.0 / (w[j + 1] - w[j]);
+  double sre = (n[j + 1] - n[j]) * inv_dx;
+  double sim = (k[j + 1] - k[j]) * inv_dx;
+  double re = sre * (x - w[j]) + n[j];
+  double im = sim * (x - w[j]) + k[j];
+  if (re != re) {
+    re = sre * (x - w[j + 1]) + n[j + 1];
+    if (re != re && n[j] == n[j + 1]) re = n[j];
+  }
+  if (im != im) {
+    im = sim * (x - w[j + 1]) + k[j + 1];
+    if (im != im && k[j] == k[j + 1]) im = k[j];
+  }
+  return {re, im};
+}
+
+// ---------------------------------------------------------------------------
+// multilayer transfer matrices (optika/materials/_layers.py, matrices.py,
+// _multilayers.py, profiles.py)
+// ---------------------------------------------------------------------------
+
+struct StackView {
+  const optk_layer *layers;
+  const optk_stack *stacks;
+  const optk_chemical *chemicals;
+  const double *tables;
+};
+
+// snells_law_scalar (optika/materials/_snells_law.py:13-38)
+template <typename T> OPTK_HD C<T> snell_scalar(C<T> cos_i, C<T> n1, C<T> n2) {
+  C<T> one = {T(1), T(0)};
+  C<T> sin_i = csqrt(csub(one, cmul(cos_i, cos_i)));
+  C<T> sin_t = cdiv(cmul(n1, sin_i), n2);
+  return csqrt(csub(one, cmul(sin_t, sin_t)));
+}
+
+// interface roughness factor (profiles.py:103-126 and each profile)
+template <typename T>
+OPTK_HD T profile_factor(int kind, T width, T wavelength, C<T> direction, C<T> n) {
+  // k = -2 pi n direction / wavelength ; s = Re(-2 k)
+  C<T> k = cdiv(cmul(cscale(T(-2) * T(M_PI), n), direction), C<T>{wavelength, T(0)});
+  T s = T(-2) * k.re;
+  switch (kind) {
+  case OPTK_PROFILE_ERF: {
+    T sw = s * width;
+    return exp(-(sw * sw) / T(2));
+  }
+  case OPTK_PROFILE_EXPONENTIAL: {
+    T sw = s * width;
+    return T(1) / (T(1) + (sw * sw) / T(2));
+  }
+  case OPTK_PROFILE_LINEAR: {
+    T x = T(1.7320508075688772) * width * s;
+    return sin(x) / x;
+  }
+  case OPTK_PROFILE_SINUSOIDAL: {
+    T a = T(M_PI) / (T(M_PI) * T(M_PI) - T(8));
+    T x = a * width * s;
+    T x1 = x - T(M_PI) / T(2);
+    T x2 = x + T(M_PI) / T(2);
+    return T(M_PI) * (sin(x1) / x1 + sin(x2) / x2) / T(4);
+  }
+  default:
+    return T(1);
+  }
+}
+
+struct LayerState {
+  C<double> n;          // index of the medium before the next layer
+  C<double> direction;  // cosine in that medium
+  bool where;           // transfer matrices still valid
+};
+
+// Layer.transfer (optika/materials/_layers.py:229-277) for one polarization.
+OPTK_HD M2<double> layer_transfer(const optk_layer &layer, const StackView &sv, double wavelength,
+                                  bool pol_s, LayerState &st, double thickness) {
+  C<double> n_int = {1.0, 0.0};
+  if (layer.chemical >= 0) n_int = chemical_n(sv.chemicals[layer.chemical], sv.tables, wavelength);
+  C<double> d_int = snell_scalar(st.direction, st.n, n_int);
+
+  // matrices.refraction (matrices.py:127-161)
+  C<double> one = {1.0, 0.0};
+  C<double> di = pol_s ? st.direction : cconj(st.direction);
+  C<double> dj = pol_s ? d_int : cconj(d_int);
+  C<double> imp_i = pol_s ? st.n : cdiv(one, st.n);
+  C<double> imp_j = pol_s ? n_int : cdiv(one, n_int);
+  C<double> qi = cmul(di, imp_i);
+  C<double> qj = cmul(dj, imp_j);
+  C<double> a = cadd(qi, qj);
+  C<double> r = cdiv(csub(qi, qj), a);
+  C<double> t = cdiv(cscale(2.0, qi), a);
+  if (layer.profile_kind != OPTK_PROFILE_NONE) {
+    double f = profile_factor<double>(layer.profile_kind, layer.profile_width, wavelength, di, st.n);
+    r = cscale(f, r);  // r * factor (real factor promoted to complex)
+  }
+  C<double> inv_t = cdiv(one, t);
+  C<double> r_t = cdiv(r, t);
+  M2<double> refraction = {inv_t, r_t, r_t, inv_t};
+  if (!st.where) refraction = midentity<double>();
+
+  // matrices.propagation (matrices.py:241-246):
+  // beta = 2 pi h n cos / wavelength ; U = diag(exp(-i beta), exp(i beta))
+  C<double> beta =
+      cdiv(cmul(cscale(2.0 * M_PI * thickness, n_int), d_int), C<double>{wavelength, 0.0});
+  // -1j * beta = (beta.im, -beta.re) ; +1j * beta = (-beta.im, beta.re)
+  C<double> u00 = cexp(C<double>{beta.im, -beta.re});
+  C<double> u11 = cexp(C<double>{-beta.im, beta.re});
+  bool where_prop = cabs(u00) < 1e10;
+  st.where = st.where && where_prop;
+  M2<double> zero_prop = {u00, {0.0, 0.0}, {0.0, 0.0}, u11};
+  M2<double> transfer = mmul(refraction, zero_prop);
+  if (!st.where) transfer = refraction;
+  st.n = n_int;
+  st.direction = d_int;
+  return transfer;
+}
+
+OPTK_HD M2<double> matrix_power(M2<double> m, int exponent) {
+  // binary powering, numpy.linalg.matrix_power's order of products
+  if (exponent <= 0) return midentity<double>();
+  M2<double> z = m, result = m;
+  bool have_z = false, have_result = false;
+  int e = exponent;
+  while (e > 0) {
+    if (have_z) z = mmul(z, z);
+    have_z = true;
+    int bit = e & 1;
+    e >>= 1;
+    if (bit) {
+      result = have_result ? mmul(result, z) : z;
+      have_result = true;
+    }
+  }
+  return result;
+}
+
+OPTK_HD M2<double> sequence_transfer(const StackView &sv, int first, int count, double wavelength,
+                                     bool pol_s, LayerState &st) {
+  M2<double> result = midentity<double>();
+  for (int i = 0; i < count; ++i) {
+    const optk_layer &L = sv.layers[first + i];
+    result = mmul(result, layer_transfer(L, sv, wavelength, pol_s, st, L.thickness));
+  }
+  return result;
+}
+
+struct MultilayerOut {
+  C<double> r[2];  // s, p
+  C<double> t[2];
+  bool where[2];
+};
+
+// multilayer_coefficients (optika/materials/_multilayers.py:186-237)
+OPTK_HD MultilayerOut multilayer_rt(const StackView &sv, int stack_index, double wavelength,
+                                    C<double> direction, C<double> n_ambient) {
+  const optk_stack &stk = sv.stacks[stack_index];
+  MultilayerOut out;
+  for (int p = 0; p < 2; ++p) {
+    bool pol_s = (p == 0);
+    LayerState st = {n_ambient, direction, true};
+    M2<double> m = midentity<double>();
+    bool first_block = true;
+    for (int b = 0; b < stk.num_blocks; ++b) {
+      const optk_block &blk = stk.blocks[b];
+      M2<double> x;
+      if (blk.kind == OPTK_BLOCK_PERIODIC) {
+        M2<double> start =
+            sequence_transfer(sv, blk.first_layer, blk.num_layers, wavelength, pol_s, st);
+        M2<double> periodic =
+            sequence_transfer(sv, blk.first_layer, blk.num_layers, wavelength, pol_s, st);
+        x = mmul(start, matrix_power(periodic, blk.num_periods - 1));
+      } else {
+        x = sequence_transfer(sv, blk.first_layer, blk.num_layers, wavelength, pol_s, st);
+      }
+      // a single top-level block is returned by its own transfer(); several
+      // blocks are folded left from the identity (LayerSequence.transfer)
+      m = (first_block && stk.num_blocks == 1) ? x : mmul(m, x);
+      first_block = false;
+    }
+    // substrate (thickness forced to zero)
+    optk_layer sub = {-1, OPTK_PROFILE_NONE, 0.0, 0.0};
+    if (stk.substrate_layer >= 0) sub = sv.layers[stk.substrate_layer];
+    M2<double> ms = layer_transfer(sub, sv, wavelength, pol_s, st, 0.0);
+    M2<double> mt = mmul(m, ms);
+    out.r[p] = cdiv(mt.c, mt.a);
+    C<double> t = cdiv(C<double>{1.0, 0.0}, mt.a);
+    if (!st.where) t = {0.0, 0.0};
+    out.t[p] = t;
+    out.where[p] = st.where;
+  }
+  return out;
+}
+
+struct MultilayerRT {
+  double R[2], T[2];
+};
+
+// multilayer_efficiency (optika/materials/_multilayers.py:485-532)
+OPTK_HD MultilayerRT multilayer_efficiency(const StackView &sv, int stack_index, double wavelength,
+                                           C<double> direction, C<double> n_ambient,
+                                           bool want_transmission, MultilayerOut *coeffs = nullptr) {
+  MultilayerOut co = multilayer_rt(sv, stack_index, wavelength, direction, n_ambient);
+  if (coeffs) *coeffs = co;
+  MultilayerRT out;
+  for (int p = 0; p < 2; ++p) {
+    double a = cabs(co.r[p]);
+    out.R[p] = a * a;
+    out.T[p] = 0.0;
+  }
+  if (want_transmission) {
+    const optk_stack &stk = sv.stacks[stack_index];
+    C<double> n_sub = {1.0, 0.0};
+    if (stk.substrate_layer >= 0) {
+      const optk_layer &sub = sv.layers[stk.substrate_layer];
+      if (sub.chemical >= 0) n_sub = chemical_n(sv.chemicals[sub.chemical], sv.tables, wavelength);
+    }
+    C<double> d_sub = snell_scalar(direction, n_ambient, n_sub);
+    C<double> one = {1.0, 0.0};
+    C<double> q_amb_s = cmul(direction, n_ambient);
+    C<double> q_amb_p = cmul(cconj(direction), cdiv(one, n_ambient));
+    C<double> q_sub_s = cmul(d_sub, n_sub);
+    C<double> q_sub_p = cmul(cconj(d_sub), cdiv(one, n_sub));
+    double ts = cabs(co.t[0]);
+    double tp = cabs(co.t[1]);
+    out.T[0] = (ts * ts) * cdiv(q_sub_s, q_amb_s).re;
+    out.T[1] = (tp * tp) * cdiv(q_sub_p, q_amb_p).re;
+  }
+  return out;
+}
+
+}  // namespace optk

@@ -1,0 +1,1155 @@
+"""numpy float64 restatement of the reference trace path (test-only oracle).
+
+Rays are dicts of numpy arrays: ``wavelength`` (N,), ``position`` (3, N),
+``direction`` (3, N), ``intensity``, ``attenuation``, ``index_refraction``
+(N,) and ``unvignetted`` (N,) bool.  Units: mm, nm, degrees, 1/mm.
+
+The system description is read from the host configuration objects of
+``optika_amd`` by attribute name only (radius, conic, half_width, ...);
+every derived quantity (rotation matrices, polygon vertices, sensor
+aperture, n/k interpolation) is recomputed here independently.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import scipy.special
+
+__all__ = [
+    "make_rays",
+    "transform_apply",
+    "transform_inverse",
+    "sag_value",
+    "sag_normal",
+    "sag_intercept",
+    "sag_propagate",
+    "snells_law",
+    "snells_law_scalar",
+    "nk_interp",
+    "multilayer_coefficients",
+    "multilayer_efficiency",
+    "spacing_vector",
+    "incident_effective",
+    "rulings_efficiency",
+    "material_index_refraction",
+    "material_attenuation",
+    "material_efficiency",
+    "aperture_mask",
+    "point_in_polygon",
+    "surface_propagate",
+    "propagate",
+    "rayfunction_input",
+    "raytrace",
+    "rayfunction",
+    "collect",
+    "direction_from_angles",
+]
+
+WAVELENGTH_MM = 1e-6  # nm -> mm
+
+
+# ---------------------------------------------------------------------------
+# rays and vectors
+# ---------------------------------------------------------------------------
+
+
+def make_rays(wavelength, position, direction, intensity=1.0, attenuation=0.0,
+              index_refraction=1.0, unvignetted=True) -> dict:
+    """Broadcast ray fields to (N,) / (3, N) float64 arrays."""
+    position = np.asarray(position, dtype=np.float64)
+    direction = np.asarray(direction, dtype=np.float64)
+    n = max(np.size(wavelength), position.shape[-1] if position.ndim > 1 else 1,
+            direction.shape[-1] if direction.ndim > 1 else 1, np.size(intensity),
+            np.size(attenuation), np.size(index_refraction), np.size(unvignetted))
+
+    def vec(v):
+        v = np.asarray(v, dtype=np.float64)
+        if v.ndim == 1:
+            v = v[:, None]
+        return np.ascontiguousarray(np.broadcast_to(v, (3, n)))
+
+    def sca(v, dt=np.float64):
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=dt), (n,)))
+
+    return dict(
+        wavelength=sca(wavelength),
+        position=vec(position),
+        direction=vec(direction),
+        intensity=sca(intensity),
+        attenuation=sca(attenuation),
+        index_refraction=sca(index_refraction),
+        unvignetted=sca(unvignetted, bool),
+    )
+
+
+def _copy(rays: dict) -> dict:
+    return {k: v.copy() for k, v in rays.items()}
+
+
+def _dot(a, b):
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+
+
+def _length(a):
+    return np.sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2])
+
+
+def _cross(a, b):
+    return np.stack([
+        a[1] * b[2] - a[2] * b[1],
+        a[2] * b[0] - a[0] * b[2],
+        a[0] * b[1] - a[1] * b[0],
+    ])
+
+
+def _vec(v):
+    """A host vector config (object with x, y, z) as a (3, 1) array."""
+    return np.array([[float(v.x)], [float(v.y)], [float(v.z)]])
+
+
+# ---------------------------------------------------------------------------
+# transformations (named_arrays.transformations; convention fixed from
+# optika/systems/_sequential.py:1853-1912 -- see oracle/__init__.py)
+# ---------------------------------------------------------------------------
+
+
+def _rotation_matrix(axis: str, angle_deg: float) -> np.ndarray:
+    a = np.deg2rad(np.float64(angle_deg))
+    c, s = np.cos(a), np.sin(a)
+    if axis == "x":
+        return np.array([[1, 0, 0], [0, c, -s], [0, s, c]], dtype=np.float64)
+    if axis == "y":
+        return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]], dtype=np.float64)
+    return np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]], dtype=np.float64)
+
+
+def _primitives(transformation) -> list[tuple]:
+    """Flatten a transformation into ``("t", vector)`` / ``("r", matrix)``
+    steps in application order."""
+    out = []
+    for e in transformation.elements():
+        name = type(e).__name__
+        if name == "Cartesian3dTranslation":
+            out.append(("t", np.array([e.x, e.y, e.z], dtype=np.float64)))
+        elif name.startswith("Cartesian3dRotation"):
+            out.append(("r", _rotation_matrix(name[-1].lower(), e.angle)))
+        else:
+            raise TypeError(f"unknown transformation {name}")
+    return out
+
+
+def _matvec(m, v):
+    return np.stack([
+        m[0, 0] * v[0] + m[0, 1] * v[1] + m[0, 2] * v[2],
+        m[1, 0] * v[0] + m[1, 1] * v[1] + m[1, 2] * v[2],
+        m[2, 0] * v[0] + m[2, 1] * v[1] + m[2, 2] * v[2],
+    ])
+
+
+def _apply_steps(steps, position, direction):
+    for kind, value in steps:
+        if kind == "t":
+            position = position + value[:, None]
+        else:
+            position = _matvec(value, position)
+            if direction is not None:
+                direction = _matvec(value, direction)
+    return position, direction
+
+
+def _inverse_steps(steps):
+    inv = []
+    for kind, value in reversed(steps):
+        if kind == "t":
+            inv.append(("t", -value))
+        else:
+            inv.append(("r", value.T.copy()))
+    return inv
+
+
+def transform_apply(transformation, position, direction=None):
+    """Local -> parent frame (``RayVectorArray.__array_matmul__/__array_add__``,
+    ``optika/rays/_ray_vectors.py:105-166``)."""
+    if transformation is None:
+        return position, direction
+    return _apply_steps(_primitives(transformation), position, direction)
+
+
+def transform_inverse(transformation, position, direction=None):
+    """Parent -> local frame."""
+    if transformation is None:
+        return position, direction
+    return _apply_steps(_inverse_steps(_primitives(transformation)), position, direction)
+
+
+def _rays_inverse(transformation, rays):
+    if transformation is None:
+        return rays
+    r = dict(rays)
+    r["position"], r["direction"] = transform_inverse(transformation, rays["position"], rays["direction"])
+    return r
+
+
+def _rays_apply(transformation, rays):
+    if transformation is None:
+        return rays
+    r = dict(rays)
+    r["position"], r["direction"] = transform_apply(transformation, rays["position"], rays["direction"])
+    return r
+
+
+# ---------------------------------------------------------------------------
+# sags (optika/sags/*.py)
+# ---------------------------------------------------------------------------
+
+
+def _sag_kind(sag) -> str:
+    return type(sag).__name__
+
+
+def sag_value(sag, position):
+    """``AbstractSag.__call__`` for each sag type."""
+    kind = _sag_kind(sag)
+    T = sag.transformation
+    if kind == "NoSag":  # _flat.py:26-39
+        p, _ = transform_inverse(T, position)
+        p = np.stack([p[0], p[1], np.zeros_like(p[2])])
+        p, _ = transform_apply(T, p)
+        return p[2]
+    p, _ = transform_inverse(T, position)
+    x, y = p[0], p[1]
+    if kind == "SphericalSag":  # _spherical.py:100-123
+        c = 1 / sag.radius
+        r2 = np.square(x) + np.square(y)
+        return c * r2 / (1 + np.sqrt(1 - np.square(c) * r2))
+    if kind in ("ConicSag", "ParabolicSag"):  # _conic.py:31-48
+        c = 1 / sag.radius
+        conic = sag.conic
+        r2 = np.square(x) + np.square(y)
+        return c * r2 / (1 + np.sqrt(1 - (1 + conic) * np.square(c) * r2))
+    if kind == "CylindricalSag":  # _cylindrical.py:75-91
+        c = 1 / sag.radius
+        r2 = np.square(x)
+        return c * r2 / (1 + np.sqrt(1 - np.square(c) * r2))
+    if kind == "ToroidalSag":  # _toroidal.py:38-57
+        c = 1 / sag.radius
+        r = sag.radius_of_rotation
+        x2 = np.square(x)
+        y2 = np.square(y)
+        zy = c * y2 / (1 + np.sqrt(1 - np.square(c) * y2))
+        return r - np.sqrt(np.square(r - zy) - x2)
+    raise TypeError(kind)
+
+
+def sag_normal(sag, position):
+    """``AbstractSag.normal`` for each sag type (the sag transformation acts
+    on the position only, as in the reference)."""
+    kind = _sag_kind(sag)
+    n = position.shape[-1]
+    if kind == "NoSag":  # _flat.py:41-45
+        return np.stack([np.zeros(n), np.zeros(n), -np.ones(n)])
+    p, _ = transform_inverse(sag.transformation, position)
+    x, y = p[0], p[1]
+    if kind == "SphericalSag":  # _spherical.py:125-146
+        c = 1 / sag.radius
+        nx = c * x
+        ny = c * y
+        nz = -np.sqrt(1 - nx * nx - ny * ny)
+        return np.stack([nx, ny, nz])
+    if kind == "ConicSag":  # _conic.py:50-75
+        c = 1 / sag.radius
+        conic = sag.conic
+        x2 = np.square(x)
+        y2 = np.square(y)
+        c2 = np.square(c)
+        g = np.sqrt(1 - (1 + conic) * c2 * (x2 + y2))
+        dzdx, dzdy = c * x / g, c * y / g
+        v = np.stack([dzdx, dzdy, -np.ones_like(dzdx)])
+        return v / _length(v)
+    if kind == "ParabolicSag":  # _parabolic.py:40-63 (numexpr)
+        r = sag.radius
+        q = np.sqrt((x / r) * (x / r) + (y / r) * (y / r) + 1)
+        return np.stack([x / q / r, y / q / r, np.full_like(x, -r) / q / r])
+    if kind == "CylindricalSag":  # _cylindrical.py:93-113
+        r = sag.radius
+        nx = x / r
+        ny = np.zeros_like(nx)
+        nz = -np.sqrt(1 - nx * nx)
+        return np.stack([nx, ny, nz])
+    if kind == "ToroidalSag":  # _toroidal.py:59-88
+        c = 1 / sag.radius
+        r = sag.radius_of_rotation
+        x2 = np.square(x)
+        y2 = np.square(y)
+        c2 = np.square(c)
+        g = np.sqrt(1 - c2 * y2)
+        zy = c * y2 / (1 + g)
+        f = np.sqrt(np.square(r - zy) - x2)
+        dzdx = x / f
+        dzydy = c * y / g
+        dzdy = (r - zy) * dzydy / f
+        v = np.stack([dzdx, dzdy, -np.ones_like(dzdx)])
+        return v / _length(v)
+    raise TypeError(kind)
+
+
+def _intercept_conic(sag, o, u):
+    """Closed-form quadric intercept (``_conic.py:77-170``)."""
+    c = 1 / sag.radius
+    kp1 = 1 + sag.conic
+    a = c * (np.square(u[0]) + np.square(u[1]) + kp1 * np.square(u[2]))
+    b = 2 * (c * (o[0] * u[0] + o[1] * u[1] + kp1 * o[2] * u[2]) - u[2])
+    cc = c * (np.square(o[0]) + np.square(o[1]) + kp1 * np.square(o[2])) - 2 * o[2]
+    disc = np.square(b) - 4 * a * cc
+    real = disc >= 0
+    sqrt_disc = np.sqrt(np.where(real, disc, 0))
+    degenerate = np.abs(a) < 1e-12
+    denominator = np.where(degenerate, 1.0, 2 * a)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t_linear = -cc / b
+
+        def root(sign):
+            t = np.where(degenerate, t_linear, (-b + sign * sqrt_disc) / denominator)
+            pos = o + u * t
+            r2 = np.square(pos[0]) + np.square(pos[1])
+            on_sheet = (pos[2] * (c * r2 - pos[2])) >= 0
+            valid = real & on_sheet
+            return np.where(valid, t, np.inf)
+
+        t_a = root(-1)
+        t_b = root(+1)
+    t = np.where(np.abs(t_a) <= np.abs(t_b), t_a, t_b)
+    return o + u * t
+
+
+def _intercept_parabola(sag, o, u):
+    """numexpr expression of ``_parabolic.py:142-151``."""
+    f = sag.focal_length
+    ox, oy, oz = o
+    ux, uy, uz = u
+    den = ux * ux + uy * uy
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tq = (
+            (-ox) * ux - oy * uy + 2 * f * uz
+            - np.sign(f * uz) * np.sqrt(
+                -((oy * ux) * (oy * ux)) - (ox * uy) * (ox * uy)
+                + 2 * oy * uy * (ox * ux - 2 * f * uz)
+                + 4 * f * (oz * (ux * ux + uy * uy) - ox * ux * uz + f * (uz * uz))
+            )
+        ) / (ux * ux + uy * uy)
+        tl = (ox * ox + oy * oy - 4 * f * oz) / (4 * f * uz)
+    t = np.where(den > 1e-10, tq, tl)
+    return o + u * t
+
+
+def _intercept_sphere(sag, o, u):
+    """numexpr expression of ``_spherical.py:148-191``."""
+    r = sag.radius
+    px, py, pz = o[0], o[1], o[2] - r
+    ux, uy, uz = u
+    s = ux * px + uy * py + uz * pz
+    with np.errstate(invalid="ignore"):
+        t = -s - np.sign(r * uz) * np.sqrt(s * s - (px * px + py * py + pz * pz - r * r))
+    return o + u * t
+
+
+def _intercept_cylinder(sag, o, u):
+    """Cross-product intercept of ``_cylindrical.py:115-160`` (axis along y)."""
+    r = sag.radius
+    bx, by, bz = -o[0], -o[1], r - o[2]
+    ncx, ncy, ncz = -u[2], np.zeros_like(u[0]), u[0]  # u x (0, 1, 0)
+    nca2 = ncx * ncx + ncy * ncy + ncz * ncz
+    bcx, bcy, bcz = -bz, np.zeros_like(bx), bx  # b x (0, 1, 0)
+    negative_b = ncx * bcx + ncy * bcy + ncz * bcz
+    b_squared = nca2 * np.square(r)
+    bn = bx * ncx + by * ncy + bz * ncz
+    four_ac = np.square(bn)
+    disc = b_squared - four_ac
+    sgn = np.sign(r * u[2])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d = np.where(disc > 0, (negative_b - sgn * np.sqrt(disc)) / nca2, -o[2] / u[2])
+    return o + d * u
+
+
+def _toroid_f_and_slope(sag, x, y):
+    c = 1 / sag.radius
+    r = sag.radius_of_rotation
+    x2 = x * x
+    y2 = y * y
+    g = np.sqrt(1 - c * c * y2)
+    zy = c * y2 / (1 + g)
+    f = np.sqrt((r - zy) * (r - zy) - x2)
+    z = r - f
+    dzdx = x / f
+    dzdy = (r - zy) * (c * y / g) / f
+    return z, dzdx, dzdy
+
+
+NEWTON_MAX_ITER = 64
+NEWTON_TOL = 1e-13  # mm
+
+
+def _intercept_toroid(sag, o, u):
+    """``AbstractSag.intercept`` (``optika/sags/_abc.py:76-107``): the root of
+    ``line(t).z - sag(line(t))`` with the line in the surface frame and the
+    sag transformation applied inside ``sag()``.  The reference's secant
+    solver is replaced by a Newton iteration from the same guess ``t = 0``,
+    iterated until the step falls below 1e-13 mm or 64 iterations."""
+    t = np.zeros(o.shape[-1])
+    active = np.ones(o.shape[-1], dtype=bool)
+    _, v = transform_inverse(sag.transformation, np.zeros_like(u), u)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for _ in range(NEWTON_MAX_ITER):
+            line = o + u * t
+            q, _ = transform_inverse(sag.transformation, line)
+            z, dzdx, dzdy = _toroid_f_and_slope(sag, q[0], q[1])
+            fval = line[2] - z
+            fder = u[2] - (dzdx * v[0] + dzdy * v[1])
+            step = fval / fder
+            step = np.where(active, step, 0.0)
+            t = t - step
+            active = active & ~(np.abs(step) <= NEWTON_TOL)
+            if not np.any(active):
+                break
+    return o + u * t
+
+
+def sag_intercept(sag, rays):
+    """New position of each ray on the sag (``AbstractSag.intercept``)."""
+    kind = _sag_kind(sag)
+    T = sag.transformation
+    if kind == "ToroidalSag":  # no frame change: _abc.py:91-107
+        out = dict(rays)
+        out["position"] = _intercept_toroid(sag, rays["position"], rays["direction"])
+        return out
+    r = _rays_inverse(T, rays)
+    o, u = r["position"], r["direction"]
+    if kind == "NoSag":  # _flat.py:47-64
+        with np.errstate(divide="ignore", invalid="ignore"):
+            d = -o[2] / u[2]
+        pos = o + u * d
+    elif kind == "SphericalSag":
+        pos = _intercept_sphere(sag, o, u)
+    elif kind == "ConicSag":
+        pos = _intercept_conic(sag, o, u)
+    elif kind == "ParabolicSag":
+        pos = _intercept_parabola(sag, o, u)
+    elif kind == "CylindricalSag":
+        pos = _intercept_cylinder(sag, o, u)
+    else:
+        raise TypeError(kind)
+    out = dict(r)
+    out["position"] = pos
+    return _rays_apply(T, out)
+
+
+def sag_propagate(sag, rays):
+    """``AbstractSag.propagate_rays`` (``optika/sags/_abc.py:109-122``)."""
+    result = sag_intercept(sag, rays)
+    displacement = result["position"] - rays["position"]
+    with np.errstate(invalid="ignore"):
+        f = np.exp(-result["attenuation"] * _length(displacement))
+    result = dict(result)
+    result["intensity"] = f * result["intensity"]
+    return result
+
+
+# ---------------------------------------------------------------------------
+# Snell's law (optika/materials/_snells_law.py)
+# ---------------------------------------------------------------------------
+
+
+def snells_law(direction, n1, n2, normal, is_mirror):
+    """numba kernel of ``_snells_law.py:294-366``."""
+    a_x, a_y, a_z = direction
+    u_x, u_y, u_z = normal
+    a2 = a_x * a_x + a_y * a_y + a_z * a_z
+    r = n1 / n2
+    r2 = r * r
+    au = a_x * u_x + a_y * u_y + a_z * u_z
+    au2 = au * au
+    sgn = -np.copysign(1.0, au)
+    with np.errstate(invalid="ignore"):
+        d = -au + sgn * (2 * float(is_mirror) - 1) * np.sqrt(1 / r2 + au2 - a2)
+    return np.stack([r * (a_x + d * u_x), r * (a_y + d * u_y), r * (a_z + d * u_z)])
+
+
+def snells_law_scalar(cos_incidence, n1, n2):
+    """``_snells_law.py:13-38`` (complex-safe square roots)."""
+    sin_i = np.emath.sqrt(1 - np.square(cos_incidence))
+    sin_t = n1 * sin_i / n2
+    return np.emath.sqrt(1 - np.square(sin_t))
+
+
+# ---------------------------------------------------------------------------
+# chemicals and multilayers
+# ---------------------------------------------------------------------------
+
+
+def _nk_rows(chemical) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Read a ``.nk`` table (``optika/chemicals/_chemicals.py:101-131``)."""
+    import pathlib
+
+    name = chemical.file_nk if hasattr(chemical, "file_nk") else f"{chemical}.nk"
+    base = pathlib.Path(__file__).resolve().parent.parent / "optika_amd" / "data" / "nk"
+    skip = 0
+    with open(base / name) as f:
+        for line in f:
+            if line.startswith(";"):
+                skip += 1
+            else:
+                break
+    w, n, k = np.loadtxt(base / name, skiprows=skip, unpack=True)
+    return w, n, k
+
+
+def nk_interp(chemical, wavelength_nm):
+    """``Chemical.n``: ``np.interp`` of ``n + i k`` at the wavelength in
+    Angstroms (``_chemicals.py:101-144``)."""
+    w, n, k = _nk_rows(chemical)
+    x = np.asarray(wavelength_nm, dtype=np.float64) * 10.0
+    return np.interp(x, w, n + 1j * k)
+
+
+def _layer_n(layer, wavelength):
+    chem = getattr(layer, "_chemical", None)
+    if chem is None:
+        return 1
+    return nk_interp(chem, wavelength)
+
+
+def _profile_factor(profile, wavelength, direction, n):
+    """``AbstractInterfaceProfile.reflectivity`` (``profiles.py:103-126``)."""
+    k = -2 * np.pi * n * direction / wavelength
+    s = np.real(-2 * k)
+    w = profile.width
+    name = type(profile).__name__
+    if name == "ErfInterfaceProfile":  # profiles.py:221-222
+        return np.exp(-np.square(s * w) / 2)
+    if name == "ExponentialInterfaceProfile":  # profiles.py:320-324
+        return 1 / (1 + np.square(s * w) / 2)
+    if name == "LinearInterfaceProfile":  # profiles.py:424-430
+        x = np.sqrt(3) * w * s
+        return np.sin(x) / x
+    if name == "SinusoidalInterfaceProfile":  # profiles.py:532-541
+        a = np.pi / (np.square(np.pi) - 8)
+        x = a * w * s
+        x1 = x - np.pi / 2
+        x2 = x + np.pi / 2
+        return np.pi * (np.sin(x1) / x1 + np.sin(x2) / x2) / 4
+    raise TypeError(name)
+
+
+def _mat_refraction(wavelength, dl, dr, pol_s, nl, nr, interface):
+    """``matrices.refraction`` (``matrices.py:20-161``)."""
+    di = dl if pol_s else np.conj(dl)
+    dj = dr if pol_s else np.conj(dr)
+    imp_i = nl if pol_s else 1 / nl
+    imp_j = nr if pol_s else 1 / nr
+    qi = di * imp_i
+    qj = dj * imp_j
+    a = qi + qj
+    r = (qi - qj) / a
+    t = 2 * qi / a
+    if interface is not None:
+        r = r * _profile_factor(interface, wavelength, di, nl)
+    one = np.ones_like(r)
+    return [[one / t, r / t], [r / t, one / t]]
+
+
+def _mat_propagation(wavelength, direction, thickness, n):
+    """``matrices.propagation`` (``matrices.py:164-246``)."""
+    beta = 2 * np.pi * thickness * n * direction / wavelength
+    zero = np.zeros_like(beta)
+    return [[np.exp(-1j * beta), zero], [zero, np.exp(+1j * beta)]]
+
+
+def _matmul2(a, b):
+    return [
+        [a[0][0] * b[0][0] + a[0][1] * b[1][0], a[0][0] * b[0][1] + a[0][1] * b[1][1]],
+        [a[1][0] * b[0][0] + a[1][1] * b[1][0], a[1][0] * b[0][1] + a[1][1] * b[1][1]],
+    ]
+
+
+def _where2(cond, a, b):
+    return [[np.where(cond, a[i][j], b[i][j]) for j in range(2)] for i in range(2)]
+
+
+def _identity2(shape):
+    one = np.ones(shape, dtype=complex)
+    zero = np.zeros(shape, dtype=complex)
+    return [[one, zero], [zero, one]]
+
+
+def _power2(m, exponent: int, shape):
+    """Binary powering, the algorithm of ``numpy.linalg.matrix_power``."""
+    if exponent == 0:
+        return _identity2(shape)
+    z = None
+    result = None
+    n = exponent
+    while n > 0:
+        z = m if z is None else _matmul2(z, z)
+        n, bit = divmod(n, 2)
+        if bit:
+            result = z if result is None else _matmul2(result, z)
+    return result
+
+
+def _layer_transfer(layer, wavelength, direction, pol_s, n, where):
+    """``Layer.transfer`` (``_layers.py:229-277``)."""
+    n_internal = _layer_n(layer, wavelength)
+    direction_internal = snells_law_scalar(direction, n, n_internal)
+    refraction = _mat_refraction(wavelength, direction, direction_internal, pol_s, n,
+                                 n_internal, layer.interface)
+    shape = np.shape(direction_internal)
+    refraction = _where2(where, refraction, _identity2(shape))
+    thickness = layer.thickness if layer.thickness is not None else 0.0
+    propagation = _mat_propagation(wavelength, direction_internal, thickness, n_internal)
+    where = where & (np.abs(propagation[0][0]) < 1e10)
+    transfer = _matmul2(refraction, propagation)
+    transfer = _where2(where, transfer, refraction)
+    return n_internal, direction_internal, transfer, where
+
+
+def _stack_transfer(layers, wavelength, direction, pol_s, n, where):
+    """``LayerSequence.transfer`` / ``PeriodicLayerSequence.transfer``
+    (``_layers.py:473-499``, ``611-645``)."""
+    name = type(layers).__name__
+    if name == "Layer":
+        return _layer_transfer(layers, wavelength, direction, pol_s, n, where)
+    if name == "PeriodicLayerSequence":
+        period = _Seq(list(layers.layers))
+        n, direction, start, where = _stack_transfer(period, wavelength, direction, pol_s, n, where)
+        n, direction, periodic, where = _stack_transfer(period, wavelength, direction, pol_s, n, where)
+        periodic = _power2(periodic, layers.num_periods - 1, np.shape(direction))
+        return n, direction, _matmul2(start, periodic), where
+    seq = list(layers.layers) if hasattr(layers, "layers") else list(layers)
+    result = _identity2(np.broadcast_shapes(np.shape(wavelength), np.shape(direction)))
+    for layer in seq:
+        n, direction, m, where = _stack_transfer(layer, wavelength, direction, pol_s, n, where)
+        result = _matmul2(result, m)
+    return n, direction, result, where
+
+
+class _Seq:
+    def __init__(self, layers):
+        self.layers = layers
+
+
+class _VacuumLayer:
+    chemical = None
+    _chemical = None
+    thickness = 0.0
+    interface = None
+
+
+class _SubstrateLayer:
+    """A substrate with its thickness forced to zero (``_multilayers.py:190-193``)."""
+
+    def __init__(self, layer):
+        self._chemical = getattr(layer, "_chemical", None)
+        self.interface = getattr(layer, "interface", None)
+        self.thickness = 0.0
+
+
+def multilayer_coefficients(wavelength, direction, n, layers, substrate):
+    """Complex ``(r, t)`` per polarization (``_multilayers.py:33-237``);
+    returns ``((r_s, r_p), (t_s, t_p))``."""
+    substrate = _SubstrateLayer(substrate) if substrate is not None else _VacuumLayer()
+    if layers is None:
+        layers = _Seq([])
+    elif isinstance(layers, (list, tuple)):
+        layers = _Seq(list(layers))
+    shape = np.broadcast_shapes(np.shape(wavelength), np.shape(direction), np.shape(n))
+    rs, ts = [], []
+    for pol_s in (True, False):
+        where = np.ones(shape, dtype=bool)
+        n1, d1, m_layers, where = _stack_transfer(layers, wavelength, direction, pol_s, n, where)
+        n2, d2, m_sub, where = _layer_transfer(substrate, wavelength, d1, pol_s, n1, where)
+        m = _matmul2(m_layers, m_sub)
+        r = m[1][0] / m[0][0]
+        t = 1 / m[0][0]
+        t = np.where(where, t, 0)
+        rs.append(r)
+        ts.append(t)
+    return (rs[0], rs[1]), (ts[0], ts[1])
+
+
+def multilayer_efficiency(wavelength, direction, n, layers, substrate):
+    """Reflectivity and transmissivity per polarization
+    (``_multilayers.py:240-532``); returns ``((R_s, R_p), (T_s, T_p))``."""
+    (r_s, r_p), (t_s, t_p) = multilayer_coefficients(wavelength, direction, n, layers, substrate)
+    sub = _SubstrateLayer(substrate) if substrate is not None else _VacuumLayer()
+    n_sub = _layer_n(sub, wavelength)
+    d_sub = snells_law_scalar(direction, n, n_sub)
+    q_amb_s = direction * n
+    q_amb_p = np.conj(direction) * (1 / n)
+    q_sub_s = d_sub * n_sub
+    q_sub_p = np.conj(d_sub) * (1 / n_sub)
+    R = (np.square(np.abs(r_s)), np.square(np.abs(r_p)))
+    T = (
+        np.square(np.abs(t_s)) * np.real(q_sub_s / q_amb_s),
+        np.square(np.abs(t_p)) * np.real(q_sub_p / q_amb_p),
+    )
+    return R, T
+
+
+# ---------------------------------------------------------------------------
+# rulings (optika/rulings/_rulings.py, _spacing.py)
+# ---------------------------------------------------------------------------
+
+
+def _spacing_of(rulings):
+    s = rulings.spacing
+    if hasattr(s, "kind") and type(s).__name__.endswith("RulingSpacing"):
+        return s
+    return None  # scalar constant spacing along x (_rulings.py:156-168)
+
+
+def spacing_vector(rulings, position, normal):
+    """``spacing_(position, normal)``: the local ruling vector ``d g``
+    (``_spacing.py:45-328``); returns (vector, wavelength unit scale)."""
+    s = _spacing_of(rulings)
+    n = position.shape[-1]
+    if s is None:
+        d = float(rulings.spacing)
+        return np.stack([np.full(n, d), np.zeros(n), np.zeros(n)]), WAVELENGTH_MM
+    name = type(s).__name__
+    if name == "ConstantRulingSpacing":  # _spacing.py:69-74
+        return np.broadcast_to(s.constant * _vec(s.normal), (3, n)).copy(), WAVELENGTH_MM
+    if name == "Polynomial1dRulingSpacing":  # _spacing.py:109-128
+        p = position
+        if s.transformation is not None:
+            p, _ = transform_apply(s.transformation, p)
+        g = _vec(s.normal)
+        x = p[0] * g[0] + p[1] * g[1] + p[2] * g[2]
+        result = np.zeros(n)
+        for power, coef in s.coefficients.items():
+            if power == 0:
+                xp = np.ones(n)
+            elif power == 1:
+                xp = x
+            elif power == 2:
+                xp = x * x
+            else:
+                xp = x ** power
+            result = result + coef * xp
+        return result * g, WAVELENGTH_MM
+    if name == "HolographicRulingSpacing":  # _spacing.py:295-328
+        d1 = 2 * float(s.is_diverging_1) - 1
+        d2 = 2 * float(s.is_diverging_2) - 1
+        r1 = position - _vec(s.x1)
+        r2 = position - _vec(s.x2)
+        r1 = d1 * (r1 / _length(r1))
+        r2 = d2 * (r2 / _length(r2))
+        dr = r1 - r2
+        aq = _cross(normal, dr)
+        a = _length(aq)
+        q = aq / a
+        spacing = s.wavelength / a
+        return spacing * _cross(q, normal), 1.0
+    raise TypeError(name)
+
+
+def incident_effective(wavelength, direction, n, normal, order, kappa, scale):
+    """Grating equation (``_rulings.py:24-128``, ``170-204``)."""
+    spacing = _length(kappa)
+    g = kappa / spacing
+    w = wavelength * scale
+    sgn = np.sign(direction[0] * normal[0] + direction[1] * normal[1] + direction[2] * normal[2])
+    return direction + sgn * order * w * g / (n * spacing)
+
+
+def rulings_efficiency(rulings, order, rays, normal):
+    """Groove efficiency of each ruling model (``_rulings.py:246-1073``)."""
+    name = type(rulings).__name__
+    wavelength = rays["wavelength"]
+    if name == "Rulings":
+        return np.ones_like(wavelength)
+    if name == "MeasuredRulings":  # _rulings.py:287-313
+        w, e = rulings.efficiency_measured
+        return np.interp(wavelength, np.asarray(w, float), np.asarray(e, float))
+    kappa, _ = spacing_vector(rulings, rays["position"], normal)
+    normal_rulings = kappa / _length(kappa)
+    par = _cross(normal, normal_rulings)
+    par = par / _length(par)
+    direction = rays["direction"]
+    direction = direction - _dot(direction, par)
+    cos_theta = -_dot(direction, normal)
+    i = order
+    if name == "SinusoidalRulings":  # _rulings.py:404-457
+        d = rulings.depth
+        gamma = np.pi * d / (wavelength * cos_theta)
+        return scipy.special.jv(i, 2 * gamma)
+    if name == "SquareRulings":  # _rulings.py:548-614
+        d = rulings.depth / (np.pi / 4)
+        gamma = np.pi * d / (wavelength * cos_theta)
+        if i == 0:
+            return np.square(np.cos(np.pi * gamma / 2))
+        if i % 2 == 0:
+            return np.zeros_like(gamma)
+        return np.square(2 * np.sin(np.pi * gamma / 2) / (i * np.pi))
+    if name == "SawtoothRulings":  # _rulings.py:705-758
+        d = rulings.depth / (np.pi / 2)
+        gamma = np.pi * d / (wavelength * cos_theta)
+        return np.square(np.sin(np.pi * gamma) / (np.pi * (gamma + i)))
+    if name == "TriangularRulings":  # _rulings.py:849-911
+        d = rulings.depth / (np.square(np.pi) / 8)
+        gamma = np.pi * d / (wavelength * cos_theta)
+        a = gamma / (np.square(np.pi * gamma / 2) + np.square(i))
+        if i % 2 == 0:
+            return np.square(a * np.sin(np.square(np.pi) * gamma / 4))
+        return np.square(a * np.cos(np.square(np.pi) * gamma / 4))
+    if name == "RectangularRulings":  # _rulings.py:1008-1073
+        a = 2 * np.pi * rulings.ratio_duty
+        amplitude = np.pi / (2 * np.sqrt(2 * (1 - np.cos(a))))
+        d = rulings.depth / amplitude
+        gamma = np.pi * d / (wavelength * cos_theta)
+        b = np.square(np.sin(np.pi * gamma / np.sqrt(2 * (1 - np.cos(a)))))
+        if i == 0:
+            return 1 - ((2 * a / np.pi) - np.square(a / np.pi)) * b
+        return (2 / np.square(i * np.pi)) * (1 - np.cos(i * a)) * b
+    raise TypeError(name)
+
+
+# ---------------------------------------------------------------------------
+# materials (optika/materials/_materials.py, _multilayers.py)
+# ---------------------------------------------------------------------------
+
+
+def _material_name(material) -> str:
+    name = type(material).__name__
+    if name == "IdealSensorMaterial":
+        return "Vacuum"
+    return name
+
+
+def material_index_refraction(material, rays):
+    name = _material_name(material)
+    if name == "Vacuum":
+        return np.ones_like(rays["wavelength"])
+    if name == "Glass":  # _materials.py:428-438 (c in um^2 -> nm^2)
+        w2 = np.square(rays["wavelength"])
+        m = material
+        n2 = 1 + (m.b1 * w2 / (w2 - m.c1 * 1e6) + m.b2 * w2 / (w2 - m.c2 * 1e6)
+                  + m.b3 * w2 / (w2 - m.c3 * 1e6))
+        return np.sqrt(n2)
+    return rays["index_refraction"]
+
+
+def material_attenuation(material, rays):
+    name = _material_name(material)
+    if name in ("Vacuum", "Glass"):
+        return np.zeros_like(rays["wavelength"])
+    return rays["attenuation"]
+
+
+def material_is_mirror(material) -> bool:
+    return _material_name(material) in ("Mirror", "MeasuredMirror", "MultilayerMirror")
+
+
+def material_efficiency(material, rays, normal):
+    name = _material_name(material)
+    if name in ("Vacuum", "Glass", "Mirror"):
+        return np.ones_like(rays["wavelength"])
+    if name == "MeasuredMirror":  # _materials.py:279-305
+        w, e = material.efficiency_measured
+        return np.interp(rays["wavelength"], np.asarray(w, float), np.asarray(e, float))
+    if name in ("MultilayerMirror", "MultilayerFilm"):  # _multilayers.py:839-935
+        wavelength = rays["wavelength"]
+        k = rays["attenuation"] * wavelength / (4 * np.pi) * 1e-6
+        n = rays["index_refraction"] + k * 1j
+        direction = -_dot(rays["direction"], normal)
+        substrate = getattr(material, "substrate", None) if name == "MultilayerMirror" else None
+        R, T = multilayer_efficiency(wavelength, direction, n, material.layers, substrate)
+        if name == "MultilayerMirror":
+            return (R[0] + R[1]) / 2
+        return (T[0] + T[1]) / 2
+    raise TypeError(name)
+
+
+# ---------------------------------------------------------------------------
+# apertures (optika/apertures/_apertures.py)
+# ---------------------------------------------------------------------------
+
+
+def point_in_polygon(x, y, vx, vy):
+    """Crossing-number test (convention for ``na.geometry.point_in_polygon``)."""
+    inside = np.zeros(np.shape(x), dtype=bool)
+    nv = len(vx)
+    j = nv - 1
+    for i in range(nv):
+        xi, yi, xj, yj = vx[i], vy[i], vx[j], vy[j]
+        cond = (yi > y) != (yj > y)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            xc = (xj - xi) * (y - yi) / (yj - yi) + xi
+        inside = np.where(cond & (x < xc), ~inside, inside)
+        j = i
+    return inside
+
+
+def _polygon_vertices(aperture):
+    name = type(aperture).__name__
+    if name == "PolygonalAperture":
+        v = aperture.vertices
+        return np.asarray(v.x, float), np.asarray(v.y, float)
+    if name in ("RegularPolygonalAperture", "OctagonalAperture"):  # _apertures.py:1009-1027
+        nv = 8 if name == "OctagonalAperture" else aperture.num_vertices
+        ang = np.deg2rad(np.linspace(0, 360, nv, endpoint=False))
+        return aperture.radius * np.cos(ang), aperture.radius * np.sin(ang)
+    if name == "IsoscelesTrapezoidalAperture":  # _apertures.py:1109-1134
+        m = np.tan(np.deg2rad(aperture.angle) / 2)
+        xl, xr = aperture.x_left, aperture.x_right
+        return np.array([xl, xr, xr, xl]), np.array([m * xl, m * xr, -(m * xr), -(m * xl)])
+    raise TypeError(name)
+
+
+def aperture_mask(aperture, position):
+    """``AbstractAperture.__call__`` for every aperture type."""
+    name = type(aperture).__name__
+    p, _ = transform_inverse(aperture.transformation, position)
+    x, y = p[0], p[1]
+    if name == "CircularAperture":  # _apertures.py:292-314
+        mask = np.sqrt(x * x + y * y) <= aperture.radius
+    elif name == "CircularSectorAperture":  # _apertures.py:438-480
+        r = aperture.radius
+        a0 = np.deg2rad(aperture.angle_start)
+        a1 = np.deg2rad(aperture.angle_stop)
+        mask_r = np.sqrt(x * x + y * y) <= r
+        angle = np.arctan2(y, x)
+        ap = np.mod(angle, 2 * np.pi)
+        an = np.mod(angle, -2 * np.pi)
+        mask = mask_r & (((a0 < ap) & (ap < a1)) | ((a0 < an) & (an < a1)))
+    elif name == "EllipticalAperture":  # _apertures.py:640-663
+        rx, ry = aperture.radius.x, aperture.radius.y
+        mask = np.square(x / rx) + np.square(y / ry) <= 1
+    elif name == "RectangularAperture":  # _apertures.py:941-968
+        hx, hy = aperture.half_width.x, aperture.half_width.y
+        mask = ((-hx <= x) & (x <= hx)) & ((-hy <= y) & (y <= hy))
+    else:  # polygons, _apertures.py:739-778
+        vx, vy = _polygon_vertices(aperture)
+        mask = point_in_polygon(x, y, vx, vy)
+    if aperture.inverted:
+        mask = ~mask
+    if not aperture.active:
+        mask = np.ones_like(mask)
+    return mask
+
+
+# ---------------------------------------------------------------------------
+# surfaces, systems, sensors
+# ---------------------------------------------------------------------------
+
+
+def _orders_of(rulings) -> list[int]:
+    o = rulings.diffraction_order
+    if np.ndim(o) == 0:
+        return [int(o)]
+    return [int(v) for v in np.asarray(o).ravel()]
+
+
+def _sensor_half_width(sensor):
+    w = sensor.width_pixel
+    wx, wy = (w.x, w.y) if hasattr(w, "x") else (w, w)
+    n = sensor.num_pixel
+    return wx * n.x / 2, wy * n.y / 2
+
+
+class RectangularAperture:
+    """The sensor's light-sensitive area (``optika/sensors/_sensors.py:83-90``)."""
+
+    transformation = None
+    inverted = False
+    active = True
+    unit = "mm"
+
+    def __init__(self, hx, hy):
+        self.half_width = type("HalfWidth", (), {"x": hx, "y": hy})()
+
+
+class NoSag:
+    """The sensor's flat sag (``optika/sensors/_sensors.py:38-40``)."""
+
+    transformation = None
+
+
+def _is_sensor(surface) -> bool:
+    return type(surface).__name__ == "ImagingSensor"
+
+
+def _aperture_of(surface):
+    if _is_sensor(surface):
+        return RectangularAperture(*_sensor_half_width(surface))
+    return surface.aperture
+
+
+def _sag_of(surface):
+    return NoSag() if _is_sensor(surface) else surface.sag
+
+
+def surface_propagate(surface, rays, order: int = 0):
+    """``AbstractSurface.propagate_rays`` (``optika/surfaces.py:123-198``).
+    ``order`` selects the diffraction order when the rulings carry several."""
+    sag = _sag_of(surface)
+    material = surface.material
+    aperture = _aperture_of(surface)
+    rulings = getattr(surface, "rulings", None)
+    T = surface.transformation
+
+    rays = _rays_inverse(T, rays)
+    rays_1 = sag_propagate(sag, rays)
+    position_1 = rays_1["position"]
+    normal = sag_normal(sag, position_1)
+
+    if rulings is not None:
+        kappa, scale = spacing_vector(rulings, position_1, normal)
+        rays_1 = dict(rays_1)
+        rays_1["direction"] = incident_effective(
+            rays_1["wavelength"], rays_1["direction"], rays_1["index_refraction"],
+            normal, order, kappa, scale,
+        )
+
+    wavelength_1 = rays_1["wavelength"]
+    a = rays_1["direction"]
+    intensity_1 = rays_1["intensity"]
+    n1 = rays_1["index_refraction"]
+    n2 = material_index_refraction(material, rays_1)
+    r = n1 / n2
+    wavelength_2 = wavelength_1 / r
+    b = snells_law(a, n1, n2, normal, material_is_mirror(material))
+    efficiency = material_efficiency(material, rays_1, normal)
+    if rulings is not None:
+        efficiency = efficiency * rulings_efficiency(rulings, order, rays_1, normal)
+    intensity_2 = intensity_1 * efficiency
+    attenuation_2 = material_attenuation(material, rays_1)
+
+    rays_2 = dict(
+        wavelength=wavelength_2,
+        position=position_1,
+        direction=b,
+        intensity=intensity_2,
+        attenuation=attenuation_2,
+        index_refraction=n2,
+        unvignetted=rays_1["unvignetted"],
+    )
+    if aperture is not None:
+        target = rays_2["direction"] if getattr(aperture, "unit", "mm") == "direction" else rays_2["position"]
+        rays_2["unvignetted"] = rays_2["unvignetted"] & aperture_mask(aperture, target)
+    return _rays_apply(T, rays_2)
+
+
+def system_orders(surfaces) -> list[int]:
+    """The diffraction-order axis shared by every multi-order surface."""
+    count = 1
+    for s in surfaces:
+        rl = getattr(s, "rulings", None)
+        if rl is not None:
+            k = len(_orders_of(rl))
+            if k > 1:
+                if count not in (1, k):
+                    raise ValueError("diffraction-order axes of different lengths")
+                count = k
+    return list(range(count))
+
+
+def propagate(surfaces, rays, accumulate: bool = False):
+    """``propagators.propagate_rays`` / ``accumulate_rays``
+    (``optika/propagators.py:19-74``).  Returns a list over order paths; each
+    entry is the final rays, or the list of rays after every surface."""
+    paths = []
+    for p in system_orders(surfaces):
+        r = rays
+        hist = []
+        for s in surfaces:
+            rl = getattr(s, "rulings", None)
+            order = 0
+            if rl is not None:
+                orders = _orders_of(rl)
+                order = orders[p] if len(orders) > 1 else orders[0]
+            r = surface_propagate(s, r, order)
+            hist.append(r)
+        paths.append(hist if accumulate else r)
+    return paths
+
+
+def direction_from_angles(ax_deg, ay_deg):
+    """``optika.direction`` (``optika/_util.py:41-73``)."""
+    ax = np.deg2rad(np.asarray(ax_deg, dtype=np.float64))
+    ay = np.deg2rad(np.asarray(ay_deg, dtype=np.float64))
+    return np.stack([-np.cos(ay) * np.sin(ax), -np.sin(ay), np.cos(ay) * np.cos(ax)])
+
+
+def _surfaces_all(system):
+    """``_sequential.py:92-112`` (the field-stop flag does not affect the trace)."""
+    out = [system.object] if system.object is not None else []
+    out += list(system.surfaces)
+    if system.sensor is not None:
+        out.append(system.sensor)
+    return out
+
+
+def _object_at_infinity(system) -> bool:
+    ap = system.object.aperture if system.object is not None else None
+    return ap is None or getattr(ap, "unit", "mm") == "direction"
+
+
+def rayfunction_input(system, wavelength, field_x, field_y, pupil_x, pupil_y, intensity=1.0):
+    """Physical-unit input rays (``_sequential.py:791-828``, with the early
+    return of ``_denormalize_grid`` ``:755-756``) and the system's inverse
+    transformation (``:905-909``)."""
+    if _object_at_infinity(system):
+        px, py = pupil_x, pupil_y
+        d = direction_from_angles(field_x, field_y)
+    else:
+        px, py = field_x, field_y
+        d = direction_from_angles(pupil_x, pupil_y)
+    n = max(np.size(px), np.size(py), d.shape[-1], np.size(wavelength))
+    pos = np.stack(np.broadcast_arrays(
+        np.asarray(px, float) * np.ones(n), np.asarray(py, float) * np.ones(n), np.zeros(n)))
+    d = np.broadcast_to(d.reshape(3, -1), (3, n))
+    rays = make_rays(wavelength, pos, d, intensity=intensity)
+    obj = system.object
+    if obj is not None and obj.transformation is not None:
+        rays = _rays_apply(obj.transformation, rays)
+    if system.transformation is not None:
+        rays = _rays_inverse(system.transformation, rays)
+    return rays
+
+
+def raytrace(system, wavelength, field_x, field_y, pupil_x, pupil_y, intensity=1.0,
+             accumulate=False):
+    rays = rayfunction_input(system, wavelength, field_x, field_y, pupil_x, pupil_y, intensity)
+    return propagate(_surfaces_all(system), rays, accumulate=accumulate)
+
+
+def rayfunction(system, wavelength, field_x, field_y, pupil_x, pupil_y, intensity=1.0):
+    """Last-surface rays in sensor coordinates (``_sequential.py:927-988``)."""
+    paths = raytrace(system, wavelength, field_x, field_y, pupil_x, pupil_y, intensity)
+    out = []
+    for r in paths:
+        if system.sensor is not None and system.sensor.transformation is not None:
+            r = _rays_inverse(system.sensor.transformation, r)
+        out.append(r)
+    return out
+
+
+def collect(sensor, rays, wavelength_edges, where=True):
+    """``AbstractImagingSensor.collect`` (``optika/sensors/_sensors.py:92-171``)
+    with ``numpy.histogramdd`` binning on ``numpy.linspace`` pixel edges
+    spanning the sensor aperture."""
+    where = np.asarray(where) & rays["unvignetted"]
+    hx, hy = _sensor_half_width(sensor)
+    nx, ny = int(sensor.num_pixel.x), int(sensor.num_pixel.y)
+    ex = np.linspace(-hx, hx, nx + 1)
+    ey = np.linspace(-hy, hy, ny + 1)
+    direction = -(rays["direction"][2] * -1.0)  # ideal material: -d . (0, 0, -1)
+    flux = rays["intensity"] * where
+    sample = np.stack([rays["wavelength"], rays["position"][0], rays["position"][1]], axis=-1)
+    bins = [np.asarray(wavelength_edges, float), ex, ey]
+    image, _ = np.histogramdd(sample, bins=bins, weights=flux)
+    moment, _ = np.histogramdd(sample, bins=bins, weights=flux * direction)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        cosine = np.where(image > 0, moment / image, 1.0)
+    return image, cosine
+
