@@ -275,3 +275,50 @@ def trace_system(system, grid: ObjectVectorArray, intensity=None, accumulate: bo
         outputs = _make_rays(out, _shape(P, n), {
             "wavelength": wl, "attenuation": 0.0, "index_refraction": 1.0})
     return RayFunctionArray(inputs=grid, outputs=outputs)
+
+
+class GridTracePlan:
+    """A prepared ``rayfunction`` / ``raytrace`` launch for a fixed ray count.
+
+    Compiles the system once, preallocates the output tensors and the
+    ``optk_grid`` / ``optk_rays`` records, so each call is one C-ABI launch
+    on the current stream with no host-side allocation or record building.
+    The returned rays are views of the plan's buffers, overwritten by the
+    next call.
+    """
+
+    def __init__(self, system, n: int, sensor_frame: bool = True, precision: str = "f64"):
+        if _precision(precision) != _lib.F64:
+            raise ValueError("GridTracePlan traces in float64; use the f32 plan for float32")
+        self.dev = device()
+        surfaces = system.surfaces_all
+        obj_t = system.object.transformation if system.object is not None else None
+        post = system.sensor.transformation if (sensor_frame and system.sensor is not None) else None
+        self.cs = compiled(surfaces, pre=(obj_t, system.transformation), pre_is_inverse=(0, 1),
+                           post=post)
+        self.n = int(n)
+        self.paths = self.cs.num_paths
+        self.flags = _lib.TRACE_APPLY_POST if post is not None else 0
+        self.invariant = _compile.invariant_fields(surfaces)
+        fields = {"intensity"} if self.invariant else {
+            "wavelength", "intensity", "attenuation", "index_refraction"}
+        self.out = _Out(self.paths * self.n, self.dev, torch.float64, fields)
+        self.grid = _lib.Grid()
+        self.grid.object_at_infinity = 1 if system.object_is_at_infinity else 0
+        self.grid.intensity_default = 1.0
+        self.num_surfaces = self.cs.num_surfaces
+
+    def __call__(self, field_x, field_y, pupil_x, pupil_y, wavelength, stream=None):
+        g = self.grid
+        for k, t in enumerate((field_x, field_y)):
+            g.field[k] = t.data_ptr()
+        for k, t in enumerate((pupil_x, pupil_y)):
+            g.pupil[k] = t.data_ptr()
+        g.wavelength = wavelength.data_ptr()
+        s = stream_handle() if stream is None else ctypes.c_void_p(stream)
+        _lib.check(_lib.lib().optk_trace_grid(self.cs.handle, _lib.F64, ctypes.byref(g),
+                                              ctypes.byref(self.out.rec), None, self.n,
+                                              self.flags, s))
+        scalars = {"wavelength": wavelength if self.paths == 1 else wavelength.expand(self.paths, self.n),
+                   "attenuation": 0.0, "index_refraction": 1.0}
+        return _make_rays(self.out, _shape(self.paths, self.n), scalars)

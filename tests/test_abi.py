@@ -1,0 +1,55 @@
+"""The C ABI: the library builds, loads (no GPU needed) and exports every
+entry point declared in include/optk.h with matching record layouts."""
+
+import ctypes
+import pathlib
+import re
+
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    text = (ROOT / "include" / "optk.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int32_t|int64_t|const char \*)\s*(optk_\w+)\(", text,
+                                 re.MULTILINE)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for n in ("optk_abi_version", "optk_last_error", "optk_sizeof", "optk_system_create",
+              "optk_system_destroy", "optk_trace", "optk_trace_grid", "optk_eval", "optk_collect"):
+        assert n in names
+
+
+def test_library_loads_and_exports_every_symbol():
+    from optika_amd import _lib
+
+    if not _lib.library_path().exists():
+        pytest.skip("liboptk.so not built (run python -m optika_amd._build)")
+    L = _lib.lib()  # checks ABI version and every record size
+    for name in declared_functions():
+        assert hasattr(L, name), name
+    assert L.optk_abi_version() == 1
+
+
+def test_invalid_arguments_are_rejected_without_a_device():
+    from optika_amd import _lib
+
+    if not _lib.library_path().exists():
+        pytest.skip("liboptk.so not built")
+    L = _lib.lib()
+    assert L.optk_system_create(None, None) == -1
+    assert b"null" in L.optk_last_error()
+    d = _lib.SystemDesc()
+    d.num_surfaces = 1
+    d.num_paths = 1
+    handle = ctypes.c_void_p()
+    # surfaces pointer missing -> rejected before any device call
+    assert L.optk_system_create(ctypes.byref(d), ctypes.byref(handle)) == -1
+    d.num_paths = 99
+    d.num_surfaces = 0
+    assert L.optk_system_create(ctypes.byref(d), ctypes.byref(handle)) == -1
+    assert b"num_paths" in L.optk_last_error()
+    assert L.optk_sizeof(b"no_such_record") == -1
