@@ -227,7 +227,9 @@ def snells_law(direction, index_refraction, index_refraction_new, normal=None,
 
 
 def _split_complex(v, dev):
-    t = torch.as_tensor(v) if not isinstance(v, torch.Tensor) else v
+    # numpy keeps Python scalars in float64 / complex128 (torch would make
+    # them float32 / complex64)
+    t = torch.as_tensor(np.asarray(v)) if not isinstance(v, torch.Tensor) else v
     if torch.is_complex(t):
         return t.real.to(dev, torch.float64), t.imag.to(dev, torch.float64)
     t = t.to(dev, torch.float64)

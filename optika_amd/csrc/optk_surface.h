@@ -336,7 +336,11 @@ OPTK_HD double rulings_efficiency(const optk_surface &s, int order, double wavel
   switch (s.rulings_kind) {
   case OPTK_RULINGS_SINUSOIDAL: {  // J_i(2 gamma), unsquared
     double gamma = pi * s.rulings_p[0] / (wavelength * cos_theta);
-    return jn(order, 2.0 * gamma);
+    // J_{-n}(x) = (-1)^n J_n(x), applied explicitly: the device jn is
+    // only relied on for n >= 0
+    int m = order < 0 ? -order : order;
+    double v = jn(m, 2.0 * gamma);
+    return (order < 0 && (m & 1)) ? -v : v;
   }
   case OPTK_RULINGS_SQUARE: {
     double dd = s.rulings_p[0] / (pi / 4.0);
