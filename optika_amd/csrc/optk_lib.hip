@@ -48,8 +48,31 @@ struct optk_system {
   optk_affine pre[2];
   int32_t pre_is_inverse[2];
   optk_affine post;
+  int32_t features = 0;  // FEAT_* the surfaces need
+  TraceSpec *spec = nullptr;  // device copy of views + transformations
   std::vector<optk_surface> host_surfaces;
 };
+
+static int32_t features_of(const optk_system_desc *d) {
+  int32_t f = 0;
+  for (int i = 0; i < d->num_surfaces; ++i) {
+    const optk_surface &s = d->surfaces[i];
+    if (s.sag_kind == OPTK_SAG_TOROID) f |= FEAT_TOROID;
+    if (s.rulings_kind != OPTK_RULINGS_NONE) f |= FEAT_RULINGS;
+    if (s.material_kind == OPTK_MATERIAL_MULTILAYER_MIRROR ||
+        s.material_kind == OPTK_MATERIAL_MULTILAYER_FILM)
+      f |= FEAT_MULTILAYER;
+  }
+  return f;
+}
+
+// kernel variant that covers a feature set: geometric only, rulings and
+// toroids without multilayers, or everything
+static int32_t variant_of(int32_t features) {
+  if (features == 0) return 0;
+  if (!(features & FEAT_MULTILAYER)) return FEAT_TOROID | FEAT_RULINGS;
+  return FEAT_ALL;
+}
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -79,12 +102,15 @@ struct GridIn {
   int32_t object_at_infinity;
 };
 
-struct TraceParams {
-  TraceSpec spec;
-  int32_t has_accum;
+// Per-launch arguments.  The system (surfaces, tables, transformations) is
+// device-resident and reached through one pointer, so the kernel argument
+// block stays small and the record fields are fetched with scalar loads
+// where they are used.
+struct TraceArgs {
+  const TraceSpec *spec;  // device copy
   int64_t n;
+  int32_t apply_post;
   RaysOut out;
-  RaysOut accum;
 };
 
 __device__ __forceinline__ double ld(const double *p, int64_t i, double dflt) {
@@ -106,23 +132,40 @@ __device__ __forceinline__ void store(const RaysOut &o, int64_t j, const Ray<dou
 }
 
 // every diffraction-order path of one ray, through every surface
-__device__ __forceinline__ void trace_one(const TraceParams &P, int64_t i, const Ray<double> &r0) {
-  const SystemView &sys = P.spec.sys;
-  const int64_t n = P.n;
-  for (int path = 0; path < sys.num_paths; ++path) {
+template <int F, bool ACCUM>
+__device__ __forceinline__ void trace_one(const TraceArgs &A, const RaysOut &accum, int64_t i,
+                                          const Ray<double> &r0) {
+  const TraceSpec &spec = *A.spec;
+  const SystemView &sys = spec.sys;
+  const int64_t n = A.n;
+  // only rulings create diffraction-order paths: without them one path
+  const int num_paths = (F & FEAT_RULINGS) ? sys.num_paths : 1;
+  const int num_surfaces = sys.num_surfaces;
+  for (int path = 0; path < num_paths; ++path) {
     Ray<double> r = r0;
-    for (int k = 0; k < sys.num_surfaces; ++k) {
-      propagate_surface(sys.surfaces[k], sys, path, r);
-      if (P.has_accum) store(P.accum, ((int64_t)k * sys.num_paths + path) * n + i, r);
+    for (int k = 0; k < num_surfaces; ++k) {
+      propagate_surface<F>(sys.surfaces[k], sys, path, r);
+      if (ACCUM) store(accum, ((int64_t)k * num_paths + path) * n + i, r);
     }
-    apply_post(P.spec, r);
-    store(P.out, (int64_t)path * n + i, r);
+    if (A.apply_post) {
+      xf_inverse_pos(spec.post, r.px, r.py, r.pz);
+      xf_inverse_dir(spec.post, r.dx, r.dy, r.dz);
+    }
+    store(A.out, (int64_t)path * n + i, r);
   }
 }
 
-__global__ __launch_bounds__(256) void trace_rays_f64(const TraceParams P, const RaysIn in) {
+// occupancy targets: the geometric kernels keep <= 128 VGPRs (4 waves per
+// SIMD) to hide HBM latency; the multilayer kernels are VALU-bound
+template <int F> struct Waves {
+  static constexpr int value = (F & FEAT_MULTILAYER) ? 2 : 4;
+};
+
+template <int F, bool ACCUM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
+void trace_rays_f64(const TraceArgs A, const RaysIn in, const RaysOut accum) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
+  if (i >= A.n) return;
   Ray<double> r;
   r.px = in.p[0][i];
   r.py = in.p[1][i];
@@ -135,19 +178,32 @@ __global__ __launch_bounds__(256) void trace_rays_f64(const TraceParams P, const
   r.attenuation = ld(in.attenuation, i, in.attenuation_default);
   r.index = ld(in.index, i, in.index_default);
   r.unvignetted = in.unvignetted ? (in.unvignetted[i] != 0) : (in.unvignetted_default != 0);
-  apply_pre(P.spec, r);
-  trace_one(P, i, r);
+  apply_pre(*A.spec, r);
+  trace_one<F, ACCUM>(A, accum, i, r);
 }
 
-__global__ __launch_bounds__(256) void trace_grid_f64(const TraceParams P, const GridIn g) {
+template <int F, bool ACCUM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
+void trace_grid_f64(const TraceArgs A, const GridIn g, const RaysOut accum) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
+  if (i >= A.n) return;
   Ray<double> r = grid_ray(ld(g.field[0], i, g.field_default[0]), ld(g.field[1], i, g.field_default[1]),
                            ld(g.pupil[0], i, g.pupil_default[0]), ld(g.pupil[1], i, g.pupil_default[1]),
                            ld(g.wavelength, i, g.wavelength_default),
                            ld(g.intensity, i, g.intensity_default), g.object_at_infinity);
-  apply_pre(P.spec, r);
-  trace_one(P, i, r);
+  apply_pre(*A.spec, r);
+  trace_one<F, ACCUM>(A, accum, i, r);
+}
+
+// launch the variant covering the system's features; accumulation always
+// uses the all-features kernel
+template <typename In>
+static void launch_trace(void (*k0)(TraceArgs, In, RaysOut), void (*k3)(TraceArgs, In, RaysOut),
+                         void (*k7)(TraceArgs, In, RaysOut), void (*k7a)(TraceArgs, In, RaysOut),
+                         int features, bool accum, int64_t n, hipStream_t stream,
+                         const TraceArgs &A, const In &in, const RaysOut &acc) {
+  auto kern = accum ? k7a : (features == 0 ? k0 : (features == (FEAT_TOROID | FEAT_RULINGS) ? k3 : k7));
+  hipLaunchKernelGGL(kern, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A, in, acc);
 }
 
 // per-operator evaluation --------------------------------------------------
@@ -463,6 +519,7 @@ int32_t optk_system_destroy(optk_system *sys) {
   (void)hipFree(sys->stacks);
   (void)hipFree(sys->chemicals);
   (void)hipFree(sys->tables);
+  (void)hipFree(sys->spec);
   delete sys;
   return OPTK_OK;
 }
@@ -489,11 +546,25 @@ int32_t optk_system_create(const optk_system_desc *desc, optk_system **out) {
   s->pre_is_inverse[1] = desc->pre_is_inverse[1];
   s->post = desc->post;
   s->host_surfaces.assign(desc->surfaces, desc->surfaces + desc->num_surfaces);
+  s->features = variant_of(features_of(desc));
   if ((rc = upload(&s->surfaces, desc->surfaces, desc->num_surfaces)) ||
       (rc = upload(&s->layers, desc->layers, desc->num_layers)) ||
       (rc = upload(&s->stacks, desc->stacks, desc->num_stacks)) ||
       (rc = upload(&s->chemicals, desc->chemicals, desc->num_chemicals)) ||
       (rc = upload(&s->tables, desc->tables, desc->num_tables))) {
+    optk_system_destroy(s);
+    return rc;
+  }
+  TraceSpec spec;
+  std::memset(&spec, 0, sizeof(spec));
+  spec.sys = view_of(s);
+  spec.pre[0] = s->pre[0];
+  spec.pre[1] = s->pre[1];
+  spec.pre_is_inverse[0] = s->pre_is_inverse[0];
+  spec.pre_is_inverse[1] = s->pre_is_inverse[1];
+  spec.post = s->post;
+  spec.apply_post = 0;
+  if ((rc = upload(&s->spec, &spec, 1))) {
     optk_system_destroy(s);
     return rc;
   }
@@ -516,31 +587,27 @@ static void fill_out(RaysOut &o, const optk_rays *r) {
 }
 
 static int32_t prepare(const optk_system *sys, int32_t precision, const optk_rays *out,
-                       const optk_rays *accum, int64_t n, int32_t flags, TraceParams &P) {
+                       const optk_rays *accum, int64_t n, int32_t flags, TraceArgs &A,
+                       RaysOut &acc) {
   if (!sys) return fail(OPTK_EINVAL, "null system");
   if (!out) return fail(OPTK_EINVAL, "null output rays");
   if (n < 0) return fail(OPTK_EINVAL, "negative ray count");
   if (precision != OPTK_F64) return fail(OPTK_EINVAL, "this entry point supports OPTK_F64 only");
-  std::memset(&P, 0, sizeof(P));
-  P.spec.sys = view_of(sys);
-  P.spec.pre[0] = sys->pre[0];
-  P.spec.pre[1] = sys->pre[1];
-  P.spec.pre_is_inverse[0] = sys->pre_is_inverse[0];
-  P.spec.pre_is_inverse[1] = sys->pre_is_inverse[1];
-  P.spec.post = sys->post;
-  P.spec.apply_post = (flags & OPTK_TRACE_APPLY_POST) ? 1 : 0;
-  P.n = n;
-  fill_out(P.out, out);
-  P.has_accum = accum ? 1 : 0;
-  fill_out(P.accum, accum);
+  std::memset(&A, 0, sizeof(A));
+  A.spec = sys->spec;
+  A.n = n;
+  A.apply_post = (flags & OPTK_TRACE_APPLY_POST) ? 1 : 0;
+  fill_out(A.out, out);
+  fill_out(acc, accum);
   return OPTK_OK;
 }
 
 int32_t optk_trace(const optk_system *sys, int32_t precision, const optk_rays *in,
                    const optk_rays *out, const optk_rays *accum, int64_t n, int32_t flags,
                    void *hip_stream) {
-  TraceParams P;
-  int32_t rc = prepare(sys, precision, out, accum, n, flags, P);
+  TraceArgs A;
+  RaysOut acc;
+  int32_t rc = prepare(sys, precision, out, accum, n, flags, A, acc);
   if (rc) return rc;
   if (!in) return fail(OPTK_EINVAL, "null input rays");
   for (int k = 0; k < 3; ++k)
@@ -561,7 +628,9 @@ int32_t optk_trace(const optk_system *sys, int32_t precision, const optk_rays *i
   ri.attenuation_default = in->attenuation_default;
   ri.index_default = in->index_default;
   ri.unvignetted_default = in->unvignetted_default;
-  hipLaunchKernelGGL(trace_rays_f64, grid_for(n), dim3(256), 0, (hipStream_t)hip_stream, P, ri);
+  launch_trace<RaysIn>(trace_rays_f64<0, false>, trace_rays_f64<FEAT_TOROID | FEAT_RULINGS, false>,
+                       trace_rays_f64<FEAT_ALL, false>, trace_rays_f64<FEAT_ALL, true>, sys->features,
+                       accum != nullptr, n, (hipStream_t)hip_stream, A, ri, acc);
   OPTK_HIP_CHECK(hipGetLastError());
   return OPTK_OK;
 }
@@ -569,8 +638,9 @@ int32_t optk_trace(const optk_system *sys, int32_t precision, const optk_rays *i
 int32_t optk_trace_grid(const optk_system *sys, int32_t precision, const optk_grid *in,
                         const optk_rays *out, const optk_rays *accum, int64_t n, int32_t flags,
                         void *hip_stream) {
-  TraceParams P;
-  int32_t rc = prepare(sys, precision, out, accum, n, flags, P);
+  TraceArgs A;
+  RaysOut acc;
+  int32_t rc = prepare(sys, precision, out, accum, n, flags, A, acc);
   if (rc) return rc;
   if (!in) return fail(OPTK_EINVAL, "null input grid");
   if (n == 0) return OPTK_OK;
@@ -586,7 +656,9 @@ int32_t optk_trace_grid(const optk_system *sys, int32_t precision, const optk_gr
   g.wavelength_default = in->wavelength_default;
   g.intensity_default = in->intensity_default;
   g.object_at_infinity = in->object_at_infinity;
-  hipLaunchKernelGGL(trace_grid_f64, grid_for(n), dim3(256), 0, (hipStream_t)hip_stream, P, g);
+  launch_trace<GridIn>(trace_grid_f64<0, false>, trace_grid_f64<FEAT_TOROID | FEAT_RULINGS, false>,
+                       trace_grid_f64<FEAT_ALL, false>, trace_grid_f64<FEAT_ALL, true>, sys->features,
+                       accum != nullptr, n, (hipStream_t)hip_stream, A, g, acc);
   OPTK_HIP_CHECK(hipGetLastError());
   return OPTK_OK;
 }

@@ -7,6 +7,17 @@
 
 namespace optk {
 
+// Compile-time feature masks of the trace kernels: a system that needs no
+// toroid Newton solve, no rulings or no multilayer transfer matrices runs a
+// kernel without that code, which keeps its register budget (and hence its
+// occupancy) at what the remaining operators need.
+enum : int {
+  FEAT_TOROID = 1,
+  FEAT_RULINGS = 2,
+  FEAT_MULTILAYER = 4,
+  FEAT_ALL = 7,
+};
+
 struct SystemView {
   const optk_surface *surfaces;
   int num_surfaces;
@@ -210,9 +221,9 @@ OPTK_HD double toroid_intercept_t(const optk_surface &s, double ox, double oy, d
 
 // AbstractSag.propagate_rays (optika/sags/_abc.py:109-122) in the surface
 // frame: move to the intercept and attenuate.
-OPTK_HD void sag_propagate(const optk_surface &s, Ray<double> &r) {
+template <int F = FEAT_ALL> OPTK_HD void sag_propagate(const optk_surface &s, Ray<double> &r) {
   double nx, ny, nz;
-  if (s.sag_kind == OPTK_SAG_TOROID) {
+  if ((F & FEAT_TOROID) && s.sag_kind == OPTK_SAG_TOROID) {
     double t = toroid_intercept_t(s, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
     nx = r.px + r.dx * t;
     ny = r.py + r.dy * t;
@@ -412,6 +423,7 @@ OPTK_HD double material_attenuation(const optk_surface &s, double attenuation) {
              : attenuation;
 }
 
+template <int F = FEAT_ALL>
 OPTK_HD double material_efficiency(const optk_surface &s, const StackView &sv, double wavelength,
                                    V3<double> a, double index, double attenuation,
                                    V3<double> normal) {
@@ -422,6 +434,7 @@ OPTK_HD double material_efficiency(const optk_surface &s, const StackView &sv, d
   }
   case OPTK_MATERIAL_MULTILAYER_MIRROR:
   case OPTK_MATERIAL_MULTILAYER_FILM: {  // _multilayers.py:839-935
+    if (!(F & FEAT_MULTILAYER)) return 1.0;
     double k = attenuation * wavelength / (4.0 * M_PI) * 1e-6;
     C<double> n = {index, k};
     double cosv = -dot3(a, normal);
@@ -508,18 +521,20 @@ OPTK_HD bool aperture_contains(const optk_surface &s, const double *tables, doub
 // the surface step
 // ---------------------------------------------------------------------------
 
+template <int F = FEAT_ALL>
 OPTK_HD void propagate_surface(const optk_surface &s, const SystemView &sys, int path,
                                Ray<double> &r) {
   xf_inverse_pos(s.transform, r.px, r.py, r.pz);
   xf_inverse_dir(s.transform, r.dx, r.dy, r.dz);
 
-  sag_propagate(s, r);
+  sag_propagate<F>(s, r);
   V3<double> p1 = {r.px, r.py, r.pz};
   V3<double> normal = sag_normal(s, p1.x, p1.y, p1.z);
 
   V3<double> a = {r.dx, r.dy, r.dz};
   int order = 0;
-  if (s.rulings_kind != OPTK_RULINGS_NONE) {
+  const bool ruled = (F & FEAT_RULINGS) && s.rulings_kind != OPTK_RULINGS_NONE;
+  if (ruled) {
     order = path_order(s, path);
     V3<double> kappa = spacing_vector(s, p1, normal);
     a = incident_effective(s, order, r.wavelength, a, r.index, normal, kappa);
@@ -531,8 +546,8 @@ OPTK_HD void propagate_surface(const optk_surface &s, const SystemView &sys, int
   double wavelength_2 = r.wavelength / ratio;
   V3<double> b = snells_law(a, n1, n2, normal, s.is_mirror != 0);
   double eff =
-      material_efficiency(s, sys.sv, r.wavelength, a, r.index, r.attenuation, normal);
-  if (s.rulings_kind != OPTK_RULINGS_NONE)
+      material_efficiency<F>(s, sys.sv, r.wavelength, a, r.index, r.attenuation, normal);
+  if (ruled)
     eff = eff * rulings_efficiency(s, order, r.wavelength, a, p1, normal, sys.sv.tables);
   r.intensity = r.intensity * eff;
   r.attenuation = material_attenuation(s, r.attenuation);
