@@ -48,6 +48,7 @@ extern "C" {
 #define OPTK_F32 1 /* float32 compensated form (float I/O) */
 
 #define OPTK_MAX_ORDERS 8
+#define OPTK_MAX_SURFACES 64
 #define OPTK_MAX_POLY 8
 
 /* kinds -------------------------------------------------------------------*/

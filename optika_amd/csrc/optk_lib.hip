@@ -103,13 +103,15 @@ struct GridIn {
 };
 
 // Per-launch arguments.  The system (surfaces, tables, transformations) is
-// device-resident and reached through one pointer, so the kernel argument
-// block stays small and the record fields are fetched with scalar loads
-// where they are used.
+// device-resident; each block stages the transformation spec and every
+// surface record into LDS once, then a grid-stride loop traces its rays
+// reading the records from LDS (uniform addresses: broadcast reads).
 struct TraceArgs {
-  const TraceSpec *spec;  // device copy
-  int64_t n;
+  const TraceSpec *spec;         // device copy
+  const optk_surface *surfaces;  // device copy
+  int32_t num_surfaces;
   int32_t apply_post;
+  int64_t n;
   RaysOut out;
 };
 
@@ -131,41 +133,7 @@ __device__ __forceinline__ void store(const RaysOut &o, int64_t j, const Ray<dou
   if (o.unvignetted) o.unvignetted[j] = r.unvignetted ? 1 : 0;
 }
 
-// every diffraction-order path of one ray, through every surface
-template <int F, bool ACCUM>
-__device__ __forceinline__ void trace_one(const TraceArgs &A, const RaysOut &accum, int64_t i,
-                                          const Ray<double> &r0) {
-  const TraceSpec &spec = *A.spec;
-  const SystemView &sys = spec.sys;
-  const int64_t n = A.n;
-  // only rulings create diffraction-order paths: without them one path
-  const int num_paths = (F & FEAT_RULINGS) ? sys.num_paths : 1;
-  const int num_surfaces = sys.num_surfaces;
-  for (int path = 0; path < num_paths; ++path) {
-    Ray<double> r = r0;
-    for (int k = 0; k < num_surfaces; ++k) {
-      propagate_surface<F>(sys.surfaces[k], sys, path, r);
-      if (ACCUM) store(accum, ((int64_t)k * num_paths + path) * n + i, r);
-    }
-    if (A.apply_post) {
-      xf_inverse_pos(spec.post, r.px, r.py, r.pz);
-      xf_inverse_dir(spec.post, r.dx, r.dy, r.dz);
-    }
-    store(A.out, (int64_t)path * n + i, r);
-  }
-}
-
-// occupancy targets: the geometric kernels keep <= 128 VGPRs (4 waves per
-// SIMD) to hide HBM latency; the multilayer kernels are VALU-bound
-template <int F> struct Waves {
-  static constexpr int value = (F & FEAT_MULTILAYER) ? 2 : 4;
-};
-
-template <int F, bool ACCUM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
-void trace_rays_f64(const TraceArgs A, const RaysIn in, const RaysOut accum) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n) return;
+__device__ __forceinline__ Ray<double> load_ray(const RaysIn &in, int64_t i) {
   Ray<double> r;
   r.px = in.p[0][i];
   r.py = in.p[1][i];
@@ -178,21 +146,77 @@ void trace_rays_f64(const TraceArgs A, const RaysIn in, const RaysOut accum) {
   r.attenuation = ld(in.attenuation, i, in.attenuation_default);
   r.index = ld(in.index, i, in.index_default);
   r.unvignetted = in.unvignetted ? (in.unvignetted[i] != 0) : (in.unvignetted_default != 0);
-  apply_pre(*A.spec, r);
-  trace_one<F, ACCUM>(A, accum, i, r);
+  return r;
+}
+
+__device__ __forceinline__ Ray<double> load_ray(const GridIn &g, int64_t i) {
+  return grid_ray(ld(g.field[0], i, g.field_default[0]), ld(g.field[1], i, g.field_default[1]),
+                  ld(g.pupil[0], i, g.pupil_default[0]), ld(g.pupil[1], i, g.pupil_default[1]),
+                  ld(g.wavelength, i, g.wavelength_default), ld(g.intensity, i, g.intensity_default),
+                  g.object_at_infinity);
+}
+
+// bytes of LDS a launch stages: the spec, then the surface records
+static size_t staged_bytes(int num_surfaces) {
+  return sizeof(TraceSpec) + sizeof(optk_surface) * (size_t)num_surfaces;
+}
+
+// occupancy targets: the geometric kernels keep <= 128 VGPRs (4 waves per
+// SIMD) to hide HBM latency; the multilayer kernels are VALU-bound
+template <int F> struct Waves {
+  static constexpr int value = (F & FEAT_MULTILAYER) ? 2 : 4;
+};
+
+template <int F, bool ACCUM, typename In>
+__device__ __forceinline__ void trace_body(const TraceArgs &A, const In &in, const RaysOut &accum) {
+  extern __shared__ double staged[];
+  TraceSpec *s_spec = reinterpret_cast<TraceSpec *>(staged);
+  optk_surface *s_surf = reinterpret_cast<optk_surface *>(s_spec + 1);
+  {
+    const uint64_t *src_spec = reinterpret_cast<const uint64_t *>(A.spec);
+    uint64_t *dst_spec = reinterpret_cast<uint64_t *>(s_spec);
+    for (int w = threadIdx.x; w < (int)(sizeof(TraceSpec) / 8); w += blockDim.x) dst_spec[w] = src_spec[w];
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(A.surfaces);
+    uint64_t *dst = reinterpret_cast<uint64_t *>(s_surf);
+    const int words = A.num_surfaces * (int)(sizeof(optk_surface) / 8);
+    for (int w = threadIdx.x; w < words; w += blockDim.x) dst[w] = src[w];
+  }
+  __syncthreads();
+  SystemView sys = s_spec->sys;
+  sys.surfaces = s_surf;
+  // only rulings create diffraction-order paths: without them one path
+  const int num_paths = (F & FEAT_RULINGS) ? sys.num_paths : 1;
+  const int num_surfaces = A.num_surfaces;
+  const int64_t n = A.n;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    Ray<double> r0 = load_ray(in, i);
+    apply_pre(*s_spec, r0);
+    for (int path = 0; path < num_paths; ++path) {
+      Ray<double> r = r0;
+      for (int k = 0; k < num_surfaces; ++k) {
+        propagate_surface<F>(s_surf[k], sys, path, r);
+        if (ACCUM) store(accum, ((int64_t)k * num_paths + path) * n + i, r);
+      }
+      if (A.apply_post) {
+        xf_inverse_pos(s_spec->post, r.px, r.py, r.pz);
+        xf_inverse_dir(s_spec->post, r.dx, r.dy, r.dz);
+      }
+      store(A.out, (int64_t)path * n + i, r);
+    }
+  }
+}
+
+template <int F, bool ACCUM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
+void trace_rays_f64(const TraceArgs A, const RaysIn in, const RaysOut accum) {
+  trace_body<F, ACCUM>(A, in, accum);
 }
 
 template <int F, bool ACCUM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
 void trace_grid_f64(const TraceArgs A, const GridIn g, const RaysOut accum) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n) return;
-  Ray<double> r = grid_ray(ld(g.field[0], i, g.field_default[0]), ld(g.field[1], i, g.field_default[1]),
-                           ld(g.pupil[0], i, g.pupil_default[0]), ld(g.pupil[1], i, g.pupil_default[1]),
-                           ld(g.wavelength, i, g.wavelength_default),
-                           ld(g.intensity, i, g.intensity_default), g.object_at_infinity);
-  apply_pre(*A.spec, r);
-  trace_one<F, ACCUM>(A, accum, i, r);
+  trace_body<F, ACCUM>(A, g, accum);
 }
 
 // launch the variant covering the system's features; accumulation always
@@ -203,7 +227,12 @@ static void launch_trace(void (*k0)(TraceArgs, In, RaysOut), void (*k3)(TraceArg
                          int features, bool accum, int64_t n, hipStream_t stream,
                          const TraceArgs &A, const In &in, const RaysOut &acc) {
   auto kern = accum ? k7a : (features == 0 ? k0 : (features == (FEAT_TOROID | FEAT_RULINGS) ? k3 : k7));
-  hipLaunchKernelGGL(kern, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A, in, acc);
+  // enough blocks to fill every CU a few times over; each block loops
+  int64_t blocks = (n + 255) / 256;
+  const int64_t max_blocks = 256 * 16;
+  if (blocks > max_blocks) blocks = max_blocks;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), staged_bytes(A.num_surfaces), stream, A,
+                     in, acc);
 }
 
 // per-operator evaluation --------------------------------------------------
@@ -426,7 +455,9 @@ static bool valid_affine(const optk_affine &a) {
 // Check every index a kernel will follow, so no kernel can read outside the
 // uploaded records.
 static int32_t validate(const optk_system_desc *d) {
-  if (d->num_surfaces < 0 || d->num_surfaces > 4096) return fail(OPTK_EINVAL, "num_surfaces out of range");
+  // every surface record is staged in LDS (about 0.9 KiB each)
+  if (d->num_surfaces < 0 || d->num_surfaces > OPTK_MAX_SURFACES)
+    return fail(OPTK_EINVAL, "num_surfaces out of range (at most 64)");
   if (d->num_paths < 1 || d->num_paths > OPTK_MAX_ORDERS) return fail(OPTK_EINVAL, "num_paths out of range");
   for (int c = 0; c < d->num_chemicals; ++c) {
     const optk_chemical &ch = d->chemicals[c];
@@ -486,6 +517,8 @@ static int32_t validate(const optk_system_desc *d) {
   if (!valid_affine(d->post)) return fail(OPTK_EINVAL, "bad post transformation");
   return OPTK_OK;
 }
+
+static_assert(sizeof(optk_surface) % 8 == 0 && sizeof(TraceSpec) % 8 == 0, "records are staged as words");
 
 static dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
@@ -595,6 +628,8 @@ static int32_t prepare(const optk_system *sys, int32_t precision, const optk_ray
   if (precision != OPTK_F64) return fail(OPTK_EINVAL, "this entry point supports OPTK_F64 only");
   std::memset(&A, 0, sizeof(A));
   A.spec = sys->spec;
+  A.surfaces = sys->surfaces;
+  A.num_surfaces = sys->num_surfaces;
   A.n = n;
   A.apply_post = (flags & OPTK_TRACE_APPLY_POST) ? 1 : 0;
   fill_out(A.out, out);

@@ -145,7 +145,8 @@ OPTK_HD double sag_intercept_t(const optk_surface &s, double ox, double oy, doub
     double sq = sqrt(real ? disc : 0.0);
     bool degenerate = fabs(a) < 1e-12;
     double denom = degenerate ? 1.0 : 2.0 * a;
-    double t_lin = -cc / b;
+    double t_lin = 0.0;
+    if (degenerate) t_lin = -cc / b;  // only degenerate lanes use it
     double tr[2];
     for (int k = 0; k < 2; ++k) {
       double sgn = k == 0 ? -1.0 : 1.0;
@@ -245,8 +246,13 @@ template <int F = FEAT_ALL> OPTK_HD void sag_propagate(const optk_surface &s, Ra
     r.dz = uz;
   }
   double ddx = nx - r.px, ddy = ny - r.py, ddz = nz - r.pz;
-  double len = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-  double f = exp(-r.attenuation * len);
+  // exp(-alpha |d|) is exactly 1 for alpha = 0 and a finite displacement:
+  // skip the square root and the exponential for non-attenuating media
+  double f = 1.0;
+  if (!(r.attenuation == 0.0 && fabs(ddx) < 1e150 && fabs(ddy) < 1e150 && fabs(ddz) < 1e150)) {
+    double len = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+    f = exp(-r.attenuation * len);
+  }
   r.intensity = f * r.intensity;
   r.px = nx;
   r.py = ny;
@@ -448,14 +454,21 @@ OPTK_HD double material_efficiency(const optk_surface &s, const StackView &sv, d
 }
 
 // vector Snell's law (optika/materials/_snells_law.py:341-366)
+// n1 / n2, with the (exact) value 1 when the indices agree: mirrors and
+// vacuum-to-vacuum surfaces then skip two float64 divisions
+OPTK_HD double index_ratio(double n1, double n2) {
+  return (n1 == n2 && n1 != 0.0 && fabs(n1) < INFINITY) ? 1.0 : n1 / n2;
+}
+
 OPTK_HD V3<double> snells_law(V3<double> a, double n1, double n2, V3<double> u, bool is_mirror) {
   double a2 = a.x * a.x + a.y * a.y + a.z * a.z;
-  double r = n1 / n2;
+  double r = index_ratio(n1, n2);
   double r2 = r * r;
   double au = a.x * u.x + a.y * u.y + a.z * u.z;
   double au2 = au * au;
   double sgn = -copysign(1.0, au);
-  double d = -au + sgn * (2.0 * (double)is_mirror - 1.0) * sqrt(1.0 / r2 + au2 - a2);
+  double inv_r2 = r == 1.0 ? 1.0 : 1.0 / r2;
+  double d = -au + sgn * (2.0 * (double)is_mirror - 1.0) * sqrt(inv_r2 + au2 - a2);
   return {r * (a.x + d * u.x), r * (a.y + d * u.y), r * (a.z + d * u.z)};
 }
 
@@ -542,8 +555,8 @@ OPTK_HD void propagate_surface(const optk_surface &s, const SystemView &sys, int
 
   double n1 = r.index;
   double n2 = material_index(s, r.wavelength, n1);
-  double ratio = n1 / n2;
-  double wavelength_2 = r.wavelength / ratio;
+  double ratio = index_ratio(n1, n2);
+  double wavelength_2 = ratio == 1.0 ? r.wavelength : r.wavelength / ratio;
   V3<double> b = snells_law(a, n1, n2, normal, s.is_mirror != 0);
   double eff =
       material_efficiency<F>(s, sys.sv, r.wavelength, a, r.index, r.attenuation, normal);
