@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--cpu-rays", type=int, default=1 << 18,
                    help="rays in the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="also run a 256 MiB device copy (to scale HBM byte counters)")
     return p.parse_args()
 
 
@@ -132,6 +134,13 @@ def main():
         "kernel_avg_ms": round(kernel_avg_ms, 4),
         "bytes_per_ray": BYTES_PER_RAY_F64,
     }
+
+    if args.calibrate_copy:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
 
     cpu = None
     if not args.no_cpu:
