@@ -122,6 +122,7 @@ typedef struct optk_surface {
   int32_t pad0;
   double sag_p[4];            /* sphere: R | conic: R, k | parabola: R, -1, f |
                                  cylinder: R | toroid: R, R_rotation */
+  double sag_c;               /* 1 / sag_p[0]; filled in by optk_system_create */
   optk_affine sag_transform;  /* sag frame -> surface frame */
 
   /* material */

@@ -59,6 +59,7 @@ class Surface(ctypes.Structure):
         ("transform", Affine),
         ("sag_kind", c_int32), ("pad0", c_int32),
         ("sag_p", c_double * 4),
+        ("sag_c", c_double),
         ("sag_transform", Affine),
         ("material_kind", c_int32), ("is_mirror", c_int32),
         ("material_p", c_double * 6),

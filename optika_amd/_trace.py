@@ -102,7 +102,8 @@ class _Out:
         for f in ("wavelength", "intensity", "attenuation", "index_refraction"):
             if f in fields:
                 self.t[f] = mk()
-        self.t["unvignetted"] = torch.empty(total, dtype=torch.uint8, device=dev)
+        # torch.bool is one byte holding 0 or 1: the kernel writes it directly
+        self.t["unvignetted"] = torch.empty(total, dtype=torch.bool, device=dev)
         r = _lib.Rays()
         for k in range(3):
             r.position[k] = self.t["position"][k].data_ptr()
@@ -130,7 +131,7 @@ def _make_rays(out: _Out, shape, scalars: dict) -> RayVectorArray:
         intensity=field("intensity"),
         attenuation=field("attenuation"),
         index_refraction=field("index_refraction"),
-        unvignetted=shp(out.t["unvignetted"]).bool(),
+        unvignetted=shp(out.t["unvignetted"]),
     )
 
 

@@ -579,8 +579,9 @@ int32_t optk_system_create(const optk_system_desc *desc, optk_system **out) {
   s->pre_is_inverse[1] = desc->pre_is_inverse[1];
   s->post = desc->post;
   s->host_surfaces.assign(desc->surfaces, desc->surfaces + desc->num_surfaces);
+  for (auto &surf : s->host_surfaces) derive_constants(surf);
   s->features = variant_of(features_of(desc));
-  if ((rc = upload(&s->surfaces, desc->surfaces, desc->num_surfaces)) ||
+  if ((rc = upload(&s->surfaces, s->host_surfaces.data(), desc->num_surfaces)) ||
       (rc = upload(&s->layers, desc->layers, desc->num_layers)) ||
       (rc = upload(&s->stacks, desc->stacks, desc->num_stacks)) ||
       (rc = upload(&s->chemicals, desc->chemicals, desc->num_chemicals)) ||

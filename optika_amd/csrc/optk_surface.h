@@ -18,6 +18,11 @@ enum : int {
   FEAT_ALL = 7,
 };
 
+// Per-surface constants the kernels read instead of recomputing per ray.
+// The curvature is the same IEEE quotient the reference forms once per call
+// (``c = 1 / self.radius``), so precomputing it changes no result.
+OPTK_HD void derive_constants(optk_surface &s) { s.sag_c = 1.0 / s.sag_p[0]; }
+
 struct SystemView {
   const optk_surface *surfaces;
   int num_surfaces;
@@ -40,24 +45,24 @@ OPTK_HD double sag_z(const optk_surface &s, double x, double y, double z) {
   xf_inverse_pos(T, x, y, z);
   switch (s.sag_kind) {
   case OPTK_SAG_SPHERE: {  // _spherical.py:100-123
-    double c = 1.0 / s.sag_p[0];
+    double c = s.sag_c;
     double r2 = x * x + y * y;
     return c * r2 / (1.0 + sqrt(1.0 - c * c * r2));
   }
   case OPTK_SAG_CONIC:
   case OPTK_SAG_PARABOLA: {  // _conic.py:31-48
-    double c = 1.0 / s.sag_p[0];
+    double c = s.sag_c;
     double conic = s.sag_p[1];
     double r2 = x * x + y * y;
     return c * r2 / (1.0 + sqrt(1.0 - (1.0 + conic) * (c * c) * r2));
   }
   case OPTK_SAG_CYLINDER: {  // _cylindrical.py:75-91
-    double c = 1.0 / s.sag_p[0];
+    double c = s.sag_c;
     double r2 = x * x;
     return c * r2 / (1.0 + sqrt(1.0 - c * c * r2));
   }
   case OPTK_SAG_TOROID: {  // _toroidal.py:38-57
-    double c = 1.0 / s.sag_p[0];
+    double c = s.sag_c;
     double r = s.sag_p[1];
     double x2 = x * x;
     double y2 = y * y;
@@ -74,13 +79,13 @@ OPTK_HD V3<double> sag_normal(const optk_surface &s, double x, double y, double 
   xf_inverse_pos(s.sag_transform, x, y, z);
   switch (s.sag_kind) {
   case OPTK_SAG_SPHERE: {  // _spherical.py:125-146 (not renormalised)
-    double c = 1.0 / s.sag_p[0];
+    double c = s.sag_c;
     double nx = c * x;
     double ny = c * y;
     return {nx, ny, -sqrt(1.0 - nx * nx - ny * ny)};
   }
   case OPTK_SAG_CONIC: {  // _conic.py:50-75
-    double c = 1.0 / s.sag_p[0];
+    double c = s.sag_c;
     double conic = s.sag_p[1];
     double x2 = x * x;
     double y2 = y * y;
@@ -100,7 +105,7 @@ OPTK_HD V3<double> sag_normal(const optk_surface &s, double x, double y, double 
     return {nx, 0.0, -sqrt(1.0 - nx * nx)};
   }
   case OPTK_SAG_TOROID: {  // _toroidal.py:59-88
-    double c = 1.0 / s.sag_p[0];
+    double c = s.sag_c;
     double r = s.sag_p[1];
     double x2 = x * x;
     double y2 = y * y;
@@ -135,7 +140,7 @@ OPTK_HD double sag_intercept_t(const optk_surface &s, double ox, double oy, doub
     return -q - sign_of(r * uz) * sqrt(q * q - (px * px + py * py + pz * pz - r * r));
   }
   case OPTK_SAG_CONIC: {  // _conic.py:120-170
-    double c = 1.0 / s.sag_p[0];
+    double c = s.sag_c;
     double kp1 = 1.0 + s.sag_p[1];
     double a = c * (ux * ux + uy * uy + kp1 * (uz * uz));
     double b = 2.0 * (c * (ox * ux + oy * uy + kp1 * oz * uz) - uz);
@@ -193,7 +198,7 @@ OPTK_HD double sag_intercept_t(const optk_surface &s, double ox, double oy, doub
 OPTK_HD double toroid_intercept_t(const optk_surface &s, double ox, double oy, double oz, double ux,
                                   double uy, double uz) {
   const optk_affine &T = s.sag_transform;
-  double c = 1.0 / s.sag_p[0];
+  double c = s.sag_c;
   double r = s.sag_p[1];
   double vx = ux, vy = uy, vz = uz;  // direction in the sag frame (chain rule)
   xf_inverse_dir(T, vx, vy, vz);

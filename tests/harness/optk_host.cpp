@@ -3,6 +3,7 @@
 // device arithmetic can be compared with the numpy oracle on a machine
 // without a GPU.  Never linked into or loaded by optika_amd.
 #include <cstdint>
+#include <vector>
 
 #include "../../optika_amd/csrc/optk_surface.h"
 
@@ -22,6 +23,12 @@ SystemView view_of(const optk_system_desc *d) {
 }
 }  // namespace
 
+std::vector<optk_surface> derived(const optk_system_desc *d) {
+  std::vector<optk_surface> v(d->surfaces, d->surfaces + d->num_surfaces);
+  for (auto &s : v) derive_constants(s);
+  return v;
+}
+
 extern "C" {
 
 // Trace n grid rays; out[0..9] = px py pz dx dy dz wavelength intensity
@@ -30,8 +37,10 @@ int host_trace_grid(const optk_system_desc *d, const double *fx, const double *f
                     const double *ux, const double *uy, const double *wl, const double *inten,
                     int64_t n, int object_at_infinity, int post, double **out,
                     uint8_t *mask) {
+  std::vector<optk_surface> surf = derived(d);
   TraceSpec P;
   P.sys = view_of(d);
+  P.sys.surfaces = surf.data();
   P.pre[0] = d->pre[0];
   P.pre[1] = d->pre[1];
   P.pre_is_inverse[0] = d->pre_is_inverse[0];
@@ -43,7 +52,7 @@ int host_trace_grid(const optk_system_desc *d, const double *fx, const double *f
     apply_pre(P, r0);
     for (int path = 0; path < d->num_paths; ++path) {
       Ray<double> r = r0;
-      for (int k = 0; k < d->num_surfaces; ++k) propagate_surface(d->surfaces[k], P.sys, path, r);
+      for (int k = 0; k < d->num_surfaces; ++k) propagate_surface(surf[k], P.sys, path, r);
       apply_post(P, r);
       int64_t j = (int64_t)path * n + i;
       double v[10] = {r.px, r.py, r.pz, r.dx, r.dy, r.dz, r.wavelength, r.intensity,
