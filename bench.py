@@ -36,6 +36,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 #  system and are returned as the input / constants, not rewritten)
 BYTES_PER_RAY_F64 = 97
 
+# HBM bytes per launch of trace_grid_f64 at 2^24 rays, from rocprofv3 --pmc:
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 256 MiB device copy);
+# the runs and commands are in profiles/trace_grid_f64_newtonian_2p24.md
+TRAFFIC_BYTES_F64_2P24 = (342198 * 2.0 + 950396 * 1.0) * 1024
+
 
 def parse():
     p = argparse.ArgumentParser()
@@ -129,7 +134,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": TRAFFIC_BYTES_F64_2P24 if n == 1 << 24 else None,
         "kernel": "trace_grid_f64",
         "kernel_avg_ms": round(kernel_avg_ms, 4),
         "bytes_per_ray": BYTES_PER_RAY_F64,

@@ -295,8 +295,8 @@ __global__ __launch_bounds__(256) void eval_f64(const EvalParams E) {
     break;
   }
   case OPTK_OP_MULTILAYER: {
-    MultilayerOut co;
-    MultilayerRT rt = multilayer_efficiency(E.sys.sv, E.stack, in[0][i], C<double>{in[1][i], in[2][i]},
+    MultilayerOut<double> co;
+    MultilayerRT<double> rt = multilayer_efficiency<double>(E.sys.sv, E.stack, in[0][i], C<double>{in[1][i], in[2][i]},
                                             C<double>{in[3][i], in[4][i]}, true, &co);
     out[0][i] = rt.R[0];
     out[1][i] = rt.R[1];

@@ -127,6 +127,9 @@ template <typename T> OPTK_HD C<T> cexp(C<T> z) {
   return {e * c, e * s};
 }
 
+template <typename T, typename U> OPTK_HD C<T> to_c_impl(C<U> z) { return {T(z.re), T(z.im)}; }
+template <typename T> OPTK_HD C<T> to_c(C<double> z) { return to_c_impl<T, double>(z); }
+
 template <typename T> struct M2 {
   C<T> a, b, c, d;  // [[a, b], [c, d]]
 };
@@ -323,60 +326,61 @@ OPTK_HD T profile_factor(int kind, T width, T wavelength, C<T> direction, C<T> n
   }
 }
 
-struct LayerState {
-  C<double> n;          // index of the medium before the next layer
-  C<double> direction;  // cosine in that medium
+template <typename T> struct LayerState {
+  C<T> n;          // index of the medium before the next layer
+  C<T> direction;  // cosine in that medium
   bool where;           // transfer matrices still valid
 };
 
 // Layer.transfer (optika/materials/_layers.py:229-277) for one polarization.
-OPTK_HD M2<double> layer_transfer(const optk_layer &layer, const StackView &sv, double wavelength,
-                                  bool pol_s, LayerState &st, double thickness) {
-  C<double> n_int = {1.0, 0.0};
-  if (layer.chemical >= 0) n_int = chemical_n(sv.chemicals[layer.chemical], sv.tables, wavelength);
-  C<double> d_int = snell_scalar(st.direction, st.n, n_int);
+template <typename T>
+OPTK_HD M2<T> layer_transfer(const optk_layer &layer, const StackView &sv, T wavelength,
+                              bool pol_s, LayerState<T> &st, T thickness) {
+  C<T> n_int = {T(1), T(0)};
+  if (layer.chemical >= 0) n_int = to_c<T>(chemical_n(sv.chemicals[layer.chemical], sv.tables, (double)wavelength));
+  C<T> d_int = snell_scalar(st.direction, st.n, n_int);
 
   // matrices.refraction (matrices.py:127-161)
-  C<double> one = {1.0, 0.0};
-  C<double> di = pol_s ? st.direction : cconj(st.direction);
-  C<double> dj = pol_s ? d_int : cconj(d_int);
-  C<double> imp_i = pol_s ? st.n : cdiv(one, st.n);
-  C<double> imp_j = pol_s ? n_int : cdiv(one, n_int);
-  C<double> qi = cmul(di, imp_i);
-  C<double> qj = cmul(dj, imp_j);
-  C<double> a = cadd(qi, qj);
-  C<double> r = cdiv(csub(qi, qj), a);
-  C<double> t = cdiv(cscale(2.0, qi), a);
+  C<T> one = {T(1), T(0)};
+  C<T> di = pol_s ? st.direction : cconj(st.direction);
+  C<T> dj = pol_s ? d_int : cconj(d_int);
+  C<T> imp_i = pol_s ? st.n : cdiv(one, st.n);
+  C<T> imp_j = pol_s ? n_int : cdiv(one, n_int);
+  C<T> qi = cmul(di, imp_i);
+  C<T> qj = cmul(dj, imp_j);
+  C<T> a = cadd(qi, qj);
+  C<T> r = cdiv(csub(qi, qj), a);
+  C<T> t = cdiv(cscale(T(2), qi), a);
   if (layer.profile_kind != OPTK_PROFILE_NONE) {
-    double f = profile_factor<double>(layer.profile_kind, layer.profile_width, wavelength, di, st.n);
+    T f = profile_factor<T>(layer.profile_kind, T(layer.profile_width), wavelength, di, st.n);
     r = cscale(f, r);  // r * factor (real factor promoted to complex)
   }
-  C<double> inv_t = cdiv(one, t);
-  C<double> r_t = cdiv(r, t);
-  M2<double> refraction = {inv_t, r_t, r_t, inv_t};
-  if (!st.where) refraction = midentity<double>();
+  C<T> inv_t = cdiv(one, t);
+  C<T> r_t = cdiv(r, t);
+  M2<T> refraction = {inv_t, r_t, r_t, inv_t};
+  if (!st.where) refraction = midentity<T>();
 
   // matrices.propagation (matrices.py:241-246):
   // beta = 2 pi h n cos / wavelength ; U = diag(exp(-i beta), exp(i beta))
-  C<double> beta =
-      cdiv(cmul(cscale(2.0 * M_PI * thickness, n_int), d_int), C<double>{wavelength, 0.0});
+  C<T> beta =
+      cdiv(cmul(cscale(T(2.0 * M_PI) * thickness, n_int), d_int), C<T>{wavelength, T(0)});
   // -1j * beta = (beta.im, -beta.re) ; +1j * beta = (-beta.im, beta.re)
-  C<double> u00 = cexp(C<double>{beta.im, -beta.re});
-  C<double> u11 = cexp(C<double>{-beta.im, beta.re});
-  bool where_prop = cabs(u00) < 1e10;
+  C<T> u00 = cexp(C<T>{beta.im, -beta.re});
+  C<T> u11 = cexp(C<T>{-beta.im, beta.re});
+  bool where_prop = cabs(u00) < T(1e10);
   st.where = st.where && where_prop;
-  M2<double> zero_prop = {u00, {0.0, 0.0}, {0.0, 0.0}, u11};
-  M2<double> transfer = mmul(refraction, zero_prop);
+  M2<T> zero_prop = {u00, {T(0), T(0)}, {T(0), T(0)}, u11};
+  M2<T> transfer = mmul(refraction, zero_prop);
   if (!st.where) transfer = refraction;
   st.n = n_int;
   st.direction = d_int;
   return transfer;
 }
 
-OPTK_HD M2<double> matrix_power(M2<double> m, int exponent) {
+template <typename T> OPTK_HD M2<T> matrix_power(M2<T> m, int exponent) {
   // binary powering, numpy.linalg.matrix_power's order of products
-  if (exponent <= 0) return midentity<double>();
-  M2<double> z = m, result = m;
+  if (exponent <= 0) return midentity<T>();
+  M2<T> z = m, result = m;
   bool have_z = false, have_result = false;
   int e = exponent;
   while (e > 0) {
@@ -392,43 +396,45 @@ OPTK_HD M2<double> matrix_power(M2<double> m, int exponent) {
   return result;
 }
 
-OPTK_HD M2<double> sequence_transfer(const StackView &sv, int first, int count, double wavelength,
-                                     bool pol_s, LayerState &st) {
-  M2<double> result = midentity<double>();
+template <typename T>
+OPTK_HD M2<T> sequence_transfer(const StackView &sv, int first, int count, T wavelength,
+                                 bool pol_s, LayerState<T> &st) {
+  M2<T> result = midentity<T>();
   for (int i = 0; i < count; ++i) {
     const optk_layer &L = sv.layers[first + i];
-    result = mmul(result, layer_transfer(L, sv, wavelength, pol_s, st, L.thickness));
+    result = mmul(result, layer_transfer<T>(L, sv, wavelength, pol_s, st, T(L.thickness)));
   }
   return result;
 }
 
-struct MultilayerOut {
-  C<double> r[2];  // s, p
-  C<double> t[2];
+template <typename T> struct MultilayerOut {
+  C<T> r[2];  // s, p
+  C<T> t[2];
   bool where[2];
 };
 
 // multilayer_coefficients (optika/materials/_multilayers.py:186-237)
-OPTK_HD MultilayerOut multilayer_rt(const StackView &sv, int stack_index, double wavelength,
-                                    C<double> direction, C<double> n_ambient) {
+template <typename T>
+OPTK_HD MultilayerOut<T> multilayer_rt(const StackView &sv, int stack_index, T wavelength,
+                                       C<T> direction, C<T> n_ambient) {
   const optk_stack &stk = sv.stacks[stack_index];
-  MultilayerOut out;
+  MultilayerOut<T> out;
   for (int p = 0; p < 2; ++p) {
     bool pol_s = (p == 0);
-    LayerState st = {n_ambient, direction, true};
-    M2<double> m = midentity<double>();
+    LayerState<T> st = {n_ambient, direction, true};
+    M2<T> m = midentity<T>();
     bool first_block = true;
     for (int b = 0; b < stk.num_blocks; ++b) {
       const optk_block &blk = stk.blocks[b];
-      M2<double> x;
+      M2<T> x;
       if (blk.kind == OPTK_BLOCK_PERIODIC) {
-        M2<double> start =
-            sequence_transfer(sv, blk.first_layer, blk.num_layers, wavelength, pol_s, st);
-        M2<double> periodic =
-            sequence_transfer(sv, blk.first_layer, blk.num_layers, wavelength, pol_s, st);
+        M2<T> start =
+            sequence_transfer<T>(sv, blk.first_layer, blk.num_layers, wavelength, pol_s, st);
+        M2<T> periodic =
+            sequence_transfer<T>(sv, blk.first_layer, blk.num_layers, wavelength, pol_s, st);
         x = mmul(start, matrix_power(periodic, blk.num_periods - 1));
       } else {
-        x = sequence_transfer(sv, blk.first_layer, blk.num_layers, wavelength, pol_s, st);
+        x = sequence_transfer<T>(sv, blk.first_layer, blk.num_layers, wavelength, pol_s, st);
       }
       // a single top-level block is returned by its own transfer(); several
       // blocks are folded left from the identity (LayerSequence.transfer)
@@ -438,48 +444,50 @@ OPTK_HD MultilayerOut multilayer_rt(const StackView &sv, int stack_index, double
     // substrate (thickness forced to zero)
     optk_layer sub = {-1, OPTK_PROFILE_NONE, 0.0, 0.0};
     if (stk.substrate_layer >= 0) sub = sv.layers[stk.substrate_layer];
-    M2<double> ms = layer_transfer(sub, sv, wavelength, pol_s, st, 0.0);
-    M2<double> mt = mmul(m, ms);
+    M2<T> ms = layer_transfer<T>(sub, sv, wavelength, pol_s, st, T(0));
+    M2<T> mt = mmul(m, ms);
     out.r[p] = cdiv(mt.c, mt.a);
-    C<double> t = cdiv(C<double>{1.0, 0.0}, mt.a);
-    if (!st.where) t = {0.0, 0.0};
+    C<T> t = cdiv(C<T>{T(1), T(0)}, mt.a);
+    if (!st.where) t = {T(0), T(0)};
     out.t[p] = t;
     out.where[p] = st.where;
   }
   return out;
 }
 
-struct MultilayerRT {
-  double R[2], T[2];
+template <typename Real> struct MultilayerRT {
+  Real R[2], T[2];
 };
 
 // multilayer_efficiency (optika/materials/_multilayers.py:485-532)
-OPTK_HD MultilayerRT multilayer_efficiency(const StackView &sv, int stack_index, double wavelength,
-                                           C<double> direction, C<double> n_ambient,
-                                           bool want_transmission, MultilayerOut *coeffs = nullptr) {
-  MultilayerOut co = multilayer_rt(sv, stack_index, wavelength, direction, n_ambient);
+template <typename T>
+OPTK_HD MultilayerRT<T> multilayer_efficiency(const StackView &sv, int stack_index, T wavelength,
+                                              C<T> direction, C<T> n_ambient,
+                                              bool want_transmission,
+                                              MultilayerOut<T> *coeffs = nullptr) {
+  MultilayerOut<T> co = multilayer_rt<T>(sv, stack_index, wavelength, direction, n_ambient);
   if (coeffs) *coeffs = co;
-  MultilayerRT out;
+  MultilayerRT<T> out;
   for (int p = 0; p < 2; ++p) {
-    double a = cabs(co.r[p]);
+    T a = cabs(co.r[p]);
     out.R[p] = a * a;
-    out.T[p] = 0.0;
+    out.T[p] = T(0);
   }
   if (want_transmission) {
     const optk_stack &stk = sv.stacks[stack_index];
-    C<double> n_sub = {1.0, 0.0};
+    C<T> n_sub = {T(1), T(0)};
     if (stk.substrate_layer >= 0) {
       const optk_layer &sub = sv.layers[stk.substrate_layer];
-      if (sub.chemical >= 0) n_sub = chemical_n(sv.chemicals[sub.chemical], sv.tables, wavelength);
+      if (sub.chemical >= 0) n_sub = to_c<T>(chemical_n(sv.chemicals[sub.chemical], sv.tables, (double)wavelength));
     }
-    C<double> d_sub = snell_scalar(direction, n_ambient, n_sub);
-    C<double> one = {1.0, 0.0};
-    C<double> q_amb_s = cmul(direction, n_ambient);
-    C<double> q_amb_p = cmul(cconj(direction), cdiv(one, n_ambient));
-    C<double> q_sub_s = cmul(d_sub, n_sub);
-    C<double> q_sub_p = cmul(cconj(d_sub), cdiv(one, n_sub));
-    double ts = cabs(co.t[0]);
-    double tp = cabs(co.t[1]);
+    C<T> d_sub = snell_scalar(direction, n_ambient, n_sub);
+    C<T> one = {T(1), T(0)};
+    C<T> q_amb_s = cmul(direction, n_ambient);
+    C<T> q_amb_p = cmul(cconj(direction), cdiv(one, n_ambient));
+    C<T> q_sub_s = cmul(d_sub, n_sub);
+    C<T> q_sub_p = cmul(cconj(d_sub), cdiv(one, n_sub));
+    T ts = cabs(co.t[0]);
+    T tp = cabs(co.t[1]);
     out.T[0] = (ts * ts) * cdiv(q_sub_s, q_amb_s).re;
     out.T[1] = (tp * tp) * cdiv(q_sub_p, q_amb_p).re;
   }

@@ -450,7 +450,8 @@ OPTK_HD double material_efficiency(const optk_surface &s, const StackView &sv, d
     C<double> n = {index, k};
     double cosv = -dot3(a, normal);
     bool film = s.material_kind == OPTK_MATERIAL_MULTILAYER_FILM;
-    MultilayerRT rt = multilayer_efficiency(sv, s.stack, wavelength, C<double>{cosv, 0.0}, n, film);
+    MultilayerRT<double> rt =
+        multilayer_efficiency<double>(sv, s.stack, wavelength, C<double>{cosv, 0.0}, n, film);
     return film ? (rt.T[0] + rt.T[1]) / 2.0 : (rt.R[0] + rt.R[1]) / 2.0;
   }
   default:

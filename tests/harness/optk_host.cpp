@@ -70,7 +70,7 @@ int host_multilayer(const optk_system_desc *d, int stack, const double *wl, cons
                     double *Rs, double *Rp, double *Ts, double *Tp) {
   SystemView v = view_of(d);
   for (int64_t i = 0; i < n; ++i) {
-    MultilayerRT rt = multilayer_efficiency(v.sv, stack, wl[i], C<double>{cos_re[i], cos_im[i]},
+    MultilayerRT<double> rt = multilayer_efficiency<double>(v.sv, stack, wl[i], C<double>{cos_re[i], cos_im[i]},
                                             C<double>{n_re[i], n_im[i]}, true);
     Rs[i] = rt.R[0];
     Rp[i] = rt.R[1];
