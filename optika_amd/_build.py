@@ -50,8 +50,17 @@ def _torch_lib() -> str:
     return str(pathlib.Path(spec.origin).parent / "lib")
 
 
+# per-source flags: the float64 trace keeps the reference's rounding sequence
+# (no FMA contraction); the float32 form contracts and uses the hardware's
+# float32 division and square root
+_SOURCES = {
+    "optk_lib.hip": ["-ffp-contract=off"],
+    "optk_fast.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+}
+
+
 def sources() -> list[pathlib.Path]:
-    return [CSRC / "optk_lib.hip"]
+    return [CSRC / name for name in _SOURCES]
 
 
 def build(verbose: bool = True) -> pathlib.Path:
@@ -60,16 +69,18 @@ def build(verbose: bool = True) -> pathlib.Path:
     objs = []
     tmpdir = HERE / "_build"
     tmpdir.mkdir(exist_ok=True)
+    procs = []
     for src in sources():
         obj = tmpdir / (src.stem + ".o")
-        # the float64 trace keeps the reference's rounding sequence: no FMA
-        flags = ["-ffp-contract=off"]
-        cmd = [_hipcc(), *COMMON, *flags, "-c", str(src), "-o", str(obj)]
+        cmd = [_hipcc(), *COMMON, *_SOURCES[src.name], "-c", str(src), "-o", str(obj)]
         cmd = [c for c in cmd if c != "-shared"]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
         objs.append(str(obj))
+    for cmd, proc in procs:
+        if proc.wait() != 0:
+            raise subprocess.CalledProcessError(proc.returncode, cmd)
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp),
            f"-L{tl}", f"-Wl,-rpath,{tl}"]

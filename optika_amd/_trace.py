@@ -146,10 +146,7 @@ def trace_sequence(surfaces: Sequence, rays: RayVectorArray, accumulate: bool = 
                    precision: str = "f64") -> RayVectorArray:
     """``propagate_rays`` / ``accumulate_rays`` over ``surfaces``."""
     prec = _precision(precision)
-    if prec != _lib.F64:
-        from . import _trace32
-
-        return _trace32.trace_sequence_f32(surfaces, rays, accumulate)
+    dtype = torch.float64 if prec == _lib.F64 else torch.float32
     dev = device()
     cs = compiled(surfaces)
     n = rays.num
@@ -158,22 +155,22 @@ def trace_sequence(surfaces: Sequence, rays: RayVectorArray, accumulate: bool = 
     keep = []
     rin = _lib.Rays()
     for k, c in enumerate(rays.position):
-        t = _as_tensor(c, n, dev)
+        t = _as_tensor(c, n, dev, dtype)
         if t is None:
-            t = torch.full((n,), _scalar(c), dtype=torch.float64, device=dev)
+            t = torch.full((n,), _scalar(c), dtype=dtype, device=dev)
         keep.append(t)
         rin.position[k] = t.data_ptr()
     for k, c in enumerate(rays.direction):
-        t = _as_tensor(c, n, dev)
+        t = _as_tensor(c, n, dev, dtype)
         if t is None:
-            t = torch.full((n,), _scalar(c), dtype=torch.float64, device=dev)
+            t = torch.full((n,), _scalar(c), dtype=dtype, device=dev)
         keep.append(t)
         rin.direction[k] = t.data_ptr()
     scalars_in = {}
     for f, dflt in (("wavelength", "wavelength_default"), ("intensity", "intensity_default"),
                     ("attenuation", "attenuation_default"), ("index_refraction", "index_default")):
         v = getattr(rays, f)
-        t = _as_tensor(v, n, dev)
+        t = _as_tensor(v, n, dev, dtype)
         if t is None:
             setattr(rin, dflt, _scalar(v))
             scalars_in[f] = _scalar(v)
@@ -189,10 +186,10 @@ def trace_sequence(surfaces: Sequence, rays: RayVectorArray, accumulate: bool = 
         rin.unvignetted = tu.data_ptr()
 
     fields = {"wavelength", "intensity", "attenuation", "index_refraction"}
-    out = _Out(P * n, dev, torch.float64, fields)
-    acc = _Out(cs.num_surfaces * P * n, dev, torch.float64, fields) if accumulate else None
+    out = _Out(P * n, dev, dtype, fields)
+    acc = _Out(cs.num_surfaces * P * n, dev, dtype, fields) if accumulate else None
     L = _lib.lib()
-    _lib.check(L.optk_trace(cs.handle, _lib.F64, ctypes.byref(rin), ctypes.byref(out.rec),
+    _lib.check(L.optk_trace(cs.handle, prec, ctypes.byref(rin), ctypes.byref(out.rec),
                             ctypes.byref(acc.rec) if acc else None, n, 0, stream_handle()))
     if accumulate:
         return _make_rays(acc, _shape(P, n, cs.num_surfaces), {})
@@ -222,10 +219,7 @@ def trace_system(system, grid: ObjectVectorArray, intensity=None, accumulate: bo
     surfaces = system.surfaces_all
     obj_t = system.object.transformation if system.object is not None else None
     post = system.sensor.transformation if (sensor_frame and system.sensor is not None) else None
-    if prec != _lib.F64:
-        from . import _trace32
-
-        return _trace32.trace_system_f32(system, grid, intensity, accumulate, sensor_frame)
+    dtype = torch.float64 if prec == _lib.F64 else torch.float32
     dev = device()
     cs = compiled(surfaces, pre=(obj_t, system.transformation), pre_is_inverse=(0, 1), post=post)
     vals, n = _grid_fields(grid, intensity)
@@ -235,7 +229,7 @@ def trace_system(system, grid: ObjectVectorArray, intensity=None, accumulate: bo
     keep = []
 
     def put(name, arr_field, idx, dflt_field):
-        t = _as_tensor(vals[name], n, dev)
+        t = _as_tensor(vals[name], n, dev, dtype)
         if t is None:
             if idx is None:
                 setattr(g, dflt_field, _scalar(vals[name]))
@@ -260,13 +254,13 @@ def trace_system(system, grid: ObjectVectorArray, intensity=None, accumulate: bo
     invariant = _compile.invariant_fields(surfaces)
     fields = {"intensity"} if invariant else {"wavelength", "intensity", "attenuation",
                                                "index_refraction"}
-    out = _Out(P * n, dev, torch.float64, fields)
-    acc = (_Out(cs.num_surfaces * P * n, dev, torch.float64,
+    out = _Out(P * n, dev, dtype, fields)
+    acc = (_Out(cs.num_surfaces * P * n, dev, dtype,
                 {"wavelength", "intensity", "attenuation", "index_refraction"})
            if accumulate else None)
     flags = _lib.TRACE_APPLY_POST if post is not None else 0
     L = _lib.lib()
-    _lib.check(L.optk_trace_grid(cs.handle, _lib.F64, ctypes.byref(g), ctypes.byref(out.rec),
+    _lib.check(L.optk_trace_grid(cs.handle, prec, ctypes.byref(g), ctypes.byref(out.rec),
                                  ctypes.byref(acc.rec) if acc else None, n, flags,
                                  stream_handle()))
     if accumulate:
@@ -289,8 +283,8 @@ class GridTracePlan:
     """
 
     def __init__(self, system, n: int, sensor_frame: bool = True, precision: str = "f64"):
-        if _precision(precision) != _lib.F64:
-            raise ValueError("GridTracePlan traces in float64; use the f32 plan for float32")
+        self.prec = _precision(precision)
+        self.dtype = torch.float64 if self.prec == _lib.F64 else torch.float32
         self.dev = device()
         surfaces = system.surfaces_all
         obj_t = system.object.transformation if system.object is not None else None
@@ -303,13 +297,18 @@ class GridTracePlan:
         self.invariant = _compile.invariant_fields(surfaces)
         fields = {"intensity"} if self.invariant else {
             "wavelength", "intensity", "attenuation", "index_refraction"}
-        self.out = _Out(self.paths * self.n, self.dev, torch.float64, fields)
+        self.out = _Out(self.paths * self.n, self.dev, self.dtype, fields)
         self.grid = _lib.Grid()
         self.grid.object_at_infinity = 1 if system.object_is_at_infinity else 0
         self.grid.intensity_default = 1.0
         self.num_surfaces = self.cs.num_surfaces
 
     def __call__(self, field_x, field_y, pupil_x, pupil_y, wavelength, stream=None):
+        """Trace one batch; the five inputs are contiguous device tensors of
+        the plan's dtype and length."""
+        for t in (field_x, field_y, pupil_x, pupil_y, wavelength):
+            if t.dtype != self.dtype or t.numel() != self.n or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError(f"inputs must be contiguous {self.dtype} device tensors of length {self.n}")
         g = self.grid
         for k, t in enumerate((field_x, field_y)):
             g.field[k] = t.data_ptr()
@@ -317,7 +316,7 @@ class GridTracePlan:
             g.pupil[k] = t.data_ptr()
         g.wavelength = wavelength.data_ptr()
         s = stream_handle() if stream is None else ctypes.c_void_p(stream)
-        _lib.check(_lib.lib().optk_trace_grid(self.cs.handle, _lib.F64, ctypes.byref(g),
+        _lib.check(_lib.lib().optk_trace_grid(self.cs.handle, self.prec, ctypes.byref(g),
                                               ctypes.byref(self.out.rec), None, self.n,
                                               self.flags, s))
         scalars = {"wavelength": wavelength if self.paths == 1 else wavelength.expand(self.paths, self.n),

@@ -1,0 +1,213 @@
+// optk_fast.hip -- gfx950 kernels of the float32 form of the trace
+// (OPTK_F32; physics in optk_fast.h).  Compiled with -ffp-contract=fast:
+// unlike the float64 form, this form does not reproduce numpy's rounding
+// sequence, so multiply-adds may fuse.
+
+#include "optk_internal.h"
+
+using namespace optk;
+
+namespace {
+
+struct FastArgs {
+  const FastSpec *spec;
+  const FastSurface *fast;
+  const optk_surface *surfaces;
+  StackView sv;
+  const float *tables32;
+  int32_t num_surfaces;
+  int32_t num_paths;
+  int32_t entry;       // 0: rays in the system frame, 1: grid (pre-maps)
+  int32_t apply_post;  // 1: leave in the post (sensor) frame
+  int64_t n;
+  FRaysOut out;
+};
+
+static_assert(sizeof(FastSurface) % 8 == 0 && sizeof(FastSpec) % 8 == 0 &&
+                  sizeof(optk_surface) % 8 == 0,
+              "records are staged as words");
+
+__device__ __forceinline__ float ldf(const float *p, int64_t i, float dflt) {
+  return p ? p[i] : dflt;
+}
+
+__device__ __forceinline__ FRay load_fray(const FRaysIn &in, int64_t i) {
+  FRay r;
+  r.px = in.p[0][i];
+  r.py = in.p[1][i];
+  r.pz = in.p[2][i];
+  r.dx = in.d[0][i];
+  r.dy = in.d[1][i];
+  r.dz = in.d[2][i];
+  r.wavelength = ldf(in.wavelength, i, in.wavelength_default);
+  r.intensity = ldf(in.intensity, i, in.intensity_default);
+  r.attenuation = ldf(in.attenuation, i, in.attenuation_default);
+  r.index = ldf(in.index, i, in.index_default);
+  r.unvignetted = in.unvignetted ? (in.unvignetted[i] != 0) : (in.unvignetted_default != 0);
+  return r;
+}
+
+__device__ __forceinline__ FRay load_fray(const FGridIn &g, int64_t i) {
+  return fast_grid_ray(ldf(g.field[0], i, g.field_default[0]), ldf(g.field[1], i, g.field_default[1]),
+                       ldf(g.pupil[0], i, g.pupil_default[0]), ldf(g.pupil[1], i, g.pupil_default[1]),
+                       ldf(g.wavelength, i, g.wavelength_default),
+                       ldf(g.intensity, i, g.intensity_default), g.object_at_infinity);
+}
+
+__device__ __forceinline__ void store_f(const FRaysOut &o, int64_t j, const FRay &r) {
+  if (o.p[0]) o.p[0][j] = (float)r.px;
+  if (o.p[1]) o.p[1][j] = (float)r.py;
+  if (o.p[2]) o.p[2][j] = (float)r.pz;
+  if (o.d[0]) o.d[0][j] = (float)r.dx;
+  if (o.d[1]) o.d[1][j] = (float)r.dy;
+  if (o.d[2]) o.d[2][j] = (float)r.dz;
+  if (o.wavelength) o.wavelength[j] = r.wavelength;
+  if (o.intensity) o.intensity[j] = r.intensity;
+  if (o.attenuation) o.attenuation[j] = r.attenuation;
+  if (o.index) o.index[j] = r.index;
+  if (o.unvignetted) o.unvignetted[j] = r.unvignetted ? 1 : 0;
+}
+
+size_t fast_staged_bytes(int num_surfaces) {
+  return sizeof(FastSpec) + (sizeof(FastSurface) + sizeof(optk_surface)) * (size_t)num_surfaces;
+}
+
+template <int F> struct FastWaves {
+  static constexpr int value = (F & FEAT_MULTILAYER) ? 2 : 4;
+};
+
+template <int F, bool ACCUM, typename In>
+__device__ __forceinline__ void fast_body(const FastArgs &A, const In &in, const FRaysOut &accum) {
+  extern __shared__ double staged[];
+  FastSpec *s_spec = reinterpret_cast<FastSpec *>(staged);
+  FastSurface *s_fast = reinterpret_cast<FastSurface *>(s_spec + 1);
+  optk_surface *s_surf = reinterpret_cast<optk_surface *>(s_fast + A.num_surfaces);
+  {
+    auto copy = [](uint64_t *dst, const uint64_t *src, int words) {
+      for (int w = threadIdx.x; w < words; w += blockDim.x) dst[w] = src[w];
+    };
+    copy(reinterpret_cast<uint64_t *>(s_spec), reinterpret_cast<const uint64_t *>(A.spec),
+         (int)(sizeof(FastSpec) / 8));
+    copy(reinterpret_cast<uint64_t *>(s_fast), reinterpret_cast<const uint64_t *>(A.fast),
+         A.num_surfaces * (int)(sizeof(FastSurface) / 8));
+    copy(reinterpret_cast<uint64_t *>(s_surf), reinterpret_cast<const uint64_t *>(A.surfaces),
+         A.num_surfaces * (int)(sizeof(optk_surface) / 8));
+  }
+  __syncthreads();
+  FastView v;
+  v.surfaces = s_surf;
+  v.fast = s_fast;
+  v.sv = A.sv;
+  v.tables32 = A.tables32;
+  const int num_paths = (F & FEAT_RULINGS) ? A.num_paths : 1;
+  const int num_surfaces = A.num_surfaces;
+  const int64_t n = A.n;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    FRay r0 = load_fray(in, i);
+    const optk_affine &first = s_spec->first_hop[A.entry];
+    xf_apply_pos(first, r0.px, r0.py, r0.pz);
+    xf_apply_dir(first, r0.dx, r0.dy, r0.dz);
+    for (int path = 0; path < num_paths; ++path) {
+      FRay r = r0;
+      for (int k = 0; k < num_surfaces; ++k) {
+        if (k > 0) {
+          xf_apply_pos(s_fast[k].hop, r.px, r.py, r.pz);
+          xf_apply_dir(s_fast[k].hop, r.dx, r.dy, r.dz);
+        }
+        fast_propagate<F>(s_fast[k], s_surf[k], v, path, r);
+        if (ACCUM) {
+          FRay g = r;
+          xf_apply_pos(s_fast[k].to_global, g.px, g.py, g.pz);
+          xf_apply_dir(s_fast[k].to_global, g.dx, g.dy, g.dz);
+          store_f(accum, ((int64_t)k * num_paths + path) * n + i, g);
+        }
+      }
+      const optk_affine &last = s_spec->last[A.apply_post];
+      xf_apply_pos(last, r.px, r.py, r.pz);
+      xf_apply_dir(last, r.dx, r.dy, r.dz);
+      store_f(A.out, (int64_t)path * n + i, r);
+    }
+  }
+}
+
+template <int F, bool ACCUM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
+void trace_rays_f32(const FastArgs A, const FRaysIn in, const FRaysOut accum) {
+  fast_body<F, ACCUM>(A, in, accum);
+}
+
+template <int F, bool ACCUM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
+void trace_grid_f32(const FastArgs A, const FGridIn g, const FRaysOut accum) {
+  fast_body<F, ACCUM>(A, g, accum);
+}
+
+FastArgs make_args(const optk_system *sys, const FRaysOut &out, int64_t n, int32_t entry,
+                   int32_t apply_post) {
+  FastArgs A;
+  memset(&A, 0, sizeof(A));
+  A.spec = sys->fast_spec;
+  A.fast = sys->fast;
+  A.surfaces = sys->surfaces;
+  A.sv.layers = sys->layers;
+  A.sv.stacks = sys->stacks;
+  A.sv.chemicals = sys->chemicals;
+  A.sv.tables = sys->tables;
+  A.tables32 = sys->tables32;
+  A.num_surfaces = sys->num_surfaces;
+  A.num_paths = sys->num_paths;
+  A.entry = entry;
+  A.apply_post = apply_post ? 1 : 0;
+  A.n = n;
+  A.out = out;
+  return A;
+}
+
+template <typename In>
+int32_t launch(void (*k0)(FastArgs, In, FRaysOut), void (*k3)(FastArgs, In, FRaysOut),
+               void (*k7)(FastArgs, In, FRaysOut), void (*k7a)(FastArgs, In, FRaysOut),
+               const optk_system *sys, const In &in, const FRaysOut &out, const FRaysOut *accum,
+               int64_t n, int32_t entry, int32_t apply_post, hipStream_t stream) {
+  if (n == 0) return OPTK_OK;
+  int dev = 0, lds_max = 0;
+  hipError_t e0 = hipGetDevice(&dev);
+  if (e0 == hipSuccess) e0 = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (e0 != hipSuccess) return fail(OPTK_EDEVICE, std::string("device query: ") + hipGetErrorString(e0));
+  if (fast_staged_bytes(sys->num_surfaces) > (size_t)lds_max)
+    return fail(OPTK_EINVAL, "too many surfaces for the float32 trace's LDS staging");
+  FastArgs A = make_args(sys, out, n, entry, apply_post);
+  FRaysOut acc;
+  memset(&acc, 0, sizeof(acc));
+  if (accum) acc = *accum;
+  const int f = sys->features;
+  auto kern = accum ? k7a : (f == 0 ? k0 : (f == (FEAT_TOROID | FEAT_RULINGS) ? k3 : k7));
+  int64_t blocks = (n + 255) / 256;
+  const int64_t max_blocks = 256 * 16;
+  if (blocks > max_blocks) blocks = max_blocks;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), fast_staged_bytes(sys->num_surfaces),
+                     stream, A, in, acc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("float32 trace launch: ") + hipGetErrorString(e));
+  return OPTK_OK;
+}
+
+}  // namespace
+
+namespace optk {
+
+int32_t fast_trace_rays(const optk_system *sys, const FRaysIn &in, const FRaysOut &out,
+                        const FRaysOut *accum, int64_t n, int32_t apply_post, hipStream_t stream) {
+  return launch<FRaysIn>(trace_rays_f32<0, false>, trace_rays_f32<FEAT_TOROID | FEAT_RULINGS, false>,
+                         trace_rays_f32<FEAT_ALL, false>, trace_rays_f32<FEAT_ALL, true>, sys, in, out,
+                         accum, n, 0, apply_post, stream);
+}
+
+int32_t fast_trace_grid(const optk_system *sys, const FGridIn &in, const FRaysOut &out,
+                        const FRaysOut *accum, int64_t n, int32_t apply_post, hipStream_t stream) {
+  return launch<FGridIn>(trace_grid_f32<0, false>, trace_grid_f32<FEAT_TOROID | FEAT_RULINGS, false>,
+                         trace_grid_f32<FEAT_ALL, false>, trace_grid_f32<FEAT_ALL, true>, sys, in, out,
+                         accum, n, 1, apply_post, stream);
+}
+
+}  // namespace optk

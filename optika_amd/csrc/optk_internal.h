@@ -1,0 +1,74 @@
+// optk_internal.h -- state shared by the translation units of liboptk.so
+// (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "optk_fast.h"
+
+// A compiled system: the optk_surface records and their tables on the
+// device, plus the derived records of the float32 form.
+struct optk_system {
+  optk_surface *surfaces = nullptr;
+  optk_layer *layers = nullptr;
+  optk_stack *stacks = nullptr;
+  optk_chemical *chemicals = nullptr;
+  double *tables = nullptr;
+  int32_t num_surfaces = 0;
+  int32_t num_paths = 1;
+  int32_t num_layers = 0;
+  int32_t num_stacks = 0;
+  int32_t num_chemicals = 0;
+  int64_t num_tables = 0;
+  optk_affine pre[2];
+  int32_t pre_is_inverse[2];
+  optk_affine post;
+  int32_t features = 0;                  // FEAT_* the surfaces need
+  optk::TraceSpec *spec = nullptr;       // device copy of views + transformations
+  std::vector<optk_surface> host_surfaces;
+  // float32 form
+  optk::FastSurface *fast = nullptr;     // one per surface
+  optk::FastSpec *fast_spec = nullptr;
+  float *tables32 = nullptr;             // tables[] rounded to float32
+};
+
+namespace optk {
+
+// record the thread's last error message and return `code`
+int32_t fail(int32_t code, const std::string &msg);
+
+// float32 ray buffers (device pointers; null = use the default)
+struct FRaysIn {
+  const float *p[3];
+  const float *d[3];
+  const float *wavelength, *intensity, *attenuation, *index;
+  const uint8_t *unvignetted;
+  float wavelength_default, intensity_default, attenuation_default, index_default;
+  int32_t unvignetted_default;
+};
+
+struct FRaysOut {
+  float *p[3];
+  float *d[3];
+  float *wavelength, *intensity, *attenuation, *index;
+  uint8_t *unvignetted;
+};
+
+struct FGridIn {
+  const float *field[2];
+  const float *pupil[2];
+  const float *wavelength, *intensity;
+  float field_default[2], pupil_default[2], wavelength_default, intensity_default;
+  int32_t object_at_infinity;
+};
+
+// float32-form launches (optk_fast.hip); `accum` may be null
+int32_t fast_trace_rays(const optk_system *sys, const FRaysIn &in, const FRaysOut &out,
+                        const FRaysOut *accum, int64_t n, int32_t apply_post, hipStream_t stream);
+int32_t fast_trace_grid(const optk_system *sys, const FGridIn &in, const FRaysOut &out,
+                        const FRaysOut *accum, int64_t n, int32_t apply_post, hipStream_t stream);
+
+}  // namespace optk

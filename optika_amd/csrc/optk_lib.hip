@@ -11,7 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "optk_surface.h"
+#include "optk_internal.h"
 
 using namespace optk;
 
@@ -19,10 +19,14 @@ namespace {
 
 thread_local std::string g_last_error;
 
+}  // namespace
+
+namespace optk {
 int32_t fail(int32_t code, const std::string &msg) {
   g_last_error = msg;
   return code;
 }
+}  // namespace optk
 
 #define OPTK_HIP_CHECK(expr)                                                         \
   do {                                                                               \
@@ -30,28 +34,6 @@ int32_t fail(int32_t code, const std::string &msg) {
     if (e_ != hipSuccess)                                                            \
       return fail(OPTK_EDEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
   } while (0)
-
-}  // namespace
-
-struct optk_system {
-  optk_surface *surfaces = nullptr;
-  optk_layer *layers = nullptr;
-  optk_stack *stacks = nullptr;
-  optk_chemical *chemicals = nullptr;
-  double *tables = nullptr;
-  int32_t num_surfaces = 0;
-  int32_t num_paths = 1;
-  int32_t num_layers = 0;
-  int32_t num_stacks = 0;
-  int32_t num_chemicals = 0;
-  int64_t num_tables = 0;
-  optk_affine pre[2];
-  int32_t pre_is_inverse[2];
-  optk_affine post;
-  int32_t features = 0;  // FEAT_* the surfaces need
-  TraceSpec *spec = nullptr;  // device copy of views + transformations
-  std::vector<optk_surface> host_surfaces;
-};
 
 static int32_t features_of(const optk_system_desc *d) {
   int32_t f = 0;
@@ -553,6 +535,9 @@ int32_t optk_system_destroy(optk_system *sys) {
   (void)hipFree(sys->chemicals);
   (void)hipFree(sys->tables);
   (void)hipFree(sys->spec);
+  (void)hipFree(sys->fast);
+  (void)hipFree(sys->fast_spec);
+  (void)hipFree(sys->tables32);
   delete sys;
   return OPTK_OK;
 }
@@ -602,6 +587,19 @@ int32_t optk_system_create(const optk_system_desc *desc, optk_system **out) {
     optk_system_destroy(s);
     return rc;
   }
+  // float32 form: composed transformations and float32 tables
+  std::vector<FastSurface> fast(desc->num_surfaces);
+  FastSpec fast_spec;
+  std::memset(&fast_spec, 0, sizeof(fast_spec));
+  fast_build(s->host_surfaces.data(), desc->num_surfaces, desc->pre, desc->pre_is_inverse,
+             desc->post, fast.data(), fast_spec);
+  std::vector<float> tables32(desc->tables, desc->tables + desc->num_tables);
+  if ((rc = upload(&s->fast, fast.data(), desc->num_surfaces)) ||
+      (rc = upload(&s->fast_spec, &fast_spec, 1)) ||
+      (rc = upload(&s->tables32, tables32.data(), desc->num_tables))) {
+    optk_system_destroy(s);
+    return rc;
+  }
   *out = s;
   return OPTK_OK;
 }
@@ -620,13 +618,27 @@ static void fill_out(RaysOut &o, const optk_rays *r) {
   o.unvignetted = r->unvignetted;
 }
 
+static void fill_out32(FRaysOut &o, const optk_rays *r) {
+  std::memset(&o, 0, sizeof(o));
+  if (!r) return;
+  for (int k = 0; k < 3; ++k) {
+    o.p[k] = (float *)r->position[k];
+    o.d[k] = (float *)r->direction[k];
+  }
+  o.wavelength = (float *)r->wavelength;
+  o.intensity = (float *)r->intensity;
+  o.attenuation = (float *)r->attenuation;
+  o.index = (float *)r->index_refraction;
+  o.unvignetted = r->unvignetted;
+}
+
 static int32_t prepare(const optk_system *sys, int32_t precision, const optk_rays *out,
                        const optk_rays *accum, int64_t n, int32_t flags, TraceArgs &A,
                        RaysOut &acc) {
   if (!sys) return fail(OPTK_EINVAL, "null system");
   if (!out) return fail(OPTK_EINVAL, "null output rays");
   if (n < 0) return fail(OPTK_EINVAL, "negative ray count");
-  if (precision != OPTK_F64) return fail(OPTK_EINVAL, "this entry point supports OPTK_F64 only");
+  if (precision != OPTK_F64 && precision != OPTK_F32) return fail(OPTK_EINVAL, "bad precision");
   std::memset(&A, 0, sizeof(A));
   A.spec = sys->spec;
   A.surfaces = sys->surfaces;
@@ -649,6 +661,28 @@ int32_t optk_trace(const optk_system *sys, int32_t precision, const optk_rays *i
   for (int k = 0; k < 3; ++k)
     if (!in->position[k] || !in->direction[k]) return fail(OPTK_EINVAL, "input position and direction are required");
   if (n == 0) return OPTK_OK;
+  if (precision == OPTK_F32) {
+    FRaysIn fi;
+    for (int k = 0; k < 3; ++k) {
+      fi.p[k] = (const float *)in->position[k];
+      fi.d[k] = (const float *)in->direction[k];
+    }
+    fi.wavelength = (const float *)in->wavelength;
+    fi.intensity = (const float *)in->intensity;
+    fi.attenuation = (const float *)in->attenuation;
+    fi.index = (const float *)in->index_refraction;
+    fi.unvignetted = in->unvignetted;
+    fi.wavelength_default = (float)in->wavelength_default;
+    fi.intensity_default = (float)in->intensity_default;
+    fi.attenuation_default = (float)in->attenuation_default;
+    fi.index_default = (float)in->index_default;
+    fi.unvignetted_default = in->unvignetted_default;
+    FRaysOut fo, fa;
+    fill_out32(fo, out);
+    fill_out32(fa, accum);
+    return fast_trace_rays(sys, fi, fo, accum ? &fa : nullptr, n, A.apply_post,
+                           (hipStream_t)hip_stream);
+  }
   RaysIn ri;
   for (int k = 0; k < 3; ++k) {
     ri.p[k] = (const double *)in->position[k];
@@ -680,6 +714,25 @@ int32_t optk_trace_grid(const optk_system *sys, int32_t precision, const optk_gr
   if (rc) return rc;
   if (!in) return fail(OPTK_EINVAL, "null input grid");
   if (n == 0) return OPTK_OK;
+  if (precision == OPTK_F32) {
+    FGridIn fg;
+    for (int k = 0; k < 2; ++k) {
+      fg.field[k] = (const float *)in->field[k];
+      fg.pupil[k] = (const float *)in->pupil[k];
+      fg.field_default[k] = (float)in->field_default[k];
+      fg.pupil_default[k] = (float)in->pupil_default[k];
+    }
+    fg.wavelength = (const float *)in->wavelength;
+    fg.intensity = (const float *)in->intensity;
+    fg.wavelength_default = (float)in->wavelength_default;
+    fg.intensity_default = (float)in->intensity_default;
+    fg.object_at_infinity = in->object_at_infinity;
+    FRaysOut fo, fa;
+    fill_out32(fo, out);
+    fill_out32(fa, accum);
+    return fast_trace_grid(sys, fg, fo, accum ? &fa : nullptr, n, A.apply_post,
+                           (hipStream_t)hip_stream);
+  }
   GridIn g;
   for (int k = 0; k < 2; ++k) {
     g.field[k] = (const double *)in->field[k];

@@ -24,6 +24,7 @@ _lib = None
 
 def build() -> pathlib.Path:
     srcs = [HERE / "optk_host.cpp", ROOT / "optika_amd/csrc/optk_surface.h",
+            ROOT / "optika_amd/csrc/optk_fast.h",
             ROOT / "optika_amd/csrc/optk_physics.h", ROOT / "include/optk.h"]
     if LIB.exists() and LIB.stat().st_mtime > max(s.stat().st_mtime for s in srcs):
         return LIB
@@ -93,6 +94,74 @@ def trace_grid(system, grid, sensor_frame: bool = True):
                       ctypes.c_int(1 if system.object_is_at_infinity else 0),
                       ctypes.c_int(1 if post is not None else 0), arr,
                       mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    res = []
+    for p in range(P):
+        s = slice(p * n, (p + 1) * n)
+        res.append(dict(
+            position=np.stack([outs[0][s], outs[1][s], outs[2][s]]),
+            direction=np.stack([outs[3][s], outs[4][s], outs[5][s]]),
+            wavelength=outs[6][s], intensity=outs[7][s], attenuation=outs[8][s],
+            index_refraction=outs[9][s], unvignetted=mask[s].astype(bool)))
+    return res
+
+
+def trace_grid_f32(system, grid, sensor_frame: bool = True):
+    """Host run of the float32 form (float32 inputs: the grid is rounded to
+    float32 first).  Same result layout as :func:`trace_grid`."""
+    from optika_amd import _compile
+
+    obj_t = system.object.transformation if system.object is not None else None
+    post = system.sensor.transformation if sensor_frame else None
+    rec = _compile.build_records(system.surfaces_all, pre=(obj_t, system.transformation),
+                                 pre_is_inverse=(0, 1), post=post)
+    d, tab = _desc(rec)
+    n = grid.num
+    P = rec.num_paths
+    fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
+    ins = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float32), (n,)))
+           for v in (grid.field_x, grid.field_y, grid.pupil_x, grid.pupil_y, grid.wavelength)]
+    inten = np.ones(n, np.float32)
+    outs = [np.empty(P * n, np.float32) for _ in range(10)]
+    mask = np.empty(P * n, dtype=np.uint8)
+    arr = (ctypes.POINTER(ctypes.c_float) * 10)(*[fp(o) for o in outs])
+    L = lib()
+    L.host_trace_grid_f32.restype = ctypes.c_int
+    L.host_trace_grid_f32(ctypes.byref(d), *[fp(a) for a in ins], fp(inten), ctypes.c_int64(n),
+                          ctypes.c_int(1 if system.object_is_at_infinity else 0),
+                          ctypes.c_int(1 if post is not None else 0), arr,
+                          mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    res = []
+    for p in range(P):
+        s = slice(p * n, (p + 1) * n)
+        res.append(dict(
+            position=np.stack([outs[0][s], outs[1][s], outs[2][s]]),
+            direction=np.stack([outs[3][s], outs[4][s], outs[5][s]]),
+            wavelength=outs[6][s], intensity=outs[7][s], attenuation=outs[8][s],
+            index_refraction=outs[9][s], unvignetted=mask[s].astype(bool)))
+    return res
+
+
+def trace_rays_f32(surfaces, rays: dict):
+    """Host run of the float32 form over a surface sequence; `rays` is an
+    oracle-style ray dict (rounded to float32 on the way in)."""
+    from optika_amd import _compile
+
+    rec = _compile.build_records(list(surfaces))
+    d, tab = _desc(rec)
+    n = np.asarray(rays["wavelength"]).size
+    P = rec.num_paths
+    fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
+    b = lambda v: np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float32), (n,)))  # noqa: E731
+    ins = [b(c) for c in rays["position"]] + [b(c) for c in rays["direction"]] + [
+        b(rays[k]) for k in ("wavelength", "intensity", "attenuation", "index_refraction")]
+    outs = [np.empty(P * n, np.float32) for _ in range(10)]
+    mask = np.empty(P * n, dtype=np.uint8)
+    inp = (ctypes.POINTER(ctypes.c_float) * 10)(*[fp(a) for a in ins])
+    arr = (ctypes.POINTER(ctypes.c_float) * 10)(*[fp(o) for o in outs])
+    L = lib()
+    L.host_trace_rays_f32.restype = ctypes.c_int
+    L.host_trace_rays_f32(ctypes.byref(d), inp, ctypes.c_int64(n), arr,
+                          mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
     res = []
     for p in range(P):
         s = slice(p * n, (p + 1) * n)

@@ -5,7 +5,7 @@
 #include <cstdint>
 #include <vector>
 
-#include "../../optika_amd/csrc/optk_surface.h"
+#include "../../optika_amd/csrc/optk_fast.h"
 
 using namespace optk;
 
@@ -58,6 +58,99 @@ int host_trace_grid(const optk_system_desc *d, const double *fx, const double *f
       double v[10] = {r.px, r.py, r.pz, r.dx, r.dy, r.dz, r.wavelength, r.intensity,
                       r.attenuation, r.index};
       for (int c = 0; c < 10; ++c) out[c][j] = v[c];
+      mask[j] = r.unvignetted ? 1 : 0;
+    }
+  }
+  return 0;
+}
+
+// The float32 form (optk_fast.h) on the host: float32 grid in, float32 rays
+// out (same layout as host_trace_grid).
+int host_trace_grid_f32(const optk_system_desc *d, const float *fx, const float *fy,
+                        const float *ux, const float *uy, const float *wl, const float *inten,
+                        int64_t n, int object_at_infinity, int post, float **out,
+                        uint8_t *mask) {
+  std::vector<optk_surface> surf = derived(d);
+  std::vector<FastSurface> fast(d->num_surfaces);
+  FastSpec spec;
+  fast_build(surf.data(), d->num_surfaces, d->pre, d->pre_is_inverse, d->post, fast.data(), spec);
+  std::vector<float> tables32(d->tables, d->tables + d->num_tables);
+  FastView v;
+  v.surfaces = surf.data();
+  v.fast = fast.data();
+  v.sv = view_of(d).sv;
+  v.tables32 = tables32.data();
+  for (int64_t i = 0; i < n; ++i) {
+    FRay r0 = fast_grid_ray(fx[i], fy[i], ux[i], uy[i], wl[i], inten[i], object_at_infinity);
+    xf_apply_pos(spec.first_hop[1], r0.px, r0.py, r0.pz);
+    xf_apply_dir(spec.first_hop[1], r0.dx, r0.dy, r0.dz);
+    for (int path = 0; path < d->num_paths; ++path) {
+      FRay r = r0;
+      for (int k = 0; k < d->num_surfaces; ++k) {
+        if (k > 0) {
+          xf_apply_pos(fast[k].hop, r.px, r.py, r.pz);
+          xf_apply_dir(fast[k].hop, r.dx, r.dy, r.dz);
+        }
+        fast_propagate<FEAT_ALL>(fast[k], surf[k], v, path, r);
+      }
+      const optk_affine &last = spec.last[post ? 1 : 0];
+      xf_apply_pos(last, r.px, r.py, r.pz);
+      xf_apply_dir(last, r.dx, r.dy, r.dz);
+      int64_t j = (int64_t)path * n + i;
+      float vals[10] = {(float)r.px, (float)r.py, (float)r.pz, (float)r.dx, (float)r.dy,
+                        (float)r.dz, r.wavelength, r.intensity, r.attenuation, r.index};
+      for (int c = 0; c < 10; ++c) out[c][j] = vals[c];
+      mask[j] = r.unvignetted ? 1 : 0;
+    }
+  }
+  return 0;
+}
+
+// The float32 form over a surface sequence: float32 rays in (system frame),
+// float32 rays out.  in[0..9] / out[0..9] = px py pz dx dy dz wavelength
+// intensity attenuation index.
+int host_trace_rays_f32(const optk_system_desc *d, const float *const *in, int64_t n,
+                        float **out, uint8_t *mask) {
+  std::vector<optk_surface> surf = derived(d);
+  std::vector<FastSurface> fast(d->num_surfaces);
+  FastSpec spec;
+  fast_build(surf.data(), d->num_surfaces, d->pre, d->pre_is_inverse, d->post, fast.data(), spec);
+  std::vector<float> tables32(d->tables, d->tables + d->num_tables);
+  FastView v;
+  v.surfaces = surf.data();
+  v.fast = fast.data();
+  v.sv = view_of(d).sv;
+  v.tables32 = tables32.data();
+  for (int64_t i = 0; i < n; ++i) {
+    FRay r0;
+    r0.px = in[0][i];
+    r0.py = in[1][i];
+    r0.pz = in[2][i];
+    r0.dx = in[3][i];
+    r0.dy = in[4][i];
+    r0.dz = in[5][i];
+    r0.wavelength = in[6][i];
+    r0.intensity = in[7][i];
+    r0.attenuation = in[8][i];
+    r0.index = in[9][i];
+    r0.unvignetted = true;
+    xf_apply_pos(spec.first_hop[0], r0.px, r0.py, r0.pz);
+    xf_apply_dir(spec.first_hop[0], r0.dx, r0.dy, r0.dz);
+    for (int path = 0; path < d->num_paths; ++path) {
+      FRay r = r0;
+      for (int k = 0; k < d->num_surfaces; ++k) {
+        if (k > 0) {
+          xf_apply_pos(fast[k].hop, r.px, r.py, r.pz);
+          xf_apply_dir(fast[k].hop, r.dx, r.dy, r.dz);
+        }
+        fast_propagate<FEAT_ALL>(fast[k], surf[k], v, path, r);
+      }
+      xf_apply_pos(spec.last[0], r.px, r.py, r.pz);
+      xf_apply_dir(spec.last[0], r.dx, r.dy, r.dz);
+      int64_t j = (int64_t)path * n + i;
+      float vals[10] = {(float)r.px, (float)r.py, (float)r.pz, (float)r.dx, (float)r.dy,
+                        (float)r.dz, r.wavelength, r.intensity, r.attenuation, r.index};
+      for (int c = 0; c < 10; ++c) out[c][j] = vals[c];
       mask[j] = r.unvignetted ? 1 : 0;
     }
   }

@@ -1,0 +1,105 @@
+"""The float32 form's per-ray code (optika_amd/csrc/optk_fast.h), compiled for
+the host by tests/harness, against the numpy float64 oracle.
+
+Same bars as the GPU tests of the float32 form: RMS spot error <= 1e-3
+pixel, worst ray <= 1e-2 pixel, RMS direction error <= 1e-7, masks agree on
+>= 99.999 % of rays.  Inputs are rounded to float32 once and the oracle
+traces those same values in float64.
+"""
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle
+from harness import trace_grid, trace_grid_f32, trace_rays_f32
+from optika_amd import surfaces, workloads
+from optika_amd.transformations import Cartesian3dTranslation
+
+PIXEL = 0.015
+
+
+def _f32(g):
+    return dataclasses.replace(g, **{k: np.asarray(getattr(g, k), np.float32).astype(np.float64)
+                                     for k in ("field_x", "field_y", "pupil_x", "pupil_y",
+                                               "wavelength")})
+
+
+def _spot_stats(a, b):
+    both = a["unvignetted"] & b["unvignetted"]
+    agree = (a["unvignetted"] == b["unvignetted"]).mean()
+    d = (b["position"][:2, both].astype(np.float64) - a["position"][:2, both]) / PIXEL
+    err = np.sqrt((d * d).sum(0))
+    dd = b["direction"][:, both].astype(np.float64) - a["direction"][:, both]
+    derr = np.sqrt((dd * dd).sum(0))
+    return dict(rays=int(both.sum()), agree=agree, rms=np.sqrt((err ** 2).mean()),
+                max=err.max(), dir_rms=np.sqrt((derr ** 2).mean()))
+
+
+def _check(st, label):
+    print(label, st)
+    assert st["agree"] >= 0.99999
+    assert st["rms"] <= 1e-3
+    assert st["max"] <= 1e-2
+    assert st["dir_rms"] <= 1e-7
+
+
+def test_newtonian_f32_host():
+    g = _f32(workloads.newtonian_grid_random(100000, seed=1))
+    s = workloads.newtonian()
+    # the host float64 build equals the oracle bit-for-bit (test_harness_parity)
+    ref = trace_grid(s, g)[0]
+    got = trace_grid_f32(s, g)[0]
+    _check(_spot_stats(ref, got), "newtonian")
+
+
+def test_spectrograph_f32_host():
+    g = _f32(workloads.spectrograph_grid(20000, seed=2))
+    s = workloads.spectrograph()
+    ref = trace_grid(s, g)
+    got = trace_grid_f32(s, g)
+    st = _spot_stats(ref[0], got[0])
+    _check(st, "spectrograph order -1")
+    assert st["rays"] > 10000
+    # order +1 leaves the sensor: both forms vignette every ray
+    assert not ref[1]["unvignetted"].any() and not got[1]["unvignetted"].any()
+    i64, i32 = ref[0]["intensity"], got[0]["intensity"].astype(np.float64)
+    m = ref[0]["unvignetted"]
+    assert np.max(np.abs(i32[m] - i64[m]) / i64[m]) < 1e-4
+
+
+def _zoo():
+    from test_gpu_parity import ZOO, _zoo_rays
+
+    return ZOO, _zoo_rays
+
+
+@pytest.mark.parametrize("name", [
+    "conic_mirror_ellipse", "cylinder_vacuum_polygon", "parabola_measured_mirror_sector",
+    "rough_multilayer_mirror", "sawtooth_poly_grating", "sinusoidal_grating",
+    "sphere_glass_circle", "toroid_rotated_sag_octagon", "rectangular_holographic_grating",
+    "measured_grating_multilayer_film", "flat_trapezoid_inverted", "square_grating",
+    "triangular_grating"])
+def test_operator_zoo_f32_host(name):
+    ZOO, _zoo_rays = _zoo()
+    surf = ZOO[name]
+    wl = (12.0, 70.0) if "multilayer" in name else (400.0, 700.0)
+    r = _zoo_rays(n=1500, wl=wl)
+    r = {k: (np.asarray(v, np.float32).astype(np.float64) if k != "unvignetted" else v)
+         for k, v in r.items()}
+    after = surfaces.Surface(transformation=Cartesian3dTranslation(z=120.0))
+    ref = oracle.propagate([surf, after], r)
+    got = trace_rays_f32([surf, after], r)
+    for a, b in zip(ref, got):
+        assert np.array_equal(a["unvignetted"], b["unvignetted"])
+        pa, pb = a["position"], b["position"].astype(np.float64)
+        assert np.array_equal(np.isnan(pa), np.isnan(pb))
+        fin = np.isfinite(pa) & np.isfinite(pb)
+        assert np.all(np.abs(pb[fin] - pa[fin]) <= 2e-5 + 2e-7 * np.abs(pa[fin]))
+        da, db = a["direction"], b["direction"].astype(np.float64)
+        fin = np.isfinite(da)
+        assert np.all(np.abs(db[fin] - da[fin]) <= 1e-7)
+        ia, ib = a["intensity"], b["intensity"].astype(np.float64)
+        fin = np.isfinite(ia) & (np.abs(ia) > 1e-6)
+        assert np.all(np.abs(ib[fin] - ia[fin]) <= 1e-4 * np.abs(ia[fin]) + 1e-7)
