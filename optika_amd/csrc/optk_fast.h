@@ -593,23 +593,8 @@ OPTK_HD double fast_index(const optk_surface &s, double wavelength, double n1) {
 }
 
 // ---------------------------------------------------------------------------
-// float32 linear interpolation (numpy.interp rules) over float32 tables
+// float32 refractive index tables
 // ---------------------------------------------------------------------------
-
-OPTK_HD float interp1f(const float *xp, const float *fp, int count, float x) {
-  if (!(x >= xp[0])) return (x != x) ? x : fp[0];
-  if (x >= xp[count - 1]) return fp[count - 1];
-  int lo = 0, hi = count - 1;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (x < xp[mid])
-      hi = mid;
-    else
-      lo = mid;
-  }
-  if (x == xp[lo]) return fp[lo];
-  return (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]) * (x - xp[lo]) + fp[lo];
-}
 
 OPTK_HD Cf chemical_nf(const optk_chemical &ch, const float *tables, float wavelength_nm) {
   const float *w = tables + ch.offset;
@@ -949,7 +934,7 @@ OPTK_HD void fast_propagate(const FastSurface &f, const optk_surface &s, const F
   if (ruled) {
     order = path_order(s, path);
     V3<double> p1 = {hx, hy, hz};
-    V3<double> kappa = spacing_vector(s, p1, normal);
+    V3<double> kappa = spacing_vector<double>(s, p1, normal);
     a = incident_effective(s, order, (double)r.wavelength, a, (double)r.index, normal, kappa);
   }
 
@@ -959,9 +944,10 @@ OPTK_HD void fast_propagate(const FastSurface &f, const optk_surface &s, const F
   float eff = fast_material_efficiency<F>(s, v.sv, v.tables32, r.wavelength, a, r.index,
                                           r.attenuation, normal);
   if (ruled) {
-    V3<double> p1 = {hx, hy, hz};
-    eff = eff * (float)rulings_efficiency(s, order, (double)r.wavelength, a, p1, normal,
-                                          v.sv.tables);
+    V3<float> p1 = {(float)hx, (float)hy, (float)hz};
+    V3<float> af = {(float)a.x, (float)a.y, (float)a.z};
+    V3<float> nf = {(float)normal.x, (float)normal.y, (float)normal.z};
+    eff = eff * rulings_efficiency<float>(s, order, r.wavelength, af, p1, nf, v.tables32);
   }
   r.intensity = r.intensity * eff;
   r.attenuation = (float)material_attenuation(s, (double)r.attenuation);

@@ -295,7 +295,7 @@ __global__ __launch_bounds__(256) void eval_f64(const EvalParams E) {
     break;
   }
   case OPTK_OP_SPACING: {
-    V3<double> k = spacing_vector(s, {in[0][i], in[1][i], in[2][i]}, {in[3][i], in[4][i], in[5][i]});
+    V3<double> k = spacing_vector<double>(s, {in[0][i], in[1][i], in[2][i]}, {in[3][i], in[4][i], in[5][i]});
     out[0][i] = k.x;
     out[1][i] = k.y;
     out[2][i] = k.z;
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void eval_f64(const EvalParams E) {
     break;
   }
   case OPTK_OP_RULINGS_EFF:
-    out[0][i] = rulings_efficiency(s, path_order(s, E.iparam), in[0][i], {in[1][i], in[2][i], in[3][i]},
+    out[0][i] = rulings_efficiency<double>(s, path_order(s, E.iparam), in[0][i], {in[1][i], in[2][i], in[3][i]},
                                    {in[4][i], in[5][i], in[6][i]}, {in[7][i], in[8][i], in[9][i]},
                                    E.sys.sv.tables);
     break;

@@ -243,6 +243,22 @@ OPTK_HD double interp1(const double *xp, const double *fp, int count, double x) 
   return interp_at(xp, fp, count, x, j);
 }
 
+// the same rules over float32 tables
+OPTK_HD float interp1f(const float *xp, const float *fp, int count, float x) {
+  if (!(x >= xp[0])) return (x != x) ? x : fp[0];
+  if (x >= xp[count - 1]) return fp[count - 1];
+  int lo = 0, hi = count - 1;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (x < xp[mid])
+      hi = mid;
+    else
+      lo = mid;
+  }
+  if (x == xp[lo]) return fp[lo];
+  return (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]) * (x - xp[lo]) + fp[lo];
+}
+
 // n + i k of a chemical at wavelength (nm); the table is in Angstrom.
 // numpy.interp of a complex fp interpolates both parts with one reciprocal
 // of the knot spacing (numpy's complex interp loop); the same is done here.

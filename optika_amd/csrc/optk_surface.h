@@ -269,49 +269,51 @@ template <int F = FEAT_ALL> OPTK_HD void sag_propagate(const optk_surface &s, Ra
 // ---------------------------------------------------------------------------
 
 // spacing_(position, normal): the local ruling vector d g
-OPTK_HD V3<double> spacing_vector(const optk_surface &s, V3<double> p, V3<double> n) {
+template <typename T>
+OPTK_HD V3<T> spacing_vector(const optk_surface &s, V3<T> p, V3<T> n) {
   switch (s.spacing_kind) {
   case OPTK_SPACING_CONSTANT:  // _spacing.py:69-74
-    return {s.spacing_constant * s.spacing_normal[0], s.spacing_constant * s.spacing_normal[1],
-            s.spacing_constant * s.spacing_normal[2]};
+    return {T(s.spacing_constant) * T(s.spacing_normal[0]),
+            T(s.spacing_constant) * T(s.spacing_normal[1]),
+            T(s.spacing_constant) * T(s.spacing_normal[2])};
   case OPTK_SPACING_POLY1D: {  // _spacing.py:109-128
     xf_apply_pos(s.spacing_transform, p.x, p.y, p.z);
-    V3<double> g = {s.spacing_normal[0], s.spacing_normal[1], s.spacing_normal[2]};
-    double x = dot3(p, g);
-    double d = 0.0;
+    V3<T> g = {T(s.spacing_normal[0]), T(s.spacing_normal[1]), T(s.spacing_normal[2])};
+    T x = dot3(p, g);
+    T d = T(0);
     for (int k = 0; k < s.num_terms; ++k) {
       int pw = s.poly_power[k];
-      double xp;
+      T xp;
       if (pw == 0)
-        xp = 1.0;
+        xp = T(1);
       else if (pw == 1)
         xp = x;
       else if (pw == 2)
         xp = x * x;
       else
-        xp = pow(x, (double)pw);
-      d = d + s.poly_coeff[k] * xp;
+        xp = pow(x, (T)pw);
+      d = d + T(s.poly_coeff[k]) * xp;
     }
     return {d * g.x, d * g.y, d * g.z};
   }
   case OPTK_SPACING_HOLOGRAPHIC: {  // _spacing.py:295-328
-    double d1 = 2.0 * (double)s.holo_diverging1 - 1.0;
-    double d2 = 2.0 * (double)s.holo_diverging2 - 1.0;
-    V3<double> r1 = {p.x - s.holo_x1[0], p.y - s.holo_x1[1], p.z - s.holo_x1[2]};
-    V3<double> r2 = {p.x - s.holo_x2[0], p.y - s.holo_x2[1], p.z - s.holo_x2[2]};
-    double l1 = len3(r1), l2 = len3(r2);
+    T d1 = T(2) * (T)s.holo_diverging1 - T(1);
+    T d2 = T(2) * (T)s.holo_diverging2 - T(1);
+    V3<T> r1 = {p.x - T(s.holo_x1[0]), p.y - T(s.holo_x1[1]), p.z - T(s.holo_x1[2])};
+    V3<T> r2 = {p.x - T(s.holo_x2[0]), p.y - T(s.holo_x2[1]), p.z - T(s.holo_x2[2])};
+    T l1 = len3(r1), l2 = len3(r2);
     r1 = {d1 * (r1.x / l1), d1 * (r1.y / l1), d1 * (r1.z / l1)};
     r2 = {d2 * (r2.x / l2), d2 * (r2.y / l2), d2 * (r2.z / l2)};
-    V3<double> dr = {r1.x - r2.x, r1.y - r2.y, r1.z - r2.z};
-    V3<double> aq = cross3(n, dr);
-    double a = len3(aq);
-    V3<double> q = div3(aq, a);
-    double spacing = s.holo_wavelength / a;
-    V3<double> qn = cross3(q, n);
+    V3<T> dr = {r1.x - r2.x, r1.y - r2.y, r1.z - r2.z};
+    V3<T> aq = cross3(n, dr);
+    T a = len3(aq);
+    V3<T> q = div3(aq, a);
+    T spacing = T(s.holo_wavelength) / a;
+    V3<T> qn = cross3(q, n);
     return {spacing * qn.x, spacing * qn.y, spacing * qn.z};
   }
   }
-  return {0.0, 0.0, 0.0};
+  return {T(0), T(0), T(0)};
 }
 
 OPTK_HD int path_order(const optk_surface &s, int path) {
@@ -332,81 +334,93 @@ OPTK_HD V3<double> incident_effective(const optk_surface &s, int order, double w
           a.z + sgn * m * w * g.z / den};
 }
 
+// Bessel function of the first kind of integer order, per real type
+OPTK_HD double bessel_jn(int m, double x) { return jn(m, x); }
+OPTK_HD float bessel_jn(int m, float x) { return jnf(m, x); }
+
+// linear interpolation in a (wavelength row, value row) table, per real type
+OPTK_HD double table_interp(const double *w, int count, double x) {
+  return interp1(w, w + count, count, x);
+}
+
+OPTK_HD float table_interp(const float *w, int count, float x) {
+  return interp1f(w, w + count, count, x);
+}
+
 // groove efficiencies (_rulings.py:246-1073)
-OPTK_HD double rulings_efficiency(const optk_surface &s, int order, double wavelength,
-                                  V3<double> a_eff, V3<double> position, V3<double> normal,
-                                  const double *tables) {
+template <typename T>
+OPTK_HD T rulings_efficiency(const optk_surface &s, int order, T wavelength, V3<T> a_eff,
+                             V3<T> position, V3<T> normal, const T *tables) {
   switch (s.rulings_kind) {
   case OPTK_RULINGS_IDEAL:
-    return 1.0;
-  case OPTK_RULINGS_MEASURED: {
-    const double *w = tables + s.rulings_table;
-    return interp1(w, w + s.rulings_table_len, s.rulings_table_len, wavelength);
-  }
+    return T(1);
+  case OPTK_RULINGS_MEASURED:
+    return table_interp(tables + s.rulings_table, s.rulings_table_len, wavelength);
   default:
     break;
   }
-  V3<double> kappa = spacing_vector(s, position, normal);
-  V3<double> nr = div3(kappa, len3(kappa));
-  V3<double> par = cross3(normal, nr);
+  V3<T> kappa = spacing_vector<T>(s, position, normal);
+  V3<T> nr = div3(kappa, len3(kappa));
+  V3<T> par = cross3(normal, nr);
   par = div3(par, len3(par));
-  double proj = dot3(a_eff, par);
-  V3<double> d = {a_eff.x - proj, a_eff.y - proj, a_eff.z - proj};
-  double cos_theta = -dot3(d, normal);
-  double i = (double)order;
-  const double pi = M_PI;
+  T proj = dot3(a_eff, par);
+  V3<T> d = {a_eff.x - proj, a_eff.y - proj, a_eff.z - proj};
+  T cos_theta = -dot3(d, normal);
+  T i = (T)order;
+  const T pi = T(M_PI);
+  const T depth = T(s.rulings_p[0]);
   switch (s.rulings_kind) {
   case OPTK_RULINGS_SINUSOIDAL: {  // J_i(2 gamma), unsquared
-    double gamma = pi * s.rulings_p[0] / (wavelength * cos_theta);
-    // J_{-n}(x) = (-1)^n J_n(x), applied explicitly: the device jn is
-    // only relied on for n >= 0
+    T gamma = pi * depth / (wavelength * cos_theta);
+    // J_{-n}(x) = (-1)^n J_n(x), applied explicitly: the device Bessel
+    // functions are only relied on for n >= 0
     int m = order < 0 ? -order : order;
-    double v = jn(m, 2.0 * gamma);
+    T v = bessel_jn(m, T(2) * gamma);
     return (order < 0 && (m & 1)) ? -v : v;
   }
   case OPTK_RULINGS_SQUARE: {
-    double dd = s.rulings_p[0] / (pi / 4.0);
-    double gamma = pi * dd / (wavelength * cos_theta);
+    T dd = depth / (pi / T(4));
+    T gamma = pi * dd / (wavelength * cos_theta);
     if (order == 0) {
-      double cg = cos(pi * gamma / 2.0);
+      T cg = cos(pi * gamma / T(2));
       return cg * cg;
     }
-    if ((order & 1) == 0) return 0.0;
-    double v = 2.0 * sin(pi * gamma / 2.0) / (i * pi);
+    if ((order & 1) == 0) return T(0);
+    T v = T(2) * sin(pi * gamma / T(2)) / (i * pi);
     return v * v;
   }
   case OPTK_RULINGS_SAWTOOTH: {
-    double dd = s.rulings_p[0] / (pi / 2.0);
-    double gamma = pi * dd / (wavelength * cos_theta);
-    double v = sin(pi * gamma) / (pi * (gamma + i));
+    T dd = depth / (pi / T(2));
+    T gamma = pi * dd / (wavelength * cos_theta);
+    T v = sin(pi * gamma) / (pi * (gamma + i));
     return v * v;
   }
   case OPTK_RULINGS_TRIANGULAR: {
-    double dd = s.rulings_p[0] / ((pi * pi) / 8.0);
-    double gamma = pi * dd / (wavelength * cos_theta);
-    double pg = pi * gamma / 2.0;
-    double a = gamma / (pg * pg + i * i);
-    double arg = (pi * pi) * gamma / 4.0;
-    double v = (order & 1) == 0 ? a * sin(arg) : a * cos(arg);
+    T dd = depth / ((pi * pi) / T(8));
+    T gamma = pi * dd / (wavelength * cos_theta);
+    T pg = pi * gamma / T(2);
+    T a = gamma / (pg * pg + i * i);
+    T arg = (pi * pi) * gamma / T(4);
+    T v = (order & 1) == 0 ? a * sin(arg) : a * cos(arg);
     return v * v;
   }
   case OPTK_RULINGS_RECTANGULAR: {
-    double a = 2.0 * pi * s.rulings_p[1];
-    double root = sqrt(2.0 * (1.0 - cos(a)));
-    double amplitude = pi / (2.0 * root);
-    double dd = s.rulings_p[0] / amplitude;
-    double gamma = pi * dd / (wavelength * cos_theta);
-    double sb = sin(pi * gamma / root);
-    double b = sb * sb;
+    T a = T(2) * pi * T(s.rulings_p[1]);
+    T root = sqrt(T(2) * (T(1) - cos(a)));
+    T amplitude = pi / (T(2) * root);
+    T dd = depth / amplitude;
+    T gamma = pi * dd / (wavelength * cos_theta);
+    T sb = sin(pi * gamma / root);
+    T b = sb * sb;
     if (order == 0) {
-      double ap = a / pi;
-      return 1.0 - ((2.0 * a / pi) - ap * ap) * b;
+      T ap = a / pi;
+      return T(1) - ((T(2) * a / pi) - ap * ap) * b;
     }
-    double ip = i * pi;
-    return (2.0 / (ip * ip)) * (1.0 - cos(i * a)) * b;
+    T ip = i * pi;
+    return (T(2) / (ip * ip)) * (T(1) - cos(i * a)) * b;
   }
   }
-  return 1.0;
+  return T(1);
 }
 
 // ---------------------------------------------------------------------------
@@ -555,7 +569,7 @@ OPTK_HD void propagate_surface(const optk_surface &s, const SystemView &sys, int
   const bool ruled = (F & FEAT_RULINGS) && s.rulings_kind != OPTK_RULINGS_NONE;
   if (ruled) {
     order = path_order(s, path);
-    V3<double> kappa = spacing_vector(s, p1, normal);
+    V3<double> kappa = spacing_vector<double>(s, p1, normal);
     a = incident_effective(s, order, r.wavelength, a, r.index, normal, kappa);
   }
 
@@ -567,7 +581,7 @@ OPTK_HD void propagate_surface(const optk_surface &s, const SystemView &sys, int
   double eff =
       material_efficiency<F>(s, sys.sv, r.wavelength, a, r.index, r.attenuation, normal);
   if (ruled)
-    eff = eff * rulings_efficiency(s, order, r.wavelength, a, p1, normal, sys.sv.tables);
+    eff = eff * rulings_efficiency<double>(s, order, r.wavelength, a, p1, normal, sys.sv.tables);
   r.intensity = r.intensity * eff;
   r.attenuation = material_attenuation(s, r.attenuation);
   r.index = n2;
