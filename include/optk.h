@@ -24,6 +24,9 @@
  *                    materials (optika/materials/_materials.py:28-78,
  *                    _snells_law.py:13-366, _multilayers.py:33-532)
  *   optk_collect     optika/sensors/_sensors.py:92-171 (sensor binning)
+ *   optk_batch_*     n-dimensional configurations of a system
+ *                    (optika/systems/_sequential.py:2125-2132,
+ *                    optika/surfaces.py:386-395), with ragged ray counts
  */
 #ifndef OPTK_H
 #define OPTK_H
@@ -338,6 +341,66 @@ typedef struct optk_collect_args {
 } optk_collect_args;
 
 int32_t optk_collect(const optk_collect_args *args, int64_t n, void *hip_stream);
+
+/* Batched trace: many instances of one system layout (the same surface
+ * kinds, layer stacks and tables; different parameters: tolerance
+ * perturbations), each with its own number of rays, in ONE persistent
+ * launch.  Replaces the reference's n-dimensional configurations, where
+ * surface parameters broadcast along named axes of the ray arrays
+ * (SequentialSystem.shape, optika/systems/_sequential.py:2125-2132;
+ * Surface.shape, optika/surfaces.py:386-395) and every instance sees the
+ * same (dense) ray grid. */
+typedef struct optk_batch optk_batch;
+
+typedef struct optk_batch_desc {
+  const optk_system_desc *base;  /* layers, stacks, chemicals, tables and
+                                    num_surfaces / num_paths of every instance */
+  const optk_surface *surfaces;  /* num_instances * base->num_surfaces records,
+                                    instance-major */
+  const optk_affine *pre;        /* num_instances * 2, or NULL: base->pre */
+  const optk_affine *post;       /* num_instances, or NULL: base->post */
+  int32_t num_instances;
+  int32_t pad;
+} optk_batch_desc;
+
+/* launch shape of the persistent kernel: threads per block, rays per thread
+ * per work tile (a tile is block * rays_per_thread rays of one instance),
+ * blocks resident per CU (0: as many as fit) */
+typedef struct optk_tune {
+  int32_t block;
+  int32_t rays_per_thread;
+  int32_t blocks_per_cu;
+  int32_t pad;
+  double ms; /* measured time of one batched trace (optk_batch_autotune) */
+} optk_tune;
+
+/* per (instance, path): unvignetted count, sum of intensity, sums of x, y,
+ * x^2, y^2 of the final positions (unvignetted rays), RMS spot radius about
+ * the centroid (mm), throughput = sum of intensity / rays traced */
+#define OPTK_BATCH_STATS 8
+#define OPTK_BATCH_GRAPH 2 /* flag: capture the launch in a hipGraph once,
+                              replay it while the arguments are unchanged */
+
+int32_t optk_batch_create(const optk_batch_desc *desc, optk_batch **out);
+int32_t optk_batch_destroy(optk_batch *batch);
+
+/* Trace the rays of every instance: instance i owns grid rays
+ * offsets[i] .. offsets[i+1]-1 (offsets: host array of num_instances + 1,
+ * offsets[0] = 0).  out (device rays, num_paths * n, path-major) may be
+ * NULL; stats (device, num_instances * num_paths * OPTK_BATCH_STATS
+ * doubles) is overwritten.  tune NULL: the last autotuned shape, else a
+ * default.  flags: OPTK_TRACE_APPLY_POST, OPTK_BATCH_GRAPH.  One call at a
+ * time per batch (the work-queue counter is the batch's). */
+int32_t optk_batch_trace(optk_batch *batch, int32_t precision, const optk_grid *in,
+                         const int64_t *offsets, const optk_rays *out, double *stats,
+                         const optk_tune *tune, int32_t flags, void *hip_stream);
+
+/* Time every candidate launch shape on this batch and these inputs (each
+ * run is a full optk_batch_trace without graph capture); return the
+ * fastest, which later calls with tune == NULL use. */
+int32_t optk_batch_autotune(optk_batch *batch, int32_t precision, const optk_grid *in,
+                            const int64_t *offsets, double *stats, int32_t flags,
+                            void *hip_stream, optk_tune *best);
 
 #ifdef __cplusplus
 }

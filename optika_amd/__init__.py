@@ -6,7 +6,8 @@ path: ``systems.SequentialSystem`` (``raytrace``, ``rayfunction``),
 ``surfaces.Surface.propagate_rays``, ``propagators.propagate_rays`` /
 ``accumulate_rays``, the ``sags``, ``rulings``, ``apertures`` and
 ``materials`` operators and ``sensors.ImagingSensor.collect`` keep their
-names and arguments.  Units are fixed instead of carried: millimetres,
+names and arguments.  ``BatchedSystem`` traces many perturbed instances of
+one layout, each with its own ray count, in one persistent launch.  Units are fixed instead of carried: millimetres,
 nanometres, degrees, 1/mm.
 
 All of them run on the GPU through ``liboptk.so`` (hand-written gfx950 HIP
@@ -29,8 +30,11 @@ from . import (
     transformations,
     vectors,
 )
+from .batch import STAT_FIELDS, BatchedSystem
 
 __all__ = [
+    "BatchedSystem",
+    "STAT_FIELDS",
     "apertures",
     "chemicals",
     "materials",

@@ -56,6 +56,7 @@ def _torch_lib() -> str:
 _SOURCES = {
     "optk_lib.hip": ["-ffp-contract=off"],
     "optk_fast.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "optk_batch.hip": ["-ffp-contract=off"],
 }
 
 

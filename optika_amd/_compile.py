@@ -315,27 +315,34 @@ def build_records(surfaces: Sequence, pre=(None, None), pre_is_inverse=(0, 0),
     return rec
 
 
+def system_desc(rec: HostRecords):
+    """The ``optk_system_desc`` of host records, and the array it points at
+    (keep it alive while the descriptor is in use)."""
+    d = _lib.SystemDesc()
+    d.surfaces = ctypes.cast(rec.surfaces, ctypes.POINTER(_lib.Surface))
+    d.num_surfaces = rec.num_surfaces
+    d.num_paths = rec.num_paths
+    d.layers = ctypes.cast(rec.layers, ctypes.POINTER(_lib.Layer))
+    d.num_layers = rec.num_layers
+    d.stacks = ctypes.cast(rec.stacks, ctypes.POINTER(_lib.Stack))
+    d.num_stacks = rec.num_stacks
+    d.chemicals = ctypes.cast(rec.chemicals, ctypes.POINTER(_lib.Chemical))
+    d.num_chemicals = rec.num_chemicals
+    tab = np.ascontiguousarray(rec.tables, dtype=np.float64)
+    d.tables = tab.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    d.num_tables = tab.size
+    d.pre[0], d.pre[1] = rec.pre[0], rec.pre[1]
+    d.pre_is_inverse[0], d.pre_is_inverse[1] = rec.pre_is_inverse
+    d.post = rec.post
+    return d, tab
+
+
 class CompiledSystem:
     """A system uploaded to the device (owns an ``optk_system*``)."""
 
     def __init__(self, rec: HostRecords):
         L = _lib.lib()
-        d = _lib.SystemDesc()
-        d.surfaces = ctypes.cast(rec.surfaces, ctypes.POINTER(_lib.Surface))
-        d.num_surfaces = rec.num_surfaces
-        d.num_paths = rec.num_paths
-        d.layers = ctypes.cast(rec.layers, ctypes.POINTER(_lib.Layer))
-        d.num_layers = rec.num_layers
-        d.stacks = ctypes.cast(rec.stacks, ctypes.POINTER(_lib.Stack))
-        d.num_stacks = rec.num_stacks
-        d.chemicals = ctypes.cast(rec.chemicals, ctypes.POINTER(_lib.Chemical))
-        d.num_chemicals = rec.num_chemicals
-        tab = np.ascontiguousarray(rec.tables, dtype=np.float64)
-        d.tables = tab.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-        d.num_tables = tab.size
-        d.pre[0], d.pre[1] = rec.pre[0], rec.pre[1]
-        d.pre_is_inverse[0], d.pre_is_inverse[1] = rec.pre_is_inverse
-        d.post = rec.post
+        d, tab = system_desc(rec)
         handle = ctypes.c_void_p()
         _lib.check(L.optk_system_create(ctypes.byref(d), ctypes.byref(handle)))
         self.handle = handle

@@ -18,7 +18,8 @@ import pathlib
 import torch  # noqa: F401  (must precede loading liboptk.so)
 
 __all__ = ["lib", "check", "OptkError", "Affine", "Surface", "Layer", "Block", "Stack",
-           "Chemical", "SystemDesc", "Rays", "Grid", "EvalArgs", "CollectArgs"]
+           "Chemical", "SystemDesc", "Rays", "Grid", "EvalArgs", "CollectArgs", "BatchDesc",
+           "Tune"]
 
 _LIB_PATH = pathlib.Path(os.environ.get(
     "OPTK_LIBRARY", pathlib.Path(__file__).resolve().parent / "liboptk.so"))
@@ -179,6 +180,25 @@ class CollectArgs(ctypes.Structure):
     ]
 
 
+class BatchDesc(ctypes.Structure):
+    _fields_ = [
+        ("base", ctypes.POINTER(SystemDesc)),
+        ("surfaces", ctypes.POINTER(Surface)),
+        ("pre", ctypes.POINTER(Affine)),
+        ("post", ctypes.POINTER(Affine)),
+        ("num_instances", c_int32), ("pad", c_int32),
+    ]
+
+
+class Tune(ctypes.Structure):
+    _fields_ = [("block", c_int32), ("rays_per_thread", c_int32), ("blocks_per_cu", c_int32),
+                ("pad", c_int32), ("ms", c_double)]
+
+
+BATCH_STATS = 8
+BATCH_GRAPH = 2
+
+
 class OptkError(RuntimeError):
     """An error reported by liboptk.so."""
 
@@ -215,12 +235,24 @@ def lib() -> ctypes.CDLL:
     L.optk_eval.argtypes = [c_void_p, c_int32, ctypes.POINTER(EvalArgs), c_int64, c_void_p]
     L.optk_collect.restype = c_int32
     L.optk_collect.argtypes = [ctypes.POINTER(CollectArgs), c_int64, c_void_p]
+    L.optk_batch_create.restype = c_int32
+    L.optk_batch_create.argtypes = [ctypes.POINTER(BatchDesc), ctypes.POINTER(c_void_p)]
+    L.optk_batch_destroy.restype = c_int32
+    L.optk_batch_destroy.argtypes = [c_void_p]
+    L.optk_batch_trace.restype = c_int32
+    L.optk_batch_trace.argtypes = [c_void_p, c_int32, ctypes.POINTER(Grid), ctypes.POINTER(c_int64),
+                                   ctypes.POINTER(Rays), c_void_p, ctypes.POINTER(Tune), c_int32,
+                                   c_void_p]
+    L.optk_batch_autotune.restype = c_int32
+    L.optk_batch_autotune.argtypes = [c_void_p, c_int32, ctypes.POINTER(Grid), ctypes.POINTER(c_int64),
+                                      c_void_p, c_int32, c_void_p, ctypes.POINTER(Tune)]
     if L.optk_abi_version() != 1:
         raise OptkError(f"liboptk ABI {L.optk_abi_version()} != 1")
     for name, cls in [("optk_affine", Affine), ("optk_surface", Surface), ("optk_layer", Layer),
                       ("optk_block", Block), ("optk_stack", Stack), ("optk_chemical", Chemical),
                       ("optk_system_desc", SystemDesc), ("optk_rays", Rays), ("optk_grid", Grid),
-                      ("optk_eval_args", EvalArgs), ("optk_collect_args", CollectArgs)]:
+                      ("optk_eval_args", EvalArgs), ("optk_collect_args", CollectArgs),
+                      ("optk_batch_desc", BatchDesc), ("optk_tune", Tune)]:
         size = L.optk_sizeof(name.encode())
         if size != ctypes.sizeof(cls):
             raise OptkError(f"record {name}: C size {size} != binding size {ctypes.sizeof(cls)}")

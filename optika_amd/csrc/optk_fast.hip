@@ -3,7 +3,7 @@
 // unlike the float64 form, this form does not reproduce numpy's rounding
 // sequence, so multiply-adds may fuse.
 
-#include "optk_internal.h"
+#include "optk_batch.h"
 
 using namespace optk;
 
@@ -192,6 +192,71 @@ int32_t launch(void (*k0)(FastArgs, In, FRaysOut), void (*k3)(FastArgs, In, FRay
   return OPTK_OK;
 }
 
+// batched persistent trace, float32 form ------------------------------------
+
+template <int F> struct Tracer32 {
+  const BatchArgs &A;
+  const FGridIn &in;
+  const FRaysOut &out;
+  FastSpec *s_spec;
+  FastSurface *s_fast;
+  optk_surface *s_surf;
+
+  __device__ int num_paths() const { return (F & FEAT_RULINGS) ? A.num_paths : 1; }
+
+  __device__ void stage(int instance) {
+    stage_words(s_spec, A.fast_specs + instance, sizeof(FastSpec));
+    stage_words(s_fast, A.fast + (int64_t)instance * A.num_surfaces,
+                sizeof(FastSurface) * (size_t)A.num_surfaces);
+    stage_words(s_surf, A.surfaces + (int64_t)instance * A.num_surfaces,
+                sizeof(optk_surface) * (size_t)A.num_surfaces);
+  }
+
+  __device__ void trace(int64_t i, int path, double &x, double &y, double &inten, bool &unv) {
+    FastView v;
+    v.surfaces = s_surf;
+    v.fast = s_fast;
+    v.sv = A.sv;
+    v.tables32 = A.tables32;
+    FRay r = load_fray(in, i);
+    xf_apply_pos(s_spec->first_hop[1], r.px, r.py, r.pz);
+    xf_apply_dir(s_spec->first_hop[1], r.dx, r.dy, r.dz);
+    for (int k = 0; k < A.num_surfaces; ++k) {
+      if (k > 0) {
+        xf_apply_pos(s_fast[k].hop, r.px, r.py, r.pz);
+        xf_apply_dir(s_fast[k].hop, r.dx, r.dy, r.dz);
+      }
+      fast_propagate<F>(s_fast[k], s_surf[k], v, path, r);
+    }
+    const optk_affine &last = s_spec->last[A.apply_post];
+    xf_apply_pos(last, r.px, r.py, r.pz);
+    xf_apply_dir(last, r.dx, r.dy, r.dz);
+    store_f(out, (int64_t)path * A.n + i, r);
+    // statistics of the float32 result, as a caller reading the outputs sees it
+    x = (double)(float)r.px;
+    y = (double)(float)r.py;
+    inten = r.intensity;
+    unv = r.unvignetted;
+  }
+};
+
+template <int F>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
+void batch_grid_f32(const BatchArgs A, const FGridIn in, const FRaysOut out) {
+  extern __shared__ double staged[];
+  FastSpec *spec = reinterpret_cast<FastSpec *>(staged);
+  FastSurface *fast = reinterpret_cast<FastSurface *>(spec + 1);
+  optk_surface *surf = reinterpret_cast<optk_surface *>(fast + A.num_surfaces);
+  Tracer32<F> tr{A, in, out, spec, fast, surf};
+  batch_loop(A, tr);
+}
+
+void *batch_kernel_f32(int features) {
+  if (features == 0) return (void *)batch_grid_f32<0>;
+  if (features == (FEAT_TOROID | FEAT_RULINGS)) return (void *)batch_grid_f32<FEAT_TOROID | FEAT_RULINGS>;
+  return (void *)batch_grid_f32<FEAT_ALL>;
+}
+
 }  // namespace
 
 namespace optk {
@@ -208,6 +273,29 @@ int32_t fast_trace_grid(const optk_system *sys, const FGridIn &in, const FRaysOu
   return launch<FGridIn>(trace_grid_f32<0, false>, trace_grid_f32<FEAT_TOROID | FEAT_RULINGS, false>,
                          trace_grid_f32<FEAT_ALL, false>, trace_grid_f32<FEAT_ALL, true>, sys, in, out,
                          accum, n, 1, apply_post, stream);
+}
+
+size_t batch_lds_f32(int num_surfaces) { return fast_staged_bytes(num_surfaces); }
+
+int batch_occupancy_f32(int features, int block, size_t lds) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel_f32(features), block, lds) !=
+      hipSuccess)
+    return 0;
+  return nb;
+}
+
+hipError_t batch_launch_f32(int features, int blocks, int block, const BatchArgs &A,
+                            const FGridIn &in, const FRaysOut &out, hipStream_t stream) {
+  size_t lds = fast_staged_bytes(A.num_surfaces);
+  if (features == 0)
+    hipLaunchKernelGGL(batch_grid_f32<0>, dim3(blocks), dim3(block), lds, stream, A, in, out);
+  else if (features == (FEAT_TOROID | FEAT_RULINGS))
+    hipLaunchKernelGGL(batch_grid_f32<FEAT_TOROID | FEAT_RULINGS>, dim3(blocks), dim3(block), lds,
+                       stream, A, in, out);
+  else
+    hipLaunchKernelGGL(batch_grid_f32<FEAT_ALL>, dim3(blocks), dim3(block), lds, stream, A, in, out);
+  return hipGetLastError();
 }
 
 }  // namespace optk

@@ -40,6 +40,36 @@ namespace optk {
 // record the thread's last error message and return `code`
 int32_t fail(int32_t code, const std::string &msg);
 
+// check every index and range a kernel will follow (OPTK_OK or an error)
+int32_t validate_desc(const optk_system_desc *d);
+// the kernel feature variant (FEAT_*) covering the system's surfaces
+int32_t system_features(const optk_system_desc *d);
+
+// float64 ray buffers (device pointers; null = use the default)
+struct RaysIn {
+  const double *p[3];
+  const double *d[3];
+  const double *wavelength, *intensity, *attenuation, *index;
+  const uint8_t *unvignetted;
+  double wavelength_default, intensity_default, attenuation_default, index_default;
+  int32_t unvignetted_default;
+};
+
+struct RaysOut {
+  double *p[3];
+  double *d[3];
+  double *wavelength, *intensity, *attenuation, *index;
+  uint8_t *unvignetted;
+};
+
+struct GridIn {
+  const double *field[2];
+  const double *pupil[2];
+  const double *wavelength, *intensity;
+  double field_default[2], pupil_default[2], wavelength_default, intensity_default;
+  int32_t object_at_infinity;
+};
+
 // float32 ray buffers (device pointers; null = use the default)
 struct FRaysIn {
   const float *p[3];
@@ -70,5 +100,44 @@ int32_t fast_trace_rays(const optk_system *sys, const FRaysIn &in, const FRaysOu
                         const FRaysOut *accum, int64_t n, int32_t apply_post, hipStream_t stream);
 int32_t fast_trace_grid(const optk_system *sys, const FGridIn &in, const FRaysOut &out,
                         const FRaysOut *accum, int64_t n, int32_t apply_post, hipStream_t stream);
+
+// ---------------------------------------------------------------------------
+// batched persistent trace (optk_batch_*)
+// ---------------------------------------------------------------------------
+
+// a work tile: `count` consecutive rays of one instance
+struct BatchTile {
+  int64_t start;
+  int32_t instance;
+  int32_t count;
+};
+
+struct BatchArgs {
+  const BatchTile *tiles;
+  int64_t num_tiles;
+  int32_t *counter;                // work-queue head, zeroed before each launch
+  int32_t rays_per_thread;
+  int32_t num_surfaces;
+  int32_t num_paths;
+  int32_t apply_post;
+  const optk_surface *surfaces;    // [instance][surface]
+  const TraceSpec *specs64;        // [instance]
+  const FastSurface *fast;         // [instance][surface]
+  const FastSpec *fast_specs;      // [instance]
+  StackView sv;
+  const float *tables32;
+  double *stats;                   // [instance][path][OPTK_BATCH_STATS]
+  int64_t n;                       // rays in the batch (output stride per path)
+};
+
+size_t batch_lds_f64(int num_surfaces);
+size_t batch_lds_f32(int num_surfaces);
+// resident blocks per CU of the kernel covering `features` (0 on error)
+int batch_occupancy_f64(int features, int block, size_t lds);
+int batch_occupancy_f32(int features, int block, size_t lds);
+hipError_t batch_launch_f64(int features, int blocks, int block, const BatchArgs &A,
+                            const GridIn &in, const RaysOut &out, hipStream_t stream);
+hipError_t batch_launch_f32(int features, int blocks, int block, const BatchArgs &A,
+                            const FGridIn &in, const FRaysOut &out, hipStream_t stream);
 
 }  // namespace optk

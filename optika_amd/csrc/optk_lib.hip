@@ -11,7 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "optk_internal.h"
+#include "optk_batch.h"
 
 using namespace optk;
 
@@ -59,30 +59,6 @@ static int32_t variant_of(int32_t features) {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-
-struct RaysIn {
-  const double *p[3];
-  const double *d[3];
-  const double *wavelength, *intensity, *attenuation, *index;
-  const uint8_t *unvignetted;
-  double wavelength_default, intensity_default, attenuation_default, index_default;
-  int32_t unvignetted_default;
-};
-
-struct RaysOut {
-  double *p[3];
-  double *d[3];
-  double *wavelength, *intensity, *attenuation, *index;
-  uint8_t *unvignetted;
-};
-
-struct GridIn {
-  const double *field[2];
-  const double *pupil[2];
-  const double *wavelength, *intensity;
-  double field_default[2], pupil_default[2], wavelength_default, intensity_default;
-  int32_t object_at_infinity;
-};
 
 // Per-launch arguments.  The system (surfaces, tables, transformations) is
 // device-resident; each block stages the transformation spec and every
@@ -216,6 +192,83 @@ static void launch_trace(void (*k0)(TraceArgs, In, RaysOut), void (*k3)(TraceArg
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), staged_bytes(A.num_surfaces), stream, A,
                      in, acc);
 }
+
+// batched persistent trace, float64 form ------------------------------------
+
+template <int F> struct Tracer64 {
+  const BatchArgs &A;
+  const GridIn &in;
+  const RaysOut &out;
+  TraceSpec *s_spec;
+  optk_surface *s_surf;
+
+  __device__ int num_paths() const { return (F & FEAT_RULINGS) ? A.num_paths : 1; }
+
+  __device__ void stage(int instance) {
+    stage_words(s_spec, A.specs64 + instance, sizeof(TraceSpec));
+    stage_words(s_surf, A.surfaces + (int64_t)instance * A.num_surfaces,
+                sizeof(optk_surface) * (size_t)A.num_surfaces);
+  }
+
+  __device__ void trace(int64_t i, int path, double &x, double &y, double &inten, bool &unv) {
+    Ray<double> r = load_ray(in, i);
+    apply_pre(*s_spec, r);
+    SystemView sys = s_spec->sys;
+    sys.surfaces = s_surf;
+    for (int k = 0; k < A.num_surfaces; ++k) propagate_surface<F>(s_surf[k], sys, path, r);
+    if (A.apply_post) {
+      xf_inverse_pos(s_spec->post, r.px, r.py, r.pz);
+      xf_inverse_dir(s_spec->post, r.dx, r.dy, r.dz);
+    }
+    store(out, (int64_t)path * A.n + i, r);
+    x = r.px;
+    y = r.py;
+    inten = r.intensity;
+    unv = r.unvignetted;
+  }
+};
+
+template <int F>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
+void batch_grid_f64(const BatchArgs A, const GridIn in, const RaysOut out) {
+  extern __shared__ double staged[];
+  Tracer64<F> tr{A, in, out, reinterpret_cast<TraceSpec *>(staged),
+                 reinterpret_cast<optk_surface *>(reinterpret_cast<TraceSpec *>(staged) + 1)};
+  batch_loop(A, tr);
+}
+
+namespace optk {
+
+size_t batch_lds_f64(int num_surfaces) { return staged_bytes(num_surfaces); }
+
+static void *batch_kernel_f64(int features) {
+  if (features == 0) return (void *)batch_grid_f64<0>;
+  if (features == (FEAT_TOROID | FEAT_RULINGS)) return (void *)batch_grid_f64<FEAT_TOROID | FEAT_RULINGS>;
+  return (void *)batch_grid_f64<FEAT_ALL>;
+}
+
+int batch_occupancy_f64(int features, int block, size_t lds) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel_f64(features), block, lds) !=
+      hipSuccess)
+    return 0;
+  return nb;
+}
+
+hipError_t batch_launch_f64(int features, int blocks, int block, const BatchArgs &A,
+                            const GridIn &in, const RaysOut &out, hipStream_t stream) {
+  size_t lds = staged_bytes(A.num_surfaces);
+  if (features == 0)
+    hipLaunchKernelGGL(batch_grid_f64<0>, dim3(blocks), dim3(block), lds, stream, A, in, out);
+  else if (features == (FEAT_TOROID | FEAT_RULINGS))
+    hipLaunchKernelGGL(batch_grid_f64<FEAT_TOROID | FEAT_RULINGS>, dim3(blocks), dim3(block), lds,
+                       stream, A, in, out);
+  else
+    hipLaunchKernelGGL(batch_grid_f64<FEAT_ALL>, dim3(blocks), dim3(block), lds, stream, A, in, out);
+  return hipGetLastError();
+}
+
+}  // namespace optk
 
 // per-operator evaluation --------------------------------------------------
 
@@ -436,7 +489,8 @@ static bool valid_affine(const optk_affine &a) {
 
 // Check every index a kernel will follow, so no kernel can read outside the
 // uploaded records.
-static int32_t validate(const optk_system_desc *d) {
+namespace optk {
+int32_t validate_desc(const optk_system_desc *d) {
   // every surface record is staged in LDS (about 0.9 KiB each)
   if (d->num_surfaces < 0 || d->num_surfaces > OPTK_MAX_SURFACES)
     return fail(OPTK_EINVAL, "num_surfaces out of range (at most 64)");
@@ -500,6 +554,9 @@ static int32_t validate(const optk_system_desc *d) {
   return OPTK_OK;
 }
 
+int32_t system_features(const optk_system_desc *d) { return variant_of(features_of(d)); }
+}  // namespace optk
+
 static_assert(sizeof(optk_surface) % 8 == 0 && sizeof(TraceSpec) % 8 == 0, "records are staged as words");
 
 static dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
@@ -524,6 +581,8 @@ int64_t optk_sizeof(const char *name) {
   if (s == "optk_grid") return sizeof(optk_grid);
   if (s == "optk_eval_args") return sizeof(optk_eval_args);
   if (s == "optk_collect_args") return sizeof(optk_collect_args);
+  if (s == "optk_batch_desc") return sizeof(optk_batch_desc);
+  if (s == "optk_tune") return sizeof(optk_tune);
   return -1;
 }
 
@@ -549,7 +608,7 @@ int32_t optk_system_create(const optk_system_desc *desc, optk_system **out) {
       (desc->num_stacks > 0 && !desc->stacks) || (desc->num_chemicals > 0 && !desc->chemicals) ||
       (desc->num_tables > 0 && !desc->tables))
     return fail(OPTK_EINVAL, "null record array");
-  int32_t rc = validate(desc);
+  int32_t rc = validate_desc(desc);
   if (rc) return rc;
   optk_system *s = new optk_system();
   s->num_surfaces = desc->num_surfaces;

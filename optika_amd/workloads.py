@@ -29,6 +29,7 @@ import numpy as np
 
 from . import apertures, materials, rulings, sags, sensors, surfaces, systems
 from .transformations import (
+    Cartesian3dRotationX,
     Cartesian3dRotationY,
     Cartesian3dTranslation,
     TransformationList,
@@ -42,6 +43,8 @@ __all__ = [
     "newtonian_grid_random",
     "spectrograph",
     "spectrograph_grid",
+    "Perturbation",
+    "tolerance_batch",
     "NEWTONIAN_SURFACES",
     "SPECTROGRAPH_SURFACES",
 ]
@@ -226,37 +229,74 @@ def _spectrograph_layout():
     return grating_pos, grating_angle, stop_pos, sensor_pos, sensor_angle
 
 
-def spectrograph() -> systems.SequentialSystem:
+@dataclasses.dataclass
+class Perturbation:
+    """Errors of one built spectrograph: decenters (mm) and tilts (deg) of
+    the primary and the grating, relative radius errors, relative errors of
+    the three line-spacing coefficients."""
+
+    primary_decenter: tuple = (0.0, 0.0)
+    primary_tilt: tuple = (0.0, 0.0)
+    primary_radius: float = 0.0
+    grating_decenter: tuple = (0.0, 0.0)
+    grating_tilt: tuple = (0.0, 0.0)
+    grating_radii: tuple = (0.0, 0.0)
+    spacing: tuple = (0.0, 0.0, 0.0)
+
+    @classmethod
+    def gaussian(cls, rng: np.random.Generator, decenter=0.05, tilt=0.01, radius=1e-3,
+                 spacing=5e-3) -> "Perturbation":
+        n = lambda k, s: tuple(float(v) for v in rng.normal(0.0, s, k))  # noqa: E731
+        return cls(primary_decenter=n(2, decenter), primary_tilt=n(2, tilt),
+                   primary_radius=n(1, radius)[0], grating_decenter=n(2, decenter),
+                   grating_tilt=n(2, tilt), grating_radii=n(2, radius), spacing=n(3, spacing))
+
+
+def spectrograph(perturbation: Perturbation | None = None) -> systems.SequentialSystem:
+    """The spectrograph, optionally with the manufacturing and alignment
+    errors of ``perturbation`` (the layout itself stays the nominal one)."""
+    p = perturbation or Perturbation()
     grating_pos, grating_angle, stop_pos, sensor_pos, sensor_angle = _spectrograph_layout()
 
-    def frame(pos, angle):
-        return TransformationList([
+    def frame(pos, angle, tilt=(0.0, 0.0), decenter=(0.0, 0.0)):
+        steps = []
+        if tilt != (0.0, 0.0):
+            steps += [Cartesian3dRotationX(tilt[0]), Cartesian3dRotationY(tilt[1])]
+        steps += [
             Cartesian3dRotationY(angle),
-            Cartesian3dTranslation(x=float(pos[0]), y=float(pos[1]), z=float(pos[2])),
-        ])
+            Cartesian3dTranslation(x=float(pos[0]) + decenter[0], y=float(pos[1]) + decenter[1],
+                                   z=float(pos[2])),
+        ]
+        return TransformationList(steps)
 
+    if p.primary_tilt != (0.0, 0.0) or p.primary_decenter != (0.0, 0.0):
+        primary_frame = frame((0.0, 0.0, Z_SPEC_PRIMARY), 0.0, p.primary_tilt, p.primary_decenter)
+    else:
+        primary_frame = Cartesian3dTranslation(z=Z_SPEC_PRIMARY)
     primary = surfaces.Surface(
         name="primary",
-        sag=sags.ConicSag(radius=-3000.0, conic=-0.95,
+        sag=sags.ConicSag(radius=-3000.0 * (1.0 + p.primary_radius), conic=-0.95,
                           transformation=Cartesian3dTranslation(x=-DECENTER)),
         material=mo_si_multilayer(),
         aperture=apertures.CircularAperture(45.0),
         is_pupil_stop=True,
-        transformation=Cartesian3dTranslation(z=Z_SPEC_PRIMARY),
+        transformation=primary_frame,
     )
+    e0, e1, e2 = p.spacing
     grating = surfaces.Surface(
         name="grating",
-        sag=sags.ToroidalSag(radius=980.0, radius_of_rotation=1000.0),
+        sag=sags.ToroidalSag(radius=980.0 * (1.0 + p.grating_radii[0]),
+                             radius_of_rotation=1000.0 * (1.0 + p.grating_radii[1])),
         material=mo_si_multilayer(),
         rulings=rulings.Rulings(
             spacing=rulings.Polynomial1dRulingSpacing(
-                coefficients={0: D0, 1: D0 * C1, 2: D0 * C2},
+                coefficients={0: D0 * (1.0 + e0), 1: D0 * C1 * (1.0 + e1), 2: D0 * C2 * (1.0 + e2)},
                 normal=(1.0, 0.0, 0.0),
             ),
             diffraction_order=[-1, 1],
         ),
         aperture=apertures.RectangularAperture(half_width=(30.0, 30.0)),
-        transformation=frame(grating_pos, grating_angle),
+        transformation=frame(grating_pos, grating_angle, p.grating_tilt, p.grating_decenter),
     )
     field_stop = surfaces.Surface(
         name="field_stop",
@@ -283,3 +323,22 @@ def spectrograph_grid(n: int, seed: int = 2) -> Grid:
     fy = rng.uniform(-0.05, 0.05, n)
     wl = rng.uniform(58.0, 63.0, n)
     return Grid(fx, fy, r * np.cos(th), r * np.sin(th), wl)
+
+
+def tolerance_batch(num_instances: int = 4096, seed: int = 3, min_rays: int = 1000,
+                    max_rays: int = 64000):
+    """Batched tolerance analysis of the spectrograph: ``num_instances``
+    perturbed copies (decenters sigma 50 um, tilts sigma 0.01 deg, radii
+    sigma 0.1 %, line-spacing coefficients sigma 0.5 %), each traced with
+    its own ray count, log-uniform in [min_rays, max_rays).
+
+    Returns (systems, counts, grid): the grid holds every instance's rays
+    back to back (instance i owns rays offsets[i]:offsets[i+1], offsets the
+    running sum of counts), drawn like :func:`spectrograph_grid`.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    perts = [Perturbation.gaussian(rng) for _ in range(num_instances)]
+    counts = np.floor(np.exp(rng.uniform(math.log(min_rays), math.log(max_rays),
+                                         num_instances))).astype(np.int64)
+    grid = spectrograph_grid(int(counts.sum()), seed=seed + 1000)
+    return [spectrograph(p) for p in perts], counts, grid

@@ -19,7 +19,8 @@ def declared_functions():
 def test_header_declares_entry_points():
     names = declared_functions()
     for n in ("optk_abi_version", "optk_last_error", "optk_sizeof", "optk_system_create",
-              "optk_system_destroy", "optk_trace", "optk_trace_grid", "optk_eval", "optk_collect"):
+              "optk_system_destroy", "optk_trace", "optk_trace_grid", "optk_eval", "optk_collect",
+              "optk_batch_create", "optk_batch_destroy", "optk_batch_trace", "optk_batch_autotune"):
         assert n in names
 
 
@@ -53,3 +54,22 @@ def test_invalid_arguments_are_rejected_without_a_device():
     assert L.optk_system_create(ctypes.byref(d), ctypes.byref(handle)) == -1
     assert b"num_paths" in L.optk_last_error()
     assert L.optk_sizeof(b"no_such_record") == -1
+
+
+def test_batch_arguments_are_rejected_without_a_device():
+    from optika_amd import _lib
+
+    if not _lib.library_path().exists():
+        pytest.skip("liboptk.so not built")
+    L = _lib.lib()
+    handle = ctypes.c_void_p()
+    assert L.optk_batch_create(None, ctypes.byref(handle)) == -1
+    desc = _lib.BatchDesc()
+    assert L.optk_batch_create(ctypes.byref(desc), ctypes.byref(handle)) == -1  # no base
+    base = _lib.SystemDesc()
+    desc.base = ctypes.pointer(base)
+    desc.num_instances = 0
+    assert L.optk_batch_create(ctypes.byref(desc), ctypes.byref(handle)) == -1
+    assert b"num_instances" in L.optk_last_error()
+    assert L.optk_batch_trace(None, 0, None, None, None, None, None, 0, None) == -1
+    assert L.optk_batch_destroy(None) == 0
