@@ -1,20 +1,26 @@
 """Benchmark: rays·surfaces/s of the sequential trace on one MI355X.
 
-Workload (one "step"): ``SequentialSystem.rayfunction`` of the Newtonian
-telescope (primary, fold mirror, sensor: 3 counted surfaces) over 2^24
-rays with pupil positions uniform on the 100 mm disc, field angles uniform
-in +-0.25 deg and wavelengths uniform in 400-700 nm (seed 1).  Inputs are
-resident in HBM before the timed region; every step traces all rays through
-every surface into preallocated output buffers.
+Default workload (one "step"): ``SequentialSystem.rayfunction`` of the
+Newtonian telescope (primary, fold mirror, sensor: 3 counted surfaces) over
+2^24 rays with pupil positions uniform on the 100 mm disc, field angles
+uniform in +-0.25 deg and wavelengths uniform in 400-700 nm (seed 1).
+Inputs are resident in HBM before the timed region; every step traces all
+rays through every surface into preallocated output buffers.
 
-Prints one JSON line (see the fields at the end of ``main``).  ``value`` is
-the float32 compensated form; ``forms`` holds the float64 drop-in form
-(identical to the oracle to 1e-12) and the float32 form's RMS spot and
-direction error against it over all rays of the run.  The roofline object
-is for the float32 kernel (trace_grid_f32), timed with HIP events on the
-stream it runs on; ``cpu_baseline`` times the numpy float64 oracle (a
-restatement of the reference's vectorized numpy trace) on a bounded sample
-on this machine's host.
+Prints one JSON line (see the fields at the end of each workload).
+``value`` is the float32 compensated form; ``forms`` holds the float64
+drop-in form (identical to the oracle to 1e-12) and the float32 form's RMS
+spot and direction error against it over all rays of the run.  The roofline
+object is for the float32 kernel, timed with HIP events on the stream it
+runs on; ``cpu_baseline`` times the numpy float64 oracle (a restatement of
+the reference's vectorized numpy trace) on a bounded sample on this
+machine's host.
+
+Other workloads (``--workload``): ``spectrograph`` (the EUV spectrograph,
+4 counted surfaces, two diffraction orders, Mo/Si multilayers on both
+mirrors; 50M rays) and ``tolerance`` (4096 perturbed spectrographs with
+log-uniform ray counts in one persistent batched launch, autotuned and
+replayed from a hipGraph).
 """
 
 from __future__ import annotations
@@ -55,8 +61,12 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--rays", type=int, default=1 << 24)
-    p.add_argument("--cpu-rays", type=int, default=1 << 18,
+    p.add_argument("--workload", choices=("newtonian", "spectrograph", "tolerance"),
+                   default="newtonian")
+    p.add_argument("--rays", type=int, default=None,
+                   help="rays per step (default: 2^24 newtonian, 50M spectrograph)")
+    p.add_argument("--instances", type=int, default=4096, help="tolerance: instances")
+    p.add_argument("--cpu-rays", type=int, default=None,
                    help="rays in the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -64,7 +74,11 @@ def parse():
     return p.parse_args()
 
 
-def cpu_baseline(system, grid, sample: int) -> dict:
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(system, grid, sample: int, surfaces: int, paths: int = 1) -> dict:
     """Time the numpy float64 oracle on a bounded sample: 1 warm-up, median
     of 3 repeats, one host thread."""
     import oracle
@@ -78,26 +92,27 @@ def cpu_baseline(system, grid, sample: int) -> dict:
         t1 = time.perf_counter()
         if rep > 0:
             times.append(t1 - t0)
-        print(f"[bench] cpu baseline repeat {rep}: {t1 - t0:.2f} s", file=sys.stderr, flush=True)
+        log(f"cpu baseline repeat {rep}: {t1 - t0:.2f} s")
     med = statistics.median(times)
     return {
-        "value": sample * 3 / med,
+        "value": sample * paths * surfaces / med,
         "unit": "rays·surfaces/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{sample} rays of the same workload (first {sample} of the seeded grid), "
-                  f"numpy float64 oracle, median of 3 after 1 warm-up, "
-                  f"{os.cpu_count()} host CPUs present, 1 thread used",
+        "sample": f"{sample} rays of the same workload (first {sample} of the seeded grid"
+                  f"{', both diffraction orders' if paths > 1 else ''}), numpy float64 oracle, "
+                  f"median of 3 after 1 warm-up, {os.cpu_count()} host CPUs present, "
+                  f"1 thread used",
     }
 
 
-def _time_form(plan, inputs, steps, warmup, stream):
-    """Warm up, then time `steps` launches: wall clock between two device
-    synchronisations, and each launch with HIP events on `stream`."""
+def time_calls(fn, steps, warmup, stream):
+    """Warm up, then time `steps` calls: wall clock between two device
+    synchronisations, and each call with HIP events on `stream`."""
     import torch
 
     for _ in range(warmup):
-        plan(*inputs)
+        fn()
     torch.cuda.synchronize()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
@@ -105,7 +120,7 @@ def _time_form(plan, inputs, steps, warmup, stream):
     t0 = time.perf_counter()
     for k in range(steps):
         starts[k].record(stream)
-        plan(*inputs)
+        fn()
         ends[k].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -113,7 +128,7 @@ def _time_form(plan, inputs, steps, warmup, stream):
     return (t1 - t0) * 1e3 / steps, sum(kernel_ms) / len(kernel_ms)
 
 
-def _spot_error(r64, r32) -> dict:
+def spot_error(r64, r32) -> dict:
     """float32 form against float64 form, over rays unvignetted in both."""
     import torch
 
@@ -124,13 +139,243 @@ def _spot_error(r64, r32) -> dict:
     err = torch.sqrt(dx * dx + dy * dy)
     dd = torch.stack([(a.double() - b)[both] for a, b in zip(r32.direction, r64.direction)])
     derr = torch.sqrt((dd * dd).sum(0))
-    return {
-        "rms_spot_px": torch.sqrt((err * err).mean()).item(),
-        "max_spot_px": err.max().item(),
-        "rms_direction": torch.sqrt((derr * derr).mean()).item(),
+    out = {
+        "rms_spot_px": torch.sqrt((err * err).mean()).item() if err.numel() else None,
+        "max_spot_px": err.max().item() if err.numel() else None,
+        "rms_direction": torch.sqrt((derr * derr).mean()).item() if derr.numel() else None,
         "mask_agreement": (m64 == m32).double().mean().item(),
         "rays_compared": int(both.sum().item()),
     }
+    if isinstance(r64.intensity, torch.Tensor) and err.numel():
+        i64 = r64.intensity[both]
+        i32 = r32.intensity[both].double()
+        rel = torch.abs(i32 - i64) / torch.clamp(torch.abs(i64), min=1e-30)
+        out["rms_intensity_rel"] = torch.sqrt((rel * rel).mean()).item()
+    return out
+
+
+def header(args, metric_value, ms, dtype, config, forms, roofline, cpu) -> dict:
+    return {
+        "metric": "rays·surfaces/s on 1 MI355X: FP64 vs FP32 vs surrogate; RMS spot error vs FP64",
+        "value": metric_value,
+        "unit": "rays·surfaces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": config,
+        "forms": forms,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+
+
+def calibrate_copy(dev):
+    import torch
+
+    src = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def two_forms(args, system, grid, n, surfaces, paths, kernel32, kernel64, traffic32=None,
+              traffic64=None):
+    """Time the float64 and float32 forms of a grid trace on the same
+    (float32-representable) inputs and compare them over every ray."""
+    import torch
+
+    from optika_amd import _trace
+
+    dev = torch.device("cuda")
+    t32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev)  # noqa: E731
+    in32 = [t32(grid.field_x), t32(grid.field_y), t32(grid.pupil_x), t32(grid.pupil_y),
+            t32(grid.wavelength)]
+    in64 = [t.double() for t in in32]
+    plan64 = _trace.GridTracePlan(system, n, sensor_frame=True, precision="f64")
+    plan32 = _trace.GridTracePlan(system, n, sensor_frame=True, precision="f32")
+    stream = torch.cuda.current_stream()
+    ms64, k64 = time_calls(lambda: plan64(*in64), args.steps, args.warmup, stream)
+    log(f"f64 trace: {ms64:.3f} ms/step, kernel {k64:.3f} ms")
+    ms32, k32 = time_calls(lambda: plan32(*in32), args.steps, args.warmup, stream)
+    log(f"f32 trace: {ms32:.3f} ms/step, kernel {k32:.3f} ms")
+    r64, r32 = plan64(*in64), plan32(*in32)
+    errs = []
+    for p in range(paths):
+        a = r64 if paths == 1 else r64[p]
+        b = r32 if paths == 1 else r32[p]
+        errs.append(spot_error(a, b))
+    torch.cuda.synchronize()
+    log(f"f32 vs f64: {errs}")
+    units = n * paths * surfaces
+    value64 = units / (ms64 / 1e3)
+    value32 = units / (ms32 / 1e3)
+    achieved = BYTES_PER_RAY_F32 * n * paths / (k32 / 1e3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic32,
+        "kernel": kernel32,
+        "kernel_avg_ms": round(k32, 4),
+        "bytes_per_ray": BYTES_PER_RAY_F32,
+    }
+    f64_achieved = BYTES_PER_RAY_F64 * n * paths / (k64 / 1e3) / 1e9
+    err = errs[0] if paths == 1 else {f"path{p}": e for p, e in enumerate(errs)}
+    forms = {
+        "f64": {"value": value64, "ms_per_step": ms64, "kernel_avg_ms": round(k64, 4),
+                "kernel": kernel64, "bytes_per_ray": BYTES_PER_RAY_F64,
+                "achieved_gbs": round(f64_achieved, 1), "traffic": traffic64,
+                "note": "drop-in form: oracle parity 1e-12, identical masks"},
+        "f32": {"value": value32, "ms_per_step": ms32, "kernel_avg_ms": round(k32, 4),
+                "speedup_vs_f64": value32 / value64, "error_vs_f64": err},
+        "surrogate": None,
+    }
+    return value32, ms32, forms, roofline
+
+
+def run_newtonian(args):
+    from optika_amd import workloads
+
+    n = args.rays or (1 << 24)
+    system = workloads.newtonian()
+    S = workloads.NEWTONIAN_SURFACES
+    log(f"generating {n} rays")
+    grid = workloads.newtonian_grid_random(n, seed=1)
+    is_2p24 = n == 1 << 24
+    value, ms, forms, roofline = two_forms(
+        args, system, grid, n, S, 1, "trace_grid_f32<0, false>", "trace_grid_f64<0, false>",
+        TRAFFIC_BYTES_F32_2P24 if is_2p24 else None, TRAFFIC_BYTES_F64_2P24 if is_2p24 else None)
+    if args.calibrate_copy:
+        import torch
+
+        calibrate_copy(torch.device("cuda"))
+    cpu = None if args.no_cpu else cpu_baseline(system, grid, min(args.cpu_rays or 1 << 18, n), S)
+    config = {
+        "workload": "newtonian telescope rayfunction, 2^24 rays (pupil uniform on r=100 mm "
+                    "disc, field +-0.25 deg, 400-700 nm, seed 1)" if is_2p24 else
+                    f"newtonian telescope rayfunction, {n} rays",
+        "rays": n,
+        "surfaces": S,
+        "form": "float32 compensated form (float32 I/O); accuracy vs the float64 form in "
+                "forms.f32",
+    }
+    return header(args, value, ms, "f32", config, forms, roofline, cpu)
+
+
+def run_spectrograph(args):
+    from optika_amd import workloads
+
+    n = args.rays or 50_000_000
+    system = workloads.spectrograph()
+    S = workloads.SPECTROGRAPH_SURFACES
+    log(f"generating {n} rays")
+    grid = workloads.spectrograph_grid(n, seed=2)
+    value, ms, forms, roofline = two_forms(args, system, grid, n, S, 2,
+                                           "trace_grid_f32<7, false>", "trace_grid_f64<7, false>")
+    cpu = None if args.no_cpu else cpu_baseline(system, grid, min(args.cpu_rays or 1 << 12, n),
+                                                S, paths=2)
+    config = {
+        "workload": f"EUV spectrograph rayfunction, {n} rays x 2 diffraction orders (pupil "
+                    "r=40 mm disc, field +-0.05 deg, 58-63 nm, seed 2), Mo/Si 40-period "
+                    "multilayers on primary and grating",
+        "rays": n,
+        "paths": 2,
+        "surfaces": S,
+        "form": "float32 compensated form",
+    }
+    return header(args, value, ms, "f32", config, forms, roofline, cpu)
+
+
+def run_tolerance(args):
+    import torch
+
+    from optika_amd import BatchedSystem, workloads
+
+    N = args.instances
+    log(f"building {N} perturbed spectrographs")
+    systems, counts, grid = workloads.tolerance_batch(N, seed=3)
+    n = int(counts.sum())
+    S = workloads.SPECTROGRAPH_SURFACES
+    P = 2
+    dev = torch.device("cuda")
+    t32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev)  # noqa: E731
+    in32 = [t32(grid.field_x), t32(grid.field_y), t32(grid.pupil_x), t32(grid.pupil_y),
+            t32(grid.wavelength)]
+    in64 = [t.double() for t in in32]
+    b = BatchedSystem(systems)
+    log(f"{N} instances, {n} rays; autotuning")
+    tune32 = b.autotune(*in32, counts=counts, precision="f32")
+    log(f"f32 tune: block {tune32.block}, rays/thread {tune32.rays_per_thread}, {tune32.ms:.3f} ms")
+    tune64 = b.autotune(*in64, counts=counts, precision="f64")
+    log(f"f64 tune: block {tune64.block}, rays/thread {tune64.rays_per_thread}, {tune64.ms:.3f} ms")
+    stats32 = b.stats_tensor()
+    stats64 = b.stats_tensor()
+    stream = torch.cuda.current_stream()
+    ms32, _ = time_calls(lambda: b.trace(*in32, counts=counts, precision="f32", graph=True,
+                                         stats=stats32), args.steps, args.warmup, stream)
+    log(f"f32 batch: {ms32:.3f} ms/step")
+    ms64, _ = time_calls(lambda: b.trace(*in64, counts=counts, precision="f64", graph=True,
+                                         stats=stats64), max(2, args.steps // 4), 1, stream)
+    log(f"f64 batch: {ms64:.3f} ms/step")
+    s32, s64 = stats32.cpu().numpy(), stats64.cpu().numpy()
+    live = s64[:, 0, 0] > 0
+    rms_err_um = np.abs(s32[live, 0, 6] - s64[live, 0, 6]) * 1e3
+    thr_rel = np.abs(s32[live, 0, 7] - s64[live, 0, 7]) / np.maximum(s64[live, 0, 7], 1e-30)
+    units = n * P * S
+    value32 = units / (ms32 / 1e3)
+    value64 = units / (ms64 / 1e3)
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        k = 0
+        sample = 0
+        while k < N and sample < (args.cpu_rays or 8000):
+            sample += int(counts[k])
+            k += 1
+        t0 = time.perf_counter()
+        off = 0
+        for i in range(k):
+            sub = grid.subset(int(counts[i]), start=off)
+            oracle.rayfunction(systems[i], sub.wavelength, sub.field_x, sub.field_y, sub.pupil_x,
+                               sub.pupil_y)
+            off += int(counts[i])
+        dt = time.perf_counter() - t0
+        cpu = {"value": sample * P * S / dt, "unit": "rays·surfaces/s", "cores": 1, "kind": "port",
+               "sample": f"first {k} instances ({sample} rays, both orders), numpy float64 "
+                         f"oracle, one pass, {os.cpu_count()} host CPUs present, 1 thread used"}
+    forms = {
+        "f64": {"value": value64, "ms_per_step": ms64,
+                "tune": {"block": tune64.block, "rays_per_thread": tune64.rays_per_thread}},
+        "f32": {"value": value32, "ms_per_step": ms32, "speedup_vs_f64": value32 / value64,
+                "tune": {"block": tune32.block, "rays_per_thread": tune32.rays_per_thread},
+                "spot_rms_abs_err_um_max": float(rms_err_um.max()),
+                "throughput_rel_err_max": float(thr_rel.max())},
+        "surrogate": None,
+        "per_instance_spot_rms_um": {
+            "median": float(np.median(s64[live, 0, 6]) * 1e3),
+            "p95": float(np.percentile(s64[live, 0, 6], 95) * 1e3)},
+    }
+    config = {
+        "workload": f"tolerance batch: {N} perturbed EUV spectrographs (decenter 50 um, tilt "
+                    "0.01 deg, radii 0.1 %, line spacing 0.5 %, seed 3), ray counts log-uniform "
+                    f"1k-64k ({n} rays) x 2 orders, one persistent launch, hipGraph replay",
+        "instances": N,
+        "rays": n,
+        "paths": P,
+        "surfaces": S,
+        "form": "float32 compensated form, autotuned",
+    }
+    return header(args, value32, ms32, "f32", config, forms, None, cpu)
 
 
 def main():
@@ -139,96 +384,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(rays shard with no exchange; run one process per GPU)", file=sys.stderr)
         sys.exit(2)
-
-    import torch
-
-    from optika_amd import _trace, workloads
-
-    dev = torch.device("cuda")
-    n = args.rays
-    system = workloads.newtonian()
-    surfaces_counted = workloads.NEWTONIAN_SURFACES
-    print(f"[bench] generating {n} rays", file=sys.stderr, flush=True)
-    grid = workloads.newtonian_grid_random(n, seed=1)
-    # float32 inputs; the float64 form traces the same values widened
-    t32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev)  # noqa: E731
-    in32 = [t32(grid.field_x), t32(grid.field_y), t32(grid.pupil_x), t32(grid.pupil_y),
-            t32(grid.wavelength)]
-    in64 = [t.double() for t in in32]
-    plan64 = _trace.GridTracePlan(system, n, sensor_frame=True, precision="f64")
-    plan32 = _trace.GridTracePlan(system, n, sensor_frame=True, precision="f32")
-    stream = torch.cuda.current_stream()
-
-    ms64, k64 = _time_form(plan64, in64, args.steps, args.warmup, stream)
-    print(f"[bench] f64 trace: {ms64:.3f} ms/step, kernel {k64:.3f} ms", file=sys.stderr, flush=True)
-    ms32, k32 = _time_form(plan32, in32, args.steps, args.warmup, stream)
-    print(f"[bench] f32 trace: {ms32:.3f} ms/step, kernel {k32:.3f} ms", file=sys.stderr, flush=True)
-    err = _spot_error(plan64(*in64), plan32(*in32))
-    torch.cuda.synchronize()
-    print(f"[bench] f32 vs f64: {err}", file=sys.stderr, flush=True)
-
-    value64 = n * surfaces_counted / (ms64 / 1e3)
-    value32 = n * surfaces_counted / (ms32 / 1e3)
-    achieved = BYTES_PER_RAY_F32 * n / (k32 / 1e3) / 1e9
-    roofline = {
-        "bound": "hbm",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": TRAFFIC_BYTES_F32_2P24 if n == 1 << 24 else None,
-        "kernel": "trace_grid_f32",
-        "kernel_avg_ms": round(k32, 4),
-        "bytes_per_ray": BYTES_PER_RAY_F32,
-    }
-
-    if args.calibrate_copy:
-        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(system, grid, min(args.cpu_rays, n))
-
-    f64_achieved = BYTES_PER_RAY_F64 * n / (k64 / 1e3) / 1e9
-    out = {
-        "metric": "rays·surfaces/s on 1 MI355X: FP64 vs FP32 vs surrogate; RMS spot error vs FP64",
-        "value": value32,
-        "unit": "rays·surfaces/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms32,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": "newtonian telescope rayfunction, 2^24 rays (pupil uniform on r=100 mm "
-                        "disc, field +-0.25 deg, 400-700 nm, seed 1)",
-            "rays": n,
-            "surfaces": surfaces_counted,
-            "form": "float32 compensated form (float32 I/O); accuracy vs the float64 form "
-                    "in forms.f32",
-        },
-        "forms": {
-            "f64": {"value": value64, "ms_per_step": ms64, "kernel_avg_ms": round(k64, 4),
-                    "kernel": "trace_grid_f64", "bytes_per_ray": BYTES_PER_RAY_F64,
-                    "achieved_gbs": round(f64_achieved, 1),
-                    "traffic": TRAFFIC_BYTES_F64_2P24 if n == 1 << 24 else None,
-                    "note": "drop-in form: oracle parity 1e-12, identical masks"},
-            "f32": {"value": value32, "ms_per_step": ms32, "kernel_avg_ms": round(k32, 4),
-                    "speedup_vs_f64": value32 / value64, **err},
-            "surrogate": None,
-        },
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+    run = {"newtonian": run_newtonian, "spectrograph": run_spectrograph,
+           "tolerance": run_tolerance}[args.workload]
+    print(json.dumps(run(args)), flush=True)
 
 
 if __name__ == "__main__":
