@@ -402,6 +402,57 @@ int32_t optk_batch_autotune(optk_batch *batch, int32_t precision, const optk_gri
                             const int64_t *offsets, double *stats, int32_t flags,
                             void *hip_stream, optk_tune *best);
 
+/* Learned surrogate (no counterpart in the reference: it stands in for the
+ * trace of one system, which produces its training data and judges it).
+ * Kernels for a 5 -> 256 -> 256 -> 256 -> 256 -> 3 SiLU MLP with bf16
+ * weights/activations and float32 accumulation on MFMA; the host side
+ * (optika_amd/surrogate.py) sequences them. */
+
+/* C[m][n] = sum_k A(m, k) B(k, n); A, B bf16 (uint16 bits), C float32.
+ * A(m, k) = a[m * lda + k] (trans_a = 0) or a[k * lda + m] (trans_a = 1);
+ * B(k, n) = b[k * ldb + n] (trans_b = 0) or b[n * ldb + k] (trans_b = 1).
+ * m, n: multiples of 64; k: a multiple of 32 * split_k.  split_k > 1 sums
+ * partial products over k in `workspace` (split_k * m * n floats, ldc == n). */
+int32_t optk_gemm_bf16(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k,
+                       const void *a, int64_t lda, const void *b, int64_t ldb, float *c,
+                       int64_t ldc, int32_t split_k, float *workspace, void *hip_stream);
+
+/* elementwise and small-layer pieces of the forward and backward passes
+ * (argument use per op below; rows x cols row-major arrays) */
+enum {
+  OPTK_MLP_BIAS_SILU = 0,     /* f32[0] pre (+= bias, in place), f32[1] bias; u16[0] act = silu */
+  OPTK_MLP_DENSE_IN = 1,      /* f32[0] x [rows][inner], f32[1] w [cols][inner], f32[2] b,
+                                 f32[3] pre (or NULL); u16[0] act */
+  OPTK_MLP_DENSE_OUT = 2,     /* u16[0] act [rows][cols], f32[0] w [inner][cols], f32[1] b,
+                                 f32[2] y [rows][inner] */
+  OPTK_MLP_MSE_GRAD = 3,      /* f32[0] y, f32[1] target, f32[2] weight per column, f32[3] dy,
+                                 f32[4] loss partial per 256 elements */
+  OPTK_MLP_DENSE_OUT_BACK = 4,/* f32[0] dy [rows][inner], f32[1] w [inner][cols], f32[2] pre;
+                                 u16[0] dpre (bf16), f32[3] dpre (float32) */
+  OPTK_MLP_SILU_BACK = 5,     /* f32[0] dact, f32[1] pre; u16[0] dpre, f32[2] dpre (float32) */
+  OPTK_MLP_OUTER_GRAD = 6,    /* out = sum_r x[r][c] g[r][j]: f32[0] x (or u16[0] bf16 x),
+                                 f32[1] g [rows][inner], f32[2] out ([inner][cols], or
+                                 [cols][inner] with flag = 1), f32[3] workspace of count floats */
+  OPTK_MLP_COLUMN_SUM = 7,    /* f32[0] g [rows][cols]; f32[1] sums [cols] */
+  OPTK_MLP_ADAM = 8,          /* f32[0] params, f32[1] grads, f32[2] m, f32[3] v (count each);
+                                 u16[0] bf16 copy of params (or NULL);
+                                 scalars: lr, beta1, beta2, eps, 1 - beta1^t, 1 - beta2^t */
+  OPTK_MLP_TO_BF16 = 9        /* f32[0] x -> u16[0] (rows * cols) */
+};
+typedef struct optk_mlp_args {
+  float *f32[8];
+  uint16_t *u16[4];
+  int64_t rows;
+  int64_t count;
+  int32_t cols;
+  int32_t inner;
+  int32_t flag;
+  int32_t pad;
+  float scalars[8];
+} optk_mlp_args;
+
+int32_t optk_mlp_op(int32_t op, const optk_mlp_args *args, void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

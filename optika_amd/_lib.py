@@ -195,6 +195,23 @@ class Tune(ctypes.Structure):
                 ("pad", c_int32), ("ms", c_double)]
 
 
+class MlpArgs(ctypes.Structure):
+    _fields_ = [("f32", c_void_p * 8), ("u16", c_void_p * 4), ("rows", c_int64), ("count", c_int64),
+                ("cols", c_int32), ("inner", c_int32), ("flag", c_int32), ("pad", c_int32),
+                ("scalars", ctypes.c_float * 8)]
+
+
+MLP_BIAS_SILU = 0
+MLP_DENSE_IN = 1
+MLP_DENSE_OUT = 2
+MLP_MSE_GRAD = 3
+MLP_DENSE_OUT_BACK = 4
+MLP_SILU_BACK = 5
+MLP_OUTER_GRAD = 6
+MLP_COLUMN_SUM = 7
+MLP_ADAM = 8
+MLP_TO_BF16 = 9
+
 BATCH_STATS = 8
 BATCH_GRAPH = 2
 
@@ -246,13 +263,19 @@ def lib() -> ctypes.CDLL:
     L.optk_batch_autotune.restype = c_int32
     L.optk_batch_autotune.argtypes = [c_void_p, c_int32, ctypes.POINTER(Grid), ctypes.POINTER(c_int64),
                                       c_void_p, c_int32, c_void_p, ctypes.POINTER(Tune)]
+    L.optk_gemm_bf16.restype = c_int32
+    L.optk_gemm_bf16.argtypes = [c_int32, c_int32, c_int64, c_int64, c_int64, c_void_p, c_int64,
+                                 c_void_p, c_int64, c_void_p, c_int64, c_int32, c_void_p, c_void_p]
+    L.optk_mlp_op.restype = c_int32
+    L.optk_mlp_op.argtypes = [c_int32, ctypes.POINTER(MlpArgs), c_void_p]
     if L.optk_abi_version() != 1:
         raise OptkError(f"liboptk ABI {L.optk_abi_version()} != 1")
     for name, cls in [("optk_affine", Affine), ("optk_surface", Surface), ("optk_layer", Layer),
                       ("optk_block", Block), ("optk_stack", Stack), ("optk_chemical", Chemical),
                       ("optk_system_desc", SystemDesc), ("optk_rays", Rays), ("optk_grid", Grid),
                       ("optk_eval_args", EvalArgs), ("optk_collect_args", CollectArgs),
-                      ("optk_batch_desc", BatchDesc), ("optk_tune", Tune)]:
+                      ("optk_batch_desc", BatchDesc), ("optk_tune", Tune),
+                      ("optk_mlp_args", MlpArgs)]:
         size = L.optk_sizeof(name.encode())
         if size != ctypes.sizeof(cls):
             raise OptkError(f"record {name}: C size {size} != binding size {ctypes.sizeof(cls)}")

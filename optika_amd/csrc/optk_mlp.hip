@@ -1,0 +1,433 @@
+// optk_mlp.hip -- kernels of the learned surrogate (a 5 -> 4 x 256 SiLU -> 3
+// MLP): a bf16 MFMA GEMM with float32 accumulation (gfx950
+// v_mfma_f32_16x16x32_bf16), the elementwise and small-layer kernels of the
+// forward and backward passes, Adam, and the fused inference kernel.
+//
+// The surrogate has no counterpart in the reference (it replaces traces of
+// the spectrograph by a learned map); the float64 trace generates its
+// training data and judges its error.
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "optk_internal.h"
+
+using namespace optk;
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// float -> bfloat16 bits, round to nearest even (NaN stays NaN)
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ float silu(float x) { return x / (1.0f + __expf(-x)); }
+
+// d silu / dx = s (1 + x (1 - s)), s = sigmoid(x)
+__device__ __forceinline__ float silu_grad(float x) {
+  float s = 1.0f / (1.0f + __expf(-x));
+  return s * (1.0f + x * (1.0f - s));
+}
+
+// ---------------------------------------------------------------------------
+// GEMM: C[m][n] = sum_k A(m, k) B(k, n), bf16 operands, float32 result.
+// A(m, k) = a[m * lda + k] (TA = 0) or a[k * lda + m] (TA = 1);
+// B(k, n) = b[k * ldb + n] (TB = 0) or b[n * ldb + k] (TB = 1).
+// Block tile 64 x 64 x 32, 4 waves each owning a 32 x 32 quarter as 2 x 2
+// MFMA 16x16x32 tiles.  Split-K: block z covers k in [z kc, (z + 1) kc) and
+// writes C + z * m * n (a partial sum, reduced by gemm_reduce).
+// Requires m, n multiples of 64 and k a multiple of 32 (checked on the host).
+// ---------------------------------------------------------------------------
+
+constexpr int BM = 64, BN = 64, BK = 32;
+
+template <int TA, int TB>
+__global__ __launch_bounds__(256) void gemm_bf16(const uint16_t *__restrict__ a, int64_t lda,
+                                                 const uint16_t *__restrict__ b, int64_t ldb,
+                                                 float *__restrict__ c, int64_t ldc, int64_t m,
+                                                 int64_t n, int64_t kc) {
+  // LDS tiles with k contiguous: As[row][k], Bs[col][k] (+8 pad against bank conflicts)
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM][BK + 8];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][BK + 8];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int64_t m0 = (int64_t)blockIdx.y * BM, n0 = (int64_t)blockIdx.x * BN;
+  const int64_t k_begin = (int64_t)blockIdx.z * kc;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  floatx4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  for (int64_t k0 = k_begin; k0 < k_begin + kc; k0 += BK) {
+    // stage: thread t covers row t / 4, k columns (t % 4) * 8 .. + 8
+    {
+      const int r = tid >> 2, kk = (tid & 3) * 8;
+      if (TA == 0) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(a + (m0 + r) * lda + k0 + kk);
+        *reinterpret_cast<uint4 *>(&As[r][kk]) = v;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) As[r][kk + j] = a[(k0 + kk + j) * lda + m0 + r];
+      }
+      if (TB == 1) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(b + (n0 + r) * ldb + k0 + kk);
+        *reinterpret_cast<uint4 *>(&Bs[r][kk]) = v;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) Bs[r][kk + j] = b[(k0 + kk + j) * ldb + n0 + r];
+      }
+    }
+    __syncthreads();
+    // A operand: lane holds A[row = lane % 16][k = 8 (lane / 16) .. + 8];
+    // B operand: lane holds B[k = 8 (lane / 16) .. + 8][col = lane % 16]
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    bf16x8 af[2], bfr[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      uint4 va = *reinterpret_cast<const uint4 *>(&As[wm + 16 * i + fr][fk]);
+      af[i] = __builtin_bit_cast(bf16x8, va);
+      uint4 vb = *reinterpret_cast<const uint4 *>(&Bs[wn + 16 * i + fr][fk]);
+      bfr[i] = __builtin_bit_cast(bf16x8, vb);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // result: lane holds C[row = 4 (lane / 16) + r][col = lane % 16] of each tile
+  float *cz = c + (int64_t)blockIdx.z * m * n;
+  const int64_t ld = gridDim.z > 1 ? n : ldc;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        const int64_t col = n0 + wn + 16 * j + (lane & 15);
+        cz[row * ld + col] = acc[i][j][r];
+      }
+}
+
+// c[i] = sum over splits of partial[s][i] (m * n elements)
+__global__ __launch_bounds__(256) void gemm_reduce(const float *__restrict__ partial, int splits,
+                                                   int64_t count, float *__restrict__ c) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float s = 0.0f;
+  for (int z = 0; z < splits; ++z) s += partial[(int64_t)z * count + i];
+  c[i] = s;
+}
+
+// ---------------------------------------------------------------------------
+// elementwise and small-layer kernels
+// ---------------------------------------------------------------------------
+
+// pre[r][o] += bias[o]; act[r][o] = bf16(silu(pre[r][o]))
+__global__ __launch_bounds__(256) void bias_silu(float *__restrict__ pre, const float *__restrict__ bias,
+                                                 uint16_t *__restrict__ act, int64_t rows, int cols) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * cols) return;
+  float v = pre[i] + bias[i % cols];
+  pre[i] = v;
+  act[i] = to_bf16(silu(v));
+}
+
+// first layer: pre[r][o] = sum_i x[r][i] w[o][i] + b[o] (float32, in < 8 inputs)
+__global__ __launch_bounds__(256) void dense_in(const float *__restrict__ x, int in,
+                                                const float *__restrict__ w, const float *__restrict__ bias,
+                                                float *__restrict__ pre, uint16_t *__restrict__ act,
+                                                int64_t rows, int cols) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * cols) return;
+  const int64_t r = i / cols;
+  const int o = (int)(i % cols);
+  float v = bias[o];
+  for (int k = 0; k < in; ++k) v += x[r * in + k] * w[o * in + k];
+  if (pre) pre[i] = v;
+  act[i] = to_bf16(silu(v));
+}
+
+// last layer: y[r][j] = sum_i act[r][i] w[j][i] + b[j]; one thread per row
+__global__ __launch_bounds__(256) void dense_out(const uint16_t *__restrict__ act, int hidden,
+                                                 const float *__restrict__ w, const float *__restrict__ bias,
+                                                 float *__restrict__ y, int out, int64_t rows) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  float acc[8];
+  for (int j = 0; j < out; ++j) acc[j] = bias[j];
+  for (int i = 0; i < hidden; ++i) {
+    float h = from_bf16(act[r * hidden + i]);
+    for (int j = 0; j < out; ++j) acc[j] += h * w[j * hidden + i];
+  }
+  for (int j = 0; j < out; ++j) y[r * out + j] = acc[j];
+}
+
+// weighted mean-square loss gradient: dy = 2 w_j (y - t) / rows; loss
+// partial sums per block into loss_partial[blockIdx.x]
+__global__ __launch_bounds__(256) void mse_grad(const float *__restrict__ y, const float *__restrict__ t,
+                                                const float *__restrict__ wj, float *__restrict__ dy,
+                                                int out, int64_t rows, float *__restrict__ loss_partial) {
+  __shared__ float red[256];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (i < rows * out) {
+    int j = (int)(i % out);
+    float d = y[i] - t[i];
+    dy[i] = 2.0f * wj[j] * d / (float)rows;
+    l = wj[j] * d * d / (float)rows;
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss_partial[blockIdx.x] = red[0];
+}
+
+// last layer backward, activations side:
+// dpre[r][i] = (sum_j dy[r][j] w[j][i]) * silu'(pre[r][i]) -> bf16
+__global__ __launch_bounds__(256) void dense_out_back(const float *__restrict__ dy, int out,
+                                                     const float *__restrict__ w,
+                                                     const float *__restrict__ pre,
+                                                     uint16_t *__restrict__ dpre, float *__restrict__ dpre32,
+                                                     int64_t rows, int hidden) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * hidden) return;
+  const int64_t r = i / hidden;
+  const int h = (int)(i % hidden);
+  float s = 0.0f;
+  for (int j = 0; j < out; ++j) s += dy[r * out + j] * w[j * hidden + h];
+  float g = s * silu_grad(pre[i]);
+  dpre[i] = to_bf16(g);
+  dpre32[i] = g;
+}
+
+// dpre = dact * silu'(pre) (float32 in, bf16 and float32 out)
+__global__ __launch_bounds__(256) void silu_back(const float *__restrict__ dact,
+                                                 const float *__restrict__ pre,
+                                                 uint16_t *__restrict__ dpre, float *__restrict__ dpre32,
+                                                 int64_t count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float g = dact[i] * silu_grad(pre[i]);
+  dpre[i] = to_bf16(g);
+  dpre32[i] = g;
+}
+
+// Partial outer products for the small layers' weight gradients: block b
+// sums rows [b R, (b + 1) R) of x[r][c] * g[r][j] for c < cols (one thread
+// per column, cols <= 256) and j < inner (<= 8); the partials are reduced
+// over blocks by gemm_reduce.  x is float32 (x32) or bf16 (x16).  Layout of
+// a partial: [j][c] (transposed = 0) or [c][j] (transposed = 1).
+__global__ __launch_bounds__(256) void outer_grad(const float *__restrict__ x32,
+                                                  const uint16_t *__restrict__ x16, int cols,
+                                                  const float *__restrict__ g, int inner,
+                                                  int64_t rows, int64_t rows_per_block,
+                                                  int transposed, float *__restrict__ partial) {
+  const int c = threadIdx.x;
+  if (c >= cols) return;
+  float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  int64_t r1 = r0 + rows_per_block;
+  if (r1 > rows) r1 = rows;
+  for (int64_t r = r0; r < r1; ++r) {
+    float xv = x32 ? x32[r * cols + c] : from_bf16(x16[r * cols + c]);
+    for (int j = 0; j < inner; ++j) acc[j] += xv * g[r * inner + j];
+  }
+  float *out = partial + (int64_t)blockIdx.x * cols * inner;
+  for (int j = 0; j < inner; ++j) {
+    if (transposed)
+      out[(int64_t)c * inner + j] = acc[j];
+    else
+      out[(int64_t)j * cols + c] = acc[j];
+  }
+}
+
+// column sums: gb[o] = sum_r g[r][o]
+__global__ __launch_bounds__(256) void column_sum(const float *__restrict__ g, int64_t rows, int cols,
+                                                  float *__restrict__ gb) {
+  __shared__ float red[256];
+  const int o = blockIdx.x;
+  float s = 0.0f;
+  for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) s += g[r * cols + o];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) gb[o] = red[0];
+}
+
+// Adam (Kingma & Ba 2015) with bias correction; also refreshes the bf16 copy
+__global__ __launch_bounds__(256) void adam(float *__restrict__ p, const float *__restrict__ g,
+                                            float *__restrict__ m1, float *__restrict__ m2,
+                                            uint16_t *__restrict__ p16, int64_t count, float lr,
+                                            float beta1, float beta2, float eps, float c1, float c2) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float gi = g[i];
+  float a = beta1 * m1[i] + (1.0f - beta1) * gi;
+  float b = beta2 * m2[i] + (1.0f - beta2) * gi * gi;
+  m1[i] = a;
+  m2[i] = b;
+  float v = p[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+  p[i] = v;
+  if (p16) p16[i] = to_bf16(v);
+}
+
+__global__ __launch_bounds__(256) void to_bf16_kernel(const float *__restrict__ x,
+                                                      uint16_t *__restrict__ y, int64_t count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) y[i] = to_bf16(x[i]);
+}
+
+unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+extern "C" {
+
+int32_t optk_gemm_bf16(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k,
+                       const void *a, int64_t lda, const void *b, int64_t ldb, float *c,
+                       int64_t ldc, int32_t split_k, float *workspace, void *hip_stream) {
+  if (!a || !b || !c) return fail(OPTK_EINVAL, "gemm: null operand");
+  if (m <= 0 || n <= 0 || k <= 0 || m % BM || n % BN) return fail(OPTK_EINVAL, "gemm: m and n must be positive multiples of 64");
+  if (split_k < 1) return fail(OPTK_EINVAL, "gemm: split_k must be >= 1");
+  if (k % ((int64_t)BK * split_k)) return fail(OPTK_EINVAL, "gemm: k must be a multiple of 32 * split_k");
+  if (split_k > 1 && !workspace) return fail(OPTK_EINVAL, "gemm: split_k > 1 needs a workspace of split_k * m * n floats");
+  if ((trans_a ? lda < m : lda < k) || (trans_b ? ldb < k : ldb < n) || ldc < n)
+    return fail(OPTK_EINVAL, "gemm: leading dimension too small");
+  if ((!trans_a && lda % 8) || (trans_b && ldb % 8)) return fail(OPTK_EINVAL, "gemm: contiguous operands need leading dimensions that are multiples of 8");
+  if ((!trans_a && ((uintptr_t)a % 16)) || (trans_b && ((uintptr_t)b % 16)))
+    return fail(OPTK_EINVAL, "gemm: operands must be 16-byte aligned");
+  hipStream_t stream = (hipStream_t)hip_stream;
+  dim3 grid((unsigned)(n / BN), (unsigned)(m / BM), (unsigned)split_k);
+  float *target = split_k > 1 ? workspace : c;
+  const int64_t kc = k / split_k;
+  const uint16_t *A = (const uint16_t *)a, *B = (const uint16_t *)b;
+  if (!trans_a && !trans_b)
+    hipLaunchKernelGGL((gemm_bf16<0, 0>), grid, dim3(256), 0, stream, A, lda, B, ldb, target, ldc, m, n, kc);
+  else if (!trans_a && trans_b)
+    hipLaunchKernelGGL((gemm_bf16<0, 1>), grid, dim3(256), 0, stream, A, lda, B, ldb, target, ldc, m, n, kc);
+  else if (trans_a && !trans_b)
+    hipLaunchKernelGGL((gemm_bf16<1, 0>), grid, dim3(256), 0, stream, A, lda, B, ldb, target, ldc, m, n, kc);
+  else
+    hipLaunchKernelGGL((gemm_bf16<1, 1>), grid, dim3(256), 0, stream, A, lda, B, ldb, target, ldc, m, n, kc);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && split_k > 1) {
+    if (ldc != n) return fail(OPTK_EINVAL, "gemm: split_k > 1 needs ldc == n");
+    hipLaunchKernelGGL(gemm_reduce, dim3(blocks_for(m * n)), dim3(256), 0, stream, workspace,
+                       split_k, m * n, c);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("gemm launch: ") + hipGetErrorString(e));
+  return OPTK_OK;
+}
+
+int32_t optk_mlp_op(int32_t op, const optk_mlp_args *A, void *hip_stream) {
+  if (!A) return fail(OPTK_EINVAL, "null argument");
+  hipStream_t s = (hipStream_t)hip_stream;
+  const int64_t rows = A->rows;
+  if (rows < 0 || A->cols < 0 || A->inner < 0 ||
+      (A->inner > 8 && (op == OPTK_MLP_DENSE_IN || op == OPTK_MLP_DENSE_OUT || op == OPTK_MLP_DENSE_OUT_BACK)))
+    return fail(OPTK_EINVAL, "mlp op: bad sizes");
+  const int64_t count = rows * A->cols;
+  if (count == 0 && op != OPTK_MLP_ADAM) return OPTK_OK;
+#define NEED(cond, msg)                     \
+  do {                                      \
+    if (!(cond)) return fail(OPTK_EINVAL, msg); \
+  } while (0)
+  switch (op) {
+  case OPTK_MLP_BIAS_SILU:
+    NEED(A->f32[0] && A->f32[1] && A->u16[0], "bias_silu: pre, bias, act");
+    hipLaunchKernelGGL(bias_silu, dim3(blocks_for(count)), dim3(256), 0, s, A->f32[0], A->f32[1],
+                       A->u16[0], rows, A->cols);
+    break;
+  case OPTK_MLP_DENSE_IN:
+    NEED(A->f32[0] && A->f32[1] && A->f32[2] && A->u16[0], "dense_in: x, w, b, act");
+    hipLaunchKernelGGL(dense_in, dim3(blocks_for(count)), dim3(256), 0, s, A->f32[0], A->inner,
+                       A->f32[1], A->f32[2], A->f32[3], A->u16[0], rows, A->cols);
+    break;
+  case OPTK_MLP_DENSE_OUT:
+    NEED(A->u16[0] && A->f32[0] && A->f32[1] && A->f32[2], "dense_out: act, w, b, y");
+    hipLaunchKernelGGL(dense_out, dim3(blocks_for(rows)), dim3(256), 0, s, A->u16[0], A->cols,
+                       A->f32[0], A->f32[1], A->f32[2], A->inner, rows);
+    break;
+  case OPTK_MLP_MSE_GRAD:
+    NEED(A->f32[0] && A->f32[1] && A->f32[2] && A->f32[3] && A->f32[4], "mse: y, t, w, dy, loss");
+    hipLaunchKernelGGL(mse_grad, dim3(blocks_for(count)), dim3(256), 0, s, A->f32[0], A->f32[1],
+                       A->f32[2], A->f32[3], A->cols, rows, A->f32[4]);
+    break;
+  case OPTK_MLP_DENSE_OUT_BACK:
+    NEED(A->f32[0] && A->f32[1] && A->f32[2] && A->u16[0] && A->f32[3], "dense_out_back: dy, w, pre, dpre, dpre32");
+    hipLaunchKernelGGL(dense_out_back, dim3(blocks_for(count)), dim3(256), 0, s, A->f32[0], A->inner,
+                       A->f32[1], A->f32[2], A->u16[0], A->f32[3], rows, A->cols);
+    break;
+  case OPTK_MLP_SILU_BACK:
+    NEED(A->f32[0] && A->f32[1] && A->u16[0] && A->f32[2], "silu_back: dact, pre, dpre, dpre32");
+    hipLaunchKernelGGL(silu_back, dim3(blocks_for(count)), dim3(256), 0, s, A->f32[0], A->f32[1],
+                       A->u16[0], A->f32[2], count);
+    break;
+  case OPTK_MLP_OUTER_GRAD: {
+    // f32[0] or u16[0]: x [rows][cols]; f32[1]: g [rows][inner];
+    // f32[2]: out (cols * inner); f32[3]: workspace (blocks * cols * inner)
+    NEED((A->f32[0] || A->u16[0]) && A->f32[1] && A->f32[2] && A->f32[3], "outer_grad: x, g, out, workspace");
+    NEED(A->cols <= 256 && A->inner >= 1 && A->inner <= 8, "outer_grad: cols <= 256, 1 <= inner <= 8");
+    const int64_t rpb = 256;
+    const int64_t nblk = (rows + rpb - 1) / rpb;
+    NEED(A->count >= nblk * A->cols * A->inner, "outer_grad: workspace too small");
+    hipLaunchKernelGGL(outer_grad, dim3((unsigned)nblk), dim3(256), 0, s, A->f32[0], A->u16[0],
+                       A->cols, A->f32[1], A->inner, rows, rpb, A->flag, A->f32[3]);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("outer_grad: ") + hipGetErrorString(e));
+    const int64_t cnt = (int64_t)A->cols * A->inner;
+    hipLaunchKernelGGL(gemm_reduce, dim3(blocks_for(cnt)), dim3(256), 0, s, A->f32[3], (int)nblk,
+                       cnt, A->f32[2]);
+    break;
+  }
+  case OPTK_MLP_COLUMN_SUM:
+    NEED(A->f32[0] && A->f32[1], "column_sum: g, gb");
+    hipLaunchKernelGGL(column_sum, dim3((unsigned)A->cols), dim3(256), 0, s, A->f32[0], rows,
+                       A->cols, A->f32[1]);
+    break;
+  case OPTK_MLP_ADAM: {
+    NEED(A->f32[0] && A->f32[1] && A->f32[2] && A->f32[3], "adam: p, g, m, v");
+    const int64_t nparam = A->count;
+    if (nparam <= 0) return OPTK_OK;
+    hipLaunchKernelGGL(adam, dim3(blocks_for(nparam)), dim3(256), 0, s, A->f32[0], A->f32[1],
+                       A->f32[2], A->f32[3], A->u16[0], nparam, A->scalars[0], A->scalars[1],
+                       A->scalars[2], A->scalars[3], A->scalars[4], A->scalars[5]);
+    break;
+  }
+  case OPTK_MLP_TO_BF16:
+    NEED(A->f32[0] && A->u16[0], "to_bf16: x, y");
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(count)), dim3(256), 0, s, A->f32[0],
+                       A->u16[0], count);
+    break;
+  default:
+    return fail(OPTK_EINVAL, "unknown mlp op");
+  }
+#undef NEED
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("mlp op launch: ") + hipGetErrorString(e));
+  return OPTK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,518 @@
+"""A learned surrogate of one system's ``rayfunction``.
+
+A small MLP (5 -> 256 -> 256 -> 256 -> 256 -> 3, SiLU, bf16 weights and
+activations, float32 accumulation on MFMA) maps (field x, y; pupil x, y;
+wavelength) to (sensor x, y; log intensity) for one diffraction-order path.
+It is trained on rays the float64 trace produces, by the library's own
+kernels (``optk_gemm_bf16``, ``optk_mlp_op``; Adam), and evaluated against
+the float64 trace.
+
+The network does not predict positions from scratch: a least-squares cubic
+polynomial in the five normalised inputs (the method without the network)
+carries the smooth part of the map, and the network learns the residual,
+scaled to unit variance.  The same polynomial predicts where each ray
+crosses every aperture; rays predicted within ``edge_margin`` (0.5 mm) of
+an aperture edge, or in an input-space cell where validation showed the
+surrogate above its error bound, fall back to the trace.
+
+No counterpart in the reference.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+
+import numpy as np
+import torch
+
+from . import _lib, _trace
+
+__all__ = ["Surrogate", "poly_features", "NUM_POLY"]
+
+WIDTHS = (5, 256, 256, 256, 256, 3)
+NUM_POLY = 56  # monomials of total degree <= 3 in 5 variables
+
+
+def _exponents(degree=3, nvar=5):
+    out = []
+
+    def rec(prefix, remaining, var):
+        if var == nvar:
+            out.append(tuple(prefix))
+            return
+        for e in range(remaining + 1):
+            rec(prefix + [e], remaining - e, var + 1)
+
+    rec([], degree, 0)
+    out.sort(key=lambda e: (sum(e), e[::-1]))
+    return out
+
+
+EXPONENTS = _exponents()
+assert len(EXPONENTS) == NUM_POLY
+
+
+def poly_features(u: torch.Tensor) -> torch.Tensor:
+    """Monomials of total degree <= 3 of the normalised inputs u [n][5]
+    (float64), in ``EXPONENTS`` order."""
+    cols = []
+    pw = [[torch.ones_like(u[:, v])] for v in range(5)]
+    for v in range(5):
+        for _ in range(3):
+            pw[v].append(pw[v][-1] * u[:, v])
+    for e in EXPONENTS:
+        c = torch.ones_like(u[:, 0])
+        for v, k in enumerate(e):
+            if k:
+                c = c * pw[v][k]
+        cols.append(c)
+    return torch.stack(cols, 1)
+
+
+def _param_layout(widths=WIDTHS):
+    """(offset, shape) of W and b of each layer in the flat parameter vector."""
+    lay, o = [], 0
+    for a, b in zip(widths[:-1], widths[1:]):
+        lay.append(((o, (b, a)), (o + a * b, (b,))))
+        o += a * b + b
+    return lay, o
+
+
+LAYOUT, NUM_PARAMS = _param_layout()
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return _trace.stream_handle()
+
+
+def _op(op, f32=(), u16=(), rows=0, cols=0, inner=0, count=0, flag=0, scalars=()):
+    a = _lib.MlpArgs()
+    for i, t in enumerate(f32):
+        a.f32[i] = None if t is None else t.data_ptr()
+    for i, t in enumerate(u16):
+        a.u16[i] = None if t is None else t.data_ptr()
+    a.rows, a.cols, a.inner, a.count, a.flag = rows, cols, inner, count, flag
+    for i, v in enumerate(scalars):
+        a.scalars[i] = v
+    _lib.check(_lib.lib().optk_mlp_op(op, ctypes.byref(a), _stream()))
+
+
+def _gemm(ta, tb, m, n, k, a, lda, b, ldb, c, ldc, split=1, ws=None):
+    _lib.check(_lib.lib().optk_gemm_bf16(ta, tb, m, n, k, _ptr(a), lda, _ptr(b), ldb, _ptr(c), ldc,
+                                         split, _ptr(ws), _stream()))
+
+
+class MLP:
+    """Parameters (float32 master + bf16 copy) and Adam state of the
+    network, with forward/backward passes on the library's kernels."""
+
+    def __init__(self, dev, seed: int = 0):
+        self.dev = dev
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        parts = []
+        for (wo, ws), (bo, bs) in LAYOUT:
+            bound = math.sqrt(6.0 / ws[1])
+            parts.append((torch.rand(ws, generator=g, dtype=torch.float64) * 2 - 1) * bound)
+            parts.append(torch.zeros(bs, dtype=torch.float64))
+        flat = torch.cat([p.reshape(-1) for p in parts]).to(torch.float32)
+        self.set_params(flat.to(dev))
+
+    def set_params(self, flat32: torch.Tensor):
+        assert flat32.numel() == NUM_PARAMS
+        self.p = flat32.contiguous().clone()
+        self.p16 = torch.empty(NUM_PARAMS, dtype=torch.int16, device=self.dev)
+        _op(_lib.MLP_TO_BF16, f32=[self.p], u16=[self.p16], rows=1, cols=NUM_PARAMS)
+        self.m = torch.zeros_like(self.p)
+        self.v = torch.zeros_like(self.p)
+        self.g = torch.zeros_like(self.p)
+        self.t = 0
+
+    def view(self, flat, layer, which):
+        off, shape = LAYOUT[layer][which]
+        return flat[off:off + int(np.prod(shape))]
+
+    # -- forward ---------------------------------------------------------
+    def _buffers(self, rows, train):
+        key = (rows, train)
+        if getattr(self, "_buf_key", None) != key:
+            f = lambda *s: torch.empty(s, dtype=torch.float32, device=self.dev)  # noqa: E731
+            h = lambda *s: torch.empty(s, dtype=torch.int16, device=self.dev)  # noqa: E731
+            H = WIDTHS[1]
+            self.pre = [f(rows, H) for _ in range(4)]
+            self.act = [h(rows, H) for _ in range(4)]
+            self.y = f(rows, 3)
+            if train:
+                self.dy = f(rows, 3)
+                self.dact = f(rows, H)
+                self.dpre16 = h(rows, H)
+                self.dpre32 = f(rows, H)
+                self.ones = torch.ones(rows, 1, dtype=torch.float32, device=self.dev)
+                split = max(1, min(64, rows // 1024))
+                while rows % (32 * split):
+                    split //= 2
+                self.split = split
+                self.ws = f(max(split * H * H, ((rows + 255) // 256) * H * 8))
+                self.loss_part = f((rows * 3 + 255) // 256)
+            self._buf_key = key
+
+    def forward(self, x: torch.Tensor, train: bool = False) -> torch.Tensor:
+        """x: [rows][5] float32 (normalised inputs), rows a multiple of 64."""
+        rows = x.shape[0]
+        assert rows % 64 == 0 and x.dtype == torch.float32 and x.is_contiguous()
+        self._buffers(rows, train)
+        H = WIDTHS[1]
+        _op(_lib.MLP_DENSE_IN, f32=[x, self.view(self.p, 0, 0), self.view(self.p, 0, 1),
+                                    self.pre[0]],
+            u16=[self.act[0]], rows=rows, cols=H, inner=5)
+        for layer in (1, 2, 3):
+            w16 = self.view(self.p16, layer, 0)
+            _gemm(0, 1, rows, H, H, self.act[layer - 1], H, w16, H, self.pre[layer], H)
+            _op(_lib.MLP_BIAS_SILU, f32=[self.pre[layer], self.view(self.p, layer, 1)],
+                u16=[self.act[layer]], rows=rows, cols=H)
+        _op(_lib.MLP_DENSE_OUT, f32=[self.view(self.p, 4, 0), self.view(self.p, 4, 1), self.y],
+            u16=[self.act[3]], rows=rows, cols=H, inner=3)
+        return self.y
+
+    # -- backward --------------------------------------------------------
+    def loss_grad(self, x, target, weight) -> torch.Tensor:
+        """Forward, weighted-MSE loss and gradients into ``self.g``;
+        returns the loss (a device scalar)."""
+        rows = x.shape[0]
+        H = WIDTHS[1]
+        self.forward(x, train=True)
+        _op(_lib.MLP_MSE_GRAD, f32=[self.y, target, weight, self.dy, self.loss_part], rows=rows,
+            cols=3)
+        g = self.g
+        # output layer
+        _op(_lib.MLP_OUTER_GRAD, f32=[None, self.dy, self.view(g, 4, 0), self.ws],
+            u16=[self.act[3]], rows=rows, cols=H, inner=3, count=self.ws.numel(), flag=0)
+        _op(_lib.MLP_COLUMN_SUM, f32=[self.dy, self.view(g, 4, 1)], rows=rows, cols=3)
+        _op(_lib.MLP_DENSE_OUT_BACK, f32=[self.dy, self.view(self.p, 4, 0), self.pre[3],
+                                          self.dpre32], u16=[self.dpre16], rows=rows, cols=H,
+            inner=3)
+        for layer in (3, 2, 1):
+            # weight gradient: gW[o][i] = sum_r dpre[r][o] act[r][i]
+            _gemm(1, 0, H, H, rows, self.dpre16, H, self.act[layer - 1], H, self.view(g, layer, 0),
+                  H, self.split, self.ws)
+            _op(_lib.MLP_OUTER_GRAD, f32=[self.dpre32, self.ones, self.view(g, layer, 1), self.ws],
+                rows=rows, cols=H, inner=1, count=self.ws.numel(), flag=1)
+            # back to the previous layer's activations
+            _gemm(0, 0, rows, H, H, self.dpre16, H, self.view(self.p16, layer, 0), H, self.dact, H)
+            _op(_lib.MLP_SILU_BACK, f32=[self.dact, self.pre[layer - 1], self.dpre32],
+                u16=[self.dpre16], rows=rows, cols=H)
+        # first layer (float32)
+        _op(_lib.MLP_OUTER_GRAD, f32=[self.dpre32, x, self.view(g, 0, 0), self.ws], rows=rows,
+            cols=H, inner=5, count=self.ws.numel(), flag=1)
+        _op(_lib.MLP_OUTER_GRAD, f32=[self.dpre32, self.ones, self.view(g, 0, 1), self.ws],
+            rows=rows, cols=H, inner=1, count=self.ws.numel(), flag=1)
+        return self.loss_part.sum()
+
+    def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.t += 1
+        c1 = 1.0 - beta1 ** self.t
+        c2 = 1.0 - beta2 ** self.t
+        _op(_lib.MLP_ADAM, f32=[self.p, self.g, self.m, self.v], u16=[self.p16],
+            count=NUM_PARAMS, scalars=[lr, beta1, beta2, eps, c1, c2])
+
+
+@dataclasses.dataclass
+class Normalisation:
+    lo: np.ndarray  # input ranges (field x, y deg; pupil x, y mm; wavelength nm)
+    hi: np.ndarray
+
+    def apply(self, cols):
+        mid = (self.lo + self.hi) / 2
+        half = (self.hi - self.lo) / 2
+        return [(c - float(mid[i])) / float(half[i]) for i, c in enumerate(cols)]
+
+
+# ---------------------------------------------------------------------------
+# apertures: signed distance to the edge (positive inside)
+# ---------------------------------------------------------------------------
+
+@dataclasses.dataclass
+class ApertureEdge:
+    """One active aperture of the system, in the frame where it clips."""
+
+    surface: int        # index into surfaces_all
+    kind: int           # OPTK_APERTURE_*
+    params: np.ndarray  # aperture_p
+    vertices: np.ndarray | None  # polygon [nv][2]
+    inverted: bool
+    m: np.ndarray       # system frame -> aperture frame: p_ap = m @ p + t
+    t: np.ndarray
+
+    def distance(self, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
+        if self.kind == 1:  # circle
+            d = float(self.params[0]) - torch.sqrt(x * x + y * y)
+        elif self.kind == 4:  # rectangle
+            d = torch.minimum(float(self.params[0]) - torch.abs(x), float(self.params[1]) - torch.abs(y))
+        elif self.kind == 5:  # polygon: distance to the boundary, signed by the crossing rule
+            v = self.vertices
+            dmin = None
+            inside = torch.zeros_like(x, dtype=torch.bool)
+            nv = len(v)
+            for i in range(nv):
+                x1, y1 = float(v[i - 1][0]), float(v[i - 1][1])
+                x2, y2 = float(v[i][0]), float(v[i][1])
+                ex, ey = x2 - x1, y2 - y1
+                L2 = ex * ex + ey * ey
+                s = torch.clamp(((x - x1) * ex + (y - y1) * ey) / L2, 0.0, 1.0)
+                dx, dy = x - (x1 + s * ex), y - (y1 + s * ey)
+                dd = torch.sqrt(dx * dx + dy * dy)
+                dmin = dd if dmin is None else torch.minimum(dmin, dd)
+                cond = (y1 > y) != (y2 > y)
+                xc = (x2 - x1) * (y - y1) / (y2 - y1 if y2 != y1 else 1e-300) + x1
+                inside = inside ^ (cond & (x < xc))
+            d = torch.where(inside, dmin, -dmin)
+        else:  # sectors and ellipses: no distance model, always near an edge
+            d = torch.zeros_like(x)
+        return -d if self.inverted else d
+
+
+def _aperture_edges(system) -> list[ApertureEdge]:
+    from . import _compile
+
+    rec = _compile.build_records(system.surfaces_all)
+    edges = []
+    for k in range(rec.num_surfaces):
+        s = rec.surfaces[k]
+        if s.aperture_kind == 0 or not (s.aperture_flags & 1) or (s.aperture_flags & 4):
+            continue
+
+        def mt(a):
+            if a.kind == 0:
+                return np.eye(3), np.zeros(3)
+            m = np.array(a.m[:]).reshape(3, 3) if a.kind == 2 else np.eye(3)
+            return m, np.array(a.t[:])
+
+        mT, tT = mt(s.transform)
+        mA, tA = mt(s.aperture_transform)
+        # system -> surface: mT^T (p - tT); surface -> aperture: mA^T (q - tA)
+        m = mA.T @ mT.T
+        t = -mA.T @ (mT.T @ tT) - mA.T @ tA
+        verts = None
+        if s.aperture_kind == 5:
+            tab = rec.tables
+            o, c = s.polygon_offset, s.polygon_count
+            verts = np.stack([tab[o:o + c], tab[o + c:o + 2 * c]], 1)
+        edges.append(ApertureEdge(k, int(s.aperture_kind), np.array(s.aperture_p[:]), verts,
+                                  bool(s.aperture_flags & 2), m, t))
+    return edges
+
+
+# ---------------------------------------------------------------------------
+# the surrogate
+# ---------------------------------------------------------------------------
+
+class Surrogate:
+    """Learned surrogate of ``system.rayfunction`` for diffraction-order
+    path ``path``, over the input box [lo, hi] (field x, y in degrees;
+    pupil x, y in mm; wavelength in nm)."""
+
+    def __init__(self, system, lo, hi, path: int = 0, pixel: float = 0.015,
+                 edge_margin: float = 0.5, bound_px: float = 0.05, seed: int = 0):
+        self.system = system
+        self.path = path
+        self.norm = Normalisation(np.asarray(lo, float), np.asarray(hi, float))
+        self.pixel = pixel
+        self.edge_margin = edge_margin
+        self.bound_px = bound_px
+        self.dev = _trace.device()
+        self.mlp = MLP(self.dev, seed)
+        self.edges = _aperture_edges(system)
+        self.coef = None       # polynomial coefficients [NUM_POLY][targets]
+        self.scale = None      # residual scales of the three network targets
+        self.err_map = None    # [16][8][8] flags of input cells above the bound
+        self.relevant = []     # apertures that come within reach of the rays
+
+    # -- data ------------------------------------------------------------
+    def normalised(self, fx, fy, px, py, wl) -> torch.Tensor:
+        u = self.norm.apply([fx.double(), fy.double(), px.double(), py.double(), wl.double()])
+        return torch.stack(u, 1)
+
+    def trace64(self, fx, fy, px, py, wl) -> dict:
+        """The float64 trace: sensor-frame position and intensity of path
+        ``path``, plus the aperture-frame position at every aperture."""
+        from .vectors import Cartesian2dVectorArray
+
+        kw = dict(normalized_field=False, normalized_pupil=False, precision="f64")
+        rf = self.system.rayfunction(wavelength=wl, field=Cartesian2dVectorArray(fx, fy),
+                                     pupil=Cartesian2dVectorArray(px, py), **kw).outputs
+        acc = self.system.raytrace(wavelength=wl, field=Cartesian2dVectorArray(fx, fy),
+                                   pupil=Cartesian2dVectorArray(px, py), accumulate=True,
+                                   **kw).outputs
+        r = rf if rf.position.x.ndim == 1 else rf[self.path]
+        out = {"x": r.position.x, "y": r.position.y, "intensity": r.intensity,
+               "unvignetted": r.unvignetted, "ap": []}
+        for e in self.edges:
+            # accumulate axis 0 = surface, then path
+            s = acc[e.surface]
+            s = s if s.position.x.ndim == 1 else s[self.path]
+            p = torch.stack([s.position.x, s.position.y, s.position.z], 0)
+            m = torch.as_tensor(e.m, device=p.device)
+            t = torch.as_tensor(e.t, device=p.device)
+            q = m @ p + t[:, None]
+            out["ap"].append((q[0], q[1]))
+        return out
+
+    def _targets(self, d: dict) -> torch.Tensor:
+        cols = [d["x"], d["y"], torch.log(torch.clamp(d["intensity"], min=1e-300))]
+        for ax, ay in d["ap"]:
+            cols += [ax, ay]
+        return torch.stack(cols, 1)
+
+    # -- fitting -----------------------------------------------------------
+    def fit_polynomial(self, u: torch.Tensor, targets: torch.Tensor, valid: torch.Tensor):
+        """Least squares of every target on the cubic monomials (normal
+        equations in float64)."""
+        F = poly_features(u[valid])
+        T = targets[valid]
+        A = F.T @ F
+        B = F.T @ T
+        self.coef = torch.linalg.solve(A, B)
+
+    def poly(self, u: torch.Tensor) -> torch.Tensor:
+        return poly_features(u) @ self.coef
+
+    # -- training ----------------------------------------------------------
+    def prepare(self, grid_fn, n_train: int, chunk: int = 1 << 22, poly_rays: int = 1 << 22):
+        """Trace ``n_train`` rays from ``grid_fn(count, seed)`` with the
+        float64 trace, fit the polynomial, and keep the network's inputs
+        and normalised residual targets on the device."""
+        dev = self.dev
+        us, ts, oks = [], [], []
+        done, seed = 0, 1000
+        while done < n_train:
+            m = min(chunk, n_train - done)
+            g = grid_fn(m, seed)
+            seed += 1
+            t = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=dev).double()  # noqa: E731
+            fx, fy, px, py, wl = t(g.field_x), t(g.field_y), t(g.pupil_x), t(g.pupil_y), t(g.wavelength)
+            d = self.trace64(fx, fy, px, py, wl)
+            u = self.normalised(fx, fy, px, py, wl)
+            tg = self._targets(d)
+            ok = torch.isfinite(tg).all(1)
+            us.append(u)
+            ts.append(tg)
+            oks.append(ok)
+            done += m
+        u = torch.cat(us)
+        tg = torch.cat(ts)
+        ok = torch.cat(oks)
+        k = min(poly_rays, u.shape[0])
+        self.fit_polynomial(u[:k], tg[:k], ok[:k])
+        # apertures the rays come near (anything within 4 margins of an edge)
+        self.relevant = []
+        for i, e in enumerate(self.edges):
+            ax, ay = tg[ok, 3 + 2 * i], tg[ok, 4 + 2 * i]
+            if torch.abs(e.distance(ax, ay)).min().item() < 4 * self.edge_margin:
+                self.relevant.append(i)
+        res = tg[:, :3] - self.poly(u)[:, :3]
+        self.scale = res[ok].std(0).clamp(min=1e-12)
+        self.train_u = u[ok].float().contiguous()
+        self.train_t = (res[ok] / self.scale).float().contiguous()
+        return int(ok.sum().item())
+
+    def train(self, epochs: float = 1.0, batch: int = 65536, lr: float = 2e-3,
+              weights=(1.0, 1.0, 0.25), log_every: int = 0) -> list:
+        """Adam over the prepared rays (cosine-decayed learning rate)."""
+        n = self.train_u.shape[0]
+        steps = max(1, int(epochs * n / batch))
+        w = torch.tensor(weights, dtype=torch.float32, device=self.dev)
+        losses = []
+        g = torch.Generator(device=self.dev).manual_seed(1234)
+        perm = None
+        pos = n
+        for step in range(steps):
+            if pos + batch > n:
+                perm = torch.randperm(n, device=self.dev, generator=g)
+                pos = 0
+            idx = perm[pos:pos + batch]
+            pos += batch
+            x = self.train_u.index_select(0, idx).contiguous()
+            t = self.train_t.index_select(0, idx).contiguous()
+            loss = self.mlp.loss_grad(x, t, w)
+            cur = lr * 0.5 * (1.0 + math.cos(math.pi * step / steps))
+            self.mlp.adam_step(cur)
+            if log_every and (step % log_every == 0 or step == steps - 1):
+                losses.append((step, float(loss.item())))
+        return losses
+
+    # -- prediction --------------------------------------------------------
+    def predict(self, fx, fy, px, py, wl):
+        """Surrogate outputs for float32/float64 device tensors: dict of
+        x, y, intensity (float64), unvignetted, fallback (bool)."""
+        n = fx.numel()
+        u = self.normalised(fx, fy, px, py, wl)
+        P = self.poly(u)
+        pad = (-n) % 64
+        u32 = u.float()
+        if pad:
+            u32 = torch.cat([u32, torch.zeros(pad, 5, dtype=torch.float32, device=self.dev)])
+        y = self.mlp.forward(u32.contiguous())[:n].double()
+        out = P[:, :3] + y * self.scale
+        near = torch.zeros(n, dtype=torch.bool, device=self.dev)
+        inside = torch.ones(n, dtype=torch.bool, device=self.dev)
+        for i in self.relevant:
+            e = self.edges[i]
+            d = e.distance(P[:, 3 + 2 * i], P[:, 4 + 2 * i])
+            near |= torch.abs(d) < self.edge_margin
+            inside &= d > 0
+        fallback = near | ~torch.isfinite(out).all(1)
+        if self.err_map is not None:
+            fallback |= self._cell_flag(u)
+        return {"x": out[:, 0], "y": out[:, 1], "intensity": torch.exp(out[:, 2]),
+                "unvignetted": inside, "fallback": fallback}
+
+    def _cell(self, u):
+        b = torch.clamp(((u[:, [4, 0, 1]] + 1) * 0.5 * torch.tensor([16, 8, 8], device=u.device,
+                                                                     dtype=u.dtype)).long(), 0)
+        b = torch.minimum(b, torch.tensor([15, 7, 7], device=u.device))
+        return (b[:, 0] * 8 + b[:, 1]) * 8 + b[:, 2]
+
+    def _cell_flag(self, u):
+        return self.err_map[self._cell(u)]
+
+    def calibrate(self, fx, fy, px, py, wl, ref=None):
+        """Flag input cells (wavelength x field x x field y, 16 x 8 x 8)
+        whose worst validation error exceeds the bound."""
+        if ref is None:
+            ref = self.trace64(fx, fy, px, py, wl)
+        self.err_map = None
+        p = self.predict(fx, fy, px, py, wl)
+        ok = ref["unvignetted"] & p["unvignetted"] & ~p["fallback"]
+        err = torch.sqrt((p["x"] - ref["x"]) ** 2 + (p["y"] - ref["y"]) ** 2) / self.pixel
+        cell = self._cell(self.normalised(fx, fy, px, py, wl))
+        worst = torch.zeros(16 * 8 * 8, dtype=torch.float64, device=self.dev)
+        worst.scatter_reduce_(0, cell[ok], err[ok], reduce="amax")
+        self.err_map = worst > self.bound_px
+        return worst
+
+    def run(self, fx, fy, px, py, wl, precision: str = "f32"):
+        """Surrogate outputs with the flagged rays traced: returns the
+        prediction dict (x, y, intensity, unvignetted) and the fallback
+        mask."""
+        p = self.predict(fx, fy, px, py, wl)
+        fb = torch.nonzero(p["fallback"]).reshape(-1)
+        if fb.numel():
+            from .vectors import Cartesian2dVectorArray
+
+            dt = torch.float64 if precision == "f64" else torch.float32
+            sel = lambda a: a.index_select(0, fb).to(dt).contiguous()  # noqa: E731
+            r = self.system.rayfunction(
+                wavelength=sel(wl), field=Cartesian2dVectorArray(sel(fx), sel(fy)),
+                pupil=Cartesian2dVectorArray(sel(px), sel(py)), normalized_field=False,
+                normalized_pupil=False, precision=precision).outputs
+            r = r if r.position.x.ndim == 1 else r[self.path]
+            p["x"] = p["x"].index_copy(0, fb, r.position.x.double())
+            p["y"] = p["y"].index_copy(0, fb, r.position.y.double())
+            p["intensity"] = p["intensity"].index_copy(0, fb, r.intensity.double())
+            p["unvignetted"] = p["unvignetted"].index_copy(0, fb, r.unvignetted)
+        return p

@@ -1,0 +1,135 @@
+"""The surrogate's kernels on the GPU: the bf16 MFMA GEMM against float64
+products of the same bf16 values, the gradients of the training kernels
+against the CPU network definition (oracle/mlp.py), and the trained
+surrogate against the float64 trace -- including what the network gains
+over the polynomial alone (the method without it)."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp as ref_mlp
+from optika_amd import surrogate, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16(rng, shape, scale=1.0):
+    x = ref_mlp.bf16_round(rng.normal(0, scale, shape))
+    bits = (x.astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+    return x, torch.as_tensor(bits.view(np.int16), device="cuda")
+
+
+@pytest.mark.parametrize("ta,tb,split", [(0, 0, 1), (0, 1, 1), (1, 0, 1), (1, 1, 1), (1, 0, 4)])
+def test_gemm_bf16_matches_float64(ta, tb, split):
+    rng = np.random.default_rng(ta * 2 + tb + 10 * split)
+    m, n, k = 128, 192, 256
+    A, a = _bf16(rng, (k, m) if ta else (m, k))
+    B, b = _bf16(rng, (n, k) if tb else (k, n))
+    Am = A.T if ta else A
+    Bm = B.T if tb else B
+    want = Am @ Bm
+    c = torch.empty(m, n, dtype=torch.float32, device="cuda")
+    ws = torch.empty(split * m * n, dtype=torch.float32, device="cuda") if split > 1 else None
+    print(f"case: gemm ta={ta} tb={tb} split={split}", flush=True)
+    surrogate._gemm(ta, tb, m, n, k, a, m if ta else k, b, k if tb else n, c, n, split, ws)
+    got = c.cpu().numpy().astype(np.float64)
+    bound = 1e-5 * (np.abs(Am) @ np.abs(Bm)) + 1e-6
+    assert np.all(np.abs(got - want) <= bound), np.max(np.abs(got - want) / bound)
+
+
+def test_training_gradients_match_cpu_network():
+    rng = np.random.default_rng(3)
+    dev = torch.device("cuda")
+    net = surrogate.MLP(dev, seed=5)
+    flat = ref_mlp.init_params(rng) * 0.7 + rng.normal(0, 0.02, surrogate.NUM_PARAMS)
+    net.set_params(torch.as_tensor(flat, dtype=torch.float32, device=dev))
+    p32 = net.p.cpu().numpy().astype(np.float64)  # the float32 master values
+    rows = 256
+    x = rng.uniform(-1, 1, (rows, 5)).astype(np.float32)
+    t = rng.normal(0, 1, (rows, 3)).astype(np.float32)
+    w = np.array([1.0, 1.0, 0.25], dtype=np.float32)
+    print("case: mlp gradients vs cpu", flush=True)
+    loss = net.loss_grad(torch.as_tensor(x, device=dev), torch.as_tensor(t, device=dev),
+                         torch.as_tensor(w, device=dev))
+    g = net.g.cpu().numpy().astype(np.float64)
+    l_bf, g_bf = ref_mlp.loss_and_grad(p32, x, t, w, bf16=True)
+    l_ex, g_ex = ref_mlp.loss_and_grad(p32, x, t, w, bf16=False)
+    assert abs(loss.item() - l_bf) <= 1e-4 * abs(l_bf)
+    sizes = [int(np.prod(s)) for s in ref_mlp.param_shapes()]
+    starts = np.concatenate([[0], np.cumsum(sizes)])
+    for i in range(len(sizes)):
+        sl = slice(starts[i], starts[i + 1])
+        rel_bf = np.linalg.norm(g[sl] - g_bf[sl]) / np.linalg.norm(g_bf[sl])
+        rel_ex = np.linalg.norm(g[sl] - g_ex[sl]) / np.linalg.norm(g_ex[sl])
+        print(f"  block {i}: rel err vs bf16-emulating {rel_bf:.2e}, vs exact float64 {rel_ex:.2e}")
+        assert rel_bf < 2e-2, (i, rel_bf)
+        assert rel_ex < 5e-2, (i, rel_ex)
+
+
+def test_adam_step_matches_formula():
+    dev = torch.device("cuda")
+    net = surrogate.MLP(dev, seed=1)
+    p0 = net.p.clone()
+    net.g.copy_(torch.linspace(-1, 1, surrogate.NUM_PARAMS, device=dev))
+    net.adam_step(1e-3)
+    g = net.g.double()
+    m = 0.1 * g
+    v = 0.001 * g * g
+    want = p0.double() - 1e-3 * (m / 0.1) / (torch.sqrt(v / 0.001) + 1e-8)
+    assert torch.max(torch.abs(net.p.double() - want)).item() < 1e-6
+
+
+@pytest.fixture(scope="module")
+def trained():
+    s = workloads.spectrograph()
+    lo = [-0.05, -0.05, -40.0, -40.0, 58.0]
+    hi = [0.05, 0.05, 40.0, 40.0, 63.0]
+    sg = surrogate.Surrogate(s, lo, hi, path=0, seed=0)
+    print("case: surrogate train (2M rays)", flush=True)
+    n = sg.prepare(lambda m, seed: workloads.spectrograph_grid(m, seed=seed), 1 << 21,
+                   chunk=1 << 20, poly_rays=1 << 20)
+    losses = sg.train(epochs=6, batch=65536, lr=2e-3, log_every=20)
+    print(f"  {n} training rays; losses {losses}", flush=True)
+    return sg
+
+
+def _eval_grid(n, seed):
+    g = workloads.spectrograph_grid(n, seed=seed)
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float32), device="cuda").double()  # noqa: E731
+    return t(g.field_x), t(g.field_y), t(g.pupil_x), t(g.pupil_y), t(g.wavelength)
+
+
+def test_network_beats_polynomial_alone(trained):
+    """Ablation: the same prediction with the network's correction removed."""
+    sg = trained
+    ins = _eval_grid(1 << 18, seed=77)
+    ref = sg.trace64(*ins)
+    p = sg.predict(*ins)
+    u = sg.normalised(*ins)
+    poly = sg.poly(u)
+    ok = ref["unvignetted"] & ~p["fallback"]
+    e_net = torch.sqrt((p["x"] - ref["x"]) ** 2 + (p["y"] - ref["y"]) ** 2)[ok] / sg.pixel
+    e_poly = torch.sqrt((poly[:, 0] - ref["x"]) ** 2 + (poly[:, 1] - ref["y"]) ** 2)[ok] / sg.pixel
+    rms_net = torch.sqrt((e_net ** 2).mean()).item()
+    rms_poly = torch.sqrt((e_poly ** 2).mean()).item()
+    print(f"ablation: RMS spot error polynomial alone {rms_poly:.4f} px, with network "
+          f"{rms_net:.4f} px ({rms_poly / rms_net:.1f}x)", flush=True)
+    assert rms_net < 0.5 * rms_poly
+
+
+def test_surrogate_with_fallback_meets_bound(trained):
+    sg = trained
+    sg.calibrate(*_eval_grid(1 << 18, seed=78))
+    ins = _eval_grid(1 << 18, seed=79)
+    ref = sg.trace64(*ins)
+    p = sg.run(*ins, precision="f32")
+    both = ref["unvignetted"] & p["unvignetted"]
+    err = torch.sqrt((p["x"] - ref["x"]) ** 2 + (p["y"] - ref["y"]) ** 2)[both] / sg.pixel
+    rms = torch.sqrt((err ** 2).mean()).item()
+    agree = (ref["unvignetted"] == p["unvignetted"]).double().mean().item()
+    frac = sg.predict(*ins)["fallback"].double().mean().item()
+    print(f"surrogate + fallback: RMS {rms:.4f} px, mask agreement {agree:.5f}, "
+          f"fallback fraction {frac:.3f}", flush=True)
+    assert rms <= 0.05
+    assert agree >= 0.999
