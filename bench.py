@@ -61,8 +61,11 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", choices=("newtonian", "spectrograph", "tolerance"),
+    p.add_argument("--workload", choices=("newtonian", "spectrograph", "tolerance", "surrogate"),
                    default="newtonian")
+    p.add_argument("--train-rays", type=int, default=200_000_000,
+                   help="surrogate: rays traced (float64) for training")
+    p.add_argument("--epochs", type=float, default=2.0, help="surrogate: training epochs")
     p.add_argument("--rays", type=int, default=None,
                    help="rays per step (default: 2^24 newtonian, 50M spectrograph)")
     p.add_argument("--instances", type=int, default=4096, help="tolerance: instances")
@@ -378,6 +381,123 @@ def run_tolerance(args):
     return header(args, value32, ms32, "f32", config, forms, None, cpu)
 
 
+def run_surrogate(args):
+    """Surrogate of the spectrograph's order -1 rayfunction: train on
+    float64-traced rays, calibrate, then time fused inference over --rays
+    rays (default 1e9) next to the float64 and float32 traces of the same
+    rays, and judge the error against the float64 trace on a subsample."""
+    import torch
+
+    from optika_amd import _trace, surrogate, workloads
+    from optika_amd.vectors import Cartesian2dVectorArray
+
+    dev = torch.device("cuda")
+    system = workloads.spectrograph()
+    S = workloads.SPECTROGRAPH_SURFACES
+    lo = [-0.05, -0.05, -40.0, -40.0, 58.0]
+    hi = [0.05, 0.05, 40.0, 40.0, 63.0]
+    sg = surrogate.Surrogate(system, lo, hi, path=0, seed=0)
+    t0 = time.perf_counter()
+    n_train = sg.prepare(lambda m, seed: workloads.spectrograph_grid(m, seed=seed), args.train_rays,
+                         chunk=1 << 23, poly_rays=1 << 22)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    log(f"traced {args.train_rays} training rays ({n_train} usable) in {t1 - t0:.1f} s")
+    losses = sg.train(epochs=args.epochs, batch=65536, lr=2e-3, log_every=500)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    steps = max(1, int(args.epochs * n_train / 65536))
+    log(f"trained {steps} Adam steps in {t2 - t1:.1f} s; losses {losses}")
+
+    def inputs(n, seed):
+        g = workloads.spectrograph_grid(n, seed=seed)
+        t32 = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=dev)  # noqa: E731
+        return [t32(g.field_x), t32(g.field_y), t32(g.pupil_x), t32(g.pupil_y), t32(g.wavelength)]
+
+    cal = inputs(1 << 20, 501)
+    sg.calibrate(*[t.double() for t in cal])
+    # accuracy on a held-out subsample, surrogate + fallback vs float64 trace
+    ev = inputs(1 << 20, 502)
+    ref = sg.trace64(*[t.double() for t in ev])
+    out = sg.infer(*ev)
+    fb = out["list"][:int(out["count"].item())].long()
+    x, y, unv = out["x"].double().clone(), out["y"].double().clone(), out["unvignetted"].clone()
+    if fb.numel():
+        sel = [t[fb].contiguous() for t in ev]
+        r = sg.system.rayfunction(
+            wavelength=sel[4], field=Cartesian2dVectorArray(sel[0], sel[1]),
+            pupil=Cartesian2dVectorArray(sel[2], sel[3]), normalized_field=False,
+            normalized_pupil=False, precision="f32").outputs[0]
+        x[fb], y[fb], unv[fb] = r.position.x.double(), r.position.y.double(), r.unvignetted
+    both = ref["unvignetted"] & unv
+    err = torch.sqrt((x - ref["x"]) ** 2 + (y - ref["y"]) ** 2)[both] / PIXEL_MM
+    nofb = both & ~out["fallback"]
+    err_nofb = torch.sqrt((x - ref["x"]) ** 2 + (y - ref["y"]) ** 2)[nofb] / PIXEL_MM
+    acc = {"rms_spot_px": torch.sqrt((err ** 2).mean()).item(),
+           "rms_spot_px_surrogate_rays": torch.sqrt((err_nofb ** 2).mean()).item(),
+           "max_spot_px_surrogate_rays": err_nofb.max().item(),
+           "mask_agreement": (ref["unvignetted"] == unv).double().mean().item(),
+           "fallback_fraction": fb.numel() / ev[0].numel()}
+    log(f"surrogate accuracy: {acc}")
+    del ref
+    # throughput: fused inference over n rays, in one call
+    n = args.rays or 1_000_000_000
+    chunk = 1 << 26
+    big = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(5)]
+    for c0 in range(0, n, chunk):
+        m = min(chunk, n - c0)
+        part = inputs(m, 1000 + c0 // chunk)
+        for b_, p_ in zip(big, part):
+            b_[c0:c0 + m].copy_(p_)
+    res = {"x": torch.empty(n, dtype=torch.float32, device=dev),
+           "y": torch.empty(n, dtype=torch.float32, device=dev),
+           "intensity": torch.empty(n, dtype=torch.float32, device=dev),
+           "unvignetted": torch.empty(n, dtype=torch.bool, device=dev),
+           "fallback": torch.empty(n, dtype=torch.bool, device=dev),
+           "list": torch.empty(n // 4, dtype=torch.int32, device=dev),
+           "count": torch.zeros(1, dtype=torch.int32, device=dev)}
+    stream = torch.cuda.current_stream()
+    ms_sur, k_sur = time_calls(lambda: sg.infer(*big, out=res), max(2, args.steps // 4), 1, stream)
+    fb_count = int(res["count"].item())
+    log(f"surrogate inference: {ms_sur:.2f} ms for {n} rays, {fb_count} fall back")
+    # float64 and float32 traces of the first 2^24 of the same rays
+    m = min(n, 1 << 24)
+    sub32 = [t[:m] for t in big]
+    sub64 = [t.double() for t in sub32]
+    plan64 = _trace.GridTracePlan(system, m, sensor_frame=True, precision="f64")
+    plan32 = _trace.GridTracePlan(system, m, sensor_frame=True, precision="f32")
+    ms64, _ = time_calls(lambda: plan64(*sub64), 3, 1, stream)
+    ms32, _ = time_calls(lambda: plan32(*sub32), 5, 1, stream)
+    units = S  # surfaces per ray of the emulated path
+    v_sur = n * units / (ms_sur / 1e3)
+    v64 = m * 2 * S / (ms64 / 1e3)
+    v32 = m * 2 * S / (ms32 / 1e3)
+    flops_per_ray = 2 * (5 * 256 + 3 * 256 * 256 + 256 * 3)
+    achieved = n * flops_per_ray / (k_sur / 1e3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                "frac": round(achieved / 2500.0, 4), "traffic": None,
+                "kernel": "surrogate_infer", "kernel_avg_ms": round(k_sur, 3),
+                "flops_per_ray": flops_per_ray}
+    forms = {
+        "f64": {"value": v64, "ms_per_step": ms64, "rays": m, "paths": 2},
+        "f32": {"value": v32, "ms_per_step": ms32, "rays": m, "paths": 2},
+        "surrogate": {"value": v_sur, "ms_per_step": ms_sur, "rays": n,
+                      "fallback_rays": fb_count, "speedup_vs_f64": v_sur / v64,
+                      "speedup_vs_f32": v_sur / v32, "accuracy_vs_f64": acc,
+                      "training": {"rays": args.train_rays, "usable": n_train, "adam_steps": steps,
+                                   "batch": 65536, "trace_s": t1 - t0, "train_s": t2 - t1}},
+    }
+    config = {
+        "workload": f"learned surrogate of the EUV spectrograph (order -1): 5 -> 4 x 256 SiLU -> 3 "
+                    f"MLP, bf16 MFMA, trained on {args.train_rays} float64-traced rays, inference "
+                    f"on {n} rays (fallback rays counted, not traced, in the timed region)",
+        "rays": n,
+        "surfaces": S,
+        "form": "surrogate (value counts surfaces of the emulated path per ray)",
+    }
+    return header(args, v_sur, ms_sur, "bf16", config, forms, roofline, None)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -385,7 +505,7 @@ def main():
               "(rays shard with no exchange; run one process per GPU)", file=sys.stderr)
         sys.exit(2)
     run = {"newtonian": run_newtonian, "spectrograph": run_spectrograph,
-           "tolerance": run_tolerance}[args.workload]
+           "tolerance": run_tolerance, "surrogate": run_surrogate}[args.workload]
     print(json.dumps(run(args)), flush=True)
 
 

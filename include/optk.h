@@ -453,6 +453,59 @@ typedef struct optk_mlp_args {
 
 int32_t optk_mlp_op(int32_t op, const optk_mlp_args *args, void *hip_stream);
 
+/* Fused inference of a trained surrogate.  Per ray: inputs normalised as
+ * (v - in_mid) * in_inv_half; the network's three outputs scaled by
+ * `scale` and added to a cubic polynomial of the normalised inputs
+ * (OPTK_SURROGATE_POLY monomials; poly_exp packs each monomial's exponents,
+ * 2 bits per input; poly[t][j] its coefficient for target j).  Targets:
+ * sensor x, y, log intensity, then (x, y) in the frame of every listed
+ * aperture.  A ray falls back to the trace when it is predicted within
+ * edge_margin of an aperture edge, its input cell (16 wavelength x 8 field
+ * x x 8 field y bins) is flagged in error_map, or a prediction is not
+ * finite; fallback rays are appended to fallback_list (one atomic per
+ * wavefront; order unspecified; entries past fallback_capacity are
+ * counted but not written). */
+#define OPTK_SURROGATE_POLY 56
+#define OPTK_SURROGATE_MAX_APERTURES 4
+#define OPTK_SURROGATE_MAX_TARGETS (3 + 2 * OPTK_SURROGATE_MAX_APERTURES)
+#define OPTK_SURROGATE_MAX_VERTICES 16
+typedef struct optk_surrogate_aperture {
+  int32_t kind;                 /* OPTK_APERTURE_CIRCLE, _RECTANGLE or _POLYGON */
+  int32_t inverted;
+  int32_t num_vertices;
+  int32_t pad;
+  float p[4];                   /* circle: r; rectangle: half widths */
+  float vx[OPTK_SURROGATE_MAX_VERTICES];
+  float vy[OPTK_SURROGATE_MAX_VERTICES];
+} optk_surrogate_aperture;
+typedef struct optk_surrogate_desc {
+  const float *w1, *b1;         /* [256][5], [256] float32 */
+  const uint16_t *w16[3];       /* hidden layers, bf16 [256 out][256 in] */
+  const float *b[3];            /* hidden biases [256] */
+  const float *w5, *b5;         /* [3][256], [3] float32 */
+  const float *poly;            /* [OPTK_SURROGATE_POLY][OPTK_SURROGATE_MAX_TARGETS] */
+  const uint8_t *error_map;     /* 16 * 8 * 8 flags, or NULL */
+  float in_mid[5], in_inv_half[5];
+  float scale[3];
+  float edge_margin;
+  uint32_t poly_exp[OPTK_SURROGATE_POLY];
+  int32_t num_targets;
+  int32_t num_apertures;
+  optk_surrogate_aperture apertures[OPTK_SURROGATE_MAX_APERTURES];
+} optk_surrogate_desc;
+typedef struct optk_surrogate_io {
+  const float *field_x, *field_y, *pupil_x, *pupil_y, *wavelength;
+  float *x, *y, *intensity;
+  uint8_t *unvignetted, *fallback;
+  int32_t *fallback_list;       /* or NULL */
+  int32_t *fallback_count;      /* device counter, incremented (not cleared) */
+  int64_t fallback_capacity;
+  int64_t n;
+} optk_surrogate_io;
+
+int32_t optk_surrogate_infer(const optk_surrogate_desc *desc, const optk_surrogate_io *io,
+                             void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -201,6 +201,37 @@ class MlpArgs(ctypes.Structure):
                 ("scalars", ctypes.c_float * 8)]
 
 
+SURROGATE_POLY = 56
+SURROGATE_MAX_APERTURES = 4
+SURROGATE_MAX_TARGETS = 3 + 2 * SURROGATE_MAX_APERTURES
+SURROGATE_MAX_VERTICES = 16
+
+
+class SurrogateAperture(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("inverted", c_int32), ("num_vertices", c_int32),
+                ("pad", c_int32), ("p", ctypes.c_float * 4),
+                ("vx", ctypes.c_float * SURROGATE_MAX_VERTICES),
+                ("vy", ctypes.c_float * SURROGATE_MAX_VERTICES)]
+
+
+class SurrogateDesc(ctypes.Structure):
+    _fields_ = [("w1", c_void_p), ("b1", c_void_p), ("w16", c_void_p * 3), ("b", c_void_p * 3),
+                ("w5", c_void_p), ("b5", c_void_p), ("poly", c_void_p), ("error_map", c_void_p),
+                ("in_mid", ctypes.c_float * 5), ("in_inv_half", ctypes.c_float * 5),
+                ("scale", ctypes.c_float * 3), ("edge_margin", ctypes.c_float),
+                ("poly_exp", ctypes.c_uint32 * SURROGATE_POLY),
+                ("num_targets", c_int32), ("num_apertures", c_int32),
+                ("apertures", SurrogateAperture * SURROGATE_MAX_APERTURES)]
+
+
+class SurrogateIO(ctypes.Structure):
+    _fields_ = [("field_x", c_void_p), ("field_y", c_void_p), ("pupil_x", c_void_p),
+                ("pupil_y", c_void_p), ("wavelength", c_void_p), ("x", c_void_p), ("y", c_void_p),
+                ("intensity", c_void_p), ("unvignetted", c_void_p), ("fallback", c_void_p),
+                ("fallback_list", c_void_p), ("fallback_count", c_void_p),
+                ("fallback_capacity", c_int64), ("n", c_int64)]
+
+
 MLP_BIAS_SILU = 0
 MLP_DENSE_IN = 1
 MLP_DENSE_OUT = 2
@@ -268,6 +299,9 @@ def lib() -> ctypes.CDLL:
                                  c_void_p, c_int64, c_void_p, c_int64, c_int32, c_void_p, c_void_p]
     L.optk_mlp_op.restype = c_int32
     L.optk_mlp_op.argtypes = [c_int32, ctypes.POINTER(MlpArgs), c_void_p]
+    L.optk_surrogate_infer.restype = c_int32
+    L.optk_surrogate_infer.argtypes = [ctypes.POINTER(SurrogateDesc), ctypes.POINTER(SurrogateIO),
+                                       c_void_p]
     if L.optk_abi_version() != 1:
         raise OptkError(f"liboptk ABI {L.optk_abi_version()} != 1")
     for name, cls in [("optk_affine", Affine), ("optk_surface", Surface), ("optk_layer", Layer),
@@ -275,7 +309,8 @@ def lib() -> ctypes.CDLL:
                       ("optk_system_desc", SystemDesc), ("optk_rays", Rays), ("optk_grid", Grid),
                       ("optk_eval_args", EvalArgs), ("optk_collect_args", CollectArgs),
                       ("optk_batch_desc", BatchDesc), ("optk_tune", Tune),
-                      ("optk_mlp_args", MlpArgs)]:
+                      ("optk_mlp_args", MlpArgs), ("optk_surrogate_desc", SurrogateDesc),
+                      ("optk_surrogate_io", SurrogateIO)]:
         size = L.optk_sizeof(name.encode())
         if size != ctypes.sizeof(cls):
             raise OptkError(f"record {name}: C size {size} != binding size {ctypes.sizeof(cls)}")

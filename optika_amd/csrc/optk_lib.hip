@@ -584,6 +584,8 @@ int64_t optk_sizeof(const char *name) {
   if (s == "optk_batch_desc") return sizeof(optk_batch_desc);
   if (s == "optk_tune") return sizeof(optk_tune);
   if (s == "optk_mlp_args") return sizeof(optk_mlp_args);
+  if (s == "optk_surrogate_desc") return sizeof(optk_surrogate_desc);
+  if (s == "optk_surrogate_io") return sizeof(optk_surrogate_io);
   return -1;
 }
 

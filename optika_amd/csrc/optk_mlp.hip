@@ -300,6 +300,200 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(const float *__restrict__ 
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+
+// ---------------------------------------------------------------------------
+// fused inference: input normalisation, the five layers, the polynomial,
+// aperture-edge distances, the error-map lookup and the compaction of the
+// rays that fall back to the trace -- one kernel, activations in LDS.
+//
+// Block: 512 threads (8 waves), a tile of 256 rays.  Layer 1 (5 -> 256) on
+// VALU in float32 writes bf16 activations act[ray][feature] to LDS.  Each
+// hidden layer is a 256 x 256 x 256 product: wave w computes rays
+// [128 (w / 4), +128) x features [64 (w % 4), +64) with 8 x 4 MFMA 16x16x32
+// tiles, A operands (activations) from LDS, B operands (weights, row-major
+// [out][in], so k runs contiguously) straight from global memory (L2).
+// After a barrier the waves overwrite act with bias + SiLU of their tile.
+// The output layer, polynomial and edge tests run one thread per ray.
+// ---------------------------------------------------------------------------
+
+constexpr int INF_RAYS = 256;
+constexpr int INF_HIDDEN = 256;
+constexpr int INF_PAD = 8;
+
+__global__ __launch_bounds__(512) void surrogate_infer(const optk_surrogate_desc D, const optk_surrogate_io IO) {
+  __shared__ __attribute__((aligned(16))) uint16_t act[INF_RAYS][INF_HIDDEN + INF_PAD];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int64_t n = IO.n;
+  const int64_t tiles = (n + INF_RAYS - 1) / INF_RAYS;
+  for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const int64_t base = tile * INF_RAYS;
+    // ---- layer 1 ------------------------------------------------------
+    {
+      const int r = tid & (INF_RAYS - 1);
+      const int64_t i = base + r;
+      float u[5];
+      const float *in[5] = {IO.field_x, IO.field_y, IO.pupil_x, IO.pupil_y, IO.wavelength};
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        float v = i < n ? in[k][i] : 0.0f;
+        u[k] = (v - D.in_mid[k]) * D.in_inv_half[k];
+      }
+      const int f0 = (tid >> 8) * (INF_HIDDEN / 2);
+      for (int f = f0; f < f0 + INF_HIDDEN / 2; ++f) {
+        float v = D.b1[f];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) v += D.w1[f * 5 + k] * u[k];
+        act[r][f] = to_bf16(silu(v));
+      }
+    }
+    __syncthreads();
+    // ---- hidden layers ------------------------------------------------
+    const int rbase = (wave >> 2) * 128;
+    const int obase = (wave & 3) * 64;
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    for (int layer = 0; layer < 3; ++layer) {
+      const uint16_t *W = D.w16[layer];
+      floatx4 acc[8][4];
+#pragma unroll
+      for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int kc = 0; kc < INF_HIDDEN; kc += 32) {
+        bf16x8 bw[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+          uint4 v = *reinterpret_cast<const uint4 *>(W + (int64_t)(obase + 16 * nt + fr) * INF_HIDDEN + kc + fk);
+          bw[nt] = __builtin_bit_cast(bf16x8, v);
+        }
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) {
+          uint4 v = *reinterpret_cast<const uint4 *>(&act[rbase + 16 * mt + fr][kc + fk]);
+          bf16x8 a = __builtin_bit_cast(bf16x8, v);
+#pragma unroll
+          for (int nt = 0; nt < 4; ++nt)
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[nt], acc[mt][nt], 0, 0, 0);
+        }
+      }
+      __syncthreads();  // every wave has read this layer's inputs
+      const float *bias = D.b[layer];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const int col = obase + 16 * nt + (lane & 15);
+        const float bcol = bias[col];
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int row = rbase + 16 * mt + 4 * (lane >> 4) + q;
+            act[row][col] = to_bf16(silu(acc[mt][nt][q] + bcol));
+          }
+      }
+      __syncthreads();
+    }
+    // ---- output layer, polynomial, apertures, fallback ------------------
+    bool fallback = false;
+    int64_t i = base + tid;
+    const bool live = tid < INF_RAYS && i < n;
+    if (live) {
+      float y[3] = {D.b5[0], D.b5[1], D.b5[2]};
+      for (int f = 0; f < INF_HIDDEN; f += 8) {
+        uint4 v = *reinterpret_cast<const uint4 *>(&act[tid][f]);
+        const uint16_t *h = reinterpret_cast<const uint16_t *>(&v);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const float a = from_bf16(h[q]);
+          y[0] += a * D.w5[f + q];
+          y[1] += a * D.w5[INF_HIDDEN + f + q];
+          y[2] += a * D.w5[2 * INF_HIDDEN + f + q];
+        }
+      }
+      float u[5];
+      const float *in[5] = {IO.field_x, IO.field_y, IO.pupil_x, IO.pupil_y, IO.wavelength};
+#pragma unroll
+      for (int k = 0; k < 5; ++k) u[k] = (in[k][i] - D.in_mid[k]) * D.in_inv_half[k];
+      // cubic monomials, in the order of optk_surrogate_desc.poly_exp
+      float tgt[OPTK_SURROGATE_MAX_TARGETS];
+      for (int j = 0; j < D.num_targets; ++j) tgt[j] = 0.0f;
+      float pw[5][4];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        pw[k][0] = 1.0f;
+        pw[k][1] = u[k];
+        pw[k][2] = u[k] * u[k];
+        pw[k][3] = pw[k][2] * u[k];
+      }
+      for (int t = 0; t < OPTK_SURROGATE_POLY; ++t) {
+        const uint32_t e = D.poly_exp[t];  // 2 bits per variable
+        float mono = pw[0][e & 3] * pw[1][(e >> 2) & 3] * pw[2][(e >> 4) & 3] * pw[3][(e >> 6) & 3] *
+                     pw[4][(e >> 8) & 3];
+        for (int j = 0; j < D.num_targets; ++j) tgt[j] += mono * D.poly[t * OPTK_SURROGATE_MAX_TARGETS + j];
+      }
+      const float x = tgt[0] + D.scale[0] * y[0];
+      const float yy = tgt[1] + D.scale[1] * y[1];
+      const float li = tgt[2] + D.scale[2] * y[2];
+      bool inside = true, near = false;
+      for (int a = 0; a < D.num_apertures; ++a) {
+        const optk_surrogate_aperture &ap = D.apertures[a];
+        const float ax = tgt[3 + 2 * a], ay = tgt[4 + 2 * a];
+        float d;
+        if (ap.kind == OPTK_APERTURE_CIRCLE) {
+          d = ap.p[0] - sqrtf(ax * ax + ay * ay);
+        } else if (ap.kind == OPTK_APERTURE_RECTANGLE) {
+          d = fminf(ap.p[0] - fabsf(ax), ap.p[1] - fabsf(ay));
+        } else if (ap.kind == OPTK_APERTURE_POLYGON) {
+          float dmin = INFINITY;
+          bool in_poly = false;
+          for (int v = 0; v < ap.num_vertices; ++v) {
+            const int w = v == 0 ? ap.num_vertices - 1 : v - 1;
+            const float x1 = ap.vx[w], y1 = ap.vy[w], x2 = ap.vx[v], y2 = ap.vy[v];
+            const float ex = x2 - x1, ey = y2 - y1;
+            float s = ((ax - x1) * ex + (ay - y1) * ey) / (ex * ex + ey * ey);
+            s = fminf(fmaxf(s, 0.0f), 1.0f);
+            const float dx = ax - (x1 + s * ex), dy = ay - (y1 + s * ey);
+            dmin = fminf(dmin, sqrtf(dx * dx + dy * dy));
+            if (((y1 > ay) != (y2 > ay)) && (ax < (x2 - x1) * (ay - y1) / (y2 - y1) + x1)) in_poly = !in_poly;
+          }
+          d = in_poly ? dmin : -dmin;
+        } else {
+          d = 0.0f;  // no distance model: always near
+        }
+        if (ap.inverted) d = -d;
+        inside = inside && d > 0.0f;
+        near = near || fabsf(d) < D.edge_margin;
+      }
+      // error-map cell: 16 wavelength x 8 field x x 8 field y bins of [-1, 1]
+      int cw = (int)((u[4] + 1.0f) * 8.0f), cx = (int)((u[0] + 1.0f) * 4.0f), cy = (int)((u[1] + 1.0f) * 4.0f);
+      cw = cw < 0 ? 0 : (cw > 15 ? 15 : cw);
+      cx = cx < 0 ? 0 : (cx > 7 ? 7 : cx);
+      cy = cy < 0 ? 0 : (cy > 7 ? 7 : cy);
+      const bool flagged = D.error_map ? D.error_map[(cw * 8 + cx) * 8 + cy] != 0 : false;
+      fallback = near || flagged || !(fabsf(x) < INFINITY && fabsf(yy) < INFINITY && fabsf(li) < INFINITY);
+      IO.x[i] = x;
+      IO.y[i] = yy;
+      IO.intensity[i] = __expf(li);
+      IO.unvignetted[i] = inside ? 1 : 0;
+      IO.fallback[i] = fallback ? 1 : 0;
+    }
+    // ---- compaction of the fallback rays: one atomic per wave ------------
+    if (IO.fallback_list) {
+      const uint64_t mask = __ballot(live && fallback);
+      if (mask) {
+        int32_t wbase = 0;
+        const int leader = __ffsll((long long)mask) - 1;
+        if (lane == leader) wbase = atomicAdd(IO.fallback_count, (int32_t)__popcll(mask));
+        wbase = __shfl(wbase, leader, 64);
+        if (live && fallback) {
+          const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+          const int32_t slot = wbase + rank;
+          if (slot < IO.fallback_capacity) IO.fallback_list[slot] = (int32_t)i;
+        }
+      }
+    }
+    __syncthreads();  // act is rewritten by the next tile's first layer
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -431,3 +625,40 @@ int32_t optk_mlp_op(int32_t op, const optk_mlp_args *A, void *hip_stream) {
 }
 
 }  // extern "C"
+
+extern "C" int32_t optk_surrogate_infer(const optk_surrogate_desc *desc, const optk_surrogate_io *io,
+                                        void *hip_stream) {
+  if (!desc || !io) return fail(OPTK_EINVAL, "null argument");
+  const optk_surrogate_desc &D = *desc;
+  const optk_surrogate_io &IO = *io;
+  if (IO.n < 0) return fail(OPTK_EINVAL, "negative count");
+  if (IO.n == 0) return OPTK_OK;
+  if (IO.n > INT32_MAX) return fail(OPTK_EINVAL, "at most 2^31 - 1 rays per call");
+  if (!IO.field_x || !IO.field_y || !IO.pupil_x || !IO.pupil_y || !IO.wavelength || !IO.x || !IO.y ||
+      !IO.intensity || !IO.unvignetted || !IO.fallback)
+    return fail(OPTK_EINVAL, "missing ray array");
+  if (IO.fallback_list && (!IO.fallback_count || IO.fallback_capacity < 0))
+    return fail(OPTK_EINVAL, "fallback list needs a counter and a capacity");
+  if (!D.w1 || !D.b1 || !D.w5 || !D.b5) return fail(OPTK_EINVAL, "missing weights");
+  for (int l = 0; l < 3; ++l) {
+    if (!D.w16[l] || !D.b[l]) return fail(OPTK_EINVAL, "missing weights");
+    if ((uintptr_t)D.w16[l] % 16) return fail(OPTK_EINVAL, "bf16 weights must be 16-byte aligned");
+  }
+  if (D.num_targets != 3 + 2 * D.num_apertures || D.num_apertures < 0 ||
+      D.num_apertures > OPTK_SURROGATE_MAX_APERTURES)
+    return fail(OPTK_EINVAL, "num_targets must be 3 + 2 * num_apertures");
+  for (int a = 0; a < D.num_apertures; ++a)
+    if (D.apertures[a].num_vertices < 0 || D.apertures[a].num_vertices > OPTK_SURROGATE_MAX_VERTICES)
+      return fail(OPTK_EINVAL, "aperture vertex count out of range");
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("device query: ") + hipGetErrorString(e));
+  const int64_t tiles = (IO.n + INF_RAYS - 1) / INF_RAYS;
+  int64_t blocks = cus > 0 ? (int64_t)cus * 4 : 1024;
+  if (blocks > tiles) blocks = tiles;
+  hipLaunchKernelGGL(surrogate_infer, dim3((unsigned)blocks), dim3(512), 0, (hipStream_t)hip_stream, D, IO);
+  e = hipGetLastError();
+  if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("surrogate launch: ") + hipGetErrorString(e));
+  return OPTK_OK;
+}

@@ -345,37 +345,57 @@ OPTK_HD T profile_factor(int kind, T width, T wavelength, C<T> direction, C<T> n
 template <typename T> struct LayerState {
   C<T> n;          // index of the medium before the next layer
   C<T> direction;  // cosine in that medium
-  bool where;           // transfer matrices still valid
+  bool where;      // transfer matrices still valid
 };
 
-// Layer.transfer (optika/materials/_layers.py:229-277) for one polarization.
-template <typename T>
-OPTK_HD M2<T> layer_transfer(const optk_layer &layer, const StackView &sv, T wavelength,
-                              bool pol_s, LayerState<T> &st, T thickness) {
-  C<T> n_int = {T(1), T(0)};
-  if (layer.chemical >= 0) n_int = to_c<T>(chemical_n(sv.chemicals[layer.chemical], sv.tables, (double)wavelength));
-  C<T> d_int = snell_scalar(st.direction, st.n, n_int);
+// transfer matrices of both polarizations (s, p)
+template <typename T> struct M2Pair {
+  M2<T> s, p;
+};
 
-  // matrices.refraction (matrices.py:127-161)
+// matrices.refraction (matrices.py:127-161) for one polarization, from the
+// incident medium (n_i, cos_i) into (n_j, cos_j)
+template <typename T>
+OPTK_HD M2<T> refraction_matrix(const optk_layer &layer, T wavelength, bool pol_s, C<T> n_i,
+                                C<T> cos_i, C<T> n_j, C<T> cos_j) {
   C<T> one = {T(1), T(0)};
-  C<T> di = pol_s ? st.direction : cconj(st.direction);
-  C<T> dj = pol_s ? d_int : cconj(d_int);
-  C<T> imp_i = pol_s ? st.n : cdiv(one, st.n);
-  C<T> imp_j = pol_s ? n_int : cdiv(one, n_int);
+  C<T> di = pol_s ? cos_i : cconj(cos_i);
+  C<T> dj = pol_s ? cos_j : cconj(cos_j);
+  C<T> imp_i = pol_s ? n_i : cdiv(one, n_i);
+  C<T> imp_j = pol_s ? n_j : cdiv(one, n_j);
   C<T> qi = cmul(di, imp_i);
   C<T> qj = cmul(dj, imp_j);
   C<T> a = cadd(qi, qj);
   C<T> r = cdiv(csub(qi, qj), a);
   C<T> t = cdiv(cscale(T(2), qi), a);
   if (layer.profile_kind != OPTK_PROFILE_NONE) {
-    T f = profile_factor<T>(layer.profile_kind, T(layer.profile_width), wavelength, di, st.n);
+    T f = profile_factor<T>(layer.profile_kind, T(layer.profile_width), wavelength, di, n_i);
     r = cscale(f, r);  // r * factor (real factor promoted to complex)
   }
   C<T> inv_t = cdiv(one, t);
   C<T> r_t = cdiv(r, t);
-  M2<T> refraction = {inv_t, r_t, r_t, inv_t};
-  if (!st.where) refraction = midentity<T>();
+  return {inv_t, r_t, r_t, inv_t};
+}
 
+// Layer.transfer (optika/materials/_layers.py:229-277) for both
+// polarizations at once.  Everything but the refraction matrix (the index,
+// the refracted cosine, the propagation phases and the validity guard)
+// depends on the medium only, so it is evaluated once; each polarization's
+// matrix keeps the reference's operation order, so the result is the same
+// as two separate per-polarization passes.  `n_int` is the layer's index
+// at this wavelength (chemical_n).
+template <typename T>
+OPTK_HD M2Pair<T> layer_transfer2(const optk_layer &layer, C<T> n_int, T wavelength,
+                                  LayerState<T> &st, T thickness) {
+  C<T> d_int = snell_scalar(st.direction, st.n, n_int);
+  M2Pair<T> refr;
+  if (st.where) {
+    refr.s = refraction_matrix<T>(layer, wavelength, true, st.n, st.direction, n_int, d_int);
+    refr.p = refraction_matrix<T>(layer, wavelength, false, st.n, st.direction, n_int, d_int);
+  } else {
+    refr.s = midentity<T>();
+    refr.p = midentity<T>();
+  }
   // matrices.propagation (matrices.py:241-246):
   // beta = 2 pi h n cos / wavelength ; U = diag(exp(-i beta), exp(i beta))
   C<T> beta =
@@ -386,11 +406,12 @@ OPTK_HD M2<T> layer_transfer(const optk_layer &layer, const StackView &sv, T wav
   bool where_prop = cabs(u00) < T(1e10);
   st.where = st.where && where_prop;
   M2<T> zero_prop = {u00, {T(0), T(0)}, {T(0), T(0)}, u11};
-  M2<T> transfer = mmul(refraction, zero_prop);
-  if (!st.where) transfer = refraction;
+  M2Pair<T> out;
+  out.s = st.where ? mmul(refr.s, zero_prop) : refr.s;
+  out.p = st.where ? mmul(refr.p, zero_prop) : refr.p;
   st.n = n_int;
   st.direction = d_int;
-  return transfer;
+  return out;
 }
 
 template <typename T> OPTK_HD M2<T> matrix_power(M2<T> m, int exponent) {
@@ -413,12 +434,36 @@ template <typename T> OPTK_HD M2<T> matrix_power(M2<T> m, int exponent) {
 }
 
 template <typename T>
-OPTK_HD M2<T> sequence_transfer(const StackView &sv, int first, int count, T wavelength,
-                                 bool pol_s, LayerState<T> &st) {
-  M2<T> result = midentity<T>();
+OPTK_HD C<T> layer_index(const optk_layer &L, const StackView &sv, T wavelength) {
+  if (L.chemical < 0) return {T(1), T(0)};
+  return to_c<T>(chemical_n(sv.chemicals[L.chemical], sv.tables, (double)wavelength));
+}
+
+// LayerSequence.transfer over layers first .. first + count - 1, both
+// polarizations; `n_cache` holds the indices of the first 4 layers when
+// `cached` (a periodic block's second pass reuses its first pass's lookups)
+template <typename T>
+OPTK_HD M2Pair<T> sequence_transfer2(const StackView &sv, int first, int count, T wavelength,
+                                     LayerState<T> &st, C<T> n_cache[4], bool cached) {
+  M2Pair<T> result = {midentity<T>(), midentity<T>()};
   for (int i = 0; i < count; ++i) {
     const optk_layer &L = sv.layers[first + i];
-    result = mmul(result, layer_transfer<T>(L, sv, wavelength, pol_s, st, T(L.thickness)));
+    C<T> n_int;
+    if (i < 4 && cached) {
+      n_int = n_cache[0];
+      if (i == 1) n_int = n_cache[1];
+      if (i == 2) n_int = n_cache[2];
+      if (i == 3) n_int = n_cache[3];
+    } else {
+      n_int = layer_index<T>(L, sv, wavelength);
+      if (i == 0) n_cache[0] = n_int;
+      if (i == 1) n_cache[1] = n_int;
+      if (i == 2) n_cache[2] = n_int;
+      if (i == 3) n_cache[3] = n_int;
+    }
+    M2Pair<T> x = layer_transfer2<T>(L, n_int, wavelength, st, T(L.thickness));
+    result.s = mmul(result.s, x.s);
+    result.p = mmul(result.p, x.p);
   }
   return result;
 }
@@ -435,35 +480,41 @@ OPTK_HD MultilayerOut<T> multilayer_rt(const StackView &sv, int stack_index, T w
                                        C<T> direction, C<T> n_ambient) {
   const optk_stack &stk = sv.stacks[stack_index];
   MultilayerOut<T> out;
-  for (int p = 0; p < 2; ++p) {
-    bool pol_s = (p == 0);
-    LayerState<T> st = {n_ambient, direction, true};
-    M2<T> m = midentity<T>();
-    bool first_block = true;
-    for (int b = 0; b < stk.num_blocks; ++b) {
-      const optk_block &blk = stk.blocks[b];
-      M2<T> x;
-      if (blk.kind == OPTK_BLOCK_PERIODIC) {
-        M2<T> start =
-            sequence_transfer<T>(sv, blk.first_layer, blk.num_layers, wavelength, pol_s, st);
-        M2<T> periodic =
-            sequence_transfer<T>(sv, blk.first_layer, blk.num_layers, wavelength, pol_s, st);
-        x = mmul(start, matrix_power(periodic, blk.num_periods - 1));
-      } else {
-        x = sequence_transfer<T>(sv, blk.first_layer, blk.num_layers, wavelength, pol_s, st);
-      }
-      // a single top-level block is returned by its own transfer(); several
-      // blocks are folded left from the identity (LayerSequence.transfer)
-      m = (first_block && stk.num_blocks == 1) ? x : mmul(m, x);
-      first_block = false;
+  LayerState<T> st = {n_ambient, direction, true};
+  M2Pair<T> m = {midentity<T>(), midentity<T>()};
+  bool first_block = true;
+  for (int b = 0; b < stk.num_blocks; ++b) {
+    const optk_block &blk = stk.blocks[b];
+    C<T> n_cache[4];
+    M2Pair<T> x;
+    if (blk.kind == OPTK_BLOCK_PERIODIC) {
+      M2Pair<T> start =
+          sequence_transfer2<T>(sv, blk.first_layer, blk.num_layers, wavelength, st, n_cache, false);
+      M2Pair<T> periodic =
+          sequence_transfer2<T>(sv, blk.first_layer, blk.num_layers, wavelength, st, n_cache, true);
+      x.s = mmul(start.s, matrix_power(periodic.s, blk.num_periods - 1));
+      x.p = mmul(start.p, matrix_power(periodic.p, blk.num_periods - 1));
+    } else {
+      x = sequence_transfer2<T>(sv, blk.first_layer, blk.num_layers, wavelength, st, n_cache, false);
     }
-    // substrate (thickness forced to zero)
-    optk_layer sub = {-1, OPTK_PROFILE_NONE, 0.0, 0.0};
-    if (stk.substrate_layer >= 0) sub = sv.layers[stk.substrate_layer];
-    M2<T> ms = layer_transfer<T>(sub, sv, wavelength, pol_s, st, T(0));
-    M2<T> mt = mmul(m, ms);
-    out.r[p] = cdiv(mt.c, mt.a);
-    C<T> t = cdiv(C<T>{T(1), T(0)}, mt.a);
+    // a single top-level block is returned by its own transfer(); several
+    // blocks are folded left from the identity (LayerSequence.transfer)
+    if (first_block && stk.num_blocks == 1) {
+      m = x;
+    } else {
+      m.s = mmul(m.s, x.s);
+      m.p = mmul(m.p, x.p);
+    }
+    first_block = false;
+  }
+  // substrate (thickness forced to zero)
+  optk_layer sub = {-1, OPTK_PROFILE_NONE, 0.0, 0.0};
+  if (stk.substrate_layer >= 0) sub = sv.layers[stk.substrate_layer];
+  M2Pair<T> ms = layer_transfer2<T>(sub, layer_index<T>(sub, sv, wavelength), wavelength, st, T(0));
+  M2<T> mt[2] = {mmul(m.s, ms.s), mmul(m.p, ms.p)};
+  for (int p = 0; p < 2; ++p) {
+    out.r[p] = cdiv(mt[p].c, mt[p].a);
+    C<T> t = cdiv(C<T>{T(1), T(0)}, mt[p].a);
     if (!st.where) t = {T(0), T(0)};
     out.t[p] = t;
     out.where[p] = st.where;

@@ -423,6 +423,7 @@ class Surrogate:
     def train(self, epochs: float = 1.0, batch: int = 65536, lr: float = 2e-3,
               weights=(1.0, 1.0, 0.25), log_every: int = 0) -> list:
         """Adam over the prepared rays (cosine-decayed learning rate)."""
+        self.invalidate()
         n = self.train_u.shape[0]
         steps = max(1, int(epochs * n / batch))
         w = torch.tensor(weights, dtype=torch.float32, device=self.dev)
@@ -485,6 +486,7 @@ class Surrogate:
         whose worst validation error exceeds the bound."""
         if ref is None:
             ref = self.trace64(fx, fy, px, py, wl)
+        self.invalidate()
         self.err_map = None
         p = self.predict(fx, fy, px, py, wl)
         ok = ref["unvignetted"] & p["unvignetted"] & ~p["fallback"]
@@ -516,3 +518,97 @@ class Surrogate:
             p["intensity"] = p["intensity"].index_copy(0, fb, r.intensity.double())
             p["unvignetted"] = p["unvignetted"].index_copy(0, fb, r.unvignetted)
         return p
+
+    # -- fused inference -----------------------------------------------------
+    def _fused_desc(self):
+        """The optk_surrogate_desc of the trained model (weights are read in
+        place from the parameter tensors)."""
+        L = _lib
+        d = L.SurrogateDesc()
+        m = self.mlp
+        d.w1 = m.view(m.p, 0, 0).data_ptr()
+        d.b1 = m.view(m.p, 0, 1).data_ptr()
+        for k, layer in enumerate((1, 2, 3)):
+            d.w16[k] = m.view(m.p16, layer, 0).data_ptr()
+            d.b[k] = m.view(m.p, layer, 1).data_ptr()
+        d.w5 = m.view(m.p, 4, 0).data_ptr()
+        d.b5 = m.view(m.p, 4, 1).data_ptr()
+        T = 3 + 2 * len(self.relevant)
+        if len(self.relevant) > L.SURROGATE_MAX_APERTURES:
+            raise ValueError("too many apertures within reach for the fused kernel")
+        cols = [0, 1, 2]
+        for i in self.relevant:
+            cols += [3 + 2 * i, 4 + 2 * i]
+        coef = torch.zeros(NUM_POLY, L.SURROGATE_MAX_TARGETS, dtype=torch.float32, device=self.dev)
+        coef[:, :T] = self.coef[:, cols].float()
+        self._coef32 = coef.contiguous()
+        d.poly = self._coef32.data_ptr()
+        if self.err_map is not None:
+            self._emap = self.err_map.to(torch.uint8).contiguous()
+            d.error_map = self._emap.data_ptr()
+        mid = (self.norm.lo + self.norm.hi) / 2
+        half = (self.norm.hi - self.norm.lo) / 2
+        for k in range(5):
+            d.in_mid[k] = float(mid[k])
+            d.in_inv_half[k] = float(1.0 / half[k])
+        for k in range(3):
+            d.scale[k] = float(self.scale[k])
+        d.edge_margin = self.edge_margin
+        for t, e in enumerate(EXPONENTS):
+            d.poly_exp[t] = sum(int(k) << (2 * v) for v, k in enumerate(e))
+        d.num_targets = T
+        d.num_apertures = len(self.relevant)
+        for a, i in enumerate(self.relevant):
+            e = self.edges[i]
+            ap = d.apertures[a]
+            ap.kind = e.kind if e.kind in (1, 4, 5) else 0
+            ap.inverted = 1 if e.inverted else 0
+            for k in range(min(4, len(e.params))):
+                ap.p[k] = float(e.params[k])
+            if e.vertices is not None:
+                if len(e.vertices) > L.SURROGATE_MAX_VERTICES:
+                    raise ValueError("polygon with too many vertices for the fused kernel")
+                ap.num_vertices = len(e.vertices)
+                for v, (vx, vy) in enumerate(e.vertices):
+                    ap.vx[v], ap.vy[v] = float(vx), float(vy)
+        return d
+
+    def infer(self, fx, fy, px, py, wl, out=None, want_list=True):
+        """Fused inference (one kernel): float32 device inputs; returns a
+        dict of x, y, intensity (float32), unvignetted, fallback (bool) and
+        the fallback ray indices (when ``want_list``)."""
+        n = fx.numel()
+        dev = self.dev
+        if out is None:
+            out = {"x": torch.empty(n, dtype=torch.float32, device=dev),
+                   "y": torch.empty(n, dtype=torch.float32, device=dev),
+                   "intensity": torch.empty(n, dtype=torch.float32, device=dev),
+                   "unvignetted": torch.empty(n, dtype=torch.bool, device=dev),
+                   "fallback": torch.empty(n, dtype=torch.bool, device=dev),
+                   "list": torch.empty(n if want_list else 0, dtype=torch.int32, device=dev),
+                   "count": torch.zeros(1, dtype=torch.int32, device=dev)}
+        else:
+            out["count"].zero_()
+        if not hasattr(self, "_desc_cache"):
+            self._desc_cache = self._fused_desc()
+        io = _lib.SurrogateIO()
+        for name, t in (("field_x", fx), ("field_y", fy), ("pupil_x", px), ("pupil_y", py),
+                        ("wavelength", wl)):
+            if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != n:
+                raise ValueError(f"{name}: expected a contiguous float32 tensor of {n} rays")
+            setattr(io, name, t.data_ptr())
+        io.x, io.y, io.intensity = out["x"].data_ptr(), out["y"].data_ptr(), out["intensity"].data_ptr()
+        io.unvignetted, io.fallback = out["unvignetted"].data_ptr(), out["fallback"].data_ptr()
+        if want_list:
+            io.fallback_list = out["list"].data_ptr()
+            io.fallback_count = out["count"].data_ptr()
+            io.fallback_capacity = out["list"].numel()
+        io.n = n
+        _lib.check(_lib.lib().optk_surrogate_infer(ctypes.byref(self._desc_cache), ctypes.byref(io),
+                                                   _stream()))
+        return out
+
+    def invalidate(self):
+        """Forget the fused descriptor (after retraining or recalibrating)."""
+        if hasattr(self, "_desc_cache"):
+            del self._desc_cache

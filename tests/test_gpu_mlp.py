@@ -133,3 +133,29 @@ def test_surrogate_with_fallback_meets_bound(trained):
           f"fallback fraction {frac:.3f}", flush=True)
     assert rms <= 0.05
     assert agree >= 0.999
+
+
+def test_fused_inference_matches_layered_path(trained):
+    """The one-kernel inference against the layer-by-layer kernels and the
+    float64 polynomial: positions within 1e-3 of the residual scale plus the
+    float32 polynomial's rounding, identical fallback decisions except at
+    the margin."""
+    sg = trained
+    ins = _eval_grid(100_000, seed=80)
+    ref = sg.predict(*ins)
+    f32 = [t.float().contiguous() for t in ins]
+    out = sg.infer(*f32)
+    torch.cuda.synchronize()
+    ok = ~ref["fallback"] & ~out["fallback"]
+    tol = 1e-3 * float(sg.scale[:2].max()) + 2e-5
+    dx = torch.abs(out["x"].double() - ref["x"])[ok]
+    dy = torch.abs(out["y"].double() - ref["y"])[ok]
+    print(f"fused vs layered: max |dx| {dx.max().item():.2e} mm, |dy| {dy.max().item():.2e} mm, "
+          f"tol {tol:.2e}", flush=True)
+    assert dx.max().item() <= tol and dy.max().item() <= tol
+    agree = (out["fallback"] == ref["fallback"]).double().mean().item()
+    assert agree > 0.999
+    count = int(out["count"].item())
+    assert count == int(out["fallback"].sum().item())
+    lst = out["list"][:count].long()
+    assert torch.all(out["fallback"][lst]) and torch.unique(lst).numel() == count
