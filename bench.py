@@ -189,7 +189,7 @@ def calibrate_copy(dev):
 
 
 def two_forms(args, system, grid, n, surfaces, paths, kernel32, kernel64, traffic32=None,
-              traffic64=None):
+              traffic64=None, sensor=None):
     """Time the float64 and float32 forms of a grid trace on the same
     (float32-representable) inputs and compare them over every ray."""
     import torch
@@ -216,6 +216,21 @@ def two_forms(args, system, grid, n, surfaces, paths, kernel32, kernel64, traffi
         errs.append(spot_error(a, b))
     torch.cuda.synchronize()
     log(f"f32 vs f64: {errs}")
+    hist = None
+    if sensor is not None:
+        from optika_amd import _collect
+
+        # the sensor image (optk_collect) of both forms' rays
+        edges, half, num = sensor
+        args_h = (edges, (-half[0], half[0]), (-half[1], half[1]), num[0], num[1])
+        img64, _ = _collect.histogram(r64, *args_h)
+        ms_h, _ = time_calls(lambda: _collect.histogram(r32, *args_h), 5, 1, stream)
+        img32, _ = _collect.histogram(r32, *args_h)
+        d = (img32 - img64).abs().sum().item()
+        hist = {"pixels": f"{num[0]}x{num[1]}", "ms_per_image": ms_h,
+                "f64_total": img64.sum().item(), "f32_total": img32.sum().item(),
+                "l1_difference_rel": d / max(img64.abs().sum().item(), 1e-300)}
+        log(f"histogram: {hist}")
     units = n * paths * surfaces
     value64 = units / (ms64 / 1e3)
     value32 = units / (ms32 / 1e3)
@@ -242,6 +257,8 @@ def two_forms(args, system, grid, n, surfaces, paths, kernel32, kernel64, traffi
                 "speedup_vs_f64": value32 / value64, "error_vs_f64": err},
         "surrogate": None,
     }
+    if hist is not None:
+        forms["sensor_histogram"] = hist
     return value32, ms32, forms, roofline
 
 
@@ -256,7 +273,8 @@ def run_newtonian(args):
     is_2p24 = n == 1 << 24
     value, ms, forms, roofline = two_forms(
         args, system, grid, n, S, 1, "trace_grid_f32<0, false>", "trace_grid_f64<0, false>",
-        TRAFFIC_BYTES_F32_2P24 if is_2p24 else None, TRAFFIC_BYTES_F64_2P24 if is_2p24 else None)
+        TRAFFIC_BYTES_F32_2P24 if is_2p24 else None, TRAFFIC_BYTES_F64_2P24 if is_2p24 else None,
+        sensor=([400.0, 700.0], (15.36, 15.36), (2048, 2048)))
     if args.calibrate_copy:
         import torch
 

@@ -85,9 +85,10 @@ def test_batch_matches_single_traces(batch_case, precision):
             r = ref[p]
             got_m = out.t["unvignetted"][p * n + lo:p * n + hi]
             assert torch.equal(got_m, r.unvignetted), f"instance {i} path {p}: masks"
+            pos, dirs = list(r.position), list(r.direction)
             for k in range(3):
                 a = out.t["position"][k][p * n + lo:p * n + hi].double()
-                e = r.position[k].double()
+                e = pos[k].double()
                 fin = torch.isfinite(e)
                 if precision == "f64":
                     tol = 1e-12 * torch.clamp(torch.abs(e[fin]), min=1.0)
@@ -95,7 +96,7 @@ def test_batch_matches_single_traces(batch_case, precision):
                     tol = 1e-5 + 1e-7 * torch.abs(e[fin])
                 assert torch.all(torch.abs(a[fin] - e[fin]) <= tol), f"instance {i} path {p} pos {k}"
                 a = out.t["direction"][k][p * n + lo:p * n + hi].double()
-                e = r.direction[k].double()
+                e = dirs[k].double()
                 fin = torch.isfinite(e)
                 tol = 1e-12 if precision == "f64" else 1e-7
                 assert torch.all(torch.abs(a[fin] - e[fin]) <= tol), f"instance {i} path {p} dir {k}"

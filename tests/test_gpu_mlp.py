@@ -113,8 +113,8 @@ def test_network_beats_polynomial_alone(trained):
     e_poly = torch.sqrt((poly[:, 0] - ref["x"]) ** 2 + (poly[:, 1] - ref["y"]) ** 2)[ok] / sg.pixel
     rms_net = torch.sqrt((e_net ** 2).mean()).item()
     rms_poly = torch.sqrt((e_poly ** 2).mean()).item()
-    print(f"ablation: RMS spot error polynomial alone {rms_poly:.4f} px, with network "
-          f"{rms_net:.4f} px ({rms_poly / rms_net:.1f}x)", flush=True)
+    print(f"ablation: RMS spot error polynomial alone {rms_poly:.3e} px, with network "
+          f"{rms_net:.3e} px ({rms_poly / rms_net:.1f}x)", flush=True)
     assert rms_net < 0.5 * rms_poly
 
 
@@ -129,7 +129,7 @@ def test_surrogate_with_fallback_meets_bound(trained):
     rms = torch.sqrt((err ** 2).mean()).item()
     agree = (ref["unvignetted"] == p["unvignetted"]).double().mean().item()
     frac = sg.predict(*ins)["fallback"].double().mean().item()
-    print(f"surrogate + fallback: RMS {rms:.4f} px, mask agreement {agree:.5f}, "
+    print(f"surrogate + fallback: RMS {rms:.3e} px, mask agreement {agree:.5f}, "
           f"fallback fraction {frac:.3f}", flush=True)
     assert rms <= 0.05
     assert agree >= 0.999
