@@ -57,7 +57,7 @@ _SOURCES = {
     "optk_lib.hip": ["-ffp-contract=off"],
     "optk_fast.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "optk_batch.hip": ["-ffp-contract=off"],
-    "optk_mlp.hip": ["-ffp-contract=fast"],
+    "optk_mlp.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 
 

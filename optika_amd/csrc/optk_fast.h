@@ -812,6 +812,23 @@ OPTK_HD float fast_multilayer(const StackView &sv, const float *tables32, int st
   return 0.5f * (ts * gs + tp * gp);
 }
 
+// The float64 multilayer (optk_physics.h) for the rays the float32 pass
+// hands back.  Kept out of line on the device: its registers are then
+// allocated for this rare path alone instead of raising the pressure (and
+// the spills) of every ray's main path.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __attribute__((noinline))
+#else
+inline
+#endif
+float multilayer_fallback64(const StackView &sv, int stack, float wavelength, double cos_theta,
+                            float index, float attenuation, bool film) {
+  double k64 = (double)attenuation * (double)wavelength / (4.0 * M_PI) * 1e-6;
+  MultilayerRT<double> rt = multilayer_efficiency<double>(
+      sv, stack, (double)wavelength, C<double>{cos_theta, 0.0}, C<double>{(double)index, k64}, film);
+  return (float)(film ? (rt.T[0] + rt.T[1]) / 2.0 : (rt.R[0] + rt.R[1]) / 2.0);
+}
+
 template <int F>
 OPTK_HD float fast_material_efficiency(const optk_surface &s, const StackView &sv,
                                       const float *tables32, float wavelength, V3<double> a,
@@ -831,12 +848,8 @@ OPTK_HD float fast_material_efficiency(const optk_surface &s, const StackView &s
                                 Cf{index, k}, film);
     if (eff < 0.0f) {
       // grazing incidence or the validity guard: the float64 routine
-      double k64 = (double)attenuation * (double)wavelength / (4.0 * M_PI) * 1e-6;
       double c64 = -(a.x * normal.x + a.y * normal.y + a.z * normal.z);
-      MultilayerRT<double> rt = multilayer_efficiency<double>(
-          sv, s.stack, (double)wavelength, C<double>{c64, 0.0}, C<double>{(double)index, k64},
-          film);
-      eff = (float)(film ? (rt.T[0] + rt.T[1]) / 2.0 : (rt.R[0] + rt.R[1]) / 2.0);
+      eff = multilayer_fallback64(sv, s.stack, wavelength, c64, index, attenuation, film);
     }
     return eff;
   }
@@ -967,18 +980,36 @@ OPTK_HD void fast_propagate(const FastSurface &f, const optk_surface &s, const F
   }
 }
 
-// optika.direction (optika/_util.py:41-73) for the float32 inputs, returned
-// as a float64 unit vector
+// sin and cos of a float64 angle: Maclaurin series through x^11 / x^12 for
+// |x| <= 0.2 rad (truncation below 1e-18 relative); beyond, the float32
+// library functions (about 1e-7 relative: field angles of imaging systems
+// are small, and the float64 library sincos would cost the whole kernel
+// registers for a branch it rarely takes)
+OPTK_HD void small_sincos(double x, double &s, double &c) {
+  if (fabs(x) <= 0.2) {
+    const double x2 = x * x;
+    s = x * (1.0 - x2 / 6.0 * (1.0 - x2 / 20.0 * (1.0 - x2 / 42.0 * (1.0 - x2 / 72.0 *
+                                                                          (1.0 - x2 / 110.0)))));
+    c = 1.0 - x2 / 2.0 * (1.0 - x2 / 12.0 * (1.0 - x2 / 30.0 * (1.0 - x2 / 56.0 *
+                                                                    (1.0 - x2 / 90.0 * (1.0 - x2 / 132.0)))));
+    return;
+  }
+  float sf, cf;
+  sincosf((float)x, &sf, &cf);
+  s = sf;
+  c = cf;
+}
+
+// optika.direction (optika/_util.py:41-73) for the float32 inputs, in
+// float64
 OPTK_HD void fast_direction(float ax_deg, float ay_deg, double &dx, double &dy, double &dz) {
-  const float deg = 0.017453292519943295f;
-  float sx, cx, sy, cy;
-  sincosf(ax_deg * deg, &sx, &cx);
-  sincosf(ay_deg * deg, &sy, &cy);
-  double x = -(double)cy * (double)sx, y = -(double)sy, z = (double)cy * (double)cx;
-  double inv = rsqrt64(x * x + y * y + z * z);
-  dx = x * inv;
-  dy = y * inv;
-  dz = z * inv;
+  const double deg = 0.017453292519943295;  // pi / 180
+  double sx, cx, sy, cy;
+  small_sincos((double)ax_deg * deg, sx, cx);
+  small_sincos((double)ay_deg * deg, sy, cy);
+  dx = -cy * sx;
+  dy = -sy;
+  dz = cy * cx;
 }
 
 OPTK_HD FRay fast_grid_ray(float fx, float fy, float ux, float uy, float wavelength,

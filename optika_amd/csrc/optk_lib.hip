@@ -429,31 +429,61 @@ __device__ __forceinline__ int edges_bin(double x, const double *e, int num) {
   return lo;
 }
 
+// One ray per thread.  Rays of a wavefront that land in the same bin are
+// combined before the atomic: the wave picks the first pending lane's bin,
+// sums the weights of every lane in that bin with shuffles, and one lane
+// adds the sums.  Images of point sources (many rays per pixel in a row)
+// then cost one atomic per pixel per wave instead of one per ray; when a
+// round finds fewer than 4 lanes in a bin the remaining lanes (scattered
+// rays) use their own atomics.
 __global__ __launch_bounds__(256) void collect_kernel(const CollectParams C) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.n) return;
-  if (C.unvignetted && !C.unvignetted[i]) return;
-  double x, y, dz, w, inten;
-  if (C.px) {
-    x = C.px[i];
-    y = C.py[i];
-    dz = C.dz[i];
-    w = C.wavelength[i];
-    inten = C.intensity[i];
-  } else {
-    x = C.pxf[i];
-    y = C.pyf[i];
-    dz = C.dzf[i];
-    w = C.wavelengthf[i];
-    inten = C.intensityf[i];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int64_t idx = -1;
+  double inten = 0.0, dz = 0.0;
+  if (i < C.n && !(C.unvignetted && !C.unvignetted[i])) {
+    double x, y, w;
+    if (C.px) {
+      x = C.px[i];
+      y = C.py[i];
+      dz = C.dz[i];
+      w = C.wavelength[i];
+      inten = C.intensity[i];
+    } else {
+      x = C.pxf[i];
+      y = C.pyf[i];
+      dz = C.dzf[i];
+      w = C.wavelengthf[i];
+      inten = C.intensityf[i];
+    }
+    int bw = edges_bin(w, C.wedges, C.nw);
+    int bx = linspace_bin(x, C.x_lo, C.x_hi, C.nx);
+    int by = linspace_bin(y, C.y_lo, C.y_hi, C.ny);
+    if (bw >= 0 && bx >= 0 && by >= 0) idx = ((int64_t)bw * C.nx + bx) * C.ny + by;
   }
-  int bw = edges_bin(w, C.wedges, C.nw);
-  int bx = linspace_bin(x, C.x_lo, C.x_hi, C.nx);
-  int by = linspace_bin(y, C.y_lo, C.y_hi, C.ny);
-  if (bw < 0 || bx < 0 || by < 0) return;
-  int64_t idx = ((int64_t)bw * C.nx + bx) * C.ny + by;
-  unsafeAtomicAdd(&C.image[idx], inten);
-  unsafeAtomicAdd(&C.moment[idx], inten * dz);
+  uint64_t pending = __ballot(idx >= 0);
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const int64_t key = __shfl(idx, leader, 64);
+    const bool mine = idx == key && ((pending >> lane) & 1ull);
+    const uint64_t group = __ballot(mine);
+    if (__popcll(group) < 4) break;  // scattered: per-lane atomics below
+    double a = mine ? inten : 0.0;
+    double b = mine ? inten * dz : 0.0;
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, 64);
+      b += __shfl_xor(b, off, 64);
+    }
+    if (lane == leader) {
+      unsafeAtomicAdd(&C.image[key], a);
+      unsafeAtomicAdd(&C.moment[key], b);
+    }
+    pending &= ~group;
+  }
+  if ((pending >> lane) & 1ull) {
+    unsafeAtomicAdd(&C.image[idx], inten);
+    unsafeAtomicAdd(&C.moment[idx], inten * dz);
+  }
 }
 
 // ---------------------------------------------------------------------------

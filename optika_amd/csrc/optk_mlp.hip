@@ -21,12 +21,12 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-// float -> bfloat16 bits, round to nearest even (NaN stays NaN)
-__device__ __forceinline__ uint16_t to_bf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+// float -> bfloat16 bits, round to nearest even (gfx950 v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint16_t to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
+// two floats -> two bfloat16 in one 32-bit word (a in the low half)
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+  return (uint32_t)to_bf16(a) | ((uint32_t)to_bf16(b) << 16);
 }
 
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
@@ -306,51 +306,63 @@ unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 // aperture-edge distances, the error-map lookup and the compaction of the
 // rays that fall back to the trace -- one kernel, activations in LDS.
 //
-// Block: 512 threads (8 waves), a tile of 256 rays.  Layer 1 (5 -> 256) on
-// VALU in float32 writes bf16 activations act[ray][feature] to LDS.  Each
-// hidden layer is a 256 x 256 x 256 product: wave w computes rays
-// [128 (w / 4), +128) x features [64 (w % 4), +64) with 8 x 4 MFMA 16x16x32
-// tiles, A operands (activations) from LDS, B operands (weights, row-major
-// [out][in], so k runs contiguously) straight from global memory (L2).
-// After a barrier the waves overwrite act with bias + SiLU of their tile.
-// The output layer, polynomial and edge tests run one thread per ray.
+// Block: 256 threads (4 waves), a tile of 128 rays; two blocks share a CU,
+// so one block's MFMA phase overlaps the other's VALU phase (SiLU, layer 1,
+// the output layer and the polynomial).  Layer 1 (5 -> 256) on VALU in
+// float32 writes bf16 activations act[ray][feature] to LDS.  Each hidden
+// layer is a 128 x 256 x 256 product: wave w computes all 128 rays x
+// features [64 w, 64 w + 64) as 8 x 4 MFMA 16x16x32 tiles, A operands
+// (activations) from LDS, B operands (weights, row-major [out][in], k
+// contiguous) straight from global memory (L2-resident).  The B operand of
+// n-tile t and lane column c is weight row 64 w + 4 c + t, so each lane's
+// four accumulator tiles hold four consecutive features: the epilogue
+// (bias, SiLU, bf16) writes them as one 8-byte store per row.  The output
+// layer, polynomial and edge tests run one thread per ray.
 // ---------------------------------------------------------------------------
 
-constexpr int INF_RAYS = 256;
+constexpr int INF_RAYS = 128;
 constexpr int INF_HIDDEN = 256;
 constexpr int INF_PAD = 8;
 
-__global__ __launch_bounds__(512) void surrogate_infer(const optk_surrogate_desc D, const optk_surrogate_io IO) {
+__global__ __launch_bounds__(256, 2) void surrogate_infer(const optk_surrogate_desc D,
+                                                          const optk_surrogate_io IO) {
   __shared__ __attribute__((aligned(16))) uint16_t act[INF_RAYS][INF_HIDDEN + INF_PAD];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int64_t n = IO.n;
   const int64_t tiles = (n + INF_RAYS - 1) / INF_RAYS;
+  const float *in[5] = {IO.field_x, IO.field_y, IO.pupil_x, IO.pupil_y, IO.wavelength};
   for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const int64_t base = tile * INF_RAYS;
-    // ---- layer 1 ------------------------------------------------------
+    // ---- layer 1: thread t -> ray t % 128, features [128 (t / 128), +128) ----
     {
       const int r = tid & (INF_RAYS - 1);
       const int64_t i = base + r;
       float u[5];
-      const float *in[5] = {IO.field_x, IO.field_y, IO.pupil_x, IO.pupil_y, IO.wavelength};
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
         float v = i < n ? in[k][i] : 0.0f;
         u[k] = (v - D.in_mid[k]) * D.in_inv_half[k];
       }
-      const int f0 = (tid >> 8) * (INF_HIDDEN / 2);
-      for (int f = f0; f < f0 + INF_HIDDEN / 2; ++f) {
-        float v = D.b1[f];
+      const int f0 = (tid >> 7) * (INF_HIDDEN / 2);
+      for (int f = f0; f < f0 + INF_HIDDEN / 2; f += 8) {
+        uint32_t w[4];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) v += D.w1[f * 5 + k] * u[k];
-        act[r][f] = to_bf16(silu(v));
+        for (int q = 0; q < 4; ++q) {
+          float v0 = D.b1[f + 2 * q], v1 = D.b1[f + 2 * q + 1];
+#pragma unroll
+          for (int k = 0; k < 5; ++k) {
+            v0 += D.w1[(f + 2 * q) * 5 + k] * u[k];
+            v1 += D.w1[(f + 2 * q + 1) * 5 + k] * u[k];
+          }
+          w[q] = pack_bf16(silu(v0), silu(v1));
+        }
+        *reinterpret_cast<uint4 *>(&act[r][f]) = uint4{w[0], w[1], w[2], w[3]};
       }
     }
     __syncthreads();
     // ---- hidden layers ------------------------------------------------
-    const int rbase = (wave >> 2) * 128;
-    const int obase = (wave & 3) * 64;
+    const int obase = wave * 64;
     const int fr = lane & 15, fk = (lane >> 4) * 8;
     for (int layer = 0; layer < 3; ++layer) {
       const uint16_t *W = D.w16[layer];
@@ -363,12 +375,12 @@ __global__ __launch_bounds__(512) void surrogate_infer(const optk_surrogate_desc
         bf16x8 bw[4];
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) {
-          uint4 v = *reinterpret_cast<const uint4 *>(W + (int64_t)(obase + 16 * nt + fr) * INF_HIDDEN + kc + fk);
+          uint4 v = *reinterpret_cast<const uint4 *>(W + (int64_t)(obase + 4 * fr + nt) * INF_HIDDEN + kc + fk);
           bw[nt] = __builtin_bit_cast(bf16x8, v);
         }
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt) {
-          uint4 v = *reinterpret_cast<const uint4 *>(&act[rbase + 16 * mt + fr][kc + fk]);
+          uint4 v = *reinterpret_cast<const uint4 *>(&act[16 * mt + fr][kc + fk]);
           bf16x8 a = __builtin_bit_cast(bf16x8, v);
 #pragma unroll
           for (int nt = 0; nt < 4; ++nt)
@@ -376,45 +388,46 @@ __global__ __launch_bounds__(512) void surrogate_infer(const optk_surrogate_desc
         }
       }
       __syncthreads();  // every wave has read this layer's inputs
-      const float *bias = D.b[layer];
+      // lane (row group g, column c): features obase + 4 c + {0..3}
+      const int col = obase + 4 * fr;
+      const float b0 = D.b[layer][col], b1 = D.b[layer][col + 1];
+      const float b2 = D.b[layer][col + 2], b3 = D.b[layer][col + 3];
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-        const int col = obase + 16 * nt + (lane & 15);
-        const float bcol = bias[col];
+      for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
-        for (int mt = 0; mt < 8; ++mt)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int row = rbase + 16 * mt + 4 * (lane >> 4) + q;
-            act[row][col] = to_bf16(silu(acc[mt][nt][q] + bcol));
-          }
-      }
+        for (int q = 0; q < 4; ++q) {
+          const int row = 16 * mt + 4 * (lane >> 4) + q;
+          uint2 v;
+          v.x = pack_bf16(silu(acc[mt][0][q] + b0), silu(acc[mt][1][q] + b1));
+          v.y = pack_bf16(silu(acc[mt][2][q] + b2), silu(acc[mt][3][q] + b3));
+          *reinterpret_cast<uint2 *>(&act[row][col]) = v;
+        }
       __syncthreads();
     }
     // ---- output layer, polynomial, apertures, fallback ------------------
     bool fallback = false;
-    int64_t i = base + tid;
+    const int64_t i = base + tid;
     const bool live = tid < INF_RAYS && i < n;
     if (live) {
-      float y[3] = {D.b5[0], D.b5[1], D.b5[2]};
+      float y0 = D.b5[0], y1 = D.b5[1], y2 = D.b5[2];
       for (int f = 0; f < INF_HIDDEN; f += 8) {
         uint4 v = *reinterpret_cast<const uint4 *>(&act[tid][f]);
-        const uint16_t *h = reinterpret_cast<const uint16_t *>(&v);
+        const uint32_t h[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-          const float a = from_bf16(h[q]);
-          y[0] += a * D.w5[f + q];
-          y[1] += a * D.w5[INF_HIDDEN + f + q];
-          y[2] += a * D.w5[2 * INF_HIDDEN + f + q];
+          const float a = __uint_as_float((q & 1) ? (h[q >> 1] & 0xffff0000u) : (h[q >> 1] << 16));
+          y0 += a * D.w5[f + q];
+          y1 += a * D.w5[INF_HIDDEN + f + q];
+          y2 += a * D.w5[2 * INF_HIDDEN + f + q];
         }
       }
       float u[5];
-      const float *in[5] = {IO.field_x, IO.field_y, IO.pupil_x, IO.pupil_y, IO.wavelength};
 #pragma unroll
       for (int k = 0; k < 5; ++k) u[k] = (in[k][i] - D.in_mid[k]) * D.in_inv_half[k];
       // cubic monomials, in the order of optk_surrogate_desc.poly_exp
       float tgt[OPTK_SURROGATE_MAX_TARGETS];
-      for (int j = 0; j < D.num_targets; ++j) tgt[j] = 0.0f;
+#pragma unroll
+      for (int j = 0; j < OPTK_SURROGATE_MAX_TARGETS; ++j) tgt[j] = 0.0f;
       float pw[5][4];
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
@@ -423,17 +436,27 @@ __global__ __launch_bounds__(512) void surrogate_infer(const optk_surrogate_desc
         pw[k][2] = u[k] * u[k];
         pw[k][3] = pw[k][2] * u[k];
       }
+      const int nt = D.num_targets;
       for (int t = 0; t < OPTK_SURROGATE_POLY; ++t) {
         const uint32_t e = D.poly_exp[t];  // 2 bits per variable
-        float mono = pw[0][e & 3] * pw[1][(e >> 2) & 3] * pw[2][(e >> 4) & 3] * pw[3][(e >> 6) & 3] *
-                     pw[4][(e >> 8) & 3];
-        for (int j = 0; j < D.num_targets; ++j) tgt[j] += mono * D.poly[t * OPTK_SURROGATE_MAX_TARGETS + j];
+        float mono = 1.0f;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          const uint32_t ek = (e >> (2 * k)) & 3u;
+          mono *= ek == 0 ? 1.0f : (ek == 1 ? pw[k][1] : (ek == 2 ? pw[k][2] : pw[k][3]));
+        }
+        const float *c = D.poly + t * OPTK_SURROGATE_MAX_TARGETS;
+#pragma unroll
+        for (int j = 0; j < OPTK_SURROGATE_MAX_TARGETS; ++j)
+          if (j < nt) tgt[j] += mono * c[j];
       }
-      const float x = tgt[0] + D.scale[0] * y[0];
-      const float yy = tgt[1] + D.scale[1] * y[1];
-      const float li = tgt[2] + D.scale[2] * y[2];
+      const float x = tgt[0] + D.scale[0] * y0;
+      const float yy = tgt[1] + D.scale[1] * y1;
+      const float li = tgt[2] + D.scale[2] * y2;
       bool inside = true, near = false;
-      for (int a = 0; a < D.num_apertures; ++a) {
+#pragma unroll
+      for (int a = 0; a < OPTK_SURROGATE_MAX_APERTURES; ++a) {
+        if (a >= D.num_apertures) break;
         const optk_surrogate_aperture &ap = D.apertures[a];
         const float ax = tgt[3 + 2 * a], ay = tgt[4 + 2 * a];
         float d;
@@ -476,7 +499,7 @@ __global__ __launch_bounds__(512) void surrogate_infer(const optk_surrogate_desc
       IO.fallback[i] = fallback ? 1 : 0;
     }
     // ---- compaction of the fallback rays: one atomic per wave ------------
-    if (IO.fallback_list) {
+    if (IO.fallback_list && wave < INF_RAYS / 64) {
       const uint64_t mask = __ballot(live && fallback);
       if (mask) {
         int32_t wbase = 0;
@@ -655,9 +678,9 @@ extern "C" int32_t optk_surrogate_infer(const optk_surrogate_desc *desc, const o
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("device query: ") + hipGetErrorString(e));
   const int64_t tiles = (IO.n + INF_RAYS - 1) / INF_RAYS;
-  int64_t blocks = cus > 0 ? (int64_t)cus * 4 : 1024;
+  int64_t blocks = cus > 0 ? (int64_t)cus * 2 : 1024;  // two resident blocks per CU
   if (blocks > tiles) blocks = tiles;
-  hipLaunchKernelGGL(surrogate_infer, dim3((unsigned)blocks), dim3(512), 0, (hipStream_t)hip_stream, D, IO);
+  hipLaunchKernelGGL(surrogate_infer, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)hip_stream, D, IO);
   e = hipGetLastError();
   if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("surrogate launch: ") + hipGetErrorString(e));
   return OPTK_OK;

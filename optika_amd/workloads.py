@@ -326,7 +326,7 @@ def spectrograph_grid(n: int, seed: int = 2) -> Grid:
 
 
 def tolerance_batch(num_instances: int = 4096, seed: int = 3, min_rays: int = 1000,
-                    max_rays: int = 64000):
+                    max_rays: int = 64000, point_source: bool = True):
     """Batched tolerance analysis of the spectrograph: ``num_instances``
     perturbed copies (decenters sigma 50 um, tilts sigma 0.01 deg, radii
     sigma 0.1 %, line-spacing coefficients sigma 0.5 %), each traced with
@@ -334,11 +334,18 @@ def tolerance_batch(num_instances: int = 4096, seed: int = 3, min_rays: int = 10
 
     Returns (systems, counts, grid): the grid holds every instance's rays
     back to back (instance i owns rays offsets[i]:offsets[i+1], offsets the
-    running sum of counts), drawn like :func:`spectrograph_grid`.
+    running sum of counts).  With ``point_source`` every ray comes from the
+    on-axis field point at the centre wavelength (pupil uniform on the
+    r = 40 mm disc), so each instance's RMS spot radius is its image
+    quality; otherwise the rays are drawn like :func:`spectrograph_grid`.
     """
     rng = np.random.Generator(np.random.PCG64(seed))
     perts = [Perturbation.gaussian(rng) for _ in range(num_instances)]
     counts = np.floor(np.exp(rng.uniform(math.log(min_rays), math.log(max_rays),
                                          num_instances))).astype(np.int64)
     grid = spectrograph_grid(int(counts.sum()), seed=seed + 1000)
+    if point_source:
+        grid.field_x = np.zeros_like(grid.field_x)
+        grid.field_y = np.zeros_like(grid.field_y)
+        grid.wavelength = np.full_like(grid.wavelength, WAVELENGTH_CENTER)
     return [spectrograph(p) for p in perts], counts, grid

@@ -290,6 +290,24 @@ def test_sensor_collect_matches_histogramdd():
     np.testing.assert_allclose(cosine.cpu().numpy(), ref_cos, rtol=1e-12, atol=0)
 
 
+def test_sensor_collect_point_sources():
+    """The C1 grid: five field points, each imaged onto a few pixels, so
+    whole wavefronts of rays share a bin (the wave-combined atomics path)."""
+    g = workloads.newtonian_grid_c1()
+    s = workloads.newtonian()
+    print("case: collect C1 point sources", flush=True)
+    out = gpu_rayfunction(s, g)
+    edges = np.array([400.0, 700.0])
+    image, cosine = s.sensor.collect(out, edges)
+    ref_rays = oracle.rayfunction(s, g.wavelength, g.field_x, g.field_y, g.pupil_x, g.pupil_y)[0]
+    ref_image, ref_cos = oracle.collect(s.sensor, ref_rays, edges)
+    img = image.cpu().numpy()
+    assert np.array_equal(img > 0, ref_image > 0)
+    assert (img > 0).sum() < 200  # a few pixels per field point
+    np.testing.assert_allclose(img, ref_image, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(cosine.cpu().numpy(), ref_cos, rtol=1e-12, atol=0)
+
+
 # -- per-operator entry points --------------------------------------------------
 
 def test_eval_operators_match_oracle():
