@@ -51,7 +51,9 @@ TRAFFIC_BYTES_F64_2P24 = (342198 * 2.0 + 950396 * 1.0) * 1024
 
 # float32 form: in 5 x 4 B, out 7 x 4 B + 1 B mask
 BYTES_PER_RAY_F32 = 49
-TRAFFIC_BYTES_F32_2P24 = None  # not yet measured with rocprofv3 --pmc
+# HBM bytes per launch of trace_grid_f32 at 2^24 rays (rocprofv3 --pmc, the
+# same scaling; profiles/trace_grid_f32_newtonian_2p24.md)
+TRAFFIC_BYTES_F32_2P24 = (164156 * 2.0 + 475137 * 1.0) * 1024
 
 PIXEL_MM = 0.015
 
