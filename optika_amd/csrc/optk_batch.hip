@@ -52,6 +52,8 @@ struct optk_batch {
   optk_chemical *chemicals = nullptr;
   double *tables = nullptr;
   float *tables32 = nullptr;
+  ChemGrid *grids = nullptr;
+  int32_t *grid_lo = nullptr;
   // per instance
   optk_surface *surfaces = nullptr;
   TraceSpec *specs64 = nullptr;
@@ -92,6 +94,8 @@ void release(optk_batch *b) {
   (void)hipFree(b->chemicals);
   (void)hipFree(b->tables);
   (void)hipFree(b->tables32);
+  (void)hipFree(b->grids);
+  (void)hipFree(b->grid_lo);
   (void)hipFree(b->surfaces);
   (void)hipFree(b->specs64);
   (void)hipFree(b->fast);
@@ -237,6 +241,8 @@ int32_t enqueue(optk_batch *b, int32_t precision, const Inputs &io, double *stat
   A.specs64 = b->specs64;
   A.fast = b->fast;
   A.fast_specs = b->fast_specs;
+  A.sv.grids = b->grids;
+  A.sv.grid_lo = b->grid_lo;
   A.sv.layers = b->layers;
   A.sv.stacks = b->stacks;
   A.sv.chemicals = b->chemicals;
@@ -345,6 +351,8 @@ int32_t optk_batch_create(const optk_batch_desc *desc, optk_batch **out) {
       (rc = upload(&b->chemicals, base.chemicals, base.num_chemicals)) ||
       (rc = upload(&b->tables, base.tables, base.num_tables)) ||
       (rc = upload(&b->tables32, tables32.data(), base.num_tables)) ||
+      (rc = upload_chem_grids(base.chemicals, base.num_chemicals, base.tables, &b->grids,
+                              &b->grid_lo)) ||
       (rc = upload(&b->surfaces, surfaces.data(), (int64_t)N * S))) {
     release(b);
     delete b;
@@ -359,6 +367,8 @@ int32_t optk_batch_create(const optk_batch_desc *desc, optk_batch **out) {
     sp.sys.surfaces = nullptr;  // the kernel points it at the staged copy
     sp.sys.num_surfaces = S;
     sp.sys.num_paths = base.num_paths;
+    sp.sys.sv.grids = b->grids;
+    sp.sys.sv.grid_lo = b->grid_lo;
     sp.sys.sv.layers = b->layers;
     sp.sys.sv.stacks = b->stacks;
     sp.sys.sv.chemicals = b->chemicals;

@@ -596,7 +596,8 @@ OPTK_HD double fast_index(const optk_surface &s, double wavelength, double n1) {
 // float32 refractive index tables
 // ---------------------------------------------------------------------------
 
-OPTK_HD Cf chemical_nf(const optk_chemical &ch, const float *tables, float wavelength_nm) {
+OPTK_HD Cf chemical_nf(const optk_chemical &ch, const float *tables, float wavelength_nm,
+                       const ChemGrid *grid, const int32_t *lo_tab) {
   const float *w = tables + ch.offset;
   const float *n = w + ch.count;
   const float *k = n + ch.count;
@@ -604,13 +605,19 @@ OPTK_HD Cf chemical_nf(const optk_chemical &ch, const float *tables, float wavel
   float x = wavelength_nm * 10.0f;
   if (!(x >= w[0])) return (x != x) ? Cf{x, x} : Cf{n[0], k[0]};
   if (x >= w[cnt - 1]) return {n[cnt - 1], k[cnt - 1]};
-  int lo = 0, hi = cnt - 1;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (x < w[mid])
-      hi = mid;
-    else
-      lo = mid;
+  int lo;
+  if (grid) {
+    lo = grid_bracket(*grid, lo_tab, w, cnt, x);
+  } else {
+    lo = 0;
+    int hi = cnt - 1;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (x < w[mid])
+        hi = mid;
+      else
+        lo = mid;
+    }
   }
   float inv_dx = 1.0f / (w[lo + 1] - w[lo]);
   float dx = x - w[lo];
@@ -729,7 +736,8 @@ OPTK_HD MLPair ml_power(MLPair m, int exponent) {
 OPTK_HD Cf ml_layer_n(const optk_layer &L, const StackView &sv, const float *tables32,
                       float wavelength) {
   if (L.chemical < 0) return {1.0f, 0.0f};
-  return chemical_nf(sv.chemicals[L.chemical], tables32, wavelength);
+  return chemical_nf(sv.chemicals[L.chemical], tables32, wavelength,
+                     sv.grids ? sv.grids + L.chemical : nullptr, sv.grid_lo);
 }
 
 // Reflectance (mirror) or transmittance (film) averaged over s and p, or a

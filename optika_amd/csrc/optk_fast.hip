@@ -150,6 +150,8 @@ FastArgs make_args(const optk_system *sys, const FRaysOut &out, int64_t n, int32
   A.spec = sys->fast_spec;
   A.fast = sys->fast;
   A.surfaces = sys->surfaces;
+  A.sv.grids = sys->grids;
+  A.sv.grid_lo = sys->grid_lo;
   A.sv.layers = sys->layers;
   A.sv.stacks = sys->stacks;
   A.sv.chemicals = sys->chemicals;

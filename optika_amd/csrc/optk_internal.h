@@ -33,6 +33,8 @@ struct optk_system {
   optk::FastSurface *fast = nullptr;     // one per surface
   optk::FastSpec *fast_spec = nullptr;
   float *tables32 = nullptr;             // tables[] rounded to float32
+  optk::ChemGrid *grids = nullptr;       // n/k lookup grids, one per chemical
+  int32_t *grid_lo = nullptr;
 };
 
 namespace optk {
@@ -44,6 +46,10 @@ int32_t fail(int32_t code, const std::string &msg);
 int32_t validate_desc(const optk_system_desc *d);
 // the kernel feature variant (FEAT_*) covering the system's surfaces
 int32_t system_features(const optk_system_desc *d);
+
+// upload the n/k lookup grids of the chemicals (host tables) to the device
+int32_t upload_chem_grids(const optk_chemical *chems, int num_chemicals, const double *tables,
+                          ChemGrid **grids, int32_t **grid_lo);
 
 // float64 ray buffers (device pointers; null = use the default)
 struct RaysIn {

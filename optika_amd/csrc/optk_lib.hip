@@ -495,6 +495,8 @@ static SystemView view_of(const optk_system *s) {
   v.surfaces = s->surfaces;
   v.num_surfaces = s->num_surfaces;
   v.num_paths = s->num_paths;
+  v.sv.grids = s->grids;
+  v.sv.grid_lo = s->grid_lo;
   v.sv.layers = s->layers;
   v.sv.stacks = s->stacks;
   v.sv.chemicals = s->chemicals;
@@ -585,6 +587,25 @@ int32_t validate_desc(const optk_system_desc *d) {
 }
 
 int32_t system_features(const optk_system_desc *d) { return variant_of(features_of(d)); }
+
+int32_t upload_chem_grids(const optk_chemical *chems, int num_chemicals, const double *tables,
+                          ChemGrid **grids, int32_t **grid_lo) {
+  *grids = nullptr;
+  *grid_lo = nullptr;
+  if (num_chemicals <= 0) return OPTK_OK;
+  int64_t count = 0;
+  build_chem_grids(chems, num_chemicals, tables, nullptr, nullptr, &count);
+  std::vector<ChemGrid> g(num_chemicals);
+  std::vector<int32_t> lo(count);
+  build_chem_grids(chems, num_chemicals, tables, g.data(), lo.data(), nullptr);
+  hipError_t e = hipMalloc((void **)grids, sizeof(ChemGrid) * num_chemicals);
+  if (e == hipSuccess) e = hipMalloc((void **)grid_lo, sizeof(int32_t) * count);
+  if (e != hipSuccess) return fail(OPTK_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemcpy(*grids, g.data(), sizeof(ChemGrid) * num_chemicals, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(*grid_lo, lo.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  return OPTK_OK;
+}
 }  // namespace optk
 
 static_assert(sizeof(optk_surface) % 8 == 0 && sizeof(TraceSpec) % 8 == 0, "records are staged as words");
@@ -630,6 +651,8 @@ int32_t optk_system_destroy(optk_system *sys) {
   (void)hipFree(sys->fast);
   (void)hipFree(sys->fast_spec);
   (void)hipFree(sys->tables32);
+  (void)hipFree(sys->grids);
+  (void)hipFree(sys->grid_lo);
   delete sys;
   return OPTK_OK;
 }
@@ -662,7 +685,9 @@ int32_t optk_system_create(const optk_system_desc *desc, optk_system **out) {
       (rc = upload(&s->layers, desc->layers, desc->num_layers)) ||
       (rc = upload(&s->stacks, desc->stacks, desc->num_stacks)) ||
       (rc = upload(&s->chemicals, desc->chemicals, desc->num_chemicals)) ||
-      (rc = upload(&s->tables, desc->tables, desc->num_tables))) {
+      (rc = upload(&s->tables, desc->tables, desc->num_tables)) ||
+      (rc = upload_chem_grids(desc->chemicals, desc->num_chemicals, desc->tables, &s->grids,
+                              &s->grid_lo))) {
     optk_system_destroy(s);
     return rc;
   }

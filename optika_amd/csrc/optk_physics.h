@@ -259,10 +259,72 @@ OPTK_HD float interp1f(const float *xp, const float *fp, int count, float x) {
   return (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]) * (x - xp[lo]) + fp[lo];
 }
 
+// Index grid over one chemical's wavelength knots: `cells` uniform cells
+// over [w[0], w[count - 1]], lo[g] = the knot bracket at cell g's left
+// edge.  A lookup starts from its cell's entry and steps to the bracket
+// w[j] <= x < w[j + 1]; the bracket is unique, so the result is the binary
+// search's, in one load and a step or two instead of log2(count) dependent
+// loads.
+struct ChemGrid {
+  double w0, inv_dw;
+  int32_t cells, offset;  // offset into the lo array
+};
+
+OPTK_HD int grid_bracket(const ChemGrid &g, const int32_t *lo_tab, const double *w, int cnt,
+                         double x) {
+  int c = (int)((x - g.w0) * g.inv_dw);
+  c = c < 0 ? 0 : (c >= g.cells ? g.cells - 1 : c);
+  int j = lo_tab[g.offset + c];
+  while (j > 0 && w[j] > x) --j;
+  while (j < cnt - 2 && w[j + 1] <= x) ++j;
+  return j;
+}
+
+OPTK_HD int grid_bracket(const ChemGrid &g, const int32_t *lo_tab, const float *w, int cnt,
+                         float x) {
+  int c = (int)((x - (float)g.w0) * (float)g.inv_dw);
+  c = c < 0 ? 0 : (c >= g.cells ? g.cells - 1 : c);
+  int j = lo_tab[g.offset + c];
+  while (j > 0 && w[j] > x) --j;
+  while (j < cnt - 2 && w[j + 1] <= x) ++j;
+  return j;
+}
+
+// host: the grids of every chemical (4 cells per knot, at most 4096)
+inline void build_chem_grids(const optk_chemical *chems, int num_chemicals, const double *tables,
+                             ChemGrid *grids, int32_t *lo_tab, int64_t *lo_count) {
+  int64_t used = 0;
+  for (int c = 0; c < num_chemicals; ++c) {
+    const double *w = tables + chems[c].offset;
+    const int cnt = chems[c].count;
+    int cells = 4 * cnt;
+    cells = cells < 16 ? 16 : (cells > 4096 ? 4096 : cells);
+    ChemGrid g;
+    g.w0 = w[0];
+    const double span = w[cnt - 1] - w[0];
+    g.inv_dw = span > 0.0 ? (double)cells / span : 0.0;
+    g.cells = cells;
+    g.offset = (int32_t)used;
+    if (lo_tab) {
+      int j = 0;
+      for (int k = 0; k < cells; ++k) {
+        const double edge = w[0] + span * ((double)k / (double)cells);
+        while (j < cnt - 2 && w[j + 1] <= edge) ++j;
+        lo_tab[used + k] = j;
+      }
+    }
+    if (grids) grids[c] = g;
+    used += cells;
+  }
+  if (lo_count) *lo_count = used;
+}
+
 // n + i k of a chemical at wavelength (nm); the table is in Angstrom.
 // numpy.interp of a complex fp interpolates both parts with one reciprocal
 // of the knot spacing (numpy's complex interp loop); the same is done here.
-OPTK_HD C<double> chemical_n(const optk_chemical &ch, const double *tables, double wavelength_nm) {
+// With `grid`, the bracket comes from the chemical's index grid.
+OPTK_HD C<double> chemical_n(const optk_chemical &ch, const double *tables, double wavelength_nm,
+                             const ChemGrid *grid = nullptr, const int32_t *lo_tab = nullptr) {
   const double *w = tables + ch.offset;
   const double *n = w + ch.count;
   const double *k = n + ch.count;
@@ -273,7 +335,7 @@ OPTK_HD C<double> chemical_n(const optk_chemical &ch, const double *tables, doub
     return {n[0], k[0]};
   }
   if (x >= w[cnt - 1]) return {n[cnt - 1], k[cnt - 1]};
-  int j = interp_bracket(w, cnt, x);
+  int j = grid ? grid_bracket(*grid, lo_tab, w, cnt, x) : interp_bracket(w, cnt, x);
   if (x == w[j]) return {n[j], k[j]};
   double inv_dx = 1.0 / (w[j + 1] - w[j]);
   double sre = (n[j + 1] - n[j]) * inv_dx;
@@ -297,6 +359,8 @@ OPTK_HD C<double> chemical_n(const optk_chemical &ch, const double *tables, doub
 // ---------------------------------------------------------------------------
 
 struct StackView {
+  const ChemGrid *grids;   // per chemical, or null (binary search)
+  const int32_t *grid_lo;
   const optk_layer *layers;
   const optk_stack *stacks;
   const optk_chemical *chemicals;
@@ -436,7 +500,8 @@ template <typename T> OPTK_HD M2<T> matrix_power(M2<T> m, int exponent) {
 template <typename T>
 OPTK_HD C<T> layer_index(const optk_layer &L, const StackView &sv, T wavelength) {
   if (L.chemical < 0) return {T(1), T(0)};
-  return to_c<T>(chemical_n(sv.chemicals[L.chemical], sv.tables, (double)wavelength));
+  return to_c<T>(chemical_n(sv.chemicals[L.chemical], sv.tables, (double)wavelength,
+                            sv.grids ? sv.grids + L.chemical : nullptr, sv.grid_lo));
 }
 
 // LayerSequence.transfer over layers first .. first + count - 1, both
@@ -545,7 +610,7 @@ OPTK_HD MultilayerRT<T> multilayer_efficiency(const StackView &sv, int stack_ind
     C<T> n_sub = {T(1), T(0)};
     if (stk.substrate_layer >= 0) {
       const optk_layer &sub = sv.layers[stk.substrate_layer];
-      if (sub.chemical >= 0) n_sub = to_c<T>(chemical_n(sv.chemicals[sub.chemical], sv.tables, (double)wavelength));
+      n_sub = layer_index<T>(sub, sv, wavelength);
     }
     C<T> d_sub = snell_scalar(direction, n_ambient, n_sub);
     C<T> one = {T(1), T(0)};

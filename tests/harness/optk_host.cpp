@@ -10,8 +10,19 @@
 using namespace optk;
 
 namespace {
+// n/k lookup grids of the last system viewed (host copies)
+std::vector<ChemGrid> g_grids;
+std::vector<int32_t> g_grid_lo;
+
 SystemView view_of(const optk_system_desc *d) {
+  int64_t count = 0;
+  build_chem_grids(d->chemicals, d->num_chemicals, d->tables, nullptr, nullptr, &count);
+  g_grids.assign(d->num_chemicals > 0 ? d->num_chemicals : 1, ChemGrid());
+  g_grid_lo.assign(count > 0 ? count : 1, 0);
+  build_chem_grids(d->chemicals, d->num_chemicals, d->tables, g_grids.data(), g_grid_lo.data(), nullptr);
   SystemView v;
+  v.sv.grids = d->num_chemicals > 0 ? g_grids.data() : nullptr;
+  v.sv.grid_lo = g_grid_lo.data();
   v.surfaces = d->surfaces;
   v.num_surfaces = d->num_surfaces;
   v.num_paths = d->num_paths;
