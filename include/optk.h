@@ -483,6 +483,8 @@ typedef struct optk_surrogate_desc {
   const uint16_t *w16[3];       /* hidden layers, bf16 [256 out][256 in] */
   const float *b[3];            /* hidden biases [256] */
   const float *w5, *b5;         /* [3][256], [3] float32 */
+  const uint16_t *w5_16;        /* bf16 [16][256]: w5 = rows 0..2 + rows 3..5
+                                   (high and low parts), zeros below */
   const float *poly;            /* [OPTK_SURROGATE_POLY][OPTK_SURROGATE_MAX_TARGETS] */
   const uint8_t *error_map;     /* 16 * 8 * 8 flags, or NULL */
   float in_mid[5], in_inv_half[5];

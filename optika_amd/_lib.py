@@ -216,7 +216,8 @@ class SurrogateAperture(ctypes.Structure):
 
 class SurrogateDesc(ctypes.Structure):
     _fields_ = [("w1", c_void_p), ("b1", c_void_p), ("w16", c_void_p * 3), ("b", c_void_p * 3),
-                ("w5", c_void_p), ("b5", c_void_p), ("poly", c_void_p), ("error_map", c_void_p),
+                ("w5", c_void_p), ("b5", c_void_p), ("w5_16", c_void_p), ("poly", c_void_p),
+                ("error_map", c_void_p),
                 ("in_mid", ctypes.c_float * 5), ("in_inv_half", ctypes.c_float * 5),
                 ("scale", ctypes.c_float * 3), ("edge_margin", ctypes.c_float),
                 ("poly_exp", ctypes.c_uint32 * SURROGATE_POLY),

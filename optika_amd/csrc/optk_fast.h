@@ -995,11 +995,13 @@ OPTK_HD void fast_propagate(const FastSurface &f, const optk_surface &s, const F
 // registers for a branch it rarely takes)
 OPTK_HD void small_sincos(double x, double &s, double &c) {
   if (fabs(x) <= 0.2) {
+    // Horner form with the reciprocal factorial ratios as constants (no
+    // division instructions)
     const double x2 = x * x;
-    s = x * (1.0 - x2 / 6.0 * (1.0 - x2 / 20.0 * (1.0 - x2 / 42.0 * (1.0 - x2 / 72.0 *
-                                                                          (1.0 - x2 / 110.0)))));
-    c = 1.0 - x2 / 2.0 * (1.0 - x2 / 12.0 * (1.0 - x2 / 30.0 * (1.0 - x2 / 56.0 *
-                                                                    (1.0 - x2 / 90.0 * (1.0 - x2 / 132.0)))));
+    s = x * (1.0 - x2 * (1.0 / 6.0) * (1.0 - x2 * (1.0 / 20.0) * (1.0 - x2 * (1.0 / 42.0) *
+        (1.0 - x2 * (1.0 / 72.0) * (1.0 - x2 * (1.0 / 110.0))))));
+    c = 1.0 - x2 * 0.5 * (1.0 - x2 * (1.0 / 12.0) * (1.0 - x2 * (1.0 / 30.0) * (1.0 - x2 * (1.0 / 56.0) *
+        (1.0 - x2 * (1.0 / 90.0) * (1.0 - x2 * (1.0 / 132.0))))));
     return;
   }
   float sf, cf;

@@ -371,12 +371,22 @@ __global__ __launch_bounds__(256, 2) void surrogate_infer(const optk_surrogate_d
       for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+      // software pipeline: the next k-chunk's weights are requested before
+      // this chunk's 32 MFMAs, so the L2 latency hides behind them
+      const uint16_t *wrow[4];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) wrow[nt] = W + (int64_t)(obase + 4 * fr + nt) * INF_HIDDEN + fk;
+      uint4 bnext[4];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) bnext[nt] = *reinterpret_cast<const uint4 *>(wrow[nt]);
+#pragma unroll
       for (int kc = 0; kc < INF_HIDDEN; kc += 32) {
         bf16x8 bw[4];
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-          uint4 v = *reinterpret_cast<const uint4 *>(W + (int64_t)(obase + 4 * fr + nt) * INF_HIDDEN + kc + fk);
-          bw[nt] = __builtin_bit_cast(bf16x8, v);
+        for (int nt = 0; nt < 4; ++nt) bw[nt] = __builtin_bit_cast(bf16x8, bnext[nt]);
+        if (kc + 32 < INF_HIDDEN) {
+#pragma unroll
+          for (int nt = 0; nt < 4; ++nt) bnext[nt] = *reinterpret_cast<const uint4 *>(wrow[nt] + kc + 32);
         }
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt) {
@@ -404,23 +414,39 @@ __global__ __launch_bounds__(256, 2) void surrogate_infer(const optk_surrogate_d
         }
       __syncthreads();
     }
-    // ---- output layer, polynomial, apertures, fallback ------------------
+    // ---- output layer on MFMA: wave w -> rays [32 w, 32 w + 32).  The
+    // three outputs' weights are split into bf16 high and low parts (rows
+    // 0..2 and 3..5 of one 16-row tile, the rest zero), so the products
+    // carry the float32 weights to ~2^-17 at no extra MFMA ------------------
+    __shared__ float yout[INF_RAYS][8];
+    {
+      floatx4 oacc[2] = {floatx4{0.0f, 0.0f, 0.0f, 0.0f}, floatx4{0.0f, 0.0f, 0.0f, 0.0f}};
+#pragma unroll
+      for (int kc = 0; kc < INF_HIDDEN; kc += 32) {
+        uint4 vb = *reinterpret_cast<const uint4 *>(D.w5_16 + fr * INF_HIDDEN + kc + fk);
+        bf16x8 b5 = __builtin_bit_cast(bf16x8, vb);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          uint4 va = *reinterpret_cast<const uint4 *>(&act[32 * wave + 16 * q + fr][kc + fk]);
+          oacc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, va), b5, oacc[q], 0, 0, 0);
+        }
+      }
+      if (fr < 6) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) yout[32 * wave + 16 * q + 4 * (lane >> 4) + r][fr] = oacc[q][r];
+      }
+    }
+    __syncthreads();
+    // ---- polynomial, apertures, fallback: one thread per ray ------------
     bool fallback = false;
     const int64_t i = base + tid;
     const bool live = tid < INF_RAYS && i < n;
     if (live) {
-      float y0 = D.b5[0], y1 = D.b5[1], y2 = D.b5[2];
-      for (int f = 0; f < INF_HIDDEN; f += 8) {
-        uint4 v = *reinterpret_cast<const uint4 *>(&act[tid][f]);
-        const uint32_t h[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const float a = __uint_as_float((q & 1) ? (h[q >> 1] & 0xffff0000u) : (h[q >> 1] << 16));
-          y0 += a * D.w5[f + q];
-          y1 += a * D.w5[INF_HIDDEN + f + q];
-          y2 += a * D.w5[2 * INF_HIDDEN + f + q];
-        }
-      }
+      const float y0 = yout[tid][0] + yout[tid][3] + D.b5[0];
+      const float y1 = yout[tid][1] + yout[tid][4] + D.b5[1];
+      const float y2 = yout[tid][2] + yout[tid][5] + D.b5[2];
       float u[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) u[k] = (in[k][i] - D.in_mid[k]) * D.in_inv_half[k];
@@ -662,7 +688,8 @@ extern "C" int32_t optk_surrogate_infer(const optk_surrogate_desc *desc, const o
     return fail(OPTK_EINVAL, "missing ray array");
   if (IO.fallback_list && (!IO.fallback_count || IO.fallback_capacity < 0))
     return fail(OPTK_EINVAL, "fallback list needs a counter and a capacity");
-  if (!D.w1 || !D.b1 || !D.w5 || !D.b5) return fail(OPTK_EINVAL, "missing weights");
+  if (!D.w1 || !D.b1 || !D.w5 || !D.b5 || !D.w5_16) return fail(OPTK_EINVAL, "missing weights");
+  if ((uintptr_t)D.w5_16 % 16) return fail(OPTK_EINVAL, "bf16 weights must be 16-byte aligned");
   for (int l = 0; l < 3; ++l) {
     if (!D.w16[l] || !D.b[l]) return fail(OPTK_EINVAL, "missing weights");
     if ((uintptr_t)D.w16[l] % 16) return fail(OPTK_EINVAL, "bf16 weights must be 16-byte aligned");

@@ -91,6 +91,11 @@ def _stream():
     return _trace.stream_handle()
 
 
+def _bf16_round(x):
+    """float32 -> nearest bfloat16 (ties to even), as float32."""
+    return x.to(torch.bfloat16).to(torch.float32)
+
+
 def _op(op, f32=(), u16=(), rows=0, cols=0, inner=0, count=0, flag=0, scalars=()):
     a = _lib.MlpArgs()
     for i, t in enumerate(f32):
@@ -533,6 +538,16 @@ class Surrogate:
             d.b[k] = m.view(m.p, layer, 1).data_ptr()
         d.w5 = m.view(m.p, 4, 0).data_ptr()
         d.b5 = m.view(m.p, 4, 1).data_ptr()
+        # the output layer runs on MFMA in the fused kernel: bf16 high and
+        # low parts of the rows, padded to one 16-row tile
+        w5 = m.view(m.p, 4, 0).reshape(3, WIDTHS[4])
+        hi = _bf16_round(w5)
+        tile = torch.zeros(16, WIDTHS[4], dtype=torch.float32, device=self.dev)
+        tile[:3] = hi
+        tile[3:6] = w5 - hi
+        self._w5_16 = torch.empty(16 * WIDTHS[4], dtype=torch.int16, device=self.dev)
+        _op(_lib.MLP_TO_BF16, f32=[tile.reshape(-1)], u16=[self._w5_16], rows=1, cols=16 * WIDTHS[4])
+        d.w5_16 = self._w5_16.data_ptr()
         T = 3 + 2 * len(self.relevant)
         if len(self.relevant) > L.SURROGATE_MAX_APERTURES:
             raise ValueError("too many apertures within reach for the fused kernel")
