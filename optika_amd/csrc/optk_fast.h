@@ -606,7 +606,7 @@ OPTK_HD Cf chemical_nf(const optk_chemical &ch, const float *tables, float wavel
   if (!(x >= w[0])) return (x != x) ? Cf{x, x} : Cf{n[0], k[0]};
   if (x >= w[cnt - 1]) return {n[cnt - 1], k[cnt - 1]};
   int lo;
-  if (grid) {
+  if (grid && grid->cells) {
     lo = grid_bracket(*grid, lo_tab, w, cnt, x);
   } else {
     lo = 0;

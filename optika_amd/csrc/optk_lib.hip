@@ -596,7 +596,8 @@ int32_t upload_chem_grids(const optk_chemical *chems, int num_chemicals, const d
   int64_t count = 0;
   build_chem_grids(chems, num_chemicals, tables, nullptr, nullptr, &count);
   std::vector<ChemGrid> g(num_chemicals);
-  std::vector<int32_t> lo(count);
+  if (count == 0) count = 1;  // no chemical has a grid: one unused entry
+  std::vector<int32_t> lo(count, 0);
   build_chem_grids(chems, num_chemicals, tables, g.data(), lo.data(), nullptr);
   hipError_t e = hipMalloc((void **)grids, sizeof(ChemGrid) * num_chemicals);
   if (e == hipSuccess) e = hipMalloc((void **)grid_lo, sizeof(int32_t) * count);

@@ -16,6 +16,7 @@
 
 #include <math.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "../../include/optk.h"
 
@@ -259,62 +260,69 @@ OPTK_HD float interp1f(const float *xp, const float *fp, int count, float x) {
   return (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]) * (x - xp[lo]) + fp[lo];
 }
 
-// Index grid over one chemical's wavelength knots: `cells` uniform cells
-// over [w[0], w[count - 1]], lo[g] = the knot bracket at cell g's left
-// edge.  A lookup starts from its cell's entry and steps to the bracket
-// w[j] <= x < w[j + 1]; the bracket is unique, so the result is the binary
-// search's, in one load and a step or two instead of log2(count) dependent
-// loads.
+// Index grid over one chemical's wavelength knots.  Measured n/k tables
+// are often log-spaced over many octaves (0.1 A .. 10^6 A), so the cells are
+// uniform in the bit pattern of the float32 wavelength -- a piecewise-linear
+// log2: cell = (bits(x) - base) >> shift, 2^(23 - shift) cells per octave.
+// lo[g] is the knot bracket at cell g's left edge.  A lookup starts from its
+// cell's entry and steps to the bracket w[j] <= x < w[j + 1]; the bracket is
+// unique, so the result is the binary search's whatever the cell (float32
+// rounding of a float64 x included), in one load and a step or two instead
+// of log2(count) dependent loads.  cells == 0: no grid (binary search).
 struct ChemGrid {
-  double w0, inv_dw;
-  int32_t cells, offset;  // offset into the lo array
+  int32_t base, shift;    // float32 bits of w[0], cell width in bits (log2)
+  int32_t cells, offset;  // cell count, offset into the lo array
 };
 
-OPTK_HD int grid_bracket(const ChemGrid &g, const int32_t *lo_tab, const double *w, int cnt,
-                         double x) {
-  int c = (int)((x - g.w0) * g.inv_dw);
-  c = c < 0 ? 0 : (c >= g.cells ? g.cells - 1 : c);
-  int j = lo_tab[g.offset + c];
+OPTK_HD int grid_cell(const ChemGrid &g, float x) {
+  int32_t b;
+  memcpy(&b, &x, sizeof(b));
+  int32_t c = b < g.base ? 0 : (b - g.base) >> g.shift;
+  return c >= g.cells ? g.cells - 1 : c;
+}
+
+template <typename T>
+OPTK_HD int grid_bracket(const ChemGrid &g, const int32_t *lo_tab, const T *w, int cnt, T x) {
+  int j = lo_tab[g.offset + grid_cell(g, (float)x)];
   while (j > 0 && w[j] > x) --j;
   while (j < cnt - 2 && w[j + 1] <= x) ++j;
   return j;
 }
 
-OPTK_HD int grid_bracket(const ChemGrid &g, const int32_t *lo_tab, const float *w, int cnt,
-                         float x) {
-  int c = (int)((x - (float)g.w0) * (float)g.inv_dw);
-  c = c < 0 ? 0 : (c >= g.cells ? g.cells - 1 : c);
-  int j = lo_tab[g.offset + c];
-  while (j > 0 && w[j] > x) --j;
-  while (j < cnt - 2 && w[j + 1] <= x) ++j;
-  return j;
-}
-
-// host: the grids of every chemical (4 cells per knot, at most 4096)
+// host: the grids of every chemical, about 8 cells per knot (at most 16384);
+// a chemical whose first knot is not a positive normal float gets none
 inline void build_chem_grids(const optk_chemical *chems, int num_chemicals, const double *tables,
                              ChemGrid *grids, int32_t *lo_tab, int64_t *lo_count) {
   int64_t used = 0;
   for (int c = 0; c < num_chemicals; ++c) {
     const double *w = tables + chems[c].offset;
     const int cnt = chems[c].count;
-    int cells = 4 * cnt;
-    cells = cells < 16 ? 16 : (cells > 4096 ? 4096 : cells);
-    ChemGrid g;
-    g.w0 = w[0];
-    const double span = w[cnt - 1] - w[0];
-    g.inv_dw = span > 0.0 ? (double)cells / span : 0.0;
-    g.cells = cells;
-    g.offset = (int32_t)used;
-    if (lo_tab) {
-      int j = 0;
-      for (int k = 0; k < cells; ++k) {
-        const double edge = w[0] + span * ((double)k / (double)cells);
-        while (j < cnt - 2 && w[j + 1] <= edge) ++j;
-        lo_tab[used + k] = j;
+    ChemGrid g = {0, 0, 0, (int32_t)used};
+    const float f0 = (float)w[0], f1 = (float)w[cnt - 1];
+    if (cnt >= 2 && f0 >= 1.17549435e-38f && f1 < 3.0e38f && f1 >= f0) {
+      int32_t b0, b1;
+      memcpy(&b0, &f0, sizeof(b0));
+      memcpy(&b1, &f1, sizeof(b1));
+      int64_t want = 8 * (int64_t)cnt;
+      want = want < 16 ? 16 : (want > 16384 ? 16384 : want);
+      int shift = 0;
+      while (shift < 30 && (((int64_t)(b1 - b0) >> shift) + 1) > want) ++shift;
+      g.base = b0;
+      g.shift = shift;
+      g.cells = (int32_t)(((int64_t)(b1 - b0) >> shift) + 1);
+      if (lo_tab) {
+        int j = 0;
+        for (int k = 0; k < g.cells; ++k) {
+          const int32_t eb = b0 + (int32_t)((int64_t)k << shift);
+          float edge;
+          memcpy(&edge, &eb, sizeof(edge));
+          while (j < cnt - 2 && w[j + 1] <= (double)edge) ++j;
+          lo_tab[used + k] = j;
+        }
       }
     }
     if (grids) grids[c] = g;
-    used += cells;
+    used += g.cells;
   }
   if (lo_count) *lo_count = used;
 }
@@ -335,7 +343,7 @@ OPTK_HD C<double> chemical_n(const optk_chemical &ch, const double *tables, doub
     return {n[0], k[0]};
   }
   if (x >= w[cnt - 1]) return {n[cnt - 1], k[cnt - 1]};
-  int j = grid ? grid_bracket(*grid, lo_tab, w, cnt, x) : interp_bracket(w, cnt, x);
+  int j = grid && grid->cells ? grid_bracket(*grid, lo_tab, w, cnt, x) : interp_bracket(w, cnt, x);
   if (x == w[j]) return {n[j], k[j]};
   double inv_dx = 1.0 / (w[j + 1] - w[j]);
   double sre = (n[j + 1] - n[j]) * inv_dx;

@@ -190,3 +190,17 @@ def multilayer(layers, substrate, wavelength, cos, n):
     lib().host_multilayer(ctypes.byref(d), ctypes.c_int(0), _dp(wl), _dp(cre), _dp(cim), _dp(nre),
                           _dp(nim), ctypes.c_int64(m), *[_dp(o) for o in outs])
     return outs
+
+
+def grid_bracket(w, x):
+    """Brackets of ``x`` in the ascending table ``w`` through the index grid
+    of optk_physics.h: (j, steps walked, cell count)."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    j = np.empty(x.size, np.int32)
+    steps = np.empty(x.size, np.int32)
+    f = lib().host_grid_bracket
+    f.restype = ctypes.c_int
+    cells = f(_dp(w), ctypes.c_int(w.size), _dp(x), ctypes.c_int(x.size),
+              j.ctypes.data_as(ctypes.c_void_p), steps.ctypes.data_as(ctypes.c_void_p))
+    return j, steps, cells

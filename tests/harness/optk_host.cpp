@@ -42,6 +42,31 @@ std::vector<optk_surface> derived(const optk_system_desc *d) {
 
 extern "C" {
 
+// Knot brackets of n wavelengths (Angstrom) in one ascending table through
+// its index grid: j[i] (w[j] <= x < w[j + 1], clamped to 0 .. count - 2) and
+// steps[i], the knots walked from the grid cell's entry.  Returns the
+// grid's cell count (0: no grid).
+int host_grid_bracket(const double *w, int count, const double *x, int n, int32_t *j,
+                      int32_t *steps) {
+  optk_chemical ch = {0, count};
+  ChemGrid g;
+  int64_t cells = 0;
+  build_chem_grids(&ch, 1, w, nullptr, nullptr, &cells);
+  std::vector<int32_t> lo(cells > 0 ? cells : 1, 0);
+  build_chem_grids(&ch, 1, w, &g, lo.data(), nullptr);
+  for (int i = 0; i < n; ++i) {
+    if (!g.cells) {
+      j[i] = -1;
+      steps[i] = 0;
+      continue;
+    }
+    const int start = lo[grid_cell(g, (float)x[i])];
+    j[i] = grid_bracket(g, lo.data(), w, count, x[i]);
+    steps[i] = j[i] > start ? j[i] - start : start - j[i];
+  }
+  return g.cells;
+}
+
 // Trace n grid rays; out[0..9] = px py pz dx dy dz wavelength intensity
 // attenuation index (num_paths * n each, path-major), mask = unvignetted.
 int host_trace_grid(const optk_system_desc *d, const double *fx, const double *fy,

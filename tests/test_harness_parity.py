@@ -61,3 +61,49 @@ def test_multilayer_kernel_code(cos):
     (oRs, oRp), (oTs, oTp) = oracle.multilayer_efficiency(wl, cos, 1.0 + 0j, layers, sub)
     for a, b in ((Rs, oRs), (Rp, oRp), (Ts, oTs), (Tp, oTp)):
         np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-15)
+
+
+def _tables_of(system):
+    from optika_amd import _compile
+
+    rec = _compile.build_records(system.surfaces_all)
+    t = np.asarray(rec.tables)
+    return [t[rec.chemicals[i].offset:rec.chemicals[i].offset + rec.chemicals[i].count]
+            for i in range(rec.num_chemicals)]
+
+
+@pytest.mark.parametrize("kind", ["measured", "linear", "short", "nonpositive"])
+def test_chemical_grid_brackets(kind):
+    """The n/k index grid gives numpy.searchsorted's bracket for every
+    wavelength (knots, midpoints, float32-rounding neighbours, random) and
+    starts less than one knot from it on average in the EUV band of the
+    measured log-spaced tables, so a lookup is one load and a short walk."""
+    from harness import grid_bracket
+
+    rng = np.random.default_rng(4)
+    if kind == "measured":
+        tables = _tables_of(workloads.spectrograph())
+        assert min(t.size for t in tables) > 1000 and tables[0][-1] / tables[0][0] > 1e6
+    elif kind == "linear":
+        tables = [np.linspace(500.0, 700.0, 301)]
+    elif kind == "short":
+        tables = [np.array([100.0, 400.0]), np.array([1.0, 2.0, 2.5])]
+    else:
+        tables = [np.linspace(-5.0, 5.0, 11)]
+    for w in tables:
+        mids = 0.5 * (w[1:] + w[:-1])
+        x = np.concatenate([w[:-1], mids, np.nextafter(w[1:-1], -np.inf), np.nextafter(w[:-1], np.inf),
+                            w[:-1].astype(np.float32).astype(np.float64),
+                            np.exp(rng.uniform(np.log(max(w[0], 1e-3)), np.log(w[-1]), 20000))
+                            if w[0] > 0 else rng.uniform(w[0], w[-1], 20000)])
+        x = x[(x >= w[0]) & (x < w[-1])]
+        j, steps, cells = grid_bracket(w, x)
+        if kind == "nonpositive":
+            assert cells == 0  # no grid: the binary search brackets instead
+            continue
+        want = np.clip(np.searchsorted(w, x, side="right") - 1, 0, w.size - 2)
+        np.testing.assert_array_equal(j, want)
+        assert cells <= 16384
+        if kind == "measured":
+            eband = (x > 100.0) & (x < 2000.0)  # EUV .. soft x-ray, where traces look up
+            assert steps.max() <= 16 and steps[eband].mean() < 1.0, (steps.max(), steps[eband].mean())
