@@ -508,6 +508,24 @@ typedef struct optk_surrogate_io {
 int32_t optk_surrogate_infer(const optk_surrogate_desc *desc, const optk_surrogate_io *io,
                              void *hip_stream);
 
+/* The surrogate's polynomial baseline in float64 (the fit of the targets on
+ * the OPTK_SURROGATE_POLY cubic monomials F of 5 normalised inputs).
+ * u: [n][5] row-major; targets: [n][num_targets] (num_targets <=
+ * OPTK_SURROGATE_MAX_TARGETS); valid: n flags, rays flagged 0 add nothing;
+ * poly_exp: host array of OPTK_SURROGATE_POLY packed exponents (2 bits per
+ * input).  optk_poly_normal writes the normal equations
+ *   out[i][j] = sum_r F[r][i] G[r][j],  G = [F | targets],
+ * an [OPTK_SURROGATE_POLY][OPTK_SURROGATE_POLY + num_targets] matrix summed
+ * in a fixed order (deterministic), using optk_poly_workspace(n) bytes of
+ * caller-owned device workspace.  optk_poly_eval writes
+ * out[n][num_targets] = F coef, coef [OPTK_SURROGATE_POLY][num_targets]. */
+size_t optk_poly_workspace(int64_t n);
+int32_t optk_poly_normal(const double *u, const double *targets, const uint8_t *valid, int64_t n,
+                         int32_t num_targets, const uint32_t *poly_exp, double *out,
+                         void *workspace, size_t workspace_bytes, void *hip_stream);
+int32_t optk_poly_eval(const double *u, const double *coef, int64_t n, int32_t num_targets,
+                       const uint32_t *poly_exp, double *out, void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

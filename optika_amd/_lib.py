@@ -303,6 +303,13 @@ def lib() -> ctypes.CDLL:
     L.optk_surrogate_infer.restype = c_int32
     L.optk_surrogate_infer.argtypes = [ctypes.POINTER(SurrogateDesc), ctypes.POINTER(SurrogateIO),
                                        c_void_p]
+    L.optk_poly_workspace.restype = ctypes.c_size_t
+    L.optk_poly_workspace.argtypes = [c_int64]
+    L.optk_poly_normal.restype = c_int32
+    L.optk_poly_normal.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p,
+                                   c_void_p, ctypes.c_size_t, c_void_p]
+    L.optk_poly_eval.restype = c_int32
+    L.optk_poly_eval.argtypes = [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p, c_void_p]
     if L.optk_abi_version() != 1:
         raise OptkError(f"liboptk ABI {L.optk_abi_version()} != 1")
     for name, cls in [("optk_affine", Affine), ("optk_surface", Surface), ("optk_layer", Layer),

@@ -301,6 +301,128 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(const float *__restrict__ 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 
+
+// ---------------------------------------------------------------------------
+// polynomial baseline in float64: normal equations and evaluation
+// ---------------------------------------------------------------------------
+
+constexpr int POLY_N = OPTK_SURROGATE_POLY;
+constexpr int POLY_TILE = 64;                                   // rays per LDS tile
+constexpr int POLY_COLS = POLY_N + OPTK_SURROGATE_MAX_TARGETS;  // F | targets
+constexpr int POLY_PER_THREAD = (POLY_N * POLY_COLS + 255) / 256;
+constexpr int POLY_BLOCKS = 512;                                // partial sums
+
+struct PolyExp {
+  uint32_t e[POLY_N];
+};
+
+// powers 1, u, u^2, u^3 of every input, and one monomial: the product of the
+// powers in input order (the same rounding as a column-by-column product)
+__device__ inline void poly_powers(const double *u, double pw[5][4]) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    pw[k][0] = 1.0;
+    pw[k][1] = u[k];
+    pw[k][2] = u[k] * u[k];
+    pw[k][3] = pw[k][2] * u[k];
+  }
+}
+
+__device__ inline double poly_monomial(const double pw[5][4], uint32_t e) {
+  double c = 1.0;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const uint32_t ek = (e >> (2 * k)) & 3u;
+    if (ek) c = c * pw[k][ek];
+  }
+  return c;
+}
+
+// one partial sum of the normal equations per block over a stride of
+// 64-ray tiles; thread t owns entries t, t + 256, ...
+__global__ __launch_bounds__(256) void poly_normal_partial(const double *__restrict__ u,
+                                                           const double *__restrict__ tg,
+                                                           const uint8_t *__restrict__ valid,
+                                                           int64_t n, int nt, PolyExp X,
+                                                           double *__restrict__ partial) {
+  __shared__ double G[POLY_TILE][POLY_COLS + 1];
+  const int t = threadIdx.x;
+  const int cols = POLY_N + nt;
+  const int entries = POLY_N * cols;
+  double acc[POLY_PER_THREAD];
+#pragma unroll
+  for (int q = 0; q < POLY_PER_THREAD; ++q) acc[q] = 0.0;
+  const int64_t tiles = (n + POLY_TILE - 1) / POLY_TILE;
+  for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    __syncthreads();
+    {
+      // thread t fills ray t % 64: monomials [14 (t / 64), +14), and the
+      // targets in the last quarter
+      const int r = t & (POLY_TILE - 1), part = t >> 6;
+      const int64_t i = tile * POLY_TILE + r;
+      const bool ok = i < n && valid[i];
+      double ui[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      if (ok)
+        for (int k = 0; k < 5; ++k) ui[k] = u[i * 5 + k];
+      double pw[5][4];
+      poly_powers(ui, pw);
+      for (int m = part * (POLY_N / 4); m < (part + 1) * (POLY_N / 4); ++m)
+        G[r][m] = ok ? poly_monomial(pw, X.e[m]) : 0.0;
+      if (part == 3)
+        for (int j = 0; j < nt; ++j) G[r][POLY_N + j] = ok ? tg[i * nt + j] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < POLY_PER_THREAD; ++q) {
+      const int p = t + 256 * q;
+      if (p < entries) {
+        const int a = p / cols, b = p - a * cols;
+        double s = acc[q];
+        for (int r = 0; r < POLY_TILE; ++r) s = fma(G[r][a], G[r][b], s);
+        acc[q] = s;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < POLY_PER_THREAD; ++q) {
+    const int p = t + 256 * q;
+    if (p < entries) partial[(int64_t)blockIdx.x * entries + p] = acc[q];
+  }
+}
+
+__global__ __launch_bounds__(256) void poly_normal_reduce(const double *__restrict__ partial,
+                                                          int blocks, int entries,
+                                                          double *__restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= entries) return;
+  double s = 0.0;
+  for (int b = 0; b < blocks; ++b) s += partial[(int64_t)b * entries + p];
+  out[p] = s;
+}
+
+__global__ __launch_bounds__(256) void poly_eval_kernel(const double *__restrict__ u,
+                                                        const double *__restrict__ coef, int64_t n,
+                                                        int nt, PolyExp X, double *__restrict__ out) {
+  __shared__ double C[POLY_N * OPTK_SURROGATE_MAX_TARGETS];
+  for (int k = threadIdx.x; k < POLY_N * nt; k += blockDim.x) C[k] = coef[k];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double ui[5], pw[5][4];
+  for (int k = 0; k < 5; ++k) ui[k] = u[i * 5 + k];
+  poly_powers(ui, pw);
+  double s[OPTK_SURROGATE_MAX_TARGETS];
+#pragma unroll
+  for (int j = 0; j < OPTK_SURROGATE_MAX_TARGETS; ++j) s[j] = 0.0;
+  for (int m = 0; m < POLY_N; ++m) {
+    const double f = poly_monomial(pw, X.e[m]);
+#pragma unroll
+    for (int j = 0; j < OPTK_SURROGATE_MAX_TARGETS; ++j)
+      if (j < nt) s[j] = fma(f, C[m * nt + j], s[j]);
+  }
+  for (int j = 0; j < nt; ++j) out[i * nt + j] = s[j];
+}
+
 // ---------------------------------------------------------------------------
 // fused inference: input normalisation, the five layers, the polynomial,
 // aperture-edge distances, the error-map lookup and the compaction of the
@@ -710,5 +832,62 @@ extern "C" int32_t optk_surrogate_infer(const optk_surrogate_desc *desc, const o
   hipLaunchKernelGGL(surrogate_infer, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)hip_stream, D, IO);
   e = hipGetLastError();
   if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("surrogate launch: ") + hipGetErrorString(e));
+  return OPTK_OK;
+}
+
+extern "C" size_t optk_poly_workspace(int64_t n) {
+  (void)n;
+  return sizeof(double) * (size_t)POLY_BLOCKS * POLY_N * POLY_COLS;
+}
+
+static int32_t poly_check(const double *u, int64_t n, int32_t nt, const uint32_t *poly_exp,
+                          const double *out, PolyExp &X) {
+  if (!u || !out || !poly_exp) return fail(OPTK_EINVAL, "null argument");
+  if (n < 0) return fail(OPTK_EINVAL, "negative count");
+  if (nt < 1 || nt > OPTK_SURROGATE_MAX_TARGETS) return fail(OPTK_EINVAL, "num_targets out of range");
+  for (int m = 0; m < POLY_N; ++m) {
+    uint32_t e = poly_exp[m], deg = 0;
+    if (e >> 10) return fail(OPTK_EINVAL, "poly_exp: 2 bits for each of 5 inputs");
+    for (int k = 0; k < 5; ++k) deg += (e >> (2 * k)) & 3u;
+    if (deg > 3) return fail(OPTK_EINVAL, "poly_exp: total degree above 3");
+    X.e[m] = e;
+  }
+  return OPTK_OK;
+}
+
+extern "C" int32_t optk_poly_normal(const double *u, const double *targets, const uint8_t *valid,
+                                    int64_t n, int32_t num_targets, const uint32_t *poly_exp,
+                                    double *out, void *workspace, size_t workspace_bytes,
+                                    void *hip_stream) {
+  PolyExp X;
+  int32_t rc = poly_check(u, n, num_targets, poly_exp, out, X);
+  if (rc) return rc;
+  if (!targets || !valid || !workspace) return fail(OPTK_EINVAL, "null argument");
+  if (workspace_bytes < optk_poly_workspace(n)) return fail(OPTK_EINVAL, "workspace too small");
+  const int entries = POLY_N * (POLY_N + num_targets);
+  const int64_t tiles = (n + POLY_TILE - 1) / POLY_TILE;
+  const int blocks = (int)(tiles < POLY_BLOCKS ? (tiles > 0 ? tiles : 1) : POLY_BLOCKS);
+  hipStream_t s = (hipStream_t)hip_stream;
+  double *partial = (double *)workspace;
+  hipLaunchKernelGGL(poly_normal_partial, dim3(blocks), dim3(256), 0, s, u, targets, valid, n,
+                     (int)num_targets, X, partial);
+  hipLaunchKernelGGL(poly_normal_reduce, dim3((entries + 255) / 256), dim3(256), 0, s, partial,
+                     blocks, entries, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("poly normal: ") + hipGetErrorString(e));
+  return OPTK_OK;
+}
+
+extern "C" int32_t optk_poly_eval(const double *u, const double *coef, int64_t n, int32_t num_targets,
+                                  const uint32_t *poly_exp, double *out, void *hip_stream) {
+  PolyExp X;
+  int32_t rc = poly_check(u, n, num_targets, poly_exp, out, X);
+  if (rc) return rc;
+  if (!coef) return fail(OPTK_EINVAL, "null argument");
+  if (n == 0) return OPTK_OK;
+  hipLaunchKernelGGL(poly_eval_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)hip_stream, u, coef, n, (int)num_targets, X, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("poly eval: ") + hipGetErrorString(e));
   return OPTK_OK;
 }

@@ -29,7 +29,7 @@ import torch
 
 from . import _lib, _trace
 
-__all__ = ["Surrogate", "poly_features", "NUM_POLY"]
+__all__ = ["Surrogate", "NUM_POLY", "EXPONENTS"]
 
 WIDTHS = (5, 256, 256, 256, 256, 3)
 NUM_POLY = 56  # monomials of total degree <= 3 in 5 variables
@@ -52,23 +52,9 @@ def _exponents(degree=3, nvar=5):
 
 EXPONENTS = _exponents()
 assert len(EXPONENTS) == NUM_POLY
-
-
-def poly_features(u: torch.Tensor) -> torch.Tensor:
-    """Monomials of total degree <= 3 of the normalised inputs u [n][5]
-    (float64), in ``EXPONENTS`` order."""
-    cols = []
-    pw = [[torch.ones_like(u[:, v])] for v in range(5)]
-    for v in range(5):
-        for _ in range(3):
-            pw[v].append(pw[v][-1] * u[:, v])
-    for e in EXPONENTS:
-        c = torch.ones_like(u[:, 0])
-        for v, k in enumerate(e):
-            if k:
-                c = c * pw[v][k]
-        cols.append(c)
-    return torch.stack(cols, 1)
+# the monomials' exponents packed 2 bits per input, as the kernels take them
+_EXP_CODES = np.array([sum(int(k) << (2 * v) for v, k in enumerate(e)) for e in EXPONENTS],
+                      dtype=np.uint32)
 
 
 def _param_layout(widths=WIDTHS):
@@ -375,16 +361,34 @@ class Surrogate:
 
     # -- fitting -----------------------------------------------------------
     def fit_polynomial(self, u: torch.Tensor, targets: torch.Tensor, valid: torch.Tensor):
-        """Least squares of every target on the cubic monomials (normal
-        equations in float64)."""
-        F = poly_features(u[valid])
-        T = targets[valid]
-        A = F.T @ F
-        B = F.T @ T
-        self.coef = torch.linalg.solve(A, B)
+        """Least squares of every target on the cubic monomials: the normal
+        equations accumulated in float64 on the device (optk_poly_normal),
+        solved on the host."""
+        n, nt = u.shape[0], targets.shape[1]
+        u = u.double().contiguous()
+        tg = targets.double().contiguous()
+        ok = valid.to(torch.uint8).contiguous()
+        out = torch.empty(NUM_POLY, NUM_POLY + nt, dtype=torch.float64, device=self.dev)
+        nbytes = int(_lib.lib().optk_poly_workspace(n))
+        ws = torch.empty(nbytes // 8 + 1, dtype=torch.float64, device=self.dev)
+        _lib.check(_lib.lib().optk_poly_normal(
+            _ptr(u), _ptr(tg), _ptr(ok), n, nt, _EXP_CODES.ctypes.data_as(ctypes.c_void_p),
+            _ptr(out), _ptr(ws), nbytes, _stream()))
+        m = out.cpu().numpy()
+        coef = np.linalg.solve(m[:, :NUM_POLY], m[:, NUM_POLY:])
+        self.coef = torch.as_tensor(coef, dtype=torch.float64, device=self.dev)
 
     def poly(self, u: torch.Tensor) -> torch.Tensor:
-        return poly_features(u) @ self.coef
+        """The fitted polynomial at normalised inputs u [n][5]: [n][targets]
+        (optk_poly_eval, float64)."""
+        u = u.double().contiguous()
+        n, nt = u.shape[0], self.coef.shape[1]
+        out = torch.empty(n, nt, dtype=torch.float64, device=self.dev)
+        coef = self.coef.contiguous()
+        _lib.check(_lib.lib().optk_poly_eval(_ptr(u), _ptr(coef), n, nt,
+                                             _EXP_CODES.ctypes.data_as(ctypes.c_void_p),
+                                             _ptr(out), _stream()))
+        return out
 
     # -- training ----------------------------------------------------------
     def prepare(self, grid_fn, n_train: int, chunk: int = 1 << 22, poly_rays: int = 1 << 22):
@@ -570,7 +574,7 @@ class Surrogate:
             d.scale[k] = float(self.scale[k])
         d.edge_margin = self.edge_margin
         for t, e in enumerate(EXPONENTS):
-            d.poly_exp[t] = sum(int(k) << (2 * v) for v, k in enumerate(e))
+            d.poly_exp[t] = int(_EXP_CODES[t])
         d.num_targets = T
         d.num_apertures = len(self.relevant)
         for a, i in enumerate(self.relevant):

@@ -112,3 +112,17 @@ def init_params(rng: np.random.Generator, widths=WIDTHS) -> np.ndarray:
         parts.append(rng.uniform(-bound, bound, (b, a)).ravel())
         parts.append(np.zeros(b))
     return np.concatenate(parts)
+
+
+def poly_features(u, exponents):
+    """Monomials prod_k u[:, k] ** e[k] of float64 inputs u [n][5], one
+    column per exponent tuple (the surrogate's polynomial baseline)."""
+    u = np.asarray(u, dtype=np.float64)
+    cols = []
+    for e in exponents:
+        c = np.ones(u.shape[0])
+        for k, p in enumerate(e):
+            if p:
+                c = c * u[:, k] ** p
+        cols.append(c)
+    return np.stack(cols, 1)

@@ -12,7 +12,7 @@ ROOT = pathlib.Path(__file__).resolve().parent.parent
 
 def declared_functions():
     text = (ROOT / "include" / "optk.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int32_t|int64_t|const char \*)\s*(optk_\w+)\(", text,
+    return sorted(set(re.findall(r"^\s*(?:int32_t|int64_t|size_t|const char \*)\s*(optk_\w+)\(", text,
                                  re.MULTILINE)))
 
 
@@ -73,3 +73,27 @@ def test_batch_arguments_are_rejected_without_a_device():
     assert b"num_instances" in L.optk_last_error()
     assert L.optk_batch_trace(None, 0, None, None, None, None, None, 0, None) == -1
     assert L.optk_batch_destroy(None) == 0
+
+
+def test_polynomial_arguments_are_rejected_without_a_device():
+    import numpy as np
+
+    from optika_amd import _lib, surrogate
+
+    if not _lib.library_path().exists():
+        pytest.skip("liboptk.so not built")
+    L = _lib.lib()
+    codes = surrogate._EXP_CODES.copy()
+    buf = ctypes.c_double()
+    p = ctypes.addressof(buf)
+    assert L.optk_poly_eval(p, p, 1, 0, codes.ctypes.data, p, None) == -1
+    assert b"num_targets" in L.optk_last_error()
+    bad = codes.copy()
+    bad[3] = 3 | (3 << 2)  # degree 6
+    assert L.optk_poly_eval(p, p, 1, 3, bad.ctypes.data, p, None) == -1
+    assert b"degree" in L.optk_last_error()
+    assert L.optk_poly_normal(p, None, p, 1, 3, codes.ctypes.data, p, p, 1 << 30, None) == -1
+    assert L.optk_poly_normal(p, p, p, 1, 3, codes.ctypes.data, p, p, 8, None) == -1
+    assert b"workspace" in L.optk_last_error()
+    assert L.optk_poly_workspace(10) >= 8 * 56 * 67
+    assert np.uint32 == codes.dtype.type

@@ -159,3 +159,49 @@ def test_fused_inference_matches_layered_path(trained):
     assert count == int(out["fallback"].sum().item())
     lst = out["list"][:count].long()
     assert torch.all(out["fallback"][lst]) and torch.unique(lst).numel() == count
+
+
+@pytest.mark.parametrize("n", [1, 6400, 100_037])
+def test_polynomial_normal_equations_and_eval(n):
+    """optk_poly_normal / optk_poly_eval against numpy float64 on the same
+    monomials: rays flagged invalid (and their non-finite targets) add
+    nothing; the sums are deterministic."""
+    rng = np.random.default_rng(n)
+    u = rng.uniform(-1, 1, (n, 5))
+    nt = 7
+    tg = rng.normal(0, 1, (n, nt))
+    valid = rng.uniform(0, 1, n) > 0.2
+    tg[~valid, 0] = np.nan
+    dev = torch.device("cuda")
+    sg = surrogate.Surrogate.__new__(surrogate.Surrogate)
+    sg.dev = dev
+    print(f"case: poly normal/eval n={n}", flush=True)
+    ut = torch.as_tensor(u, device=dev)
+    sg.fit_polynomial(ut, torch.as_tensor(tg, device=dev), torch.as_tensor(valid, device=dev))
+    F = ref_mlp.poly_features(u, surrogate.EXPONENTS)
+    G = np.concatenate([F, np.nan_to_num(tg)], 1)
+    A = F[valid].T @ G[valid]
+    if valid.sum() >= 56:
+        want = np.linalg.solve(A[:, :56], A[:, 56:])
+        np.testing.assert_allclose(sg.coef.cpu().numpy(), want, rtol=1e-8, atol=1e-10)
+    # the accumulated matrix itself, twice (deterministic)
+    from optika_amd import _lib
+
+    outs = []
+    for _ in range(2):
+        out = torch.empty(56, 56 + nt, dtype=torch.float64, device=dev)
+        nb = int(_lib.lib().optk_poly_workspace(n))
+        ws = torch.empty(nb // 8 + 1, dtype=torch.float64, device=dev)
+        tgt = torch.as_tensor(tg, device=dev)
+        ok = torch.as_tensor(valid.astype(np.uint8), device=dev)
+        _lib.check(_lib.lib().optk_poly_normal(
+            ut.data_ptr(), tgt.data_ptr(), ok.data_ptr(), n, nt,
+            surrogate._EXP_CODES.ctypes.data, out.data_ptr(), ws.data_ptr(), nb, None))
+        outs.append(out.cpu().numpy())
+    scale = np.abs(F[valid]).T @ np.abs(G[valid]) + 1e-300
+    assert np.all(np.abs(outs[0] - A) <= 1e-13 * scale)
+    assert np.array_equal(outs[0], outs[1])
+    coef = rng.normal(0, 1, (56, nt))
+    sg.coef = torch.as_tensor(coef, device=dev)
+    got = sg.poly(ut).cpu().numpy()
+    np.testing.assert_allclose(got, F @ coef, rtol=0, atol=1e-13 * (np.abs(F) @ np.abs(coef)).max())

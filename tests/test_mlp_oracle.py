@@ -39,3 +39,26 @@ def test_bf16_rounding_is_round_to_nearest_even():
 
 def test_parameter_count():
     assert sum(int(np.prod(s)) for s in mlp.param_shapes()) == 199683
+
+
+def test_polynomial_monomials_and_codes():
+    """The 56 monomials of total degree <= 3 in 5 inputs: distinct, packed
+    codes that decode to the exponents, and a fit on them that recovers a
+    cubic exactly (normal equations, as the device accumulates them)."""
+    from optika_amd import surrogate
+
+    ex = surrogate.EXPONENTS
+    assert len(ex) == 56 == len(set(ex)) and all(sum(e) <= 3 for e in ex)
+    for e, c in zip(ex, surrogate._EXP_CODES):
+        assert tuple((int(c) >> (2 * k)) & 3 for k in range(5)) == tuple(e)
+    rng = np.random.default_rng(7)
+    u = rng.uniform(-1, 1, (4000, 5))
+    F = mlp.poly_features(u, ex)
+    # independent statement of one cubic: 0.3 + u0 - 2 u1 u4 + 0.5 u2^3 - u0 u3^2
+    y = 0.3 + u[:, 0] - 2 * u[:, 1] * u[:, 4] + 0.5 * u[:, 2] ** 3 - u[:, 0] * u[:, 3] ** 2
+    coef = np.linalg.solve(F.T @ F, F.T @ y)
+    want = np.zeros(56)
+    for e, c in [((0, 0, 0, 0, 0), 0.3), ((1, 0, 0, 0, 0), 1.0), ((0, 1, 0, 0, 1), -2.0),
+                 ((0, 0, 3, 0, 0), 0.5), ((1, 0, 0, 2, 0), -1.0)]:
+        want[ex.index(e)] = c
+    np.testing.assert_allclose(coef, want, atol=1e-10)
