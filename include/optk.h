@@ -480,6 +480,8 @@ typedef struct optk_surrogate_aperture {
 } optk_surrogate_aperture;
 typedef struct optk_surrogate_desc {
   const float *w1, *b1;         /* [256][5], [256] float32 */
+  const uint16_t *w1_16;        /* bf16 [256][32]: row o = [hi(w1[o]) | hi(w1[o]) |
+                                   lo(w1[o]) | 0], lo = bf16(w - hi) */
   const uint16_t *w16[3];       /* hidden layers, bf16 [256 out][256 in] */
   const float *b[3];            /* hidden biases [256] */
   const float *w5, *b5;         /* [3][256], [3] float32 */

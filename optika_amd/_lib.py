@@ -215,7 +215,8 @@ class SurrogateAperture(ctypes.Structure):
 
 
 class SurrogateDesc(ctypes.Structure):
-    _fields_ = [("w1", c_void_p), ("b1", c_void_p), ("w16", c_void_p * 3), ("b", c_void_p * 3),
+    _fields_ = [("w1", c_void_p), ("b1", c_void_p), ("w1_16", c_void_p), ("w16", c_void_p * 3),
+                ("b", c_void_p * 3),
                 ("w5", c_void_p), ("b5", c_void_p), ("w5_16", c_void_p), ("poly", c_void_p),
                 ("error_map", c_void_p),
                 ("in_mid", ctypes.c_float * 5), ("in_inv_half", ctypes.c_float * 5),

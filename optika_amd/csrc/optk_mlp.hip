@@ -31,11 +31,16 @@ __device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
 
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ __forceinline__ float silu(float x) { return x / (1.0f + __expf(-x)); }
+// sigmoid through the hardware exp2 and reciprocal (1 ulp each): a plain
+// division would expand to a range-scaled sequence several times longer.
+// exp(-x) = inf (x < -88) gives 0, the limit.
+__device__ __forceinline__ float sigmoid(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
+
+__device__ __forceinline__ float silu(float x) { return x * sigmoid(x); }
 
 // d silu / dx = s (1 + x (1 - s)), s = sigmoid(x)
 __device__ __forceinline__ float silu_grad(float x) {
-  float s = 1.0f / (1.0f + __expf(-x));
+  float s = sigmoid(x);
   return s * (1.0f + x * (1.0f - s));
 }
 
@@ -428,25 +433,36 @@ __global__ __launch_bounds__(256) void poly_eval_kernel(const double *__restrict
 // aperture-edge distances, the error-map lookup and the compaction of the
 // rays that fall back to the trace -- one kernel, activations in LDS.
 //
-// Block: 256 threads (4 waves), a tile of 128 rays; two blocks share a CU,
-// so one block's MFMA phase overlaps the other's VALU phase (SiLU, layer 1,
-// the output layer and the polynomial).  Layer 1 (5 -> 256) on VALU in
-// float32 writes bf16 activations act[ray][feature] to LDS.  Each hidden
-// layer is a 128 x 256 x 256 product: wave w computes all 128 rays x
-// features [64 w, 64 w + 64) as 8 x 4 MFMA 16x16x32 tiles, A operands
-// (activations) from LDS, B operands (weights, row-major [out][in], k
-// contiguous) straight from global memory (L2-resident).  The B operand of
-// n-tile t and lane column c is weight row 64 w + 4 c + t, so each lane's
-// four accumulator tiles hold four consecutive features: the epilogue
-// (bias, SiLU, bf16) writes them as one 8-byte store per row.  The output
-// layer, polynomial and edge tests run one thread per ray.
+// Block: 256 threads (4 waves), a tile of 128 rays (64: four blocks); two
+// blocks share a CU, so one block's MFMA phase overlaps the other's VALU phase (SiLU, the
+// polynomial).  Every layer runs on MFMA 16x16x32 bf16 with float32
+// accumulation.  Layer 1 (5 -> 256): the normalised inputs and the weights
+// are split into bf16 high and low parts, k row [hi | lo | hi] against
+// weight row [w_hi | w_hi | w_lo], so the one 32-wide k step carries the
+// float32 product to ~2^-16.  Hidden layers: a 128 x 256 x 256 product
+// each.  Wave w computes all 128 rays x features [64 w, 64 w + 64) as
+// 8 x 4 tiles, A operands (activations act[ray][feature], bf16) from LDS,
+// B operands (weights, row-major [out][in], k contiguous) straight from
+// global memory (L2-resident), accumulators started at the bias.  The B
+// operand of n-tile t and lane column c is weight row 64 w + 4 c + t, so
+// each lane's four accumulator tiles hold four consecutive features: the
+// epilogue (SiLU, bf16) writes them as one 8-byte store per row.  The
+// output layer (3 rows, high and low bf16 parts) is one more MFMA tile per
+// 16 rays; the polynomial and edge tests run one thread per ray.
 // ---------------------------------------------------------------------------
 
-constexpr int INF_RAYS = 128;
+#ifndef OPTK_INF_RAYS
+#define OPTK_INF_RAYS 128
+#endif
+constexpr int INF_RAYS = OPTK_INF_RAYS;  // rays per tile: 64 or 128
+constexpr int INF_MT = INF_RAYS / 16;    // 16-ray MFMA row tiles per tile
+constexpr int INF_OQ = INF_MT / 4;       // output-layer row tiles per wave
+constexpr int INF_BLOCKS_PER_CU = INF_RAYS == 64 ? 4 : 2;
+static_assert(INF_RAYS == 64 || INF_RAYS == 128, "tile of 64 or 128 rays");
 constexpr int INF_HIDDEN = 256;
 constexpr int INF_PAD = 8;
 
-__global__ __launch_bounds__(256, 2) void surrogate_infer(const optk_surrogate_desc D,
+__global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const optk_surrogate_desc D,
                                                           const optk_surrogate_io IO) {
   __shared__ __attribute__((aligned(16))) uint16_t act[INF_RAYS][INF_HIDDEN + INF_PAD];
   const int tid = threadIdx.x;
@@ -456,108 +472,117 @@ __global__ __launch_bounds__(256, 2) void surrogate_infer(const optk_surrogate_d
   const float *in[5] = {IO.field_x, IO.field_y, IO.pupil_x, IO.pupil_y, IO.wavelength};
   for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const int64_t base = tile * INF_RAYS;
-    // ---- layer 1: thread t -> ray t % 128, features [128 (t / 128), +128) ----
-    {
-      const int r = tid & (INF_RAYS - 1);
-      const int64_t i = base + r;
-      float u[5];
+    // ---- inputs: thread t < 128 -> ray t, normalised and split into bf16
+    // high and low parts, as the 32-wide k row [hi | lo | hi | 0] that meets
+    // the first layer's weight rows [w_hi | w_hi | w_lo | 0] ---------------
+    if (tid < INF_RAYS) {
+      const int64_t i = base + tid;
+      float hi[5], lo[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
-        float v = i < n ? in[k][i] : 0.0f;
-        u[k] = (v - D.in_mid[k]) * D.in_inv_half[k];
+        const float v = i < n ? in[k][i] : 0.0f;
+        const float u = (v - D.in_mid[k]) * D.in_inv_half[k];
+        hi[k] = from_bf16(to_bf16(u));
+        lo[k] = u - hi[k];
       }
-      const int f0 = (tid >> 7) * (INF_HIDDEN / 2);
-      for (int f = f0; f < f0 + INF_HIDDEN / 2; f += 8) {
-        uint32_t w[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          float v0 = D.b1[f + 2 * q], v1 = D.b1[f + 2 * q + 1];
-#pragma unroll
-          for (int k = 0; k < 5; ++k) {
-            v0 += D.w1[(f + 2 * q) * 5 + k] * u[k];
-            v1 += D.w1[(f + 2 * q + 1) * 5 + k] * u[k];
-          }
-          w[q] = pack_bf16(silu(v0), silu(v1));
-        }
-        *reinterpret_cast<uint4 *>(&act[r][f]) = uint4{w[0], w[1], w[2], w[3]};
-      }
+      uint4 row0, row1;
+      row0.x = pack_bf16(hi[0], hi[1]);
+      row0.y = pack_bf16(hi[2], hi[3]);
+      row0.z = pack_bf16(hi[4], lo[0]);
+      row0.w = pack_bf16(lo[1], lo[2]);
+      row1.x = pack_bf16(lo[3], lo[4]);
+      row1.y = pack_bf16(hi[0], hi[1]);
+      row1.z = pack_bf16(hi[2], hi[3]);
+      row1.w = pack_bf16(hi[4], 0.0f);
+      *reinterpret_cast<uint4 *>(&act[tid][0]) = row0;
+      *reinterpret_cast<uint4 *>(&act[tid][8]) = row1;
+      *reinterpret_cast<uint4 *>(&act[tid][16]) = uint4{0u, 0u, 0u, 0u};
+      *reinterpret_cast<uint4 *>(&act[tid][24]) = uint4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
-    // ---- hidden layers ------------------------------------------------
+    // ---- four layers on MFMA: wave w -> features [64 w, 64 w + 64) of all
+    // 128 rays; the first layer's k extent is one 32-wide step ------------
     const int obase = wave * 64;
     const int fr = lane & 15, fk = (lane >> 4) * 8;
-    for (int layer = 0; layer < 3; ++layer) {
-      const uint16_t *W = D.w16[layer];
-      floatx4 acc[8][4];
+    for (int layer = 0; layer < 4; ++layer) {
+      const uint16_t *W = layer == 0 ? D.w1_16 : D.w16[layer - 1];
+      const float *bias = layer == 0 ? D.b1 : D.b[layer - 1];
+      const int K = layer == 0 ? 32 : INF_HIDDEN;
+      // lane (row group g, column c) accumulates features obase + 4 c + nt,
+      // started at the bias
+      const int col = obase + 4 * fr;
+      floatx4 acc[INF_MT][4];
 #pragma unroll
-      for (int mt = 0; mt < 8; ++mt)
+      for (int nt = 0; nt < 4; ++nt) {
+        const float bn = bias[col + nt];
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int mt = 0; mt < INF_MT; ++mt) acc[mt][nt] = floatx4{bn, bn, bn, bn};
+      }
       // software pipeline: the next k-chunk's weights are requested before
       // this chunk's 32 MFMAs, so the L2 latency hides behind them
       const uint16_t *wrow[4];
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) wrow[nt] = W + (int64_t)(obase + 4 * fr + nt) * INF_HIDDEN + fk;
+      for (int nt = 0; nt < 4; ++nt) wrow[nt] = W + (int64_t)(col + nt) * K + fk;
       uint4 bnext[4];
 #pragma unroll
       for (int nt = 0; nt < 4; ++nt) bnext[nt] = *reinterpret_cast<const uint4 *>(wrow[nt]);
 #pragma unroll
       for (int kc = 0; kc < INF_HIDDEN; kc += 32) {
-        bf16x8 bw[4];
+        if (kc < K) {
+          bf16x8 bw[4];
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) bw[nt] = __builtin_bit_cast(bf16x8, bnext[nt]);
-        if (kc + 32 < INF_HIDDEN) {
+          for (int nt = 0; nt < 4; ++nt) bw[nt] = __builtin_bit_cast(bf16x8, bnext[nt]);
+          if (kc + 32 < K) {
 #pragma unroll
-          for (int nt = 0; nt < 4; ++nt) bnext[nt] = *reinterpret_cast<const uint4 *>(wrow[nt] + kc + 32);
-        }
+            for (int nt = 0; nt < 4; ++nt) bnext[nt] = *reinterpret_cast<const uint4 *>(wrow[nt] + kc + 32);
+          }
 #pragma unroll
-        for (int mt = 0; mt < 8; ++mt) {
-          uint4 v = *reinterpret_cast<const uint4 *>(&act[16 * mt + fr][kc + fk]);
-          bf16x8 a = __builtin_bit_cast(bf16x8, v);
+          for (int mt = 0; mt < INF_MT; ++mt) {
+            uint4 v = *reinterpret_cast<const uint4 *>(&act[16 * mt + fr][kc + fk]);
+            bf16x8 a = __builtin_bit_cast(bf16x8, v);
 #pragma unroll
-          for (int nt = 0; nt < 4; ++nt)
-            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[nt], acc[mt][nt], 0, 0, 0);
+            for (int nt = 0; nt < 4; ++nt)
+              acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[nt], acc[mt][nt], 0, 0, 0);
+          }
         }
       }
       __syncthreads();  // every wave has read this layer's inputs
-      // lane (row group g, column c): features obase + 4 c + {0..3}
-      const int col = obase + 4 * fr;
-      const float b0 = D.b[layer][col], b1 = D.b[layer][col + 1];
-      const float b2 = D.b[layer][col + 2], b3 = D.b[layer][col + 3];
 #pragma unroll
-      for (int mt = 0; mt < 8; ++mt)
+      for (int mt = 0; mt < INF_MT; ++mt)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int row = 16 * mt + 4 * (lane >> 4) + q;
           uint2 v;
-          v.x = pack_bf16(silu(acc[mt][0][q] + b0), silu(acc[mt][1][q] + b1));
-          v.y = pack_bf16(silu(acc[mt][2][q] + b2), silu(acc[mt][3][q] + b3));
+          v.x = pack_bf16(silu(acc[mt][0][q]), silu(acc[mt][1][q]));
+          v.y = pack_bf16(silu(acc[mt][2][q]), silu(acc[mt][3][q]));
           *reinterpret_cast<uint2 *>(&act[row][col]) = v;
         }
       __syncthreads();
     }
-    // ---- output layer on MFMA: wave w -> rays [32 w, 32 w + 32).  The
-    // three outputs' weights are split into bf16 high and low parts (rows
-    // 0..2 and 3..5 of one 16-row tile, the rest zero), so the products
-    // carry the float32 weights to ~2^-17 at no extra MFMA ------------------
+    // ---- output layer on MFMA: wave w -> rays [16 INF_OQ w, +16 INF_OQ).
+    // The three outputs' weights are split into bf16 high and low parts
+    // (rows 0..2 and 3..5 of one 16-row tile, the rest zero), so the
+    // products carry the float32 weights to ~2^-17 at no extra MFMA --------
     __shared__ float yout[INF_RAYS][8];
     {
-      floatx4 oacc[2] = {floatx4{0.0f, 0.0f, 0.0f, 0.0f}, floatx4{0.0f, 0.0f, 0.0f, 0.0f}};
+      floatx4 oacc[INF_OQ];
+#pragma unroll
+      for (int q = 0; q < INF_OQ; ++q) oacc[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int kc = 0; kc < INF_HIDDEN; kc += 32) {
         uint4 vb = *reinterpret_cast<const uint4 *>(D.w5_16 + fr * INF_HIDDEN + kc + fk);
         bf16x8 b5 = __builtin_bit_cast(bf16x8, vb);
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          uint4 va = *reinterpret_cast<const uint4 *>(&act[32 * wave + 16 * q + fr][kc + fk]);
+        for (int q = 0; q < INF_OQ; ++q) {
+          uint4 va = *reinterpret_cast<const uint4 *>(&act[16 * (INF_OQ * wave + q) + fr][kc + fk]);
           oacc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, va), b5, oacc[q], 0, 0, 0);
         }
       }
       if (fr < 6) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+        for (int q = 0; q < INF_OQ; ++q)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) yout[32 * wave + 16 * q + 4 * (lane >> 4) + r][fr] = oacc[q][r];
+          for (int r = 0; r < 4; ++r) yout[16 * (INF_OQ * wave + q) + 4 * (lane >> 4) + r][fr] = oacc[q][r];
       }
     }
     __syncthreads();
@@ -810,7 +835,8 @@ extern "C" int32_t optk_surrogate_infer(const optk_surrogate_desc *desc, const o
     return fail(OPTK_EINVAL, "missing ray array");
   if (IO.fallback_list && (!IO.fallback_count || IO.fallback_capacity < 0))
     return fail(OPTK_EINVAL, "fallback list needs a counter and a capacity");
-  if (!D.w1 || !D.b1 || !D.w5 || !D.b5 || !D.w5_16) return fail(OPTK_EINVAL, "missing weights");
+  if (!D.w1_16 || !D.b1 || !D.w5 || !D.b5 || !D.w5_16) return fail(OPTK_EINVAL, "missing weights");
+  if ((uintptr_t)D.w1_16 % 16) return fail(OPTK_EINVAL, "bf16 weights must be 16-byte aligned");
   if ((uintptr_t)D.w5_16 % 16) return fail(OPTK_EINVAL, "bf16 weights must be 16-byte aligned");
   for (int l = 0; l < 3; ++l) {
     if (!D.w16[l] || !D.b[l]) return fail(OPTK_EINVAL, "missing weights");
@@ -827,7 +853,7 @@ extern "C" int32_t optk_surrogate_infer(const optk_surrogate_desc *desc, const o
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("device query: ") + hipGetErrorString(e));
   const int64_t tiles = (IO.n + INF_RAYS - 1) / INF_RAYS;
-  int64_t blocks = cus > 0 ? (int64_t)cus * 2 : 1024;  // two resident blocks per CU
+  int64_t blocks = cus > 0 ? (int64_t)cus * INF_BLOCKS_PER_CU : 1024;  // all resident
   if (blocks > tiles) blocks = tiles;
   hipLaunchKernelGGL(surrogate_infer, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)hip_stream, D, IO);
   e = hipGetLastError();

@@ -542,6 +542,16 @@ class Surrogate:
             d.b[k] = m.view(m.p, layer, 1).data_ptr()
         d.w5 = m.view(m.p, 4, 0).data_ptr()
         d.b5 = m.view(m.p, 4, 1).data_ptr()
+        # the first layer on MFMA: row o = [hi | hi | lo | 0] of w1[o]
+        w1 = m.view(m.p, 0, 0).reshape(WIDTHS[1], WIDTHS[0])
+        hi1 = _bf16_round(w1)
+        rows = torch.zeros(WIDTHS[1], 32, dtype=torch.float32, device=self.dev)
+        rows[:, 0:5] = hi1
+        rows[:, 5:10] = hi1
+        rows[:, 10:15] = w1 - hi1
+        self._w1_16 = torch.empty(WIDTHS[1] * 32, dtype=torch.int16, device=self.dev)
+        _op(_lib.MLP_TO_BF16, f32=[rows.reshape(-1)], u16=[self._w1_16], rows=1, cols=WIDTHS[1] * 32)
+        d.w1_16 = self._w1_16.data_ptr()
         # the output layer runs on MFMA in the fused kernel: bf16 high and
         # low parts of the rows, padded to one 16-row tile
         w5 = m.view(m.p, 4, 0).reshape(3, WIDTHS[4])
