@@ -79,6 +79,20 @@ class AbstractAperture:
 
         return _ops.aperture_mask(self, position)
 
+    def _wire_local(self, num: int) -> np.ndarray:
+        raise NotImplementedError(f"{type(self).__name__} has no wire")
+
+    def wire(self, num: None | int = None) -> Cartesian3dVectorArray:
+        """``num`` points on the aperture's outline in surface coordinates,
+        the aperture's transformation applied (``_apertures.py:121-130``).
+        Host float64 arrays."""
+        num = self.samples_wire if num is None else int(num)
+        p = self._wire_local(num)
+        if self.transformation is not None:
+            a = self.transformation.affine()
+            p = a.matrix @ p + a.vector[:, None]
+        return Cartesian3dVectorArray(p[0], p[1], p[2])
+
     def clip_rays(self, rays):
         from . import _ops
 
@@ -96,6 +110,12 @@ class CircularAperture(AbstractAperture):
 
     def _params(self):
         return (float(self.radius), 0.0, 0.0, 0.0)
+
+    def _wire_local(self, num):
+        """Azimuths 0..360 degrees inclusive (``_apertures.py:338-355``)."""
+        az = np.deg2rad(np.linspace(0.0, 360.0, num))
+        r = float(self.radius)
+        return np.stack([r * np.cos(az), r * np.sin(az), np.zeros(num)])
 
     @property
     def bound_lower(self):
@@ -125,6 +145,27 @@ class CircularSectorAperture(AbstractAperture):
             0.0,
         )
 
+    def _wire_local(self, num):
+        """Three segments -- the arm out from the vertex, the arc, the arm
+        back -- sharing the points evenly, the last one closing the outline
+        (``_apertures.py:548-590``)."""
+        a0, a1 = np.deg2rad(np.float64(self.angle_start)), np.deg2rad(np.float64(self.angle_stop))
+        r0 = float(self.radius)
+        per = num / 3
+        parts, done = [], 0
+        for seg in range(3):
+            k = int((seg + 1) * per - done)
+            done += k
+            t = np.linspace(0.0, 1.0, k, endpoint=done == num)
+            if seg == 0:
+                r, a = r0 * t, np.full(k, a0)
+            elif seg == 1:
+                r, a = np.full(k, r0), a0 + (a1 - a0) * t
+            else:
+                r, a = r0 * (1 - t), np.full(k, a1)
+            parts.append(np.stack([r * np.cos(a), r * np.sin(a), np.zeros(k)]))
+        return np.concatenate(parts, axis=1)
+
 
 @dataclasses.dataclass(eq=False)
 class EllipticalAperture(AbstractAperture):
@@ -138,6 +179,12 @@ class EllipticalAperture(AbstractAperture):
 
     def _params(self):
         return (float(self.radius.x), float(self.radius.y), 0.0, 0.0)
+
+    def _wire_local(self, num):
+        """Azimuths 0..360 degrees inclusive (``_apertures.py:704-720``)."""
+        az = np.deg2rad(np.linspace(0.0, 360.0, num))
+        return np.stack([float(self.radius.x) * np.cos(az), float(self.radius.y) * np.sin(az),
+                         np.zeros(num)])
 
 
 @dataclasses.dataclass(eq=False)
@@ -164,6 +211,24 @@ class AbstractPolygonalAperture(AbstractAperture):
                 f"polygon must have at most {MAX_POLYGON_VERTICES} vertices"
             )
         return xy
+
+    def _wire_local(self, num):
+        """The sides in vertex order, each from its vertex toward the next,
+        sharing the points evenly; the last side ends on the first vertex
+        (``_apertures.py:794-830``)."""
+        v = self.vertices
+        xyz = np.stack(np.broadcast_arrays(np.asarray(v.x, np.float64), np.asarray(v.y, np.float64),
+                                           np.asarray(v.z, np.float64)))
+        nv = xyz.shape[1]
+        per = num / nv
+        parts, done = [], 0
+        for i in range(nv):
+            k = int((i + 1) * per - done)
+            done += k
+            t = np.linspace(0.0, 1.0, k, endpoint=done == num)
+            left, right = xyz[:, i], xyz[:, (i + 1) % nv]
+            parts.append(left[:, None] + (right - left)[:, None] * t[None, :])
+        return np.concatenate(parts, axis=1)
 
     @property
     def bound_lower(self):

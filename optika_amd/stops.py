@@ -1,0 +1,363 @@
+"""The stop solve: rays that graze the edges of both the field stop and the
+pupil stop, and the physical extent of the field and pupil they give.
+
+Mirrors ``optika/systems/_sequential.py:357-789``:
+
+* ``solve_stop_rays`` -- ``_calc_rayfunction_stops_only`` (``:396-662``):
+  for each (pupil stop, field stop) pair, every point on the first stop's
+  outline (``aperture.wire``) is joined to every point on the last stop's
+  outline by a ray whose free component (direction for a stop in
+  millimetres, position for an angular stop) is found by Newton's method
+  with a forward-difference Jacobian;
+* ``rayfunction_stops`` -- ``_calc_rayfunction_stops`` (``:664-712``): those
+  rays propagated back to the object surface;
+* ``field_pupil`` / ``denormalize`` -- ``field_min`` .. ``pupil_max`` and
+  ``_denormalize_grid`` (``:724-789``).
+
+Every ray propagation -- each Newton iteration's residual and Jacobian
+columns as one batch, and the walk back to the object -- runs through the
+fused trace on the device (``optk_trace``).  The host keeps the outlines, the
+seeds and the 2 x 2 Newton updates (float64), as the reference does.
+
+The reference's root finder is ``named_arrays.optimize.root_newton`` with
+``named_arrays.jacobian`` (``named_arrays~=2.1``, not vendored): Newton steps
+``x <- x - J(x)^-1 f(x)`` until every residual component is below
+``max_abs_error``, the Jacobian by forward differences of step ``dx``.
+Its iteration count and stopping details are not visible here; the solve
+converges to the same roots (residual below ``1e-9`` of the target
+outline's extent).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _trace
+from .rays import RayVectorArray
+from .vectors import Cartesian3dVectorArray
+
+__all__ = ["solve_stop_rays", "rayfunction_stops", "field_pupil", "denormalize", "angles"]
+
+MAX_ITERATIONS = 100
+
+
+def angles(dx, dy, dz):
+    """Azimuth and elevation (degrees) of direction cosines
+    (``optika/_util.py:69-97``)."""
+    length = np.sqrt(dx * dx + dy * dy + dz * dz)
+    return -np.rad2deg(np.arctan2(dx, dz)), -np.rad2deg(np.arcsin(dy / length))
+
+
+def _apply(t, p, direction: bool = False, inverse: bool = False):
+    """A transformation of points or directions p [3][...] (host float64)."""
+    if t is None:
+        return p
+    if inverse:
+        t = t.inverse
+    a = t.affine()
+    out = np.tensordot(a.matrix, p, axes=1)
+    if not direction:
+        out = out + a.vector.reshape((3,) + (1,) * (p.ndim - 1))
+    return out
+
+
+def _sag(surface, p):
+    """z of the surface's sag at the points p [3][...] (x, y used)."""
+    if surface.sag is None:
+        return np.zeros(p.shape[1:])
+    from .vectors import Cartesian3dVectorArray as V
+
+    dev = _trace.device()
+    flat = [torch.as_tensor(np.ascontiguousarray(c.reshape(-1)), device=dev) for c in p]
+    z = surface.sag(V(flat[0], flat[1], flat[2]))
+    return z.reshape(-1).cpu().numpy().reshape(p.shape[1:])
+
+
+def anchor_surface(subsystem):
+    """The first surface after the subsystem's start with optical power -- a
+    mirror, a curved sag or rulings -- else the last
+    (``_sequential.py:347-370``)."""
+    from .sags import NoSag
+
+    for s in subsystem[1:]:
+        if s.material is not None and getattr(s.material, "is_mirror", False):
+            return s
+        if s.sag is not None and not isinstance(s.sag, NoSag):
+            return s
+        if s.rulings is not None:
+            return s
+    return subsystem[-1]
+
+
+def _propagate(surfaces, position, direction, wavelength):
+    """Rays [3][N] through ``surfaces`` on the device (system frame in and
+    out); returns position, direction [3][N] on the host."""
+    dev = _trace.device()
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)  # noqa: E731
+    rays = RayVectorArray(
+        wavelength=t(wavelength),
+        position=Cartesian3dVectorArray(t(position[0]), t(position[1]), t(position[2])),
+        direction=Cartesian3dVectorArray(t(direction[0]), t(direction[1]), t(direction[2])),
+    )
+    out = _trace.trace_sequence(list(surfaces), rays)
+    if out.position.x.ndim != 1:
+        raise NotImplementedError(
+            "the stop solve needs a single ray path between the stops "
+            "(one diffraction order per grating)"
+        )
+    pos = np.stack([out.position.x.cpu().numpy(), out.position.y.cpu().numpy(),
+                    out.position.z.cpu().numpy()])
+    dirs = np.stack([out.direction.x.cpu().numpy(), out.direction.y.cpu().numpy(),
+                     out.direction.z.cpu().numpy()])
+    return pos, dirs
+
+
+def _wire(surface, num) -> np.ndarray:
+    ap = surface.aperture
+    if ap is None:
+        raise ValueError(f"stop surface {surface.name!r} has no aperture")
+    w = ap.wire(num)
+    return np.stack([np.asarray(w.x, np.float64), np.asarray(w.y, np.float64),
+                     np.asarray(w.z, np.float64)]), ap.clips_direction
+
+
+def solve_stop_rays(system, wavelength, samples_pupil_stop: int = 21,
+                    samples_field_stop: int = 21) -> dict:
+    """Rays on the first stop joining the outlines of the field and pupil
+    stops (``_sequential.py:396-662``), in the system frame.
+
+    ``wavelength``: 1-D array (nm).  Returns ``position``, ``direction``
+    [3][S1][S2][W] and the outline grids ``field``, ``pupil`` [2][S1][S2][W],
+    with S1 the first stop's outline samples (``samples_pupil_stop``) and S2
+    the last stop's (``samples_field_stop``)."""
+    surfaces = system.surfaces_all
+    ip = [i for i, s in enumerate(surfaces) if s.is_pupil_stop]
+    ifs = [i for i, s in enumerate(surfaces) if s.is_field_stop]
+    if not ip:
+        raise ValueError("pupil not defined")
+    if not ifs:
+        raise ValueError("field stop not defined")
+    wl = np.atleast_1d(np.asarray(wavelength, np.float64))
+    result = None
+    while ip and ifs:
+        i_p, i_f = ip[0], ifs[0]
+        first, last = min(i_p, i_f), max(i_p, i_f)
+        sub = surfaces[first:last + 1]
+        s_first, s_last = sub[0], sub[-1]
+        g1, first_angular = _wire(s_first, samples_pupil_stop)
+        g2, last_angular = _wire(s_last, samples_field_stop)
+        S1, S2, W = g1.shape[1], g2.shape[1], wl.size
+        shape = (S1, S2, W)
+        gf = np.broadcast_to(g1[:2, :, None, None], (2,) + shape)  # first stop outline
+        gl = np.broadcast_to(g2[:2, None, :, None], (2,) + shape)  # last stop outline
+        if s_first.is_pupil_stop:
+            ip.pop(0)
+            pupil_grid, field_grid = gf, gl
+        else:
+            ifs.pop(0)
+            pupil_grid, field_grid = gl, gf
+        wave = np.broadcast_to(wl[None, None, :], shape)
+
+        if not first_angular:
+            # outline in millimetres: the ray starts on it, free direction
+            pos = np.zeros((3,) + shape)
+            pos[:2] = gf
+            pos[2] = _sag(s_first, pos)
+            dirs = np.zeros((3,) + shape)
+            dirs[2] = 1.0
+            variable = "direction"
+        else:
+            # angular outline: the direction is fixed, free position
+            dirs = np.zeros((3,) + shape)
+            dirs[:2] = gf
+            dirs[2] = np.sqrt(1 - (gf[0] ** 2 + gf[1] ** 2))
+            pos = np.zeros((3,) + shape)
+            variable = "position"
+
+        # seed: aim at the target point on the last stop when no powered
+        # surface lies between the stops, else at the first powered surface
+        anchor = anchor_surface(sub)
+        if anchor is s_last and not last_angular:
+            aim = np.zeros((3,) + shape)
+            aim[:2] = gl
+            aim[2] = _sag(s_last, aim)
+            aim = _apply(s_last.transformation, aim)
+        else:
+            aim = np.zeros((3,) + shape)
+            aim = _apply(anchor.transformation, aim)
+        aim = _apply(s_first.transformation, aim, inverse=True)
+        if variable == "direction":
+            d = aim - pos
+            d = d / np.sqrt((d * d).sum(0))
+            flip = np.sign(d[2])
+            nz = d[2] != 0
+            dirs = np.stack([np.where(nz, flip * d[0], 0.0), np.where(nz, flip * d[1], 0.0),
+                             np.where(nz, flip * d[2], 1.0)])
+        else:
+            t = aim[2] / dirs[2]
+            pos = np.stack([aim[0] - dirs[0] * t, aim[1] - dirs[1] * t, np.zeros(shape)])
+            pos[2] = _sag(s_first, pos)
+        pos = _apply(s_first.transformation, pos)
+        dirs = _apply(s_first.transformation, dirs, direction=True)
+
+        target = "direction" if last_angular else "position"
+        scale = max(np.ptp(g2[0]), np.ptp(g2[1]))
+        max_abs_error = 1e-9 * max(scale, 1.0)
+        dx = 1e-6 if variable == "direction" else 1e-6 * max(scale, 1.0)
+
+        def zfunc(a):
+            if variable == "direction":
+                return np.sqrt(1 - (a[0] ** 2 + a[1] ** 2))
+            return _sag(s_first, np.stack([a[0], a[1], np.zeros(a.shape[1:])]))
+
+        def residual(a, reps):
+            """f(a) for reps stacked copies of the grid (a: [2][reps * N])."""
+            n = a.shape[1]
+            p = np.tile(pos.reshape(3, -1), (1, reps))
+            d = np.tile(dirs.reshape(3, -1), (1, reps))
+            v = np.stack([a[0], a[1], zfunc(a)])
+            if variable == "direction":
+                d = v
+            else:
+                p = v
+            p2, d2 = _propagate(sub[1:], p, d, np.tile(wave.reshape(-1), reps))
+            if target == "position":
+                q = _apply(s_last.transformation, p2, inverse=True)
+            else:
+                q = _apply(s_last.transformation, d2, direction=True, inverse=True)
+            return q[:2] - np.tile(gl.reshape(2, -1), (1, reps))
+
+        x0 = (dirs if variable == "direction" else pos)[:2].reshape(2, -1).copy()
+        root = _newton(residual, x0, dx, max_abs_error, s_first, s_last)
+        z = zfunc(root)
+        v = np.stack([root[0], root[1], z]).reshape((3,) + shape)
+        if variable == "direction":
+            dirs = v
+        else:
+            pos = v
+        result = dict(position=pos, direction=dirs, field=np.array(field_grid),
+                      pupil=np.array(pupil_grid), wavelength=wave, first=first)
+    return result
+
+
+def _newton(residual, x, dx, tol, s_first, s_last):
+    """Newton's method on independent 2-vectors x [2][N]: the residual and
+    both forward-difference Jacobian columns in one device batch."""
+    n = x.shape[1]
+    for _ in range(MAX_ITERATIONS):
+        batch = np.concatenate([x, x + np.array([[dx], [0.0]]), x + np.array([[0.0], [dx]])], axis=1)
+        f = residual(batch, 3)
+        f0, fx, fy = f[:, :n], f[:, n:2 * n], f[:, 2 * n:]
+        if np.all(np.abs(f0) < tol):
+            return x
+        j00, j10 = (fx[0] - f0[0]) / dx, (fx[1] - f0[1]) / dx
+        j01, j11 = (fy[0] - f0[0]) / dx, (fy[1] - f0[1]) / dx
+        det = j00 * j11 - j01 * j10
+        step0 = (j11 * f0[0] - j01 * f0[1]) / det
+        step1 = (-j10 * f0[0] + j00 * f0[1]) / det
+        x = x - np.stack([step0, step1])
+    raise ValueError(
+        f"Could not solve for the rays connecting the stop surfaces {s_first.name!r} and "
+        f"{s_last.name!r}. If the field stop is only partially reachable at a single "
+        f"wavelength (e.g. a spectrograph sensor), consider marking the object surface, "
+        f"with an angular (dimensionless, sine of the half-angle) aperture, as the field "
+        f"stop instead."
+    )
+
+
+def rayfunction_stops(system, wavelength, samples_pupil_stop: int = 21,
+                      samples_field_stop: int = 21) -> dict:
+    """The stop rays propagated back to the object surface
+    (``_sequential.py:664-712``): ``position``, ``direction`` [3][S1][S2][W]
+    in the system frame, directions pointing into the system."""
+    surfaces = system.surfaces_all
+    ip = [i for i, s in enumerate(surfaces) if s.is_pupil_stop]
+    ifs = [i for i, s in enumerate(surfaces) if s.is_field_stop]
+    if not ip:
+        raise ValueError("Pupil stop is not defined for this system.")
+    if not ifs:
+        raise ValueError("Field stop is not defined for this system.")
+    index_stop = min(ip[-1], ifs[-1])
+    back = surfaces[index_stop::-1]
+    r = solve_stop_rays(system, wavelength, samples_pupil_stop, samples_field_stop)
+    shape = r["position"].shape[1:]
+    p, d = _propagate(back, r["position"].reshape(3, -1), r["direction"].reshape(3, -1),
+                      r["wavelength"].reshape(-1))
+    obj = back[-1]
+    pl = _apply(obj.transformation, p, inverse=True)
+    dl = _apply(obj.transformation, d, direction=True, inverse=True)
+    if obj.sag is not None:
+        from . import _ops
+
+        dev = _trace.device()
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)  # noqa: E731
+        nrm = _ops.sag_normal(obj.sag, Cartesian3dVectorArray(t(pl[0]), t(pl[1]), t(pl[2])))
+        nrm = np.stack([nrm.x.cpu().numpy(), nrm.y.cpu().numpy(), nrm.z.cpu().numpy()])
+    else:
+        nrm = np.array([0.0, 0.0, -1.0])[:, None]
+    away = (dl * nrm).sum(0) > 0
+    d = np.where(away[None, :], -d, d)
+    pos, dirs = p.reshape((3,) + shape), d.reshape((3,) + shape)
+    if system.transformation is not None:
+        pos = _apply(system.transformation, pos)
+        dirs = _apply(system.transformation, dirs, direction=True)
+    return dict(position=pos, direction=dirs, wavelength=r["wavelength"])
+
+
+def field_pupil(system, rays: dict):
+    """(field, pupil) [2][S1][S2][W] of stop rays: angles (degrees) and
+    positions (mm), which is which by where the object is
+    (``_sequential.py:724-786``)."""
+    p, d = rays["position"], rays["direction"]
+    ax, ay = angles(d[0], d[1], d[2])
+    ang = np.stack([ax, ay])
+    xy = p[:2]
+    if system.object_is_at_infinity:
+        return ang, xy
+    return xy, ang
+
+
+def denormalize(system, wavelength, field, pupil, normalized_field=True, normalized_pupil=True,
+                samples: int = 21):
+    """``_denormalize_grid`` (``_sequential.py:748-789``): map normalized
+    field and pupil coordinates in [-1, 1] onto the extent the stop rays
+    span at each ray's wavelength.  ``wavelength``, ``field.x`` ... are
+    device tensors (or scalars) of one length; returns the new field and
+    pupil components (device float64)."""
+    from .vectors import Cartesian2dVectorArray
+
+    dev = _trace.device()
+    if isinstance(wavelength, torch.Tensor):
+        wl_host = wavelength.detach().reshape(-1).double().cpu().numpy()
+        per_ray = wavelength.ndim > 0
+    else:
+        wl_host = np.atleast_1d(np.asarray(wavelength, np.float64)).reshape(-1)
+        per_ray = np.ndim(wavelength) > 0
+    uniq, inverse = np.unique(wl_host, return_inverse=True)
+    rays = rayfunction_stops(system, uniq, samples, samples)
+    fld, pup = field_pupil(system, rays)
+    idx = torch.as_tensor(inverse.reshape(-1), device=dev) if per_ray else None
+
+    def extent(v):
+        lo = v.min(axis=(1, 2))
+        return lo, v.max(axis=(1, 2)) - lo  # [2][W]
+
+    def apply(vec, ext):
+        lo, ptp = (torch.as_tensor(a, device=dev) for a in ext)
+        out = []
+        for k, c in enumerate((vec.x, vec.y)):
+            c = c if isinstance(c, torch.Tensor) else torch.as_tensor(np.asarray(c, np.float64))
+            c = c.to(device=dev, dtype=torch.float64)
+            if idx is None:
+                lo_k, ptp_k = lo[k][0], ptp[k][0]
+            else:
+                lo_k, ptp_k = lo[k][idx], ptp[k][idx]
+            out.append(ptp_k * (c + 1) / 2 + lo_k)
+        return Cartesian2dVectorArray(out[0], out[1])
+
+    if normalized_field:
+        field = apply(field, extent(fld))
+    if normalized_pupil:
+        pupil = apply(pupil, extent(pup))
+    return field, pupil

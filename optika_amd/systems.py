@@ -11,9 +11,10 @@ the object and system transformations, every surface, and for
 ``rayfunction`` the final change into sensor coordinates -- runs as one
 fused device kernel launch.
 
-Field and pupil must be given in physical units (``normalized_field=False``,
-``normalized_pupil=False``): the stop solve that denormalizes a normalized
-grid (``_sequential.py:396-789``) is not part of this build and raises.
+Normalized field and pupil coordinates (the default) are mapped onto the
+physical extent found by the stop solve (``stops.py``,
+``_sequential.py:396-789``) before the trace; with
+``normalized_field=normalized_pupil=False`` they are used as given.
 """
 
 from __future__ import annotations
@@ -21,10 +22,12 @@ from __future__ import annotations
 import dataclasses
 from typing import Any, Sequence
 
+import numpy as np
+
 from .rays import RayFunctionArray
 from .surfaces import Surface
 from .transformations import AbstractTransformation
-from .vectors import ObjectVectorArray
+from .vectors import Cartesian2dVectorArray, ObjectVectorArray
 
 __all__ = ["AbstractSequentialSystem", "SequentialSystem"]
 
@@ -51,14 +54,91 @@ class AbstractSequentialSystem:
             result[0] = dataclasses.replace(result[0], is_field_stop=True)
         return result
 
-    def _check_physical(self, normalized_field: bool, normalized_pupil: bool):
-        if normalized_field or normalized_pupil:
-            raise NotImplementedError(
-                "normalized field/pupil coordinates need the stop solve "
-                "(optika/systems/_sequential.py:396-789), which this build does "
-                "not provide; pass physical coordinates with "
-                "normalized_field=False, normalized_pupil=False"
+    # -- stops (_sequential.py:293-345, 714-789) -------------------------------
+    @property
+    def index_field_stop(self) -> int:
+        idx = [i for i, s in enumerate(self.surfaces_all) if s.is_field_stop]
+        if not idx:
+            raise ValueError(
+                "Field stop is not defined for this system."
+                "Set `is_field_stop=True` for at least one surface in this system."
             )
+        return idx[-1]
+
+    @property
+    def index_pupil_stop(self) -> int:
+        idx = [i for i, s in enumerate(self.surfaces_all) if s.is_pupil_stop]
+        if not idx:
+            raise ValueError(
+                "Pupil stop is not defined for this system."
+                "Set `is_pupil_stop=True` for at least one surface in this system."
+            )
+        return idx[-1]
+
+    @property
+    def field_stop(self):
+        return self.surfaces_all[self.index_field_stop]
+
+    @property
+    def pupil_stop(self):
+        return self.surfaces_all[self.index_pupil_stop]
+
+    def _stop_wavelengths(self):
+        grid = self.grid_input if self.grid_input is not None else ObjectVectorArray()
+        w = grid.wavelength
+        if hasattr(w, "detach"):
+            w = w.detach().cpu().numpy()
+        return np.unique(np.atleast_1d(np.asarray(w, np.float64)))
+
+    @property
+    def rayfunction_stops(self) -> dict:
+        """Rays on the object surface that graze both stops' outlines
+        (21 x 21 samples) at every wavelength of ``grid_input``
+        (``_sequential.py:714-722``): ``position``, ``direction``
+        [3][21][21][W] and ``wavelength`` (host float64)."""
+        from . import stops
+
+        return stops.rayfunction_stops(self, self._stop_wavelengths(), 21, 21)
+
+    def _field_pupil(self):
+        from . import stops
+
+        return stops.field_pupil(self, self.rayfunction_stops)
+
+    @property
+    def field_min(self) -> Cartesian2dVectorArray:
+        """Lower left corner of the field of view, per wavelength of
+        ``grid_input`` (degrees at infinity, else mm; ``:724-734``)."""
+        f, _ = self._field_pupil()
+        return Cartesian2dVectorArray(*f.min(axis=(1, 2)))
+
+    @property
+    def field_max(self) -> Cartesian2dVectorArray:
+        f, _ = self._field_pupil()
+        return Cartesian2dVectorArray(*f.max(axis=(1, 2)))
+
+    @property
+    def pupil_min(self) -> Cartesian2dVectorArray:
+        """Lower left corner of the entrance pupil, per wavelength of
+        ``grid_input`` (mm at infinity, else degrees; ``:748-771``)."""
+        _, p = self._field_pupil()
+        return Cartesian2dVectorArray(*p.min(axis=(1, 2)))
+
+    @property
+    def pupil_max(self) -> Cartesian2dVectorArray:
+        _, p = self._field_pupil()
+        return Cartesian2dVectorArray(*p.max(axis=(1, 2)))
+
+    def _physical_grid(self, grid: ObjectVectorArray, normalized_field: bool,
+                       normalized_pupil: bool) -> ObjectVectorArray:
+        """``_denormalize_grid`` (``_sequential.py:748-789``)."""
+        if not normalized_field and not normalized_pupil:
+            return grid
+        from . import stops
+
+        field, pupil = stops.denormalize(self, grid.wavelength, grid.field, grid.pupil,
+                                         normalized_field, normalized_pupil)
+        return ObjectVectorArray(grid.wavelength, field, pupil)
 
     def raytrace(
         self,
@@ -80,8 +160,7 @@ class AbstractSequentialSystem:
         """
         from . import _trace
 
-        self._check_physical(normalized_field, normalized_pupil)
-        grid = self._grid(wavelength, field, pupil)
+        grid = self._input_grid(wavelength, field, pupil, normalized_field, normalized_pupil)
         return _trace.trace_system(
             self,
             grid,
@@ -105,8 +184,7 @@ class AbstractSequentialSystem:
         (``_sequential.py:927-988``)."""
         from . import _trace
 
-        self._check_physical(normalized_field, normalized_pupil)
-        grid = self._grid(wavelength, field, pupil)
+        grid = self._input_grid(wavelength, field, pupil, normalized_field, normalized_pupil)
         return _trace.trace_system(
             self,
             grid,
@@ -115,6 +193,15 @@ class AbstractSequentialSystem:
             sensor_frame=True,
             precision=precision,
         )
+
+    def _input_grid(self, wavelength, field, pupil, normalized_field, normalized_pupil):
+        """The physical input grid: with no wavelength, field or pupil given,
+        ``grid_input`` read as normalized whatever the flags
+        (``_rayfunction_input``, ``_sequential.py:830-834``, ``:888-889``)."""
+        if wavelength is None and field is None and pupil is None:
+            normalized_field = normalized_pupil = True
+        grid = self._grid(wavelength, field, pupil)
+        return self._physical_grid(grid, normalized_field, normalized_pupil)
 
     def _grid(self, wavelength, field, pupil) -> ObjectVectorArray:
         grid = self.grid_input

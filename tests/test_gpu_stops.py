@@ -1,0 +1,91 @@
+"""The stop solve on the device (optika_amd/stops.py) against the oracle's
+(oracle/stops.py, pinned by the reference's known answers and closed forms
+in tests/test_oracle_stops.py), and ``raytrace`` / ``rayfunction`` with
+normalized field and pupil coordinates (the reference's default) against the
+oracle trace of the oracle-denormalized grid."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import stops as S
+from oracle import trace as T
+from optika_amd.vectors import Cartesian2dVectorArray
+from stop_systems import RADIUS_FIELD_GRAZING, RADIUS_FIELD_NEWTONIAN, grazing, newtonian
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("make,radius", [(newtonian, RADIUS_FIELD_NEWTONIAN),
+                                         (grazing, RADIUS_FIELD_GRAZING)])
+def test_field_and_pupil_extent_match_oracle(make, radius):
+    print(f"case: stop extent {make.__name__}", flush=True)
+    system = make()
+    (flo, fhi), (plo, phi) = S.field_pupil_extent(system, [500.0])
+    got = [system.field_min, system.field_max, system.pupil_min, system.pupil_max]
+    want = [flo, fhi, plo, phi]
+    for g, w in zip(got, want):
+        np.testing.assert_allclose(np.stack([g.x, g.y]), w, rtol=1e-9, atol=1e-9)
+    assert np.all(np.abs(np.asarray(system.field_max.x) - radius) < 1e-6)
+
+
+@pytest.mark.parametrize("make", [newtonian, grazing])
+def test_normalized_rayfunction_matches_oracle(make):
+    print(f"case: normalized rayfunction {make.__name__}", flush=True)
+    system = make()
+    rng = np.random.default_rng(5)
+    n = 4000
+    u = rng.uniform(-1, 1, (4, n))
+    dev = torch.device("cuda")
+    t = lambda a: torch.as_tensor(a, device=dev)  # noqa: E731
+    out = system.rayfunction(wavelength=500.0, field=Cartesian2dVectorArray(t(u[0]), t(u[1])),
+                             pupil=Cartesian2dVectorArray(t(u[2]), t(u[3])))
+    fx, fy, px, py = S.denormalize(system, 500.0, u[0], u[1], u[2], u[3])
+    ref = T.rayfunction(system, 500.0, fx, fy, px, py)[0]
+    r = out.outputs
+    np.testing.assert_allclose(out.inputs.field.x.cpu().numpy(), fx, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(out.inputs.pupil.y.cpu().numpy(), py, rtol=0, atol=1e-9)
+    m = r.unvignetted.cpu().numpy()
+    agree = (m == ref["unvignetted"]).mean()
+    assert agree > 0.999  # rays within ~1e-9 mm of an edge may differ
+    both = m & ref["unvignetted"]
+    for k, c in enumerate((r.position.x, r.position.y, r.position.z)):
+        np.testing.assert_allclose(c.cpu().numpy()[both], ref["position"][k][both], rtol=0, atol=1e-6)
+
+
+def test_default_grid_and_per_ray_wavelengths():
+    """No arguments: ``grid_input`` read as normalized (the reference's
+    ``_rayfunction_input``); per-ray wavelengths map each ray through its
+    own wavelength's extent."""
+    from optika_amd.vectors import ObjectVectorArray
+
+    print("case: default grid / per-ray wavelengths", flush=True)
+    system = grazing()
+    dev = torch.device("cuda")
+    lin = torch.linspace(-1, 1, 5, device=dev, dtype=torch.float64)
+    gx, gy = torch.meshgrid(lin, lin, indexing="ij")
+    system.grid_input = ObjectVectorArray(
+        wavelength=500.0, field=Cartesian2dVectorArray(gx.reshape(-1), gy.reshape(-1)),
+        pupil=Cartesian2dVectorArray(gx.reshape(-1) * 0.5, gy.reshape(-1) * 0.5))
+    r = system.rayfunction()
+    fx = r.inputs.field.x.cpu().numpy()
+    assert abs(fx.max() - RADIUS_FIELD_GRAZING) < 1e-6 and abs(fx.min() + RADIUS_FIELD_GRAZING) < 1e-6
+    wl = torch.tensor([500.0, 600.0] * 10, device=dev, dtype=torch.float64)
+    f = Cartesian2dVectorArray(torch.ones(20, device=dev, dtype=torch.float64),
+                               torch.zeros(20, device=dev, dtype=torch.float64))
+    p = Cartesian2dVectorArray(torch.ones(20, device=dev, dtype=torch.float64),
+                               torch.zeros(20, device=dev, dtype=torch.float64))
+    r2 = system.raytrace(wavelength=wl, field=f, pupil=p, accumulate=False)
+    (_, fhi), (_, phi) = S.field_pupil_extent(system, [500.0, 600.0])
+    got = r2.inputs.pupil.x.cpu().numpy()
+    np.testing.assert_allclose(got[0::2], phi[0, 0], atol=1e-9)
+    np.testing.assert_allclose(got[1::2], phi[0, 1], atol=1e-9)
+
+
+def test_missing_stop_raises():
+    from optika_amd import stops
+
+    system = newtonian()
+    system.surfaces[1].is_pupil_stop = False
+    with pytest.raises(ValueError, match="upil"):
+        stops.rayfunction_stops(system, [500.0])
