@@ -20,7 +20,10 @@ Other workloads (``--workload``): ``spectrograph`` (the EUV spectrograph,
 4 counted surfaces, two diffraction orders, Mo/Si multilayers on both
 mirrors; 50M rays) and ``tolerance`` (4096 perturbed spectrographs with
 log-uniform ray counts in one persistent batched launch, autotuned and
-replayed from a hipGraph).
+replayed from a hipGraph), ``surrogate`` (the learned surrogate of the
+spectrograph: training from float64 traces, then fused inference) and
+``stops`` (normalized field/pupil coordinates: the stop solve on the device,
+then the trace of the denormalized grid, both inside each step).
 """
 
 from __future__ import annotations
@@ -63,7 +66,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", choices=("newtonian", "spectrograph", "tolerance", "surrogate"),
+    p.add_argument("--workload", choices=("newtonian", "spectrograph", "tolerance", "surrogate", "stops"),
                    default="newtonian")
     p.add_argument("--train-rays", type=int, default=200_000_000,
                    help="surrogate: rays traced (float64) for training")
@@ -518,6 +521,58 @@ def run_surrogate(args):
     return header(args, v_sur, ms_sur, "bf16", config, forms, roofline, None)
 
 
+def run_stops(args):
+    """Normalized-coordinate rayfunction of the reference's stop-test
+    Newtonian: each step solves the stops on the device (21 x 21 Newton
+    rays, every iteration's residual and Jacobian columns in one trace) and
+    traces the denormalized grid."""
+    import torch
+
+    from optika_amd import workloads
+    from optika_amd.vectors import Cartesian2dVectorArray
+
+    n = args.rays or 1 << 22
+    system = workloads.stop_newtonian()
+    S = 3
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(6)
+    u = torch.as_tensor(rng.uniform(-1, 1, (4, n)), device=dev)
+    f = Cartesian2dVectorArray(u[0], u[1])
+    pp = Cartesian2dVectorArray(u[2], u[3])
+    stream = torch.cuda.current_stream()
+
+    def step():
+        return system.rayfunction(wavelength=500.0, field=f, pupil=pp)
+
+    def solve():
+        return system.field_max
+
+    ms, _ = time_calls(step, args.steps, args.warmup, stream)
+    ms_solve, _ = time_calls(solve, args.steps, args.warmup, stream)
+    fmax = system.field_max
+    cpu = None
+    if not args.no_cpu:
+        from oracle import stops as ostops
+
+        times = []
+        for rep in range(3):
+            t0 = time.perf_counter()
+            ostops.field_pupil_extent(system, [500.0])
+            times.append(time.perf_counter() - t0)
+        cpu = {"value": 1e3 / (statistics.median(times[1:]) * 1e3), "unit": "stop solves/s",
+               "cores": 1, "kind": "port",
+               "sample": "one stop solve (21 x 21 rays, numpy float64 oracle trace), median of 2 "
+                         "after 1 warm-up, 1 thread"}
+    forms = {"stop_solve": {"ms_per_solve": ms_solve, "solves_per_s": 1e3 / ms_solve,
+                            "field_max_deg": [float(fmax.x[0]), float(fmax.y[0])],
+                            "field_max_expected_deg": workloads.RADIUS_FIELD_NEWTONIAN}}
+    config = {"workload": f"normalized rayfunction of the stop-test Newtonian (angular "
+                          f"object field stop 0.05 deg, pupil stop 250 mm after the primary), "
+                          f"{n} rays uniform in [-1, 1]^4, stop solve + trace per step",
+              "rays": n, "surfaces": S, "form": "float64 drop-in form"}
+    return header(args, n * S / (ms * 1e-3), ms, "f64", config, forms, None, cpu)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -525,7 +580,8 @@ def main():
               "(rays shard with no exchange; run one process per GPU)", file=sys.stderr)
         sys.exit(2)
     run = {"newtonian": run_newtonian, "spectrograph": run_spectrograph,
-           "tolerance": run_tolerance, "surrogate": run_surrogate}[args.workload]
+           "tolerance": run_tolerance, "surrogate": run_surrogate,
+           "stops": run_stops}[args.workload]
     print(json.dumps(run(args)), flush=True)
 
 

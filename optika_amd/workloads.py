@@ -349,3 +349,75 @@ def tolerance_batch(num_instances: int = 4096, seed: int = 3, min_rays: int = 10
         grid.field_y = np.zeros_like(grid.field_y)
         grid.wavelength = np.full_like(grid.wavelength, WAVELENGTH_CENTER)
     return [spectrograph(p) for p in perts], counts, grid
+
+
+# -- the reference's stop-solve test systems ----------------------------------
+
+RADIUS_FIELD_NEWTONIAN = 0.05  # degrees
+RADIUS_FIELD_GRAZING = 0.25
+
+
+def stop_newtonian() -> systems.SequentialSystem:
+    """The reference's stop-solve Newtonian (``_sequential_test.py:536-576``):
+    angular object aperture (field stop), pupil stop 250 mm downstream of
+    the primary."""
+    return systems.SequentialSystem(
+        object=surfaces.Surface(
+            name="source",
+            aperture=apertures.CircularAperture(
+                radius=np.sin(np.deg2rad(RADIUS_FIELD_NEWTONIAN)), unit="direction"),
+            is_field_stop=True,
+        ),
+        surfaces=[
+            surfaces.Surface(
+                name="primary",
+                sag=sags.SphericalSag(radius=-2000.0),
+                material=materials.Mirror(),
+                aperture=apertures.CircularAperture(radius=50.0),
+                transformation=Cartesian3dTranslation(z=500.0),
+            ),
+            surfaces.Surface(
+                name="aperture",
+                aperture=apertures.CircularAperture(radius=10.0),
+                transformation=Cartesian3dTranslation(z=250.0),
+                is_pupil_stop=True,
+            ),
+        ],
+        sensor=sensors.ImagingSensor(
+            name="sensor", width_pixel=15e-3, num_pixel=Cartesian2dVectorArray(128, 128),
+            transformation=Cartesian3dTranslation(z=-500.0),
+        ),
+    )
+
+
+def stop_grazing() -> systems.SequentialSystem:
+    """The reference's grazing-incidence spectrograph
+    (``_sequential_test.py:600-660``): the paraboloid is the pupil stop."""
+    return systems.SequentialSystem(
+        object=surfaces.Surface(
+            name="source",
+            aperture=apertures.CircularAperture(
+                radius=np.sin(np.deg2rad(RADIUS_FIELD_GRAZING)), unit="direction"),
+            is_field_stop=True,
+        ),
+        surfaces=[
+            surfaces.Surface(
+                name="paraboloid",
+                sag=sags.ParabolicSag(focal_length=-2000.0),
+                material=materials.Mirror(),
+                aperture=apertures.CircularAperture(radius=260.0),
+                transformation=Cartesian3dTranslation(z=2500.0),
+                is_pupil_stop=True,
+            ),
+            surfaces.Surface(
+                name="grating",
+                rulings=rulings.Rulings(spacing=10e-3, diffraction_order=1),
+                aperture=apertures.RectangularAperture(half_width=60.0),
+                transformation=Cartesian3dTranslation(z=1000.0),
+            ),
+        ],
+        sensor=sensors.ImagingSensor(
+            name="sensor", width_pixel=15e-3, num_pixel=Cartesian2dVectorArray(2048, 1024),
+            transformation=Cartesian3dTranslation(x=26.0, z=480.0),
+        ),
+    )
