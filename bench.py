@@ -547,25 +547,41 @@ def run_stops(args):
     def solve():
         return system.field_max
 
+    from optika_amd import stops as dstops
+
+    waves = np.linspace(400.0, 700.0, 64)
+
+    def solve64():
+        return dstops.rayfunction_stops(system, waves)
+
     ms, _ = time_calls(step, args.steps, args.warmup, stream)
     ms_solve, _ = time_calls(solve, args.steps, args.warmup, stream)
+    ms_solve64, _ = time_calls(solve64, max(2, args.steps // 4), 1, stream)
     fmax = system.field_max
     cpu = None
+    forms = {"stop_solve": {"ms_per_solve": ms_solve, "solves_per_s": 1e3 / ms_solve,
+                            "ms_per_solve_64_wavelengths": ms_solve64,
+                            "field_max_deg": [float(fmax.x[0]), float(fmax.y[0])],
+                            "field_max_expected_deg": workloads.RADIUS_FIELD_NEWTONIAN}}
     if not args.no_cpu:
         from oracle import stops as ostops
 
-        times = []
-        for rep in range(3):
-            t0 = time.perf_counter()
-            ostops.field_pupil_extent(system, [500.0])
-            times.append(time.perf_counter() - t0)
-        cpu = {"value": 1e3 / (statistics.median(times[1:]) * 1e3), "unit": "stop solves/s",
+        def cpu_ms(w, reps=3):
+            times = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                ostops.field_pupil_extent(system, w)
+                times.append(time.perf_counter() - t0)
+            return statistics.median(times[1:]) * 1e3
+
+        c1 = cpu_ms([500.0])
+        c64 = cpu_ms(list(waves))
+        forms["stop_solve"]["cpu_oracle_ms_per_solve"] = c1
+        forms["stop_solve"]["cpu_oracle_ms_64_wavelengths"] = c64
+        cpu = {"value": 1e3 / c64, "unit": "stop solves/s (64 wavelengths)",
                "cores": 1, "kind": "port",
-               "sample": "one stop solve (21 x 21 rays, numpy float64 oracle trace), median of 2 "
-                         "after 1 warm-up, 1 thread"}
-    forms = {"stop_solve": {"ms_per_solve": ms_solve, "solves_per_s": 1e3 / ms_solve,
-                            "field_max_deg": [float(fmax.x[0]), float(fmax.y[0])],
-                            "field_max_expected_deg": workloads.RADIUS_FIELD_NEWTONIAN}}
+               "sample": "stop solves of the same system at 64 wavelengths (21 x 21 x 64 rays, "
+                         "numpy float64 oracle trace), median of 2 after 1 warm-up, 1 thread"}
     config = {"workload": f"normalized rayfunction of the stop-test Newtonian (angular "
                           f"object field stop 0.05 deg, pupil stop 250 mm after the primary), "
                           f"{n} rays uniform in [-1, 1]^4, stop solve + trace per step",

@@ -143,12 +143,13 @@ def _shape(num_paths: int, n: int, num_surfaces: int | None = None):
 
 
 def trace_sequence(surfaces: Sequence, rays: RayVectorArray, accumulate: bool = False,
-                   precision: str = "f64") -> RayVectorArray:
-    """``propagate_rays`` / ``accumulate_rays`` over ``surfaces``."""
+                   precision: str = "f64", system=None) -> RayVectorArray:
+    """``propagate_rays`` / ``accumulate_rays`` over ``surfaces`` (or over an
+    already compiled ``system`` of them)."""
     prec = _precision(precision)
     dtype = torch.float64 if prec == _lib.F64 else torch.float32
     dev = device()
-    cs = compiled(surfaces)
+    cs = system if system is not None else compiled(surfaces)
     n = rays.num
     P = cs.num_paths
 

@@ -90,27 +90,31 @@ def anchor_surface(subsystem):
     return subsystem[-1]
 
 
-def _propagate(surfaces, position, direction, wavelength):
+def _propagate(surfaces, position, direction, wavelength, system=None):
     """Rays [3][N] through ``surfaces`` on the device (system frame in and
-    out); returns position, direction [3][N] on the host."""
+    out); returns position, direction [3][N] on the host.  One copy each
+    way."""
     dev = _trace.device()
-    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)  # noqa: E731
+    n = position.shape[-1]
+    host = np.empty((7, n))
+    host[0:3] = position
+    host[3:6] = direction
+    host[6] = wavelength
+    d = torch.as_tensor(host).to(dev)
     rays = RayVectorArray(
-        wavelength=t(wavelength),
-        position=Cartesian3dVectorArray(t(position[0]), t(position[1]), t(position[2])),
-        direction=Cartesian3dVectorArray(t(direction[0]), t(direction[1]), t(direction[2])),
+        wavelength=d[6],
+        position=Cartesian3dVectorArray(d[0], d[1], d[2]),
+        direction=Cartesian3dVectorArray(d[3], d[4], d[5]),
     )
-    out = _trace.trace_sequence(list(surfaces), rays)
+    out = _trace.trace_sequence(list(surfaces), rays, system=system)
     if out.position.x.ndim != 1:
         raise NotImplementedError(
             "the stop solve needs a single ray path between the stops "
             "(one diffraction order per grating)"
         )
-    pos = np.stack([out.position.x.cpu().numpy(), out.position.y.cpu().numpy(),
-                    out.position.z.cpu().numpy()])
-    dirs = np.stack([out.direction.x.cpu().numpy(), out.direction.y.cpu().numpy(),
-                     out.direction.z.cpu().numpy()])
-    return pos, dirs
+    back = torch.stack([out.position.x, out.position.y, out.position.z,
+                        out.direction.x, out.direction.y, out.direction.z]).cpu().numpy()
+    return back[0:3], back[3:6]
 
 
 def _wire(surface, num) -> np.ndarray:
@@ -221,13 +225,14 @@ def solve_stop_rays(system, wavelength, samples_pupil_stop: int = 21,
                 d = v
             else:
                 p = v
-            p2, d2 = _propagate(sub[1:], p, d, np.tile(wave.reshape(-1), reps))
+            p2, d2 = _propagate(sub[1:], p, d, np.tile(wave.reshape(-1), reps), forward)
             if target == "position":
                 q = _apply(s_last.transformation, p2, inverse=True)
             else:
                 q = _apply(s_last.transformation, d2, direction=True, inverse=True)
             return q[:2] - np.tile(gl.reshape(2, -1), (1, reps))
 
+        forward = _trace.compiled(list(sub[1:]))  # compiled once for every iteration
         x0 = (dirs if variable == "direction" else pos)[:2].reshape(2, -1).copy()
         root = _newton(residual, x0, dx, max_abs_error, s_first, s_last)
         z = zfunc(root)
