@@ -510,6 +510,42 @@ typedef struct optk_surrogate_io {
 int32_t optk_surrogate_infer(const optk_surrogate_desc *desc, const optk_surrogate_io *io,
                              void *hip_stream);
 
+/* The stop solve's root finding (optika/systems/_sequential.py:396-662): for
+ * each of n rays, the 2-vector x -- the free direction (x, y, with
+ * z = sqrt(1 - x^2 - y^2)) or the free position (x, y, with z the sag of
+ * surface 0 of `first` at (x, y, 0)) -- that carries the ray through every
+ * surface of `sys` onto target[i]: the position (x, y) or direction (x, y)
+ * in the frame of sys's post transformation (applied inversely, as
+ * OPTK_TRACE_APPLY_POST does).  Newton steps x -= J^-1 f with a forward
+ * difference Jacobian of step dx, per ray, until both residual components
+ * are below tol, at most max_iter steps; the reference iterates
+ * named_arrays.optimize.root_newton with named_arrays.jacobian the same way.
+ * One thread per ray, the residual and both Jacobian columns traced in
+ * registers.  x: initial guess in, root out; iterations[i]: steps taken, or
+ * max_iter + 1 when the ray did not converge.  Only the first diffraction
+ * order path is followed. */
+#define OPTK_STOP_FREE_DIRECTION 0
+#define OPTK_STOP_FREE_POSITION 1
+#define OPTK_STOP_TARGET_POSITION 0
+#define OPTK_STOP_TARGET_DIRECTION 1
+typedef struct optk_stop_args {
+  const double *position[3];    /* fixed start position (system frame) */
+  const double *direction[3];   /* fixed start direction */
+  const double *wavelength;
+  const double *target[2];
+  double *x[2];
+  int32_t *iterations;
+  double tol;
+  double dx;
+  int32_t free_component;       /* OPTK_STOP_FREE_* */
+  int32_t target_component;     /* OPTK_STOP_TARGET_* */
+  int32_t max_iter;
+  int32_t pad;
+} optk_stop_args;
+
+int32_t optk_stop_newton(const optk_system *sys, const optk_system *first,
+                         const optk_stop_args *args, int64_t n, void *hip_stream);
+
 /* The surrogate's polynomial baseline in float64 (the fit of the targets on
  * the OPTK_SURROGATE_POLY cubic monomials F of 5 normalised inputs).
  * u: [n][5] row-major; targets: [n][num_targets] (num_targets <=

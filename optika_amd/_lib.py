@@ -214,6 +214,18 @@ class SurrogateAperture(ctypes.Structure):
                 ("vy", ctypes.c_float * SURROGATE_MAX_VERTICES)]
 
 
+class StopArgs(ctypes.Structure):
+    _fields_ = [("position", c_void_p * 3), ("direction", c_void_p * 3), ("wavelength", c_void_p),
+                ("target", c_void_p * 2), ("x", c_void_p * 2), ("iterations", c_void_p),
+                ("tol", ctypes.c_double), ("dx", ctypes.c_double),
+                ("free_component", c_int32), ("target_component", c_int32),
+                ("max_iter", c_int32), ("pad", c_int32)]
+
+
+STOP_FREE_DIRECTION, STOP_FREE_POSITION = 0, 1
+STOP_TARGET_POSITION, STOP_TARGET_DIRECTION = 0, 1
+
+
 class SurrogateDesc(ctypes.Structure):
     _fields_ = [("w1", c_void_p), ("b1", c_void_p), ("w1_16", c_void_p), ("w16", c_void_p * 3),
                 ("b", c_void_p * 3),
@@ -304,6 +316,8 @@ def lib() -> ctypes.CDLL:
     L.optk_surrogate_infer.restype = c_int32
     L.optk_surrogate_infer.argtypes = [ctypes.POINTER(SurrogateDesc), ctypes.POINTER(SurrogateIO),
                                        c_void_p]
+    L.optk_stop_newton.restype = c_int32
+    L.optk_stop_newton.argtypes = [c_void_p, c_void_p, ctypes.POINTER(StopArgs), c_int64, c_void_p]
     L.optk_poly_workspace.restype = ctypes.c_size_t
     L.optk_poly_workspace.argtypes = [c_int64]
     L.optk_poly_normal.restype = c_int32
@@ -319,7 +333,7 @@ def lib() -> ctypes.CDLL:
                       ("optk_eval_args", EvalArgs), ("optk_collect_args", CollectArgs),
                       ("optk_batch_desc", BatchDesc), ("optk_tune", Tune),
                       ("optk_mlp_args", MlpArgs), ("optk_surrogate_desc", SurrogateDesc),
-                      ("optk_surrogate_io", SurrogateIO)]:
+                      ("optk_surrogate_io", SurrogateIO), ("optk_stop_args", StopArgs)]:
         size = L.optk_sizeof(name.encode())
         if size != ctypes.sizeof(cls):
             raise OptkError(f"record {name}: C size {size} != binding size {ctypes.sizeof(cls)}")

@@ -193,6 +193,87 @@ static void launch_trace(void (*k0)(TraceArgs, In, RaysOut), void (*k3)(TraceArg
                      in, acc);
 }
 
+
+// stop-solve Newton iteration, float64 form ---------------------------------
+
+struct StopParams {
+  TraceArgs T;
+  optk_surface first;  // sag of the free position's z (FREE_POSITION)
+  optk_stop_args a;
+};
+
+template <int F>
+__global__ __launch_bounds__(256) void stop_newton_f64(const StopParams P) {
+  extern __shared__ double staged[];
+  TraceSpec *s_spec = reinterpret_cast<TraceSpec *>(staged);
+  optk_surface *s_surf = reinterpret_cast<optk_surface *>(s_spec + 1);
+  stage_words(s_spec, P.T.spec, sizeof(TraceSpec));
+  stage_words(s_surf, P.T.surfaces, sizeof(optk_surface) * (size_t)P.T.num_surfaces);
+  __syncthreads();
+  SystemView sys = s_spec->sys;
+  sys.surfaces = s_surf;
+  const optk_stop_args &a = P.a;
+  const int64_t n = P.T.n;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    Ray<double> base;
+    base.px = a.position[0][i];
+    base.py = a.position[1][i];
+    base.pz = a.position[2][i];
+    base.dx = a.direction[0][i];
+    base.dy = a.direction[1][i];
+    base.dz = a.direction[2][i];
+    base.wavelength = a.wavelength[i];
+    base.intensity = 1.0;
+    base.attenuation = 0.0;
+    base.index = 1.0;
+    base.unvignetted = true;
+    const double g0 = a.target[0][i], g1 = a.target[1][i];
+    double x0 = a.x[0][i], x1 = a.x[1][i];
+    int it = 0;
+    bool converged = false;
+    for (;; ++it) {
+      // residual at x, then (unless converged) at x + dx e0 and x + dx e1
+      double f[3][2];
+      for (int e = 0; e < 3; ++e) {
+        const double y0 = e == 1 ? x0 + a.dx : x0;
+        const double y1 = e == 2 ? x1 + a.dx : x1;
+        Ray<double> r = base;
+        if (a.free_component == OPTK_STOP_FREE_DIRECTION) {
+          r.dx = y0;
+          r.dy = y1;
+          r.dz = sqrt(1.0 - (y0 * y0 + y1 * y1));
+        } else {
+          r.px = y0;
+          r.py = y1;
+          r.pz = sag_z(P.first, y0, y1, 0.0);
+        }
+        for (int k = 0; k < P.T.num_surfaces; ++k) propagate_surface<F>(s_surf[k], sys, 0, r);
+        if (P.T.apply_post) {
+          xf_inverse_pos(s_spec->post, r.px, r.py, r.pz);
+          xf_inverse_dir(s_spec->post, r.dx, r.dy, r.dz);
+        }
+        const bool on_position = a.target_component == OPTK_STOP_TARGET_POSITION;
+        f[e][0] = (on_position ? r.px : r.dx) - g0;
+        f[e][1] = (on_position ? r.py : r.dy) - g1;
+        if (e == 0) {
+          converged = fabs(f[0][0]) < a.tol && fabs(f[0][1]) < a.tol;
+          if (converged || it >= a.max_iter) break;
+        }
+      }
+      if (converged || it >= a.max_iter) break;
+      const double j00 = (f[1][0] - f[0][0]) / a.dx, j10 = (f[1][1] - f[0][1]) / a.dx;
+      const double j01 = (f[2][0] - f[0][0]) / a.dx, j11 = (f[2][1] - f[0][1]) / a.dx;
+      const double det = j00 * j11 - j01 * j10;
+      x0 -= (j11 * f[0][0] - j01 * f[0][1]) / det;
+      x1 -= (-j10 * f[0][0] + j00 * f[0][1]) / det;
+    }
+    a.x[0][i] = x0;
+    a.x[1][i] = x1;
+    a.iterations[i] = converged ? it : a.max_iter + 1;
+  }
+}
+
 // batched persistent trace, float64 form ------------------------------------
 
 template <int F> struct Tracer64 {
@@ -638,6 +719,7 @@ int64_t optk_sizeof(const char *name) {
   if (s == "optk_mlp_args") return sizeof(optk_mlp_args);
   if (s == "optk_surrogate_desc") return sizeof(optk_surrogate_desc);
   if (s == "optk_surrogate_io") return sizeof(optk_surrogate_io);
+  if (s == "optk_stop_args") return sizeof(optk_stop_args);
   return -1;
 }
 
@@ -908,6 +990,41 @@ int32_t optk_eval(const optk_system *sys, int32_t op, const optk_eval_args *args
   }
   E.out_mask = args->out_mask;
   hipLaunchKernelGGL(eval_f64, grid_for(n), dim3(256), 0, (hipStream_t)hip_stream, E);
+  OPTK_HIP_CHECK(hipGetLastError());
+  return OPTK_OK;
+}
+
+int32_t optk_stop_newton(const optk_system *sys, const optk_system *first,
+                         const optk_stop_args *args, int64_t n, void *hip_stream) {
+  if (!sys || !args) return fail(OPTK_EINVAL, "null argument");
+  if (n < 0) return fail(OPTK_EINVAL, "negative count");
+  const optk_stop_args &a = *args;
+  if (a.free_component != OPTK_STOP_FREE_DIRECTION && a.free_component != OPTK_STOP_FREE_POSITION)
+    return fail(OPTK_EINVAL, "free_component must be OPTK_STOP_FREE_DIRECTION or _POSITION");
+  if (a.target_component != OPTK_STOP_TARGET_POSITION && a.target_component != OPTK_STOP_TARGET_DIRECTION)
+    return fail(OPTK_EINVAL, "target_component must be OPTK_STOP_TARGET_POSITION or _DIRECTION");
+  if (a.free_component == OPTK_STOP_FREE_POSITION && (!first || first->num_surfaces < 1))
+    return fail(OPTK_EINVAL, "a free position needs the first surface's sag");
+  if (a.max_iter < 0 || a.max_iter > 10000) return fail(OPTK_EINVAL, "max_iter out of range");
+  if (!(a.dx > 0.0) || !(a.tol > 0.0)) return fail(OPTK_EINVAL, "dx and tol must be positive");
+  for (int k = 0; k < 3; ++k)
+    if (!a.position[k] || !a.direction[k]) return fail(OPTK_EINVAL, "missing ray array");
+  if (!a.wavelength || !a.target[0] || !a.target[1] || !a.x[0] || !a.x[1] || !a.iterations)
+    return fail(OPTK_EINVAL, "missing array");
+  if (n == 0) return OPTK_OK;
+  StopParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.T.spec = sys->spec;
+  P.T.surfaces = sys->surfaces;
+  P.T.num_surfaces = sys->num_surfaces;
+  P.T.apply_post = 1;
+  P.T.n = n;
+  if (first && first->num_surfaces > 0) P.first = first->host_surfaces[0];
+  P.a = a;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(stop_newton_f64<FEAT_ALL>, dim3((unsigned)blocks), dim3(256),
+                     staged_bytes(sys->num_surfaces), (hipStream_t)hip_stream, P);
   OPTK_HIP_CHECK(hipGetLastError());
   return OPTK_OK;
 }

@@ -14,10 +14,11 @@ Mirrors ``optika/systems/_sequential.py:357-789``:
 * ``field_pupil`` / ``denormalize`` -- ``field_min`` .. ``pupil_max`` and
   ``_denormalize_grid`` (``:724-789``).
 
-Every ray propagation -- each Newton iteration's residual and Jacobian
-columns as one batch, and the walk back to the object -- runs through the
-fused trace on the device (``optk_trace``).  The host keeps the outlines, the
-seeds and the 2 x 2 Newton updates (float64), as the reference does.
+The Newton iteration runs on the device in one launch per stop pair
+(``optk_stop_newton``): one thread per outline ray iterates its own root,
+tracing the residual and both Jacobian columns through the surfaces in
+registers; the walk back to the object is one ``optk_trace``.  The host
+keeps the outlines and the seeds (float64), as the reference does.
 
 The reference's root finder is ``named_arrays.optimize.root_newton`` with
 ``named_arrays.jacobian`` (``named_arrays~=2.1``, not vendored): Newton steps
@@ -215,26 +216,9 @@ def solve_stop_rays(system, wavelength, samples_pupil_stop: int = 21,
                 return np.sqrt(1 - (a[0] ** 2 + a[1] ** 2))
             return _sag(s_first, np.stack([a[0], a[1], np.zeros(a.shape[1:])]))
 
-        def residual(a, reps):
-            """f(a) for reps stacked copies of the grid (a: [2][reps * N])."""
-            n = a.shape[1]
-            p = np.tile(pos.reshape(3, -1), (1, reps))
-            d = np.tile(dirs.reshape(3, -1), (1, reps))
-            v = np.stack([a[0], a[1], zfunc(a)])
-            if variable == "direction":
-                d = v
-            else:
-                p = v
-            p2, d2 = _propagate(sub[1:], p, d, np.tile(wave.reshape(-1), reps), forward)
-            if target == "position":
-                q = _apply(s_last.transformation, p2, inverse=True)
-            else:
-                q = _apply(s_last.transformation, d2, direction=True, inverse=True)
-            return q[:2] - np.tile(gl.reshape(2, -1), (1, reps))
-
-        forward = _trace.compiled(list(sub[1:]))  # compiled once for every iteration
-        x0 = (dirs if variable == "direction" else pos)[:2].reshape(2, -1).copy()
-        root = _newton(residual, x0, dx, max_abs_error, s_first, s_last)
+        x0 = (dirs if variable == "direction" else pos)[:2].reshape(2, -1)
+        root = _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, x0,
+                              dx, max_abs_error)
         z = zfunc(root)
         v = np.stack([root[0], root[1], z]).reshape((3,) + shape)
         if variable == "direction":
@@ -246,29 +230,57 @@ def solve_stop_rays(system, wavelength, samples_pupil_stop: int = 21,
     return result
 
 
-def _newton(residual, x, dx, tol, s_first, s_last):
-    """Newton's method on independent 2-vectors x [2][N]: the residual and
-    both forward-difference Jacobian columns in one device batch."""
-    n = x.shape[1]
-    for _ in range(MAX_ITERATIONS):
-        batch = np.concatenate([x, x + np.array([[dx], [0.0]]), x + np.array([[0.0], [dx]])], axis=1)
-        f = residual(batch, 3)
-        f0, fx, fy = f[:, :n], f[:, n:2 * n], f[:, 2 * n:]
-        if np.all(np.abs(f0) < tol):
-            return x
-        j00, j10 = (fx[0] - f0[0]) / dx, (fx[1] - f0[1]) / dx
-        j01, j11 = (fy[0] - f0[0]) / dx, (fy[1] - f0[1]) / dx
-        det = j00 * j11 - j01 * j10
-        step0 = (j11 * f0[0] - j01 * f0[1]) / det
-        step1 = (-j10 * f0[0] + j00 * f0[1]) / det
-        x = x - np.stack([step0, step1])
-    raise ValueError(
-        f"Could not solve for the rays connecting the stop surfaces {s_first.name!r} and "
-        f"{s_last.name!r}. If the field stop is only partially reachable at a single "
-        f"wavelength (e.g. a spectrograph sensor), consider marking the object surface, "
-        f"with an angular (dimensionless, sine of the half-angle) aperture, as the field "
-        f"stop instead."
-    )
+def _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, x0, dx, tol):
+    """Newton's method for every outline ray in one device launch
+    (``optk_stop_newton``): each thread iterates its own ray, tracing the
+    residual and both forward-difference Jacobian columns through
+    ``sub[1:]`` into the last stop's frame."""
+    import ctypes
+
+    from . import _lib
+    from .surfaces import Surface
+
+    dev = _trace.device()
+    forward = _trace.compiled(list(sub[1:]), post=s_last.transformation)
+    if forward.num_paths != 1:
+        raise NotImplementedError(
+            "the stop solve needs a single ray path between the stops "
+            "(one diffraction order per grating)"
+        )
+    first = _trace.compiled([Surface(sag=s_first.sag)]) if variable == "position" else None
+    n = x0.shape[1]
+    host = np.empty((10, n))
+    host[0:3] = pos.reshape(3, -1)
+    host[3:6] = dirs.reshape(3, -1)
+    host[6] = wave.reshape(-1)
+    host[7:9] = gl.reshape(2, -1)
+    d = torch.as_tensor(host).to(dev)
+    x = torch.as_tensor(np.ascontiguousarray(x0)).to(dev)
+    iters = torch.empty(n, dtype=torch.int32, device=dev)
+    a = _lib.StopArgs()
+    for k in range(3):
+        a.position[k] = d[k].data_ptr()
+        a.direction[k] = d[3 + k].data_ptr()
+    a.wavelength = d[6].data_ptr()
+    a.target[0], a.target[1] = d[7].data_ptr(), d[8].data_ptr()
+    a.x[0], a.x[1] = x[0].data_ptr(), x[1].data_ptr()
+    a.iterations = iters.data_ptr()
+    a.tol, a.dx = float(tol), float(dx)
+    a.free_component = _lib.STOP_FREE_DIRECTION if variable == "direction" else _lib.STOP_FREE_POSITION
+    a.target_component = (_lib.STOP_TARGET_POSITION if target == "position"
+                          else _lib.STOP_TARGET_DIRECTION)
+    a.max_iter = MAX_ITERATIONS
+    _lib.check(_lib.lib().optk_stop_newton(forward.handle, first.handle if first else None,
+                                           ctypes.byref(a), n, _trace.stream_handle()))
+    if int(iters.max().item()) > MAX_ITERATIONS:
+        raise ValueError(
+            f"Could not solve for the rays connecting the stop surfaces {s_first.name!r} and "
+            f"{s_last.name!r}. If the field stop is only partially reachable at a single "
+            f"wavelength (e.g. a spectrograph sensor), consider marking the object surface, "
+            f"with an angular (dimensionless, sine of the half-angle) aperture, as the field "
+            f"stop instead."
+        )
+    return x.cpu().numpy()
 
 
 def rayfunction_stops(system, wavelength, samples_pupil_stop: int = 21,

@@ -97,3 +97,14 @@ def test_polynomial_arguments_are_rejected_without_a_device():
     assert b"workspace" in L.optk_last_error()
     assert L.optk_poly_workspace(10) >= 8 * 56 * 67
     assert np.uint32 == codes.dtype.type
+
+
+def test_stop_newton_arguments_are_rejected_without_a_device():
+    from optika_amd import _lib
+
+    if not _lib.library_path().exists():
+        pytest.skip("liboptk.so not built")
+    L = _lib.lib()
+    a = _lib.StopArgs()
+    assert L.optk_stop_newton(None, None, ctypes.byref(a), 1, None) == -1
+    assert b"null" in L.optk_last_error()
