@@ -546,6 +546,29 @@ typedef struct optk_stop_args {
 int32_t optk_stop_newton(const optk_system *sys, const optk_system *first,
                          const optk_stop_args *args, int64_t n, void *hip_stream);
 
+/* Reductions of traced rays over the pupil (optika/systems/_sequential.py:
+ * 1208-1512: distortion, vignetting, area_effective): the rays of segment s
+ * are rays [s * per_segment, (s + 1) * per_segment).  Per segment:
+ *   count       number of unvignetted rays
+ *   sum_x/_y    position sums over the unvignetted rays
+ *   all_x/_y    position sums over every ray
+ *   throughput  sum of weight[p] * intensity over the unvignetted rays
+ *               (weight: one per pupil sample p, NULL = 1; intensity NULL = 1)
+ * Sums run in a fixed order (deterministic).  Any output may be NULL. */
+typedef struct optk_pupil_args {
+  const double *x, *y;
+  const double *intensity;
+  const uint8_t *unvignetted;
+  const double *weight;
+  int64_t per_segment;
+  double *count;
+  double *sum_x, *sum_y;
+  double *all_x, *all_y;
+  double *throughput;
+} optk_pupil_args;
+
+int32_t optk_pupil_reduce(const optk_pupil_args *args, int64_t segments, void *hip_stream);
+
 /* The surrogate's polynomial baseline in float64 (the fit of the targets on
  * the OPTK_SURROGATE_POLY cubic monomials F of 5 normalised inputs).
  * u: [n][5] row-major; targets: [n][num_targets] (num_targets <=

@@ -214,6 +214,13 @@ class SurrogateAperture(ctypes.Structure):
                 ("vy", ctypes.c_float * SURROGATE_MAX_VERTICES)]
 
 
+class PupilArgs(ctypes.Structure):
+    _fields_ = [("x", c_void_p), ("y", c_void_p), ("intensity", c_void_p), ("unvignetted", c_void_p),
+                ("weight", c_void_p), ("per_segment", c_int64), ("count", c_void_p),
+                ("sum_x", c_void_p), ("sum_y", c_void_p), ("all_x", c_void_p), ("all_y", c_void_p),
+                ("throughput", c_void_p)]
+
+
 class StopArgs(ctypes.Structure):
     _fields_ = [("position", c_void_p * 3), ("direction", c_void_p * 3), ("wavelength", c_void_p),
                 ("target", c_void_p * 2), ("x", c_void_p * 2), ("iterations", c_void_p),
@@ -316,6 +323,8 @@ def lib() -> ctypes.CDLL:
     L.optk_surrogate_infer.restype = c_int32
     L.optk_surrogate_infer.argtypes = [ctypes.POINTER(SurrogateDesc), ctypes.POINTER(SurrogateIO),
                                        c_void_p]
+    L.optk_pupil_reduce.restype = c_int32
+    L.optk_pupil_reduce.argtypes = [ctypes.POINTER(PupilArgs), c_int64, c_void_p]
     L.optk_stop_newton.restype = c_int32
     L.optk_stop_newton.argtypes = [c_void_p, c_void_p, ctypes.POINTER(StopArgs), c_int64, c_void_p]
     L.optk_poly_workspace.restype = ctypes.c_size_t
@@ -333,7 +342,8 @@ def lib() -> ctypes.CDLL:
                       ("optk_eval_args", EvalArgs), ("optk_collect_args", CollectArgs),
                       ("optk_batch_desc", BatchDesc), ("optk_tune", Tune),
                       ("optk_mlp_args", MlpArgs), ("optk_surrogate_desc", SurrogateDesc),
-                      ("optk_surrogate_io", SurrogateIO), ("optk_stop_args", StopArgs)]:
+                      ("optk_surrogate_io", SurrogateIO), ("optk_stop_args", StopArgs),
+                      ("optk_pupil_args", PupilArgs)]:
         size = L.optk_sizeof(name.encode())
         if size != ctypes.sizeof(cls):
             raise OptkError(f"record {name}: C size {size} != binding size {ctypes.sizeof(cls)}")

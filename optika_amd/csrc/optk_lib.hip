@@ -274,6 +274,42 @@ __global__ __launch_bounds__(256) void stop_newton_f64(const StopParams P) {
   }
 }
 
+
+// pupil reductions: one wave per segment, lanes stride the pupil samples,
+// then a fixed butterfly over the wave ---------------------------------------
+
+__global__ __launch_bounds__(256) void pupil_reduce_kernel(const optk_pupil_args a, int64_t segments) {
+  const int lane = threadIdx.x & 63;
+  const int64_t seg = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (seg >= segments) return;  // whole waves leave together
+  const int64_t base = seg * a.per_segment;
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // count, sx, sy, ax, ay, throughput
+  for (int64_t p = lane; p < a.per_segment; p += 64) {
+    const int64_t i = base + p;
+    const double x = a.x ? a.x[i] : 0.0, y = a.y ? a.y[i] : 0.0;
+    v[3] += x;
+    v[4] += y;
+    if (a.unvignetted[i]) {
+      const double w = (a.weight ? a.weight[p] : 1.0) * (a.intensity ? a.intensity[i] : 1.0);
+      v[0] += 1.0;
+      v[1] += x;
+      v[2] += y;
+      v[5] += w;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  if (lane == 0) {
+    if (a.count) a.count[seg] = v[0];
+    if (a.sum_x) a.sum_x[seg] = v[1];
+    if (a.sum_y) a.sum_y[seg] = v[2];
+    if (a.all_x) a.all_x[seg] = v[3];
+    if (a.all_y) a.all_y[seg] = v[4];
+    if (a.throughput) a.throughput[seg] = v[5];
+  }
+}
+
 // batched persistent trace, float64 form ------------------------------------
 
 template <int F> struct Tracer64 {
@@ -720,6 +756,7 @@ int64_t optk_sizeof(const char *name) {
   if (s == "optk_surrogate_desc") return sizeof(optk_surrogate_desc);
   if (s == "optk_surrogate_io") return sizeof(optk_surrogate_io);
   if (s == "optk_stop_args") return sizeof(optk_stop_args);
+  if (s == "optk_pupil_args") return sizeof(optk_pupil_args);
   return -1;
 }
 
@@ -1025,6 +1062,22 @@ int32_t optk_stop_newton(const optk_system *sys, const optk_system *first,
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(stop_newton_f64<FEAT_ALL>, dim3((unsigned)blocks), dim3(256),
                      staged_bytes(sys->num_surfaces), (hipStream_t)hip_stream, P);
+  OPTK_HIP_CHECK(hipGetLastError());
+  return OPTK_OK;
+}
+
+int32_t optk_pupil_reduce(const optk_pupil_args *args, int64_t segments, void *hip_stream) {
+  if (!args) return fail(OPTK_EINVAL, "null argument");
+  if (segments < 0 || args->per_segment < 0) return fail(OPTK_EINVAL, "negative count");
+  if (!args->unvignetted) return fail(OPTK_EINVAL, "missing unvignetted mask");
+  if ((args->sum_x || args->all_x) && !args->x) return fail(OPTK_EINVAL, "missing x");
+  if ((args->sum_y || args->all_y) && !args->y) return fail(OPTK_EINVAL, "missing y");
+  if (segments == 0) return OPTK_OK;
+  const int64_t threads = segments * 64;
+  const int64_t blocks = (threads + 255) / 256;
+  if (blocks > INT32_MAX) return fail(OPTK_EINVAL, "too many segments");
+  hipLaunchKernelGGL(pupil_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                     (hipStream_t)hip_stream, *args, segments);
   OPTK_HIP_CHECK(hipGetLastError());
   return OPTK_OK;
 }

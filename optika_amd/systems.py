@@ -194,6 +194,103 @@ class AbstractSequentialSystem:
             precision=precision,
         )
 
+    # -- pupil reductions (_sequential.py:1208-1512) -----------------------------
+    def _grid_defaults(self, wavelength, field, pupil):
+        g = self.grid_input if self.grid_input is not None else ObjectVectorArray()
+        return (g.wavelength if wavelength is None else wavelength,
+                g.field if field is None else field,
+                g.pupil if pupil is None else pupil)
+
+    def _pupil_statistics(self, wavelength, field, pupil, normalized_field, normalized_pupil):
+        from . import pupil as _pupil
+
+        wavelength, field, pupil = self._grid_defaults(wavelength, field, pupil)
+        rays, shape = _pupil.trace_outer(self, wavelength, field, pupil, normalized_field,
+                                         normalized_pupil)
+        stats = _pupil.reduce(rays.outputs, shape)
+        if stats["count"].shape[0] != 1:
+            raise NotImplementedError("pupil models of a system with several diffraction orders")
+        W, F, P = shape
+        fx = rays.inputs.field.x.reshape(W, F, P)[:, :, 0].cpu().numpy()
+        fy = rays.inputs.field.y.reshape(W, F, P)[:, :, 0].cpu().numpy()
+        wl = rays.inputs.wavelength.reshape(W, F, P)[:, :, 0].cpu().numpy()
+        return {k: v[0] for k, v in stats.items()}, wl, fx, fy, P
+
+    def vignetting(self, wavelength=None, field=None, pupil=None, normalized_field: bool = True,
+                   normalized_pupil: bool = True, degree: int = 2):
+        """Polynomial vignetting model (``_sequential.py:1282-1367``): the
+        fraction of unvignetted rays over the pupil at each wavelength and
+        field point, normalized to a mean of one over the field, fitted where
+        any ray gets through.  ``wavelength`` [W], ``field`` [F] points,
+        ``pupil`` [P] points; the W x F x P grid is traced and reduced on the
+        device."""
+        from .radiometry import PolynomialVignettingModel
+
+        st, wl, fx, fy, P = self._pupil_statistics(wavelength, field, pupil, normalized_field,
+                                                   normalized_pupil)
+        where = st["count"] > 0
+        illumination = st["count"] / P
+        mean = np.array([illumination[w][where[w]].mean() if where[w].any() else np.nan
+                         for w in range(illumination.shape[0])])
+        illumination = illumination / mean[:, None]
+        return PolynomialVignettingModel(wavelength=wl, field_x=fx, field_y=fy,
+                                         illumination=illumination, degree=degree, where=where)
+
+    def distortion(self, wavelength=None, field=None, pupil=None, normalized_field: bool = True,
+                   normalized_pupil: bool = True, degree: int = 2):
+        """Polynomial distortion model (``_sequential.py:1208-1280``): the
+        mean sensor position over the unvignetted pupil (over every ray
+        where none gets through) at each wavelength and field point."""
+        from .distortion import PolynomialDistortionModel
+
+        st, wl, fx, fy, P = self._pupil_statistics(wavelength, field, pupil, normalized_field,
+                                                   normalized_pupil)
+        where = st["count"] > 0
+        n = np.where(where, st["count"], P)
+        sx = np.where(where, st["sum_x"], st["all_x"]) / n
+        sy = np.where(where, st["sum_y"], st["all_y"]) / n
+        return PolynomialDistortionModel(wavelength=wl, field_x=fx, field_y=fy, sensor_x=sx,
+                                         sensor_y=sy, degree=degree, where=where)
+
+    def area_effective(self, wavelength=None, field=None, pupil=None,
+                       normalized_field: bool = True, normalized_pupil: bool = True,
+                       seed: int = 0):
+        """Effective area versus wavelength (``_sequential.py:1369-1512``).
+        ``pupil`` holds the grid's vertex coordinates along x and along y
+        (default 11 each over the normalized pupil); rays are traced at one
+        random point of every cell with the cell's area as intensity, their
+        output intensity summed over the unvignetted rays and averaged over
+        the field points.  mm^2 for an object at infinity, else deg^2."""
+        from . import pupil as _pupil
+        from . import stops
+        from .radiometry import InterpolatedEffectiveAreaModel
+
+        wavelength, field, _ = self._grid_defaults(wavelength, field, None)
+        wl = np.atleast_1d(np.asarray(wavelength.detach().cpu().numpy() if hasattr(
+            wavelength, "detach") else wavelength, np.float64)).reshape(-1)
+        if pupil is None:
+            vx = vy = np.linspace(-1.0, 1.0, 11)
+        else:
+            h = lambda v: np.asarray(v.detach().cpu().numpy() if hasattr(v, "detach") else v,  # noqa: E731
+                                     np.float64).reshape(-1)
+            vx, vy = h(pupil.x), h(pupil.y)
+        if normalized_pupil:
+            uniq, inv = np.unique(wl, return_inverse=True)
+            _, pup = stops.field_pupil(self, stops.rayfunction_stops(self, uniq))
+            lo, hi = pup.min(axis=(1, 2)), pup.max(axis=(1, 2))
+            vxw = (hi[0] - lo[0])[inv][:, None] * (vx + 1) / 2 + lo[0][inv][:, None]
+            vyw = (hi[1] - lo[1])[inv][:, None] * (vy + 1) / 2 + lo[1][inv][:, None]
+        else:
+            vxw = np.broadcast_to(vx, (wl.size, vx.size))
+            vyw = np.broadcast_to(vy, (wl.size, vy.size))
+        area, cx, cy = _pupil.cell_grid(vxw, vyw, seed)
+        rays, shape = _pupil.trace_outer(self, wl, field, Cartesian2dVectorArray(cx, cy),
+                                         normalized_field, False, intensity=area)
+        st = _pupil.reduce(rays.outputs, shape)
+        if st["throughput"].shape[0] != 1:
+            raise NotImplementedError("effective area of a system with several diffraction orders")
+        return InterpolatedEffectiveAreaModel(wavelength=wl, area=st["throughput"][0].mean(axis=1))
+
     def _input_grid(self, wavelength, field, pupil, normalized_field, normalized_pupil):
         """The physical input grid: with no wavelength, field or pupil given,
         ``grid_input`` read as normalized whatever the flags

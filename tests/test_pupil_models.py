@@ -1,0 +1,58 @@
+"""Host-side parts of the pupil reductions (CPU only): the polynomial fit
+recovers a polynomial of its degree exactly, the cells of a vertex grid
+tile it, and the effective-area model interpolates linearly."""
+
+import numpy as np
+
+from optika_amd import pupil, radiometry
+from optika_amd.distortion import PolynomialDistortionModel
+from optika_amd.vectors import Cartesian2dVectorArray
+
+
+def test_polynomial_fit_recovers_its_degree():
+    rng = np.random.default_rng(1)
+    w, x, y = rng.uniform(50, 70, 300), rng.uniform(-1, 1, 300), rng.uniform(-2, 2, 300)
+    f = 0.3 + 0.01 * w - 2 * x * y + 0.5 * y ** 2 + 1e-4 * w * w
+    fit = radiometry.PolynomialFit([w, x, y], [f], degree=2)
+    w2, x2, y2 = rng.uniform(50, 70, 50), rng.uniform(-1, 1, 50), rng.uniform(-2, 2, 50)
+    g = 0.3 + 0.01 * w2 - 2 * x2 * y2 + 0.5 * y2 ** 2 + 1e-4 * w2 * w2
+    np.testing.assert_allclose(fit(w2, x2, y2)[0], g, rtol=0, atol=1e-10)
+    # points outside `where` do not enter the fit
+    bad = f.copy()
+    bad[:10] = 1e6
+    fit2 = radiometry.PolynomialFit([w, x, y], [bad], degree=2, where=np.arange(300) >= 10)
+    np.testing.assert_allclose(fit2(w2, x2, y2)[0], g, rtol=0, atol=1e-10)
+
+
+def test_distortion_model_inverts():
+    rng = np.random.default_rng(2)
+    w = rng.uniform(58, 63, 400)
+    fx, fy = rng.uniform(-0.05, 0.05, 400), rng.uniform(-0.05, 0.05, 400)
+    sx = 100 * fx + 3 * fy + 0.01 * (w - 60)
+    sy = -2 * fx + 90 * fy
+    m = PolynomialDistortionModel(wavelength=w, field_x=fx, field_y=fy, sensor_x=sx, sensor_y=sy,
+                                  degree=1)
+    d = m.distort(w, Cartesian2dVectorArray(fx, fy))
+    np.testing.assert_allclose(d.x, sx, atol=1e-10)
+    u = m.undistort(w, Cartesian2dVectorArray(sx, sy))
+    np.testing.assert_allclose(u.x, fx, atol=1e-12)
+    np.testing.assert_allclose(u.y, fy, atol=1e-12)
+
+
+def test_cells_tile_the_vertex_grid():
+    vx = np.array([[-2.0, -1.0, 0.5, 2.0], [-4.0, -2.0, 1.0, 4.0]])
+    vy = np.array([[-1.0, 0.0, 1.0], [-3.0, 0.0, 3.0]])
+    area, x, y = pupil.cell_grid(vx, vy, seed=3)
+    assert area.shape == x.shape == y.shape == (2, 6)
+    np.testing.assert_allclose(area.sum(axis=1), [4.0 * 2.0, 8.0 * 6.0])
+    cx = x.reshape(2, 3, 2)
+    cy = y.reshape(2, 3, 2)
+    for k in range(2):
+        assert np.all((cx[k] >= vx[k, :-1, None]) & (cx[k] <= vx[k, 1:, None]))
+        assert np.all((cy[k] >= vy[k, None, :-1]) & (cy[k] <= vy[k, None, 1:]))
+
+
+def test_interpolated_effective_area():
+    m = radiometry.InterpolatedEffectiveAreaModel(wavelength=np.array([400.0, 500.0, 700.0]),
+                                                  area=np.array([1.0, 3.0, 2.0]))
+    np.testing.assert_allclose(m([400.0, 450.0, 600.0, 800.0]), [1.0, 2.0, 2.5, 2.0])
