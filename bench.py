@@ -563,6 +563,31 @@ def run_stops(args):
                             "ms_per_solve_64_wavelengths": ms_solve64,
                             "field_max_deg": [float(fmax.x[0]), float(fmax.y[0])],
                             "field_max_expected_deg": workloads.RADIUS_FIELD_NEWTONIAN}}
+    # pupil reductions: the reduction kernel over the step's rays (segments
+    # of 1024 pupil samples), and a whole area_effective call
+    from optika_amd import pupil as dpupil
+
+    out = step().outputs
+    seg_p = 1024
+    segs = n // seg_p
+    shape = (1, segs, seg_p)
+    ms_reduce, _ = time_calls(lambda: dpupil.reduce(out, shape), args.steps, args.warmup, stream)
+    vtx = np.linspace(-1.0, 1.0, 41)
+    fpts = Cartesian2dVectorArray(np.array([0.0, 0.5, -0.5]), np.array([0.0, 0.0, 0.5]))
+
+    def area():
+        return system.area_effective(wavelength=np.array([500.0, 600.0]), field=fpts,
+                                     pupil=Cartesian2dVectorArray(vtx, vtx), seed=1)
+
+    ms_area, _ = time_calls(area, max(2, args.steps // 4), 1, stream)
+    forms["pupil"] = {
+        "reduce_ms": ms_reduce, "reduce_rays": n, "reduce_segments": segs,
+        "reduce_algorithmic_gbs": n * 25 / (ms_reduce * 1e-3) / 1e9,
+        "area_effective_ms": ms_area,
+        "area_effective_mm2": [float(a) for a in area()([500.0, 600.0])],
+        "area_effective_expected_mm2": float(np.pi * (10.0 / 0.75) ** 2),
+        "note": "reduce_ms includes the host copy of the per-segment results",
+    }
     if not args.no_cpu:
         from oracle import stops as ostops
 

@@ -342,6 +342,28 @@ typedef struct optk_collect_args {
 
 int32_t optk_collect(const optk_collect_args *args, int64_t n, void *hip_stream);
 
+/* Exposure of a binned image by an ideal sensor material (quantum yield 1):
+ * optika/sensors/_sensors.py:173-252 (expose) with
+ * optika/sensors/materials/_materials.py:1576-1601 (IdealSensorMaterial.signal).
+ * Per pixel: expected photons = scale * photons[i] (scale: the exposure
+ * time); with noise, a Poisson draw of that mean plus zero-mean Gaussian
+ * read noise of rms read_noise; without, the expectation itself.  Random
+ * numbers are Philox4x32-10 (Salmon et al. 2011) keyed by seed and counted
+ * by pixel, so a call is reproducible and independent of the launch shape.
+ * Poisson: multiplication of uniforms below a mean of 10, Hormann's
+ * transformed rejection (PTRS, 1993) above. */
+typedef struct optk_expose_args {
+  const double *photons;
+  double *electrons;
+  double scale;
+  double read_noise;
+  uint64_t seed;
+  int32_t noise;
+  int32_t pad;
+} optk_expose_args;
+
+int32_t optk_expose(const optk_expose_args *args, int64_t n, void *hip_stream);
+
 /* Batched trace: many instances of one system layout (the same surface
  * kinds, layer stacks and tables; different parameters: tolerance
  * perturbations), each with its own number of rays, in ONE persistent

@@ -21,7 +21,7 @@ import torch
 from . import _lib, _trace
 from .vectors import as_vector2
 
-__all__ = ["collect", "histogram"]
+__all__ = ["collect", "histogram", "expose"]
 
 
 def histogram(rays, wavelength_edges, x_range, y_range, num_x, num_y, where=None):
@@ -82,3 +82,21 @@ def collect(sensor, rays, wavelength, where=True):
     cosine = torch.where(image > 0, moment / torch.where(image > 0, image, 1.0),
                          torch.ones_like(image))
     return image, cosine
+
+
+def expose(image, scale: float, read_noise: float = 0.0, noise: bool = True, seed: int = 0):
+    """Electrons of an ideal sensor material (``optk_expose``): a Poisson
+    draw of ``scale * image`` per pixel plus Gaussian read noise, or the
+    expectation without noise.  ``image``: expected photons per unit time
+    per pixel (float64 device tensor)."""
+    dev = _trace.device()
+    img = image if isinstance(image, torch.Tensor) else torch.as_tensor(np.asarray(image))
+    img = img.to(device=dev, dtype=torch.float64).contiguous()
+    out = torch.empty_like(img)
+    a = _lib.ExposeArgs()
+    a.photons, a.electrons = img.data_ptr(), out.data_ptr()
+    a.scale, a.read_noise = float(scale), float(read_noise)
+    a.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    a.noise = 1 if noise else 0
+    _lib.check(_lib.lib().optk_expose(ctypes.byref(a), img.numel(), _trace.stream_handle()))
+    return out

@@ -214,6 +214,12 @@ class SurrogateAperture(ctypes.Structure):
                 ("vy", ctypes.c_float * SURROGATE_MAX_VERTICES)]
 
 
+class ExposeArgs(ctypes.Structure):
+    _fields_ = [("photons", c_void_p), ("electrons", c_void_p), ("scale", ctypes.c_double),
+                ("read_noise", ctypes.c_double), ("seed", ctypes.c_uint64), ("noise", c_int32),
+                ("pad", c_int32)]
+
+
 class PupilArgs(ctypes.Structure):
     _fields_ = [("x", c_void_p), ("y", c_void_p), ("intensity", c_void_p), ("unvignetted", c_void_p),
                 ("weight", c_void_p), ("per_segment", c_int64), ("count", c_void_p),
@@ -323,6 +329,8 @@ def lib() -> ctypes.CDLL:
     L.optk_surrogate_infer.restype = c_int32
     L.optk_surrogate_infer.argtypes = [ctypes.POINTER(SurrogateDesc), ctypes.POINTER(SurrogateIO),
                                        c_void_p]
+    L.optk_expose.restype = c_int32
+    L.optk_expose.argtypes = [ctypes.POINTER(ExposeArgs), c_int64, c_void_p]
     L.optk_pupil_reduce.restype = c_int32
     L.optk_pupil_reduce.argtypes = [ctypes.POINTER(PupilArgs), c_int64, c_void_p]
     L.optk_stop_newton.restype = c_int32
@@ -343,7 +351,7 @@ def lib() -> ctypes.CDLL:
                       ("optk_batch_desc", BatchDesc), ("optk_tune", Tune),
                       ("optk_mlp_args", MlpArgs), ("optk_surrogate_desc", SurrogateDesc),
                       ("optk_surrogate_io", SurrogateIO), ("optk_stop_args", StopArgs),
-                      ("optk_pupil_args", PupilArgs)]:
+                      ("optk_pupil_args", PupilArgs), ("optk_expose_args", ExposeArgs)]:
         size = L.optk_sizeof(name.encode())
         if size != ctypes.sizeof(cls):
             raise OptkError(f"record {name}: C size {size} != binding size {ctypes.sizeof(cls)}")

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "optk_batch.h"
+#include "optk_random.h"
 
 using namespace optk;
 
@@ -307,6 +308,26 @@ __global__ __launch_bounds__(256) void pupil_reduce_kernel(const optk_pupil_args
     if (a.all_x) a.all_x[seg] = v[3];
     if (a.all_y) a.all_y[seg] = v[4];
     if (a.throughput) a.throughput[seg] = v[5];
+  }
+}
+
+
+// detector exposure (random numbers and draws in optk_random.h) -----------
+
+__global__ __launch_bounds__(256) void expose_kernel(const optk_expose_args a, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double mean = a.scale * a.photons[i];
+    if (!a.noise) {
+      a.electrons[i] = mean;
+      continue;
+    }
+    UniformStream rng;
+    rng.pixel = (uint64_t)i;
+    rng.seed = a.seed;
+    double e = poisson_draw(mean, rng);
+    if (a.read_noise > 0.0) e += a.read_noise * gaussian_draw(rng);
+    a.electrons[i] = e;
   }
 }
 
@@ -757,6 +778,7 @@ int64_t optk_sizeof(const char *name) {
   if (s == "optk_surrogate_io") return sizeof(optk_surrogate_io);
   if (s == "optk_stop_args") return sizeof(optk_stop_args);
   if (s == "optk_pupil_args") return sizeof(optk_pupil_args);
+  if (s == "optk_expose_args") return sizeof(optk_expose_args);
   return -1;
 }
 
@@ -1078,6 +1100,18 @@ int32_t optk_pupil_reduce(const optk_pupil_args *args, int64_t segments, void *h
   if (blocks > INT32_MAX) return fail(OPTK_EINVAL, "too many segments");
   hipLaunchKernelGGL(pupil_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0,
                      (hipStream_t)hip_stream, *args, segments);
+  OPTK_HIP_CHECK(hipGetLastError());
+  return OPTK_OK;
+}
+
+int32_t optk_expose(const optk_expose_args *args, int64_t n, void *hip_stream) {
+  if (!args) return fail(OPTK_EINVAL, "null argument");
+  if (n < 0) return fail(OPTK_EINVAL, "negative count");
+  if (!args->photons || !args->electrons) return fail(OPTK_EINVAL, "missing image");
+  if (!(args->scale >= 0.0) || !(args->read_noise >= 0.0))
+    return fail(OPTK_EINVAL, "scale and read_noise must be non-negative");
+  if (n == 0) return OPTK_OK;
+  hipLaunchKernelGGL(expose_kernel, grid_for(n), dim3(256), 0, (hipStream_t)hip_stream, *args, n);
   OPTK_HIP_CHECK(hipGetLastError());
   return OPTK_OK;
 }

@@ -60,6 +60,31 @@ class AbstractImagingSensor(AbstractSurface):
 
         return _collect.collect(self, rays, wavelength, where=where)
 
+    def expose(self, image, direction=1, timedelta=None, noise: bool = True, seed: int = 0):
+        """Electrons measured from a per-pixel photon-flux image
+        (``_sensors.py:173-252``): the flux times the exposure time
+        (``timedelta``, default ``timedelta_exposure``), converted by the
+        sensor material -- the ideal material's quantum yield is one and its
+        noise is Poisson shot noise (``sensors/materials/_materials.py:
+        1576-1601``) -- plus zero-mean Gaussian read noise of rms
+        ``read_noise``.  Without ``noise`` the expectation.  The random
+        numbers are keyed by ``seed`` and the pixel index (the reference's
+        ``na.random`` stream is not reproduced).  ``direction`` (the refracted
+        cosine) does not change an ideal material's signal."""
+        from . import _collect
+
+        if not isinstance(self.material, IdealSensorMaterial):
+            raise NotImplementedError("exposure is built for the ideal sensor material only")
+        t = self.timedelta_exposure if timedelta is None else timedelta
+        return _collect.expose(image, float(t), float(self.read_noise), noise, seed)
+
+    def measure(self, rays, wavelength, where=True, timedelta=None, noise: bool = True,
+                seed: int = 0):
+        """``collect`` then ``expose`` (``_sensors.py:374-428``): electrons
+        per (wavelength bin, pixel)."""
+        image, direction = self.collect(rays, wavelength, where=where)
+        return self.expose(image, direction, timedelta=timedelta, noise=noise, seed=seed)
+
 
 @dataclasses.dataclass(eq=False)
 class ImagingSensor(AbstractImagingSensor):

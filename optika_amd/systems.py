@@ -291,6 +291,86 @@ class AbstractSequentialSystem:
             raise NotImplementedError("effective area of a system with several diffraction orders")
         return InterpolatedEffectiveAreaModel(wavelength=wl, area=st["throughput"][0].mean(axis=1))
 
+    # -- forward model (_sequential.py:1002-1206) --------------------------------
+    def _extents(self, wavelength):
+        """Per wavelength: (field lo, field span, pupil lo, pupil span), each
+        [2][W], from the stop solve."""
+        from . import stops
+
+        uniq, inv = np.unique(wavelength, return_inverse=True)
+        fld, pup = stops.field_pupil(self, stops.rayfunction_stops(self, uniq))
+        ext = []
+        for v in (fld, pup):
+            lo, hi = v.min(axis=(1, 2)), v.max(axis=(1, 2))
+            ext += [lo[:, inv], (hi - lo)[:, inv]]
+        return ext
+
+    def image(self, wavelength, field_x, field_y, radiance, pupil=None,
+              normalized_field: bool = False, normalized_pupil: bool = True,
+              integrate: bool = True, noise: bool = True, seed: int = 0):
+        """Forward model: the spectral radiance of a scene to the electrons
+        of every pixel (``_sequential.py:1105-1206``).
+
+        ``wavelength`` (W + 1), ``field_x`` (X + 1) and ``field_y`` (Y + 1)
+        are cell vertices, ``radiance`` is [W][X][Y] per cell (photons per
+        unit time, wavelength, field and pupil area).  ``pupil`` holds the
+        pupil grid's vertex coordinates along x and y (default one cell over
+        the normalized pupil).  One ray per (wavelength, field, pupil) cell
+        at a random point of the cell (the reference's
+        ``cell_centers(random=True)``; its random stream is not reproduced)
+        carries radiance x cell volume; the rays are traced on the device,
+        binned on the sensor's pixels over the wavelength cells and exposed
+        (``measure``).  Returns [W][Nx][Ny] electrons, or [Nx][Ny] with
+        ``integrate``."""
+        h = lambda v: np.asarray(v.detach().cpu().numpy() if hasattr(v, "detach") else v,  # noqa: E731
+                                 np.float64).reshape(-1)
+        rng = np.random.default_rng(seed)
+        wv, fxv, fyv = h(wavelength), h(field_x), h(field_y)
+        W, X, Y = wv.size - 1, fxv.size - 1, fyv.size - 1
+        R = h(radiance).reshape(W, X, Y)
+        pxv, pyv = (np.array([-1.0, 1.0]), np.array([-1.0, 1.0])) if pupil is None else \
+            (h(pupil.x), h(pupil.y))
+        PX, PY = pxv.size - 1, pyv.size - 1
+        wl = wv[:-1] + np.diff(wv) * rng.uniform(size=W)
+        fx_w, fy_w = np.broadcast_to(fxv, (W, X + 1)), np.broadcast_to(fyv, (W, Y + 1))
+        px_w, py_w = np.broadcast_to(pxv, (W, PX + 1)), np.broadcast_to(pyv, (W, PY + 1))
+        if normalized_field or normalized_pupil:
+            flo, fspan, plo, pspan = self._extents(wl)
+            if normalized_field:
+                fx_w = fspan[0][:, None] * (fxv + 1) / 2 + flo[0][:, None]
+                fy_w = fspan[1][:, None] * (fyv + 1) / 2 + flo[1][:, None]
+            if normalized_pupil:
+                px_w = pspan[0][:, None] * (pxv + 1) / 2 + plo[0][:, None]
+                py_w = pspan[1][:, None] * (pyv + 1) / 2 + plo[1][:, None]
+        shape = (W, X, Y, PX, PY)
+        u = rng.uniform(size=(4,) + shape)
+        dfx, dfy = np.diff(fx_w, axis=1), np.diff(fy_w, axis=1)
+        dpx, dpy = np.diff(px_w, axis=1), np.diff(py_w, axis=1)
+        fx = fx_w[:, :-1, None, None, None] + dfx[:, :, None, None, None] * u[0]
+        fy = fy_w[:, None, :-1, None, None] + dfy[:, None, :, None, None] * u[1]
+        px = px_w[:, None, None, :-1, None] + dpx[:, None, None, :, None] * u[2]
+        py = py_w[:, None, None, None, :-1] + dpy[:, None, None, None, :] * u[3]
+        volume = np.abs(np.diff(wv)[:, None, None, None, None] * dfx[:, :, None, None, None]
+                        * dfy[:, None, :, None, None] * dpx[:, None, None, :, None]
+                        * dpy[:, None, None, None, :])
+        flux = R[:, :, :, None, None] * volume
+        lam = np.broadcast_to(wl[:, None, None, None, None], shape)
+        import torch
+
+        from . import _trace
+
+        dev = _trace.device()
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(np.broadcast_to(a, shape)).reshape(-1),  # noqa: E731
+                                      device=dev)
+        rays = self.rayfunction(intensity=t(flux), wavelength=t(lam),
+                                field=Cartesian2dVectorArray(t(fx), t(fy)),
+                                pupil=Cartesian2dVectorArray(t(px), t(py)),
+                                normalized_field=False, normalized_pupil=False).outputs
+        if rays.position.x.ndim != 1:
+            raise NotImplementedError("image of a system with several diffraction orders")
+        electrons = self.sensor.measure(rays, wv, noise=noise, seed=seed)
+        return electrons.sum(0) if integrate else electrons
+
     def _input_grid(self, wavelength, field, pupil, normalized_field, normalized_pupil):
         """The physical input grid: with no wavelength, field or pupil given,
         ``grid_input`` read as normalized whatever the flags

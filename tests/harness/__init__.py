@@ -24,7 +24,7 @@ _lib = None
 
 def build() -> pathlib.Path:
     srcs = [HERE / "optk_host.cpp", ROOT / "optika_amd/csrc/optk_surface.h",
-            ROOT / "optika_amd/csrc/optk_fast.h",
+            ROOT / "optika_amd/csrc/optk_fast.h", ROOT / "optika_amd/csrc/optk_random.h",
             ROOT / "optika_amd/csrc/optk_physics.h", ROOT / "include/optk.h"]
     if LIB.exists() and LIB.stat().st_mtime > max(s.stat().st_mtime for s in srcs):
         return LIB
@@ -204,3 +204,20 @@ def grid_bracket(w, x):
     cells = f(_dp(w), ctypes.c_int(w.size), _dp(x), ctypes.c_int(x.size),
               j.ctypes.data_as(ctypes.c_void_p), steps.ctypes.data_as(ctypes.c_void_p))
     return j, steps, cells
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().host_philox(c, k, o)
+    return list(o)
+
+
+def expose(mean, seed, read_noise=0.0):
+    m = np.ascontiguousarray(mean, dtype=np.float64)
+    out = np.empty_like(m)
+    f = lib().host_expose
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double, ctypes.c_void_p]
+    f(m.ctypes.data, m.size, seed, read_noise, out.ctypes.data)
+    return out

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../optika_amd/csrc/optk_fast.h"
+#include "../../optika_amd/csrc/optk_random.h"
 
 using namespace optk;
 
@@ -41,6 +42,31 @@ std::vector<optk_surface> derived(const optk_system_desc *d) {
 }
 
 extern "C" {
+
+// Philox4x32-10 of one counter under one key (known-answer tests)
+void host_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  uint32_t k[2] = {key[0], key[1]};
+  for (int r = 0; r < 10; ++r) {
+    philox_round(c, k);
+    k[0] += 0x9E3779B9u;
+    k[1] += 0xBB67AE85u;
+  }
+  for (int j = 0; j < 4; ++j) out[j] = c[j];
+}
+
+// the exposure's draws for pixels 0 .. n-1: Poisson of mean[i] plus
+// read_noise times a Gaussian (the kernel's per-pixel code)
+void host_expose(const double *mean, int64_t n, uint64_t seed, double read_noise, double *out) {
+  for (int64_t i = 0; i < n; ++i) {
+    UniformStream rng;
+    rng.pixel = (uint64_t)i;
+    rng.seed = seed;
+    double e = poisson_draw(mean[i], rng);
+    if (read_noise > 0.0) e += read_noise * gaussian_draw(rng);
+    out[i] = e;
+  }
+}
 
 // Knot brackets of n wavelengths (Angstrom) in one ascending table through
 // its index grid: j[i] (w[j] <= x < w[j + 1], clamped to 0 .. count - 2) and
