@@ -42,10 +42,10 @@ def test_image_of_a_uniform_scene():
     system.sensor.timedelta_exposure = 2.0
     wv = np.array([500.0, 520.0, 540.0])
     f = np.linspace(-0.03, 0.03, 7)
-    R = np.full((2, 6, 6), 1e-3)
+    R = np.full((2, 6, 6), 1.0)
     v = np.linspace(-1, 1, 31)
     img = system.image(wv, f, f, R, pupil=Cartesian2dVectorArray(v, v), noise=False, seed=4)
-    want = 2.0 * 1e-3 * 40.0 * 0.06 ** 2 * np.pi * (10.0 / 0.75) ** 2
+    want = 2.0 * 1.0 * 40.0 * 0.06 ** 2 * np.pi * (10.0 / 0.75) ** 2
     total = float(img.sum())
     assert abs(total - want) < 0.02 * want, (total, want)
     per_band = system.image(wv, f, f, R, pupil=Cartesian2dVectorArray(v, v), noise=False, seed=4,
