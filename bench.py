@@ -232,9 +232,14 @@ def two_forms(args, system, grid, n, surfaces, paths, kernel32, kernel64, traffi
         ms_h, _ = time_calls(lambda: _collect.histogram(r32, *args_h), 5, 1, stream)
         img32, _ = _collect.histogram(r32, *args_h)
         d = (img32 - img64).abs().sum().item()
+        # exposure of the binned image (Poisson shot noise + read noise)
+        ms_e, _ = time_calls(lambda: _collect.expose(img64, 1.0, 2.0, True, 7), 5, 1, stream)
+        e = _collect.expose(img64, 1.0, 0.0, True, 7)
         hist = {"pixels": f"{num[0]}x{num[1]}", "ms_per_image": ms_h,
                 "f64_total": img64.sum().item(), "f32_total": img32.sum().item(),
-                "l1_difference_rel": d / max(img64.abs().sum().item(), 1e-300)}
+                "l1_difference_rel": d / max(img64.abs().sum().item(), 1e-300),
+                "expose_ms_per_image": ms_e,
+                "expose_electrons_vs_photons": e.sum().item() / max(img64.sum().item(), 1e-300)}
         log(f"histogram: {hist}")
     units = n * paths * surfaces
     value64 = units / (ms64 / 1e3)

@@ -27,6 +27,12 @@
  *   optk_batch_*     n-dimensional configurations of a system
  *                    (optika/systems/_sequential.py:2125-2132,
  *                    optika/surfaces.py:386-395), with ragged ray counts
+ *   optk_stop_newton the stop solve's root finding
+ *                    (optika/systems/_sequential.py:396-662)
+ *   optk_pupil_reduce the pupil reductions of distortion / vignetting /
+ *                    area_effective (optika/systems/_sequential.py:1208-1512)
+ *   optk_expose      AbstractImagingSensor.expose with the ideal sensor
+ *                    material (optika/sensors/_sensors.py:173-252)
  */
 #ifndef OPTK_H
 #define OPTK_H
