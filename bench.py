@@ -523,7 +523,12 @@ def run_surrogate(args):
         "surfaces": S,
         "form": "surrogate (value counts surfaces of the emulated path per ray)",
     }
-    return header(args, v_sur, ms_sur, "bf16", config, forms, roofline, None)
+    cpu = None
+    if not args.no_cpu:
+        # the reference's path for these rays: the numpy float64 trace
+        cpu = cpu_baseline(system, workloads.spectrograph_grid(1 << 12, seed=502),
+                           min(args.cpu_rays or 1 << 12, 1 << 12), S, paths=2)
+    return header(args, v_sur, ms_sur, "bf16", config, forms, roofline, cpu)
 
 
 def run_stops(args):
