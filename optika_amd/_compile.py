@@ -23,7 +23,6 @@ from . import _lib
 from . import apertures as _ap
 from . import materials as _mat
 from . import rulings as _rl
-from . import sags as _sags
 from .transformations import AbstractTransformation
 
 __all__ = ["CompiledSystem", "compile_surfaces", "affine_record"]

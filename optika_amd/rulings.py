@@ -14,7 +14,7 @@ model are evaluated per ray inside the fused trace kernel.
 from __future__ import annotations
 
 import dataclasses
-from typing import Any, Sequence
+from typing import Any
 
 import numpy as np
 
