@@ -108,3 +108,28 @@ def test_stop_newton_arguments_are_rejected_without_a_device():
     a = _lib.StopArgs()
     assert L.optk_stop_newton(None, None, ctypes.byref(a), 1, None) == -1
     assert b"null" in L.optk_last_error()
+
+
+def test_pupil_and_expose_arguments_are_rejected_without_a_device():
+    from optika_amd import _lib
+
+    if not _lib.library_path().exists():
+        pytest.skip("liboptk.so not built")
+    L = _lib.lib()
+    assert L.optk_pupil_reduce(None, 1, None) == -1
+    a = _lib.PupilArgs()
+    a.per_segment = 4
+    assert L.optk_pupil_reduce(ctypes.byref(a), 1, None) == -1
+    assert b"unvignetted" in L.optk_last_error()
+    assert L.optk_pupil_reduce(ctypes.byref(a), -1, None) == -1
+    e = _lib.ExposeArgs()
+    assert L.optk_expose(ctypes.byref(e), 4, None) == -1
+    assert b"image" in L.optk_last_error()
+    buf = (ctypes.c_double * 4)()
+    e.photons = e.electrons = ctypes.addressof(buf)
+    e.scale = -1.0
+    assert L.optk_expose(ctypes.byref(e), 4, None) == -1
+    assert b"non-negative" in L.optk_last_error()
+    # zero-size calls succeed without touching a device
+    e.scale = 1.0
+    assert L.optk_expose(ctypes.byref(e), 0, None) == 0
