@@ -344,9 +344,18 @@ typedef struct optk_collect_args {
   double x_lo, x_hi, y_lo, y_hi;
   double *image;
   double *moment;
+  /* optional device workspace of optk_collect_workspace() bytes: with it the
+   * rays are counting-sorted into 64 x 64-pixel tiles and each tile is
+   * summed in LDS (no global atomics); without it (or when the image has
+   * more than 4096 tiles) every ray adds to the image with float64 global
+   * atomics */
+  void *workspace;
+  uint64_t workspace_bytes;
 } optk_collect_args;
 
 int32_t optk_collect(const optk_collect_args *args, int64_t n, void *hip_stream);
+/* bytes of workspace the tiled binning of n rays needs (0: not applicable) */
+uint64_t optk_collect_workspace(const optk_collect_args *args, int64_t n);
 
 /* Exposure of a binned image by an ideal sensor material (quantum yield 1):
  * optika/sensors/_sensors.py:173-252 (expose) with

@@ -7,8 +7,10 @@ weights ``intensity`` for unvignetted rays, plus the flux-weighted mean
 refracted cosine per pixel.  The ideal sensor material does not refract, so
 the cosine is ``-d . n = d_z`` for the sensor normal ``(0, 0, -1)``.
 
-Accumulation uses float64 atomics in HBM; counts and bin assignment are
-exact, weighted sums differ from a sequential sum by rounding only.
+The rays are counting-sorted into 64 x 64-pixel tiles and each tile is
+summed with float64 LDS atomics by one block (``tiled=False``, or more than
+4096 tiles: float64 atomics in HBM); counts and bin assignment are exact,
+weighted sums differ from a sequential sum by rounding only.
 """
 
 from __future__ import annotations
@@ -24,9 +26,11 @@ from .vectors import as_vector2
 __all__ = ["collect", "histogram", "expose"]
 
 
-def histogram(rays, wavelength_edges, x_range, y_range, num_x, num_y, where=None):
+def histogram(rays, wavelength_edges, x_range, y_range, num_x, num_y, where=None, tiled=True):
     """Raw device histogram: returns ``(image, moment)`` float64 tensors of shape
-    ``(n_wavelength, num_x, num_y)``."""
+    ``(n_wavelength, num_x, num_y)``.  ``tiled``: counting-sort the rays into
+    64 x 64-pixel tiles and sum each tile in LDS (a workspace of ~18 bytes
+    per ray from the caching allocator) instead of float64 global atomics."""
     dev = _trace.device()
     n = rays.num
     f32 = isinstance(rays.position.x, torch.Tensor) and rays.position.x.dtype == torch.float32
@@ -67,7 +71,14 @@ def histogram(rays, wavelength_edges, x_range, y_range, num_x, num_y, where=None
     a.x_lo, a.x_hi = float(x_range[0]), float(x_range[1])
     a.y_lo, a.y_hi = float(y_range[0]), float(y_range[1])
     a.image, a.moment = image.data_ptr(), moment.data_ptr()
-    _lib.check(_lib.lib().optk_collect(ctypes.byref(a), n, _trace.stream_handle()))
+    L = _lib.lib()
+    ws = None
+    if tiled:
+        nbytes = int(L.optk_collect_workspace(ctypes.byref(a), n))
+        if nbytes:
+            ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+            a.workspace, a.workspace_bytes = ws.data_ptr(), nbytes
+    _lib.check(L.optk_collect(ctypes.byref(a), n, _trace.stream_handle()))
     return image, moment
 
 

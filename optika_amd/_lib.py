@@ -177,6 +177,8 @@ class CollectArgs(ctypes.Structure):
         ("x_lo", c_double), ("x_hi", c_double), ("y_lo", c_double), ("y_hi", c_double),
         ("image", c_void_p),
         ("moment", c_void_p),
+        ("workspace", c_void_p),
+        ("workspace_bytes", ctypes.c_uint64),
     ]
 
 
@@ -310,6 +312,8 @@ def lib() -> ctypes.CDLL:
     L.optk_eval.argtypes = [c_void_p, c_int32, ctypes.POINTER(EvalArgs), c_int64, c_void_p]
     L.optk_collect.restype = c_int32
     L.optk_collect.argtypes = [ctypes.POINTER(CollectArgs), c_int64, c_void_p]
+    L.optk_collect_workspace.restype = ctypes.c_uint64
+    L.optk_collect_workspace.argtypes = [ctypes.POINTER(CollectArgs), c_int64]
     L.optk_batch_create.restype = c_int32
     L.optk_batch_create.argtypes = [ctypes.POINTER(BatchDesc), ctypes.POINTER(c_void_p)]
     L.optk_batch_destroy.restype = c_int32

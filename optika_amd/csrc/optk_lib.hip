@@ -567,6 +567,177 @@ __device__ __forceinline__ int edges_bin(double x, const double *e, int num) {
   return lo;
 }
 
+// Bin of ray i (image index), or -1 when it is masked or out of range.
+__device__ __forceinline__ int64_t collect_bin(const CollectParams &C, int64_t i, double &inten,
+                                               double &dz) {
+  if (C.unvignetted && !C.unvignetted[i]) return -1;
+  double x, y, w;
+  if (C.px) {
+    x = C.px[i];
+    y = C.py[i];
+    dz = C.dz[i];
+    w = C.wavelength[i];
+    inten = C.intensity[i];
+  } else {
+    x = C.pxf[i];
+    y = C.pyf[i];
+    dz = C.dzf[i];
+    w = C.wavelengthf[i];
+    inten = C.intensityf[i];
+  }
+  const int bw = edges_bin(w, C.wedges, C.nw);
+  const int bx = linspace_bin(x, C.x_lo, C.x_hi, C.nx);
+  const int by = linspace_bin(y, C.y_lo, C.y_hi, C.ny);
+  if (bw < 0 || bx < 0 || by < 0) return -1;
+  return ((int64_t)bw * C.nx + bx) * C.ny + by;
+}
+
+// Tiled binning: a counting sort of the rays into 64 x 64-pixel tiles, then
+// one block per tile sums its rays in LDS and adds the tile to the image.
+//   count:   block b counts the rays of its chunk per tile (LDS), writes
+//            counts[b][tile]
+//   offsets: per tile, the running sum over blocks; tile totals
+//   base:    exclusive scan of the totals (one block)
+//   scatter: block b re-bins its chunk and writes (pixel in tile, weight,
+//            weighted cosine) at base[tile] + offsets[b][tile] + rank
+//   tiles:   block t sums tile t's rays with LDS float64 atomics and adds
+//            the 4096 pixels to the image (one block owns each pixel)
+constexpr int CT_SIDE = 64;
+constexpr int CT_PIX = CT_SIDE * CT_SIDE;
+constexpr int CT_MAX_TILES = 4096;
+constexpr int64_t CT_CHUNK = 16384;  // rays per block in count / scatter
+
+struct TiledParams {
+  CollectParams C;
+  int32_t tiles_x, tiles_y, num_tiles, blocks;
+  int32_t *counts;      // [blocks][num_tiles]
+  int64_t *offsets;     // [blocks][num_tiles]
+  int64_t *base;        // [num_tiles + 1]
+  uint16_t *sorted_pix;
+  double *sorted_a, *sorted_b;
+};
+
+__device__ __forceinline__ int collect_tile(const TiledParams &T, int64_t idx, int &pix) {
+  const int64_t plane = (int64_t)T.C.nx * T.C.ny;
+  const int w = (int)(idx / plane);
+  const int64_t r = idx - (int64_t)w * plane;
+  const int bx = (int)(r / T.C.ny), by = (int)(r - (int64_t)bx * T.C.ny);
+  pix = (bx % CT_SIDE) * CT_SIDE + (by % CT_SIDE);
+  return (w * T.tiles_x + bx / CT_SIDE) * T.tiles_y + by / CT_SIDE;
+}
+
+__global__ __launch_bounds__(256) void collect_count(const TiledParams T) {
+  __shared__ int32_t cnt[CT_MAX_TILES];
+  for (int t = threadIdx.x; t < T.num_tiles; t += blockDim.x) cnt[t] = 0;
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * CT_CHUNK;
+  const int64_t hi = lo + CT_CHUNK < T.C.n ? lo + CT_CHUNK : T.C.n;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    double inten, dz;
+    const int64_t idx = collect_bin(T.C, i, inten, dz);
+    if (idx < 0) continue;
+    int pix;
+    atomicAdd(&cnt[collect_tile(T, idx, pix)], 1);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < T.num_tiles; t += blockDim.x)
+    T.counts[(int64_t)blockIdx.x * T.num_tiles + t] = cnt[t];
+}
+
+__global__ __launch_bounds__(256) void collect_offsets(const TiledParams T) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T.num_tiles) return;
+  int64_t run = 0;
+  for (int b = 0; b < T.blocks; ++b) {
+    const int64_t k = (int64_t)b * T.num_tiles + t;
+    T.offsets[k] = run;
+    run += T.counts[k];
+  }
+  T.base[t] = run;  // the tile's total, scanned by collect_base
+}
+
+__global__ __launch_bounds__(1024) void collect_base(const TiledParams T) {
+  // exclusive scan of T.base[0 .. num_tiles) in place (num_tiles <= 4096):
+  // each thread scans 4 entries, then a block scan of the thread sums
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  int64_t v[4], s = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = 4 * t + j;
+    v[j] = k < T.num_tiles ? T.base[k] : 0;
+    s += v[j];
+  }
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int64_t add = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += add;
+    __syncthreads();
+  }
+  int64_t run = part[t] - s;  // exclusive prefix of this thread's group
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = 4 * t + j;
+    if (k < T.num_tiles) T.base[k] = run;
+    run += v[j];
+  }
+  if (t == 1023) T.base[T.num_tiles] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void collect_scatter(const TiledParams T) {
+  __shared__ int64_t start[CT_MAX_TILES];
+  __shared__ int32_t rank[CT_MAX_TILES];
+  for (int t = threadIdx.x; t < T.num_tiles; t += blockDim.x) {
+    start[t] = T.base[t] + T.offsets[(int64_t)blockIdx.x * T.num_tiles + t];
+    rank[t] = 0;
+  }
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * CT_CHUNK;
+  const int64_t hi = lo + CT_CHUNK < T.C.n ? lo + CT_CHUNK : T.C.n;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    double inten, dz;
+    const int64_t idx = collect_bin(T.C, i, inten, dz);
+    if (idx < 0) continue;
+    int pix;
+    const int tile = collect_tile(T, idx, pix);
+    const int64_t slot = start[tile] + atomicAdd(&rank[tile], 1);
+    T.sorted_pix[slot] = (uint16_t)pix;
+    T.sorted_a[slot] = inten;
+    T.sorted_b[slot] = inten * dz;
+  }
+}
+
+__global__ __launch_bounds__(256) void collect_tiles(const TiledParams T) {
+  __shared__ double acc_a[CT_PIX];
+  __shared__ double acc_b[CT_PIX];
+  for (int p = threadIdx.x; p < CT_PIX; p += blockDim.x) {
+    acc_a[p] = 0.0;
+    acc_b[p] = 0.0;
+  }
+  __syncthreads();
+  const int tile = blockIdx.x;
+  const int64_t lo = T.base[tile], hi = T.base[tile + 1];
+  for (int64_t j = lo + threadIdx.x; j < hi; j += blockDim.x) {
+    const int pix = T.sorted_pix[j];
+    atomicAdd(&acc_a[pix], T.sorted_a[j]);
+    atomicAdd(&acc_b[pix], T.sorted_b[j]);
+  }
+  __syncthreads();
+  if (hi == lo) return;  // nothing to add
+  const int w = tile / (T.tiles_x * T.tiles_y);
+  const int r = tile - w * T.tiles_x * T.tiles_y;
+  const int tx = r / T.tiles_y, ty = r - tx * T.tiles_y;
+  for (int p = threadIdx.x; p < CT_PIX; p += blockDim.x) {
+    const int gx = tx * CT_SIDE + p / CT_SIDE, gy = ty * CT_SIDE + p % CT_SIDE;
+    if (gx >= T.C.nx || gy >= T.C.ny) continue;
+    const int64_t k = ((int64_t)w * T.C.nx + gx) * T.C.ny + gy;
+    T.C.image[k] += acc_a[p];
+    T.C.moment[k] += acc_b[p];
+  }
+}
+
 // One ray per thread.  Rays of a wavefront that land in the same bin are
 // combined before the atomic: the wave picks the first pending lane's bin,
 // sums the weights of every lane in that bin with shuffles, and one lane
@@ -1116,6 +1287,43 @@ int32_t optk_expose(const optk_expose_args *args, int64_t n, void *hip_stream) {
   return OPTK_OK;
 }
 
+
+// Workspace of the tiled binning (sections 256-byte aligned); fills T's
+// pointers when ws is given.  0 when the image has too many tiles.
+static uint64_t tiled_layout(int nw, int nx, int ny, int64_t n, void *ws, TiledParams *T) {
+  const int64_t tx = (nx + CT_SIDE - 1) / CT_SIDE, ty = (ny + CT_SIDE - 1) / CT_SIDE;
+  const int64_t tiles = (int64_t)nw * tx * ty;
+  if (tiles > CT_MAX_TILES || n <= 0) return 0;
+  const int64_t blocks = (n + CT_CHUNK - 1) / CT_CHUNK;
+  if (blocks > INT32_MAX) return 0;
+  auto up = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
+  const uint64_t s_counts = up((uint64_t)blocks * tiles * 4);
+  const uint64_t s_offsets = up((uint64_t)blocks * tiles * 8);
+  const uint64_t s_base = up((uint64_t)(tiles + 1) * 8);
+  const uint64_t s_pix = up((uint64_t)n * 2);
+  const uint64_t s_val = up((uint64_t)n * 8);
+  const uint64_t total = s_counts + s_offsets + s_base + s_pix + 2 * s_val;
+  if (T) {
+    T->tiles_x = (int32_t)tx;
+    T->tiles_y = (int32_t)ty;
+    T->num_tiles = (int32_t)tiles;
+    T->blocks = (int32_t)blocks;
+    char *p = (char *)ws;
+    T->counts = (int32_t *)p;
+    p += s_counts;
+    T->offsets = (int64_t *)p;
+    p += s_offsets;
+    T->base = (int64_t *)p;
+    p += s_base;
+    T->sorted_pix = (uint16_t *)p;
+    p += s_pix;
+    T->sorted_a = (double *)p;
+    p += s_val;
+    T->sorted_b = (double *)p;
+  }
+  return total;
+}
+
 int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
   if (!a) return fail(OPTK_EINVAL, "null argument");
   if (n < 0) return fail(OPTK_EINVAL, "negative count");
@@ -1153,9 +1361,27 @@ int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
   C.image = a->image;
   C.moment = a->moment;
   C.n = n;
-  hipLaunchKernelGGL(collect_kernel, grid_for(n), dim3(256), 0, (hipStream_t)hip_stream, C);
+  TiledParams T;
+  const uint64_t need = tiled_layout(a->n_wavelength, a->nx, a->ny, n, a->workspace, &T);
+  hipStream_t s = (hipStream_t)hip_stream;
+  if (need == 0 || !a->workspace || a->workspace_bytes < need) {
+    hipLaunchKernelGGL(collect_kernel, grid_for(n), dim3(256), 0, s, C);
+    OPTK_HIP_CHECK(hipGetLastError());
+    return OPTK_OK;
+  }
+  T.C = C;
+  hipLaunchKernelGGL(collect_count, dim3((unsigned)T.blocks), dim3(256), 0, s, T);
+  hipLaunchKernelGGL(collect_offsets, dim3((unsigned)((T.num_tiles + 255) / 256)), dim3(256), 0, s, T);
+  hipLaunchKernelGGL(collect_base, dim3(1), dim3(1024), 0, s, T);
+  hipLaunchKernelGGL(collect_scatter, dim3((unsigned)T.blocks), dim3(256), 0, s, T);
+  hipLaunchKernelGGL(collect_tiles, dim3((unsigned)T.num_tiles), dim3(256), 0, s, T);
   OPTK_HIP_CHECK(hipGetLastError());
   return OPTK_OK;
+}
+
+uint64_t optk_collect_workspace(const optk_collect_args *a, int64_t n) {
+  if (!a || n <= 0 || a->n_wavelength < 1 || a->nx < 1 || a->ny < 1) return 0;
+  return tiled_layout(a->n_wavelength, a->nx, a->ny, n, nullptr, nullptr);
 }
 
 }  // extern "C"
