@@ -345,10 +345,10 @@ typedef struct optk_collect_args {
   double *image;
   double *moment;
   /* optional device workspace of optk_collect_workspace() bytes: with it the
-   * rays are counting-sorted into 64 x 64-pixel tiles and each tile is
+   * rays are counting-sorted into 32 x 32-pixel tiles and each tile is
    * summed in LDS (no global atomics); without it (or when the image has
-   * more than 4096 tiles) every ray adds to the image with float64 global
-   * atomics */
+   * more than 16384 tiles, or for 2^31 rays or more) every ray adds to the
+   * image with float64 global atomics */
   void *workspace;
   uint64_t workspace_bytes;
 } optk_collect_args;

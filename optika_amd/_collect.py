@@ -7,9 +7,9 @@ weights ``intensity`` for unvignetted rays, plus the flux-weighted mean
 refracted cosine per pixel.  The ideal sensor material does not refract, so
 the cosine is ``-d . n = d_z`` for the sensor normal ``(0, 0, -1)``.
 
-The rays are counting-sorted into 64 x 64-pixel tiles and each tile is
+The rays are counting-sorted into 32 x 32-pixel tiles and each tile is
 summed with float64 LDS atomics by one block (``tiled=False``, or more than
-4096 tiles: float64 atomics in HBM); counts and bin assignment are exact,
+16384 tiles: float64 atomics in HBM); counts and bin assignment are exact,
 weighted sums differ from a sequential sum by rounding only.
 """
 
@@ -29,7 +29,7 @@ __all__ = ["collect", "histogram", "expose"]
 def histogram(rays, wavelength_edges, x_range, y_range, num_x, num_y, where=None, tiled=True):
     """Raw device histogram: returns ``(image, moment)`` float64 tensors of shape
     ``(n_wavelength, num_x, num_y)``.  ``tiled``: counting-sort the rays into
-    64 x 64-pixel tiles and sum each tile in LDS (a workspace of ~18 bytes
+    32 x 32-pixel tiles and sum each tile in LDS (a workspace of ~18 bytes
     per ray from the caching allocator) instead of float64 global atomics."""
     dev = _trace.device()
     n = rays.num

@@ -592,29 +592,29 @@ __device__ __forceinline__ int64_t collect_bin(const CollectParams &C, int64_t i
   return ((int64_t)bw * C.nx + bx) * C.ny + by;
 }
 
-// Tiled binning: a counting sort of the rays into 64 x 64-pixel tiles, then
+// Tiled binning: a counting sort of the rays into 32 x 32-pixel tiles, then
 // one block per tile sums its rays in LDS and adds the tile to the image.
 //   count:   block b counts the rays of its chunk per tile (LDS), writes
-//            counts[b][tile]
-//   offsets: per tile, the running sum over blocks; tile totals
+//            its row counts[b][*] (coalesced)
+//   offsets: one wavefront per tile scans the tile's column of block counts
+//            (in place: exclusive offsets) and leaves the tile's total
 //   base:    exclusive scan of the totals (one block)
 //   scatter: block b re-bins its chunk and writes (pixel in tile, weight,
-//            weighted cosine) at base[tile] + offsets[b][tile] + rank
+//            weighted cosine) at base[tile] + offsets[tile][b] + rank
 //   tiles:   block t sums tile t's rays with LDS float64 atomics and adds
-//            the 4096 pixels to the image (one block owns each pixel)
-constexpr int CT_SIDE = 64;
+//            its pixels to the image (one block owns each pixel)
+constexpr int CT_SIDE = 32;
 constexpr int CT_PIX = CT_SIDE * CT_SIDE;
-constexpr int CT_MAX_TILES = 4096;
-constexpr int64_t CT_CHUNK = 16384;  // rays per block in count / scatter
+constexpr int CT_MAX_TILES = 16384;
+constexpr int64_t CT_CHUNK = 8192;  // rays per block in count / scatter
 
 struct TiledParams {
   CollectParams C;
   int32_t tiles_x, tiles_y, num_tiles, blocks;
-  int32_t *counts;      // [blocks][num_tiles]
-  int64_t *offsets;     // [blocks][num_tiles]
-  int64_t *base;        // [num_tiles + 1]
+  int32_t *counts;      // [blocks][num_tiles], then exclusive offsets in place
+  int32_t *base;        // [num_tiles + 1]
   uint16_t *sorted_pix;
-  double *sorted_a, *sorted_b;
+  double2 *sorted_ab;   // (weight, weighted cosine)
 };
 
 __device__ __forceinline__ int collect_tile(const TiledParams &T, int64_t idx, int &pix) {
@@ -626,8 +626,9 @@ __device__ __forceinline__ int collect_tile(const TiledParams &T, int64_t idx, i
   return (w * T.tiles_x + bx / CT_SIDE) * T.tiles_y + by / CT_SIDE;
 }
 
+// count and scatter hold one 32-bit word per tile in dynamic LDS
 __global__ __launch_bounds__(256) void collect_count(const TiledParams T) {
-  __shared__ int32_t cnt[CT_MAX_TILES];
+  extern __shared__ int32_t cnt[];
   for (int t = threadIdx.x; t < T.num_tiles; t += blockDim.x) cnt[t] = 0;
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * CT_CHUNK;
@@ -645,41 +646,51 @@ __global__ __launch_bounds__(256) void collect_count(const TiledParams T) {
 }
 
 __global__ __launch_bounds__(256) void collect_offsets(const TiledParams T) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= T.num_tiles) return;
-  int64_t run = 0;
-  for (int b = 0; b < T.blocks; ++b) {
-    const int64_t k = (int64_t)b * T.num_tiles + t;
-    T.offsets[k] = run;
-    run += T.counts[k];
+  const int lane = threadIdx.x & 63;
+  const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (t >= T.num_tiles) return;  // whole wavefronts
+  int32_t *col = T.counts + t;  // entry b at col[b * num_tiles]
+  int32_t carry = 0;
+  for (int b0 = 0; b0 < T.blocks; b0 += 64) {
+    const int b = b0 + lane;
+    const int64_t k = (int64_t)b * T.num_tiles;
+    const int32_t c = b < T.blocks ? col[k] : 0;
+    int32_t incl = c;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    if (b < T.blocks) col[k] = carry + incl - c;
+    carry += __shfl(incl, 63, 64);
   }
-  T.base[t] = run;  // the tile's total, scanned by collect_base
+  if (lane == 0) T.base[t] = carry;  // the tile's total, scanned by collect_base
 }
 
 __global__ __launch_bounds__(1024) void collect_base(const TiledParams T) {
-  // exclusive scan of T.base[0 .. num_tiles) in place (num_tiles <= 4096):
-  // each thread scans 4 entries, then a block scan of the thread sums
-  __shared__ int64_t part[1024];
+  // exclusive scan of T.base[0 .. num_tiles) in place (num_tiles <= 16384):
+  // each thread scans 16 entries, then a block scan of the thread sums
+  constexpr int PER = CT_MAX_TILES / 1024;
+  __shared__ int32_t part[1024];
   const int t = threadIdx.x;
-  int64_t v[4], s = 0;
+  int32_t v[PER], sum = 0;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int k = 4 * t + j;
+  for (int j = 0; j < PER; ++j) {
+    const int k = PER * t + j;
     v[j] = k < T.num_tiles ? T.base[k] : 0;
-    s += v[j];
+    sum += v[j];
   }
-  part[t] = s;
+  part[t] = sum;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
-    const int64_t add = t >= off ? part[t - off] : 0;
+    const int32_t add = t >= off ? part[t - off] : 0;
     __syncthreads();
     part[t] += add;
     __syncthreads();
   }
-  int64_t run = part[t] - s;  // exclusive prefix of this thread's group
+  int32_t run = part[t] - sum;  // exclusive prefix of this thread's entries
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int k = 4 * t + j;
+  for (int j = 0; j < PER; ++j) {
+    const int k = PER * t + j;
     if (k < T.num_tiles) T.base[k] = run;
     run += v[j];
   }
@@ -687,12 +698,9 @@ __global__ __launch_bounds__(1024) void collect_base(const TiledParams T) {
 }
 
 __global__ __launch_bounds__(256) void collect_scatter(const TiledParams T) {
-  __shared__ int64_t start[CT_MAX_TILES];
-  __shared__ int32_t rank[CT_MAX_TILES];
-  for (int t = threadIdx.x; t < T.num_tiles; t += blockDim.x) {
-    start[t] = T.base[t] + T.offsets[(int64_t)blockIdx.x * T.num_tiles + t];
-    rank[t] = 0;
-  }
+  extern __shared__ int32_t cursor[];
+  for (int t = threadIdx.x; t < T.num_tiles; t += blockDim.x)
+    cursor[t] = T.base[t] + T.counts[(int64_t)blockIdx.x * T.num_tiles + t];
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * CT_CHUNK;
   const int64_t hi = lo + CT_CHUNK < T.C.n ? lo + CT_CHUNK : T.C.n;
@@ -702,30 +710,30 @@ __global__ __launch_bounds__(256) void collect_scatter(const TiledParams T) {
     if (idx < 0) continue;
     int pix;
     const int tile = collect_tile(T, idx, pix);
-    const int64_t slot = start[tile] + atomicAdd(&rank[tile], 1);
+    const int32_t slot = atomicAdd(&cursor[tile], 1);
     T.sorted_pix[slot] = (uint16_t)pix;
-    T.sorted_a[slot] = inten;
-    T.sorted_b[slot] = inten * dz;
+    T.sorted_ab[slot] = double2{inten, inten * dz};
   }
 }
 
 __global__ __launch_bounds__(256) void collect_tiles(const TiledParams T) {
   __shared__ double acc_a[CT_PIX];
   __shared__ double acc_b[CT_PIX];
+  const int tile = blockIdx.x;
+  const int32_t lo = T.base[tile], hi = T.base[tile + 1];
+  if (hi == lo) return;  // whole block: nothing to add
   for (int p = threadIdx.x; p < CT_PIX; p += blockDim.x) {
     acc_a[p] = 0.0;
     acc_b[p] = 0.0;
   }
   __syncthreads();
-  const int tile = blockIdx.x;
-  const int64_t lo = T.base[tile], hi = T.base[tile + 1];
-  for (int64_t j = lo + threadIdx.x; j < hi; j += blockDim.x) {
+  for (int32_t j = lo + threadIdx.x; j < hi; j += blockDim.x) {
     const int pix = T.sorted_pix[j];
-    atomicAdd(&acc_a[pix], T.sorted_a[j]);
-    atomicAdd(&acc_b[pix], T.sorted_b[j]);
+    const double2 ab = T.sorted_ab[j];
+    atomicAdd(&acc_a[pix], ab.x);
+    atomicAdd(&acc_b[pix], ab.y);
   }
   __syncthreads();
-  if (hi == lo) return;  // nothing to add
   const int w = tile / (T.tiles_x * T.tiles_y);
   const int r = tile - w * T.tiles_x * T.tiles_y;
   const int tx = r / T.tiles_y, ty = r - tx * T.tiles_y;
@@ -1289,20 +1297,19 @@ int32_t optk_expose(const optk_expose_args *args, int64_t n, void *hip_stream) {
 
 
 // Workspace of the tiled binning (sections 256-byte aligned); fills T's
-// pointers when ws is given.  0 when the image has too many tiles.
+// pointers when ws is given.  0 when the image has too many tiles or the
+// rays are too many for 32-bit slots.
 static uint64_t tiled_layout(int nw, int nx, int ny, int64_t n, void *ws, TiledParams *T) {
   const int64_t tx = (nx + CT_SIDE - 1) / CT_SIDE, ty = (ny + CT_SIDE - 1) / CT_SIDE;
   const int64_t tiles = (int64_t)nw * tx * ty;
-  if (tiles > CT_MAX_TILES || n <= 0) return 0;
+  if (tiles > CT_MAX_TILES || n <= 0 || n > INT32_MAX) return 0;
   const int64_t blocks = (n + CT_CHUNK - 1) / CT_CHUNK;
-  if (blocks > INT32_MAX) return 0;
   auto up = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
   const uint64_t s_counts = up((uint64_t)blocks * tiles * 4);
-  const uint64_t s_offsets = up((uint64_t)blocks * tiles * 8);
-  const uint64_t s_base = up((uint64_t)(tiles + 1) * 8);
+  const uint64_t s_base = up((uint64_t)(tiles + 1) * 4);
   const uint64_t s_pix = up((uint64_t)n * 2);
-  const uint64_t s_val = up((uint64_t)n * 8);
-  const uint64_t total = s_counts + s_offsets + s_base + s_pix + 2 * s_val;
+  const uint64_t s_val = up((uint64_t)n * 16);
+  const uint64_t total = s_counts + s_base + s_pix + s_val;
   if (T) {
     T->tiles_x = (int32_t)tx;
     T->tiles_y = (int32_t)ty;
@@ -1311,15 +1318,11 @@ static uint64_t tiled_layout(int nw, int nx, int ny, int64_t n, void *ws, TiledP
     char *p = (char *)ws;
     T->counts = (int32_t *)p;
     p += s_counts;
-    T->offsets = (int64_t *)p;
-    p += s_offsets;
-    T->base = (int64_t *)p;
+    T->base = (int32_t *)p;
     p += s_base;
     T->sorted_pix = (uint16_t *)p;
     p += s_pix;
-    T->sorted_a = (double *)p;
-    p += s_val;
-    T->sorted_b = (double *)p;
+    T->sorted_ab = (double2 *)p;
   }
   return total;
 }
@@ -1370,10 +1373,11 @@ int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
     return OPTK_OK;
   }
   T.C = C;
-  hipLaunchKernelGGL(collect_count, dim3((unsigned)T.blocks), dim3(256), 0, s, T);
-  hipLaunchKernelGGL(collect_offsets, dim3((unsigned)((T.num_tiles + 255) / 256)), dim3(256), 0, s, T);
+  const size_t lds = sizeof(int32_t) * (size_t)T.num_tiles;
+  hipLaunchKernelGGL(collect_count, dim3((unsigned)T.blocks), dim3(256), lds, s, T);
+  hipLaunchKernelGGL(collect_offsets, dim3((unsigned)((T.num_tiles + 3) / 4)), dim3(256), 0, s, T);
   hipLaunchKernelGGL(collect_base, dim3(1), dim3(1024), 0, s, T);
-  hipLaunchKernelGGL(collect_scatter, dim3((unsigned)T.blocks), dim3(256), 0, s, T);
+  hipLaunchKernelGGL(collect_scatter, dim3((unsigned)T.blocks), dim3(256), lds, s, T);
   hipLaunchKernelGGL(collect_tiles, dim3((unsigned)T.num_tiles), dim3(256), 0, s, T);
   OPTK_HIP_CHECK(hipGetLastError());
   return OPTK_OK;
