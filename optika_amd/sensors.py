@@ -14,6 +14,8 @@ from __future__ import annotations
 import dataclasses
 from typing import Any
 
+import numpy as np
+
 from . import apertures as _apertures
 from . import materials as _materials
 from . import sags as _sags
@@ -106,3 +108,71 @@ class ImagingSensor(AbstractImagingSensor):
 
     def __post_init__(self):
         self.num_pixel = as_vector2(self.num_pixel)
+
+
+# -- silicon sensor absorbance (optika/sensors/materials/_materials.py:55-376) --
+
+THICKNESS_OXIDE = 5.0        # nm (50 A, Stern 1994; _stern_1994.py:3)
+THICKNESS_SUBSTRATE = 7000.0  # nm (7 um, _stern_1994.py:4)
+
+
+def _np(v):
+    """A host numpy copy of an array or device tensor."""
+    return v.detach().cpu().numpy() if hasattr(v, "detach") else np.asarray(v)
+
+
+def absorption_effective(wavelength, n_substrate, direction_substrate):
+    """Absorption coefficient per unit perpendicular depth (1/nm) of a wave
+    refracted into the light-sensitive region, ``4 pi Im(n cos) / lambda``
+    (``_materials.py:55-93``)."""
+    return 4 * np.pi * np.imag(_np(n_substrate) * _np(direction_substrate)) / \
+        _np(wavelength).astype(np.float64)
+
+
+def transmittance(wavelength, direction=1, n=1, thickness_oxide=THICKNESS_OXIDE,
+                  thickness_substrate=THICKNESS_SUBSTRATE, chemical_oxide="SiO2",
+                  chemical_substrate="Si", roughness_oxide=0.0, roughness_substrate=0.0):
+    """Fraction of the incident energy transmitted through the oxide into
+    the substrate, per polarization (``_materials.py:96-197``): the
+    multilayer transfer-matrix kernel on the device."""
+    from . import _ops
+    from .materials import Layer, profiles
+
+    oxide = Layer(chemical=chemical_oxide, thickness=float(thickness_oxide),
+                  interface=profiles.ErfInterfaceProfile(width=float(roughness_oxide)))
+    substrate = Layer(chemical=chemical_substrate, thickness=float(thickness_substrate),
+                      interface=profiles.ErfInterfaceProfile(width=float(roughness_substrate)))
+    _, t = _ops.multilayer_efficiency(wavelength, direction, n, layers=[oxide],
+                                      substrate=substrate)
+    return t
+
+
+def absorbance(wavelength, direction=1, n=1, thickness_oxide=THICKNESS_OXIDE,
+               thickness_substrate=THICKNESS_SUBSTRATE, chemical_oxide="SiO2",
+               chemical_substrate="Si", roughness_oxide=0.0, roughness_substrate=0.0,
+               method: str = "Beer-Lambert"):
+    """Fraction of the incident energy absorbed in the light-sensitive
+    region, per polarization (``_materials.py:200-376``), by the
+    Beer-Lambert method: the transmittance into the substrate times
+    ``1 - exp(-alpha d)`` with the effective absorption of the refracted
+    wave.  The ``exact`` method (interference inside the substrate) is not
+    built."""
+    from . import _ops
+    from .chemicals import Chemical
+    from .vectors import PolarizationVectorArray
+
+    if method != "Beer-Lambert":
+        raise NotImplementedError(f"absorbance method {method!r}: only 'Beer-Lambert' is built")
+    t = transmittance(wavelength, direction, n, thickness_oxide, thickness_substrate,
+                      chemical_oxide, chemical_substrate, roughness_oxide, roughness_substrate)
+    chem = chemical_substrate if not isinstance(chemical_substrate, str) else Chemical(chemical_substrate)
+    n_sub = chem.n(_np(wavelength).astype(np.float64))
+    d_sub = _ops.snells_law_scalar(direction, n, n_sub)
+    alpha = absorption_effective(wavelength, n_sub, d_sub)
+    decay = np.exp(-alpha * float(thickness_substrate))
+
+    def absorbed(tp):
+        tp = _np(tp)
+        return np.real(tp - tp * decay)
+
+    return PolarizationVectorArray(absorbed(t.s), absorbed(t.p))
