@@ -564,9 +564,18 @@ OPTK_HD V3<T> fast_normal(const FastSurface &f, T x, T y) {
 
 // Snell's law (_snells_law.py:341-366) in float64 with fast reciprocal and
 // square root
-OPTK_HD V3<double> fast_snell(V3<double> a, double n1, double n2, V3<double> u, bool mirror) {
-  double a2 = a.x * a.x + a.y * a.y + a.z * a.z;
+// `unit`: |a| = 1 (no grating before): with an unchanged index the law
+// reduces exactly to b = a (refraction) or b = a - 2 (a.u) u (mirror), the
+// square root of 1 + (a.u)^2 - |a|^2 = |a.u| being taken algebraically.
+OPTK_HD V3<double> fast_snell(V3<double> a, double n1, double n2, V3<double> u, bool mirror,
+                              bool unit) {
   bool same = n1 == n2 && n1 != 0.0 && fabs(n1) < INFINITY;
+  if (same && unit) {
+    if (!mirror) return a;
+    const double au2 = 2.0 * (a.x * u.x + a.y * u.y + a.z * u.z);
+    return {a.x - au2 * u.x, a.y - au2 * u.y, a.z - au2 * u.z};
+  }
+  double a2 = a.x * a.x + a.y * a.y + a.z * a.z;
   double r = same ? 1.0 : n1 * rcp64(n2);
   double au = a.x * u.x + a.y * u.y + a.z * u.z;
   double sgn = -copysign(1.0, au);
@@ -961,7 +970,7 @@ OPTK_HD void fast_propagate(const FastSurface &f, const optk_surface &s, const F
 
   const double n1 = r.index;
   const double n2 = fast_index(s, (double)r.wavelength, n1);
-  V3<double> b = fast_snell(a, n1, n2, normal, s.is_mirror != 0);
+  V3<double> b = fast_snell(a, n1, n2, normal, s.is_mirror != 0, !ruled);
   float eff = fast_material_efficiency<F>(s, v.sv, v.tables32, r.wavelength, a, r.index,
                                           r.attenuation, normal);
   if (ruled) {
