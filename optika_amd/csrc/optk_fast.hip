@@ -68,6 +68,48 @@ __device__ __forceinline__ void store_f(const FRaysOut &o, int64_t j, const FRay
   if (o.unvignetted) o.unvignetted[j] = r.unvignetted ? 1 : 0;
 }
 
+// I/O layouts of the kernels.  IO_ANY honours every null pointer of the
+// descriptors (a value default in, the field not written out).  IO_SENSOR
+// is the layout of the sensor-frame rayfunction: field, pupil and
+// wavelength arrays in, the intensity one value; position, direction,
+// intensity and the mask out, nothing else.  With the presence fixed at
+// compile time the kernel keeps no null tests, and none of the pointers it
+// does not use, live across its ray loop (they would otherwise crowd the
+// scalar registers the system records are read into).
+enum { IO_ANY = 0, IO_SENSOR = 1 };
+
+bool sensor_layout(const FGridIn &g, const FRaysOut &o) {
+  return g.field[0] && g.field[1] && g.pupil[0] && g.pupil[1] && g.wavelength && !g.intensity &&
+         o.p[0] && o.p[1] && o.p[2] && o.d[0] && o.d[1] && o.d[2] && o.intensity &&
+         o.unvignetted && !o.wavelength && !o.attenuation && !o.index;
+}
+
+template <int IO> __device__ __forceinline__ FRay load_ray(const FRaysIn &in, int64_t i) {
+  return load_fray(in, i);
+}
+
+template <int IO> __device__ __forceinline__ FRay load_ray(const FGridIn &g, int64_t i) {
+  if (IO == IO_SENSOR)
+    return fast_grid_ray(g.field[0][i], g.field[1][i], g.pupil[0][i], g.pupil[1][i],
+                         g.wavelength[i], g.intensity_default, g.object_at_infinity);
+  return load_fray(g, i);
+}
+
+template <int IO> __device__ __forceinline__ void store_ray(const FRaysOut &o, int64_t j, const FRay &r) {
+  if (IO == IO_SENSOR) {
+    o.p[0][j] = (float)r.px;
+    o.p[1][j] = (float)r.py;
+    o.p[2][j] = (float)r.pz;
+    o.d[0][j] = (float)r.dx;
+    o.d[1][j] = (float)r.dy;
+    o.d[2][j] = (float)r.dz;
+    o.intensity[j] = r.intensity;
+    o.unvignetted[j] = r.unvignetted ? 1 : 0;
+    return;
+  }
+  store_f(o, j, r);
+}
+
 size_t fast_staged_bytes(int num_surfaces) {
   return sizeof(FastSpec) + (sizeof(FastSurface) + sizeof(optk_surface)) * (size_t)num_surfaces;
 }
@@ -76,27 +118,23 @@ template <int F> struct FastWaves {
   static constexpr int value = (F & FEAT_MULTILAYER) ? 2 : 4;
 };
 
-template <int F, bool ACCUM, typename In>
+// The system's records are read through the constant address space: every
+// lane of a wave reads the same record at the same time, so the loads are
+// scalar-cache reads whose values land in SGPRs, which the float64 VALU
+// instructions take as operands directly (no LDS staging, no VGPRs holding
+// matrix elements, branch conditions already uniform).
+typedef __attribute__((address_space(4))) const FastSpec ConstFastSpec;
+typedef __attribute__((address_space(4))) const FastSurface ConstFastSurface;
+typedef __attribute__((address_space(4))) const optk_surface ConstSurface;
+
+template <int F, bool ACCUM, int IO, typename In>
 __device__ __forceinline__ void fast_body(const FastArgs &A, const In &in, const FRaysOut &accum) {
-  extern __shared__ double staged[];
-  FastSpec *s_spec = reinterpret_cast<FastSpec *>(staged);
-  FastSurface *s_fast = reinterpret_cast<FastSurface *>(s_spec + 1);
-  optk_surface *s_surf = reinterpret_cast<optk_surface *>(s_fast + A.num_surfaces);
-  {
-    auto copy = [](uint64_t *dst, const uint64_t *src, int words) {
-      for (int w = threadIdx.x; w < words; w += blockDim.x) dst[w] = src[w];
-    };
-    copy(reinterpret_cast<uint64_t *>(s_spec), reinterpret_cast<const uint64_t *>(A.spec),
-         (int)(sizeof(FastSpec) / 8));
-    copy(reinterpret_cast<uint64_t *>(s_fast), reinterpret_cast<const uint64_t *>(A.fast),
-         A.num_surfaces * (int)(sizeof(FastSurface) / 8));
-    copy(reinterpret_cast<uint64_t *>(s_surf), reinterpret_cast<const uint64_t *>(A.surfaces),
-         A.num_surfaces * (int)(sizeof(optk_surface) / 8));
-  }
-  __syncthreads();
+  ConstFastSpec *c_spec = (ConstFastSpec *)A.spec;
+  ConstFastSurface *c_fast = (ConstFastSurface *)A.fast;
+  ConstSurface *c_surf = (ConstSurface *)A.surfaces;
   FastView v;
-  v.surfaces = s_surf;
-  v.fast = s_fast;
+  v.surfaces = A.surfaces;
+  v.fast = A.fast;
   v.sv = A.sv;
   v.tables32 = A.tables32;
   const int num_paths = (F & FEAT_RULINGS) ? A.num_paths : 1;
@@ -104,29 +142,31 @@ __device__ __forceinline__ void fast_body(const FastArgs &A, const In &in, const
   const int64_t n = A.n;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    FRay r0 = load_fray(in, i);
-    const optk_affine &first = s_spec->first_hop[A.entry];
+    FRay r0 = load_ray<IO>(in, i);
+    const optk_affine &first = *(const optk_affine *)&c_spec->first_hop[A.entry];
     xf_apply_pos(first, r0.px, r0.py, r0.pz);
     xf_apply_dir(first, r0.dx, r0.dy, r0.dz);
     for (int path = 0; path < num_paths; ++path) {
       FRay r = r0;
       for (int k = 0; k < num_surfaces; ++k) {
+        const FastSurface &fk = *(const FastSurface *)&c_fast[k];
+        const optk_surface &sk = *(const optk_surface *)&c_surf[k];
         if (k > 0) {
-          xf_apply_pos(s_fast[k].hop, r.px, r.py, r.pz);
-          xf_apply_dir(s_fast[k].hop, r.dx, r.dy, r.dz);
+          xf_apply_pos(fk.hop, r.px, r.py, r.pz);
+          xf_apply_dir(fk.hop, r.dx, r.dy, r.dz);
         }
-        fast_propagate<F>(s_fast[k], s_surf[k], v, path, r);
+        fast_propagate<F>(fk, sk, v, path, r);
         if (ACCUM) {
           FRay g = r;
-          xf_apply_pos(s_fast[k].to_global, g.px, g.py, g.pz);
-          xf_apply_dir(s_fast[k].to_global, g.dx, g.dy, g.dz);
+          xf_apply_pos(fk.to_global, g.px, g.py, g.pz);
+          xf_apply_dir(fk.to_global, g.dx, g.dy, g.dz);
           store_f(accum, ((int64_t)k * num_paths + path) * n + i, g);
         }
       }
-      const optk_affine &last = s_spec->last[A.apply_post];
+      const optk_affine &last = *(const optk_affine *)&c_spec->last[A.apply_post];
       xf_apply_pos(last, r.px, r.py, r.pz);
       xf_apply_dir(last, r.dx, r.dy, r.dz);
-      store_f(A.out, (int64_t)path * n + i, r);
+      store_ray<IO>(A.out, (int64_t)path * n + i, r);
     }
   }
 }
@@ -134,13 +174,13 @@ __device__ __forceinline__ void fast_body(const FastArgs &A, const In &in, const
 template <int F, bool ACCUM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
 void trace_rays_f32(const FastArgs A, const FRaysIn in, const FRaysOut accum) {
-  fast_body<F, ACCUM>(A, in, accum);
+  fast_body<F, ACCUM, IO_ANY>(A, in, accum);
 }
 
-template <int F, bool ACCUM>
+template <int F, bool ACCUM, int IO = IO_ANY>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
 void trace_grid_f32(const FastArgs A, const FGridIn g, const FRaysOut accum) {
-  fast_body<F, ACCUM>(A, g, accum);
+  fast_body<F, ACCUM, IO>(A, g, accum);
 }
 
 FastArgs make_args(const optk_system *sys, const FRaysOut &out, int64_t n, int32_t entry,
@@ -166,29 +206,31 @@ FastArgs make_args(const optk_system *sys, const FRaysOut &out, int64_t n, int32
   return A;
 }
 
+// kernels of one entry point: by feature set (geometric, toroid + rulings,
+// all; all with the per-surface accumulation), for either I/O layout
+template <typename In> struct KernelSet {
+  typedef void (*Kernel)(FastArgs, In, FRaysOut);
+  Kernel any[4];
+  Kernel sensor[3];  // null where the entry point has no IO_SENSOR form
+};
+
 template <typename In>
-int32_t launch(void (*k0)(FastArgs, In, FRaysOut), void (*k3)(FastArgs, In, FRaysOut),
-               void (*k7)(FastArgs, In, FRaysOut), void (*k7a)(FastArgs, In, FRaysOut),
-               const optk_system *sys, const In &in, const FRaysOut &out, const FRaysOut *accum,
-               int64_t n, int32_t entry, int32_t apply_post, hipStream_t stream) {
+int32_t launch(const KernelSet<In> &ks, bool sensor, const optk_system *sys, const In &in,
+               const FRaysOut &out, const FRaysOut *accum, int64_t n, int32_t entry,
+               int32_t apply_post, hipStream_t stream) {
   if (n == 0) return OPTK_OK;
-  int dev = 0, lds_max = 0;
-  hipError_t e0 = hipGetDevice(&dev);
-  if (e0 == hipSuccess) e0 = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  if (e0 != hipSuccess) return fail(OPTK_EDEVICE, std::string("device query: ") + hipGetErrorString(e0));
-  if (fast_staged_bytes(sys->num_surfaces) > (size_t)lds_max)
-    return fail(OPTK_EINVAL, "too many surfaces for the float32 trace's LDS staging");
   FastArgs A = make_args(sys, out, n, entry, apply_post);
   FRaysOut acc;
   memset(&acc, 0, sizeof(acc));
   if (accum) acc = *accum;
   const int f = sys->features;
-  auto kern = accum ? k7a : (f == 0 ? k0 : (f == (FEAT_TOROID | FEAT_RULINGS) ? k3 : k7));
+  const int which = f == 0 ? 0 : (f == (FEAT_TOROID | FEAT_RULINGS) ? 1 : 2);
+  typename KernelSet<In>::Kernel kern = accum ? ks.any[3] : ks.any[which];
+  if (!accum && sensor && ks.sensor[which]) kern = ks.sensor[which];
   int64_t blocks = (n + 255) / 256;
   const int64_t max_blocks = 256 * 16;
   if (blocks > max_blocks) blocks = max_blocks;
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), fast_staged_bytes(sys->num_surfaces),
-                     stream, A, in, acc);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, stream, A, in, acc);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("float32 trace launch: ") + hipGetErrorString(e));
   return OPTK_OK;
@@ -265,16 +307,21 @@ namespace optk {
 
 int32_t fast_trace_rays(const optk_system *sys, const FRaysIn &in, const FRaysOut &out,
                         const FRaysOut *accum, int64_t n, int32_t apply_post, hipStream_t stream) {
-  return launch<FRaysIn>(trace_rays_f32<0, false>, trace_rays_f32<FEAT_TOROID | FEAT_RULINGS, false>,
-                         trace_rays_f32<FEAT_ALL, false>, trace_rays_f32<FEAT_ALL, true>, sys, in, out,
-                         accum, n, 0, apply_post, stream);
+  static const KernelSet<FRaysIn> ks = {
+      {trace_rays_f32<0, false>, trace_rays_f32<FEAT_TOROID | FEAT_RULINGS, false>,
+       trace_rays_f32<FEAT_ALL, false>, trace_rays_f32<FEAT_ALL, true>},
+      {nullptr, nullptr, nullptr}};
+  return launch<FRaysIn>(ks, false, sys, in, out, accum, n, 0, apply_post, stream);
 }
 
 int32_t fast_trace_grid(const optk_system *sys, const FGridIn &in, const FRaysOut &out,
                         const FRaysOut *accum, int64_t n, int32_t apply_post, hipStream_t stream) {
-  return launch<FGridIn>(trace_grid_f32<0, false>, trace_grid_f32<FEAT_TOROID | FEAT_RULINGS, false>,
-                         trace_grid_f32<FEAT_ALL, false>, trace_grid_f32<FEAT_ALL, true>, sys, in, out,
-                         accum, n, 1, apply_post, stream);
+  static const KernelSet<FGridIn> ks = {
+      {trace_grid_f32<0, false>, trace_grid_f32<FEAT_TOROID | FEAT_RULINGS, false>,
+       trace_grid_f32<FEAT_ALL, false>, trace_grid_f32<FEAT_ALL, true>},
+      {trace_grid_f32<0, false, IO_SENSOR>, trace_grid_f32<FEAT_TOROID | FEAT_RULINGS, false, IO_SENSOR>,
+       trace_grid_f32<FEAT_ALL, false, IO_SENSOR>}};
+  return launch<FGridIn>(ks, sensor_layout(in, out), sys, in, out, accum, n, 1, apply_post, stream);
 }
 
 size_t batch_lds_f32(int num_surfaces) { return fast_staged_bytes(num_surfaces); }
