@@ -62,9 +62,8 @@ static int32_t variant_of(int32_t features) {
 // ---------------------------------------------------------------------------
 
 // Per-launch arguments.  The system (surfaces, tables, transformations) is
-// device-resident; each block stages the transformation spec and every
-// surface record into LDS once, then a grid-stride loop traces its rays
-// reading the records from LDS (uniform addresses: broadcast reads).
+// device-resident; a grid-stride loop traces the rays, reading the records
+// with scalar loads (the batched and stop-solve kernels stage them in LDS).
 struct TraceArgs {
   const TraceSpec *spec;         // device copy
   const optk_surface *surfaces;  // device copy
@@ -115,7 +114,8 @@ __device__ __forceinline__ Ray<double> load_ray(const GridIn &g, int64_t i) {
                   g.object_at_infinity);
 }
 
-// bytes of LDS a launch stages: the spec, then the surface records
+// bytes of LDS a batched or stop-solve launch stages: the spec, then the
+// surface records
 static size_t staged_bytes(int num_surfaces) {
   return sizeof(TraceSpec) + sizeof(optk_surface) * (size_t)num_surfaces;
 }
@@ -126,42 +126,82 @@ template <int F> struct Waves {
   static constexpr int value = (F & FEAT_MULTILAYER) ? 2 : 4;
 };
 
-template <int F, bool ACCUM, typename In>
-__device__ __forceinline__ void trace_body(const TraceArgs &A, const In &in, const RaysOut &accum) {
-  extern __shared__ double staged[];
-  TraceSpec *s_spec = reinterpret_cast<TraceSpec *>(staged);
-  optk_surface *s_surf = reinterpret_cast<optk_surface *>(s_spec + 1);
-  {
-    const uint64_t *src_spec = reinterpret_cast<const uint64_t *>(A.spec);
-    uint64_t *dst_spec = reinterpret_cast<uint64_t *>(s_spec);
-    for (int w = threadIdx.x; w < (int)(sizeof(TraceSpec) / 8); w += blockDim.x) dst_spec[w] = src_spec[w];
-    const uint64_t *src = reinterpret_cast<const uint64_t *>(A.surfaces);
-    uint64_t *dst = reinterpret_cast<uint64_t *>(s_surf);
-    const int words = A.num_surfaces * (int)(sizeof(optk_surface) / 8);
-    for (int w = threadIdx.x; w < words; w += blockDim.x) dst[w] = src[w];
+// I/O layouts of the kernels.  IO_ANY honours every null pointer of the
+// descriptors (a value default in, the field not written out).  IO_SENSOR
+// is the layout of the sensor-frame rayfunction: field, pupil and
+// wavelength arrays in, the intensity one value; position, direction,
+// intensity and the mask out, nothing else.  With the presence fixed at
+// compile time the kernel keeps no null tests, and none of the pointers it
+// does not use, live across its ray loop (they would otherwise crowd the
+// scalar registers the system records are read into).
+enum { IO_ANY = 0, IO_SENSOR = 1 };
+
+static bool sensor_layout(const GridIn &g, const RaysOut &o) {
+  return g.field[0] && g.field[1] && g.pupil[0] && g.pupil[1] && g.wavelength && !g.intensity &&
+         o.p[0] && o.p[1] && o.p[2] && o.d[0] && o.d[1] && o.d[2] && o.intensity &&
+         o.unvignetted && !o.wavelength && !o.attenuation && !o.index;
+}
+
+template <int IO> __device__ __forceinline__ Ray<double> load_io(const RaysIn &in, int64_t i) {
+  return load_ray(in, i);
+}
+
+template <int IO> __device__ __forceinline__ Ray<double> load_io(const GridIn &g, int64_t i) {
+  if (IO == IO_SENSOR)
+    return grid_ray(g.field[0][i], g.field[1][i], g.pupil[0][i], g.pupil[1][i], g.wavelength[i],
+                    g.intensity_default, g.object_at_infinity);
+  return load_ray(g, i);
+}
+
+template <int IO> __device__ __forceinline__ void store_io(const RaysOut &o, int64_t j, const Ray<double> &r) {
+  if (IO == IO_SENSOR) {
+    o.p[0][j] = r.px;
+    o.p[1][j] = r.py;
+    o.p[2][j] = r.pz;
+    o.d[0][j] = r.dx;
+    o.d[1][j] = r.dy;
+    o.d[2][j] = r.dz;
+    o.intensity[j] = r.intensity;
+    o.unvignetted[j] = r.unvignetted ? 1 : 0;
+    return;
   }
-  __syncthreads();
-  SystemView sys = s_spec->sys;
-  sys.surfaces = s_surf;
+  store(o, j, r);
+}
+
+// The system's records are read through the constant address space: every
+// lane of a wave reads the same record at the same time, so the loads are
+// scalar-cache reads whose values land in SGPRs, which the VALU
+// instructions take as operands directly (no VGPRs hold record fields and
+// branch conditions on them are uniform).  The arithmetic is unchanged.
+typedef __attribute__((address_space(4))) const TraceSpec ConstTraceSpec;
+typedef __attribute__((address_space(4))) const optk_surface ConstSurface;
+
+template <int F, bool ACCUM, int IO, typename In>
+__device__ __forceinline__ void trace_body(const TraceArgs &A, const In &in, const RaysOut &accum) {
+  ConstTraceSpec *c_spec = (ConstTraceSpec *)A.spec;
+  ConstSurface *c_surf = (ConstSurface *)A.surfaces;
+  const TraceSpec &spec = *(const TraceSpec *)c_spec;
+  SystemView sys = spec.sys;
+  sys.surfaces = A.surfaces;
   // only rulings create diffraction-order paths: without them one path
   const int num_paths = (F & FEAT_RULINGS) ? sys.num_paths : 1;
   const int num_surfaces = A.num_surfaces;
   const int64_t n = A.n;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    Ray<double> r0 = load_ray(in, i);
-    apply_pre(*s_spec, r0);
+    Ray<double> r0 = load_io<IO>(in, i);
+    apply_pre(spec, r0);
     for (int path = 0; path < num_paths; ++path) {
       Ray<double> r = r0;
       for (int k = 0; k < num_surfaces; ++k) {
-        propagate_surface<F>(s_surf[k], sys, path, r);
+        propagate_surface<F>(*(const optk_surface *)&c_surf[k], sys, path, r);
         if (ACCUM) store(accum, ((int64_t)k * num_paths + path) * n + i, r);
       }
       if (A.apply_post) {
-        xf_inverse_pos(s_spec->post, r.px, r.py, r.pz);
-        xf_inverse_dir(s_spec->post, r.dx, r.dy, r.dz);
+        xf_inverse_pos(spec.post, r.px, r.py, r.pz);
+        xf_inverse_dir(spec.post, r.dx, r.dy, r.dz);
       }
-      store(A.out, (int64_t)path * n + i, r);
+      store_io<IO>(A.out, (int64_t)path * n + i, r);
     }
   }
 }
@@ -169,13 +209,13 @@ __device__ __forceinline__ void trace_body(const TraceArgs &A, const In &in, con
 template <int F, bool ACCUM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
 void trace_rays_f64(const TraceArgs A, const RaysIn in, const RaysOut accum) {
-  trace_body<F, ACCUM>(A, in, accum);
+  trace_body<F, ACCUM, IO_ANY>(A, in, accum);
 }
 
-template <int F, bool ACCUM>
+template <int F, bool ACCUM, int IO = IO_ANY>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
 void trace_grid_f64(const TraceArgs A, const GridIn g, const RaysOut accum) {
-  trace_body<F, ACCUM>(A, g, accum);
+  trace_body<F, ACCUM, IO>(A, g, accum);
 }
 
 // launch the variant covering the system's features; accumulation always
@@ -190,8 +230,7 @@ static void launch_trace(void (*k0)(TraceArgs, In, RaysOut), void (*k3)(TraceArg
   int64_t blocks = (n + 255) / 256;
   const int64_t max_blocks = 256 * 16;
   if (blocks > max_blocks) blocks = max_blocks;
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), staged_bytes(A.num_surfaces), stream, A,
-                     in, acc);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, stream, A, in, acc);
 }
 
 
@@ -1183,9 +1222,15 @@ int32_t optk_trace_grid(const optk_system *sys, int32_t precision, const optk_gr
   g.wavelength_default = in->wavelength_default;
   g.intensity_default = in->intensity_default;
   g.object_at_infinity = in->object_at_infinity;
-  launch_trace<GridIn>(trace_grid_f64<0, false>, trace_grid_f64<FEAT_TOROID | FEAT_RULINGS, false>,
-                       trace_grid_f64<FEAT_ALL, false>, trace_grid_f64<FEAT_ALL, true>, sys->features,
-                       accum != nullptr, n, (hipStream_t)hip_stream, A, g, acc);
+  if (!accum && sensor_layout(g, A.out))
+    launch_trace<GridIn>(trace_grid_f64<0, false, IO_SENSOR>,
+                         trace_grid_f64<FEAT_TOROID | FEAT_RULINGS, false, IO_SENSOR>,
+                         trace_grid_f64<FEAT_ALL, false, IO_SENSOR>, trace_grid_f64<FEAT_ALL, true>,
+                         sys->features, false, n, (hipStream_t)hip_stream, A, g, acc);
+  else
+    launch_trace<GridIn>(trace_grid_f64<0, false>, trace_grid_f64<FEAT_TOROID | FEAT_RULINGS, false>,
+                         trace_grid_f64<FEAT_ALL, false>, trace_grid_f64<FEAT_ALL, true>, sys->features,
+                         accum != nullptr, n, (hipStream_t)hip_stream, A, g, acc);
   OPTK_HIP_CHECK(hipGetLastError());
   return OPTK_OK;
 }
