@@ -997,16 +997,23 @@ OPTK_HD void fast_propagate(const FastSurface &f, const optk_surface &s, const F
   }
 }
 
-// sin and cos of a float64 angle: Maclaurin series through x^11 / x^12 for
-// |x| <= 0.2 rad (truncation below 1e-18 relative); beyond, the float32
-// library functions (about 1e-7 relative: field angles of imaging systems
-// are small, and the float64 library sincos would cost the whole kernel
-// registers for a branch it rarely takes)
+// sin and cos of a float64 angle by Maclaurin series in Horner form, the
+// reciprocal factorial ratios as constants (no division instructions):
+// through x^5 / x^6 for |x| <= 0.01 rad (next terms x^7/5040 and x^8/40320:
+// below 2.5e-16 relative), through x^11 / x^12 for |x| <= 0.2 rad
+// (truncation below 1e-18 relative); beyond, the float32 library functions
+// (about 1e-7 relative: field angles of imaging systems are small, and the
+// float64 library sincos would cost the whole kernel registers for a branch
+// it rarely takes).  A wave whose lanes all take the first branch skips the
+// others.
 OPTK_HD void small_sincos(double x, double &s, double &c) {
+  const double x2 = x * x;
+  if (fabs(x) <= 0.01) {
+    s = x * (1.0 - x2 * (1.0 / 6.0) * (1.0 - x2 * (1.0 / 20.0)));
+    c = 1.0 - x2 * 0.5 * (1.0 - x2 * (1.0 / 12.0) * (1.0 - x2 * (1.0 / 30.0)));
+    return;
+  }
   if (fabs(x) <= 0.2) {
-    // Horner form with the reciprocal factorial ratios as constants (no
-    // division instructions)
-    const double x2 = x * x;
     s = x * (1.0 - x2 * (1.0 / 6.0) * (1.0 - x2 * (1.0 / 20.0) * (1.0 - x2 * (1.0 / 42.0) *
         (1.0 - x2 * (1.0 / 72.0) * (1.0 - x2 * (1.0 / 110.0))))));
     c = 1.0 - x2 * 0.5 * (1.0 - x2 * (1.0 / 12.0) * (1.0 - x2 * (1.0 / 30.0) * (1.0 - x2 * (1.0 / 56.0) *
