@@ -114,7 +114,14 @@ size_t fast_staged_bytes(int num_surfaces) {
   return sizeof(FastSpec) + (sizeof(FastSurface) + sizeof(optk_surface)) * (size_t)num_surfaces;
 }
 
+// occupancy targets (waves per SIMD): the geometric kernel at 6 (80 VGPRs;
+// measured faster than 5 or 8 on the Newtonian workload), the rulings /
+// toroid kernel at 4, the multilayer kernels at 2 (VALU-bound, register-heavy)
 template <int F> struct FastWaves {
+  static constexpr int value = F == 0 ? 6 : ((F & FEAT_MULTILAYER) ? 2 : 4);
+};
+// the batched kernels keep 4 / 2 (at 6 the geometric one spills)
+template <int F> struct BatchWaves {
   static constexpr int value = (F & FEAT_MULTILAYER) ? 2 : 4;
 };
 
@@ -227,9 +234,8 @@ int32_t launch(const KernelSet<In> &ks, bool sensor, const optk_system *sys, con
   const int which = f == 0 ? 0 : (f == (FEAT_TOROID | FEAT_RULINGS) ? 1 : 2);
   typename KernelSet<In>::Kernel kern = accum ? ks.any[3] : ks.any[which];
   if (!accum && sensor && ks.sensor[which]) kern = ks.sensor[which];
-  int64_t blocks = (n + 255) / 256;
-  const int64_t max_blocks = 256 * 16;
-  if (blocks > max_blocks) blocks = max_blocks;
+  // whole rounds of resident blocks; each block loops over its share
+  const int64_t blocks = resident_grid((const void *)kern, 256, n);
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, stream, A, in, acc);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("float32 trace launch: ") + hipGetErrorString(e));
@@ -285,7 +291,7 @@ template <int F> struct Tracer32 {
 };
 
 template <int F>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BatchWaves<F>::value)))
 void batch_grid_f32(const BatchArgs A, const FGridIn in, const FRaysOut out) {
   extern __shared__ double staged[];
   FastSpec *spec = reinterpret_cast<FastSpec *>(staged);

@@ -8,6 +8,8 @@
 
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,6 +28,32 @@ namespace optk {
 int32_t fail(int32_t code, const std::string &msg) {
   g_last_error = msg;
   return code;
+}
+
+int64_t resident_grid(const void *kernel, int block, int64_t n) {
+  static std::mutex mu;
+  static std::map<std::pair<const void *, int>, int64_t> cache;
+  const int64_t need = (n + block - 1) / block;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return need < 4096 ? need : 4096;
+  int64_t resident = 0;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({kernel, dev});
+    if (it != cache.end()) resident = it->second;
+  }
+  if (resident == 0) {
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) == hipSuccess &&
+        cus > 0 && per_cu > 0)
+      resident = (int64_t)cus * per_cu * 4;
+    else
+      resident = 4096;
+    std::lock_guard<std::mutex> lock(mu);
+    cache[{kernel, dev}] = resident;
+  }
+  return need < resident ? need : resident;
 }
 }  // namespace optk
 
@@ -226,10 +254,8 @@ static void launch_trace(void (*k0)(TraceArgs, In, RaysOut), void (*k3)(TraceArg
                          int features, bool accum, int64_t n, hipStream_t stream,
                          const TraceArgs &A, const In &in, const RaysOut &acc) {
   auto kern = accum ? k7a : (features == 0 ? k0 : (features == (FEAT_TOROID | FEAT_RULINGS) ? k3 : k7));
-  // enough blocks to fill every CU a few times over; each block loops
-  int64_t blocks = (n + 255) / 256;
-  const int64_t max_blocks = 256 * 16;
-  if (blocks > max_blocks) blocks = max_blocks;
+  // whole rounds of resident blocks; each block loops over its share
+  const int64_t blocks = resident_grid((const void *)kern, 256, n);
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, stream, A, in, acc);
 }
 
