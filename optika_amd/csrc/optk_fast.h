@@ -56,6 +56,26 @@ OPTK_HD double rcp64(double x) {
 #endif
 }
 
+// the same for x known finite and nonzero (no special-value select)
+OPTK_HD double rcp64_finite(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+#else
+  return 1.0 / x;
+#endif
+}
+
+// reciprocal square root of x known finite and positive
+OPTK_HD double rsqrt64_finite(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double y = __builtin_amdgcn_rsq(x);
+  return fma(0.5 * y, fma(-x * y, y, 1.0), y);
+#else
+  return 1.0 / sqrt(x);
+#endif
+}
+
 OPTK_HD double rsqrt64(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   double y = __builtin_amdgcn_rsq(x);
@@ -67,13 +87,17 @@ OPTK_HD double rsqrt64(double x) {
 #endif
 }
 
+// square root from the refined reciprocal square root and one more Newton
+// step on x - g^2; zeros and +inf are returned as they are (one class test),
+// negative and NaN arguments give NaN through the hardware seed
 OPTK_HD double sqrt64(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  double y = rsqrt64(x);
+  double y = rsqrt64_finite(x);
   double g = x * y;
   double d = fma(-g, g, x);
   g = fma(d, 0.5 * y, g);
-  return (x > 0.0 && x < INFINITY) ? g : (x == 0.0 ? x : (x == INFINITY ? x : NAN));
+  // class mask: -0 (bit 5), +0 (bit 6), +inf (bit 9)
+  return __builtin_amdgcn_class(x, (1 << 5) | (1 << 6) | (1 << 9)) ? x : g;
 #else
   return sqrt(x);
 #endif
@@ -357,6 +381,35 @@ OPTK_HD void affine32_apply(const Affine32 &a, float &x, float &y, float &z) {
   z += a.t[2];
 }
 
+// p -> m p + t and d -> m d under one record, in one branch on its kind,
+// each row a chain of fused multiply-adds (the float32 form's transforms;
+// the float64 form keeps xf_apply_pos / xf_apply_dir and numpy's rounding)
+OPTK_HD void fast_xf(const optk_affine &a, double &px, double &py, double &pz, double &dx,
+                     double &dy, double &dz) {
+  if (a.kind == OPTK_XF_IDENTITY) return;
+  if (a.kind == OPTK_XF_TRANSLATE) {
+    px += a.t[0];
+    py += a.t[1];
+    pz += a.t[2];
+    return;
+  }
+  const double *m = a.m;
+  const double X = fma(m[0], px, fma(m[1], py, fma(m[2], pz, a.t[0])));
+  const double Y = fma(m[3], px, fma(m[4], py, fma(m[5], pz, a.t[1])));
+  const double Z = fma(m[6], px, fma(m[7], py, fma(m[8], pz, a.t[2])));
+  const double DX = fma(m[0], dx, fma(m[1], dy, m[2] * dz));
+  const double DY = fma(m[3], dx, fma(m[4], dy, m[5] * dz));
+  const double DZ = fma(m[6], dx, fma(m[7], dy, m[8] * dz));
+  px = X;
+  py = Y;
+  pz = Z;
+  dx = DX;
+  dy = DY;
+  dz = DZ;
+}
+
+OPTK_HD void fast_xf(const optk_affine &a, FRay &r) { fast_xf(a, r.px, r.py, r.pz, r.dx, r.dy, r.dz); }
+
 // Root of A s^2 + B s + C = 0 that the reference's closed forms select for
 // spheres, cylinders and paraboloids: (-B - sign(u_z) sqrt(disc)) / (2A)
 // (the sign(r u_z) / sign(f u_z) branch of _spherical.py:176-184,
@@ -449,7 +502,10 @@ template <> struct FastOps<float> {
 template <> struct FastOps<double> {
   static OPTK_HD double sqrt(double x) { return sqrt64(x); }
   static OPTK_HD double rcp(double x) { return rcp64(x); }
-  static OPTK_HD double rsqrt(double x) { return rsqrt64(x); }
+  // arguments of the normals' reciprocal square roots are >= 1 (toroid) or
+  // positive wherever the conic's sag is real (1 - k c^2 r^2 > 0 when
+  // (1 + k) c^2 r^2 <= 1); elsewhere the normal is NaN either way
+  static OPTK_HD double rsqrt(double x) { return rsqrt64_finite(x); }
 };
 
 template <typename T> OPTK_HD ToroidEval<T> toroid_eval(T c, T r, T x, T y) {
@@ -511,7 +567,7 @@ OPTK_HD double polish64(const FastSurface &f, double X, double Y, double Z, doub
 // path length from the current (sag-frame) position to the sag
 OPTK_HD double fast_intercept(const FastSurface &f, const FRay &r) {
   // hop to the vertex plane z = 0 in float64 (a ray parallel to it stays put)
-  double t0 = fabs(r.dz) > 1e-6 ? -r.pz * rcp64(r.dz) : 0.0;
+  double t0 = fabs(r.dz) > 1e-6 ? -r.pz * rcp64_finite(r.dz) : 0.0;
   if (f.sag_kind == OPTK_SAG_NONE && fabs(r.dz) > 1e-6) return t0;
   double X = r.px + r.dx * t0, Y = r.py + r.dy * t0, Z = r.pz + r.dz * t0;
   float ux = (float)r.dx, uy = (float)r.dy, uz = (float)r.dz;
@@ -952,8 +1008,7 @@ OPTK_HD void fast_propagate(const FastSurface &f, const optk_surface &s, const F
   float sx = (float)hx, sy = (float)hy, sz = (float)hz;  // sag frame, float
 
   // back to the surface frame
-  xf_apply_pos(f.sag_to_surface, hx, hy, hz);
-  xf_apply_dir(f.sag_to_surface, r.dx, r.dy, r.dz);
+  fast_xf(f.sag_to_surface, hx, hy, hz, r.dx, r.dy, r.dz);
   r.px = hx;
   r.py = hy;
   r.pz = hz;

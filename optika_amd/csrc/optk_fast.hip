@@ -151,28 +151,24 @@ __device__ __forceinline__ void fast_body(const FastArgs &A, const In &in, const
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     FRay r0 = load_ray<IO>(in, i);
     const optk_affine &first = *(const optk_affine *)&c_spec->first_hop[A.entry];
-    xf_apply_pos(first, r0.px, r0.py, r0.pz);
-    xf_apply_dir(first, r0.dx, r0.dy, r0.dz);
+    fast_xf(first, r0);
     for (int path = 0; path < num_paths; ++path) {
       FRay r = r0;
       for (int k = 0; k < num_surfaces; ++k) {
         const FastSurface &fk = *(const FastSurface *)&c_fast[k];
         const optk_surface &sk = *(const optk_surface *)&c_surf[k];
         if (k > 0) {
-          xf_apply_pos(fk.hop, r.px, r.py, r.pz);
-          xf_apply_dir(fk.hop, r.dx, r.dy, r.dz);
+          fast_xf(fk.hop, r);
         }
         fast_propagate<F>(fk, sk, v, path, r);
         if (ACCUM) {
           FRay g = r;
-          xf_apply_pos(fk.to_global, g.px, g.py, g.pz);
-          xf_apply_dir(fk.to_global, g.dx, g.dy, g.dz);
+          fast_xf(fk.to_global, g);
           store_f(accum, ((int64_t)k * num_paths + path) * n + i, g);
         }
       }
       const optk_affine &last = *(const optk_affine *)&c_spec->last[A.apply_post];
-      xf_apply_pos(last, r.px, r.py, r.pz);
-      xf_apply_dir(last, r.dx, r.dy, r.dz);
+      fast_xf(last, r);
       store_ray<IO>(A.out, (int64_t)path * n + i, r);
     }
   }
@@ -269,18 +265,15 @@ template <int F> struct Tracer32 {
     v.sv = A.sv;
     v.tables32 = A.tables32;
     FRay r = load_fray(in, i);
-    xf_apply_pos(s_spec->first_hop[1], r.px, r.py, r.pz);
-    xf_apply_dir(s_spec->first_hop[1], r.dx, r.dy, r.dz);
+    fast_xf(s_spec->first_hop[1], r);
     for (int k = 0; k < A.num_surfaces; ++k) {
       if (k > 0) {
-        xf_apply_pos(s_fast[k].hop, r.px, r.py, r.pz);
-        xf_apply_dir(s_fast[k].hop, r.dx, r.dy, r.dz);
+        fast_xf(s_fast[k].hop, r);
       }
       fast_propagate<F>(s_fast[k], s_surf[k], v, path, r);
     }
     const optk_affine &last = s_spec->last[A.apply_post];
-    xf_apply_pos(last, r.px, r.py, r.pz);
-    xf_apply_dir(last, r.dx, r.dy, r.dz);
+    fast_xf(last, r);
     store_f(out, (int64_t)path * A.n + i, r);
     // statistics of the float32 result, as a caller reading the outputs sees it
     x = (double)(float)r.px;
