@@ -50,13 +50,13 @@ BYTES_PER_RAY_F64 = 97
 # HBM bytes per launch of trace_grid_f64 at 2^24 rays, from rocprofv3 --pmc:
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 256 MiB device copy);
 # the runs and commands are in profiles/trace_grid_f64_newtonian_2p24.md
-TRAFFIC_BYTES_F64_2P24 = (342198 * 2.0 + 950396 * 1.0) * 1024
+TRAFFIC_BYTES_F64_2P24 = (327873 * 2.0 + 934660 * 1.0) * 1024
 
 # float32 form: in 5 x 4 B, out 7 x 4 B + 1 B mask
 BYTES_PER_RAY_F32 = 49
 # HBM bytes per launch of trace_grid_f32 at 2^24 rays (rocprofv3 --pmc, the
 # same scaling; profiles/trace_grid_f32_newtonian_2p24.md)
-TRAFFIC_BYTES_F32_2P24 = (164156 * 2.0 + 475137 * 1.0) * 1024
+TRAFFIC_BYTES_F32_2P24 = (168030 * 2.0 + 487487 * 1.0) * 1024
 
 PIXEL_MM = 0.015
 
@@ -282,7 +282,8 @@ def run_newtonian(args):
     grid = workloads.newtonian_grid_random(n, seed=1)
     is_2p24 = n == 1 << 24
     value, ms, forms, roofline = two_forms(
-        args, system, grid, n, S, 1, "trace_grid_f32<0, false>", "trace_grid_f64<0, false>",
+        args, system, grid, n, S, 1, "trace_grid_f32<0, false, IO_SENSOR>",
+        "trace_grid_f64<0, false, IO_SENSOR>",
         TRAFFIC_BYTES_F32_2P24 if is_2p24 else None, TRAFFIC_BYTES_F64_2P24 if is_2p24 else None,
         sensor=([400.0, 700.0], (15.36, 15.36), (2048, 2048)))
     if args.calibrate_copy:
@@ -311,7 +312,8 @@ def run_spectrograph(args):
     log(f"generating {n} rays")
     grid = workloads.spectrograph_grid(n, seed=2)
     value, ms, forms, roofline = two_forms(args, system, grid, n, S, 2,
-                                           "trace_grid_f32<7, false>", "trace_grid_f64<7, false>")
+                                           "trace_grid_f32<7, false, IO_SENSOR>",
+                                           "trace_grid_f64<7, false, IO_SENSOR>")
     cpu = None if args.no_cpu else cpu_baseline(system, grid, min(args.cpu_rays or 1 << 12, n),
                                                 S, paths=2)
     config = {
