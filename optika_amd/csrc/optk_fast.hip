@@ -147,8 +147,10 @@ __device__ __forceinline__ void fast_body(const FastArgs &A, const In &in, const
   const int num_paths = (F & FEAT_RULINGS) ? A.num_paths : 1;
   const int num_surfaces = A.num_surfaces;
   const int64_t n = A.n;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  // the launch block is TRACE_BLOCK threads: a compile-time stride keeps
+  // it in scalar registers (blockDim.x would be a per-lane load)
+  const int64_t stride = (int64_t)gridDim.x * TRACE_BLOCK;
+  for (int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x; i < n; i += stride) {
     FRay r0 = load_ray<IO>(in, i);
     const optk_affine &first = *(const optk_affine *)&c_spec->first_hop[A.entry];
     fast_xf(first, r0);
@@ -175,13 +177,13 @@ __device__ __forceinline__ void fast_body(const FastArgs &A, const In &in, const
 }
 
 template <int F, bool ACCUM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
 void trace_rays_f32(const FastArgs A, const FRaysIn in, const FRaysOut accum) {
   fast_body<F, ACCUM, IO_ANY>(A, in, accum);
 }
 
 template <int F, bool ACCUM, int IO = IO_ANY>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
 void trace_grid_f32(const FastArgs A, const FGridIn g, const FRaysOut accum) {
   fast_body<F, ACCUM, IO>(A, g, accum);
 }
@@ -231,8 +233,8 @@ int32_t launch(const KernelSet<In> &ks, bool sensor, const optk_system *sys, con
   typename KernelSet<In>::Kernel kern = accum ? ks.any[3] : ks.any[which];
   if (!accum && sensor && ks.sensor[which]) kern = ks.sensor[which];
   // whole rounds of resident blocks; each block loops over its share
-  const int64_t blocks = resident_grid((const void *)kern, 256, n);
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, stream, A, in, acc);
+  const int64_t blocks = resident_grid((const void *)kern, TRACE_BLOCK, n);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(TRACE_BLOCK), 0, stream, A, in, acc);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("float32 trace launch: ") + hipGetErrorString(e));
   return OPTK_OK;

@@ -51,6 +51,10 @@ int32_t fail(int32_t code, const std::string &msg);
 // fewer when n needs fewer.
 int64_t resident_grid(const void *kernel, int block, int64_t n);
 
+// threads per block of the trace kernels (grid and ray entry points, both
+// forms); their grid-stride loops use it as a compile-time constant
+constexpr int TRACE_BLOCK = 256;
+
 // check every index and range a kernel will follow (OPTK_OK or an error)
 int32_t validate_desc(const optk_system_desc *d);
 // the kernel feature variant (FEAT_*) covering the system's surfaces

@@ -215,8 +215,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs &A, const In &in, con
   const int num_paths = (F & FEAT_RULINGS) ? sys.num_paths : 1;
   const int num_surfaces = A.num_surfaces;
   const int64_t n = A.n;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  // the launch block is TRACE_BLOCK threads: a compile-time stride keeps
+  // it in scalar registers (blockDim.x would be a per-lane load)
+  const int64_t stride = (int64_t)gridDim.x * TRACE_BLOCK;
+  for (int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x; i < n; i += stride) {
     Ray<double> r0 = load_io<IO>(in, i);
     apply_pre(spec, r0);
     for (int path = 0; path < num_paths; ++path) {
@@ -235,13 +237,13 @@ __device__ __forceinline__ void trace_body(const TraceArgs &A, const In &in, con
 }
 
 template <int F, bool ACCUM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
 void trace_rays_f64(const TraceArgs A, const RaysIn in, const RaysOut accum) {
   trace_body<F, ACCUM, IO_ANY>(A, in, accum);
 }
 
 template <int F, bool ACCUM, int IO = IO_ANY>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
 void trace_grid_f64(const TraceArgs A, const GridIn g, const RaysOut accum) {
   trace_body<F, ACCUM, IO>(A, g, accum);
 }
@@ -255,8 +257,8 @@ static void launch_trace(void (*k0)(TraceArgs, In, RaysOut), void (*k3)(TraceArg
                          const TraceArgs &A, const In &in, const RaysOut &acc) {
   auto kern = accum ? k7a : (features == 0 ? k0 : (features == (FEAT_TOROID | FEAT_RULINGS) ? k3 : k7));
   // whole rounds of resident blocks; each block loops over its share
-  const int64_t blocks = resident_grid((const void *)kern, 256, n);
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, stream, A, in, acc);
+  const int64_t blocks = resident_grid((const void *)kern, TRACE_BLOCK, n);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(TRACE_BLOCK), 0, stream, A, in, acc);
 }
 
 
