@@ -165,6 +165,10 @@ struct FastSpec {
                              // [1]: the same after the object/system pre-maps
   optk_affine last[2];       // [0]: last surface frame -> system frame;
                              // [1]: -> the post frame (sensor)
+  optk_affine first_skip;    // grid entry with surface 0 elided: input frame
+                             // (before the pre-maps) -> sag frame of surface 1
+  int32_t skip_first;        // 1: surface 0 is a no-op for grid rays
+  int32_t pad;
 };
 
 #define OPTK_GRAZING_COS 0.25f   // multilayers: below this cosine, float64
@@ -303,11 +307,25 @@ inline double sag_steepness(const optk_surface &s) {
   }
 }
 
+// A surface that leaves a ray exactly as it is when the ray already lies on
+// its (flat) sag plane: no sag, vacuum on both sides (index, efficiency and
+// attenuation of the vacuum ray unchanged), no rulings, no active aperture,
+// not a mirror.  Moving such a ray to the plane is t = -0 / u_z = -0, and a
+// position plus -0 times a finite direction is the position itself.
+inline bool fast_passthrough(const optk_surface &s) {
+  return s.sag_kind == OPTK_SAG_NONE && s.material_kind == OPTK_MATERIAL_VACUUM &&
+         s.rulings_kind == OPTK_RULINGS_NONE && !s.is_mirror &&
+         (s.aperture_kind == OPTK_APERTURE_NONE || !(s.aperture_flags & OPTK_AP_ACTIVE));
+}
+
 // Build the float32-form records.  `surfaces` must already carry the derived
 // constants (derive_constants).
 inline void fast_build(const optk_surface *surfaces, int num_surfaces, const optk_affine pre[2],
                        const int32_t pre_is_inverse[2], const optk_affine &post,
                        FastSurface *out, FastSpec &spec) {
+  spec.first_skip = haff_record(haff_identity());
+  spec.skip_first = 0;
+  spec.pad = 0;
   HostAffine prev_to_global = haff_identity();  // system frame to begin with
   for (int i = 0; i < num_surfaces; ++i) {
     const optk_surface &s = surfaces[i];
@@ -346,6 +364,19 @@ inline void fast_build(const optk_surface *surfaces, int num_surfaces, const opt
     HostAffine Tl = haff_of(surfaces[num_surfaces - 1].transform);
     spec.last[0] = haff_record(Tl);
     spec.last[1] = haff_record(haff_then(Tl, haff_inverse(haff_of(post))));
+    // Grid rays start on the plane z = 0 of the object frame.  When the
+    // pre-maps and surface 0's frames carry that plane onto surface 0's sag
+    // plane (the third row of the composed map is (0, 0, m33 | 0)) and
+    // surface 0 is a pass-through, the grid kernels skip it: the input goes
+    // straight to surface 1's sag frame.  The object surface of an imaging
+    // system is such a surface.  (A ray parallel to the plane, u_z = 0, would
+    // get t = -0/0 = NaN there; the kernels reproduce that case.)
+    const HostAffine F1 = haff_then(haff_pre(pre, pre_is_inverse), to_sag0);
+    if (num_surfaces > 1 && fast_passthrough(surfaces[0]) && F1.m[6] == 0.0 && F1.m[7] == 0.0 &&
+        F1.t[2] == 0.0) {
+      spec.first_skip = haff_record(haff_then(haff_then(F1, S0), haff_of(out[1].hop)));
+      spec.skip_first = 1;
+    }
   } else {
     HostAffine pre_h = haff_pre(pre, pre_is_inverse);
     spec.first_hop[0] = haff_record(haff_identity());

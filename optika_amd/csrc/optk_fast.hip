@@ -147,19 +147,33 @@ __device__ __forceinline__ void fast_body(const FastArgs &A, const In &in, const
   const int num_paths = (F & FEAT_RULINGS) ? A.num_paths : 1;
   const int num_surfaces = A.num_surfaces;
   const int64_t n = A.n;
+  // grid entry without per-surface accumulation: a pass-through surface 0
+  // is elided (FastSpec::skip_first)
+  const bool skip = A.entry == 1 && !ACCUM && c_spec->skip_first != 0;
+  const int k0 = skip ? 1 : 0;
   // the launch block is TRACE_BLOCK threads: a compile-time stride keeps
   // it in scalar registers (blockDim.x would be a per-lane load)
   const int64_t stride = (int64_t)gridDim.x * TRACE_BLOCK;
   for (int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x; i < n; i += stride) {
     FRay r0 = load_ray<IO>(in, i);
-    const optk_affine &first = *(const optk_affine *)&c_spec->first_hop[A.entry];
-    fast_xf(first, r0);
+    if (skip) {
+      // surface 0 is a pass-through on whose plane the grid rays start; a ray
+      // parallel to it meets it at t = -0/0 = NaN
+      const bool parallel = r0.dz == 0.0;
+      fast_xf(*(const optk_affine *)&c_spec->first_skip, r0);
+      if (parallel) {
+        r0.px = r0.py = r0.pz = NAN;
+        r0.intensity = NAN;
+      }
+    } else {
+      fast_xf(*(const optk_affine *)&c_spec->first_hop[A.entry], r0);
+    }
     for (int path = 0; path < num_paths; ++path) {
       FRay r = r0;
-      for (int k = 0; k < num_surfaces; ++k) {
+      for (int k = k0; k < num_surfaces; ++k) {
         const FastSurface &fk = *(const FastSurface *)&c_fast[k];
         const optk_surface &sk = *(const optk_surface *)&c_surf[k];
-        if (k > 0) {
+        if (k > k0) {
           fast_xf(fk.hop, r);
         }
         fast_propagate<F>(fk, sk, v, path, r);
@@ -267,9 +281,21 @@ template <int F> struct Tracer32 {
     v.sv = A.sv;
     v.tables32 = A.tables32;
     FRay r = load_fray(in, i);
-    fast_xf(s_spec->first_hop[1], r);
-    for (int k = 0; k < A.num_surfaces; ++k) {
-      if (k > 0) {
+    // a pass-through surface 0 is elided as in the grid kernels
+    const bool skip = s_spec->skip_first != 0;
+    const int k0 = skip ? 1 : 0;
+    if (skip) {
+      const bool parallel = r.dz == 0.0;
+      fast_xf(s_spec->first_skip, r);
+      if (parallel) {
+        r.px = r.py = r.pz = NAN;
+        r.intensity = NAN;
+      }
+    } else {
+      fast_xf(s_spec->first_hop[1], r);
+    }
+    for (int k = k0; k < A.num_surfaces; ++k) {
+      if (k > k0) {
         fast_xf(s_fast[k].hop, r);
       }
       fast_propagate<F>(s_fast[k], s_surf[k], v, path, r);

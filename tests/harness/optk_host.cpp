@@ -3,6 +3,7 @@
 // device arithmetic can be compared with the numpy oracle on a machine
 // without a GPU.  Never linked into or loaded by optika_amd.
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "../../optika_amd/csrc/optk_fast.h"
@@ -135,6 +136,7 @@ int host_trace_grid_f32(const optk_system_desc *d, const float *fx, const float 
   std::vector<optk_surface> surf = derived(d);
   std::vector<FastSurface> fast(d->num_surfaces);
   FastSpec spec;
+  std::memset(&spec, 0, sizeof(spec));
   fast_build(surf.data(), d->num_surfaces, d->pre, d->pre_is_inverse, d->post, fast.data(), spec);
   std::vector<float> tables32(d->tables, d->tables + d->num_tables);
   FastView v;
@@ -144,20 +146,25 @@ int host_trace_grid_f32(const optk_system_desc *d, const float *fx, const float 
   v.tables32 = tables32.data();
   for (int64_t i = 0; i < n; ++i) {
     FRay r0 = fast_grid_ray(fx[i], fy[i], ux[i], uy[i], wl[i], inten[i], object_at_infinity);
-    xf_apply_pos(spec.first_hop[1], r0.px, r0.py, r0.pz);
-    xf_apply_dir(spec.first_hop[1], r0.dx, r0.dy, r0.dz);
+    // as the grid kernels: a pass-through surface 0 is elided
+    const int k0 = spec.skip_first ? 1 : 0;
+    if (spec.skip_first) {
+      const bool parallel = r0.dz == 0.0;
+      fast_xf(spec.first_skip, r0);
+      if (parallel) {
+        r0.px = r0.py = r0.pz = NAN;
+        r0.intensity = NAN;
+      }
+    } else {
+      fast_xf(spec.first_hop[1], r0);
+    }
     for (int path = 0; path < d->num_paths; ++path) {
       FRay r = r0;
-      for (int k = 0; k < d->num_surfaces; ++k) {
-        if (k > 0) {
-          xf_apply_pos(fast[k].hop, r.px, r.py, r.pz);
-          xf_apply_dir(fast[k].hop, r.dx, r.dy, r.dz);
-        }
+      for (int k = k0; k < d->num_surfaces; ++k) {
+        if (k > k0) fast_xf(fast[k].hop, r);
         fast_propagate<FEAT_ALL>(fast[k], surf[k], v, path, r);
       }
-      const optk_affine &last = spec.last[post ? 1 : 0];
-      xf_apply_pos(last, r.px, r.py, r.pz);
-      xf_apply_dir(last, r.dx, r.dy, r.dz);
+      fast_xf(spec.last[post ? 1 : 0], r);
       int64_t j = (int64_t)path * n + i;
       float vals[10] = {(float)r.px, (float)r.py, (float)r.pz, (float)r.dx, (float)r.dy,
                         (float)r.dz, r.wavelength, r.intensity, r.attenuation, r.index};
@@ -176,6 +183,7 @@ int host_trace_rays_f32(const optk_system_desc *d, const float *const *in, int64
   std::vector<optk_surface> surf = derived(d);
   std::vector<FastSurface> fast(d->num_surfaces);
   FastSpec spec;
+  std::memset(&spec, 0, sizeof(spec));
   fast_build(surf.data(), d->num_surfaces, d->pre, d->pre_is_inverse, d->post, fast.data(), spec);
   std::vector<float> tables32(d->tables, d->tables + d->num_tables);
   FastView v;
