@@ -228,8 +228,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs &A, const In &in, con
         if (ACCUM) store(accum, ((int64_t)k * num_paths + path) * n + i, r);
       }
       if (A.apply_post) {
-        xf_inverse_pos(spec.post, r.px, r.py, r.pz);
-        xf_inverse_dir(spec.post, r.dx, r.dy, r.dz);
+        xf_inverse_ray(spec.post, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
       }
       store_io<IO>(A.out, (int64_t)path * n + i, r);
     }
@@ -318,8 +317,7 @@ __global__ __launch_bounds__(256) void stop_newton_f64(const StopParams P) {
         }
         for (int k = 0; k < P.T.num_surfaces; ++k) propagate_surface<F>(s_surf[k], sys, 0, r);
         if (P.T.apply_post) {
-          xf_inverse_pos(s_spec->post, r.px, r.py, r.pz);
-          xf_inverse_dir(s_spec->post, r.dx, r.dy, r.dz);
+          xf_inverse_ray(s_spec->post, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
         }
         const bool on_position = a.target_component == OPTK_STOP_TARGET_POSITION;
         f[e][0] = (on_position ? r.px : r.dx) - g0;
@@ -422,8 +420,7 @@ template <int F> struct Tracer64 {
     sys.surfaces = s_surf;
     for (int k = 0; k < A.num_surfaces; ++k) propagate_surface<F>(s_surf[k], sys, path, r);
     if (A.apply_post) {
-      xf_inverse_pos(s_spec->post, r.px, r.py, r.pz);
-      xf_inverse_dir(s_spec->post, r.dx, r.dy, r.dz);
+      xf_inverse_ray(s_spec->post, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
     }
     store(out, (int64_t)path * A.n + i, r);
     x = r.px;

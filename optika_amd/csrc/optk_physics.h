@@ -197,6 +197,52 @@ template <typename T> OPTK_HD void xf_inverse_dir(const optk_affine &a, T &x, T 
   z = Z;
 }
 
+// a ray's position and direction under one record with one branch on its
+// kind; the arithmetic of each component is that of xf_apply_pos /
+// xf_apply_dir (or xf_inverse_pos / xf_inverse_dir), operation for operation
+template <typename T>
+OPTK_HD void xf_apply_ray(const optk_affine &a, T &px, T &py, T &pz, T &dx, T &dy, T &dz) {
+  if (a.kind == OPTK_XF_IDENTITY) return;
+  if (a.kind == OPTK_XF_AFFINE) {
+    T X = T(a.m[0]) * px + T(a.m[1]) * py + T(a.m[2]) * pz;
+    T Y = T(a.m[3]) * px + T(a.m[4]) * py + T(a.m[5]) * pz;
+    T Z = T(a.m[6]) * px + T(a.m[7]) * py + T(a.m[8]) * pz;
+    T DX = T(a.m[0]) * dx + T(a.m[1]) * dy + T(a.m[2]) * dz;
+    T DY = T(a.m[3]) * dx + T(a.m[4]) * dy + T(a.m[5]) * dz;
+    T DZ = T(a.m[6]) * dx + T(a.m[7]) * dy + T(a.m[8]) * dz;
+    px = X;
+    py = Y;
+    pz = Z;
+    dx = DX;
+    dy = DY;
+    dz = DZ;
+  }
+  px = px + T(a.t[0]);
+  py = py + T(a.t[1]);
+  pz = pz + T(a.t[2]);
+}
+template <typename T>
+OPTK_HD void xf_inverse_ray(const optk_affine &a, T &px, T &py, T &pz, T &dx, T &dy, T &dz) {
+  if (a.kind == OPTK_XF_IDENTITY) return;
+  px = px - T(a.t[0]);
+  py = py - T(a.t[1]);
+  pz = pz - T(a.t[2]);
+  if (a.kind == OPTK_XF_AFFINE) {
+    T X = T(a.m[0]) * px + T(a.m[3]) * py + T(a.m[6]) * pz;
+    T Y = T(a.m[1]) * px + T(a.m[4]) * py + T(a.m[7]) * pz;
+    T Z = T(a.m[2]) * px + T(a.m[5]) * py + T(a.m[8]) * pz;
+    T DX = T(a.m[0]) * dx + T(a.m[3]) * dy + T(a.m[6]) * dz;
+    T DY = T(a.m[1]) * dx + T(a.m[4]) * dy + T(a.m[7]) * dz;
+    T DZ = T(a.m[2]) * dx + T(a.m[5]) * dy + T(a.m[8]) * dz;
+    px = X;
+    py = Y;
+    pz = Z;
+    dx = DX;
+    dy = DY;
+    dz = DZ;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // ray state
 // ---------------------------------------------------------------------------

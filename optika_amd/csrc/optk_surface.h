@@ -238,14 +238,12 @@ template <int F = FEAT_ALL> OPTK_HD void sag_propagate(const optk_surface &s, Ra
     const optk_affine &T = s.sag_transform;
     double ox = r.px, oy = r.py, oz = r.pz;
     double ux = r.dx, uy = r.dy, uz = r.dz;
-    xf_inverse_pos(T, ox, oy, oz);
-    xf_inverse_dir(T, ux, uy, uz);
+    xf_inverse_ray(T, ox, oy, oz, ux, uy, uz);
     double t = sag_intercept_t(s, ox, oy, oz, ux, uy, uz);
     nx = ox + ux * t;
     ny = oy + uy * t;
     nz = oz + uz * t;
-    xf_apply_pos(T, nx, ny, nz);
-    xf_apply_dir(T, ux, uy, uz);
+    xf_apply_ray(T, nx, ny, nz, ux, uy, uz);
     r.dx = ux;
     r.dy = uy;
     r.dz = uz;
@@ -557,8 +555,7 @@ OPTK_HD bool aperture_contains(const optk_surface &s, const double *tables, doub
 template <int F = FEAT_ALL>
 OPTK_HD void propagate_surface(const optk_surface &s, const SystemView &sys, int path,
                                Ray<double> &r) {
-  xf_inverse_pos(s.transform, r.px, r.py, r.pz);
-  xf_inverse_dir(s.transform, r.dx, r.dy, r.dz);
+  xf_inverse_ray(s.transform, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
 
   sag_propagate<F>(s, r);
   V3<double> p1 = {r.px, r.py, r.pz};
@@ -597,8 +594,7 @@ OPTK_HD void propagate_surface(const optk_surface &s, const SystemView &sys, int
     r.unvignetted = r.unvignetted && m;
   }
 
-  xf_apply_pos(s.transform, r.px, r.py, r.pz);
-  xf_apply_dir(s.transform, r.dx, r.dy, r.dz);
+  xf_apply_ray(s.transform, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
 }
 
 // ---------------------------------------------------------------------------
@@ -619,19 +615,16 @@ OPTK_HD void apply_pre(const TraceSpec &P, Ray<double> &r) {
     const optk_affine &a = P.pre[k];
     if (a.kind == OPTK_XF_IDENTITY) continue;
     if (P.pre_is_inverse[k]) {
-      xf_inverse_pos(a, r.px, r.py, r.pz);
-      xf_inverse_dir(a, r.dx, r.dy, r.dz);
+      xf_inverse_ray(a, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
     } else {
-      xf_apply_pos(a, r.px, r.py, r.pz);
-      xf_apply_dir(a, r.dx, r.dy, r.dz);
+      xf_apply_ray(a, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
     }
   }
 }
 
 OPTK_HD void apply_post(const TraceSpec &P, Ray<double> &r) {
   if (!P.apply_post) return;
-  xf_inverse_pos(P.post, r.px, r.py, r.pz);
-  xf_inverse_dir(P.post, r.dx, r.dy, r.dz);
+  xf_inverse_ray(P.post, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
 }
 
 // optika.direction (optika/_util.py:41-73): R_y(phi_x) R_x(phi_y) z
