@@ -597,115 +597,144 @@ struct CollectParams {
   const double *wedges;
   int32_t nw, nx, ny;
   double x_lo, x_hi, y_lo, y_hi;
+  // numpy.linspace's step (hi - lo) / num, and its reciprocal for the first
+  // guess of a bin (host-computed once per call)
+  double x_step, x_inv, y_step, y_inv;
   double *image, *moment;
   int64_t n;
 };
 
 // Bin index of x among numpy.linspace(lo, hi, num + 1) edges with
 // numpy.histogramdd's rule (right edge of the last bin inclusive); -1 if out.
-__device__ __forceinline__ int linspace_bin(double x, double lo, double hi, int num) {
+// The first guess multiplies by 1/step.  Its error, and the rounding of the
+// edges e_j = j * step + lo (e_num = hi) themselves, are a few ulp of
+// num * (|lo| + |hi|) / (hi - lo) bins, below one bin for the ranges
+// optk_collect accepts (that ratio < 2^48); two bounded steps each way then
+// compare x with the exact edges, so the result is the bin the edges
+// define.  The steps compile to selects: no data-dependent loops (which
+// had cost the binning kernels their scalar registers).
+__device__ __forceinline__ int linspace_bin(double x, double lo, double hi, int num, double step,
+                                            double inv) {
   if (!(x >= lo) || !(x <= hi)) return -1;
-  double step = (hi - lo) / (double)num;
-  int k = (int)floor((x - lo) / step);
-  k = k < 0 ? 0 : (k > num ? num : k);
-  // edges: e_j = j * step + lo for j < num, e_num = hi
-  auto edge = [&](int j) { return j >= num ? hi : (double)j * step + lo; };
-  while (k < num && edge(k + 1) <= x) ++k;
-  while (k > 0 && edge(k) > x) --k;
-  // k = number of edges <= x, minus one
   if (x == hi) return num - 1;
-  return k < num ? k : -1;
+  int k = (int)floor((x - lo) * inv);
+  k = k < 0 ? 0 : (k > num - 1 ? num - 1 : k);
+  auto edge = [&](int j) { return j >= num ? hi : (double)j * step + lo; };
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+    if (k < num - 1 && edge(k + 1) <= x) ++k;
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+    if (k > 0 && edge(k) > x) --k;
+  return k;
 }
 
+// Bin of x among ascending edges e[0..num] (numpy.searchsorted(e, x,
+// 'right') - 1, the last edge inclusive); -1 if out.  Binary search by
+// descending powers of two: the trip count depends on num only, so every
+// lane of a wave runs the same steps.
 __device__ __forceinline__ int edges_bin(double x, const double *e, int num) {
   if (!(x >= e[0]) || !(x <= e[num])) return -1;
   if (x == e[num]) return num - 1;
-  int lo = 0, hi = num;  // e[lo] <= x < e[hi]
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (x < e[mid])
-      hi = mid;
-    else
-      lo = mid;
-  }
+  int lo = 0;  // largest index in [0, num) with e[lo] <= x
+  for (int step = 1 << (31 - __clz(num)); step > 0; step >>= 1)
+    if (lo + step < num && e[lo + step] <= x) lo += step;
   return lo;
-}
-
-// Bin of ray i (image index), or -1 when it is masked or out of range.
-__device__ __forceinline__ int64_t collect_bin(const CollectParams &C, int64_t i, double &inten,
-                                               double &dz) {
-  if (C.unvignetted && !C.unvignetted[i]) return -1;
-  double x, y, w;
-  if (C.px) {
-    x = C.px[i];
-    y = C.py[i];
-    dz = C.dz[i];
-    w = C.wavelength[i];
-    inten = C.intensity[i];
-  } else {
-    x = C.pxf[i];
-    y = C.pyf[i];
-    dz = C.dzf[i];
-    w = C.wavelengthf[i];
-    inten = C.intensityf[i];
-  }
-  const int bw = edges_bin(w, C.wedges, C.nw);
-  const int bx = linspace_bin(x, C.x_lo, C.x_hi, C.nx);
-  const int by = linspace_bin(y, C.y_lo, C.y_hi, C.ny);
-  if (bw < 0 || bx < 0 || by < 0) return -1;
-  return ((int64_t)bw * C.nx + bx) * C.ny + by;
 }
 
 // Tiled binning: a counting sort of the rays into 32 x 32-pixel tiles, then
 // one block per tile sums its rays in LDS and adds the tile to the image.
-//   count:   block b counts the rays of its chunk per tile (LDS), writes
-//            its row counts[b][*] (coalesced)
+//   count:   block b bins the rays of its chunk once, stores each ray's key
+//            (tile << 10 | pixel in tile, -1 when not binned), counts the
+//            rays per tile in LDS and writes its row counts[b][*] (coalesced)
 //   offsets: one wavefront per tile scans the tile's column of block counts
 //            (in place: exclusive offsets) and leaves the tile's total
 //   base:    exclusive scan of the totals (one block)
-//   scatter: block b re-bins its chunk and writes (pixel in tile, weight,
-//            weighted cosine) at base[tile] + offsets[tile][b] + rank
+//   scatter: block b reads its chunk's keys and writes one record per ray at
+//            base[tile] + offsets[tile][b] + rank: with CT_CHUNK rays per
+//            block a tile receives runs of ~CT_CHUNK / tiles records, whole
+//            cache lines for a 2048^2 image
 //   tiles:   block t sums tile t's rays with LDS float64 atomics and adds
 //            its pixels to the image (one block owns each pixel)
+// Records: float32 rays pack (pixel, weight, cosine) into one 16-byte
+// store; float64 rays keep a 2-byte pixel array beside (weight, cosine).
+// The moment weight x cosine is formed in float64 in the tiles kernel, as
+// the one-pass kernel forms it.
 constexpr int CT_SIDE = 32;
 constexpr int CT_PIX = CT_SIDE * CT_SIDE;
 constexpr int CT_MAX_TILES = 16384;
-constexpr int64_t CT_CHUNK = 8192;  // rays per block in count / scatter
+constexpr int64_t CT_CHUNK = 32768;  // rays per block in count / scatter
+constexpr int CT_THREADS = 1024;     // threads per count / scatter block
+
+struct RecordF32 {
+  uint32_t pix;
+  float inten, dz;
+  uint32_t pad;
+};
 
 struct TiledParams {
   CollectParams C;
   int32_t tiles_x, tiles_y, num_tiles, blocks;
   int32_t *counts;      // [blocks][num_tiles], then exclusive offsets in place
   int32_t *base;        // [num_tiles + 1]
-  uint16_t *sorted_pix;
-  double2 *sorted_ab;   // (weight, weighted cosine)
+  int32_t *keys;        // [n] tile << 10 | pixel, -1 when not binned
+  uint16_t *sorted_pix; // float64 rays
+  double2 *sorted_ab;   // float64 rays: (weight, cosine)
+  RecordF32 *sorted32;  // float32 rays
 };
 
-__device__ __forceinline__ int collect_tile(const TiledParams &T, int64_t idx, int &pix) {
-  const int64_t plane = (int64_t)T.C.nx * T.C.ny;
-  const int w = (int)(idx / plane);
-  const int64_t r = idx - (int64_t)w * plane;
-  const int bx = (int)(r / T.C.ny), by = (int)(r - (int64_t)bx * T.C.ny);
-  pix = (bx % CT_SIDE) * CT_SIDE + (by % CT_SIDE);
-  return (w * T.tiles_x + bx / CT_SIDE) * T.tiles_y + by / CT_SIDE;
+constexpr int CT_UNROLL = 4;  // rays per thread per step of scatter / tiles
+
+// position, wavelength and mask of one ray, for the key
+struct BinIn {
+  double x, y, w;
+  bool live;
+};
+
+template <bool F32> __device__ __forceinline__ BinIn bin_load(const CollectParams &C, int64_t i) {
+  BinIn b;
+  b.live = !(C.unvignetted && !C.unvignetted[i]);
+  if (F32) {
+    b.x = C.pxf[i];
+    b.y = C.pyf[i];
+    b.w = C.wavelengthf[i];
+  } else {
+    b.x = C.px[i];
+    b.y = C.py[i];
+    b.w = C.wavelength[i];
+  }
+  return b;
 }
 
-// count and scatter hold one 32-bit word per tile in dynamic LDS
-__global__ __launch_bounds__(256) void collect_count(const TiledParams T) {
+// tile << 10 | pixel in tile, or -1 when the ray is masked or outside
+__device__ __forceinline__ int32_t bin_key(const TiledParams &T, const BinIn &b) {
+  if (!b.live) return -1;
+  const int bw = edges_bin(b.w, T.C.wedges, T.C.nw);
+  const int bx = linspace_bin(b.x, T.C.x_lo, T.C.x_hi, T.C.nx, T.C.x_step, T.C.x_inv);
+  const int by = linspace_bin(b.y, T.C.y_lo, T.C.y_hi, T.C.ny, T.C.y_step, T.C.y_inv);
+  if (bw < 0 || bx < 0 || by < 0) return -1;
+  const int tile = (bw * T.tiles_x + bx / CT_SIDE) * T.tiles_y + by / CT_SIDE;
+  return (tile << 10) | ((bx % CT_SIDE) * CT_SIDE + by % CT_SIDE);
+}
+
+// count and scatter hold one 32-bit word per tile in dynamic LDS; both are
+// specialised by the rays' precision (one set of array pointers live)
+template <bool F32>
+__global__ __launch_bounds__(CT_THREADS) void collect_count(const TiledParams T) {
   extern __shared__ int32_t cnt[];
-  for (int t = threadIdx.x; t < T.num_tiles; t += blockDim.x) cnt[t] = 0;
+  for (int t = threadIdx.x; t < T.num_tiles; t += CT_THREADS) cnt[t] = 0;
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * CT_CHUNK;
   const int64_t hi = lo + CT_CHUNK < T.C.n ? lo + CT_CHUNK : T.C.n;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    double inten, dz;
-    const int64_t idx = collect_bin(T.C, i, inten, dz);
-    if (idx < 0) continue;
-    int pix;
-    atomicAdd(&cnt[collect_tile(T, idx, pix)], 1);
+  // one ray per thread per step (unrolling the binning's rare exact walks
+  // would exhaust the scalar registers that hold their exec masks)
+  for (int64_t i = lo + threadIdx.x; i < hi; i += CT_THREADS) {
+    const int32_t key = bin_key(T, bin_load<F32>(T.C, i));
+    if (key >= 0) atomicAdd(&cnt[key >> 10], 1);
+    T.keys[i] = key;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < T.num_tiles; t += blockDim.x)
+  for (int t = threadIdx.x; t < T.num_tiles; t += CT_THREADS)
     T.counts[(int64_t)blockIdx.x * T.num_tiles + t] = cnt[t];
 }
 
@@ -761,26 +790,45 @@ __global__ __launch_bounds__(1024) void collect_base(const TiledParams T) {
   if (t == 1023) T.base[T.num_tiles] = part[1023];
 }
 
-__global__ __launch_bounds__(256) void collect_scatter(const TiledParams T) {
+template <bool F32>
+__global__ __launch_bounds__(CT_THREADS) void collect_scatter(const TiledParams T) {
   extern __shared__ int32_t cursor[];
-  for (int t = threadIdx.x; t < T.num_tiles; t += blockDim.x)
+  for (int t = threadIdx.x; t < T.num_tiles; t += CT_THREADS)
     cursor[t] = T.base[t] + T.counts[(int64_t)blockIdx.x * T.num_tiles + t];
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * CT_CHUNK;
   const int64_t hi = lo + CT_CHUNK < T.C.n ? lo + CT_CHUNK : T.C.n;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    double inten, dz;
-    const int64_t idx = collect_bin(T.C, i, inten, dz);
-    if (idx < 0) continue;
-    int pix;
-    const int tile = collect_tile(T, idx, pix);
-    const int32_t slot = atomicAdd(&cursor[tile], 1);
-    T.sorted_pix[slot] = (uint16_t)pix;
-    T.sorted_ab[slot] = double2{inten, inten * dz};
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += CT_UNROLL * CT_THREADS) {
+    int32_t key[CT_UNROLL];
+    double a[CT_UNROLL], c[CT_UNROLL];
+#pragma unroll
+    for (int u = 0; u < CT_UNROLL; ++u) {
+      const int64_t i = i0 + (int64_t)u * CT_THREADS;
+      key[u] = -1;
+      a[u] = c[u] = 0.0;
+      if (i < hi) {
+        key[u] = T.keys[i];
+        a[u] = F32 ? (double)T.C.intensityf[i] : T.C.intensity[i];
+        c[u] = F32 ? (double)T.C.dzf[i] : T.C.dz[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CT_UNROLL; ++u) {
+      if (key[u] < 0) continue;
+      const int32_t slot = atomicAdd(&cursor[key[u] >> 10], 1);
+      if (F32) {
+        T.sorted32[slot] = RecordF32{(uint32_t)(key[u] & 1023), (float)a[u], (float)c[u], 0u};
+      } else {
+        T.sorted_pix[slot] = (uint16_t)(key[u] & 1023);
+        T.sorted_ab[slot] = double2{a[u], c[u]};
+      }
+    }
   }
 }
 
+template <bool F32>
 __global__ __launch_bounds__(256) void collect_tiles(const TiledParams T) {
+  // launched with 256 threads
   __shared__ double acc_a[CT_PIX];
   __shared__ double acc_b[CT_PIX];
   const int tile = blockIdx.x;
@@ -791,11 +839,33 @@ __global__ __launch_bounds__(256) void collect_tiles(const TiledParams T) {
     acc_b[p] = 0.0;
   }
   __syncthreads();
-  for (int32_t j = lo + threadIdx.x; j < hi; j += blockDim.x) {
-    const int pix = T.sorted_pix[j];
-    const double2 ab = T.sorted_ab[j];
-    atomicAdd(&acc_a[pix], ab.x);
-    atomicAdd(&acc_b[pix], ab.y);
+  for (int32_t j0 = lo + threadIdx.x; j0 < hi; j0 += CT_UNROLL * 256) {
+    int pix[CT_UNROLL];
+    double a[CT_UNROLL], c[CT_UNROLL];
+#pragma unroll
+    for (int u = 0; u < CT_UNROLL; ++u) {
+      const int32_t j = j0 + u * 256;
+      pix[u] = -1;
+      if (j < hi) {
+        if (F32) {
+          const RecordF32 r = T.sorted32[j];
+          pix[u] = (int)r.pix;
+          a[u] = r.inten;
+          c[u] = r.dz;
+        } else {
+          pix[u] = T.sorted_pix[j];
+          const double2 ab = T.sorted_ab[j];
+          a[u] = ab.x;
+          c[u] = ab.y;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CT_UNROLL; ++u) {
+      if (pix[u] < 0) continue;
+      atomicAdd(&acc_a[pix[u]], a[u]);
+      atomicAdd(&acc_b[pix[u]], a[u] * c[u]);
+    }
   }
   __syncthreads();
   const int w = tile / (T.tiles_x * T.tiles_y);
@@ -838,8 +908,8 @@ __global__ __launch_bounds__(256) void collect_kernel(const CollectParams C) {
       inten = C.intensityf[i];
     }
     int bw = edges_bin(w, C.wedges, C.nw);
-    int bx = linspace_bin(x, C.x_lo, C.x_hi, C.nx);
-    int by = linspace_bin(y, C.y_lo, C.y_hi, C.ny);
+    int bx = linspace_bin(x, C.x_lo, C.x_hi, C.nx, C.x_step, C.x_inv);
+    int by = linspace_bin(y, C.y_lo, C.y_hi, C.ny, C.y_step, C.y_inv);
     if (bw >= 0 && bx >= 0 && by >= 0) idx = ((int64_t)bw * C.nx + bx) * C.ny + by;
   }
   uint64_t pending = __ballot(idx >= 0);
@@ -1369,7 +1439,8 @@ int32_t optk_expose(const optk_expose_args *args, int64_t n, void *hip_stream) {
 // Workspace of the tiled binning (sections 256-byte aligned); fills T's
 // pointers when ws is given.  0 when the image has too many tiles or the
 // rays are too many for 32-bit slots.
-static uint64_t tiled_layout(int nw, int nx, int ny, int64_t n, void *ws, TiledParams *T) {
+static uint64_t tiled_layout(int nw, int nx, int ny, int64_t n, bool f32, void *ws,
+                             TiledParams *T) {
   const int64_t tx = (nx + CT_SIDE - 1) / CT_SIDE, ty = (ny + CT_SIDE - 1) / CT_SIDE;
   const int64_t tiles = (int64_t)nw * tx * ty;
   if (tiles > CT_MAX_TILES || n <= 0 || n > INT32_MAX) return 0;
@@ -1377,9 +1448,10 @@ static uint64_t tiled_layout(int nw, int nx, int ny, int64_t n, void *ws, TiledP
   auto up = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
   const uint64_t s_counts = up((uint64_t)blocks * tiles * 4);
   const uint64_t s_base = up((uint64_t)(tiles + 1) * 4);
-  const uint64_t s_pix = up((uint64_t)n * 2);
+  const uint64_t s_keys = up((uint64_t)n * 4);
+  const uint64_t s_pix = f32 ? 0 : up((uint64_t)n * 2);
   const uint64_t s_val = up((uint64_t)n * 16);
-  const uint64_t total = s_counts + s_base + s_pix + s_val;
+  const uint64_t total = s_counts + s_base + s_keys + s_pix + s_val;
   if (T) {
     T->tiles_x = (int32_t)tx;
     T->tiles_y = (int32_t)ty;
@@ -1390,9 +1462,12 @@ static uint64_t tiled_layout(int nw, int nx, int ny, int64_t n, void *ws, TiledP
     p += s_counts;
     T->base = (int32_t *)p;
     p += s_base;
-    T->sorted_pix = (uint16_t *)p;
+    T->keys = (int32_t *)p;
+    p += s_keys;
+    T->sorted_pix = f32 ? nullptr : (uint16_t *)p;
     p += s_pix;
-    T->sorted_ab = (double2 *)p;
+    T->sorted_ab = f32 ? nullptr : (double2 *)p;
+    T->sorted32 = f32 ? (RecordF32 *)p : nullptr;
   }
   return total;
 }
@@ -1405,6 +1480,13 @@ int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
       !a->wavelength_edges || !a->image || !a->moment)
     return fail(OPTK_EINVAL, "missing array");
   if (!(a->x_hi > a->x_lo) || !(a->y_hi > a->y_lo)) return fail(OPTK_EINVAL, "empty pixel range");
+  // the device bin search is exact while num * (|lo| + |hi|) / (hi - lo) < 2^48
+  auto narrow = [](double lo, double hi, int num) {
+    return !((double)num * (fabs(lo) + fabs(hi)) < 281474976710656.0 * (hi - lo));
+  };
+  if (narrow(a->x_lo, a->x_hi, a->nx) || narrow(a->y_lo, a->y_hi, a->ny))
+    return fail(OPTK_EINVAL, "pixel range too narrow for its distance from the origin "
+                             "(num * (|lo| + |hi|) / (hi - lo) must be below 2^48)");
   if (a->precision != OPTK_F64 && a->precision != OPTK_F32) return fail(OPTK_EINVAL, "bad precision");
   if (n == 0) return OPTK_OK;
   CollectParams C;
@@ -1434,8 +1516,14 @@ int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
   C.image = a->image;
   C.moment = a->moment;
   C.n = n;
+  // numpy.linspace's step, as the edges are formed (e_j = j * step + lo)
+  C.x_step = (C.x_hi - C.x_lo) / (double)C.nx;
+  C.y_step = (C.y_hi - C.y_lo) / (double)C.ny;
+  C.x_inv = 1.0 / C.x_step;
+  C.y_inv = 1.0 / C.y_step;
+  const bool f32 = a->precision == OPTK_F32;
   TiledParams T;
-  const uint64_t need = tiled_layout(a->n_wavelength, a->nx, a->ny, n, a->workspace, &T);
+  const uint64_t need = tiled_layout(a->n_wavelength, a->nx, a->ny, n, f32, a->workspace, &T);
   hipStream_t s = (hipStream_t)hip_stream;
   if (need == 0 || !a->workspace || a->workspace_bytes < need) {
     hipLaunchKernelGGL(collect_kernel, grid_for(n), dim3(256), 0, s, C);
@@ -1444,18 +1532,26 @@ int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
   }
   T.C = C;
   const size_t lds = sizeof(int32_t) * (size_t)T.num_tiles;
-  hipLaunchKernelGGL(collect_count, dim3((unsigned)T.blocks), dim3(256), lds, s, T);
+  if (f32)
+    hipLaunchKernelGGL(collect_count<true>, dim3((unsigned)T.blocks), dim3(CT_THREADS), lds, s, T);
+  else
+    hipLaunchKernelGGL(collect_count<false>, dim3((unsigned)T.blocks), dim3(CT_THREADS), lds, s, T);
   hipLaunchKernelGGL(collect_offsets, dim3((unsigned)((T.num_tiles + 3) / 4)), dim3(256), 0, s, T);
   hipLaunchKernelGGL(collect_base, dim3(1), dim3(1024), 0, s, T);
-  hipLaunchKernelGGL(collect_scatter, dim3((unsigned)T.blocks), dim3(256), lds, s, T);
-  hipLaunchKernelGGL(collect_tiles, dim3((unsigned)T.num_tiles), dim3(256), 0, s, T);
+  if (f32) {
+    hipLaunchKernelGGL(collect_scatter<true>, dim3((unsigned)T.blocks), dim3(CT_THREADS), lds, s, T);
+    hipLaunchKernelGGL(collect_tiles<true>, dim3((unsigned)T.num_tiles), dim3(256), 0, s, T);
+  } else {
+    hipLaunchKernelGGL(collect_scatter<false>, dim3((unsigned)T.blocks), dim3(CT_THREADS), lds, s, T);
+    hipLaunchKernelGGL(collect_tiles<false>, dim3((unsigned)T.num_tiles), dim3(256), 0, s, T);
+  }
   OPTK_HIP_CHECK(hipGetLastError());
   return OPTK_OK;
 }
 
 uint64_t optk_collect_workspace(const optk_collect_args *a, int64_t n) {
   if (!a || n <= 0 || a->n_wavelength < 1 || a->nx < 1 || a->ny < 1) return 0;
-  return tiled_layout(a->n_wavelength, a->nx, a->ny, n, nullptr, nullptr);
+  return tiled_layout(a->n_wavelength, a->nx, a->ny, n, a->precision == OPTK_F32, nullptr, nullptr);
 }
 
 }  // extern "C"

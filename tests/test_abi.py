@@ -133,3 +133,31 @@ def test_pupil_and_expose_arguments_are_rejected_without_a_device():
     # zero-size calls succeed without touching a device
     e.scale = 1.0
     assert L.optk_expose(ctypes.byref(e), 0, None) == 0
+
+
+def test_collect_arguments_are_rejected_without_a_device():
+    from optika_amd import _lib
+
+    if not _lib.library_path().exists():
+        pytest.skip("liboptk.so not built")
+    L = _lib.lib()
+    assert L.optk_collect(None, 1, None) == -1
+    a = _lib.CollectArgs()
+    buf = (ctypes.c_double * 4)()
+    p = ctypes.addressof(buf)
+    a.position[0] = a.position[1] = p
+    a.direction_z = a.wavelength = a.intensity = a.wavelength_edges = p
+    a.image = a.moment = p
+    a.n_wavelength, a.nx, a.ny, a.precision = 1, 2048, 2048, _lib.F64
+    a.x_lo, a.x_hi, a.y_lo, a.y_hi = -15.36, 15.36, -15.36, 15.36
+    # an empty range, then a range so narrow for its offset from the origin
+    # that the device bin search could not guarantee numpy's bins
+    a.x_hi = a.x_lo
+    assert L.optk_collect(ctypes.byref(a), 4, None) == -1
+    assert b"empty pixel range" in L.optk_last_error()
+    a.x_lo, a.x_hi = 1e6, 1e6 + 1e-6
+    assert L.optk_collect(ctypes.byref(a), 4, None) == -1
+    assert b"too narrow" in L.optk_last_error()
+    # zero rays: accepted without touching a device
+    a.x_lo, a.x_hi = -15.36, 15.36
+    assert L.optk_collect(ctypes.byref(a), 0, None) == 0

@@ -308,18 +308,21 @@ def test_sensor_collect_point_sources():
     np.testing.assert_allclose(cosine.cpu().numpy(), ref_cos, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
 @pytest.mark.parametrize("n,edges", [(1 << 20, [400.0, 500.0, 600.0, 700.0]),
-                                     (40_000, [400.0, 700.0]), (3, [400.0, 700.0])])
-def test_collect_tiled_equals_atomic_path(n, edges):
+                                     (40_000, [400.0, 700.0]), (32_769, [400.0, 700.0]),
+                                     (3, [400.0, 700.0])])
+def test_collect_tiled_equals_atomic_path(n, edges, precision):
     """The tiled counting-sort binning and the global-atomic binning give the
-    same image (rounding of the order of additions only), also for ragged
-    ray counts that leave the last block's chunk partly empty."""
+    same image (rounding of the order of additions only), for float64 and
+    float32 rays, also for ragged ray counts that leave the last block's
+    chunk (32768 rays) nearly or partly empty."""
     from optika_amd import _collect
 
     g = workloads.newtonian_grid_random(n, seed=31)
     s = workloads.newtonian()
-    print(f"case: collect tiled vs atomic n={n}", flush=True)
-    out = gpu_rayfunction(s, g)
+    print(f"case: collect tiled vs atomic n={n} {precision}", flush=True)
+    out = gpu_rayfunction(s, g, precision=precision)
     hw = 15e-3 * 2048 / 2
     a, ma = _collect.histogram(out, edges, (-hw, hw), (-hw, hw), 2048, 2048, tiled=True)
     b, mb = _collect.histogram(out, edges, (-hw, hw), (-hw, hw), 2048, 2048, tiled=False)
