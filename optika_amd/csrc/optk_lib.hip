@@ -654,8 +654,9 @@ __device__ __forceinline__ int edges_bin(double x, const double *e, int num) {
 //            base[tile] + offsets[tile][b] + rank: with CT_CHUNK rays per
 //            block a tile receives runs of ~CT_CHUNK / tiles records, whole
 //            cache lines for a 2048^2 image
-//   tiles:   block t sums tile t's rays with LDS float64 atomics and adds
-//            its pixels to the image (one block owns each pixel)
+//   tiles:   blocks sum a tile's rays with LDS float64 atomics and add its
+//            pixels to the image (one block per tile, or CT_SPLIT blocks and
+//            float64 atomics for a tile holding many rays)
 // Records: float32 rays pack (pixel, weight, cosine) into one 16-byte
 // store; float64 rays keep a 2-byte pixel array beside (weight, cosine).
 // The moment weight x cosine is formed in float64 in the tiles kernel, as
@@ -826,13 +827,27 @@ __global__ __launch_bounds__(CT_THREADS) void collect_scatter(const TiledParams 
   }
 }
 
+// Tiles kernel: CT_SPLIT blocks per tile (blockIdx.x = tile * CT_SPLIT +
+// part).  A tile with at most CT_SPLIT_MIN rays is summed by part 0 alone,
+// which adds its pixels to the image with plain read-modify-writes; a
+// fuller tile (images concentrate the rays in a few tiles) is split into
+// CT_SPLIT equal record ranges whose partial sums are added with float64
+// atomics.  Every pixel of a tile is thus written one way only.
+constexpr int CT_SPLIT = 4;
+constexpr int32_t CT_SPLIT_MIN = 4096;
+
 template <bool F32>
 __global__ __launch_bounds__(256) void collect_tiles(const TiledParams T) {
   // launched with 256 threads
   __shared__ double acc_a[CT_PIX];
   __shared__ double acc_b[CT_PIX];
-  const int tile = blockIdx.x;
-  const int32_t lo = T.base[tile], hi = T.base[tile + 1];
+  const int tile = blockIdx.x / CT_SPLIT, part = blockIdx.x % CT_SPLIT;
+  const int32_t t_lo = T.base[tile], t_hi = T.base[tile + 1];
+  const bool split = t_hi - t_lo > CT_SPLIT_MIN;
+  if (!split && part != 0) return;  // whole block
+  const int32_t len = t_hi - t_lo;
+  const int32_t lo = split ? t_lo + (int32_t)((int64_t)len * part / CT_SPLIT) : t_lo;
+  const int32_t hi = split ? t_lo + (int32_t)((int64_t)len * (part + 1) / CT_SPLIT) : t_hi;
   if (hi == lo) return;  // whole block: nothing to add
   for (int p = threadIdx.x; p < CT_PIX; p += blockDim.x) {
     acc_a[p] = 0.0;
@@ -875,8 +890,15 @@ __global__ __launch_bounds__(256) void collect_tiles(const TiledParams T) {
     const int gx = tx * CT_SIDE + p / CT_SIDE, gy = ty * CT_SIDE + p % CT_SIDE;
     if (gx >= T.C.nx || gy >= T.C.ny) continue;
     const int64_t k = ((int64_t)w * T.C.nx + gx) * T.C.ny + gy;
-    T.C.image[k] += acc_a[p];
-    T.C.moment[k] += acc_b[p];
+    if (split) {
+      if (acc_a[p] != 0.0 || acc_b[p] != 0.0) {
+        unsafeAtomicAdd(&T.C.image[k], acc_a[p]);
+        unsafeAtomicAdd(&T.C.moment[k], acc_b[p]);
+      }
+    } else {
+      T.C.image[k] += acc_a[p];
+      T.C.moment[k] += acc_b[p];
+    }
   }
 }
 
@@ -1540,10 +1562,10 @@ int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
   hipLaunchKernelGGL(collect_base, dim3(1), dim3(1024), 0, s, T);
   if (f32) {
     hipLaunchKernelGGL(collect_scatter<true>, dim3((unsigned)T.blocks), dim3(CT_THREADS), lds, s, T);
-    hipLaunchKernelGGL(collect_tiles<true>, dim3((unsigned)T.num_tiles), dim3(256), 0, s, T);
+    hipLaunchKernelGGL(collect_tiles<true>, dim3((unsigned)(T.num_tiles * CT_SPLIT)), dim3(256), 0, s, T);
   } else {
     hipLaunchKernelGGL(collect_scatter<false>, dim3((unsigned)T.blocks), dim3(CT_THREADS), lds, s, T);
-    hipLaunchKernelGGL(collect_tiles<false>, dim3((unsigned)T.num_tiles), dim3(256), 0, s, T);
+    hipLaunchKernelGGL(collect_tiles<false>, dim3((unsigned)(T.num_tiles * CT_SPLIT)), dim3(256), 0, s, T);
   }
   OPTK_HIP_CHECK(hipGetLastError());
   return OPTK_OK;
