@@ -329,7 +329,10 @@ int32_t optk_eval(const optk_system *sys, int32_t op, const optk_eval_args *args
  * numpy.linspace(lo, hi, num + 1), right edge of the last bin inclusive.
  * Rays with unvignetted == 0 or outside every bin are skipped.
  * image and moment are float64 device arrays of n_wavelength * nx * ny,
- * accumulated into (not cleared).  moment gets intensity * d_z. */
+ * accumulated into (not cleared).  moment gets intensity * d_z.
+ * A pixel range must satisfy num * (|lo| + |hi|) / (hi - lo) < 2^48 (the
+ * device bin search is exact there); narrower ones are rejected (EINVAL).
+ * The workspace size depends on precision (set it before the query). */
 typedef struct optk_collect_args {
   const void *position[2];
   const void *direction_z;
@@ -346,7 +349,8 @@ typedef struct optk_collect_args {
   double *moment;
   /* optional device workspace of optk_collect_workspace() bytes: with it the
    * rays are counting-sorted into 32 x 32-pixel tiles and each tile is
-   * summed in LDS (no global atomics); without it (or when the image has
+   * summed in LDS (tiles holding many rays by several blocks, which add
+ * with float64 atomics); without it (or when the image has
    * more than 16384 tiles, or for 2^31 rays or more) every ray adds to the
    * image with float64 global atomics */
   void *workspace;
