@@ -50,13 +50,13 @@ BYTES_PER_RAY_F64 = 97
 # HBM bytes per launch of trace_grid_f64 at 2^24 rays, from rocprofv3 --pmc:
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 256 MiB device copy);
 # the runs and commands are in profiles/trace_grid_f64_newtonian_2p24.md
-TRAFFIC_BYTES_F64_2P24 = (327873 * 2.0 + 934660 * 1.0) * 1024
+TRAFFIC_BYTES_F64_2P24 = (327867 * 2.0 + 934303 * 1.0) * 1024
 
 # float32 form: in 5 x 4 B, out 7 x 4 B + 1 B mask
 BYTES_PER_RAY_F32 = 49
 # HBM bytes per launch of trace_grid_f32 at 2^24 rays (rocprofv3 --pmc, the
 # same scaling; profiles/trace_grid_f32_newtonian_2p24.md)
-TRAFFIC_BYTES_F32_2P24 = (168030 * 2.0 + 487487 * 1.0) * 1024
+TRAFFIC_BYTES_F32_2P24 = (163964 * 2.0 + 475159 * 1.0) * 1024
 
 PIXEL_MM = 0.015
 
