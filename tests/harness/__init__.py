@@ -105,6 +105,21 @@ def trace_grid(system, grid, sensor_frame: bool = True):
     return res
 
 
+def fast_skip_first(system, sensor_frame: bool = True) -> bool:
+    """Whether the float32 form's grid kernels elide the system's surface 0
+    (a pass-through plane the grid rays start on)."""
+    from optika_amd import _compile
+
+    obj_t = system.object.transformation if system.object is not None else None
+    post = system.sensor.transformation if sensor_frame else None
+    rec = _compile.build_records(system.surfaces_all, pre=(obj_t, system.transformation),
+                                 pre_is_inverse=(0, 1), post=post)
+    d, _ = _desc(rec)
+    L = lib()
+    L.host_fast_skip_first.restype = ctypes.c_int
+    return bool(L.host_fast_skip_first(ctypes.byref(d)))
+
+
 def trace_grid_f32(system, grid, sensor_frame: bool = True):
     """Host run of the float32 form (float32 inputs: the grid is rounded to
     float32 first).  Same result layout as :func:`trace_grid`."""

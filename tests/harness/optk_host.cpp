@@ -175,6 +175,17 @@ int host_trace_grid_f32(const optk_system_desc *d, const float *fx, const float 
   return 0;
 }
 
+// 1 when the float32 form's grid kernels elide surface 0 of the system
+// (FastSpec::skip_first), else 0.
+int host_fast_skip_first(const optk_system_desc *d) {
+  std::vector<optk_surface> surf = derived(d);
+  std::vector<FastSurface> fast(d->num_surfaces);
+  FastSpec spec;
+  std::memset(&spec, 0, sizeof(spec));
+  fast_build(surf.data(), d->num_surfaces, d->pre, d->pre_is_inverse, d->post, fast.data(), spec);
+  return spec.skip_first;
+}
+
 // The float32 form over a surface sequence: float32 rays in (system frame),
 // float32 rays out.  in[0..9] / out[0..9] = px py pz dx dy dz wavelength
 // intensity attenuation index.

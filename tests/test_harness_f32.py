@@ -103,3 +103,24 @@ def test_operator_zoo_f32_host(name):
         ia, ib = a["intensity"], b["intensity"].astype(np.float64)
         fin = np.isfinite(ia) & (np.abs(ia) > 1e-6)
         assert np.all(np.abs(ib[fin] - ia[fin]) <= 1e-4 * np.abs(ia[fin]) + 1e-7)
+
+
+def test_pass_through_object_surface_elision():
+    """The float32 grid kernels skip surface 0 only when it leaves every grid
+    ray unchanged: the Newtonian's and the spectrograph's object planes
+    qualify; an object surface with an aperture (it vignettes) or with a sag
+    (the rays start on its vertex plane, not on it) does not, and the
+    float32 form then still agrees with the float64 form."""
+    from harness import fast_skip_first
+    from optika_amd import apertures, sags
+
+    assert fast_skip_first(workloads.newtonian())
+    assert fast_skip_first(workloads.spectrograph())
+    g = _f32(workloads.newtonian_grid_random(20000, seed=4))
+    for label, obj in (
+            ("aperture", surfaces.Surface(aperture=apertures.CircularAperture(10.0))),
+            ("sag", surfaces.Surface(sag=sags.SphericalSag(radius=5000.0)))):
+        s = workloads.newtonian()
+        s.object = obj
+        assert not fast_skip_first(s), label
+        _check(_spot_stats(trace_grid(s, g)[0], trace_grid_f32(s, g)[0]), f"object {label}")
