@@ -80,7 +80,13 @@ enum {
   OPTK_MATERIAL_MEASURED_MIRROR = 2,
   OPTK_MATERIAL_GLASS = 3,
   OPTK_MATERIAL_MULTILAYER_MIRROR = 4,
-  OPTK_MATERIAL_MULTILAYER_FILM = 5
+  OPTK_MATERIAL_MULTILAYER_FILM = 5,
+  /* BackIlluminatedSiliconSensorMaterial (optika/sensors/materials/
+   * _materials.py:2247-2282): efficiency = the s/p average of the
+   * Beer-Lambert absorbance in the silicon (:2231-2243, :200-376); stack =
+   * the oxide layer over the silicon substrate layer, material_p[0] = the
+   * substrate thickness (nm); index and attenuation pass through */
+  OPTK_MATERIAL_SILICON_SENSOR = 6
 };
 enum {
   OPTK_RULINGS_NONE = 0,
@@ -137,7 +143,8 @@ typedef struct optk_surface {
   /* material */
   int32_t material_kind;
   int32_t is_mirror;
-  double material_p[6];       /* glass: b1, b2, b3, c1, c2, c3 (c in nm^2) */
+  double material_p[6];       /* glass: b1, b2, b3, c1, c2, c3 (c in nm^2) |
+                                 silicon sensor: substrate thickness (nm) */
   int32_t material_table;     /* measured mirror: index into tables[] (wavelength
                                  row then efficiency row), -1 if none */
   int32_t material_table_len;

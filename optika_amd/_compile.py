@@ -184,6 +184,12 @@ def _fill_surface(rec: _lib.Surface, surface, tables: _Tables, num_paths: int) -
         rec.material_table_len = w.size
     elif isinstance(material, _mat.AbstractMultilayerMaterial):
         rec.stack = tables.stack(material.layers, getattr(material, "substrate", None))
+    elif material.kind == _mat.MATERIAL_SILICON_SENSOR:
+        # the oxide over the silicon substrate, and the substrate thickness
+        # of the Beer-Lambert decay
+        oxide, substrate = material._layers()
+        rec.stack = tables.stack([oxide], substrate)
+        rec.material_p[0] = float(material.thickness_substrate)
 
     # rulings
     rec.rulings_kind = 0

@@ -839,8 +839,11 @@ OPTK_HD Cf ml_layer_n(const optk_layer &L, const StackView &sv, const float *tab
 // Reflectance (mirror) or transmittance (film) averaged over s and p, or a
 // negative value when the float32 evaluation does not apply (grazing
 // incidence, the validity guard, more than 4 layers in a periodic block).
+// im_q_sub (optional, film only): Im(n_sub cos_sub) of the wave refracted
+// into the substrate
 OPTK_HD float fast_multilayer(const StackView &sv, const float *tables32, int stack_index,
-                              float wavelength, Cf cos_amb, Cf n_amb, bool film) {
+                              float wavelength, Cf cos_amb, Cf n_amb, bool film,
+                              float *im_q_sub = nullptr) {
   if (fabsf(cos_amb.re) < OPTK_GRAZING_COS) return -1.0f;
   const optk_stack &stk = sv.stacks[stack_index];
   // Snell invariant: kappa^2 = n_amb^2 (1 - cos_amb^2)
@@ -913,6 +916,7 @@ OPTK_HD float fast_multilayer(const StackView &sv, const float *tables32, int st
   float tp = 1.0f / fabs2(m.p.a);
   float gs = fcdiv(msub.qs, amb.qs).re;
   float gp = fcdiv(msub.qp, amb.qp).re;
+  if (im_q_sub) *im_q_sub = msub.qs.im;
   return 0.5f * (ts * gs + tp * gp);
 }
 
@@ -931,6 +935,20 @@ float multilayer_fallback64(const StackView &sv, int stack, float wavelength, do
   MultilayerRT<double> rt = multilayer_efficiency<double>(
       sv, stack, (double)wavelength, C<double>{cos_theta, 0.0}, C<double>{(double)index, k64}, film);
   return (float)(film ? (rt.T[0] + rt.T[1]) / 2.0 : (rt.R[0] + rt.R[1]) / 2.0);
+}
+
+// the silicon sensor's absorbance in float64, for the same rays
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __attribute__((noinline))
+#else
+inline
+#endif
+float silicon_fallback64(const StackView &sv, int stack, double thickness, float wavelength,
+                         double cos_theta, float index, float attenuation) {
+  double k64 = (double)attenuation * (double)wavelength / (4.0 * M_PI) * 1e-6;
+  return (float)silicon_absorbance_average(sv, stack, thickness, (double)wavelength,
+                                           C<double>{cos_theta, 0.0},
+                                           C<double>{(double)index, k64});
 }
 
 template <int F>
@@ -956,6 +974,21 @@ OPTK_HD float fast_material_efficiency(const optk_surface &s, const StackView &s
       eff = multilayer_fallback64(sv, s.stack, wavelength, c64, index, attenuation, film);
     }
     return eff;
+  }
+  case OPTK_MATERIAL_SILICON_SENSOR: {  // _materials.py:2231-2243
+    if (!(F & FEAT_MULTILAYER)) return 1.0f;
+    float k = attenuation * wavelength / (4.0f * 3.14159265358979f) * 1e-6f;
+    float cosv = (float)(-(a.x * normal.x + a.y * normal.y + a.z * normal.z));
+    float im_q = 0.0f;
+    float t = fast_multilayer(sv, tables32, s.stack, wavelength, Cf{cosv, 0.0f}, Cf{index, k},
+                              true, &im_q);
+    if (t < 0.0f) {
+      double c64 = -(a.x * normal.x + a.y * normal.y + a.z * normal.z);
+      return silicon_fallback64(sv, s.stack, s.material_p[0], wavelength, c64, index, attenuation);
+    }
+    // the decay 1 - exp(-alpha d) is the same for both polarizations
+    float absorption = 4.0f * 3.14159265358979f * im_q / wavelength;
+    return t - t * expf(-absorption * (float)s.material_p[0]);
   }
   default:
     return 1.0f;

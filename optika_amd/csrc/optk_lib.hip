@@ -71,7 +71,8 @@ static int32_t features_of(const optk_system_desc *d) {
     if (s.sag_kind == OPTK_SAG_TOROID) f |= FEAT_TOROID;
     if (s.rulings_kind != OPTK_RULINGS_NONE) f |= FEAT_RULINGS;
     if (s.material_kind == OPTK_MATERIAL_MULTILAYER_MIRROR ||
-        s.material_kind == OPTK_MATERIAL_MULTILAYER_FILM)
+        s.material_kind == OPTK_MATERIAL_MULTILAYER_FILM ||
+        s.material_kind == OPTK_MATERIAL_SILICON_SENSOR)
       f |= FEAT_MULTILAYER;
   }
   return f;
@@ -1026,13 +1027,14 @@ int32_t validate_desc(const optk_system_desc *d) {
         !valid_affine(s.aperture_transform))
       return fail(OPTK_EINVAL, "bad transformation kind");
     if (s.sag_kind < OPTK_SAG_NONE || s.sag_kind > OPTK_SAG_TOROID) return fail(OPTK_EINVAL, "bad sag kind");
-    if (s.material_kind < OPTK_MATERIAL_VACUUM || s.material_kind > OPTK_MATERIAL_MULTILAYER_FILM)
+    if (s.material_kind < OPTK_MATERIAL_VACUUM || s.material_kind > OPTK_MATERIAL_SILICON_SENSOR)
       return fail(OPTK_EINVAL, "bad material kind");
     if (s.material_kind == OPTK_MATERIAL_MEASURED_MIRROR &&
         (s.material_table < 0 || s.material_table_len < 2 ||
          (int64_t)s.material_table + 2LL * s.material_table_len > d->num_tables))
       return fail(OPTK_EINVAL, "measured-mirror table out of range");
-    if ((s.material_kind == OPTK_MATERIAL_MULTILAYER_MIRROR || s.material_kind == OPTK_MATERIAL_MULTILAYER_FILM) &&
+    if ((s.material_kind == OPTK_MATERIAL_MULTILAYER_MIRROR || s.material_kind == OPTK_MATERIAL_MULTILAYER_FILM ||
+         s.material_kind == OPTK_MATERIAL_SILICON_SENSOR) &&
         (s.stack < 0 || s.stack >= d->num_stacks))
       return fail(OPTK_EINVAL, "multilayer stack index out of range");
     if (s.rulings_kind < OPTK_RULINGS_NONE || s.rulings_kind > OPTK_RULINGS_RECTANGULAR)

@@ -446,6 +446,28 @@ OPTK_HD double material_attenuation(const optk_surface &s, double attenuation) {
              : attenuation;
 }
 
+// BackIlluminatedSiliconSensorMaterial.efficiency (optika/sensors/materials/
+// _materials.py:2231-2243): the s/p average of absorbance(...,
+// method="Beer-Lambert") (_materials.py:200-376) -- the transmittance
+// through the oxide into the silicon times 1 - exp(-alpha d), with
+// alpha = 4 pi Im(n_sub cos_sub) / wavelength the absorption per unit
+// perpendicular depth of the refracted wave (absorption_effective,
+// _materials.py:55-93), in the reference's operation order.
+OPTK_HD double silicon_absorbance_average(const StackView &sv, int stack, double thickness,
+                                          double wavelength, C<double> direction,
+                                          C<double> n) {
+  MultilayerRT<double> rt = multilayer_efficiency<double>(sv, stack, wavelength, direction, n, true);
+  const optk_stack &stk = sv.stacks[stack];
+  C<double> n_sub = {1.0, 0.0};
+  if (stk.substrate_layer >= 0) n_sub = layer_index<double>(sv.layers[stk.substrate_layer], sv, wavelength);
+  C<double> d_sub = snell_scalar(direction, n, n_sub);
+  double absorption = 4.0 * M_PI * cmul(n_sub, d_sub).im / wavelength;
+  double decay = exp(-absorption * thickness);
+  double s_abs = rt.T[0] - rt.T[0] * decay;
+  double p_abs = rt.T[1] - rt.T[1] * decay;
+  return (s_abs + p_abs) / 2.0;
+}
+
 template <int F = FEAT_ALL>
 OPTK_HD double material_efficiency(const optk_surface &s, const StackView &sv, double wavelength,
                                    V3<double> a, double index, double attenuation,
@@ -465,6 +487,14 @@ OPTK_HD double material_efficiency(const optk_surface &s, const StackView &sv, d
     MultilayerRT<double> rt =
         multilayer_efficiency<double>(sv, s.stack, wavelength, C<double>{cosv, 0.0}, n, film);
     return film ? (rt.T[0] + rt.T[1]) / 2.0 : (rt.R[0] + rt.R[1]) / 2.0;
+  }
+  case OPTK_MATERIAL_SILICON_SENSOR: {  // _materials.py:2231-2243
+    if (!(F & FEAT_MULTILAYER)) return 1.0;
+    double k = attenuation * wavelength / (4.0 * M_PI) * 1e-6;
+    C<double> n = {index, k};
+    double cosv = -dot3(a, normal);
+    return silicon_absorbance_average(sv, s.stack, s.material_p[0], wavelength,
+                                      C<double>{cosv, 0.0}, n);
   }
   default:
     return 1.0;

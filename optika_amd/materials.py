@@ -50,6 +50,7 @@ MATERIAL_MEASURED_MIRROR = 2
 MATERIAL_GLASS = 3
 MATERIAL_MULTILAYER_MIRROR = 4
 MATERIAL_MULTILAYER_FILM = 5
+MATERIAL_SILICON_SENSOR = 6
 
 PROFILE_NONE = 0
 PROFILE_ERF = 1

@@ -23,7 +23,9 @@ from .surfaces import AbstractSurface
 from .transformations import AbstractTransformation
 from .vectors import Cartesian2dVectorArray, as_vector2
 
-__all__ = ["IdealSensorMaterial", "AbstractImagingSensor", "ImagingSensor"]
+__all__ = ["IdealSensorMaterial", "BackIlluminatedSiliconSensorMaterial", "AbstractImagingSensor",
+           "ImagingSensor", "absorption_effective", "transmittance", "absorbance",
+           "charge_collection_efficiency", "quantum_efficiency_effective"]
 
 
 @dataclasses.dataclass(eq=False)
@@ -114,6 +116,8 @@ class ImagingSensor(AbstractImagingSensor):
 
 THICKNESS_OXIDE = 5.0        # nm (50 A, Stern 1994; _stern_1994.py:3)
 THICKNESS_SUBSTRATE = 7000.0  # nm (7 um, _stern_1994.py:4)
+THICKNESS_IMPLANT = 231.7     # nm (2317 A, _stern_1994.py:5)
+CCE_BACKSURFACE = 0.21        # _stern_1994.py:7
 
 
 def _np(v):
@@ -176,3 +180,117 @@ def absorbance(wavelength, direction=1, n=1, thickness_oxide=THICKNESS_OXIDE,
         return np.real(tp - tp * decay)
 
     return PolarizationVectorArray(absorbed(t.s), absorbed(t.p))
+
+
+def charge_collection_efficiency(absorption, thickness_implant=THICKNESS_IMPLANT,
+                                 cce_backsurface=CCE_BACKSURFACE):
+    """Average charge collection efficiency of the piecewise-linear
+    differential profile (``_materials.py:377-497``):
+    ``eta_0 + (1 - eta_0) (1 - exp(-alpha W)) / (alpha W)`` with the
+    absorption per unit perpendicular depth ``alpha`` (1/nm) and the implant
+    thickness ``W`` (nm)."""
+    z0 = _np(absorption) * thickness_implant
+    exp_z0 = np.exp(-z0)
+    term_1 = cce_backsurface
+    term_2 = ((1 - cce_backsurface) / z0) * (1 - exp_z0)
+    return term_1 + term_2
+
+
+def quantum_efficiency_effective(wavelength, direction=1, n=1, thickness_oxide=THICKNESS_OXIDE,
+                                 thickness_implant=THICKNESS_IMPLANT,
+                                 thickness_substrate=THICKNESS_SUBSTRATE,
+                                 cce_backsurface=CCE_BACKSURFACE, chemical_oxide="SiO2",
+                                 chemical_substrate="Si", roughness_oxide=0.0,
+                                 roughness_substrate=0.0):
+    """Effective quantum efficiency ``absorbance.average * CCE``
+    (``_materials.py:500-687``)."""
+    from . import _ops
+    from .chemicals import Chemical
+
+    a = absorbance(wavelength, direction, n, thickness_oxide, thickness_substrate,
+                   chemical_oxide, chemical_substrate, roughness_oxide, roughness_substrate)
+    chem = chemical_substrate if not isinstance(chemical_substrate, str) else Chemical(chemical_substrate)
+    n_sub = chem.n(_np(wavelength).astype(np.float64))
+    d_sub = _ops.snells_law_scalar(direction, n, n_sub)
+    cce = charge_collection_efficiency(absorption_effective(wavelength, n_sub, d_sub),
+                                       thickness_implant, cce_backsurface)
+    return _np(a.average) * cce
+
+
+def _cos_incidence(direction, normal):
+    """``-direction @ normal``, with the reference's defaults (normal
+    incidence on a surface facing -z)."""
+    from .vectors import Cartesian3dVectorArray, as_vector3
+
+    if direction is None:
+        direction = Cartesian3dVectorArray(0.0, 0.0, 1.0)
+        normal = Cartesian3dVectorArray(0.0, 0.0, -1.0)
+    d, nrm = as_vector3(direction), as_vector3(normal)
+    return -(_np(d.x) * _np(nrm.x) + _np(d.y) * _np(nrm.y) + _np(d.z) * _np(nrm.z))
+
+
+@dataclasses.dataclass(eq=False)
+class BackIlluminatedSiliconSensorMaterial(_materials.AbstractMaterial):
+    """The light-sensitive material of a back-illuminated silicon sensor
+    (``optika/sensors/materials/_materials.py:1723-2282``), by the method of
+    Stern 1994.  Thicknesses and roughnesses in nm.  In a trace its
+    efficiency -- the s/p average of the Beer-Lambert absorbance in the
+    silicon (``:2231-2243``) -- is evaluated per ray inside the trace
+    kernels (the oxide over the substrate is a multilayer stack of the
+    system); index and attenuation pass through and it is not a mirror.
+    ``depletion`` and ``eqe_measured`` are kept for the interface; the
+    charge-diffusion and measured-EQE paths are not built."""
+
+    thickness_oxide: float = 0.0
+    thickness_implant: float = 0.0
+    thickness_substrate: float = 0.0
+    roughness_oxide: float = 0.0
+    roughness_substrate: float = 0.0
+    cce_backsurface: float = 0.0
+    depletion: Any = None
+    eqe_measured: Any = None
+    chemical: str = "Si"
+    chemical_oxide: str = "SiO2"
+
+    def __post_init__(self):
+        self.kind = _materials.MATERIAL_SILICON_SENSOR
+
+    def _layers(self):
+        """(oxide layer, substrate layer) of the multilayer stack."""
+        oxide = _materials.Layer(chemical=self.chemical_oxide, thickness=float(self.thickness_oxide),
+                                 interface=_materials.profiles.ErfInterfaceProfile(
+                                     width=float(self.roughness_oxide)))
+        substrate = _materials.Layer(chemical=self.chemical,
+                                     thickness=float(self.thickness_substrate),
+                                     interface=_materials.profiles.ErfInterfaceProfile(
+                                         width=float(self.roughness_substrate)))
+        return oxide, substrate
+
+    def transmittance(self, wavelength, direction=None, n=1, normal=None):
+        """``_materials.py:1791-1830``."""
+        return transmittance(wavelength, _cos_incidence(direction, normal), n,
+                             self.thickness_oxide, self.thickness_substrate, self.chemical_oxide,
+                             self.chemical, self.roughness_oxide, self.roughness_substrate)
+
+    def absorbance(self, wavelength, direction=None, n=1, normal=None):
+        """``_materials.py:1832-1871``."""
+        return absorbance(wavelength, _cos_incidence(direction, normal), n, self.thickness_oxide,
+                          self.thickness_substrate, self.chemical_oxide, self.chemical,
+                          self.roughness_oxide, self.roughness_substrate)
+
+    def charge_collection_efficiency(self, wavelength, direction=None, n=1, normal=None):
+        """``_materials.py:1873-1921``."""
+        from . import _ops
+        from .chemicals import Chemical
+
+        n_sub = Chemical(self.chemical).n(_np(wavelength).astype(np.float64))
+        d_sub = _ops.snells_law_scalar(_cos_incidence(direction, normal), n, n_sub)
+        return charge_collection_efficiency(absorption_effective(wavelength, n_sub, d_sub),
+                                            self.thickness_implant, self.cce_backsurface)
+
+    def quantum_efficiency_effective(self, wavelength, direction=None, n=1, normal=None):
+        """``_materials.py:1923-1966``."""
+        return quantum_efficiency_effective(
+            wavelength, _cos_incidence(direction, normal), n, self.thickness_oxide,
+            self.thickness_implant, self.thickness_substrate, self.cce_backsurface,
+            self.chemical_oxide, self.chemical, self.roughness_oxide, self.roughness_substrate)

@@ -866,6 +866,30 @@ def material_efficiency(material, rays, normal):
         if name == "MultilayerMirror":
             return (R[0] + R[1]) / 2
         return (T[0] + T[1]) / 2
+    if name == "BackIlluminatedSiliconSensorMaterial":
+        # efficiency (optika/sensors/materials/_materials.py:2231-2243) = the
+        # s/p average of absorbance(method="Beer-Lambert") (:200-376): the
+        # transmittance through the oxide into the silicon (:96-197) times
+        # 1 - exp(-alpha d), alpha from absorption_effective (:55-93)
+        from optika_amd.materials import Layer, profiles
+
+        wavelength = rays["wavelength"]
+        k = rays["attenuation"] * wavelength / (4 * np.pi) * 1e-6
+        n = rays["index_refraction"] + k * 1j
+        direction = -_dot(rays["direction"], normal)
+        m = material
+        oxide = Layer(chemical=m.chemical_oxide, thickness=float(m.thickness_oxide),
+                      interface=profiles.ErfInterfaceProfile(width=float(m.roughness_oxide)))
+        substrate = Layer(chemical=m.chemical, thickness=float(m.thickness_substrate),
+                          interface=profiles.ErfInterfaceProfile(width=float(m.roughness_substrate)))
+        _, T = multilayer_efficiency(wavelength, direction, n, [oxide], substrate)
+        n_substrate = _layer_n(_SubstrateLayer(substrate), wavelength)
+        direction_substrate = snells_law_scalar(direction, n, n_substrate)
+        absorption = 4 * np.pi * np.imag(n_substrate * direction_substrate) / wavelength
+        decay = np.exp(-absorption * float(m.thickness_substrate))
+        s_abs = np.real(T[0] - T[0] * decay)
+        p_abs = np.real(T[1] - T[1] * decay)
+        return (s_abs + p_abs) / 2
     raise TypeError(name)
 
 
