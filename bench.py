@@ -415,7 +415,7 @@ def run_surrogate(args):
     """Surrogate of the spectrograph's order -1 rayfunction: train on
     float64-traced rays, calibrate, then time fused inference over --rays
     rays (default 1e9) next to the float64 and float32 traces of the same
-    rays, and judge the error against the float64 trace on a subsample."""
+    rays, and measure the error against the float64 trace on a subsample."""
     import torch
 
     from optika_amd import _trace, surrogate, workloads
