@@ -29,7 +29,7 @@ __all__ = ["collect", "histogram", "expose"]
 def histogram(rays, wavelength_edges, x_range, y_range, num_x, num_y, where=None, tiled=True):
     """Raw device histogram: returns ``(image, moment)`` float64 tensors of shape
     ``(n_wavelength, num_x, num_y)``.  ``tiled``: counting-sort the rays into
-    32 x 32-pixel tiles and sum each tile in LDS (a workspace of ~18 bytes
+    32 x 32-pixel tiles and sum each tile in LDS (a workspace of ~20 bytes
     per ray from the caching allocator) instead of float64 global atomics."""
     dev = _trace.device()
     n = rays.num
@@ -53,10 +53,14 @@ def histogram(rays, wavelength_edges, x_range, y_range, num_x, num_y, where=None
     if where is not None and not (np.ndim(where) == 0 and bool(where)):
         w = torch.as_tensor(where, device=dev).reshape(-1).to(torch.uint8)
         mask = mask & w.expand(n).contiguous()
-    edges = torch.as_tensor(np.asarray(wavelength_edges, dtype=np.float64), device=dev)
-    nw = edges.numel() - 1
-    if nw < 1 or not bool(torch.all(edges[1:] > edges[:-1])):
+    if isinstance(wavelength_edges, torch.Tensor):
+        wavelength_edges = wavelength_edges.detach().cpu().numpy()
+    e_host = np.asarray(wavelength_edges, dtype=np.float64).reshape(-1)
+    nw = e_host.size - 1
+    # checked on the host: a device check would synchronise every call
+    if nw < 1 or not bool(np.all(e_host[1:] > e_host[:-1])):
         raise ValueError("wavelength edges must hold at least two ascending values")
+    edges = torch.as_tensor(e_host, device=dev)
     image = torch.zeros((nw, num_x, num_y), dtype=torch.float64, device=dev)
     moment = torch.zeros_like(image)
     a = _lib.CollectArgs()
