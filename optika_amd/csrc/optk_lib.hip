@@ -828,6 +828,105 @@ __global__ __launch_bounds__(CT_THREADS) void collect_scatter(const TiledParams 
   }
 }
 
+// Float32 rays, images of at most CT_STAGED_TILES tiles: the scatter
+// stages each sub-chunk of CT_SUB rays in LDS in tile order (a local
+// counting sort) and then writes it out, so that one tile's records leave
+// the block as consecutive stores, whole cache lines, instead of one
+// scattered 16-byte store per ray that concurrent blocks complete piecewise.
+//   1. zero the local counts            2. load, rank within the tile (LDS atomic)
+//   3. scan the counts (local offsets)  4. place records and global slots in LDS
+//   5. write out in local order, advance the tiles' cursors
+constexpr int CT_SUB = CT_UNROLL * CT_THREADS;  // rays per sub-chunk
+constexpr int CT_STAGED_TILES = 4096;
+
+__global__ __launch_bounds__(CT_THREADS) void collect_scatter_staged(const TiledParams T) {
+  extern __shared__ int32_t dyn[];
+  int32_t *cursor = dyn;                    // [num_tiles] next global slot of each tile
+  int32_t *loff = dyn + T.num_tiles;        // [num_tiles] local counts, then local offsets
+  int32_t *lcnt = dyn + 2 * T.num_tiles;    // [num_tiles] local counts of the pass
+  __shared__ RecordF32 stage[CT_SUB];
+  __shared__ int32_t dest[CT_SUB];
+  __shared__ int32_t wsum[CT_THREADS / 64];
+  __shared__ int32_t total;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int per = (T.num_tiles + CT_THREADS - 1) / CT_THREADS;  // tiles scanned per thread
+  const int t0 = tid * per;
+  for (int t = tid; t < T.num_tiles; t += CT_THREADS)
+    cursor[t] = T.base[t] + T.counts[(int64_t)blockIdx.x * T.num_tiles + t];
+  const int64_t lo = (int64_t)blockIdx.x * CT_CHUNK;
+  const int64_t hi = lo + CT_CHUNK < T.C.n ? lo + CT_CHUNK : T.C.n;
+  for (int64_t s0 = lo; s0 < hi; s0 += CT_SUB) {
+    for (int t = tid; t < T.num_tiles; t += CT_THREADS) loff[t] = 0;
+    __syncthreads();
+    int32_t key[CT_UNROLL], rank[CT_UNROLL];
+    float a[CT_UNROLL], c[CT_UNROLL];
+#pragma unroll
+    for (int u = 0; u < CT_UNROLL; ++u) {
+      const int64_t i = s0 + (int64_t)u * CT_THREADS + tid;
+      key[u] = -1;
+      a[u] = c[u] = 0.0f;
+      if (i < hi) {
+        key[u] = T.keys[i];
+        a[u] = T.C.intensityf[i];
+        c[u] = T.C.dzf[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CT_UNROLL; ++u) rank[u] = key[u] >= 0 ? atomicAdd(&loff[key[u] >> 10], 1) : 0;
+    __syncthreads();
+    // exclusive scan of loff: per-thread runs of `per` tiles, wave scan, scan of the wave sums
+    int32_t sum = 0;
+    for (int j = 0; j < per; ++j) {
+      const int t = t0 + j;
+      if (t < T.num_tiles) sum += loff[t];
+    }
+    int32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+      const int32_t w = lane < CT_THREADS / 64 ? wsum[lane] : 0;
+      int32_t wi = w;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int32_t v = __shfl_up(wi, off, 64);
+        if (lane >= off) wi += v;
+      }
+      if (lane < CT_THREADS / 64) wsum[lane] = wi - w;
+      if (lane == 63) total = wi;
+    }
+    __syncthreads();
+    int32_t run = wsum[wave] + incl - sum;
+    for (int j = 0; j < per; ++j) {
+      const int t = t0 + j;
+      if (t < T.num_tiles) {
+        const int32_t cnt = loff[t];
+        lcnt[t] = cnt;
+        loff[t] = run;
+        run += cnt;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < CT_UNROLL; ++u) {
+      if (key[u] < 0) continue;
+      const int tile = key[u] >> 10;
+      const int32_t p = loff[tile] + rank[u];
+      stage[p] = RecordF32{(uint32_t)(key[u] & 1023), a[u], c[u], 0u};
+      dest[p] = cursor[tile] + rank[u];
+    }
+    __syncthreads();
+    const int32_t n_sub = total;
+    for (int32_t p = tid; p < n_sub; p += CT_THREADS) T.sorted32[dest[p]] = stage[p];
+    for (int j = 0; j < per; ++j) {
+      const int t = t0 + j;
+      if (t < T.num_tiles) cursor[t] += lcnt[t];
+    }
+  }
+}
+
 // Tiles kernel: CT_SPLIT blocks per tile (blockIdx.x = tile * CT_SPLIT +
 // part).  A tile with at most CT_SPLIT_MIN rays is summed by part 0 alone,
 // which adds its pixels to the image with plain read-modify-writes; a
@@ -1562,7 +1661,10 @@ int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
     hipLaunchKernelGGL(collect_count<false>, dim3((unsigned)T.blocks), dim3(CT_THREADS), lds, s, T);
   hipLaunchKernelGGL(collect_offsets, dim3((unsigned)((T.num_tiles + 3) / 4)), dim3(256), 0, s, T);
   hipLaunchKernelGGL(collect_base, dim3(1), dim3(1024), 0, s, T);
-  if (f32) {
+  if (f32 && T.num_tiles <= CT_STAGED_TILES) {
+    hipLaunchKernelGGL(collect_scatter_staged, dim3((unsigned)T.blocks), dim3(CT_THREADS), 3 * lds, s, T);
+    hipLaunchKernelGGL(collect_tiles<true>, dim3((unsigned)(T.num_tiles * CT_SPLIT)), dim3(256), 0, s, T);
+  } else if (f32) {
     hipLaunchKernelGGL(collect_scatter<true>, dim3((unsigned)T.blocks), dim3(CT_THREADS), lds, s, T);
     hipLaunchKernelGGL(collect_tiles<true>, dim3((unsigned)(T.num_tiles * CT_SPLIT)), dim3(256), 0, s, T);
   } else {
