@@ -11,6 +11,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "optk_batch.h"
@@ -828,23 +829,34 @@ __global__ __launch_bounds__(CT_THREADS) void collect_scatter(const TiledParams 
   }
 }
 
-// Float32 rays, images of at most CT_STAGED_TILES tiles: the scatter
-// stages each sub-chunk of CT_SUB rays in LDS in tile order (a local
-// counting sort) and then writes it out, so that one tile's records leave
-// the block as consecutive stores, whole cache lines, instead of one
-// scattered 16-byte store per ray that concurrent blocks complete piecewise.
+// Images of at most CT_STAGED_TILES tiles: the scatter stages each
+// sub-chunk of CT_SUB rays in LDS in tile order (a local counting sort) and
+// then writes it out, so that one tile's records leave the block as
+// consecutive stores, whole cache lines, instead of one scattered store per
+// ray that concurrent blocks complete piecewise.  Float32 rays stage their
+// 16-byte records; float64 rays stage (weight, cosine) and the 2-byte pixel.
 //   1. zero the local counts            2. load, rank within the tile (LDS atomic)
 //   3. scan the counts (local offsets)  4. place records and global slots in LDS
 //   5. write out in local order, advance the tiles' cursors
 constexpr int CT_SUB = CT_UNROLL * CT_THREADS;  // rays per sub-chunk
 constexpr int CT_STAGED_TILES = 4096;
 
+template <bool F32> struct StageRecord;
+template <> struct StageRecord<true> {
+  RecordF32 r;
+};
+template <> struct StageRecord<false> {
+  double2 ab;
+};
+
+template <bool F32>
 __global__ __launch_bounds__(CT_THREADS) void collect_scatter_staged(const TiledParams T) {
   extern __shared__ int32_t dyn[];
   int32_t *cursor = dyn;                    // [num_tiles] next global slot of each tile
   int32_t *loff = dyn + T.num_tiles;        // [num_tiles] local counts, then local offsets
   int32_t *lcnt = dyn + 2 * T.num_tiles;    // [num_tiles] local counts of the pass
-  __shared__ RecordF32 stage[CT_SUB];
+  __shared__ StageRecord<F32> stage[CT_SUB];
+  __shared__ uint16_t stage_pix[F32 ? 1 : CT_SUB];
   __shared__ int32_t dest[CT_SUB];
   __shared__ int32_t wsum[CT_THREADS / 64];
   __shared__ int32_t total;
@@ -858,17 +870,23 @@ __global__ __launch_bounds__(CT_THREADS) void collect_scatter_staged(const Tiled
   for (int64_t s0 = lo; s0 < hi; s0 += CT_SUB) {
     for (int t = tid; t < T.num_tiles; t += CT_THREADS) loff[t] = 0;
     __syncthreads();
+    typedef typename std::conditional<F32, float, double>::type Real;
     int32_t key[CT_UNROLL], rank[CT_UNROLL];
-    float a[CT_UNROLL], c[CT_UNROLL];
+    Real a[CT_UNROLL], c[CT_UNROLL];
 #pragma unroll
     for (int u = 0; u < CT_UNROLL; ++u) {
       const int64_t i = s0 + (int64_t)u * CT_THREADS + tid;
       key[u] = -1;
-      a[u] = c[u] = 0.0f;
+      a[u] = c[u] = Real(0);
       if (i < hi) {
         key[u] = T.keys[i];
-        a[u] = T.C.intensityf[i];
-        c[u] = T.C.dzf[i];
+        if constexpr (F32) {
+          a[u] = T.C.intensityf[i];
+          c[u] = T.C.dzf[i];
+        } else {
+          a[u] = T.C.intensity[i];
+          c[u] = T.C.dz[i];
+        }
       }
     }
 #pragma unroll
@@ -914,12 +932,24 @@ __global__ __launch_bounds__(CT_THREADS) void collect_scatter_staged(const Tiled
       if (key[u] < 0) continue;
       const int tile = key[u] >> 10;
       const int32_t p = loff[tile] + rank[u];
-      stage[p] = RecordF32{(uint32_t)(key[u] & 1023), a[u], c[u], 0u};
+      if constexpr (F32) {
+        stage[p].r = RecordF32{(uint32_t)(key[u] & 1023), a[u], c[u], 0u};
+      } else {
+        stage[p].ab = double2{a[u], c[u]};
+        stage_pix[p] = (uint16_t)(key[u] & 1023);
+      }
       dest[p] = cursor[tile] + rank[u];
     }
     __syncthreads();
     const int32_t n_sub = total;
-    for (int32_t p = tid; p < n_sub; p += CT_THREADS) T.sorted32[dest[p]] = stage[p];
+    for (int32_t p = tid; p < n_sub; p += CT_THREADS) {
+      if constexpr (F32) {
+        T.sorted32[dest[p]] = stage[p].r;
+      } else {
+        T.sorted_ab[dest[p]] = stage[p].ab;
+        T.sorted_pix[dest[p]] = stage_pix[p];
+      }
+    }
     for (int j = 0; j < per; ++j) {
       const int t = t0 + j;
       if (t < T.num_tiles) cursor[t] += lcnt[t];
@@ -1661,9 +1691,16 @@ int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
     hipLaunchKernelGGL(collect_count<false>, dim3((unsigned)T.blocks), dim3(CT_THREADS), lds, s, T);
   hipLaunchKernelGGL(collect_offsets, dim3((unsigned)((T.num_tiles + 3) / 4)), dim3(256), 0, s, T);
   hipLaunchKernelGGL(collect_base, dim3(1), dim3(1024), 0, s, T);
-  if (f32 && T.num_tiles <= CT_STAGED_TILES) {
-    hipLaunchKernelGGL(collect_scatter_staged, dim3((unsigned)T.blocks), dim3(CT_THREADS), 3 * lds, s, T);
-    hipLaunchKernelGGL(collect_tiles<true>, dim3((unsigned)(T.num_tiles * CT_SPLIT)), dim3(256), 0, s, T);
+  if (T.num_tiles <= CT_STAGED_TILES) {
+    if (f32) {
+      hipLaunchKernelGGL(collect_scatter_staged<true>, dim3((unsigned)T.blocks), dim3(CT_THREADS),
+                         3 * lds, s, T);
+      hipLaunchKernelGGL(collect_tiles<true>, dim3((unsigned)(T.num_tiles * CT_SPLIT)), dim3(256), 0, s, T);
+    } else {
+      hipLaunchKernelGGL(collect_scatter_staged<false>, dim3((unsigned)T.blocks), dim3(CT_THREADS),
+                         3 * lds, s, T);
+      hipLaunchKernelGGL(collect_tiles<false>, dim3((unsigned)(T.num_tiles * CT_SPLIT)), dim3(256), 0, s, T);
+    }
   } else if (f32) {
     hipLaunchKernelGGL(collect_scatter<true>, dim3((unsigned)T.blocks), dim3(CT_THREADS), lds, s, T);
     hipLaunchKernelGGL(collect_tiles<true>, dim3((unsigned)(T.num_tiles * CT_SPLIT)), dim3(256), 0, s, T);
