@@ -556,10 +556,12 @@ def run_stops(args):
     def step():
         return system.rayfunction(wavelength=500.0, field=f, pupil=pp)
 
-    def solve():
-        return system.field_max
-
     from optika_amd import stops as dstops
+
+    def solve():
+        # the solve itself (system.rayfunction_stops caches its result, as
+        # the reference's cached_property does)
+        return dstops.field_pupil(system, dstops.rayfunction_stops(system, np.array([500.0])))
 
     waves = np.linspace(400.0, 700.0, 64)
 

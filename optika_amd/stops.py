@@ -345,16 +345,19 @@ def denormalize(system, wavelength, field, pupil, normalized_field=True, normali
     from .vectors import Cartesian2dVectorArray
 
     dev = _trace.device()
-    if isinstance(wavelength, torch.Tensor):
-        wl_host = wavelength.detach().reshape(-1).double().cpu().numpy()
-        per_ray = wavelength.ndim > 0
+    if isinstance(wavelength, torch.Tensor) and wavelength.ndim > 0:
+        # the distinct wavelengths are found on the device; only they (and
+        # not a copy of every ray's wavelength) come to the host
+        w = wavelength.detach().reshape(-1).to(device=dev, dtype=torch.float64)
+        uniq_t, idx = torch.unique(w, sorted=True, return_inverse=True)
+        uniq = uniq_t.cpu().numpy()
     else:
-        wl_host = np.atleast_1d(np.asarray(wavelength, np.float64)).reshape(-1)
-        per_ray = np.ndim(wavelength) > 0
-    uniq, inverse = np.unique(wl_host, return_inverse=True)
+        w = wavelength.detach().cpu().numpy() if isinstance(wavelength, torch.Tensor) else wavelength
+        wl_host = np.atleast_1d(np.asarray(w, np.float64)).reshape(-1)
+        uniq, inverse = np.unique(wl_host, return_inverse=True)
+        idx = torch.as_tensor(inverse.reshape(-1), device=dev) if np.ndim(w) > 0 else None
     rays = rayfunction_stops(system, uniq, samples, samples)
     fld, pup = field_pupil(system, rays)
-    idx = torch.as_tensor(inverse.reshape(-1), device=dev) if per_ray else None
 
     def extent(v):
         lo = v.min(axis=(1, 2))

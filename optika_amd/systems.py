@@ -20,6 +20,7 @@ physical extent found by the stop solve (``stops.py``,
 from __future__ import annotations
 
 import dataclasses
+import functools
 from typing import Any, Sequence
 
 import numpy as np
@@ -90,12 +91,13 @@ class AbstractSequentialSystem:
             w = w.detach().cpu().numpy()
         return np.unique(np.atleast_1d(np.asarray(w, np.float64)))
 
-    @property
+    @functools.cached_property
     def rayfunction_stops(self) -> dict:
         """Rays on the object surface that graze both stops' outlines
         (21 x 21 samples) at every wavelength of ``grid_input``
-        (``_sequential.py:714-722``): ``position``, ``direction``
-        [3][21][21][W] and ``wavelength`` (host float64)."""
+        (``_sequential.py:683-696``, cached on first use as there):
+        ``position``, ``direction`` [3][21][21][W] and ``wavelength`` (host
+        float64)."""
         from . import stops
 
         return stops.rayfunction_stops(self, self._stop_wavelengths(), 21, 21)
