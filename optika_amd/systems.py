@@ -320,9 +320,9 @@ class AbstractSequentialSystem:
         the normalized pupil).  One ray per (wavelength, field, pupil) cell
         at a random point of the cell (the reference's
         ``cell_centers(random=True)``; its random stream is not reproduced)
-        carries radiance x cell volume; the rays are traced on the device,
-        binned on the sensor's pixels over the wavelength cells and exposed
-        (``measure``).  Returns [W][Nx][Ny] electrons, or [Nx][Ny] with
+        carries radiance x cell volume; the rays are generated and traced on
+        the device, binned on the sensor's pixels over the wavelength cells
+        and exposed (``measure``).  Returns [W][Nx][Ny] electrons, or [Nx][Ny] with
         ``integrate``."""
         h = lambda v: np.asarray(v.detach().cpu().numpy() if hasattr(v, "detach") else v,  # noqa: E731
                                  np.float64).reshape(-1)
@@ -345,28 +345,33 @@ class AbstractSequentialSystem:
                 px_w = pspan[0][:, None] * (pxv + 1) / 2 + plo[0][:, None]
                 py_w = pspan[1][:, None] * (pyv + 1) / 2 + plo[1][:, None]
         shape = (W, X, Y, PX, PY)
-        u = rng.uniform(size=(4,) + shape)
-        dfx, dfy = np.diff(fx_w, axis=1), np.diff(fy_w, axis=1)
-        dpx, dpy = np.diff(px_w, axis=1), np.diff(py_w, axis=1)
-        fx = fx_w[:, :-1, None, None, None] + dfx[:, :, None, None, None] * u[0]
-        fy = fy_w[:, None, :-1, None, None] + dfy[:, None, :, None, None] * u[1]
-        px = px_w[:, None, None, :-1, None] + dpx[:, None, None, :, None] * u[2]
-        py = py_w[:, None, None, None, :-1] + dpy[:, None, None, None, :] * u[3]
-        volume = np.abs(np.diff(wv)[:, None, None, None, None] * dfx[:, :, None, None, None]
-                        * dfy[:, None, :, None, None] * dpx[:, None, None, :, None]
-                        * dpy[:, None, None, None, :])
-        flux = R[:, :, :, None, None] * volume
-        lam = np.broadcast_to(wl[:, None, None, None, None], shape)
         import torch
 
         from . import _trace
 
+        # the rays of every cell are generated on the device (a seeded
+        # device generator places each ray in its cell)
         dev = _trace.device()
-        t = lambda a: torch.as_tensor(np.ascontiguousarray(np.broadcast_to(a, shape)).reshape(-1),  # noqa: E731
-                                      device=dev)
-        rays = self.rayfunction(intensity=t(flux), wavelength=t(lam),
-                                field=Cartesian2dVectorArray(t(fx), t(fy)),
-                                pupil=Cartesian2dVectorArray(t(px), t(py)),
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(int(seed))
+        T = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float64), device=dev)  # noqa: E731
+        fxw, fyw, pxw, pyw = T(fx_w), T(fy_w), T(px_w), T(py_w)
+        u = torch.rand((4,) + shape, generator=gen, device=dev, dtype=torch.float64)
+        dfx, dfy = fxw[:, 1:] - fxw[:, :-1], fyw[:, 1:] - fyw[:, :-1]
+        dpx, dpy = pxw[:, 1:] - pxw[:, :-1], pyw[:, 1:] - pyw[:, :-1]
+        fx = fxw[:, :-1, None, None, None] + dfx[:, :, None, None, None] * u[0]
+        fy = fyw[:, None, :-1, None, None] + dfy[:, None, :, None, None] * u[1]
+        px = pxw[:, None, None, :-1, None] + dpx[:, None, None, :, None] * u[2]
+        py = pyw[:, None, None, None, :-1] + dpy[:, None, None, None, :] * u[3]
+        volume = (T(np.diff(wv))[:, None, None, None, None] * dfx[:, :, None, None, None]
+                  * dfy[:, None, :, None, None] * dpx[:, None, None, :, None]
+                  * dpy[:, None, None, None, :]).abs()
+        flux = T(R)[:, :, :, None, None] * volume
+        lam = T(wl)[:, None, None, None, None].expand(shape)
+        flat = lambda a: a.expand(shape).reshape(-1).contiguous()  # noqa: E731
+        rays = self.rayfunction(intensity=flat(flux), wavelength=flat(lam),
+                                field=Cartesian2dVectorArray(flat(fx), flat(fy)),
+                                pupil=Cartesian2dVectorArray(flat(px), flat(py)),
                                 normalized_field=False, normalized_pupil=False).outputs
         if rays.position.x.ndim != 1:
             raise NotImplementedError("image of a system with several diffraction orders")
