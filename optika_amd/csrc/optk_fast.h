@@ -76,17 +76,6 @@ OPTK_HD double rsqrt64_finite(double x) {
 #endif
 }
 
-OPTK_HD double rsqrt64(double x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  double y = __builtin_amdgcn_rsq(x);
-  double e = fma(-x * y, y, 1.0);
-  double y1 = fma(0.5 * y, e, y);
-  return (e == e) ? y1 : y;
-#else
-  return 1.0 / sqrt(x);
-#endif
-}
-
 // square root from the refined reciprocal square root and one more Newton
 // step on x - g^2; zeros and +inf are returned as they are (one class test),
 // negative and NaN arguments give NaN through the hardware seed
