@@ -23,7 +23,8 @@ log-uniform ray counts in one persistent batched launch, autotuned and
 replayed from a hipGraph), ``surrogate`` (the learned surrogate of the
 spectrograph: training from float64 traces, then fused inference) and
 ``stops`` (normalized field/pupil coordinates: the stop solve on the device,
-then the trace of the denormalized grid, both inside each step).
+then the trace of the denormalized grid, both inside each step) and
+``rays`` (``propagate_rays``, the ray entry point, on the Newtonian).
 """
 
 from __future__ import annotations
@@ -66,7 +67,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", choices=("newtonian", "spectrograph", "tolerance", "surrogate", "stops"),
+    p.add_argument("--workload", choices=("newtonian", "spectrograph", "tolerance", "surrogate", "stops",
+                                          "rays"),
                    default="newtonian")
     p.add_argument("--train-rays", type=int, default=200_000_000,
                    help="surrogate: rays traced (float64) for training")
@@ -628,6 +630,64 @@ def run_stops(args):
     return header(args, n * S / (ms * 1e-3), ms, "f64", config, forms, None, cpu)
 
 
+# algorithmic bytes per ray of the ray entry point (optk_trace: position 3,
+# direction 3, wavelength, intensity, attenuation, index and the mask in;
+# the same ten fields and the mask out)
+BYTES_PER_RAY_RAYS_F32 = 10 * 4 + 1 + 10 * 4 + 1
+BYTES_PER_RAY_RAYS_F64 = 10 * 8 + 1 + 10 * 8 + 1
+
+
+def run_rays(args):
+    """``propagate_rays`` over the Newtonian's surfaces (the ray entry point,
+    ``optk_trace``): 2^24 rays given in the system frame -- on the object
+    plane at the seeded grid's pupil positions, directions within ~1e-3 rad
+    of the axis -- through every surface, both forms."""
+    import torch
+
+    from optika_amd import _trace, workloads
+    from optika_amd.rays import RayVectorArray
+    from optika_amd.vectors import Cartesian3dVectorArray
+
+    n = args.rays or (1 << 24)
+    system = workloads.newtonian()
+    S = workloads.NEWTONIAN_SURFACES
+    grid = workloads.newtonian_grid_random(n, seed=1)
+    surfaces = system.surfaces_all
+    cs = _trace.compiled(surfaces)
+    stream = torch.cuda.current_stream()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(3)
+    forms = {}
+    for prec, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        t = lambda a: torch.as_tensor(np.asarray(a), device="cuda", dtype=dt)  # noqa: E731
+        d = torch.randn(3, n, device="cuda", dtype=torch.float64, generator=gen) * 1e-3
+        d[2] = 1.0
+        d = (d / d.norm(dim=0)).to(dt)
+        zero = torch.zeros(n, device="cuda", dtype=dt)
+        rays = RayVectorArray(wavelength=t(grid.wavelength),
+                              position=Cartesian3dVectorArray(t(grid.pupil_x), t(grid.pupil_y), zero),
+                              direction=Cartesian3dVectorArray(d[0], d[1], d[2]),
+                              intensity=torch.ones_like(zero), attenuation=zero.clone(),
+                              index_refraction=torch.ones_like(zero),
+                              unvignetted=torch.ones(n, device="cuda", dtype=torch.bool))
+        ms, k = time_calls(lambda: _trace.trace_sequence(surfaces, rays, precision=prec, system=cs),
+                           args.steps, args.warmup, stream)
+        b = BYTES_PER_RAY_RAYS_F32 if prec == "f32" else BYTES_PER_RAY_RAYS_F64
+        forms[prec] = {"value": n * S / (ms * 1e-3), "ms_per_step": ms, "kernel_avg_ms": round(k, 4),
+                       "kernel": f"trace_rays_{prec}<0, false>", "bytes_per_ray": b,
+                       "achieved_gbs": round(b * n / (k * 1e-3) / 1e9, 1)}
+        log(f"{prec} rays: {ms:.3f} ms/step, kernel {k:.3f} ms")
+    f32 = forms["f32"]
+    roofline = {"bound": "hbm", "achieved": f32["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(f32["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": f32["kernel"], "kernel_avg_ms": f32["kernel_avg_ms"],
+                "bytes_per_ray": BYTES_PER_RAY_RAYS_F32}
+    config = {"workload": f"newtonian telescope propagate_rays (ray entry point), {n} rays in the "
+                          "system frame through every surface",
+              "rays": n, "surfaces": S, "form": "float32 compensated form; float64 in forms.f64"}
+    return header(args, f32["value"], f32["ms_per_step"], "f32", config, forms, roofline, None)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -636,7 +696,7 @@ def main():
         sys.exit(2)
     run = {"newtonian": run_newtonian, "spectrograph": run_spectrograph,
            "tolerance": run_tolerance, "surrogate": run_surrogate,
-           "stops": run_stops}[args.workload]
+           "stops": run_stops, "rays": run_rays}[args.workload]
     print(json.dumps(run(args)), flush=True)
 
 
