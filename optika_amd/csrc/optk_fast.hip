@@ -76,7 +76,14 @@ __device__ __forceinline__ void store_f(const FRaysOut &o, int64_t j, const FRay
 // compile time the kernel keeps no null tests, and none of the pointers it
 // does not use, live across its ray loop (they would otherwise crowd the
 // scalar registers the system records are read into).
-enum { IO_ANY = 0, IO_SENSOR = 1 };
+// IO_FULL: every output field is written (the ray entry point's usual
+// layout); the inputs keep their null tests.
+enum { IO_ANY = 0, IO_SENSOR = 1, IO_FULL = 2 };
+
+bool full_output(const FRaysOut &o) {
+  return o.p[0] && o.p[1] && o.p[2] && o.d[0] && o.d[1] && o.d[2] && o.wavelength && o.intensity &&
+         o.attenuation && o.index && o.unvignetted;
+}
 
 bool sensor_layout(const FGridIn &g, const FRaysOut &o) {
   return g.field[0] && g.field[1] && g.pupil[0] && g.pupil[1] && g.wavelength && !g.intensity &&
@@ -96,6 +103,20 @@ template <int IO> __device__ __forceinline__ FRay load_ray(const FGridIn &g, int
 }
 
 template <int IO> __device__ __forceinline__ void store_ray(const FRaysOut &o, int64_t j, const FRay &r) {
+  if (IO == IO_FULL) {
+    o.p[0][j] = (float)r.px;
+    o.p[1][j] = (float)r.py;
+    o.p[2][j] = (float)r.pz;
+    o.d[0][j] = (float)r.dx;
+    o.d[1][j] = (float)r.dy;
+    o.d[2][j] = (float)r.dz;
+    o.wavelength[j] = r.wavelength;
+    o.intensity[j] = r.intensity;
+    o.attenuation[j] = r.attenuation;
+    o.index[j] = r.index;
+    o.unvignetted[j] = r.unvignetted ? 1 : 0;
+    return;
+  }
   if (IO == IO_SENSOR) {
     o.p[0][j] = (float)r.px;
     o.p[1][j] = (float)r.py;
@@ -190,10 +211,10 @@ __device__ __forceinline__ void fast_body(const FastArgs &A, const In &in, const
   }
 }
 
-template <int F, bool ACCUM>
+template <int F, bool ACCUM, int IO = IO_ANY>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
 void trace_rays_f32(const FastArgs A, const FRaysIn in, const FRaysOut accum) {
-  fast_body<F, ACCUM, IO_ANY>(A, in, accum);
+  fast_body<F, ACCUM, IO>(A, in, accum);
 }
 
 template <int F, bool ACCUM, int IO = IO_ANY>
@@ -230,11 +251,12 @@ FastArgs make_args(const optk_system *sys, const FRaysOut &out, int64_t n, int32
 template <typename In> struct KernelSet {
   typedef void (*Kernel)(FastArgs, In, FRaysOut);
   Kernel any[4];
-  Kernel sensor[3];  // null where the entry point has no IO_SENSOR form
+  Kernel special[3];  // the entry point's fixed-layout form (IO_SENSOR for the
+                      // grid, IO_FULL for rays), by feature set
 };
 
 template <typename In>
-int32_t launch(const KernelSet<In> &ks, bool sensor, const optk_system *sys, const In &in,
+int32_t launch(const KernelSet<In> &ks, bool special, const optk_system *sys, const In &in,
                const FRaysOut &out, const FRaysOut *accum, int64_t n, int32_t entry,
                int32_t apply_post, hipStream_t stream) {
   if (n == 0) return OPTK_OK;
@@ -245,7 +267,7 @@ int32_t launch(const KernelSet<In> &ks, bool sensor, const optk_system *sys, con
   const int f = sys->features;
   const int which = f == 0 ? 0 : (f == (FEAT_TOROID | FEAT_RULINGS) ? 1 : 2);
   typename KernelSet<In>::Kernel kern = accum ? ks.any[3] : ks.any[which];
-  if (!accum && sensor && ks.sensor[which]) kern = ks.sensor[which];
+  if (!accum && special) kern = ks.special[which];
   // whole rounds of resident blocks; each block loops over its share
   const int64_t blocks = resident_grid((const void *)kern, TRACE_BLOCK, n);
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(TRACE_BLOCK), 0, stream, A, in, acc);
@@ -337,8 +359,9 @@ int32_t fast_trace_rays(const optk_system *sys, const FRaysIn &in, const FRaysOu
   static const KernelSet<FRaysIn> ks = {
       {trace_rays_f32<0, false>, trace_rays_f32<FEAT_TOROID | FEAT_RULINGS, false>,
        trace_rays_f32<FEAT_ALL, false>, trace_rays_f32<FEAT_ALL, true>},
-      {nullptr, nullptr, nullptr}};
-  return launch<FRaysIn>(ks, false, sys, in, out, accum, n, 0, apply_post, stream);
+      {trace_rays_f32<0, false, IO_FULL>, trace_rays_f32<FEAT_TOROID | FEAT_RULINGS, false, IO_FULL>,
+       trace_rays_f32<FEAT_ALL, false, IO_FULL>}};
+  return launch<FRaysIn>(ks, full_output(out), sys, in, out, accum, n, 0, apply_post, stream);
 }
 
 int32_t fast_trace_grid(const optk_system *sys, const FGridIn &in, const FRaysOut &out,

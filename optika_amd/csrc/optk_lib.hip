@@ -164,7 +164,14 @@ template <int F> struct Waves {
 // compile time the kernel keeps no null tests, and none of the pointers it
 // does not use, live across its ray loop (they would otherwise crowd the
 // scalar registers the system records are read into).
-enum { IO_ANY = 0, IO_SENSOR = 1 };
+// IO_FULL: every output field is written (the ray entry point's usual
+// layout); the inputs keep their null tests.
+enum { IO_ANY = 0, IO_SENSOR = 1, IO_FULL = 2 };
+
+static bool full_output(const RaysOut &o) {
+  return o.p[0] && o.p[1] && o.p[2] && o.d[0] && o.d[1] && o.d[2] && o.wavelength && o.intensity &&
+         o.attenuation && o.index && o.unvignetted;
+}
 
 static bool sensor_layout(const GridIn &g, const RaysOut &o) {
   return g.field[0] && g.field[1] && g.pupil[0] && g.pupil[1] && g.wavelength && !g.intensity &&
@@ -184,6 +191,20 @@ template <int IO> __device__ __forceinline__ Ray<double> load_io(const GridIn &g
 }
 
 template <int IO> __device__ __forceinline__ void store_io(const RaysOut &o, int64_t j, const Ray<double> &r) {
+  if (IO == IO_FULL) {
+    o.p[0][j] = r.px;
+    o.p[1][j] = r.py;
+    o.p[2][j] = r.pz;
+    o.d[0][j] = r.dx;
+    o.d[1][j] = r.dy;
+    o.d[2][j] = r.dz;
+    o.wavelength[j] = r.wavelength;
+    o.intensity[j] = r.intensity;
+    o.attenuation[j] = r.attenuation;
+    o.index[j] = r.index;
+    o.unvignetted[j] = r.unvignetted ? 1 : 0;
+    return;
+  }
   if (IO == IO_SENSOR) {
     o.p[0][j] = r.px;
     o.p[1][j] = r.py;
@@ -237,10 +258,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs &A, const In &in, con
   }
 }
 
-template <int F, bool ACCUM>
+template <int F, bool ACCUM, int IO = IO_ANY>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
 void trace_rays_f64(const TraceArgs A, const RaysIn in, const RaysOut accum) {
-  trace_body<F, ACCUM, IO_ANY>(A, in, accum);
+  trace_body<F, ACCUM, IO>(A, in, accum);
 }
 
 template <int F, bool ACCUM, int IO = IO_ANY>
@@ -1423,9 +1444,15 @@ int32_t optk_trace(const optk_system *sys, int32_t precision, const optk_rays *i
   ri.attenuation_default = in->attenuation_default;
   ri.index_default = in->index_default;
   ri.unvignetted_default = in->unvignetted_default;
-  launch_trace<RaysIn>(trace_rays_f64<0, false>, trace_rays_f64<FEAT_TOROID | FEAT_RULINGS, false>,
-                       trace_rays_f64<FEAT_ALL, false>, trace_rays_f64<FEAT_ALL, true>, sys->features,
-                       accum != nullptr, n, (hipStream_t)hip_stream, A, ri, acc);
+  if (!accum && full_output(A.out))
+    launch_trace<RaysIn>(trace_rays_f64<0, false, IO_FULL>,
+                         trace_rays_f64<FEAT_TOROID | FEAT_RULINGS, false, IO_FULL>,
+                         trace_rays_f64<FEAT_ALL, false, IO_FULL>, trace_rays_f64<FEAT_ALL, true>,
+                         sys->features, false, n, (hipStream_t)hip_stream, A, ri, acc);
+  else
+    launch_trace<RaysIn>(trace_rays_f64<0, false>, trace_rays_f64<FEAT_TOROID | FEAT_RULINGS, false>,
+                         trace_rays_f64<FEAT_ALL, false>, trace_rays_f64<FEAT_ALL, true>, sys->features,
+                         accum != nullptr, n, (hipStream_t)hip_stream, A, ri, acc);
   OPTK_HIP_CHECK(hipGetLastError());
   return OPTK_OK;
 }
