@@ -618,9 +618,10 @@ OPTK_HD V3<T> fast_normal(const FastSurface &f, T x, T y) {
   }
   case OPTK_SAG_CONIC:
   case OPTK_SAG_PARABOLA: {
-    // (c x / g, c y / g, -1) / |.| = (c x, c y, -g) / sqrt(1 - k c^2 r^2)
+    // (c x / g, c y / g, -1) / |.| = (c x, c y, -g) / sqrt(1 - k c^2 r^2);
+    // a paraboloid (1 + k = 0, uniform per surface) has g = 1 exactly
     T c2r2 = c * c * (x * x + y * y);
-    T g = Op::sqrt(T(1) - kp1 * c2r2);
+    T g = kp1 == T(0) ? T(1) : Op::sqrt(T(1) - kp1 * c2r2);
     T inv = Op::rsqrt(T(1) - (kp1 - T(1)) * c2r2);
     return {c * x * inv, c * y * inv, -g * inv};
   }
