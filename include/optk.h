@@ -31,6 +31,8 @@
  *                    (optika/systems/_sequential.py:396-662)
  *   optk_pupil_reduce the pupil reductions of distortion / vignetting /
  *                    area_effective (optika/systems/_sequential.py:1208-1512)
+ *   optk_denormalize the normalized grid's denormalization
+ *                    (optika/systems/_sequential.py:748-789)
  *   optk_expose      AbstractImagingSensor.expose with the ideal sensor
  *                    material (optika/sensors/_sensors.py:173-252)
  */
@@ -616,6 +618,26 @@ typedef struct optk_pupil_args {
 } optk_pupil_args;
 
 int32_t optk_pupil_reduce(const optk_pupil_args *args, int64_t segments, void *hip_stream);
+
+/* Normalized field and pupil coordinates onto the extent the stop rays span
+ * (optika/systems/_sequential.py:748-789, _denormalize_grid): for each of
+ * up to 4 components c (the first NULL in[c] ends the list) and each of the
+ * n values,
+ *   out[c][i] = ptp[c][w] * (in[c][i] + 1) / 2 + lo[c][w],
+ * in float64 in the reference's operation order, w = table_index[i]
+ * (table_index NULL: w = 0; entries outside [0, n_table) are clamped into
+ * it).  lo and ptp are device arrays of n_table entries (one per distinct
+ * wavelength).  out[c] may equal in[c].  One launch for all components. */
+typedef struct optk_denormalize_args {
+  const double *in[4];
+  double *out[4];
+  const double *lo[4];
+  const double *ptp[4];
+  const int64_t *table_index;
+  int64_t n_table;
+} optk_denormalize_args;
+
+int32_t optk_denormalize(const optk_denormalize_args *args, int64_t n, void *hip_stream);
 
 /* The surrogate's polynomial baseline in float64 (the fit of the targets on
  * the OPTK_SURROGATE_POLY cubic monomials F of 5 normalised inputs).

@@ -229,6 +229,11 @@ class PupilArgs(ctypes.Structure):
                 ("throughput", c_void_p)]
 
 
+class DenormalizeArgs(ctypes.Structure):
+    _fields_ = [("in_", c_void_p * 4), ("out", c_void_p * 4), ("lo", c_void_p * 4), ("ptp", c_void_p * 4),
+                ("table_index", c_void_p), ("n_table", c_int64)]
+
+
 class StopArgs(ctypes.Structure):
     _fields_ = [("position", c_void_p * 3), ("direction", c_void_p * 3), ("wavelength", c_void_p),
                 ("target", c_void_p * 2), ("x", c_void_p * 2), ("iterations", c_void_p),
@@ -337,6 +342,8 @@ def lib() -> ctypes.CDLL:
     L.optk_expose.argtypes = [ctypes.POINTER(ExposeArgs), c_int64, c_void_p]
     L.optk_pupil_reduce.restype = c_int32
     L.optk_pupil_reduce.argtypes = [ctypes.POINTER(PupilArgs), c_int64, c_void_p]
+    L.optk_denormalize.restype = c_int32
+    L.optk_denormalize.argtypes = [ctypes.POINTER(DenormalizeArgs), c_int64, c_void_p]
     L.optk_stop_newton.restype = c_int32
     L.optk_stop_newton.argtypes = [c_void_p, c_void_p, ctypes.POINTER(StopArgs), c_int64, c_void_p]
     L.optk_poly_workspace.restype = ctypes.c_size_t
@@ -355,7 +362,8 @@ def lib() -> ctypes.CDLL:
                       ("optk_batch_desc", BatchDesc), ("optk_tune", Tune),
                       ("optk_mlp_args", MlpArgs), ("optk_surrogate_desc", SurrogateDesc),
                       ("optk_surrogate_io", SurrogateIO), ("optk_stop_args", StopArgs),
-                      ("optk_pupil_args", PupilArgs), ("optk_expose_args", ExposeArgs)]:
+                      ("optk_pupil_args", PupilArgs), ("optk_denormalize_args", DenormalizeArgs),
+                      ("optk_expose_args", ExposeArgs)]:
         size = L.optk_sizeof(name.encode())
         if size != ctypes.sizeof(cls):
             raise OptkError(f"record {name}: C size {size} != binding size {ctypes.sizeof(cls)}")

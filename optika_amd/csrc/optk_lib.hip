@@ -364,6 +364,26 @@ __global__ __launch_bounds__(256) void stop_newton_f64(const StopParams P) {
 }
 
 
+// denormalization of normalized grid coordinates: one thread per value,
+// every component in the same pass (reads 8 B per component and the 8 B
+// table index, writes 8 B per component) ------------------------------------
+
+__global__ __launch_bounds__(256) void denormalize_kernel(const optk_denormalize_args a, int ncomp,
+                                                          int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t w = 0;
+  if (a.table_index) {
+    w = a.table_index[i];
+    w = w < 0 ? 0 : (w >= a.n_table ? a.n_table - 1 : w);
+  }
+  for (int c = 0; c < ncomp; ++c) {
+    const double v = a.in[c][i];
+    a.out[c][i] = a.ptp[c][w] * (v + 1.0) / 2.0 + a.lo[c][w];  // _sequential.py:777-785
+  }
+}
+
+
 // pupil reductions: one wave per segment, lanes stride the pupil samples,
 // then a fixed butterfly over the wave ---------------------------------------
 
@@ -1265,6 +1285,7 @@ int64_t optk_sizeof(const char *name) {
   if (s == "optk_surrogate_io") return sizeof(optk_surrogate_io);
   if (s == "optk_stop_args") return sizeof(optk_stop_args);
   if (s == "optk_pupil_args") return sizeof(optk_pupil_args);
+  if (s == "optk_denormalize_args") return sizeof(optk_denormalize_args);
   if (s == "optk_expose_args") return sizeof(optk_expose_args);
   return -1;
 }
@@ -1599,6 +1620,23 @@ int32_t optk_pupil_reduce(const optk_pupil_args *args, int64_t segments, void *h
   if (blocks > INT32_MAX) return fail(OPTK_EINVAL, "too many segments");
   hipLaunchKernelGGL(pupil_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0,
                      (hipStream_t)hip_stream, *args, segments);
+  OPTK_HIP_CHECK(hipGetLastError());
+  return OPTK_OK;
+}
+
+int32_t optk_denormalize(const optk_denormalize_args *args, int64_t n, void *hip_stream) {
+  if (!args) return fail(OPTK_EINVAL, "null argument");
+  if (n < 0) return fail(OPTK_EINVAL, "negative count");
+  if (args->n_table < 1) return fail(OPTK_EINVAL, "empty extent table");
+  int ncomp = 0;
+  while (ncomp < 4 && args->in[ncomp]) ++ncomp;
+  for (int c = 0; c < ncomp; ++c)
+    if (!args->out[c] || !args->lo[c] || !args->ptp[c])
+      return fail(OPTK_EINVAL, "missing output or extent array");
+  if (n == 0 || ncomp == 0) return OPTK_OK;
+  if ((n + 255) / 256 > INT32_MAX) return fail(OPTK_EINVAL, "too many values");
+  hipLaunchKernelGGL(denormalize_kernel, grid_for(n), dim3(256), 0, (hipStream_t)hip_stream, *args,
+                     ncomp, n);
   OPTK_HIP_CHECK(hipGetLastError());
   return OPTK_OK;
 }

@@ -31,10 +31,12 @@ outline's extent).
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
-from . import _trace
+from . import _lib, _trace
 from .rays import RayVectorArray
 from .vectors import Cartesian3dVectorArray
 
@@ -235,9 +237,6 @@ def _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, 
     (``optk_stop_newton``): each thread iterates its own ray, tracing the
     residual and both forward-difference Jacobian columns through
     ``sub[1:]`` into the last stop's frame."""
-    import ctypes
-
-    from . import _lib
     from .surfaces import Surface
 
     dev = _trace.device()
@@ -335,6 +334,43 @@ def field_pupil(system, rays: dict):
     return xy, ang
 
 
+def _denormalize_device(components, idx, table, dev):
+    """``ptp * (c + 1) / 2 + lo`` (``_sequential.py:777-785``) for each
+    component c, lo and ptp taken from ``table[c][0 | 1][w]`` at each value's
+    wavelength entry w = ``idx`` (None: entry 0).  Each result has the
+    broadcast shape of its component and ``idx``; components of one shape
+    share one ``optk_denormalize`` launch."""
+    ins = []
+    for c in components:
+        c = c if isinstance(c, torch.Tensor) else torch.as_tensor(np.asarray(c, np.float64))
+        ins.append(c.detach().to(device=dev, dtype=torch.float64))
+    groups = {}
+    for k, c in enumerate(ins):
+        shape = tuple(torch.broadcast_shapes(c.shape, idx.shape)) if idx is not None else tuple(c.shape)
+        groups.setdefault(shape, []).append(k)
+    outs = [None] * len(ins)
+    stride = table.shape[1] * table.shape[2]
+    for shape, ks in groups.items():
+        a = _lib.DenormalizeArgs()
+        n = int(np.prod(shape, dtype=np.int64))
+        keep = []
+        for j, k in enumerate(ks):
+            src = ins[k].expand(shape).contiguous()
+            dst = torch.empty(shape, dtype=torch.float64, device=dev)
+            keep.append(src)
+            outs[k] = dst
+            a.in_[j], a.out[j] = src.data_ptr(), dst.data_ptr()
+            a.lo[j] = table.data_ptr() + 8 * (k * stride)
+            a.ptp[j] = table.data_ptr() + 8 * (k * stride + table.shape[2])
+        if idx is not None:
+            w = idx.expand(shape).contiguous()
+            keep.append(w)
+            a.table_index = w.data_ptr()
+        a.n_table = table.shape[2]
+        _lib.check(_lib.lib().optk_denormalize(ctypes.byref(a), n, _trace.stream_handle()))
+    return outs
+
+
 def denormalize(system, wavelength, field, pupil, normalized_field=True, normalized_pupil=True,
                 samples: int = 21):
     """``_denormalize_grid`` (``_sequential.py:748-789``): map normalized
@@ -363,21 +399,24 @@ def denormalize(system, wavelength, field, pupil, normalized_field=True, normali
         lo = v.min(axis=(1, 2))
         return lo, v.max(axis=(1, 2)) - lo  # [2][W]
 
-    def apply(vec, ext):
-        lo, ptp = (torch.as_tensor(a, device=dev) for a in ext)
-        out = []
-        for k, c in enumerate((vec.x, vec.y)):
-            c = c if isinstance(c, torch.Tensor) else torch.as_tensor(np.asarray(c, np.float64))
-            c = c.to(device=dev, dtype=torch.float64)
-            if idx is None:
-                lo_k, ptp_k = lo[k][0], ptp[k][0]
-            else:
-                lo_k, ptp_k = lo[k][idx], ptp[k][idx]
-            out.append(ptp_k * (c + 1) / 2 + lo_k)
-        return Cartesian2dVectorArray(out[0], out[1])
-
+    if not (normalized_field or normalized_pupil):
+        return field, pupil
+    # (input component, lo, ptp) in the order field x, field y, pupil x, pupil y
+    items = []
     if normalized_field:
-        field = apply(field, extent(fld))
+        lo, ptp = extent(fld)
+        items += [(field.x, lo[0], ptp[0]), (field.y, lo[1], ptp[1])]
     if normalized_pupil:
-        pupil = apply(pupil, extent(pup))
+        lo, ptp = extent(pup)
+        items += [(pupil.x, lo[0], ptp[0]), (pupil.y, lo[1], ptp[1])]
+    table = torch.as_tensor(np.stack([np.stack([np.asarray(a, np.float64).reshape(-1),
+                                                np.asarray(b, np.float64).reshape(-1)])
+                                      for _, a, b in items]), device=dev)  # [C][2][W]
+    outs = _denormalize_device([c for c, _, _ in items], idx, table, dev)
+    k = 0
+    if normalized_field:
+        field = Cartesian2dVectorArray(outs[0], outs[1])
+        k = 2
+    if normalized_pupil:
+        pupil = Cartesian2dVectorArray(outs[k], outs[k + 1])
     return field, pupil

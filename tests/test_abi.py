@@ -161,3 +161,24 @@ def test_collect_arguments_are_rejected_without_a_device():
     # zero rays: accepted without touching a device
     a.x_lo, a.x_hi = -15.36, 15.36
     assert L.optk_collect(ctypes.byref(a), 0, None) == 0
+
+
+def test_denormalize_arguments_are_rejected_without_a_device():
+    from optika_amd import _lib
+
+    if not _lib.library_path().exists():
+        pytest.skip("liboptk.so not built")
+    L = _lib.lib()
+    assert L.optk_denormalize(None, 1, None) == -1
+    a = _lib.DenormalizeArgs()
+    assert L.optk_denormalize(ctypes.byref(a), 1, None) == -1
+    assert b"table" in L.optk_last_error()
+    a.n_table = 1
+    assert L.optk_denormalize(ctypes.byref(a), -1, None) == -1
+    buf = (ctypes.c_double * 4)()
+    a.in_[0] = ctypes.addressof(buf)
+    assert L.optk_denormalize(ctypes.byref(a), 4, None) == -1
+    assert b"missing" in L.optk_last_error()
+    # no components: nothing to do, no device call
+    a.in_[0] = None
+    assert L.optk_denormalize(ctypes.byref(a), 4, None) == 0
