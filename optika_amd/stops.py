@@ -71,10 +71,16 @@ def _sag(surface, p):
         return np.zeros(p.shape[1:])
     from .vectors import Cartesian3dVectorArray as V
 
-    dev = _trace.device()
-    flat = [torch.as_tensor(np.ascontiguousarray(c.reshape(-1)), device=dev) for c in p]
-    z = surface.sag(V(flat[0], flat[1], flat[2]))
+    # one copy each way
+    d = _to_device(p)
+    z = surface.sag(V(d[0], d[1], d[2]))
     return z.reshape(-1).cpu().numpy().reshape(p.shape[1:])
+
+
+def _to_device(p) -> torch.Tensor:
+    """Host components p [k][...] as one device copy, [k][n] float64."""
+    host = np.ascontiguousarray(np.asarray(p, np.float64).reshape(len(p), -1))
+    return torch.as_tensor(host).to(_trace.device())
 
 
 def anchor_surface(subsystem):
@@ -248,13 +254,16 @@ def _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, 
         )
     first = _trace.compiled([Surface(sag=s_first.sag)]) if variable == "position" else None
     n = x0.shape[1]
-    host = np.empty((10, n))
+    # rays, targets and the initial roots in one copy; the roots are solved
+    # in place (rows 9 and 10)
+    host = np.empty((11, n))
     host[0:3] = pos.reshape(3, -1)
     host[3:6] = dirs.reshape(3, -1)
     host[6] = wave.reshape(-1)
     host[7:9] = gl.reshape(2, -1)
+    host[9:11] = x0
     d = torch.as_tensor(host).to(dev)
-    x = torch.as_tensor(np.ascontiguousarray(x0)).to(dev)
+    x = d[9:11]
     iters = torch.empty(n, dtype=torch.int32, device=dev)
     a = _lib.StopArgs()
     for k in range(3):
@@ -271,7 +280,9 @@ def _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, 
     a.max_iter = MAX_ITERATIONS
     _lib.check(_lib.lib().optk_stop_newton(forward.handle, first.handle if first else None,
                                            ctypes.byref(a), n, _trace.stream_handle()))
-    if int(iters.max().item()) > MAX_ITERATIONS:
+    # the roots and the largest iteration count in one copy back
+    back = torch.cat([x.reshape(-1), iters.max().to(torch.float64).reshape(1)]).cpu().numpy()
+    if back[-1] > MAX_ITERATIONS:
         raise ValueError(
             f"Could not solve for the rays connecting the stop surfaces {s_first.name!r} and "
             f"{s_last.name!r}. If the field stop is only partially reachable at a single "
@@ -279,7 +290,7 @@ def _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, 
             f"with an angular (dimensionless, sine of the half-angle) aperture, as the field "
             f"stop instead."
         )
-    return x.cpu().numpy()
+    return back[:-1].reshape(2, n)
 
 
 def rayfunction_stops(system, wavelength, samples_pupil_stop: int = 21,
@@ -306,10 +317,9 @@ def rayfunction_stops(system, wavelength, samples_pupil_stop: int = 21,
     if obj.sag is not None:
         from . import _ops
 
-        dev = _trace.device()
-        t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)  # noqa: E731
-        nrm = _ops.sag_normal(obj.sag, Cartesian3dVectorArray(t(pl[0]), t(pl[1]), t(pl[2])))
-        nrm = np.stack([nrm.x.cpu().numpy(), nrm.y.cpu().numpy(), nrm.z.cpu().numpy()])
+        q = _to_device(pl)
+        nrm = _ops.sag_normal(obj.sag, Cartesian3dVectorArray(q[0], q[1], q[2]))
+        nrm = torch.stack([nrm.x, nrm.y, nrm.z]).cpu().numpy()
     else:
         nrm = np.array([0.0, 0.0, -1.0])[:, None]
     away = (dl * nrm).sum(0) > 0
