@@ -585,6 +585,11 @@ typedef struct optk_stop_args {
   const double *target[2];
   double *x[2];
   int32_t *iterations;
+  double *z;                    /* optional (NULL: not written): the free
+                                   component's third coordinate at the root,
+                                   sqrt(1 - x0^2 - x1^2) for a direction, the
+                                   first surface's sag at (x0, x1) for a
+                                   position */
   double tol;
   double dx;
   int32_t free_component;       /* OPTK_STOP_FREE_* */

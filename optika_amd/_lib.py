@@ -237,7 +237,7 @@ class DenormalizeArgs(ctypes.Structure):
 class StopArgs(ctypes.Structure):
     _fields_ = [("position", c_void_p * 3), ("direction", c_void_p * 3), ("wavelength", c_void_p),
                 ("target", c_void_p * 2), ("x", c_void_p * 2), ("iterations", c_void_p),
-                ("tol", ctypes.c_double), ("dx", ctypes.c_double),
+                ("z", c_void_p), ("tol", ctypes.c_double), ("dx", ctypes.c_double),
                 ("free_component", c_int32), ("target_component", c_int32),
                 ("max_iter", c_int32), ("pad", c_int32)]
 

@@ -359,6 +359,9 @@ __global__ __launch_bounds__(256) void stop_newton_f64(const StopParams P) {
     }
     a.x[0][i] = x0;
     a.x[1][i] = x1;
+    if (a.z)
+      a.z[i] = a.free_component == OPTK_STOP_FREE_DIRECTION ? sqrt(1.0 - (x0 * x0 + x1 * x1))
+                                                           : sag_z(P.first, x0, x1, 0.0);
     a.iterations[i] = converged ? it : a.max_iter + 1;
   }
 }

@@ -219,16 +219,12 @@ def solve_stop_rays(system, wavelength, samples_pupil_stop: int = 21,
         max_abs_error = 1e-9 * max(scale, 1.0)
         dx = 1e-6 if variable == "direction" else 1e-6 * max(scale, 1.0)
 
-        def zfunc(a):
-            if variable == "direction":
-                return np.sqrt(1 - (a[0] ** 2 + a[1] ** 2))
-            return _sag(s_first, np.stack([a[0], a[1], np.zeros(a.shape[1:])]))
-
         x0 = (dirs if variable == "direction" else pos)[:2].reshape(2, -1)
+        # the root and its z (sqrt(1 - x^2 - y^2) for a direction, the first
+        # surface's sag for a position) from the device
         root = _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, x0,
                               dx, max_abs_error)
-        z = zfunc(root)
-        v = np.stack([root[0], root[1], z]).reshape((3,) + shape)
+        v = root.reshape((3,) + shape)
         if variable == "direction":
             dirs = v
         else:
@@ -255,8 +251,9 @@ def _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, 
     first = _trace.compiled([Surface(sag=s_first.sag)]) if variable == "position" else None
     n = x0.shape[1]
     # rays, targets and the initial roots in one copy; the roots are solved
-    # in place (rows 9 and 10)
-    host = np.empty((11, n))
+    # in place (rows 9 and 10), the free component's z at the root goes to
+    # row 11
+    host = np.empty((12, n))
     host[0:3] = pos.reshape(3, -1)
     host[3:6] = dirs.reshape(3, -1)
     host[6] = wave.reshape(-1)
@@ -273,6 +270,7 @@ def _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, 
     a.target[0], a.target[1] = d[7].data_ptr(), d[8].data_ptr()
     a.x[0], a.x[1] = x[0].data_ptr(), x[1].data_ptr()
     a.iterations = iters.data_ptr()
+    a.z = d[11].data_ptr()
     a.tol, a.dx = float(tol), float(dx)
     a.free_component = _lib.STOP_FREE_DIRECTION if variable == "direction" else _lib.STOP_FREE_POSITION
     a.target_component = (_lib.STOP_TARGET_POSITION if target == "position"
@@ -280,8 +278,8 @@ def _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, 
     a.max_iter = MAX_ITERATIONS
     _lib.check(_lib.lib().optk_stop_newton(forward.handle, first.handle if first else None,
                                            ctypes.byref(a), n, _trace.stream_handle()))
-    # the roots and the largest iteration count in one copy back
-    back = torch.cat([x.reshape(-1), iters.max().to(torch.float64).reshape(1)]).cpu().numpy()
+    # the roots, their z and the largest iteration count in one copy back
+    back = torch.cat([d[9:12].reshape(-1), iters.max().to(torch.float64).reshape(1)]).cpu().numpy()
     if back[-1] > MAX_ITERATIONS:
         raise ValueError(
             f"Could not solve for the rays connecting the stop surfaces {s_first.name!r} and "
@@ -290,7 +288,7 @@ def _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, 
             f"with an angular (dimensionless, sine of the half-angle) aperture, as the field "
             f"stop instead."
         )
-    return back[:-1].reshape(2, n)
+    return back[:-1].reshape(3, n)
 
 
 def rayfunction_stops(system, wavelength, samples_pupil_stop: int = 21,
