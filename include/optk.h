@@ -370,6 +370,25 @@ int32_t optk_collect(const optk_collect_args *args, int64_t n, void *hip_stream)
 /* bytes of workspace the tiled binning of n rays needs (0: not applicable) */
 uint64_t optk_collect_workspace(const optk_collect_args *args, int64_t n);
 
+/* Sensor binning that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as optk_collect with a workspace, except that
+ * args->workspace / workspace_bytes are ignored: the tiled binning uses a
+ * library-owned device array of OPTK_COLLECT_SCRATCH_BYTES, one per GPU,
+ * held until the process ends.  A call whose workspace need
+ * (optk_collect_workspace) exceeds it -- about 3.2M float32 or 2.9M float64
+ * rays on a 2048 x 2048 image -- is rejected with OPTK_EINVAL before any
+ * device work, as is one of 2^31 rays or more; images of more than 16384
+ * tiles take the one-pass path, which needs no workspace.  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (on
+ * different streams, or from host threads without ordering) race.  The
+ * array keeps the last call's contents; every entry a call reads, it has
+ * written first, so nothing needs clearing between calls.  No hipMalloc,
+ * hipFree or allocating library routine is called. */
+#define OPTK_COLLECT_SCRATCH_BYTES (64ull << 20)
+int32_t optk_collect_noalloc(const optk_collect_args *args, int64_t n, void *hip_stream);
+
 /* Exposure of a binned image by an ideal sensor material (quantum yield 1):
  * optika/sensors/_sensors.py:173-252 (expose) with
  * optika/sensors/materials/_materials.py:1576-1601 (IdealSensorMaterial.signal).

@@ -279,6 +279,8 @@ MLP_TO_BF16 = 9
 
 BATCH_STATS = 8
 BATCH_GRAPH = 2
+# device scratch of optk_collect_noalloc (bytes, one array per GPU)
+COLLECT_SCRATCH_BYTES = 64 << 20
 
 
 class OptkError(RuntimeError):
@@ -319,6 +321,8 @@ def lib() -> ctypes.CDLL:
     L.optk_collect.argtypes = [ctypes.POINTER(CollectArgs), c_int64, c_void_p]
     L.optk_collect_workspace.restype = ctypes.c_uint64
     L.optk_collect_workspace.argtypes = [ctypes.POINTER(CollectArgs), c_int64]
+    L.optk_collect_noalloc.restype = c_int32
+    L.optk_collect_noalloc.argtypes = [ctypes.POINTER(CollectArgs), c_int64, c_void_p]
     L.optk_batch_create.restype = c_int32
     L.optk_batch_create.argtypes = [ctypes.POINTER(BatchDesc), ctypes.POINTER(c_void_p)]
     L.optk_batch_destroy.restype = c_int32

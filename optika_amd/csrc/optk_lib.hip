@@ -1693,7 +1693,14 @@ static uint64_t tiled_layout(int nw, int nx, int ny, int64_t n, bool f32, void *
   return total;
 }
 
-int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
+}  // extern "C"
+
+// Sensor binning with an explicit workspace (ws may be null: the one-pass
+// kernel with float64 global atomics).  Shared by optk_collect (the
+// caller's workspace) and optk_collect_noalloc (the library's scratch).
+// Arguments of a binning call checked before any device work (OPTK_OK or
+// an error).
+static int32_t collect_validate(const optk_collect_args *a, int64_t n) {
   if (!a) return fail(OPTK_EINVAL, "null argument");
   if (n < 0) return fail(OPTK_EINVAL, "negative count");
   if (a->n_wavelength < 1 || a->nx < 1 || a->ny < 1) return fail(OPTK_EINVAL, "empty histogram");
@@ -1709,6 +1716,13 @@ int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
     return fail(OPTK_EINVAL, "pixel range too narrow for its distance from the origin "
                              "(num * (|lo| + |hi|) / (hi - lo) must be below 2^48)");
   if (a->precision != OPTK_F64 && a->precision != OPTK_F32) return fail(OPTK_EINVAL, "bad precision");
+  return OPTK_OK;
+}
+
+static int32_t collect_launch(const optk_collect_args *a, int64_t n, void *ws, uint64_t ws_bytes,
+                              void *hip_stream) {
+  const int32_t rc = collect_validate(a, n);
+  if (rc) return rc;
   if (n == 0) return OPTK_OK;
   CollectParams C;
   std::memset(&C, 0, sizeof(C));
@@ -1744,9 +1758,9 @@ int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
   C.y_inv = 1.0 / C.y_step;
   const bool f32 = a->precision == OPTK_F32;
   TiledParams T;
-  const uint64_t need = tiled_layout(a->n_wavelength, a->nx, a->ny, n, f32, a->workspace, &T);
+  const uint64_t need = tiled_layout(a->n_wavelength, a->nx, a->ny, n, f32, ws, &T);
   hipStream_t s = (hipStream_t)hip_stream;
-  if (need == 0 || !a->workspace || a->workspace_bytes < need) {
+  if (need == 0 || !ws || ws_bytes < need) {
     hipLaunchKernelGGL(collect_kernel, grid_for(n), dim3(256), 0, s, C);
     OPTK_HIP_CHECK(hipGetLastError());
     return OPTK_OK;
@@ -1778,6 +1792,72 @@ int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
   }
   OPTK_HIP_CHECK(hipGetLastError());
   return OPTK_OK;
+}
+
+// The no-allocation binning's workspace: one array per GPU, held by the
+// code object until the process ends.  Host code never passes `scratch`
+// itself to a kernel or a copy (on the host the name is a stand-in, not a
+// device address): it uses the address hipGetSymbolAddress returns for the
+// current device, looked up once per device and kept in scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[OPTK_COLLECT_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// the current device's address of `scratch` (looked up on its first use)
+static int32_t scratch_address(void **out) {
+  int dev = -1;
+  OPTK_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(OPTK_EDEVICE, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(OPTK_EDEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                    hipGetErrorString(e));
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return OPTK_OK;
+}
+
+extern "C" {
+
+int32_t optk_collect(const optk_collect_args *a, int64_t n, void *hip_stream) {
+  if (!a) return fail(OPTK_EINVAL, "null argument");
+  return collect_launch(a, n, a->workspace, a->workspace_bytes, hip_stream);
+}
+
+int32_t optk_collect_noalloc(const optk_collect_args *a, int64_t n, void *hip_stream) {
+  const int32_t bad = collect_validate(a, n);
+  if (bad) return bad;
+  uint64_t need = 0;
+  if (n > 0) {
+    // more rays than 32-bit slots hold: the tiled path does not apply, and
+    // the one-pass path would run instead; refuse, as for any size whose
+    // tiled workspace exceeds the scratch
+    if (n > INT32_MAX)
+      return fail(OPTK_EINVAL, "optk_collect_noalloc: at most 2^31 - 1 rays per call");
+    need = tiled_layout(a->n_wavelength, a->nx, a->ny, n, a->precision == OPTK_F32, nullptr,
+                        nullptr);
+    if (need > OPTK_COLLECT_SCRATCH_BYTES)
+      return fail(OPTK_EINVAL,
+                  "optk_collect_noalloc: " + std::to_string(n) + " rays need " +
+                      std::to_string(need) + " bytes of workspace, more than the " +
+                      std::to_string((unsigned long long)OPTK_COLLECT_SCRATCH_BYTES) +
+                      "-byte scratch; split the rays over several calls or use optk_collect "
+                      "with a workspace");
+  }
+  void *ws = nullptr;
+  if (need > 0) {
+    const int32_t rc = scratch_address(&ws);
+    if (rc) return rc;
+  }
+  return collect_launch(a, n, ws, need, hip_stream);
 }
 
 uint64_t optk_collect_workspace(const optk_collect_args *a, int64_t n) {

@@ -182,3 +182,31 @@ def test_denormalize_arguments_are_rejected_without_a_device():
     # no components: nothing to do, no device call
     a.in_[0] = None
     assert L.optk_denormalize(ctypes.byref(a), 4, None) == 0
+
+
+def test_noalloc_collect_rejects_oversized_calls_without_a_device():
+    """optk_collect_noalloc refuses, before any device work, a call whose
+    tiled workspace exceeds its scratch (the pointers are never read)."""
+    from optika_amd import _lib
+
+    if not _lib.library_path().exists():
+        pytest.skip("liboptk.so not built")
+    L = _lib.lib()
+    a = _lib.CollectArgs()
+    fake = 0x1000
+    a.position[0] = a.position[1] = a.direction_z = a.wavelength = a.intensity = fake
+    a.wavelength_edges = a.image = a.moment = fake
+    a.n_wavelength, a.nx, a.ny = 1, 2048, 2048
+    a.x_lo, a.x_hi, a.y_lo, a.y_hi = -15.36, 15.36, -15.36, 15.36
+    for prec, n_ok in ((_lib.F32, 3_272_688), (_lib.F64, 2_981_872)):
+        a.precision = prec
+        # the largest call the scratch holds, and one ray more
+        assert L.optk_collect_workspace(ctypes.byref(a), n_ok) <= _lib.COLLECT_SCRATCH_BYTES
+        assert L.optk_collect_workspace(ctypes.byref(a), n_ok + 1) > _lib.COLLECT_SCRATCH_BYTES
+        assert L.optk_collect_noalloc(ctypes.byref(a), n_ok + 1, None) == -1
+        assert b"scratch" in L.optk_last_error()
+    assert L.optk_collect_noalloc(ctypes.byref(a), 1 << 31, None) == -1
+    assert L.optk_collect_noalloc(None, 10, None) == -1
+    a.image = 0
+    assert L.optk_collect_noalloc(ctypes.byref(a), 10, None) == -1
+    assert b"missing" in L.optk_last_error()
