@@ -35,6 +35,15 @@ fixes an explicit convention and says so:
 * matrix ``power``: binary powering (``numpy.linalg.matrix_power``'s
   algorithm) -- pinned against repeated multiplication and the IMD curves;
   the reference's exact algorithm is unpinned.
+
+A reference quirk kept on purpose: the triangular groove efficiency uses
+``gamma / ((pi gamma / 2)^2 + i^2)`` as the reference's code does
+(``optika/rulings/_rulings.py:903``), not the ``- i^2`` of its docstring
+(Magnusson Table 1, ``:876-877``).  With the code's sign the orders do not
+sum to one (about 0.90 at gamma = 0.3); parity follows the code, and
+"fixing" the sign here or in the kernels would break it.  The pins
+(``tests/test_oracle_pins.py``) fix the code's form by its gamma -> 0
+limit and by values derived by hand at gamma = 2 / pi.
 """
 
 from .trace import *  # noqa: F401,F403
