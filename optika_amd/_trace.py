@@ -197,14 +197,25 @@ def trace_sequence(surfaces: Sequence, rays: RayVectorArray, accumulate: bool = 
     return _make_rays(out, _shape(P, n), {})
 
 
-def _grid_fields(grid: ObjectVectorArray, intensity):
+def _grid_fields(grid: ObjectVectorArray, intensity, paths: int = 1):
+    """The grid's fields and their common length n.  With ``paths`` > 1 the
+    field and pupil components may carry a leading path axis ([paths][n])."""
     vals = {
         "fx": grid.field.x, "fy": grid.field.y,
         "px": grid.pupil.x, "py": grid.pupil.y,
         "wavelength": grid.wavelength,
         "intensity": 1.0 if intensity is None else intensity,
     }
-    sizes = {s for s in (_numel(v) for v in vals.values()) if s is not None}
+
+    def per_ray(k, v):
+        m = _numel(v)
+        if m is not None and paths > 1 and k in ("fx", "fy", "px", "py") and v.ndim == 2:
+            if v.shape[0] != paths:
+                raise ValueError(f"grid field has {v.shape[0]} paths, expected {paths}")
+            return int(v.shape[1])
+        return m
+
+    sizes = {s for s in (per_ray(k, v) for k, v in vals.items()) if s is not None}
     if len(sizes) > 1:
         raise ValueError(
             f"grid fields must be scalars or 1-D arrays of one common length, got sizes {sorted(sizes)}"
@@ -213,21 +224,11 @@ def _grid_fields(grid: ObjectVectorArray, intensity):
     return vals, n
 
 
-def trace_system(system, grid: ObjectVectorArray, intensity=None, accumulate: bool = False,
-                 sensor_frame: bool = False, precision: str = "f64") -> RayFunctionArray:
-    """``SequentialSystem.raytrace`` / ``rayfunction`` in one launch."""
-    prec = _precision(precision)
-    surfaces = system.surfaces_all
-    obj_t = system.object.transformation if system.object is not None else None
-    post = system.sensor.transformation if (sensor_frame and system.sensor is not None) else None
-    dtype = torch.float64 if prec == _lib.F64 else torch.float32
-    dev = device()
-    cs = compiled(surfaces, pre=(obj_t, system.transformation), pre_is_inverse=(0, 1), post=post)
-    vals, n = _grid_fields(grid, intensity)
-    P = cs.num_paths
-
+def _grid_record(vals, n, dev, dtype, object_at_infinity, keep):
+    """The ``optk_grid`` record of the grid fields ``vals`` (1-D device
+    tensors of length n or scalars); the tensors it points at go to
+    ``keep``.  Returns (record, wavelength tensor or None)."""
     g = _lib.Grid()
-    keep = []
 
     def put(name, arr_field, idx, dflt_field):
         t = _as_tensor(vals[name], n, dev, dtype)
@@ -250,20 +251,78 @@ def trace_system(system, grid: ObjectVectorArray, intensity=None, accumulate: bo
     put("py", "pupil", 1, "pupil_default")
     tw = put("wavelength", "wavelength", None, "wavelength_default")
     put("intensity", "intensity", None, "intensity_default")
-    g.object_at_infinity = 1 if system.object_is_at_infinity else 0
+    g.object_at_infinity = 1 if object_at_infinity else 0
+    return g, tw
+
+
+def _path_grid(grid: ObjectVectorArray, paths: int) -> bool:
+    """True when the grid's field or pupil carries a leading diffraction-order
+    path axis ([paths][n] tensors, a grid denormalized per path)."""
+    comps = (grid.field.x, grid.field.y, grid.pupil.x, grid.pupil.y)
+    return paths > 1 and any(isinstance(c, torch.Tensor) and c.ndim == 2 and c.shape[0] == paths
+                             for c in comps)
+
+
+def trace_system(system, grid: ObjectVectorArray, intensity=None, accumulate: bool = False,
+                 sensor_frame: bool = False, precision: str = "f64") -> RayFunctionArray:
+    """``SequentialSystem.raytrace`` / ``rayfunction`` in one launch.
+
+    A grid whose field or pupil carries a leading diffraction-order path
+    axis (normalized coordinates denormalized per path by the stop solve of
+    a system with several orders) is traced one path at a time, each path
+    through the surfaces restricted to its order, with its own inputs."""
+    prec = _precision(precision)
+    surfaces = system.surfaces_all
+    obj_t = system.object.transformation if system.object is not None else None
+    post = system.sensor.transformation if (sensor_frame and system.sensor is not None) else None
+    dtype = torch.float64 if prec == _lib.F64 else torch.float32
+    dev = device()
+    cs = compiled(surfaces, pre=(obj_t, system.transformation), pre_is_inverse=(0, 1), post=post)
+    P = cs.num_paths
+    per_path = _path_grid(grid, P)
+    vals, n = _grid_fields(grid, intensity, P if per_path else 1)
 
     invariant = _compile.invariant_fields(surfaces)
     fields = {"intensity"} if invariant else {"wavelength", "intensity", "attenuation",
                                                "index_refraction"}
+    all_fields = {"wavelength", "intensity", "attenuation", "index_refraction"}
     out = _Out(P * n, dev, dtype, fields)
-    acc = (_Out(cs.num_surfaces * P * n, dev, dtype,
-                {"wavelength", "intensity", "attenuation", "index_refraction"})
-           if accumulate else None)
     flags = _lib.TRACE_APPLY_POST if post is not None else 0
     L = _lib.lib()
-    _lib.check(L.optk_trace_grid(cs.handle, prec, ctypes.byref(g), ctypes.byref(out.rec),
-                                 ctypes.byref(acc.rec) if acc else None, n, flags,
-                                 stream_handle()))
+    keep = []
+    if not per_path:
+        g, tw = _grid_record(vals, n, dev, dtype, system.object_is_at_infinity, keep)
+        acc = _Out(cs.num_surfaces * P * n, dev, dtype, all_fields) if accumulate else None
+        _lib.check(L.optk_trace_grid(cs.handle, prec, ctypes.byref(g), ctypes.byref(out.rec),
+                                     ctypes.byref(acc.rec) if acc else None, n, flags,
+                                     stream_handle()))
+    else:
+        from .stops import single_path
+
+        accs = []
+        tw = None
+        for p in range(P):
+            vp = {k: (v[p] if isinstance(v, torch.Tensor) and v.ndim == 2 else v)
+                  for k, v in vals.items()}
+            g, tw = _grid_record(vp, n, dev, dtype, system.object_is_at_infinity, keep)
+            cs_p = compiled(single_path(surfaces, p), pre=(obj_t, system.transformation),
+                            pre_is_inverse=(0, 1), post=post)
+            # path p of the path-major outputs: elements p * n .. (p + 1) * n
+            rec = _path_slice(out, p * n)
+            acc_p = _Out(cs_p.num_surfaces * n, dev, dtype, all_fields) if accumulate else None
+            _lib.check(L.optk_trace_grid(cs_p.handle, prec, ctypes.byref(g), ctypes.byref(rec),
+                                         ctypes.byref(acc_p.rec) if acc_p else None, n, flags,
+                                         stream_handle()))
+            accs.append(acc_p)
+        if accumulate:
+            # [surface][path][n], surface-major as the one-launch form
+            def merge(ts):
+                return torch.stack([t.view(cs.num_surfaces, n) for t in ts], 1).reshape(-1)
+
+            acc = accs[0]
+            acc.t = {k: ([merge([a.t[k][c] for a in accs]) for c in range(3)]
+                         if isinstance(v, list) else merge([a.t[k] for a in accs]))
+                     for k, v in accs[0].t.items()}
     if accumulate:
         outputs = _make_rays(acc, _shape(P, n, cs.num_surfaces), {})
     else:
@@ -271,6 +330,20 @@ def trace_system(system, grid: ObjectVectorArray, intensity=None, accumulate: bo
         outputs = _make_rays(out, _shape(P, n), {
             "wavelength": wl, "attenuation": 0.0, "index_refraction": 1.0})
     return RayFunctionArray(inputs=grid, outputs=outputs)
+
+
+def _path_slice(out: "_Out", start: int):
+    """An ``optk_rays`` record pointing at elements start.. of every output
+    tensor of ``out``."""
+    r = _lib.Rays()
+    for k in range(3):
+        r.position[k] = out.t["position"][k][start:].data_ptr()
+        r.direction[k] = out.t["direction"][k][start:].data_ptr()
+    for f in ("wavelength", "intensity", "attenuation", "index_refraction"):
+        if f in out.t:
+            setattr(r, f, out.t[f][start:].data_ptr())
+    r.unvignetted = out.t["unvignetted"][start:].data_ptr()
+    return r
 
 
 class GridTracePlan:

@@ -22,7 +22,7 @@ import torch
 from . import _lib, _trace
 from .vectors import Cartesian2dVectorArray, ObjectVectorArray
 
-__all__ = ["trace_outer", "reduce", "cell_grid"]
+__all__ = ["trace_outer", "reduce", "cell_grid", "solid_angle_cells"]
 
 
 def _flat(v, dev) -> torch.Tensor:
@@ -116,3 +116,40 @@ def cell_grid(vx, vy, seed: int = 0):
     X, Y = np.broadcast_to(X, area.shape), np.broadcast_to(Y, area.shape)
     lead = area.shape[:-2]
     return (area.reshape(lead + (-1,)), X.reshape(lead + (-1,)), Y.reshape(lead + (-1,)))
+
+
+def _unit_direction(ax_deg, ay_deg):
+    """``optika.direction`` (``optika/_util.py:41-73``) of angles in degrees:
+    (-cos y sin x, -sin y, cos y cos x), components on the last axis."""
+    x, y = np.broadcast_arrays(np.deg2rad(ax_deg), np.deg2rad(ay_deg))
+    return np.stack([-np.cos(y) * np.sin(x), -np.sin(y), np.cos(y) * np.cos(x)], axis=-1)
+
+
+def _triangle_solid_angle(a, b, c):
+    """Signed solid angle of the spherical triangle of unit vectors a, b, c
+    (Van Oosterom and Strackee, IEEE Trans. Biomed. Eng. 30, 125 (1983)):
+    tan(omega / 2) = a . (b x c) / (1 + a.b + b.c + c.a)."""
+    triple = np.sum(a * np.cross(b, c), axis=-1)
+    den = 1.0 + np.sum(a * b, axis=-1) + np.sum(b * c, axis=-1) + np.sum(c * a, axis=-1)
+    return 2.0 * np.arctan2(triple, den)
+
+
+def solid_angle_cells(ax, ay):
+    """Solid angle (steradians) of every cell of the angle grid with vertices
+    ax [..., Nx + 1] x ay [..., Ny + 1] (degrees; leading axes broadcast):
+    the cell's four vertex directions (``optika.direction``) bound a
+    spherical quadrilateral with great-circle edges, summed as two
+    triangles.  The reference weights an angular field (or pupil) cell by
+    this measure, ``optika.direction(field).solid_angle_cell(axis)``
+    (``optika/vectors/_vectors_object.py:111-118``); for small cells it
+    tends to |dx (sin y1 - sin y0)| (x, y in radians), the area of the
+    coordinate cell on the unit sphere.  Returns |omega| [..., Nx, Ny]."""
+    ax, ay = np.asarray(ax, np.float64), np.asarray(ay, np.float64)
+    lead = np.broadcast_shapes(ax.shape[:-1], ay.shape[:-1])
+    ax = np.broadcast_to(ax, lead + ax.shape[-1:])
+    ay = np.broadcast_to(ay, lead + ay.shape[-1:])
+    d = _unit_direction(ax[..., :, None], ay[..., None, :])  # [..., Nx+1, Ny+1, 3]
+    v00, v10 = d[..., :-1, :-1, :], d[..., 1:, :-1, :]
+    v11, v01 = d[..., 1:, 1:, :], d[..., :-1, 1:, :]
+    omega = _triangle_solid_angle(v00, v10, v11) + _triangle_solid_angle(v00, v11, v01)
+    return np.abs(omega)

@@ -20,6 +20,17 @@ tracing the residual and both Jacobian columns through the surfaces in
 registers; the walk back to the object is one ``optk_trace``.  The host
 keeps the outlines and the seeds (float64), as the reference does.
 
+A grating with several diffraction orders on the way (between the stops,
+or between the first stop and the object) gives the reference's stop rays
+the order axis: ``_ray_error`` broadcasts over the rulings'
+``diffraction_order``, so the residual, the Newton iterate and the solved
+rays carry it, and so do the field and pupil extents and the denormalized
+grid.  Here each order path is solved on its own (the surfaces with every
+multi-order grating restricted to that path's order); the stop rays, the
+extents and the denormalized grid gain a path axis in front of the outline
+axes, and the trace of a grid that carries one runs each path with its own
+inputs.
+
 The reference's root finder is ``named_arrays.optimize.root_newton`` with
 ``named_arrays.jacobian`` (``named_arrays~=2.1``, not vendored): Newton steps
 ``x <- x - J(x)^-1 f(x)`` until every residual component is below
@@ -32,6 +43,7 @@ outline's extent).
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 
 import numpy as np
 import torch
@@ -40,7 +52,8 @@ from . import _lib, _trace
 from .rays import RayVectorArray
 from .vectors import Cartesian3dVectorArray
 
-__all__ = ["solve_stop_rays", "rayfunction_stops", "field_pupil", "denormalize", "angles"]
+__all__ = ["solve_stop_rays", "rayfunction_stops", "field_pupil", "denormalize", "angles",
+           "num_stop_paths", "single_path"]
 
 MAX_ITERATIONS = 100
 
@@ -126,6 +139,38 @@ def _propagate(surfaces, position, direction, wavelength, system=None):
     return back[0:3], back[3:6]
 
 
+def single_path(surfaces, path: int) -> list:
+    """The surfaces with every multi-order grating restricted to the order
+    of diffraction-order path ``path`` (the order axis is shared by all
+    gratings, as the reference's named axis is)."""
+    from ._compile import _orders_of
+
+    out = []
+    for s in surfaces:
+        rl = getattr(s, "rulings", None)
+        if rl is not None:
+            orders = _orders_of(rl)
+            if len(orders) > 1:
+                s = dataclasses.replace(s, rulings=dataclasses.replace(
+                    rl, diffraction_order=orders[path]))
+        out.append(s)
+    return out
+
+
+def num_stop_paths(system) -> int:
+    """Length of the order axis the stop rays carry: that of the surfaces up
+    to the last stop, which hold every stop pair and the walk back to the
+    object."""
+    from ._compile import _num_paths
+
+    surfaces = system.surfaces_all
+    ip = [i for i, s in enumerate(surfaces) if s.is_pupil_stop]
+    ifs = [i for i, s in enumerate(surfaces) if s.is_field_stop]
+    if not ip or not ifs:
+        return 1
+    return _num_paths(surfaces[:max(ip[-1], ifs[-1]) + 1])
+
+
 def _wire(surface, num) -> np.ndarray:
     ap = surface.aperture
     if ap is None:
@@ -136,15 +181,18 @@ def _wire(surface, num) -> np.ndarray:
 
 
 def solve_stop_rays(system, wavelength, samples_pupil_stop: int = 21,
-                    samples_field_stop: int = 21) -> dict:
+                    samples_field_stop: int = 21, surfaces=None) -> dict:
     """Rays on the first stop joining the outlines of the field and pupil
     stops (``_sequential.py:396-662``), in the system frame.
 
     ``wavelength``: 1-D array (nm).  Returns ``position``, ``direction``
     [3][S1][S2][W] and the outline grids ``field``, ``pupil`` [2][S1][S2][W],
     with S1 the first stop's outline samples (``samples_pupil_stop``) and S2
-    the last stop's (``samples_field_stop``)."""
-    surfaces = system.surfaces_all
+    the last stop's (``samples_field_stop``).  ``surfaces``: the system's
+    surfaces (default ``surfaces_all``), one diffraction-order path of them
+    (``single_path``) for a system with several."""
+    if surfaces is None:
+        surfaces = system.surfaces_all
     ip = [i for i, s in enumerate(surfaces) if s.is_pupil_stop]
     ifs = [i for i, s in enumerate(surfaces) if s.is_field_stop]
     if not ip:
@@ -295,8 +343,24 @@ def rayfunction_stops(system, wavelength, samples_pupil_stop: int = 21,
                       samples_field_stop: int = 21) -> dict:
     """The stop rays propagated back to the object surface
     (``_sequential.py:664-712``): ``position``, ``direction`` [3][S1][S2][W]
-    in the system frame, directions pointing into the system."""
-    surfaces = system.surfaces_all
+    in the system frame, directions pointing into the system.  With P > 1
+    diffraction-order paths up to the last stop (``num_stop_paths``) they
+    are [3][P][S1][S2][W], each path solved and walked back on its own, and
+    ``num_paths`` = P."""
+    P = num_stop_paths(system)
+    if P == 1:
+        return _rayfunction_stops_path(system, wavelength, samples_pupil_stop, samples_field_stop,
+                                       system.surfaces_all)
+    per = [_rayfunction_stops_path(system, wavelength, samples_pupil_stop, samples_field_stop,
+                                   single_path(system.surfaces_all, p)) for p in range(P)]
+    return dict(position=np.stack([r["position"] for r in per], axis=1),
+                direction=np.stack([r["direction"] for r in per], axis=1),
+                wavelength=per[0]["wavelength"], num_paths=P)
+
+
+def _rayfunction_stops_path(system, wavelength, samples_pupil_stop, samples_field_stop,
+                            surfaces) -> dict:
+    """``rayfunction_stops`` over ``surfaces`` (a single order path)."""
     ip = [i for i, s in enumerate(surfaces) if s.is_pupil_stop]
     ifs = [i for i, s in enumerate(surfaces) if s.is_field_stop]
     if not ip:
@@ -305,7 +369,7 @@ def rayfunction_stops(system, wavelength, samples_pupil_stop: int = 21,
         raise ValueError("Field stop is not defined for this system.")
     index_stop = min(ip[-1], ifs[-1])
     back = surfaces[index_stop::-1]
-    r = solve_stop_rays(system, wavelength, samples_pupil_stop, samples_field_stop)
+    r = solve_stop_rays(system, wavelength, samples_pupil_stop, samples_field_stop, surfaces)
     shape = r["position"].shape[1:]
     p, d = _propagate(back, r["position"].reshape(3, -1), r["direction"].reshape(3, -1),
                       r["wavelength"].reshape(-1))
@@ -330,9 +394,9 @@ def rayfunction_stops(system, wavelength, samples_pupil_stop: int = 21,
 
 
 def field_pupil(system, rays: dict):
-    """(field, pupil) [2][S1][S2][W] of stop rays: angles (degrees) and
-    positions (mm), which is which by where the object is
-    (``_sequential.py:724-786``)."""
+    """(field, pupil) [2][S1][S2][W] (or [2][P][S1][S2][W]) of stop rays:
+    angles (degrees) and positions (mm), which is which by where the object
+    is (``_sequential.py:724-786``)."""
     p, d = rays["position"], rays["direction"]
     ax, ay = angles(d[0], d[1], d[2])
     ang = np.stack([ax, ay])
@@ -404,8 +468,8 @@ def denormalize(system, wavelength, field, pupil, normalized_field=True, normali
     fld, pup = field_pupil(system, rays)
 
     def extent(v):
-        lo = v.min(axis=(1, 2))
-        return lo, v.max(axis=(1, 2)) - lo  # [2][W]
+        lo = v.min(axis=(-3, -2))
+        return lo, v.max(axis=(-3, -2)) - lo  # [2][W], or [2][P][W]
 
     if not (normalized_field or normalized_pupil):
         return field, pupil
@@ -417,10 +481,18 @@ def denormalize(system, wavelength, field, pupil, normalized_field=True, normali
     if normalized_pupil:
         lo, ptp = extent(pup)
         items += [(pupil.x, lo[0], ptp[0]), (pupil.y, lo[1], ptp[1])]
-    table = torch.as_tensor(np.stack([np.stack([np.asarray(a, np.float64).reshape(-1),
-                                                np.asarray(b, np.float64).reshape(-1)])
-                                      for _, a, b in items]), device=dev)  # [C][2][W]
-    outs = _denormalize_device([c for c, _, _ in items], idx, table, dev)
+    P = rays.get("num_paths", 1)
+    per_path = []
+    for p in range(P):
+        pick = (lambda a: a) if P == 1 else (lambda a, p=p: np.asarray(a)[p])
+        table = torch.as_tensor(np.stack([np.stack([np.asarray(pick(a), np.float64).reshape(-1),
+                                                    np.asarray(pick(b), np.float64).reshape(-1)])
+                                          for _, a, b in items]), device=dev)  # [C][2][W]
+        per_path.append(_denormalize_device([c for c, _, _ in items], idx, table, dev))
+    # one path: the components as they are; several: each component gains
+    # the path axis in front ([P][...]), one denormalized grid per path
+    outs = per_path[0] if P == 1 else [torch.stack([o[c] for o in per_path])
+                                       for c in range(len(items))]
     k = 0
     if normalized_field:
         field = Cartesian2dVectorArray(outs[0], outs[1])

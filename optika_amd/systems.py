@@ -107,29 +107,32 @@ class AbstractSequentialSystem:
 
         return stops.field_pupil(self, self.rayfunction_stops)
 
+    # the extents reduce over both outline axes, the last two before the
+    # wavelength axis; a system with several diffraction-order paths up to
+    # its last stop keeps the path axis in front: [P][W]
     @property
     def field_min(self) -> Cartesian2dVectorArray:
         """Lower left corner of the field of view, per wavelength of
         ``grid_input`` (degrees at infinity, else mm; ``:724-734``)."""
         f, _ = self._field_pupil()
-        return Cartesian2dVectorArray(*f.min(axis=(1, 2)))
+        return Cartesian2dVectorArray(*f.min(axis=(-3, -2)))
 
     @property
     def field_max(self) -> Cartesian2dVectorArray:
         f, _ = self._field_pupil()
-        return Cartesian2dVectorArray(*f.max(axis=(1, 2)))
+        return Cartesian2dVectorArray(*f.max(axis=(-3, -2)))
 
     @property
     def pupil_min(self) -> Cartesian2dVectorArray:
         """Lower left corner of the entrance pupil, per wavelength of
         ``grid_input`` (mm at infinity, else degrees; ``:748-771``)."""
         _, p = self._field_pupil()
-        return Cartesian2dVectorArray(*p.min(axis=(1, 2)))
+        return Cartesian2dVectorArray(*p.min(axis=(-3, -2)))
 
     @property
     def pupil_max(self) -> Cartesian2dVectorArray:
         _, p = self._field_pupil()
-        return Cartesian2dVectorArray(*p.max(axis=(1, 2)))
+        return Cartesian2dVectorArray(*p.max(axis=(-3, -2)))
 
     def _physical_grid(self, grid: ObjectVectorArray, normalized_field: bool,
                        normalized_pupil: bool) -> ObjectVectorArray:
@@ -277,6 +280,8 @@ class AbstractSequentialSystem:
                                      np.float64).reshape(-1)
             vx, vy = h(pupil.x), h(pupil.y)
         if normalized_pupil:
+            if stops.num_stop_paths(self) > 1:
+                raise NotImplementedError("effective area of a system with several diffraction orders")
             uniq, inv = np.unique(wl, return_inverse=True)
             _, pup = stops.field_pupil(self, stops.rayfunction_stops(self, uniq))
             lo, hi = pup.min(axis=(1, 2)), pup.max(axis=(1, 2))
@@ -299,6 +304,8 @@ class AbstractSequentialSystem:
         [2][W], from the stop solve."""
         from . import stops
 
+        if stops.num_stop_paths(self) > 1:
+            raise NotImplementedError("image of a system with several diffraction orders")
         uniq, inv = np.unique(wavelength, return_inverse=True)
         fld, pup = stops.field_pupil(self, stops.rayfunction_stops(self, uniq))
         ext = []
@@ -311,39 +318,66 @@ class AbstractSequentialSystem:
               normalized_field: bool = False, normalized_pupil: bool = True,
               integrate: bool = True, noise: bool = True, seed: int = 0):
         """Forward model: the spectral radiance of a scene to the electrons
-        of every pixel (``_sequential.py:1105-1206``).
+        of every pixel (``_sequential.py:1002-1206``).
 
         ``wavelength`` (W + 1), ``field_x`` (X + 1) and ``field_y`` (Y + 1)
         are cell vertices, ``radiance`` is [W][X][Y] per cell (photons per
-        unit time, wavelength, field and pupil area).  ``pupil`` holds the
-        pupil grid's vertex coordinates along x and y (default one cell over
-        the normalized pupil).  One ray per (wavelength, field, pupil) cell
-        at a random point of the cell (the reference's
-        ``cell_centers(random=True)``; its random stream is not reproduced)
-        carries radiance x cell volume; the rays are generated and traced on
-        the device, binned on the sensor's pixels over the wavelength cells
-        and exposed (``measure``).  Returns [W][Nx][Ny] electrons, or [Nx][Ny] with
-        ``integrate``."""
+        unit time, wavelength, field measure and pupil measure).
+        ``pupil`` holds the pupil grid's vertex coordinates along x and y
+        (default one cell over the normalized pupil).
+
+        As ``_rayfunction_from_vertices`` (``:1002-1083``): normalized
+        vertices are denormalized at the vertex wavelengths (the stop solve
+        at each); each ray carries radiance x cell volume, where the volume
+        (``ObjectVectorArray.cell_area``,
+        ``optika/vectors/_vectors_object.py:52-133``) is the wavelength
+        step times the field cell's measure times the pupil cell's measure:
+        an angular field or pupil cell is measured by its solid angle in
+        steradians (``solid_angle_cells``), a positional one by its area in
+        mm^2; the field measure is averaged over the cell's two wavelength
+        vertices, the pupil measure over the cell's wavelength and field
+        vertices.  One ray per (wavelength, field, pupil) cell sits at a
+        uniformly random point of the cell: every coordinate is the
+        multilinear interpolation of the cell's vertices with uniform
+        weights (the reference's ``cell_centers(random=True)``; its random
+        stream is not reproduced).  The rays are generated and traced on the
+        device, binned on the sensor's pixels over the wavelength cells and
+        exposed (``measure``).  Returns [W][Nx][Ny] electrons, or [Nx][Ny]
+        with ``integrate``."""
+        from .pupil import solid_angle_cells
+
         h = lambda v: np.asarray(v.detach().cpu().numpy() if hasattr(v, "detach") else v,  # noqa: E731
                                  np.float64).reshape(-1)
-        rng = np.random.default_rng(seed)
         wv, fxv, fyv = h(wavelength), h(field_x), h(field_y)
         W, X, Y = wv.size - 1, fxv.size - 1, fyv.size - 1
         R = h(radiance).reshape(W, X, Y)
         pxv, pyv = (np.array([-1.0, 1.0]), np.array([-1.0, 1.0])) if pupil is None else \
             (h(pupil.x), h(pupil.y))
         PX, PY = pxv.size - 1, pyv.size - 1
-        wl = wv[:-1] + np.diff(wv) * rng.uniform(size=W)
-        fx_w, fy_w = np.broadcast_to(fxv, (W, X + 1)), np.broadcast_to(fyv, (W, Y + 1))
-        px_w, py_w = np.broadcast_to(pxv, (W, PX + 1)), np.broadcast_to(pyv, (W, PY + 1))
+        # vertex coordinates per wavelength vertex: [W + 1][.. + 1]
+        fx_v, fy_v = np.broadcast_to(fxv, (W + 1, X + 1)), np.broadcast_to(fyv, (W + 1, Y + 1))
+        px_v, py_v = np.broadcast_to(pxv, (W + 1, PX + 1)), np.broadcast_to(pyv, (W + 1, PY + 1))
         if normalized_field or normalized_pupil:
-            flo, fspan, plo, pspan = self._extents(wl)
+            flo, fspan, plo, pspan = self._extents(wv)
             if normalized_field:
-                fx_w = fspan[0][:, None] * (fxv + 1) / 2 + flo[0][:, None]
-                fy_w = fspan[1][:, None] * (fyv + 1) / 2 + flo[1][:, None]
+                fx_v = fspan[0][:, None] * (fxv + 1) / 2 + flo[0][:, None]
+                fy_v = fspan[1][:, None] * (fyv + 1) / 2 + flo[1][:, None]
             if normalized_pupil:
-                px_w = pspan[0][:, None] * (pxv + 1) / 2 + plo[0][:, None]
-                py_w = pspan[1][:, None] * (pyv + 1) / 2 + plo[1][:, None]
+                px_v = pspan[0][:, None] * (pxv + 1) / 2 + plo[0][:, None]
+                py_v = pspan[1][:, None] * (pyv + 1) / 2 + plo[1][:, None]
+        # cell measures (cell_area): the field is an angle for an object at
+        # infinity and the pupil a position, else the other way round
+        at_inf = self.object_is_at_infinity
+
+        def measure(vx, vy, angular):
+            if angular:
+                return solid_angle_cells(vx, vy)
+            return np.abs(np.diff(vx, axis=-1)[..., :, None] * np.diff(vy, axis=-1)[..., None, :])
+
+        a_field = measure(fx_v, fy_v, at_inf)           # [W + 1][X][Y]
+        a_pupil = measure(px_v, py_v, not at_inf)       # [W + 1][PX][PY]
+        a_field = (a_field[:-1] + a_field[1:]) / 2      # cell centers along wavelength
+        a_pupil = (a_pupil[:-1] + a_pupil[1:]) / 2
         shape = (W, X, Y, PX, PY)
         import torch
 
@@ -355,19 +389,30 @@ class AbstractSequentialSystem:
         gen = torch.Generator(device=dev)
         gen.manual_seed(int(seed))
         T = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float64), device=dev)  # noqa: E731
-        fxw, fyw, pxw, pyw = T(fx_w), T(fy_w), T(px_w), T(py_w)
-        u = torch.rand((4,) + shape, generator=gen, device=dev, dtype=torch.float64)
-        dfx, dfy = fxw[:, 1:] - fxw[:, :-1], fyw[:, 1:] - fyw[:, :-1]
-        dpx, dpy = pxw[:, 1:] - pxw[:, :-1], pyw[:, 1:] - pyw[:, :-1]
-        fx = fxw[:, :-1, None, None, None] + dfx[:, :, None, None, None] * u[0]
-        fy = fyw[:, None, :-1, None, None] + dfy[:, None, :, None, None] * u[1]
-        px = pxw[:, None, None, :-1, None] + dpx[:, None, None, :, None] * u[2]
-        py = pyw[:, None, None, None, :-1] + dpy[:, None, None, None, :] * u[3]
-        volume = (T(np.diff(wv))[:, None, None, None, None] * dfx[:, :, None, None, None]
-                  * dfy[:, None, :, None, None] * dpx[:, None, None, :, None]
-                  * dpy[:, None, None, None, :]).abs()
+        u = torch.rand((5,) + shape, generator=gen, device=dev, dtype=torch.float64)
+        uw = u[0]
+        wvt = T(wv)
+        lam = wvt[:-1, None, None, None, None] + (wvt[1:] - wvt[:-1])[:, None, None, None, None] * uw
+
+        def between(v, axis_shape, uc):
+            """the coordinate at weight uw between wavelength vertices and
+            uc between the cell's two vertices along its own axis"""
+            v = T(v)
+            lo = v[:-1, :-1].reshape(axis_shape)       # [W][n] at the cell's lower vertex
+            hi = v[:-1, 1:].reshape(axis_shape)
+            lo2 = v[1:, :-1].reshape(axis_shape)        # next wavelength vertex
+            hi2 = v[1:, 1:].reshape(axis_shape)
+            a = lo + (lo2 - lo) * uw
+            b = hi + (hi2 - hi) * uw
+            return a + (b - a) * uc
+
+        fx = between(fx_v, (W, X, 1, 1, 1), u[1])
+        fy = between(fy_v, (W, 1, Y, 1, 1), u[2])
+        px = between(px_v, (W, 1, 1, PX, 1), u[3])
+        py = between(py_v, (W, 1, 1, 1, PY), u[4])
+        volume = (T(np.abs(np.diff(wv)))[:, None, None, None, None]
+                  * T(a_field)[:, :, :, None, None] * T(a_pupil)[:, None, None, :, :])
         flux = T(R)[:, :, :, None, None] * volume
-        lam = T(wl)[:, None, None, None, None].expand(shape)
         flat = lambda a: a.expand(shape).reshape(-1).contiguous()  # noqa: E731
         rays = self.rayfunction(intensity=flat(flux), wavelength=flat(lam),
                                 field=Cartesian2dVectorArray(flat(fx), flat(fy)),

@@ -28,7 +28,7 @@ import numpy as np
 from . import trace as T
 
 __all__ = ["wire", "surfaces_all", "stop_rays", "rayfunction_stops", "field_pupil_extent",
-           "denormalize", "angles"]
+           "denormalize", "angles", "num_paths"]
 
 
 def angles(d):
@@ -139,15 +139,31 @@ def _to_local(t, p, direction=False):
     return T.transform_inverse(t, p)[0]
 
 
-def _trace(surfaces, pos, dirs, wl):
+def _trace(surfaces, pos, dirs, wl, path=0):
+    """Rays through ``surfaces`` along diffraction-order path ``path`` (a
+    sequence without a multi-order surface has the one path 0)."""
     rays = T.make_rays(wl, pos, dirs)
     paths = T.propagate(surfaces, rays)
-    assert len(paths) == 1
-    return paths[0]["position"], paths[0]["direction"]
+    r = paths[path if len(paths) > 1 else 0]
+    return r["position"], r["direction"]
 
 
-def stop_rays(system, wavelength, s1=21, s2=21, max_iter=100):
-    """Rays on the first stop (system frame) for the last stop pair:
+def num_paths(system) -> int:
+    """Length of the diffraction-order axis that the stop rays carry: that of
+    the surfaces up to the last stop (the stop pairs and the walk back to the
+    object).  A grating with several orders between the stops gives the
+    solve's residual, and so its root, the order axis
+    (``_sequential.py:396-662``: the named-array broadcast of ``_ray_error``
+    over the rulings' ``diffraction_order``); each order is solved on its own."""
+    flags = surfaces_all(system)
+    last = max(max(i for i, (_, _, p) in enumerate(flags) if p),
+               max(i for i, (_, f, _) in enumerate(flags) if f))
+    return len(T.system_orders([s for s, _, _ in flags[:last + 1]]))
+
+
+def stop_rays(system, wavelength, s1=21, s2=21, max_iter=100, path=0):
+    """Rays on the first stop (system frame) for the last stop pair, along
+    diffraction-order path ``path``:
     dict(position, direction [3][s1][s2][W], field, pupil [2][s1][s2][W])."""
     flags = surfaces_all(system)
     surfaces = [s for s, _, _ in flags]
@@ -212,7 +228,7 @@ def stop_rays(system, wavelength, s1=21, s2=21, max_iter=100):
         def f(x):
             v = np.stack([x[0], x[1], z_of(x)])
             p, d = (pos, v) if free_direction else (v, dirs)
-            p2, d2 = _trace(sub[1:], p, d, lam)
+            p2, d2 = _trace(sub[1:], p, d, lam, path)
             q = _to_local(last.transformation, d2 if target_direction else p2,
                           direction=target_direction)
             return q[:2] - g2
@@ -241,18 +257,19 @@ def stop_rays(system, wavelength, s1=21, s2=21, max_iter=100):
     return out
 
 
-def rayfunction_stops(system, wavelength, s1=21, s2=21):
-    """The stop rays walked back to the object (``_sequential.py:664-712``)."""
+def rayfunction_stops(system, wavelength, s1=21, s2=21, path=0):
+    """The stop rays walked back to the object (``_sequential.py:664-712``),
+    along diffraction-order path ``path``."""
     flags = surfaces_all(system)
     surfaces = [s for s, _, _ in flags]
     ip = [i for i, (_, _, p) in enumerate(flags) if p]
     ifs = [i for i, (_, f, _) in enumerate(flags) if f]
     k = min(ip[-1], ifs[-1])
     back = surfaces[k::-1]
-    r = stop_rays(system, wavelength, s1, s2)
+    r = stop_rays(system, wavelength, s1, s2, path=path)
     shape = r["position"].shape[1:]
     p, d = _trace(back, r["position"].reshape(3, -1), r["direction"].reshape(3, -1),
-                  r["wavelength"].reshape(-1))
+                  r["wavelength"].reshape(-1), path)
     obj = back[-1]
     pl = _to_local(obj.transformation, p)
     dl = _to_local(obj.transformation, d, direction=True)
@@ -265,9 +282,10 @@ def rayfunction_stops(system, wavelength, s1=21, s2=21):
     return dict(position=p.reshape((3,) + shape), direction=d.reshape((3,) + shape))
 
 
-def field_pupil_extent(system, wavelength, s=21):
-    """(field, pupil) as (min [2][W], max [2][W]) pairs."""
-    r = rayfunction_stops(system, wavelength, s, s)
+def field_pupil_extent(system, wavelength, s=21, path=0):
+    """(field, pupil) as (min [2][W], max [2][W]) pairs, along
+    diffraction-order path ``path``."""
+    r = rayfunction_stops(system, wavelength, s, s, path=path)
     ang = angles(r["direction"])
     xy = r["position"][:2]
     field, pupil = (ang, xy) if T._object_at_infinity(system) else (xy, ang)
@@ -275,10 +293,10 @@ def field_pupil_extent(system, wavelength, s=21):
     return ext(field), ext(pupil)
 
 
-def denormalize(system, wavelength, field_x, field_y, pupil_x, pupil_y, s=21):
+def denormalize(system, wavelength, field_x, field_y, pupil_x, pupil_y, s=21, path=0):
     """Normalized coordinates in [-1, 1] onto the stop rays' extent, at one
-    wavelength (scalar)."""
-    (flo, fhi), (plo, phi) = field_pupil_extent(system, [wavelength], s)
+    wavelength (scalar), along diffraction-order path ``path``."""
+    (flo, fhi), (plo, phi) = field_pupil_extent(system, [wavelength], s, path=path)
     fx = (fhi[0, 0] - flo[0, 0]) * (np.asarray(field_x) + 1) / 2 + flo[0, 0]
     fy = (fhi[1, 0] - flo[1, 0]) * (np.asarray(field_y) + 1) / 2 + flo[1, 0]
     px = (phi[0, 0] - plo[0, 0]) * (np.asarray(pupil_x) + 1) / 2 + plo[0, 0]

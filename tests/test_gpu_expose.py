@@ -6,8 +6,10 @@
   transcendental functions at an accept/reject boundary; without noise it
   is the expectation exactly;
 * ``SequentialSystem.image`` of a uniform scene through the stop-test
-  Newtonian collects radiance x wavelength band x field area x entrance
-  pupil area (pi (40/3)^2 mm^2, the 10 mm stop imaged by the primary)."""
+  Newtonian collects radiance x wavelength band x field solid angle x
+  entrance pupil area (pi (40/3)^2 mm^2, the 10 mm stop imaged by the
+  primary); the field's solid angle in steradians is the closed form
+  dx (sin y1 - sin y0) of the angular square on the unit sphere."""
 
 import numpy as np
 import pytest
@@ -42,10 +44,11 @@ def test_image_of_a_uniform_scene():
     system.sensor.timedelta_exposure = 2.0
     wv = np.array([500.0, 520.0, 540.0])
     f = np.linspace(-0.03, 0.03, 7)
-    R = np.full((2, 6, 6), 1.0)
+    R = np.full((2, 6, 6), 1e6)  # photons / (s nm sr mm^2)
     v = np.linspace(-1, 1, 31)
     img = system.image(wv, f, f, R, pupil=Cartesian2dVectorArray(v, v), noise=False, seed=4)
-    want = 2.0 * 1.0 * 40.0 * 0.06 ** 2 * np.pi * (10.0 / 0.75) ** 2
+    omega = np.deg2rad(0.06) * 2 * np.sin(np.deg2rad(0.03))  # sr
+    want = 2.0 * 1e6 * 40.0 * omega * np.pi * (10.0 / 0.75) ** 2
     total = float(img.sum())
     assert abs(total - want) < 0.02 * want, (total, want)
     per_band = system.image(wv, f, f, R, pupil=Cartesian2dVectorArray(v, v), noise=False, seed=4,

@@ -109,3 +109,75 @@ def test_device_newton_roots_land_on_the_outline():
     want = S.wire(surf[2].aperture, 21)[:2]
     got = local[:2].reshape(2, 21, 21)
     assert np.all(np.abs(got - want[:, None, :]) < 2e-8 + 1e-12)
+
+
+def test_spectrograph_normalized_rayfunction_per_order():
+    """The spectrograph's orders -1 and +1 both lie between its pupil stop
+    (the primary) and its field stop: the reference's stop solve carries the
+    order axis, so each order has its own field and pupil extent and the
+    normalized grid is denormalized per order (_sequential.py:396-789).
+    Each path's denormalized inputs and traced rays match the oracle's
+    solve and trace of that path."""
+    from optika_amd import stops, workloads
+
+    print("case: spectrograph normalized rayfunction, both orders", flush=True)
+    system = workloads.spectrograph()
+    assert stops.num_stop_paths(system) == 2
+    rng = np.random.default_rng(6)
+    n = 3000
+    u = rng.uniform(-1, 1, (4, n))
+    dev = torch.device("cuda")
+    t = lambda a: torch.as_tensor(a, device=dev)  # noqa: E731
+    out = system.rayfunction(wavelength=60.0, field=Cartesian2dVectorArray(t(u[0]), t(u[1])),
+                             pupil=Cartesian2dVectorArray(t(u[2]), t(u[3])))
+    r = out.outputs
+    assert r.position.x.shape == (2, n)
+    assert out.inputs.field.x.shape == (2, n)
+    ext = stops.field_pupil(system, stops.rayfunction_stops(system, [60.0]))
+    for p in range(2):
+        (flo, fhi), (plo, phi) = S.field_pupil_extent(system, [60.0], path=p)
+        for v, lo, hi in ((ext[0], flo, fhi), (ext[1], plo, phi)):
+            np.testing.assert_allclose(v[:, p].min(axis=(1, 2)), lo, rtol=0, atol=2e-7)
+            np.testing.assert_allclose(v[:, p].max(axis=(1, 2)), hi, rtol=0, atol=2e-7)
+        fx, fy, px, py = S.denormalize(system, 60.0, u[0], u[1], u[2], u[3], path=p)
+        np.testing.assert_allclose(out.inputs.field.x[p].cpu().numpy(), fx, rtol=0, atol=1e-8)
+        np.testing.assert_allclose(out.inputs.field.y[p].cpu().numpy(), fy, rtol=0, atol=1e-8)
+        np.testing.assert_allclose(out.inputs.pupil.x[p].cpu().numpy(), px, rtol=0, atol=2e-7)
+        np.testing.assert_allclose(out.inputs.pupil.y[p].cpu().numpy(), py, rtol=0, atol=2e-7)
+        ref = T.rayfunction(system, 60.0, fx, fy, px, py)[p]
+        m = r.unvignetted[p].cpu().numpy()
+        assert (m == ref["unvignetted"]).mean() > 0.999  # rays within ~1e-9 of an edge may differ
+        # order -1 reaches the sensor; order +1, whose field the stops allow,
+        # is clipped downstream of the field stop in the reference as here
+        assert ref["unvignetted"].mean() > (0.3 if p == 0 else -1.0)
+        for k, c in enumerate((r.position.x[p], r.position.y[p], r.position.z[p])):
+            g, w = c.cpu().numpy(), ref["position"][k]
+            fin = np.isfinite(g) & np.isfinite(w)
+            assert fin.mean() > 0.99
+            np.testing.assert_allclose(g[fin], w[fin], rtol=0, atol=2e-6)
+    # the extents differ between the orders (the +1 order sees another field)
+    assert abs(ext[0][0, 0].min() - ext[0][0, 1].min()) > 1.0
+
+
+def test_device_roots_on_a_tilted_decentred_stop_match_the_host_newton():
+    """A perturbed spectrograph (primary tilted and decentred, the first
+    stop of the solve): the device's per-ray Newton stopping rule (each ray
+    stops when its own residual is below tolerance) and the oracle's
+    whole-array rule (every ray iterates until all are below it) give roots
+    within 1e-9 of each other in direction cosine, per order path."""
+    from optika_amd import stops, workloads
+
+    print("case: tilted/decentred stop roots vs host Newton", flush=True)
+    pert = workloads.Perturbation(primary_tilt=(0.01, -0.012), primary_decenter=(0.05, -0.03),
+                                  primary_radius=0.001, grating_tilt=(0.0, 0.0),
+                                  grating_decenter=(0.0, 0.0), grating_radii=(0.0, 0.0),
+                                  spacing=(0.0, 0.0, 0.0))
+    system = workloads.spectrograph(pert)
+    surfaces = system.surfaces_all
+    for p in range(2):
+        got = stops.solve_stop_rays(system, [59.0, 62.0], surfaces=stops.single_path(surfaces, p))
+        want = S.stop_rays(system, [59.0, 62.0], path=p)
+        np.testing.assert_allclose(got["position"], want["position"], rtol=1e-12, atol=1e-12)
+        err = np.abs(got["direction"] - want["direction"]).max()
+        print(f"path {p}: max direction difference {err:.2e}", flush=True)
+        assert err < 1e-9

@@ -56,3 +56,29 @@ def test_interpolated_effective_area():
     m = radiometry.InterpolatedEffectiveAreaModel(wavelength=np.array([400.0, 500.0, 700.0]),
                                                   area=np.array([1.0, 3.0, 2.0]))
     np.testing.assert_allclose(m([400.0, 450.0, 600.0, 800.0]), [1.0, 2.0, 2.5, 2.0])
+
+
+def test_solid_angle_cells_closed_forms():
+    """The angular cell measure of the forward model (steradians): an
+    octant is pi/2; a grid covering the sphere sums to 4 pi (adjacent
+    cells share their great-circle edges); small cells tend to the area of
+    the coordinate cell on the unit sphere, dx (sin y1 - sin y0), whose
+    difference from the great-circle quadrilateral is second order in the
+    cell size; cells are symmetric in the signs of both angles."""
+    assert abs(pupil.solid_angle_cells([0.0, 90.0], [0.0, 90.0])[0, 0] - np.pi / 2) < 1e-15
+    x, y = np.linspace(-180, 180, 73), np.linspace(-90, 90, 37)
+    assert abs(pupil.solid_angle_cells(x, y).sum() - 4 * np.pi) < 1e-12
+    for (x0, x1, y0, y1), tol in (((0.1, 0.16, -0.03, 0.03), 1e-6),
+                                  ((10.0, 10.5, 40.0, 40.5), 3e-5),
+                                  ((-3.0, -2.0, -60.0, -59.0), 2e-4)):
+        exact = np.deg2rad(x1 - x0) * (np.sin(np.deg2rad(y1)) - np.sin(np.deg2rad(y0)))
+        got = pupil.solid_angle_cells([x0, x1], [y0, y1])[0, 0]
+        assert abs(got / exact - 1) < tol, (got, exact)
+    a = pupil.solid_angle_cells([0.2, 0.5], [0.1, 0.3])
+    b = pupil.solid_angle_cells([-0.5, -0.2], [-0.3, -0.1])
+    np.testing.assert_allclose(a, b, rtol=1e-12)
+    # leading axes broadcast (one vertex row per wavelength)
+    per_w = pupil.solid_angle_cells(np.array([[0.0, 1.0, 2.0], [0.0, 2.0, 4.0]]), [0.0, 1.0])
+    assert per_w.shape == (2, 2, 1)
+    np.testing.assert_allclose(per_w[1].sum(), pupil.solid_angle_cells([0.0, 4.0], [0.0, 1.0]).sum(),
+                               rtol=1e-3)
