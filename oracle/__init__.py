@@ -18,7 +18,14 @@ Where the reference's arithmetic lives in its third-party substrate
 fixes an explicit convention and says so:
 
 * transformations: standard active rotations, lists applied in order
-  (pinned by the geometry of ``optika/systems/_sequential.py:1853-1912``);
+  (pinned by the geometry of ``optika/systems/_sequential.py:1853-1912``),
+  a list composed into one affine map ``p -> m p + t`` (matrix products in
+  list order) and applied as such, inverse ``m^T (p - t)``; whether
+  ``named_arrays`` applies a list as one composed map or element by element
+  is not visible here (the two differ by rounding only: ~1e-12 mm after a
+  metre-long path through a conic whose intercept cancels catastrophically,
+  e.g. a tilted and decentred spectrograph) -- pinned against the element
+  by element application to 1e-12 and by inverse-of-apply identity;
 * ``na.interp``: ``numpy.interp`` (clamping) -- pinned by the IMD golden
   reflectivity curves of ``optika/materials/_tests/_data``;
 * ``na.optimize.root_secant`` (toroidal intercept): replaced by a Newton

@@ -146,40 +146,47 @@ def _matvec(m, v):
     ])
 
 
-def _apply_steps(steps, position, direction):
+def _compose(steps):
+    """The one affine map ``p -> m p + t`` that applies ``steps`` in order:
+    a translation adds to t; a rotation r maps (m, t) to (r m, r t), the
+    first rotation (r, r t).  Returns (m, t, has_rotation); a map without a
+    rotation only translates."""
+    m, t, rotated = np.eye(3), np.zeros(3), False
     for kind, value in steps:
         if kind == "t":
-            position = position + value[:, None]
+            t = t + value
+        elif not rotated:
+            m, t, rotated = value, value @ t + np.zeros(3), True
         else:
-            position = _matvec(value, position)
-            if direction is not None:
-                direction = _matvec(value, direction)
-    return position, direction
-
-
-def _inverse_steps(steps):
-    inv = []
-    for kind, value in reversed(steps):
-        if kind == "t":
-            inv.append(("t", -value))
-        else:
-            inv.append(("r", value.T.copy()))
-    return inv
+            m, t = value @ m, value @ t + np.zeros(3)
+    return m, t, rotated
 
 
 def transform_apply(transformation, position, direction=None):
     """Local -> parent frame (``RayVectorArray.__array_matmul__/__array_add__``,
-    ``optika/rays/_ray_vectors.py:105-166``)."""
+    ``optika/rays/_ray_vectors.py:105-166``): the list composed into one
+    affine map, positions ``m p + t``, directions ``m d``."""
     if transformation is None:
         return position, direction
-    return _apply_steps(_primitives(transformation), position, direction)
+    m, t, rotated = _compose(_primitives(transformation))
+    if rotated:
+        position = _matvec(m, position)
+        if direction is not None:
+            direction = _matvec(m, direction)
+    return position + t[:, None], direction
 
 
 def transform_inverse(transformation, position, direction=None):
-    """Parent -> local frame."""
+    """Parent -> local frame: ``m^T (p - t)``, directions ``m^T d``."""
     if transformation is None:
         return position, direction
-    return _apply_steps(_inverse_steps(_primitives(transformation)), position, direction)
+    m, t, rotated = _compose(_primitives(transformation))
+    position = position - t[:, None]
+    if rotated:
+        position = _matvec(m.T, position)
+        if direction is not None:
+            direction = _matvec(m.T, direction)
+    return position, direction
 
 
 def _rays_inverse(transformation, rays):

@@ -376,3 +376,39 @@ def test_multilayer_ambient_absorption(constant_index):
     np.testing.assert_allclose(eff, np.abs((na - ns) / (na + ns)) ** 2, rtol=1e-13)
     rays0 = oracle.make_rays(wl, np.zeros((3, 2)), np.tile([[0.0], [0.0], [1.0]], (1, 2)))
     assert not np.allclose(oracle.material_efficiency(mirror, rays0, NORMAL), eff)
+
+
+# -- transformations ----------------------------------------------------------------
+
+def test_composed_transformation_equals_element_by_element():
+    """A transformation list is applied as one composed affine map; that map
+    equals applying the elements one after another (rotations on position
+    and direction, translations on position) to rounding, and the inverse
+    undoes it."""
+    from optika_amd.transformations import (Cartesian3dRotationX, Cartesian3dRotationY,
+                                            Cartesian3dTranslation, TransformationList)
+
+    rng = np.random.default_rng(8)
+    p = rng.uniform(-500, 500, (3, 64))
+    d = rng.normal(size=(3, 64))
+    d /= np.sqrt((d ** 2).sum(0))
+    t = TransformationList([Cartesian3dRotationX(0.013), Cartesian3dTranslation(x=3.0, y=-2.0),
+                            Cartesian3dRotationY(-12.0), Cartesian3dRotationZ(135.0),
+                            Cartesian3dTranslation(z=1000.0)])
+    q, e = p.copy(), d.copy()
+    for el in t.elements():
+        if isinstance(el, Cartesian3dTranslation):
+            q = q + np.array([[el.x], [el.y], [el.z]])
+        else:
+            m = el.matrix
+            q, e = m @ q, m @ e
+    got_p, got_d = oracle.transform_apply(t, p, d)
+    np.testing.assert_allclose(got_p, q, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(got_d, e, rtol=0, atol=1e-15)
+    back_p, back_d = oracle.transform_inverse(t, got_p, got_d)
+    np.testing.assert_allclose(back_p, p, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(back_d, d, rtol=0, atol=1e-15)
+    # translations alone are added, not multiplied (infinite coordinates stay free of 0 * inf)
+    tt = TransformationList([Cartesian3dTranslation(x=1.0), Cartesian3dTranslation(z=-2.0)])
+    inf = np.array([[np.inf], [0.0], [0.0]])
+    assert np.array_equal(oracle.transform_apply(tt, inf)[0], np.array([[np.inf], [0.0], [-2.0]]))
