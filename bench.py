@@ -7,23 +7,26 @@ uniform in +-0.25 deg and wavelengths uniform in 400-700 nm (seed 1).
 Inputs are resident in HBM before the timed region; every step traces all
 rays through every surface into preallocated output buffers.
 
-Prints one JSON line (see the fields at the end of each workload).
-``value`` is the float32 compensated form; ``forms`` holds the float64
-drop-in form (identical to the oracle to 1e-12) and the float32 form's RMS
-spot and direction error against it over all rays of the run.  The roofline
-object is for the float32 kernel, timed with HIP events on the stream it
-runs on; ``cpu_baseline`` times the numpy float64 oracle (a restatement of
-the reference's vectorized numpy trace) on a bounded sample on this
-machine's host.
+Prints one JSON line.  ``value`` is the float64 drop-in form (the
+reference's arithmetic, identical to the oracle to 1e-12); ``forms`` holds
+the metric's other legs, each with its error against the float64 form: the
+float32 compensated form on the same rays, and the learned surrogate of the
+EUV spectrograph (trained here on 2e8 float64-traced rays, timed on 1e9
+inference rays, judged against the float64 trace of that system).
+``roofline`` is the float64 kernel's (``forms.f32.roofline`` the float32
+kernel's), timed with HIP events on the stream it runs on.  ``cpu_baseline``
+times the numpy float64 oracle (a restatement of the reference's
+vectorized numpy trace) on this machine's host cores, in worker processes
+started before anything touches the GPU: N cores (one process per core) and
+one core.
 
 Other workloads (``--workload``): ``spectrograph`` (the EUV spectrograph,
 4 counted surfaces, two diffraction orders, Mo/Si multilayers on both
 mirrors; 50M rays) and ``tolerance`` (4096 perturbed spectrographs with
 log-uniform ray counts in one persistent batched launch, autotuned and
-replayed from a hipGraph), ``surrogate`` (the learned surrogate of the
-spectrograph: training from float64 traces, then fused inference) and
-``stops`` (normalized field/pupil coordinates: the stop solve on the device,
-then the trace of the denormalized grid, both inside each step) and
+replayed from a hipGraph), ``surrogate`` (the learned surrogate alone) and
+``stops`` (normalized field/pupil coordinates: the stop solve on the
+device, then the trace of the denormalized grid, both inside each step) and
 ``rays`` (``propagate_rays``, the ray entry point, on the Newtonian).
 """
 
@@ -50,14 +53,16 @@ BYTES_PER_RAY_F64 = 97
 
 # HBM bytes per launch of trace_grid_f64 at 2^24 rays, from rocprofv3 --pmc:
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 256 MiB device copy);
-# the runs and commands are in profiles/trace_grid_f64_newtonian_2p24.md
+# the runs and commands are in the profile summary named here
 TRAFFIC_BYTES_F64_2P24 = (327867 * 2.0 + 934303 * 1.0) * 1024
+TRAFFIC_SOURCE_F64 = "profiles/trace_grid_f64_newtonian_2p24.md (rocprofv3 --pmc)"
 
 # float32 form: in 5 x 4 B, out 7 x 4 B + 1 B mask
 BYTES_PER_RAY_F32 = 49
 # HBM bytes per launch of trace_grid_f32 at 2^24 rays (rocprofv3 --pmc, the
 # same scaling; profiles/trace_grid_f32_newtonian_2p24.md)
 TRAFFIC_BYTES_F32_2P24 = (163964 * 2.0 + 475159 * 1.0) * 1024
+TRAFFIC_SOURCE_F32 = "profiles/trace_grid_f32_newtonian_2p24.md (rocprofv3 --pmc)"
 
 PIXEL_MM = 0.015
 
@@ -79,6 +84,10 @@ def parse():
     p.add_argument("--cpu-rays", type=int, default=None,
                    help="rays in the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-surrogate", action="store_true",
+                   help="default workload: skip the surrogate leg")
+    p.add_argument("--infer-rays", type=int, default=1_000_000_000,
+                   help="surrogate leg: inference rays per step")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a 256 MiB device copy (to scale HBM byte counters)")
     return p.parse_args()
@@ -113,6 +122,78 @@ def cpu_baseline(system, grid, sample: int, surfaces: int, paths: int = 1) -> di
                   f"{', both diffraction orders' if paths > 1 else ''}), numpy float64 oracle, "
                   f"median of 3 after 1 warm-up, {os.cpu_count()} host CPUs present, "
                   f"1 thread used",
+    }
+
+
+# the oracle's job in the worker processes of the N-core baseline (set
+# before the pool forks, so every worker inherits the system and the grid)
+_CPU_JOB = None
+
+
+def _cpu_chunk(span):
+    import oracle
+
+    system, grid = _CPU_JOB
+    sub = grid.subset(span[1], start=span[0])
+    t0 = time.perf_counter()
+    oracle.rayfunction(system, sub.wavelength, sub.field_x, sub.field_y, sub.pupil_x, sub.pupil_y)
+    return time.perf_counter() - t0
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baselines(system, grid, surfaces: int, rays_1: int, rays_n: int, paths: int = 1) -> dict:
+    """The numpy float64 oracle on this host: N cores (``rays_n`` rays split
+    over one worker process per core, wall clock of the whole map) and one
+    core (``rays_1`` rays in this process); 1 warm-up and the median of 3
+    repeats each.  Runs before anything touches the GPU: the workers are
+    forked from this process, which must not hold a GPU context.  The
+    numpy oracle holds the GIL through its many small array operations, so
+    threads do not scale it; processes do."""
+    import multiprocessing
+
+    global _CPU_JOB
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))  # the box's CPU share for one GPU
+    one = cpu_baseline(system, grid, rays_1, surfaces, paths)
+    _CPU_JOB = (system, grid)
+    per = (rays_n + cores - 1) // cores
+    spans = [(k * per, min(per, rays_n - k * per)) for k in range(cores) if k * per < rays_n]
+    walls = []
+    with multiprocessing.get_context("fork").Pool(len(spans)) as pool:
+        for rep in range(4):
+            t0 = time.perf_counter()
+            pool.map(_cpu_chunk, spans, chunksize=1)
+            dt = time.perf_counter() - t0
+            log(f"cpu baseline {len(spans)} processes, repeat {rep}: {dt:.2f} s")
+            if rep > 0:
+                walls.append(dt)
+    _CPU_JOB = None
+    med = statistics.median(walls)
+    return {
+        "value": rays_n * paths * surfaces / med,
+        "unit": "rays·surfaces/s",
+        "cores": len(spans),
+        "kind": "port",
+        "sample": f"first {rays_n} rays of the same seeded workload"
+                  f"{' (both diffraction orders)' if paths > 1 else ''}, split over "
+                  f"{len(spans)} worker processes (one per core), numpy float64 oracle, "
+                  f"median wall clock of 3 after 1 warm-up",
+        "cpu_model": _cpu_model(),
+        "host_cpus": os.cpu_count(),
+        "one_core": {"value": one["value"], "cores": 1, "sample": one["sample"]},
     }
 
 
@@ -246,32 +327,31 @@ def two_forms(args, system, grid, n, surfaces, paths, kernel32, kernel64, traffi
     units = n * paths * surfaces
     value64 = units / (ms64 / 1e3)
     value32 = units / (ms32 / 1e3)
-    achieved = BYTES_PER_RAY_F32 * n * paths / (k32 / 1e3) / 1e9
-    roofline = {
-        "bound": "hbm",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic32,
-        "kernel": kernel32,
-        "kernel_avg_ms": round(k32, 4),
-        "bytes_per_ray": BYTES_PER_RAY_F32,
-    }
-    f64_achieved = BYTES_PER_RAY_F64 * n * paths / (k64 / 1e3) / 1e9
+
+    def roof(bytes_per_ray, k_ms, kernel, traffic):
+        achieved = bytes_per_ray * n * paths / (k_ms / 1e3) / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic[0] if traffic else None,
+                "traffic_source": traffic[1] if traffic else None,
+                "kernel": kernel, "kernel_avg_ms": round(k_ms, 4),
+                "bytes_per_ray": bytes_per_ray}
+
+    roof64 = roof(BYTES_PER_RAY_F64, k64, kernel64, traffic64)
+    roof32 = roof(BYTES_PER_RAY_F32, k32, kernel32, traffic32)
     err = errs[0] if paths == 1 else {f"path{p}": e for p, e in enumerate(errs)}
     forms = {
         "f64": {"value": value64, "ms_per_step": ms64, "kernel_avg_ms": round(k64, 4),
-                "kernel": kernel64, "bytes_per_ray": BYTES_PER_RAY_F64,
-                "achieved_gbs": round(f64_achieved, 1), "traffic": traffic64,
-                "note": "drop-in form: oracle parity 1e-12, identical masks"},
+                "kernel": kernel64, "note": "drop-in form: oracle parity 1e-12, identical masks; "
+                                            "this is the line's value"},
         "f32": {"value": value32, "ms_per_step": ms32, "kernel_avg_ms": round(k32, 4),
-                "speedup_vs_f64": value32 / value64, "error_vs_f64": err},
+                "kernel": kernel32, "speedup_vs_f64": value32 / value64, "error_vs_f64": err,
+                "roofline": roof32},
         "surrogate": None,
     }
     if hist is not None:
         forms["sensor_histogram"] = hist
-    return value32, ms32, forms, roofline
+    return value64, ms64, forms, roof64
 
 
 def run_newtonian(args):
@@ -283,26 +363,34 @@ def run_newtonian(args):
     log(f"generating {n} rays")
     grid = workloads.newtonian_grid_random(n, seed=1)
     is_2p24 = n == 1 << 24
+    # the CPU baselines first: their worker processes fork from this one,
+    # which has not touched the GPU yet
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baselines(system, grid, S, rays_1=min(args.cpu_rays or 1 << 22, n),
+                            rays_n=min(4 * (args.cpu_rays or 1 << 22), n))
     value, ms, forms, roofline = two_forms(
         args, system, grid, n, S, 1, "trace_grid_f32<0, false, IO_SENSOR>",
         "trace_grid_f64<0, false, IO_SENSOR>",
-        TRAFFIC_BYTES_F32_2P24 if is_2p24 else None, TRAFFIC_BYTES_F64_2P24 if is_2p24 else None,
+        (TRAFFIC_BYTES_F32_2P24, TRAFFIC_SOURCE_F32) if is_2p24 else None,
+        (TRAFFIC_BYTES_F64_2P24, TRAFFIC_SOURCE_F64) if is_2p24 else None,
         sensor=([400.0, 700.0], (15.36, 15.36), (2048, 2048)))
     if args.calibrate_copy:
         import torch
 
         calibrate_copy(torch.device("cuda"))
-    cpu = None if args.no_cpu else cpu_baseline(system, grid, min(args.cpu_rays or 1 << 18, n), S)
+    if not args.no_surrogate:
+        forms["surrogate"] = surrogate_leg(args, steps=max(2, args.steps // 4))[0]
     config = {
         "workload": "newtonian telescope rayfunction, 2^24 rays (pupil uniform on r=100 mm "
                     "disc, field +-0.25 deg, 400-700 nm, seed 1)" if is_2p24 else
                     f"newtonian telescope rayfunction, {n} rays",
         "rays": n,
         "surfaces": S,
-        "form": "float32 compensated form (float32 I/O); accuracy vs the float64 form in "
-                "forms.f32",
+        "form": "float64 drop-in form (the reference's arithmetic); the float32 form in "
+                "forms.f32, the surrogate (of the EUV spectrograph) in forms.surrogate",
     }
-    return header(args, value, ms, "f32", config, forms, roofline, cpu)
+    return header(args, value, ms, "f64", config, forms, roofline, cpu)
 
 
 def run_spectrograph(args):
@@ -325,9 +413,9 @@ def run_spectrograph(args):
         "rays": n,
         "paths": 2,
         "surfaces": S,
-        "form": "float32 compensated form",
+        "form": "float64 drop-in form; the float32 form in forms.f32",
     }
-    return header(args, value, ms, "f32", config, forms, roofline, cpu)
+    return header(args, value, ms, "f64", config, forms, roofline, cpu)
 
 
 def run_tolerance(args):
@@ -408,16 +496,38 @@ def run_tolerance(args):
         "rays": n,
         "paths": P,
         "surfaces": S,
-        "form": "float32 compensated form, autotuned",
+        "form": "float64 drop-in form, autotuned; the float32 form in forms.f32",
     }
-    return header(args, value32, ms32, "f32", config, forms, None, cpu)
+    return header(args, value64, ms64, "f64", config, forms, None, cpu)
 
 
-def run_surrogate(args):
-    """Surrogate of the spectrograph's order -1 rayfunction: train on
-    float64-traced rays, calibrate, then time fused inference over --rays
-    rays (default 1e9) next to the float64 and float32 traces of the same
-    rays, and measure the error against the float64 trace on a subsample."""
+def device_spectrograph_inputs(n: int, seed: int, dev):
+    """Float32 device inputs drawn like ``workloads.spectrograph_grid``
+    (pupil uniform on the r = 40 mm disc, field +-0.05 deg, 58-63 nm), by a
+    seeded device generator: the inference batch of 1e9 rays is made on the
+    GPU instead of the host."""
+    import torch
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    u = lambda: torch.rand(n, generator=g, device=dev, dtype=torch.float32)  # noqa: E731
+    r = 40.0 * torch.sqrt(u())
+    th = (2 * np.pi) * u()
+    px, py = r * torch.cos(th), r * torch.sin(th)
+    del r, th
+    fx = u() * 0.1 - 0.05
+    fy = u() * 0.1 - 0.05
+    wl = u() * 5.0 + 58.0
+    return [fx, fy, px, py, wl]
+
+
+def surrogate_leg(args, steps: int):
+    """The learned surrogate of the spectrograph's order -1 rayfunction:
+    train on float64-traced rays, calibrate, then time fused inference
+    over ``args.infer_rays`` rays next to the float64 and float32 traces
+    of the same rays, and measure the error against the float64 trace on a
+    held-out subsample.  Returns (forms entry, roofline, float64 rate,
+    float32 rate)."""
     import torch
 
     from optika_amd import _trace, surrogate, workloads
@@ -438,8 +548,9 @@ def run_surrogate(args):
     losses = sg.train(epochs=args.epochs, batch=65536, lr=2e-3, log_every=500)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    steps = max(1, int(args.epochs * n_train / 65536))
-    log(f"trained {steps} Adam steps in {t2 - t1:.1f} s; losses {losses}")
+    adam = max(1, int(args.epochs * n_train / 65536))
+    log(f"trained {adam} Adam steps in {t2 - t1:.1f} s; losses {losses[-3:]}")
+    del sg.train_u, sg.train_t
 
     def inputs(n, seed):
         g = workloads.spectrograph_grid(n, seed=seed)
@@ -469,18 +580,13 @@ def run_surrogate(args):
            "rms_spot_px_surrogate_rays": torch.sqrt((err_nofb ** 2).mean()).item(),
            "max_spot_px_surrogate_rays": err_nofb.max().item(),
            "mask_agreement": (ref["unvignetted"] == unv).double().mean().item(),
-           "fallback_fraction": fb.numel() / ev[0].numel()}
+           "fallback_fraction": fb.numel() / ev[0].numel(),
+           "rays_compared": ev[0].numel()}
     log(f"surrogate accuracy: {acc}")
-    del ref
+    del ref, out, x, y, unv
     # throughput: fused inference over n rays, in one call
-    n = args.rays or 1_000_000_000
-    chunk = 1 << 26
-    big = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(5)]
-    for c0 in range(0, n, chunk):
-        m = min(chunk, n - c0)
-        part = inputs(m, 1000 + c0 // chunk)
-        for b_, p_ in zip(big, part):
-            b_[c0:c0 + m].copy_(p_)
+    n = args.infer_rays
+    big = device_spectrograph_inputs(n, 1000, dev)
     res = {"x": torch.empty(n, dtype=torch.float32, device=dev),
            "y": torch.empty(n, dtype=torch.float32, device=dev),
            "intensity": torch.empty(n, dtype=torch.float32, device=dev),
@@ -489,19 +595,21 @@ def run_surrogate(args):
            "list": torch.empty(n // 4, dtype=torch.int32, device=dev),
            "count": torch.zeros(1, dtype=torch.int32, device=dev)}
     stream = torch.cuda.current_stream()
-    ms_sur, k_sur = time_calls(lambda: sg.infer(*big, out=res), max(2, args.steps // 4), 1, stream)
+    ms_sur, k_sur = time_calls(lambda: sg.infer(*big, out=res), steps, 1, stream)
     fb_count = int(res["count"].item())
     log(f"surrogate inference: {ms_sur:.2f} ms for {n} rays, {fb_count} fall back")
     # float64 and float32 traces of the first 2^24 of the same rays
     m = min(n, 1 << 24)
-    sub32 = [t[:m] for t in big]
+    sub32 = [t[:m].contiguous() for t in big]
     sub64 = [t.double() for t in sub32]
+    del big, res
     plan64 = _trace.GridTracePlan(system, m, sensor_frame=True, precision="f64")
     plan32 = _trace.GridTracePlan(system, m, sensor_frame=True, precision="f32")
     ms64, _ = time_calls(lambda: plan64(*sub64), 3, 1, stream)
     ms32, _ = time_calls(lambda: plan32(*sub32), 5, 1, stream)
-    units = S  # surfaces per ray of the emulated path
-    v_sur = n * units / (ms_sur / 1e3)
+    del plan64, plan32, sub64, sub32
+    torch.cuda.empty_cache()
+    v_sur = n * S / (ms_sur / 1e3)  # surfaces of the emulated path per ray
     v64 = m * 2 * S / (ms64 / 1e3)
     v32 = m * 2 * S / (ms32 / 1e3)
     flops_per_ray = 2 * (5 * 256 + 3 * 256 * 256 + 256 * 3)
@@ -510,29 +618,34 @@ def run_surrogate(args):
                 "frac": round(achieved / 2500.0, 4), "traffic": None,
                 "kernel": "surrogate_infer", "kernel_avg_ms": round(k_sur, 3),
                 "flops_per_ray": flops_per_ray}
-    forms = {
-        "f64": {"value": v64, "ms_per_step": ms64, "rays": m, "paths": 2},
-        "f32": {"value": v32, "ms_per_step": ms32, "rays": m, "paths": 2},
-        "surrogate": {"value": v_sur, "ms_per_step": ms_sur, "rays": n,
-                      "fallback_rays": fb_count, "speedup_vs_f64": v_sur / v64,
-                      "speedup_vs_f32": v_sur / v32, "accuracy_vs_f64": acc,
-                      "training": {"rays": args.train_rays, "usable": n_train, "adam_steps": steps,
-                                   "batch": 65536, "trace_s": t1 - t0, "train_s": t2 - t1}},
-    }
-    config = {
-        "workload": f"learned surrogate of the EUV spectrograph (order -1): 5 -> 4 x 256 SiLU -> 3 "
-                    f"MLP, bf16 MFMA, trained on {args.train_rays} float64-traced rays, inference "
-                    f"on {n} rays (fallback rays counted, not traced, in the timed region)",
-        "rays": n,
-        "surfaces": S,
-        "form": "surrogate (value counts surfaces of the emulated path per ray)",
-    }
+    entry = {"workload": f"learned surrogate of the EUV spectrograph (order -1): 5 -> 4 x 256 SiLU "
+                         f"-> 3 MLP, bf16 MFMA, trained on {args.train_rays} float64-traced rays, "
+                         f"inference on {n} rays (fallback rays listed, not traced, in the timed "
+                         f"region)",
+             "value": v_sur, "ms_per_step": ms_sur, "rays": n, "fallback_rays": fb_count,
+             "spectrograph_f64_value": v64, "spectrograph_f32_value": v32,
+             "speedup_vs_f64": v_sur / v64, "speedup_vs_f32": v_sur / v32,
+             "error_vs_f64": acc, "roofline": roofline,
+             "training": {"rays": args.train_rays, "usable": n_train, "adam_steps": adam,
+                          "batch": 65536, "trace_s": t1 - t0, "train_s": t2 - t1}}
+    return entry, roofline, v64, v32
+
+
+def run_surrogate(args):
+    """The surrogate leg alone (its own line)."""
+    from optika_amd import workloads
+
+    S = workloads.SPECTROGRAPH_SURFACES
+    entry, roofline, v64, v32 = surrogate_leg(args, steps=max(2, args.steps // 4))
+    forms = {"f64": {"value": v64}, "f32": {"value": v32}, "surrogate": entry}
+    config = {"workload": entry["workload"], "rays": entry["rays"], "surfaces": S,
+              "form": "surrogate (value counts surfaces of the emulated path per ray)"}
     cpu = None
     if not args.no_cpu:
         # the reference's path for these rays: the numpy float64 trace
-        cpu = cpu_baseline(system, workloads.spectrograph_grid(1 << 12, seed=502),
+        cpu = cpu_baseline(workloads.spectrograph(), workloads.spectrograph_grid(1 << 12, seed=502),
                            min(args.cpu_rays or 1 << 12, 1 << 12), S, paths=2)
-    return header(args, v_sur, ms_sur, "bf16", config, forms, roofline, cpu)
+    return header(args, entry["value"], entry["ms_per_step"], "bf16", config, forms, roofline, cpu)
 
 
 def run_stops(args):

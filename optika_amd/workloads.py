@@ -348,7 +348,12 @@ def tolerance_batch(num_instances: int = 4096, seed: int = 3, min_rays: int = 10
         grid.field_x = np.zeros_like(grid.field_x)
         grid.field_y = np.zeros_like(grid.field_y)
         grid.wavelength = np.full_like(grid.wavelength, WAVELENGTH_CENTER)
-    return [spectrograph(p) for p in perts], counts, grid
+    systems_ = []
+    for p in perts:
+        s = spectrograph(p)
+        s.perturbation = p  # the errors this instance was built with
+        systems_.append(s)
+    return systems_, counts, grid
 
 
 # -- the reference's stop-solve test systems ----------------------------------
