@@ -276,7 +276,7 @@ def calibrate_copy(dev):
     torch.cuda.synchronize()
 
 
-def two_forms(args, system, grid, n, surfaces, paths, kernel32, kernel64, traffic32=None,
+def two_forms(args, system, grid, n, surfaces, paths, traffic32=None,
               traffic64=None, sensor=None):
     """Time the float64 and float32 forms of a grid trace on the same
     (float32-representable) inputs and compare them over every ray."""
@@ -291,6 +291,7 @@ def two_forms(args, system, grid, n, surfaces, paths, kernel32, kernel64, traffi
     in64 = [t.double() for t in in32]
     plan64 = _trace.GridTracePlan(system, n, sensor_frame=True, precision="f64")
     plan32 = _trace.GridTracePlan(system, n, sensor_frame=True, precision="f32")
+    kernel64, kernel32 = plan64.kernel, plan32.kernel
     stream = torch.cuda.current_stream()
     ms64, k64 = time_calls(lambda: plan64(*in64), args.steps, args.warmup, stream)
     log(f"f64 trace: {ms64:.3f} ms/step, kernel {k64:.3f} ms")
@@ -370,8 +371,7 @@ def run_newtonian(args):
         cpu = cpu_baselines(system, grid, S, rays_1=min(args.cpu_rays or 1 << 22, n),
                             rays_n=min(4 * (args.cpu_rays or 1 << 22), n))
     value, ms, forms, roofline = two_forms(
-        args, system, grid, n, S, 1, "trace_grid_f32<0, false, IO_SENSOR>",
-        "trace_grid_f64<0, false, IO_SENSOR>",
+        args, system, grid, n, S, 1,
         (TRAFFIC_BYTES_F32_2P24, TRAFFIC_SOURCE_F32) if is_2p24 else None,
         (TRAFFIC_BYTES_F64_2P24, TRAFFIC_SOURCE_F64) if is_2p24 else None,
         sensor=([400.0, 700.0], (15.36, 15.36), (2048, 2048)))
@@ -401,9 +401,7 @@ def run_spectrograph(args):
     S = workloads.SPECTROGRAPH_SURFACES
     log(f"generating {n} rays")
     grid = workloads.spectrograph_grid(n, seed=2)
-    value, ms, forms, roofline = two_forms(args, system, grid, n, S, 2,
-                                           "trace_grid_f32<7, false, IO_SENSOR>",
-                                           "trace_grid_f64<7, false, IO_SENSOR>")
+    value, ms, forms, roofline = two_forms(args, system, grid, n, S, 2)
     cpu = None if args.no_cpu else cpu_baseline(system, grid, min(args.cpu_rays or 1 << 12, n),
                                                 S, paths=2)
     config = {

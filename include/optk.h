@@ -278,6 +278,9 @@ typedef struct optk_grid {
 
 /* flags for optk_trace / optk_trace_grid */
 #define OPTK_TRACE_APPLY_POST 1 /* apply desc.post (rayfunction) */
+#define OPTK_TRACE_GENERIC 4    /* use the generic kernels even where the system's
+                                   surface kinds have a specialised kernel (the
+                                   results are the same; for comparisons) */
 
 int32_t optk_abi_version(void);
 const char *optk_last_error(void);
@@ -286,6 +289,14 @@ int64_t optk_sizeof(const char *record_name);
 
 int32_t optk_system_create(const optk_system_desc *desc, optk_system **out);
 int32_t optk_system_destroy(optk_system *sys);
+/* The specialised kernels a system runs (no reference counterpart:
+ * introspection).  The library holds trace kernels with the surface kinds
+ * of some layouts fixed at compile time (the Newtonian telescope and the
+ * EUV spectrograph); a system whose records carry exactly such a layout's
+ * kinds runs them on the grid entry point's sensor-frame layout and the ray
+ * entry point's full-output layout, with the same results as the generic
+ * kernels.  Returns the layout's index, or -1 (generic kernels only). */
+int32_t optk_system_specialised(const optk_system *sys);
 
 /* Trace n input rays through every surface in one fused pass.
  * out fields have num_paths * n elements (path-major: path p of ray i at
@@ -485,6 +496,22 @@ int32_t optk_batch_autotune(optk_batch *batch, int32_t precision, const optk_gri
 int32_t optk_gemm_bf16(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k,
                        const void *a, int64_t lda, const void *b, int64_t ldb, float *c,
                        int64_t ldc, int32_t split_k, float *workspace, void *hip_stream);
+
+/* optk_gemm_bf16 (no split) with the layer's elementwise step fused into
+ * its epilogue, so the result is not re-read by a separate kernel:
+ *   OPTK_GEMM_BIAS_SILU  c[i][j] = (A B)[i][j] + bias[j] (the pre-activation,
+ *                        float32) and out16[i][j] = bf16(silu(c[i][j]));
+ *   OPTK_GEMM_SILU_BACK  c[i][j] = (A B)[i][j] * silu'(pre[i][j]) (float32)
+ *                        and out16[i][j] = bf16(c[i][j]);
+ * silu(x) = x / (1 + e^-x).  c, pre and out16 are [m][ldc].  m and n
+ * multiples of 128, k a multiple of 32, leading dimensions multiples of 8
+ * and operands 16-byte aligned. */
+#define OPTK_GEMM_BIAS_SILU 1
+#define OPTK_GEMM_SILU_BACK 2
+int32_t optk_gemm_bf16_fused(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k,
+                             const void *a, int64_t lda, const void *b, int64_t ldb, float *c,
+                             int64_t ldc, int32_t epilogue, const float *bias, const float *pre,
+                             void *out16, void *hip_stream);
 
 /* elementwise and small-layer pieces of the forward and backward passes
  * (argument use per op below; rows x cols row-major arrays) */

@@ -355,6 +355,8 @@ class CompiledSystem:
         self.num_paths = rec.num_paths
         self.num_surfaces = rec.num_surfaces
         self._key = rec.key()
+        # the layout index of the specialised kernels it runs, or -1
+        self.signature = int(L.optk_system_specialised(handle))
 
     def __del__(self):
         h = getattr(self, "handle", None)

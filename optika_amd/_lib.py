@@ -31,6 +31,7 @@ MAX_BLOCKS = 8
 F64 = 0
 F32 = 1
 TRACE_APPLY_POST = 1
+TRACE_GENERIC = 4  # the generic kernels even where a specialised one exists
 
 OP_SAG_VALUE = 0
 OP_SAG_NORMAL = 1
@@ -277,6 +278,8 @@ MLP_COLUMN_SUM = 7
 MLP_ADAM = 8
 MLP_TO_BF16 = 9
 
+GEMM_BIAS_SILU = 1
+GEMM_SILU_BACK = 2
 BATCH_STATS = 8
 BATCH_GRAPH = 2
 # device scratch of optk_collect_noalloc (bytes, one array per GPU)
@@ -309,6 +312,8 @@ def lib() -> ctypes.CDLL:
     L.optk_system_create.argtypes = [ctypes.POINTER(SystemDesc), ctypes.POINTER(c_void_p)]
     L.optk_system_destroy.restype = c_int32
     L.optk_system_destroy.argtypes = [c_void_p]
+    L.optk_system_specialised.restype = c_int32
+    L.optk_system_specialised.argtypes = [c_void_p]
     L.optk_trace.restype = c_int32
     L.optk_trace.argtypes = [c_void_p, c_int32, ctypes.POINTER(Rays), ctypes.POINTER(Rays),
                              ctypes.POINTER(Rays), c_int64, c_int32, c_void_p]
@@ -337,6 +342,10 @@ def lib() -> ctypes.CDLL:
     L.optk_gemm_bf16.restype = c_int32
     L.optk_gemm_bf16.argtypes = [c_int32, c_int32, c_int64, c_int64, c_int64, c_void_p, c_int64,
                                  c_void_p, c_int64, c_void_p, c_int64, c_int32, c_void_p, c_void_p]
+    L.optk_gemm_bf16_fused.restype = c_int32
+    L.optk_gemm_bf16_fused.argtypes = [c_int32, c_int32, c_int64, c_int64, c_int64, c_void_p, c_int64,
+                                       c_void_p, c_int64, c_void_p, c_int64, c_int32, c_void_p,
+                                       c_void_p, c_void_p, c_void_p]
     L.optk_mlp_op.restype = c_int32
     L.optk_mlp_op.argtypes = [c_int32, ctypes.POINTER(MlpArgs), c_void_p]
     L.optk_surrogate_infer.restype = c_int32

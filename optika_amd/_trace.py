@@ -20,6 +20,10 @@ from .vectors import Cartesian3dVectorArray, ObjectVectorArray
 
 __all__ = ["trace_sequence", "trace_system", "device", "compiled"]
 
+# the layouts the trace kernels are specialised for, in the order of the
+# library's signature table (optk_system_specialised returns the index)
+SIGNATURE_NAMES = ("newtonian", "spectrograph")
+
 _PRECISIONS = {"f64": _lib.F64, "float64": _lib.F64, "f32": _lib.F32, "float32": _lib.F32}
 
 _COMPILED: "OrderedDict[bytes, _compile.CompiledSystem]" = OrderedDict()
@@ -143,9 +147,10 @@ def _shape(num_paths: int, n: int, num_surfaces: int | None = None):
 
 
 def trace_sequence(surfaces: Sequence, rays: RayVectorArray, accumulate: bool = False,
-                   precision: str = "f64", system=None) -> RayVectorArray:
+                   precision: str = "f64", system=None, generic: bool = False) -> RayVectorArray:
     """``propagate_rays`` / ``accumulate_rays`` over ``surfaces`` (or over an
-    already compiled ``system`` of them)."""
+    already compiled ``system`` of them).  ``generic``: the generic kernels
+    even where a specialised one exists (same results)."""
     prec = _precision(precision)
     dtype = torch.float64 if prec == _lib.F64 else torch.float32
     dev = device()
@@ -191,7 +196,8 @@ def trace_sequence(surfaces: Sequence, rays: RayVectorArray, accumulate: bool = 
     acc = _Out(cs.num_surfaces * P * n, dev, dtype, fields) if accumulate else None
     L = _lib.lib()
     _lib.check(L.optk_trace(cs.handle, prec, ctypes.byref(rin), ctypes.byref(out.rec),
-                            ctypes.byref(acc.rec) if acc else None, n, 0, stream_handle()))
+                            ctypes.byref(acc.rec) if acc else None, n,
+                            _lib.TRACE_GENERIC if generic else 0, stream_handle()))
     if accumulate:
         return _make_rays(acc, _shape(P, n, cs.num_surfaces), {})
     return _make_rays(out, _shape(P, n), {})
@@ -353,10 +359,12 @@ class GridTracePlan:
     ``optk_grid`` / ``optk_rays`` records, so each call is one C-ABI launch
     on the current stream with no host-side allocation or record building.
     The returned rays are views of the plan's buffers, overwritten by the
-    next call.
+    next call.  ``generic``: run the generic kernels even when the system's
+    surface kinds have a specialised kernel (same results; for comparisons).
     """
 
-    def __init__(self, system, n: int, sensor_frame: bool = True, precision: str = "f64"):
+    def __init__(self, system, n: int, sensor_frame: bool = True, precision: str = "f64",
+                 generic: bool = False):
         self.prec = _precision(precision)
         self.dtype = torch.float64 if self.prec == _lib.F64 else torch.float32
         self.dev = device()
@@ -367,7 +375,8 @@ class GridTracePlan:
                            post=post)
         self.n = int(n)
         self.paths = self.cs.num_paths
-        self.flags = _lib.TRACE_APPLY_POST if post is not None else 0
+        self.flags = (_lib.TRACE_APPLY_POST if post is not None else 0) | (
+            _lib.TRACE_GENERIC if generic else 0)
         self.invariant = _compile.invariant_fields(surfaces)
         fields = {"intensity"} if self.invariant else {
             "wavelength", "intensity", "attenuation", "index_refraction"}
@@ -376,6 +385,19 @@ class GridTracePlan:
         self.grid.object_at_infinity = 1 if system.object_is_at_infinity else 0
         self.grid.intensity_default = 1.0
         self.num_surfaces = self.cs.num_surfaces
+        self.generic = bool(generic)
+
+    @property
+    def kernel(self) -> str:
+        """The name of the kernel a call launches (its template arguments
+        abbreviated): the specialised kernel of the system's surface-kind
+        signature, or the generic kernel."""
+        prec = "f64" if self.prec == _lib.F64 else "f32"
+        sig = self.cs.signature
+        if sig >= 0 and not self.generic:
+            name = SIGNATURE_NAMES[sig] if sig < len(SIGNATURE_NAMES) else str(sig)
+            return f"trace_grid_{prec}_spec<{name} surface kinds>"
+        return f"trace_grid_{prec} (generic)"
 
     def __call__(self, field_x, field_y, pupil_x, pupil_y, wavelength, stream=None):
         """Trace one batch; the five inputs are contiguous device tensors of

@@ -448,10 +448,16 @@ OPTK_HD float branch_root(float A, float B, float C, float uz) {
 // returns the path length s from that point (INFINITY when the conic has no
 // admissible root, as the reference's t = inf).  t0 is the path length
 // already travelled to reach the point (conic roots are chosen by |t0 + s|).
+// the sag kind: compile-time (K, see Kinds) or the record's own
+template <uint32_t K> OPTK_HD int k_fsag(const FastSurface &f) {
+  return Kinds<K>::fixed ? Kinds<K>::sag : f.sag_kind;
+}
+
+template <uint32_t K = 0>
 OPTK_HD float residual_intercept(const FastSurface &f, float X, float Y, float Z, float ux,
                                  float uy, float uz, double t0) {
   const float c = f.c;
-  switch (f.sag_kind) {
+  switch (k_fsag<K>(f)) {
   case OPTK_SAG_NONE:
     return -Z / uz;
   case OPTK_SAG_SPHERE: {
@@ -560,12 +566,13 @@ OPTK_HD float toroid_residual_solve(const FastSurface &f, float X, float Y, floa
 
 // two float64 Newton steps on the residual of a steep sag, from the float32
 // solution s (steep conics and toroids)
+template <uint32_t K = 0>
 OPTK_HD double polish64(const FastSurface &f, double X, double Y, double Z, double ux, double uy,
                         double uz, double s) {
   for (int it = 0; it < 2; ++it) {
     double x = X + ux * s, y = Y + uy * s, z = Z + uz * s;
     double g, dg;
-    if (f.sag_kind == OPTK_SAG_TOROID) {
+    if (k_fsag<K>(f) == OPTK_SAG_TOROID) {
       ToroidEval<double> e = toroid_eval<double>(f.c64, f.r64, x, y);
       double zs = (double)f.surf_row[0] * x + (double)f.surf_row[1] * y +
                   (double)f.surf_row[2] * z + (double)f.surf_tz;
@@ -574,8 +581,8 @@ OPTK_HD double polish64(const FastSurface &f, double X, double Y, double Z, doub
            (e.dzdx * ux + e.dzdy * uy);
     } else {
       // implicit conic c (x^2 + y^2 + (1+k) z^2) - 2 z (cylinder: no y)
-      double yy = f.sag_kind == OPTK_SAG_CYLINDER ? 0.0 : y;
-      double uyy = f.sag_kind == OPTK_SAG_CYLINDER ? 0.0 : uy;
+      double yy = k_fsag<K>(f) == OPTK_SAG_CYLINDER ? 0.0 : y;
+      double uyy = k_fsag<K>(f) == OPTK_SAG_CYLINDER ? 0.0 : uy;
       g = f.c64 * (x * x + yy * yy + f.kp1_64 * z * z) - 2.0 * z;
       dg = 2.0 * (f.c64 * (x * ux + yy * uyy + f.kp1_64 * z * uz) - uz);
     }
@@ -585,20 +592,21 @@ OPTK_HD double polish64(const FastSurface &f, double X, double Y, double Z, doub
 }
 
 // path length from the current (sag-frame) position to the sag
+template <uint32_t K = 0>
 OPTK_HD double fast_intercept(const FastSurface &f, const FRay &r) {
   // hop to the vertex plane z = 0 in float64 (a ray parallel to it stays put)
   double t0 = fabs(r.dz) > 1e-6 ? -r.pz * rcp64_finite(r.dz) : 0.0;
-  if (f.sag_kind == OPTK_SAG_NONE && fabs(r.dz) > 1e-6) return t0;
+  if (k_fsag<K>(f) == OPTK_SAG_NONE && fabs(r.dz) > 1e-6) return t0;
   double X = r.px + r.dx * t0, Y = r.py + r.dy * t0, Z = r.pz + r.dz * t0;
   float ux = (float)r.dx, uy = (float)r.dy, uz = (float)r.dz;
   float s;
-  if (f.sag_kind == OPTK_SAG_TOROID)
+  if (k_fsag<K>(f) == OPTK_SAG_TOROID)
     s = toroid_residual_solve(f, (float)X, (float)Y, (float)Z, ux, uy, uz);
   else
-    s = residual_intercept(f, (float)X, (float)Y, (float)Z, ux, uy, uz, t0);
+    s = residual_intercept<K>(f, (float)X, (float)Y, (float)Z, ux, uy, uz, t0);
   double sd = (double)s;
-  if (f.polish && f.sag_kind != OPTK_SAG_NONE && fabs(sd) < INFINITY)
-    sd = polish64(f, X, Y, Z, r.dx, r.dy, r.dz, sd);
+  if (f.polish && k_fsag<K>(f) != OPTK_SAG_NONE && fabs(sd) < INFINITY)
+    sd = polish64<K>(f, X, Y, Z, r.dx, r.dy, r.dz, sd);
   return t0 + sd;
 }
 
@@ -606,12 +614,12 @@ OPTK_HD double fast_intercept(const FastSurface &f, const FRay &r) {
 // fast square roots of FastOps<T>.  The trace evaluates it in float64 at the
 // float64 hit point: a float32 normal tilts the outgoing ray by ~1e-8 rad,
 // which over a metre-long arm is ~1e-3 pixel.
-template <typename T>
+template <typename T, uint32_t K = 0>
 OPTK_HD V3<T> fast_normal(const FastSurface &f, T x, T y) {
   using Op = FastOps<T>;
   const T c = sizeof(T) == 8 ? T(f.c64) : T(f.c);
   const T kp1 = sizeof(T) == 8 ? T(f.kp1_64) : T(f.kp1);
-  switch (f.sag_kind) {
+  switch (k_fsag<K>(f)) {
   case OPTK_SAG_SPHERE: {
     T nx = c * x, ny = c * y;
     return {nx, ny, -Op::sqrt(T(1) - nx * nx - ny * ny)};
@@ -662,8 +670,9 @@ OPTK_HD V3<double> fast_snell(V3<double> a, double n1, double n2, V3<double> u, 
 }
 
 // Sellmeier glass (_materials.py:428-438), fast float64
+template <uint32_t K = 0>
 OPTK_HD double fast_index(const optk_surface &s, double wavelength, double n1) {
-  switch (s.material_kind) {
+  switch (k_material<K>(s)) {
   case OPTK_MATERIAL_VACUUM:
     return 1.0;
   case OPTK_MATERIAL_GLASS: {
@@ -941,11 +950,11 @@ float silicon_fallback64(const StackView &sv, int stack, double thickness, float
                                            C<double>{(double)index, k64});
 }
 
-template <int F>
+template <int F, uint32_t K = 0>
 OPTK_HD float fast_material_efficiency(const optk_surface &s, const StackView &sv,
                                       const float *tables32, float wavelength, V3<double> a,
                                       float index, float attenuation, V3<double> normal) {
-  switch (s.material_kind) {
+  switch (k_material<K>(s)) {
   case OPTK_MATERIAL_MEASURED_MIRROR: {
     const float *w = tables32 + s.material_table;
     return interp1f(w, w + s.material_table_len, s.material_table_len, wavelength);
@@ -955,7 +964,7 @@ OPTK_HD float fast_material_efficiency(const optk_surface &s, const StackView &s
     if (!(F & FEAT_MULTILAYER)) return 1.0f;
     float k = attenuation * wavelength / (4.0f * 3.14159265358979f) * 1e-6f;
     float cosv = (float)(-(a.x * normal.x + a.y * normal.y + a.z * normal.z));
-    bool film = s.material_kind == OPTK_MATERIAL_MULTILAYER_FILM;
+    bool film = k_material<K>(s) == OPTK_MATERIAL_MULTILAYER_FILM;
     float eff = fast_multilayer(sv, tables32, s.stack, wavelength, Cf{cosv, 0.0f},
                                 Cf{index, k}, film);
     if (eff < 0.0f) {
@@ -985,11 +994,12 @@ OPTK_HD float fast_material_efficiency(const optk_surface &s, const StackView &s
   }
 }
 
+template <uint32_t K = 0>
 OPTK_HD bool fast_aperture(const optk_surface &s, const Affine32 &to_aperture, const float *tables32,
                            float x, float y, float z) {
   affine32_apply(to_aperture, x, y, z);
   bool mask = true;
-  switch (s.aperture_kind) {
+  switch (k_aperture<K>(s)) {
   case OPTK_APERTURE_CIRCLE:
     mask = x * x + y * y <= (float)(s.aperture_p[0] * s.aperture_p[0]);
     break;
@@ -1034,8 +1044,9 @@ OPTK_HD bool fast_aperture(const optk_surface &s, const Affine32 &to_aperture, c
   default:
     return true;
   }
-  if (s.aperture_flags & OPTK_AP_INVERTED) mask = !mask;
-  if (!(s.aperture_flags & OPTK_AP_ACTIVE)) mask = true;
+  const int flags = k_aperture_flags<K>(s);
+  if (flags & OPTK_AP_INVERTED) mask = !mask;
+  if (!(flags & OPTK_AP_ACTIVE)) mask = true;
   return mask;
 }
 
@@ -1048,17 +1059,17 @@ struct FastView {
 
 // one surface of the float32 form: the ray arrives in this surface's sag
 // frame (the caller applies the hop) and leaves in its surface frame
-template <int F>
+template <int F, uint32_t K = 0>
 OPTK_HD void fast_propagate(const FastSurface &f, const optk_surface &s, const FastView &v,
                             int path, FRay &r) {
-  double t = fast_intercept(f, r);
+  double t = fast_intercept<K>(f, r);
   double hx = r.px + r.dx * t, hy = r.py + r.dy * t, hz = r.pz + r.dz * t;
   // attenuation over the path (|d| = 1); exactly 1 for alpha = 0
   if (!(r.attenuation == 0.0f && fabs(t) < 1e150)) {
     float len = (float)fabs(t) * sqrtf((float)(r.dx * r.dx + r.dy * r.dy + r.dz * r.dz));
     r.intensity = expf(-r.attenuation * len) * r.intensity;
   }
-  V3<double> normal = fast_normal<double>(f, hx, hy);
+  V3<double> normal = fast_normal<double, K>(f, hx, hy);
   float sx = (float)hx, sy = (float)hy, sz = (float)hz;  // sag frame, float
 
   // back to the surface frame
@@ -1069,27 +1080,27 @@ OPTK_HD void fast_propagate(const FastSurface &f, const optk_surface &s, const F
 
   V3<double> a = {r.dx, r.dy, r.dz};
   int order = 0;
-  const bool ruled = (F & FEAT_RULINGS) && s.rulings_kind != OPTK_RULINGS_NONE;
+  const bool ruled = (F & FEAT_RULINGS) && k_rulings<K>(s) != OPTK_RULINGS_NONE;
   if (ruled) {
     order = path_order(s, path);
     V3<double> p1 = {hx, hy, hz};
-    V3<double> kappa = spacing_vector<double>(s, p1, normal);
+    V3<double> kappa = spacing_vector<double, K>(s, p1, normal);
     a = incident_effective(s, order, (double)r.wavelength, a, (double)r.index, normal, kappa);
   }
 
   const double n1 = r.index;
-  const double n2 = fast_index(s, (double)r.wavelength, n1);
-  V3<double> b = fast_snell(a, n1, n2, normal, s.is_mirror != 0, !ruled);
-  float eff = fast_material_efficiency<F>(s, v.sv, v.tables32, r.wavelength, a, r.index,
-                                          r.attenuation, normal);
+  const double n2 = fast_index<K>(s, (double)r.wavelength, n1);
+  V3<double> b = fast_snell(a, n1, n2, normal, k_mirror<K>(s), !ruled);
+  float eff = fast_material_efficiency<F, K>(s, v.sv, v.tables32, r.wavelength, a, r.index,
+                                             r.attenuation, normal);
   if (ruled) {
     V3<float> p1 = {(float)hx, (float)hy, (float)hz};
     V3<float> af = {(float)a.x, (float)a.y, (float)a.z};
     V3<float> nf = {(float)normal.x, (float)normal.y, (float)normal.z};
-    eff = eff * rulings_efficiency<float>(s, order, r.wavelength, af, p1, nf, v.tables32);
+    eff = eff * rulings_efficiency<float, K>(s, order, r.wavelength, af, p1, nf, v.tables32);
   }
   r.intensity = r.intensity * eff;
-  r.attenuation = (float)material_attenuation(s, (double)r.attenuation);
+  r.attenuation = (float)material_attenuation<K>(s, (double)r.attenuation);
   if (!(n1 == n2 && n1 != 0.0 && fabs(n1) < INFINITY))
     r.wavelength = (float)((double)r.wavelength * n2 * rcp64(n1));
   r.index = (float)n2;
@@ -1097,11 +1108,11 @@ OPTK_HD void fast_propagate(const FastSurface &f, const optk_surface &s, const F
   r.dy = b.y;
   r.dz = b.z;
 
-  if (s.aperture_kind != OPTK_APERTURE_NONE) {
-    bool m = (s.aperture_flags & OPTK_AP_DIRECTION)
-                 ? fast_aperture(s, f.aperture_from_surface, v.tables32, (float)r.dx,
-                                 (float)r.dy, (float)r.dz)
-                 : fast_aperture(s, f.aperture_from_sag, v.tables32, sx, sy, sz);
+  if (k_aperture<K>(s) != OPTK_APERTURE_NONE) {
+    bool m = (k_aperture_flags<K>(s) & OPTK_AP_DIRECTION)
+                 ? fast_aperture<K>(s, f.aperture_from_surface, v.tables32, (float)r.dx,
+                                    (float)r.dy, (float)r.dz)
+                 : fast_aperture<K>(s, f.aperture_from_sag, v.tables32, sx, sy, sz);
     r.unvignetted = r.unvignetted && m;
   }
 }

@@ -4,6 +4,7 @@
 // sequence, so multiply-adds may fuse.
 
 #include "optk_batch.h"
+#include "optk_signatures.h"
 
 using namespace optk;
 
@@ -247,6 +248,80 @@ FastArgs make_args(const optk_system *sys, const FRaysOut &out, int64_t n, int32
 }
 
 // kernels of one entry point: by feature set (geometric, toroid + rulings,
+// Kernels specialised on a system's surface kinds (optk_signatures.h), one
+// ray per thread, the surfaces unrolled in order with their kinds fixed at
+// compile time; the arithmetic is that of fast_body.
+template <int F, uint32_t K>
+__device__ __forceinline__ void fast_spec_step(int k, int k0, ConstFastSurface *c_fast,
+                                               ConstSurface *c_surf, const FastView &v, int path,
+                                               FRay &r) {
+  if (k < k0) return;
+  const FastSurface &fk = *(const FastSurface *)&c_fast[k];
+  const optk_surface &sk = *(const optk_surface *)&c_surf[k];
+  if (k > k0) fast_xf(fk.hop, r);
+  fast_propagate<F, K>(fk, sk, v, path, r);
+}
+
+template <typename In, int F, int IO, uint32_t... K>
+__device__ __forceinline__ void fast_spec_body(const FastArgs &A, const In &in) {
+  const int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
+  if (i >= A.n) return;
+  ConstFastSpec *c_spec = (ConstFastSpec *)A.spec;
+  ConstFastSurface *c_fast = (ConstFastSurface *)A.fast;
+  ConstSurface *c_surf = (ConstSurface *)A.surfaces;
+  FastView v;
+  v.surfaces = A.surfaces;
+  v.fast = A.fast;
+  v.sv = A.sv;
+  v.tables32 = A.tables32;
+  const int num_paths = (F & FEAT_RULINGS) ? A.num_paths : 1;
+  const bool skip = A.entry == 1 && c_spec->skip_first != 0;
+  const int k0 = skip ? 1 : 0;
+  FRay r0 = load_ray<IO>(in, i);
+  if (skip) {
+    const bool parallel = r0.dz == 0.0;
+    fast_xf(*(const optk_affine *)&c_spec->first_skip, r0);
+    if (parallel) {
+      r0.px = r0.py = r0.pz = NAN;
+      r0.intensity = NAN;
+    }
+  } else {
+    fast_xf(*(const optk_affine *)&c_spec->first_hop[A.entry], r0);
+  }
+  for (int path = 0; path < num_paths; ++path) {
+    FRay r = r0;
+    int k = 0;
+    (fast_spec_step<F, K>(k++, k0, c_fast, c_surf, v, path, r), ...);
+    const optk_affine &last = *(const optk_affine *)&c_spec->last[A.apply_post];
+    fast_xf(last, r);
+    store_ray<IO>(A.out, (int64_t)path * A.n + i, r);
+  }
+}
+
+template <int F, int IO, uint32_t... K>
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
+void trace_grid_f32_spec(const FastArgs A, const FGridIn g) {
+  fast_spec_body<FGridIn, F, IO, K...>(A, g);
+}
+
+template <int F, int IO, uint32_t... K>
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
+void trace_rays_f32_spec(const FastArgs A, const FRaysIn in) {
+  fast_spec_body<FRaysIn, F, IO, K...>(A, in);
+}
+
+typedef void (*SpecGrid32)(FastArgs, FGridIn);
+typedef void (*SpecRays32)(FastArgs, FRaysIn);
+const SpecGrid32 SPEC_GRID32[NUM_SIGNATURES] = {
+    trace_grid_f32_spec<SIGNATURES[SIG_NEWTONIAN].features, IO_SENSOR, OPTK_SIGNATURE_NEWTONIAN>,
+    trace_grid_f32_spec<SIGNATURES[SIG_SPECTROGRAPH].features, IO_SENSOR, OPTK_SIGNATURE_SPECTROGRAPH>,
+};
+const SpecRays32 SPEC_RAYS32[NUM_SIGNATURES] = {
+    trace_rays_f32_spec<SIGNATURES[SIG_NEWTONIAN].features, IO_FULL, OPTK_SIGNATURE_NEWTONIAN>,
+    trace_rays_f32_spec<SIGNATURES[SIG_SPECTROGRAPH].features, IO_FULL, OPTK_SIGNATURE_SPECTROGRAPH>,
+};
+
+// kernels of one entry point: by feature set (geometric, toroid + rulings,
 // all; all with the per-surface accumulation), for either I/O layout
 template <typename In> struct KernelSet {
   typedef void (*Kernel)(FastArgs, In, FRaysOut);
@@ -258,9 +333,16 @@ template <typename In> struct KernelSet {
 template <typename In>
 int32_t launch(const KernelSet<In> &ks, bool special, const optk_system *sys, const In &in,
                const FRaysOut &out, const FRaysOut *accum, int64_t n, int32_t entry,
-               int32_t apply_post, hipStream_t stream) {
+               int32_t apply_post, void (*spec)(FastArgs, In), hipStream_t stream) {
   if (n == 0) return OPTK_OK;
   FastArgs A = make_args(sys, out, n, entry, apply_post);
+  if (!accum && special && spec && n / TRACE_BLOCK < INT32_MAX) {
+    hipLaunchKernelGGL(spec, dim3((unsigned)((n + TRACE_BLOCK - 1) / TRACE_BLOCK)),
+                       dim3(TRACE_BLOCK), 0, stream, A, in);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("float32 trace launch: ") + hipGetErrorString(e));
+    return OPTK_OK;
+  }
   FRaysOut acc;
   memset(&acc, 0, sizeof(acc));
   if (accum) acc = *accum;
@@ -355,23 +437,28 @@ void *batch_kernel_f32(int features) {
 namespace optk {
 
 int32_t fast_trace_rays(const optk_system *sys, const FRaysIn &in, const FRaysOut &out,
-                        const FRaysOut *accum, int64_t n, int32_t apply_post, hipStream_t stream) {
+                        const FRaysOut *accum, int64_t n, int32_t apply_post, bool generic,
+                        hipStream_t stream) {
   static const KernelSet<FRaysIn> ks = {
       {trace_rays_f32<0, false>, trace_rays_f32<FEAT_TOROID | FEAT_RULINGS, false>,
        trace_rays_f32<FEAT_ALL, false>, trace_rays_f32<FEAT_ALL, true>},
       {trace_rays_f32<0, false, IO_FULL>, trace_rays_f32<FEAT_TOROID | FEAT_RULINGS, false, IO_FULL>,
        trace_rays_f32<FEAT_ALL, false, IO_FULL>}};
-  return launch<FRaysIn>(ks, full_output(out), sys, in, out, accum, n, 0, apply_post, stream);
+  const SpecRays32 spec = (sys->signature >= 0 && !generic) ? SPEC_RAYS32[sys->signature] : nullptr;
+  return launch<FRaysIn>(ks, full_output(out), sys, in, out, accum, n, 0, apply_post, spec, stream);
 }
 
 int32_t fast_trace_grid(const optk_system *sys, const FGridIn &in, const FRaysOut &out,
-                        const FRaysOut *accum, int64_t n, int32_t apply_post, hipStream_t stream) {
+                        const FRaysOut *accum, int64_t n, int32_t apply_post, bool generic,
+                        hipStream_t stream) {
   static const KernelSet<FGridIn> ks = {
       {trace_grid_f32<0, false>, trace_grid_f32<FEAT_TOROID | FEAT_RULINGS, false>,
        trace_grid_f32<FEAT_ALL, false>, trace_grid_f32<FEAT_ALL, true>},
       {trace_grid_f32<0, false, IO_SENSOR>, trace_grid_f32<FEAT_TOROID | FEAT_RULINGS, false, IO_SENSOR>,
        trace_grid_f32<FEAT_ALL, false, IO_SENSOR>}};
-  return launch<FGridIn>(ks, sensor_layout(in, out), sys, in, out, accum, n, 1, apply_post, stream);
+  const SpecGrid32 spec = (sys->signature >= 0 && !generic) ? SPEC_GRID32[sys->signature] : nullptr;
+  return launch<FGridIn>(ks, sensor_layout(in, out), sys, in, out, accum, n, 1, apply_post, spec,
+                         stream);
 }
 
 size_t batch_lds_f32(int num_surfaces) { return fast_staged_bytes(num_surfaces); }

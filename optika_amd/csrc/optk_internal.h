@@ -27,6 +27,7 @@ struct optk_system {
   int32_t pre_is_inverse[2];
   optk_affine post;
   int32_t features = 0;                  // FEAT_* the surfaces need
+  int32_t signature = -1;                // specialised kernels (optk_signatures.h), or -1
   optk::TraceSpec *spec = nullptr;       // device copy of views + transformations
   std::vector<optk_surface> host_surfaces;
   // float32 form
@@ -115,10 +116,13 @@ struct FGridIn {
 };
 
 // float32-form launches (optk_fast.hip); `accum` may be null
+// `generic`: the generic kernels even where the system has specialised ones
 int32_t fast_trace_rays(const optk_system *sys, const FRaysIn &in, const FRaysOut &out,
-                        const FRaysOut *accum, int64_t n, int32_t apply_post, hipStream_t stream);
+                        const FRaysOut *accum, int64_t n, int32_t apply_post, bool generic,
+                        hipStream_t stream);
 int32_t fast_trace_grid(const optk_system *sys, const FGridIn &in, const FRaysOut &out,
-                        const FRaysOut *accum, int64_t n, int32_t apply_post, hipStream_t stream);
+                        const FRaysOut *accum, int64_t n, int32_t apply_post, bool generic,
+                        hipStream_t stream);
 
 // ---------------------------------------------------------------------------
 // batched persistent trace (optk_batch_*)

@@ -16,6 +16,7 @@
 
 #include "optk_batch.h"
 #include "optk_random.h"
+#include "optk_signatures.h"
 
 using namespace optk;
 
@@ -283,6 +284,60 @@ static void launch_trace(void (*k0)(TraceArgs, In, RaysOut), void (*k3)(TraceArg
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(TRACE_BLOCK), 0, stream, A, in, acc);
 }
 
+
+// Kernels specialised on a system's surface kinds (optk_signatures.h): the
+// surfaces unrolled in order, each with its kinds fixed at compile time, one
+// ray per thread.  (A grid-stride loop would keep every surface's record
+// fields live in scalar registers across its iterations, more than the
+// scalar register file holds; the compiler then spills them into VGPR
+// lanes.)  The arithmetic is that of the generic kernels.
+template <typename In, int F, int IO, uint32_t... K>
+__device__ __forceinline__ void spec_body(const TraceArgs &A, const In &in) {
+  const int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
+  if (i >= A.n) return;
+  ConstTraceSpec *c_spec = (ConstTraceSpec *)A.spec;
+  ConstSurface *c_surf = (ConstSurface *)A.surfaces;
+  const TraceSpec &spec = *(const TraceSpec *)c_spec;
+  SystemView sys = spec.sys;
+  sys.surfaces = A.surfaces;
+  const int num_paths = (F & FEAT_RULINGS) ? sys.num_paths : 1;
+  Ray<double> r0 = load_io<IO>(in, i);
+  apply_pre(spec, r0);
+  for (int path = 0; path < num_paths; ++path) {
+    Ray<double> r = r0;
+    int k = 0;
+    (propagate_surface<F, K>(*(const optk_surface *)&c_surf[k++], sys, path, r), ...);
+    if (A.apply_post) xf_inverse_ray(spec.post, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
+    store_io<IO>(A.out, (int64_t)path * A.n + i, r);
+  }
+}
+
+template <int F, int IO, uint32_t... K>
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
+void trace_grid_f64_spec(const TraceArgs A, const GridIn g) {
+  spec_body<GridIn, F, IO, K...>(A, g);
+}
+
+template <int F, int IO, uint32_t... K>
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
+void trace_rays_f64_spec(const TraceArgs A, const RaysIn in) {
+  spec_body<RaysIn, F, IO, K...>(A, in);
+}
+
+// the specialised kernels by signature (the order of SIGNATURES): grid
+// entry in the sensor-frame I/O layout, ray entry with every output field
+typedef void (*SpecGrid64)(TraceArgs, GridIn);
+typedef void (*SpecRays64)(TraceArgs, RaysIn);
+static const SpecGrid64 SPEC_GRID64[NUM_SIGNATURES] = {
+    trace_grid_f64_spec<SIGNATURES[SIG_NEWTONIAN].features, IO_SENSOR, OPTK_SIGNATURE_NEWTONIAN>,
+    trace_grid_f64_spec<SIGNATURES[SIG_SPECTROGRAPH].features, IO_SENSOR, OPTK_SIGNATURE_SPECTROGRAPH>,
+};
+static const SpecRays64 SPEC_RAYS64[NUM_SIGNATURES] = {
+    trace_rays_f64_spec<SIGNATURES[SIG_NEWTONIAN].features, IO_FULL, OPTK_SIGNATURE_NEWTONIAN>,
+    trace_rays_f64_spec<SIGNATURES[SIG_SPECTROGRAPH].features, IO_FULL, OPTK_SIGNATURE_SPECTROGRAPH>,
+};
+
+static unsigned spec_blocks(int64_t n) { return (unsigned)((n + TRACE_BLOCK - 1) / TRACE_BLOCK); }
 
 // stop-solve Newton iteration, float64 form ---------------------------------
 
@@ -1334,6 +1389,7 @@ int32_t optk_system_create(const optk_system_desc *desc, optk_system **out) {
   s->host_surfaces.assign(desc->surfaces, desc->surfaces + desc->num_surfaces);
   for (auto &surf : s->host_surfaces) derive_constants(surf);
   s->features = variant_of(features_of(desc));
+  s->signature = find_signature(s->host_surfaces.data(), desc->num_surfaces, s->features);
   if ((rc = upload(&s->surfaces, s->host_surfaces.data(), desc->num_surfaces)) ||
       (rc = upload(&s->layers, desc->layers, desc->num_layers)) ||
       (rc = upload(&s->stacks, desc->stacks, desc->num_stacks)) ||
@@ -1373,6 +1429,8 @@ int32_t optk_system_create(const optk_system_desc *desc, optk_system **out) {
   *out = s;
   return OPTK_OK;
 }
+
+int32_t optk_system_specialised(const optk_system *sys) { return sys ? sys->signature : -1; }
 
 static void fill_out(RaysOut &o, const optk_rays *r) {
   std::memset(&o, 0, sizeof(o));
@@ -1451,7 +1509,7 @@ int32_t optk_trace(const optk_system *sys, int32_t precision, const optk_rays *i
     fill_out32(fo, out);
     fill_out32(fa, accum);
     return fast_trace_rays(sys, fi, fo, accum ? &fa : nullptr, n, A.apply_post,
-                           (hipStream_t)hip_stream);
+                           (flags & OPTK_TRACE_GENERIC) != 0, (hipStream_t)hip_stream);
   }
   RaysIn ri;
   for (int k = 0; k < 3; ++k) {
@@ -1468,7 +1526,11 @@ int32_t optk_trace(const optk_system *sys, int32_t precision, const optk_rays *i
   ri.attenuation_default = in->attenuation_default;
   ri.index_default = in->index_default;
   ri.unvignetted_default = in->unvignetted_default;
-  if (!accum && full_output(A.out))
+  const bool spec_ok = sys->signature >= 0 && !(flags & OPTK_TRACE_GENERIC) && n / TRACE_BLOCK < INT32_MAX;
+  if (!accum && full_output(A.out) && spec_ok)
+    hipLaunchKernelGGL(SPEC_RAYS64[sys->signature], dim3(spec_blocks(n)), dim3(TRACE_BLOCK), 0,
+                       (hipStream_t)hip_stream, A, ri);
+  else if (!accum && full_output(A.out))
     launch_trace<RaysIn>(trace_rays_f64<0, false, IO_FULL>,
                          trace_rays_f64<FEAT_TOROID | FEAT_RULINGS, false, IO_FULL>,
                          trace_rays_f64<FEAT_ALL, false, IO_FULL>, trace_rays_f64<FEAT_ALL, true>,
@@ -1507,7 +1569,7 @@ int32_t optk_trace_grid(const optk_system *sys, int32_t precision, const optk_gr
     fill_out32(fo, out);
     fill_out32(fa, accum);
     return fast_trace_grid(sys, fg, fo, accum ? &fa : nullptr, n, A.apply_post,
-                           (hipStream_t)hip_stream);
+                           (flags & OPTK_TRACE_GENERIC) != 0, (hipStream_t)hip_stream);
   }
   GridIn g;
   for (int k = 0; k < 2; ++k) {
@@ -1521,7 +1583,11 @@ int32_t optk_trace_grid(const optk_system *sys, int32_t precision, const optk_gr
   g.wavelength_default = in->wavelength_default;
   g.intensity_default = in->intensity_default;
   g.object_at_infinity = in->object_at_infinity;
-  if (!accum && sensor_layout(g, A.out))
+  const bool spec_ok = sys->signature >= 0 && !(flags & OPTK_TRACE_GENERIC) && n / TRACE_BLOCK < INT32_MAX;
+  if (!accum && sensor_layout(g, A.out) && spec_ok)
+    hipLaunchKernelGGL(SPEC_GRID64[sys->signature], dim3(spec_blocks(n)), dim3(TRACE_BLOCK), 0,
+                       (hipStream_t)hip_stream, A, g);
+  else if (!accum && sensor_layout(g, A.out))
     launch_trace<GridIn>(trace_grid_f64<0, false, IO_SENSOR>,
                          trace_grid_f64<FEAT_TOROID | FEAT_RULINGS, false, IO_SENSOR>,
                          trace_grid_f64<FEAT_ALL, false, IO_SENSOR>, trace_grid_f64<FEAT_ALL, true>,

@@ -128,6 +128,151 @@ __global__ __launch_bounds__(256) void gemm_bf16(const uint16_t *__restrict__ a,
       }
 }
 
+// ---------------------------------------------------------------------------
+// The training GEMMs of the surrogate (65536 x 256 x 256 and 256 x 256 x
+// 65536 with bf16 operands) are bandwidth-bound on MI355X: 128 to 256 flops
+// per byte against a ridge near 300.  Block tile 128 x 128 x 32, 256 threads,
+// each wave a 64 x 64 quarter as 4 x 4 MFMA 16x16x32 tiles (16 MFMAs per
+// k step).  Operands are staged in LDS with k contiguous, double-buffered:
+// the next k step's 16-byte global loads are issued before this step's
+// MFMAs and land in the other buffer after them, one barrier per step.  An
+// operand stored with k as its row index (TA = 1, TB = 0) is loaded 16 bytes
+// along m (or n) and transposed on its way into LDS.  The epilogue can fuse
+// the layer's elementwise step (bias + SiLU forward, SiLU derivative
+// backward), so the pre-activation is not re-read by a separate kernel.
+// Requires m, n multiples of 128 and k a multiple of 32 * split.
+// ---------------------------------------------------------------------------
+
+constexpr int GM = 128, GN = 128, GK = 32, GPAD = 8;
+
+enum { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_SILU_BACK = 2 };
+
+struct GemmEpilogue {
+  float *c;          // EPI_STORE: C (or split partials); BIAS_SILU: pre out; SILU_BACK: dpre32 out
+  int64_t ldc;
+  const float *bias; // BIAS_SILU
+  const float *pre;  // SILU_BACK: pre-activation in, [m][ldc]
+  uint16_t *out16;   // BIAS_SILU: bf16 activations; SILU_BACK: bf16 dpre
+};
+
+// one 128 x 32 operand tile into LDS ([row][k], k contiguous); KC: stored
+// with k contiguous (x[row * ld + k]), else k as the row index (x[k * ld + row])
+template <bool KC> struct TileLoad {
+  uint4 v[2];
+  __device__ __forceinline__ void load(const uint16_t *__restrict__ x, int64_t ld, int64_t row0,
+                                       int64_t k0, int tid) {
+    if (KC) {
+      const int r = tid >> 2, kk = (tid & 3) * 8;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        v[h] = *reinterpret_cast<const uint4 *>(x + (row0 + r + 64 * h) * ld + k0 + kk);
+    } else {
+      const int kr = tid >> 4, rr = (tid & 15) * 8;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        v[h] = *reinterpret_cast<const uint4 *>(x + (k0 + kr + 16 * h) * ld + row0 + rr);
+    }
+  }
+  __device__ __forceinline__ void store(uint16_t (*t)[GK + GPAD], int tid) const {
+    if (KC) {
+      const int r = tid >> 2, kk = (tid & 3) * 8;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) *reinterpret_cast<uint4 *>(&t[r + 64 * h][kk]) = v[h];
+    } else {
+      const int kr = tid >> 4, rr = (tid & 15) * 8;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t w[4] = {v[h].x, v[h].y, v[h].z, v[h].w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          t[rr + 2 * j][kr + 16 * h] = (uint16_t)(w[j] & 0xffffu);
+          t[rr + 2 * j + 1][kr + 16 * h] = (uint16_t)(w[j] >> 16);
+        }
+      }
+    }
+  }
+};
+
+template <int TA, int TB, int EPI>
+__global__ __launch_bounds__(256) void gemm_bf16_128(const uint16_t *__restrict__ a, int64_t lda,
+                                                     const uint16_t *__restrict__ b, int64_t ldb,
+                                                     const GemmEpilogue E, int64_t m, int64_t n,
+                                                     int64_t kc) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][GM][GK + GPAD];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][GN][GK + GPAD];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int64_t m0 = (int64_t)blockIdx.y * GM, n0 = (int64_t)blockIdx.x * GN;
+  const int64_t k_begin = (int64_t)blockIdx.z * kc;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  floatx4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  TileLoad<TA == 0> la;
+  TileLoad<TB == 1> lb;
+  la.load(a, lda, m0, k_begin, tid);
+  lb.load(b, ldb, n0, k_begin, tid);
+  la.store(As[0], tid);
+  lb.store(Bs[0], tid);
+  __syncthreads();
+  const int steps = (int)(kc / GK);
+  for (int step = 0; step < steps; ++step) {
+    const int buf = step & 1;
+    const bool more = step + 1 < steps;
+    if (more) {
+      la.load(a, lda, m0, k_begin + (int64_t)(step + 1) * GK, tid);
+      lb.load(b, ldb, n0, k_begin + (int64_t)(step + 1) * GK, tid);
+    }
+    bf16x8 af[4], bfr[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&As[buf][wm + 16 * i + fr][fk]));
+      bfr[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&Bs[buf][wn + 16 * i + fr][fk]));
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    if (more) {
+      la.store(As[buf ^ 1], tid);
+      lb.store(Bs[buf ^ 1], tid);
+    }
+    __syncthreads();
+  }
+  // lane holds C[row = 4 (lane / 16) + r][col = lane % 16] of each 16 x 16 tile
+  float *cz = E.c + (int64_t)blockIdx.z * m * n;
+  const int64_t ld = gridDim.z > 1 ? n : E.ldc;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t col = n0 + wn + 16 * j + (lane & 15);
+      float bias = 0.0f;
+      if (EPI == EPI_BIAS_SILU) bias = E.bias[col];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        const int64_t at = row * ld + col;
+        float v = acc[i][j][r];
+        if (EPI == EPI_BIAS_SILU) {
+          v += bias;
+          cz[at] = v;
+          E.out16[at] = to_bf16(silu(v));
+        } else if (EPI == EPI_SILU_BACK) {
+          const float g = v * silu_grad(E.pre[at]);
+          cz[at] = g;
+          E.out16[at] = to_bf16(g);
+        } else {
+          cz[at] = v;
+        }
+      }
+    }
+}
+
 // c[i] = sum over splits of partial[s][i] (m * n elements)
 __global__ __launch_bounds__(256) void gemm_reduce(const float *__restrict__ partial, int splits,
                                                    int64_t count, float *__restrict__ c) {
@@ -708,10 +853,39 @@ int32_t optk_gemm_bf16(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, i
   if ((!trans_a && ((uintptr_t)a % 16)) || (trans_b && ((uintptr_t)b % 16)))
     return fail(OPTK_EINVAL, "gemm: operands must be 16-byte aligned");
   hipStream_t stream = (hipStream_t)hip_stream;
-  dim3 grid((unsigned)(n / BN), (unsigned)(m / BM), (unsigned)split_k);
   float *target = split_k > 1 ? workspace : c;
   const int64_t kc = k / split_k;
   const uint16_t *A = (const uint16_t *)a, *B = (const uint16_t *)b;
+  if (m % GM == 0 && n % GN == 0) {
+    // 128 x 128 tiles; the transposed operands also need 16-byte aligned rows
+    const bool rows_ok = (trans_a ? (lda % 8 == 0 && (uintptr_t)a % 16 == 0) : true) &&
+                         (!trans_b ? (ldb % 8 == 0 && (uintptr_t)b % 16 == 0) : true);
+    if (rows_ok) {
+      GemmEpilogue E;
+      std::memset(&E, 0, sizeof(E));
+      E.c = target;
+      E.ldc = ldc;
+      dim3 grid((unsigned)(n / GN), (unsigned)(m / GM), (unsigned)split_k);
+      if (!trans_a && !trans_b)
+        hipLaunchKernelGGL((gemm_bf16_128<0, 0, EPI_STORE>), grid, dim3(256), 0, stream, A, lda, B, ldb, E, m, n, kc);
+      else if (!trans_a && trans_b)
+        hipLaunchKernelGGL((gemm_bf16_128<0, 1, EPI_STORE>), grid, dim3(256), 0, stream, A, lda, B, ldb, E, m, n, kc);
+      else if (trans_a && !trans_b)
+        hipLaunchKernelGGL((gemm_bf16_128<1, 0, EPI_STORE>), grid, dim3(256), 0, stream, A, lda, B, ldb, E, m, n, kc);
+      else
+        hipLaunchKernelGGL((gemm_bf16_128<1, 1, EPI_STORE>), grid, dim3(256), 0, stream, A, lda, B, ldb, E, m, n, kc);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess && split_k > 1) {
+        if (ldc != n) return fail(OPTK_EINVAL, "gemm: split_k > 1 needs ldc == n");
+        hipLaunchKernelGGL(gemm_reduce, dim3(blocks_for(m * n)), dim3(256), 0, stream, workspace,
+                           split_k, m * n, c);
+        e = hipGetLastError();
+      }
+      if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("gemm launch: ") + hipGetErrorString(e));
+      return OPTK_OK;
+    }
+  }
+  dim3 grid((unsigned)(n / BN), (unsigned)(m / BM), (unsigned)split_k);
   if (!trans_a && !trans_b)
     hipLaunchKernelGGL((gemm_bf16<0, 0>), grid, dim3(256), 0, stream, A, lda, B, ldb, target, ldc, m, n, kc);
   else if (!trans_a && trans_b)
@@ -728,6 +902,50 @@ int32_t optk_gemm_bf16(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, i
     e = hipGetLastError();
   }
   if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("gemm launch: ") + hipGetErrorString(e));
+  return OPTK_OK;
+}
+
+int32_t optk_gemm_bf16_fused(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k,
+                             const void *a, int64_t lda, const void *b, int64_t ldb, float *c,
+                             int64_t ldc, int32_t epilogue, const float *bias, const float *pre,
+                             void *out16, void *hip_stream) {
+  if (!a || !b || !c || !out16) return fail(OPTK_EINVAL, "gemm_fused: null operand");
+  if (epilogue != OPTK_GEMM_BIAS_SILU && epilogue != OPTK_GEMM_SILU_BACK)
+    return fail(OPTK_EINVAL, "gemm_fused: unknown epilogue");
+  if (epilogue == OPTK_GEMM_BIAS_SILU && !bias) return fail(OPTK_EINVAL, "gemm_fused: bias missing");
+  if (epilogue == OPTK_GEMM_SILU_BACK && !pre) return fail(OPTK_EINVAL, "gemm_fused: pre missing");
+  if (m <= 0 || n <= 0 || k <= 0 || m % GM || n % GN || k % GK)
+    return fail(OPTK_EINVAL, "gemm_fused: m and n must be positive multiples of 128, k of 32");
+  if ((trans_a ? lda < m : lda < k) || (trans_b ? ldb < k : ldb < n) || ldc < n)
+    return fail(OPTK_EINVAL, "gemm_fused: leading dimension too small");
+  if (lda % 8 || ldb % 8 || (uintptr_t)a % 16 || (uintptr_t)b % 16)
+    return fail(OPTK_EINVAL, "gemm_fused: operands need 16-byte aligned rows (leading dimensions multiples of 8)");
+  GemmEpilogue E;
+  E.c = c;
+  E.ldc = ldc;
+  E.bias = bias;
+  E.pre = pre;
+  E.out16 = (uint16_t *)out16;
+  const uint16_t *A = (const uint16_t *)a, *B = (const uint16_t *)b;
+  hipStream_t stream = (hipStream_t)hip_stream;
+  dim3 grid((unsigned)(n / GN), (unsigned)(m / GM), 1);
+#define OPTK_GEMM_FUSED(TA, TB)                                                                    \
+  if (epilogue == OPTK_GEMM_BIAS_SILU)                                                             \
+    hipLaunchKernelGGL((gemm_bf16_128<TA, TB, EPI_BIAS_SILU>), grid, dim3(256), 0, stream, A, lda, B, ldb, E, m, n, k); \
+  else                                                                                             \
+    hipLaunchKernelGGL((gemm_bf16_128<TA, TB, EPI_SILU_BACK>), grid, dim3(256), 0, stream, A, lda, B, ldb, E, m, n, k);
+  if (!trans_a && !trans_b) {
+    OPTK_GEMM_FUSED(0, 0)
+  } else if (!trans_a && trans_b) {
+    OPTK_GEMM_FUSED(0, 1)
+  } else if (trans_a && !trans_b) {
+    OPTK_GEMM_FUSED(1, 0)
+  } else {
+    OPTK_GEMM_FUSED(1, 1)
+  }
+#undef OPTK_GEMM_FUSED
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("gemm_fused launch: ") + hipGetErrorString(e));
   return OPTK_OK;
 }
 
@@ -780,7 +998,9 @@ int32_t optk_mlp_op(int32_t op, const optk_mlp_args *A, void *hip_stream) {
     // f32[2]: out (cols * inner); f32[3]: workspace (blocks * cols * inner)
     NEED((A->f32[0] || A->u16[0]) && A->f32[1] && A->f32[2] && A->f32[3], "outer_grad: x, g, out, workspace");
     NEED(A->cols <= 256 && A->inner >= 1 && A->inner <= 8, "outer_grad: cols <= 256, 1 <= inner <= 8");
-    const int64_t rpb = 256;
+    // 32 rows per block: enough blocks (2048 at 65536 rows) to keep every CU
+    // busy; each thread's loads of its column are independent
+    const int64_t rpb = 32;
     const int64_t nblk = (rows + rpb - 1) / rpb;
     NEED(A->count >= nblk * A->cols * A->inner, "outer_grad: workspace too small");
     hipLaunchKernelGGL(outer_grad, dim3((unsigned)nblk), dim3(256), 0, s, A->f32[0], A->u16[0],

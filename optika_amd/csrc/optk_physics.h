@@ -147,12 +147,14 @@ template <typename T> OPTK_HD M2<T> mselect(bool cond, const M2<T> &x, const M2<
 }
 
 // ---------------------------------------------------------------------------
-// transformations
+// transformations (XK: the record's kind fixed at compile time by a
+// specialised kernel, or -1 to read it from the record)
 // ---------------------------------------------------------------------------
 
-template <typename T> OPTK_HD void xf_apply_pos(const optk_affine &a, T &x, T &y, T &z) {
-  if (a.kind == OPTK_XF_IDENTITY) return;
-  if (a.kind == OPTK_XF_AFFINE) {
+template <int XK = -1, typename T> OPTK_HD void xf_apply_pos(const optk_affine &a, T &x, T &y, T &z) {
+  const int kind = XK >= 0 ? XK : a.kind;
+  if (kind == OPTK_XF_IDENTITY) return;
+  if (kind == OPTK_XF_AFFINE) {
     T X = T(a.m[0]) * x + T(a.m[1]) * y + T(a.m[2]) * z;
     T Y = T(a.m[3]) * x + T(a.m[4]) * y + T(a.m[5]) * z;
     T Z = T(a.m[6]) * x + T(a.m[7]) * y + T(a.m[8]) * z;
@@ -164,8 +166,9 @@ template <typename T> OPTK_HD void xf_apply_pos(const optk_affine &a, T &x, T &y
   y = y + T(a.t[1]);
   z = z + T(a.t[2]);
 }
-template <typename T> OPTK_HD void xf_apply_dir(const optk_affine &a, T &x, T &y, T &z) {
-  if (a.kind != OPTK_XF_AFFINE) return;
+template <int XK = -1, typename T> OPTK_HD void xf_apply_dir(const optk_affine &a, T &x, T &y, T &z) {
+  const int kind = XK >= 0 ? XK : a.kind;
+  if (kind != OPTK_XF_AFFINE) return;
   T X = T(a.m[0]) * x + T(a.m[1]) * y + T(a.m[2]) * z;
   T Y = T(a.m[3]) * x + T(a.m[4]) * y + T(a.m[5]) * z;
   T Z = T(a.m[6]) * x + T(a.m[7]) * y + T(a.m[8]) * z;
@@ -173,12 +176,13 @@ template <typename T> OPTK_HD void xf_apply_dir(const optk_affine &a, T &x, T &y
   y = Y;
   z = Z;
 }
-template <typename T> OPTK_HD void xf_inverse_pos(const optk_affine &a, T &x, T &y, T &z) {
-  if (a.kind == OPTK_XF_IDENTITY) return;
+template <int XK = -1, typename T> OPTK_HD void xf_inverse_pos(const optk_affine &a, T &x, T &y, T &z) {
+  const int kind = XK >= 0 ? XK : a.kind;
+  if (kind == OPTK_XF_IDENTITY) return;
   x = x - T(a.t[0]);
   y = y - T(a.t[1]);
   z = z - T(a.t[2]);
-  if (a.kind == OPTK_XF_AFFINE) {
+  if (kind == OPTK_XF_AFFINE) {
     T X = T(a.m[0]) * x + T(a.m[3]) * y + T(a.m[6]) * z;
     T Y = T(a.m[1]) * x + T(a.m[4]) * y + T(a.m[7]) * z;
     T Z = T(a.m[2]) * x + T(a.m[5]) * y + T(a.m[8]) * z;
@@ -187,8 +191,9 @@ template <typename T> OPTK_HD void xf_inverse_pos(const optk_affine &a, T &x, T 
     z = Z;
   }
 }
-template <typename T> OPTK_HD void xf_inverse_dir(const optk_affine &a, T &x, T &y, T &z) {
-  if (a.kind != OPTK_XF_AFFINE) return;
+template <int XK = -1, typename T> OPTK_HD void xf_inverse_dir(const optk_affine &a, T &x, T &y, T &z) {
+  const int kind = XK >= 0 ? XK : a.kind;
+  if (kind != OPTK_XF_AFFINE) return;
   T X = T(a.m[0]) * x + T(a.m[3]) * y + T(a.m[6]) * z;
   T Y = T(a.m[1]) * x + T(a.m[4]) * y + T(a.m[7]) * z;
   T Z = T(a.m[2]) * x + T(a.m[5]) * y + T(a.m[8]) * z;
@@ -200,10 +205,11 @@ template <typename T> OPTK_HD void xf_inverse_dir(const optk_affine &a, T &x, T 
 // a ray's position and direction under one record with one branch on its
 // kind; the arithmetic of each component is that of xf_apply_pos /
 // xf_apply_dir (or xf_inverse_pos / xf_inverse_dir), operation for operation
-template <typename T>
+template <int XK = -1, typename T>
 OPTK_HD void xf_apply_ray(const optk_affine &a, T &px, T &py, T &pz, T &dx, T &dy, T &dz) {
-  if (a.kind == OPTK_XF_IDENTITY) return;
-  if (a.kind == OPTK_XF_AFFINE) {
+  const int kind = XK >= 0 ? XK : a.kind;
+  if (kind == OPTK_XF_IDENTITY) return;
+  if (kind == OPTK_XF_AFFINE) {
     T X = T(a.m[0]) * px + T(a.m[1]) * py + T(a.m[2]) * pz;
     T Y = T(a.m[3]) * px + T(a.m[4]) * py + T(a.m[5]) * pz;
     T Z = T(a.m[6]) * px + T(a.m[7]) * py + T(a.m[8]) * pz;
@@ -221,13 +227,14 @@ OPTK_HD void xf_apply_ray(const optk_affine &a, T &px, T &py, T &pz, T &dx, T &d
   py = py + T(a.t[1]);
   pz = pz + T(a.t[2]);
 }
-template <typename T>
+template <int XK = -1, typename T>
 OPTK_HD void xf_inverse_ray(const optk_affine &a, T &px, T &py, T &pz, T &dx, T &dy, T &dz) {
-  if (a.kind == OPTK_XF_IDENTITY) return;
+  const int kind = XK >= 0 ? XK : a.kind;
+  if (kind == OPTK_XF_IDENTITY) return;
   px = px - T(a.t[0]);
   py = py - T(a.t[1]);
   pz = pz - T(a.t[2]);
-  if (a.kind == OPTK_XF_AFFINE) {
+  if (kind == OPTK_XF_AFFINE) {
     T X = T(a.m[0]) * px + T(a.m[3]) * py + T(a.m[6]) * pz;
     T Y = T(a.m[1]) * px + T(a.m[4]) * py + T(a.m[7]) * pz;
     T Z = T(a.m[2]) * px + T(a.m[5]) * py + T(a.m[8]) * pz;

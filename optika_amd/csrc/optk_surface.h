@@ -18,6 +18,68 @@ enum : int {
   FEAT_ALL = 7,
 };
 
+// Compile-time kinds of one surface record.  A specialised kernel fixes the
+// kinds of every surface of a system at compile time (K != 0, built by
+// kinds_pack): each per-surface switch then folds to the one branch the
+// surface takes, and what the fixed kinds make constant -- an index of 1
+// through vacuum and mirrors, a zero attenuation, a unit index ratio --
+// propagates through the whole trace.  K == 0 reads every kind from the
+// record, as the generic kernels do.  The arithmetic of each branch is the
+// same either way.
+constexpr uint32_t KINDS_FIXED = 0x80000000u;
+
+constexpr uint32_t kinds_pack(int xf, int sag, int sag_xf, int material, int mirror, int rulings,
+                              int spacing, int aperture, int aperture_xf, int aperture_flags) {
+  return KINDS_FIXED | (uint32_t)xf | ((uint32_t)sag << 2) | ((uint32_t)sag_xf << 5) |
+         ((uint32_t)material << 7) | ((uint32_t)mirror << 10) | ((uint32_t)rulings << 11) |
+         ((uint32_t)spacing << 14) | ((uint32_t)aperture << 16) | ((uint32_t)aperture_xf << 19) |
+         ((uint32_t)aperture_flags << 22);
+}
+
+// the kinds of a record, packed as a specialised kernel fixes them
+inline uint32_t kinds_of(const optk_surface &s) {
+  return kinds_pack(s.transform.kind, s.sag_kind, s.sag_transform.kind, s.material_kind,
+                    s.is_mirror != 0, s.rulings_kind, s.spacing_kind, s.aperture_kind,
+                    s.aperture_transform.kind, s.aperture_flags);
+}
+
+template <uint32_t K> struct Kinds {
+  static constexpr bool fixed = (K & KINDS_FIXED) != 0;
+  static constexpr int xf = fixed ? (int)(K & 3u) : -1;
+  static constexpr int sag = (int)((K >> 2) & 7u);
+  static constexpr int sag_xf = fixed ? (int)((K >> 5) & 3u) : -1;
+  static constexpr int material = (int)((K >> 7) & 7u);
+  static constexpr int mirror = (int)((K >> 10) & 1u);
+  static constexpr int rulings = (int)((K >> 11) & 7u);
+  static constexpr int spacing = (int)((K >> 14) & 3u);
+  static constexpr int aperture = (int)((K >> 16) & 7u);
+  static constexpr int aperture_xf = fixed ? (int)((K >> 19) & 3u) : -1;
+  static constexpr int aperture_flags = (int)((K >> 22) & 7u);
+};
+
+// a record's kind: the compile-time one, or the record's own
+template <uint32_t K> OPTK_HD int k_sag(const optk_surface &s) {
+  return Kinds<K>::fixed ? Kinds<K>::sag : s.sag_kind;
+}
+template <uint32_t K> OPTK_HD int k_material(const optk_surface &s) {
+  return Kinds<K>::fixed ? Kinds<K>::material : s.material_kind;
+}
+template <uint32_t K> OPTK_HD bool k_mirror(const optk_surface &s) {
+  return Kinds<K>::fixed ? Kinds<K>::mirror != 0 : s.is_mirror != 0;
+}
+template <uint32_t K> OPTK_HD int k_rulings(const optk_surface &s) {
+  return Kinds<K>::fixed ? Kinds<K>::rulings : s.rulings_kind;
+}
+template <uint32_t K> OPTK_HD int k_spacing(const optk_surface &s) {
+  return Kinds<K>::fixed ? Kinds<K>::spacing : s.spacing_kind;
+}
+template <uint32_t K> OPTK_HD int k_aperture(const optk_surface &s) {
+  return Kinds<K>::fixed ? Kinds<K>::aperture : s.aperture_kind;
+}
+template <uint32_t K> OPTK_HD int k_aperture_flags(const optk_surface &s) {
+  return Kinds<K>::fixed ? Kinds<K>::aperture_flags : s.aperture_flags;
+}
+
 // Per-surface constants the kernels read instead of recomputing per ray.
 // The curvature is the same IEEE quotient the reference forms once per call
 // (``c = 1 / self.radius``), so precomputing it changes no result.
@@ -74,10 +136,11 @@ OPTK_HD double sag_z(const optk_surface &s, double x, double y, double z) {
   return 0.0;
 }
 
+template <uint32_t K = 0>
 OPTK_HD V3<double> sag_normal(const optk_surface &s, double x, double y, double z) {
-  if (s.sag_kind == OPTK_SAG_NONE) return {0.0, 0.0, -1.0};  // _flat.py:41-45
-  xf_inverse_pos(s.sag_transform, x, y, z);
-  switch (s.sag_kind) {
+  if (k_sag<K>(s) == OPTK_SAG_NONE) return {0.0, 0.0, -1.0};  // _flat.py:41-45
+  xf_inverse_pos<Kinds<K>::sag_xf>(s.sag_transform, x, y, z);
+  switch (k_sag<K>(s)) {
   case OPTK_SAG_SPHERE: {  // _spherical.py:125-146 (not renormalised)
     double c = s.sag_c;
     double nx = c * x;
@@ -128,9 +191,10 @@ OPTK_HD V3<double> sag_normal(const optk_surface &s, double x, double y, double 
 #define OPTK_NEWTON_TOL 1e-13
 
 // Intercept in the sag frame: returns the path length t along u.
+template <uint32_t K = 0>
 OPTK_HD double sag_intercept_t(const optk_surface &s, double ox, double oy, double oz, double ux,
                                double uy, double uz) {
-  switch (s.sag_kind) {
+  switch (k_sag<K>(s)) {
   case OPTK_SAG_NONE:  // _flat.py:47-64
     return -oz / uz;
   case OPTK_SAG_SPHERE: {  // _spherical.py:148-191
@@ -195,19 +259,20 @@ OPTK_HD double sag_intercept_t(const optk_surface &s, double ox, double oy, doub
 // Toroid intercept: AbstractSag.intercept (optika/sags/_abc.py:76-107) solves
 // line(t).z - sag(line(t)) = 0 along the ray in the *surface* frame, the sag
 // transformation acting inside sag() only.  Newton iteration from t = 0.
+template <uint32_t K = 0>
 OPTK_HD double toroid_intercept_t(const optk_surface &s, double ox, double oy, double oz, double ux,
                                   double uy, double uz) {
   const optk_affine &T = s.sag_transform;
   double c = s.sag_c;
   double r = s.sag_p[1];
   double vx = ux, vy = uy, vz = uz;  // direction in the sag frame (chain rule)
-  xf_inverse_dir(T, vx, vy, vz);
+  xf_inverse_dir<Kinds<K>::sag_xf>(T, vx, vy, vz);
   double t = 0.0;
   for (int it = 0; it < OPTK_NEWTON_MAX_ITER; ++it) {
     double x = ox + ux * t;
     double y = oy + uy * t;
     double zl = oz + uz * t;
-    xf_inverse_pos(T, x, y, zl);
+    xf_inverse_pos<Kinds<K>::sag_xf>(T, x, y, zl);
     double x2 = x * x, y2 = y * y;
     double g = sqrt(1.0 - c * c * y2);
     double zy = c * y2 / (1.0 + g);
@@ -227,10 +292,11 @@ OPTK_HD double toroid_intercept_t(const optk_surface &s, double ox, double oy, d
 
 // AbstractSag.propagate_rays (optika/sags/_abc.py:109-122) in the surface
 // frame: move to the intercept and attenuate.
-template <int F = FEAT_ALL> OPTK_HD void sag_propagate(const optk_surface &s, Ray<double> &r) {
+template <int F = FEAT_ALL, uint32_t K = 0>
+OPTK_HD void sag_propagate(const optk_surface &s, Ray<double> &r) {
   double nx, ny, nz;
-  if ((F & FEAT_TOROID) && s.sag_kind == OPTK_SAG_TOROID) {
-    double t = toroid_intercept_t(s, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
+  if ((F & FEAT_TOROID) && k_sag<K>(s) == OPTK_SAG_TOROID) {
+    double t = toroid_intercept_t<K>(s, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
     nx = r.px + r.dx * t;
     ny = r.py + r.dy * t;
     nz = r.pz + r.dz * t;
@@ -238,12 +304,12 @@ template <int F = FEAT_ALL> OPTK_HD void sag_propagate(const optk_surface &s, Ra
     const optk_affine &T = s.sag_transform;
     double ox = r.px, oy = r.py, oz = r.pz;
     double ux = r.dx, uy = r.dy, uz = r.dz;
-    xf_inverse_ray(T, ox, oy, oz, ux, uy, uz);
-    double t = sag_intercept_t(s, ox, oy, oz, ux, uy, uz);
+    xf_inverse_ray<Kinds<K>::sag_xf>(T, ox, oy, oz, ux, uy, uz);
+    double t = sag_intercept_t<K>(s, ox, oy, oz, ux, uy, uz);
     nx = ox + ux * t;
     ny = oy + uy * t;
     nz = oz + uz * t;
-    xf_apply_ray(T, nx, ny, nz, ux, uy, uz);
+    xf_apply_ray<Kinds<K>::sag_xf>(T, nx, ny, nz, ux, uy, uz);
     r.dx = ux;
     r.dy = uy;
     r.dz = uz;
@@ -267,9 +333,9 @@ template <int F = FEAT_ALL> OPTK_HD void sag_propagate(const optk_surface &s, Ra
 // ---------------------------------------------------------------------------
 
 // spacing_(position, normal): the local ruling vector d g
-template <typename T>
+template <typename T, uint32_t K = 0>
 OPTK_HD V3<T> spacing_vector(const optk_surface &s, V3<T> p, V3<T> n) {
-  switch (s.spacing_kind) {
+  switch (k_spacing<K>(s)) {
   case OPTK_SPACING_CONSTANT:  // _spacing.py:69-74
     return {T(s.spacing_constant) * T(s.spacing_normal[0]),
             T(s.spacing_constant) * T(s.spacing_normal[1]),
@@ -346,10 +412,10 @@ OPTK_HD float table_interp(const float *w, int count, float x) {
 }
 
 // groove efficiencies (_rulings.py:246-1073)
-template <typename T>
+template <typename T, uint32_t K = 0>
 OPTK_HD T rulings_efficiency(const optk_surface &s, int order, T wavelength, V3<T> a_eff,
                              V3<T> position, V3<T> normal, const T *tables) {
-  switch (s.rulings_kind) {
+  switch (k_rulings<K>(s)) {
   case OPTK_RULINGS_IDEAL:
     return T(1);
   case OPTK_RULINGS_MEASURED:
@@ -357,7 +423,7 @@ OPTK_HD T rulings_efficiency(const optk_surface &s, int order, T wavelength, V3<
   default:
     break;
   }
-  V3<T> kappa = spacing_vector<T>(s, position, normal);
+  V3<T> kappa = spacing_vector<T, K>(s, position, normal);
   V3<T> nr = div3(kappa, len3(kappa));
   V3<T> par = cross3(normal, nr);
   par = div3(par, len3(par));
@@ -367,7 +433,7 @@ OPTK_HD T rulings_efficiency(const optk_surface &s, int order, T wavelength, V3<
   T i = (T)order;
   const T pi = T(M_PI);
   const T depth = T(s.rulings_p[0]);
-  switch (s.rulings_kind) {
+  switch (k_rulings<K>(s)) {
   case OPTK_RULINGS_SINUSOIDAL: {  // J_i(2 gamma), unsquared
     T gamma = pi * depth / (wavelength * cos_theta);
     // J_{-n}(x) = (-1)^n J_n(x), applied explicitly: the device Bessel
@@ -425,8 +491,9 @@ OPTK_HD T rulings_efficiency(const optk_surface &s, int order, T wavelength, V3<
 // materials (optika/materials/_materials.py, _multilayers.py)
 // ---------------------------------------------------------------------------
 
+template <uint32_t K = 0>
 OPTK_HD double material_index(const optk_surface &s, double wavelength, double n1) {
-  switch (s.material_kind) {
+  switch (k_material<K>(s)) {
   case OPTK_MATERIAL_VACUUM:
     return 1.0;
   case OPTK_MATERIAL_GLASS: {  // _materials.py:428-438
@@ -440,10 +507,10 @@ OPTK_HD double material_index(const optk_surface &s, double wavelength, double n
   }
 }
 
+template <uint32_t K = 0>
 OPTK_HD double material_attenuation(const optk_surface &s, double attenuation) {
-  return (s.material_kind == OPTK_MATERIAL_VACUUM || s.material_kind == OPTK_MATERIAL_GLASS)
-             ? 0.0
-             : attenuation;
+  const int kind = k_material<K>(s);
+  return (kind == OPTK_MATERIAL_VACUUM || kind == OPTK_MATERIAL_GLASS) ? 0.0 : attenuation;
 }
 
 // BackIlluminatedSiliconSensorMaterial.efficiency (optika/sensors/materials/
@@ -468,11 +535,11 @@ OPTK_HD double silicon_absorbance_average(const StackView &sv, int stack, double
   return (s_abs + p_abs) / 2.0;
 }
 
-template <int F = FEAT_ALL>
+template <int F = FEAT_ALL, uint32_t K = 0>
 OPTK_HD double material_efficiency(const optk_surface &s, const StackView &sv, double wavelength,
                                    V3<double> a, double index, double attenuation,
                                    V3<double> normal) {
-  switch (s.material_kind) {
+  switch (k_material<K>(s)) {
   case OPTK_MATERIAL_MEASURED_MIRROR: {  // _materials.py:279-305
     const double *w = sv.tables + s.material_table;
     return interp1(w, w + s.material_table_len, s.material_table_len, wavelength);
@@ -483,7 +550,7 @@ OPTK_HD double material_efficiency(const optk_surface &s, const StackView &sv, d
     double k = attenuation * wavelength / (4.0 * M_PI) * 1e-6;
     C<double> n = {index, k};
     double cosv = -dot3(a, normal);
-    bool film = s.material_kind == OPTK_MATERIAL_MULTILAYER_FILM;
+    bool film = k_material<K>(s) == OPTK_MATERIAL_MULTILAYER_FILM;
     MultilayerRT<double> rt =
         multilayer_efficiency<double>(sv, s.stack, wavelength, C<double>{cosv, 0.0}, n, film);
     return film ? (rt.T[0] + rt.T[1]) / 2.0 : (rt.R[0] + rt.R[1]) / 2.0;
@@ -524,11 +591,12 @@ OPTK_HD V3<double> snells_law(V3<double> a, double n1, double n2, V3<double> u, 
 // apertures (optika/apertures/_apertures.py)
 // ---------------------------------------------------------------------------
 
+template <uint32_t K = 0>
 OPTK_HD bool aperture_contains(const optk_surface &s, const double *tables, double x, double y,
                                double z) {
-  xf_inverse_pos(s.aperture_transform, x, y, z);
+  xf_inverse_pos<Kinds<K>::aperture_xf>(s.aperture_transform, x, y, z);
   bool mask = true;
-  switch (s.aperture_kind) {
+  switch (k_aperture<K>(s)) {
   case OPTK_APERTURE_CIRCLE:  // _apertures.py:292-314
     mask = sqrt(x * x + y * y) <= s.aperture_p[0];
     break;
@@ -573,8 +641,9 @@ OPTK_HD bool aperture_contains(const optk_surface &s, const double *tables, doub
   default:
     return true;
   }
-  if (s.aperture_flags & OPTK_AP_INVERTED) mask = !mask;
-  if (!(s.aperture_flags & OPTK_AP_ACTIVE)) mask = true;
+  const int flags = k_aperture_flags<K>(s);
+  if (flags & OPTK_AP_INVERTED) mask = !mask;
+  if (!(flags & OPTK_AP_ACTIVE)) mask = true;
   return mask;
 }
 
@@ -582,49 +651,49 @@ OPTK_HD bool aperture_contains(const optk_surface &s, const double *tables, doub
 // the surface step
 // ---------------------------------------------------------------------------
 
-template <int F = FEAT_ALL>
+template <int F = FEAT_ALL, uint32_t K = 0>
 OPTK_HD void propagate_surface(const optk_surface &s, const SystemView &sys, int path,
                                Ray<double> &r) {
-  xf_inverse_ray(s.transform, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
+  xf_inverse_ray<Kinds<K>::xf>(s.transform, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
 
-  sag_propagate<F>(s, r);
+  sag_propagate<F, K>(s, r);
   V3<double> p1 = {r.px, r.py, r.pz};
-  V3<double> normal = sag_normal(s, p1.x, p1.y, p1.z);
+  V3<double> normal = sag_normal<K>(s, p1.x, p1.y, p1.z);
 
   V3<double> a = {r.dx, r.dy, r.dz};
   int order = 0;
-  const bool ruled = (F & FEAT_RULINGS) && s.rulings_kind != OPTK_RULINGS_NONE;
+  const bool ruled = (F & FEAT_RULINGS) && k_rulings<K>(s) != OPTK_RULINGS_NONE;
   if (ruled) {
     order = path_order(s, path);
-    V3<double> kappa = spacing_vector<double>(s, p1, normal);
+    V3<double> kappa = spacing_vector<double, K>(s, p1, normal);
     a = incident_effective(s, order, r.wavelength, a, r.index, normal, kappa);
   }
 
   double n1 = r.index;
-  double n2 = material_index(s, r.wavelength, n1);
+  double n2 = material_index<K>(s, r.wavelength, n1);
   double ratio = index_ratio(n1, n2);
   double wavelength_2 = ratio == 1.0 ? r.wavelength : r.wavelength / ratio;
-  V3<double> b = snells_law(a, n1, n2, normal, s.is_mirror != 0);
+  V3<double> b = snells_law(a, n1, n2, normal, k_mirror<K>(s));
   double eff =
-      material_efficiency<F>(s, sys.sv, r.wavelength, a, r.index, r.attenuation, normal);
+      material_efficiency<F, K>(s, sys.sv, r.wavelength, a, r.index, r.attenuation, normal);
   if (ruled)
-    eff = eff * rulings_efficiency<double>(s, order, r.wavelength, a, p1, normal, sys.sv.tables);
+    eff = eff * rulings_efficiency<double, K>(s, order, r.wavelength, a, p1, normal, sys.sv.tables);
   r.intensity = r.intensity * eff;
-  r.attenuation = material_attenuation(s, r.attenuation);
+  r.attenuation = material_attenuation<K>(s, r.attenuation);
   r.index = n2;
   r.wavelength = wavelength_2;
   r.dx = b.x;
   r.dy = b.y;
   r.dz = b.z;
 
-  if (s.aperture_kind != OPTK_APERTURE_NONE) {
-    bool m = (s.aperture_flags & OPTK_AP_DIRECTION)
-                 ? aperture_contains(s, sys.sv.tables, r.dx, r.dy, r.dz)
-                 : aperture_contains(s, sys.sv.tables, r.px, r.py, r.pz);
+  if (k_aperture<K>(s) != OPTK_APERTURE_NONE) {
+    bool m = (k_aperture_flags<K>(s) & OPTK_AP_DIRECTION)
+                 ? aperture_contains<K>(s, sys.sv.tables, r.dx, r.dy, r.dz)
+                 : aperture_contains<K>(s, sys.sv.tables, r.px, r.py, r.pz);
     r.unvignetted = r.unvignetted && m;
   }
 
-  xf_apply_ray(s.transform, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
+  xf_apply_ray<Kinds<K>::xf>(s.transform, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
 }
 
 // ---------------------------------------------------------------------------

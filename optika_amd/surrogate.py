@@ -99,6 +99,14 @@ def _gemm(ta, tb, m, n, k, a, lda, b, ldb, c, ldc, split=1, ws=None):
                                          split, _ptr(ws), _stream()))
 
 
+def _gemm_fused(ta, tb, m, n, k, a, lda, b, ldb, c, ldc, epilogue, out16, bias=None, pre=None):
+    """GEMM with the layer's bias + SiLU (forward) or SiLU derivative
+    (backward) in its epilogue (optk_gemm_bf16_fused)."""
+    _lib.check(_lib.lib().optk_gemm_bf16_fused(ta, tb, m, n, k, _ptr(a), lda, _ptr(b), ldb, _ptr(c),
+                                               ldc, epilogue, _ptr(bias), _ptr(pre), _ptr(out16),
+                                               _stream()))
+
+
 class MLP:
     """Parameters (float32 master + bf16 copy) and Adam state of the
     network, with forward/backward passes on the library's kernels."""
@@ -140,11 +148,16 @@ class MLP:
             self.y = f(rows, 3)
             if train:
                 self.dy = f(rows, 3)
-                self.dact = f(rows, H)
+                # two bf16 gradient buffers: the backward GEMM of a layer
+                # reads one while its epilogue writes the previous layer's
+                # gradient into the other
                 self.dpre16 = h(rows, H)
+                self.dpre16_b = h(rows, H)
                 self.dpre32 = f(rows, H)
                 self.ones = torch.ones(rows, 1, dtype=torch.float32, device=self.dev)
-                split = max(1, min(64, rows // 1024))
+                # split the weight-gradient GEMM's k = rows so that its four
+                # 128 x 128 output tiles make ~2 blocks per CU
+                split = max(1, min(128, rows // 512))
                 while rows % (32 * split):
                     split //= 2
                 self.split = split
@@ -163,9 +176,13 @@ class MLP:
             u16=[self.act[0]], rows=rows, cols=H, inner=5)
         for layer in (1, 2, 3):
             w16 = self.view(self.p16, layer, 0)
-            _gemm(0, 1, rows, H, H, self.act[layer - 1], H, w16, H, self.pre[layer], H)
-            _op(_lib.MLP_BIAS_SILU, f32=[self.pre[layer], self.view(self.p, layer, 1)],
-                u16=[self.act[layer]], rows=rows, cols=H)
+            if rows % 128 == 0:
+                _gemm_fused(0, 1, rows, H, H, self.act[layer - 1], H, w16, H, self.pre[layer], H,
+                            _lib.GEMM_BIAS_SILU, self.act[layer], bias=self.view(self.p, layer, 1))
+            else:
+                _gemm(0, 1, rows, H, H, self.act[layer - 1], H, w16, H, self.pre[layer], H)
+                _op(_lib.MLP_BIAS_SILU, f32=[self.pre[layer], self.view(self.p, layer, 1)],
+                    u16=[self.act[layer]], rows=rows, cols=H)
         _op(_lib.MLP_DENSE_OUT, f32=[self.view(self.p, 4, 0), self.view(self.p, 4, 1), self.y],
             u16=[self.act[3]], rows=rows, cols=H, inner=3)
         return self.y
@@ -173,8 +190,10 @@ class MLP:
     # -- backward --------------------------------------------------------
     def loss_grad(self, x, target, weight) -> torch.Tensor:
         """Forward, weighted-MSE loss and gradients into ``self.g``;
-        returns the loss (a device scalar)."""
+        returns the loss (a device scalar).  rows: a multiple of 128."""
         rows = x.shape[0]
+        if rows % 128:
+            raise ValueError("training batches must hold a multiple of 128 rows")
         H = WIDTHS[1]
         self.forward(x, train=True)
         _op(_lib.MLP_MSE_GRAD, f32=[self.y, target, weight, self.dy, self.loss_part], rows=rows,
@@ -187,16 +206,18 @@ class MLP:
         _op(_lib.MLP_DENSE_OUT_BACK, f32=[self.dy, self.view(self.p, 4, 0), self.pre[3],
                                           self.dpre32], u16=[self.dpre16], rows=rows, cols=H,
             inner=3)
+        d16, d16_next = self.dpre16, self.dpre16_b
         for layer in (3, 2, 1):
             # weight gradient: gW[o][i] = sum_r dpre[r][o] act[r][i]
-            _gemm(1, 0, H, H, rows, self.dpre16, H, self.act[layer - 1], H, self.view(g, layer, 0),
+            _gemm(1, 0, H, H, rows, d16, H, self.act[layer - 1], H, self.view(g, layer, 0),
                   H, self.split, self.ws)
             _op(_lib.MLP_OUTER_GRAD, f32=[self.dpre32, self.ones, self.view(g, layer, 1), self.ws],
                 rows=rows, cols=H, inner=1, count=self.ws.numel(), flag=1)
-            # back to the previous layer's activations
-            _gemm(0, 0, rows, H, H, self.dpre16, H, self.view(self.p16, layer, 0), H, self.dact, H)
-            _op(_lib.MLP_SILU_BACK, f32=[self.dact, self.pre[layer - 1], self.dpre32],
-                u16=[self.dpre16], rows=rows, cols=H)
+            # back to the previous layer: dpre = (dpre W) * silu'(pre), the
+            # derivative in the GEMM's epilogue
+            _gemm_fused(0, 0, rows, H, H, d16, H, self.view(self.p16, layer, 0), H, self.dpre32, H,
+                        _lib.GEMM_SILU_BACK, d16_next, pre=self.pre[layer - 1])
+            d16, d16_next = d16_next, d16
         # first layer (float32)
         _op(_lib.MLP_OUTER_GRAD, f32=[self.dpre32, x, self.view(g, 0, 0), self.ws], rows=rows,
             cols=H, inner=5, count=self.ws.numel(), flag=1)
