@@ -501,11 +501,12 @@ int32_t optk_gemm_bf16(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, i
  * its epilogue, so the result is not re-read by a separate kernel:
  *   OPTK_GEMM_BIAS_SILU  c[i][j] = (A B)[i][j] + bias[j] (the pre-activation,
  *                        float32) and out16[i][j] = bf16(silu(c[i][j]));
- *   OPTK_GEMM_SILU_BACK  c[i][j] = (A B)[i][j] * silu'(pre[i][j]) (float32)
- *                        and out16[i][j] = bf16(c[i][j]);
+ *   OPTK_GEMM_SILU_BACK  c[i][j] = (A B)[i][j] * silu'(pre[i][j]) (float32;
+ *                        c may be NULL: not stored) and
+ *                        out16[i][j] = bf16((A B)[i][j] * silu'(pre[i][j]));
  * silu(x) = x / (1 + e^-x).  c, pre and out16 are [m][ldc].  m and n
- * multiples of 128, k a multiple of 32, leading dimensions multiples of 8
- * and operands 16-byte aligned. */
+ * multiples of 128, k a multiple of 32, lda and ldb multiples of 8, ldc a
+ * multiple of 4; a, b, c, pre and bias 16-byte aligned, out16 8-byte. */
 #define OPTK_GEMM_BIAS_SILU 1
 #define OPTK_GEMM_SILU_BACK 2
 int32_t optk_gemm_bf16_fused(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k,

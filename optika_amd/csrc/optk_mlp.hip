@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -50,7 +51,7 @@ __device__ __forceinline__ float silu_grad(float x) {
 // B(k, n) = b[k * ldb + n] (TB = 0) or b[n * ldb + k] (TB = 1).
 // Block tile 64 x 64 x 32, 4 waves each owning a 32 x 32 quarter as 2 x 2
 // MFMA 16x16x32 tiles.  Split-K: block z covers k in [z kc, (z + 1) kc) and
-// writes C + z * m * n (a partial sum, reduced by gemm_reduce).
+// writes C + z * m * n (a partial sum, reduced by reduce_partials).
 // Requires m, n multiples of 64 and k a multiple of 32 (checked on the host).
 // ---------------------------------------------------------------------------
 
@@ -198,8 +199,13 @@ __global__ __launch_bounds__(256) void gemm_bf16_128(const uint16_t *__restrict_
                                                      const uint16_t *__restrict__ b, int64_t ldb,
                                                      const GemmEpilogue E, int64_t m, int64_t n,
                                                      int64_t kc) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][GM][GK + GPAD];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][GN][GK + GPAD];
+  // operand tiles during the k loop; the epilogue reuses the same memory
+  // to stage half of the C tile (64 x 128 floats)
+  __shared__ __attribute__((aligned(16))) uint16_t smem[2 * 2 * GM * (GK + GPAD)];
+  auto As = reinterpret_cast<uint16_t(*)[GM][GK + GPAD]>(smem);
+  auto Bs = reinterpret_cast<uint16_t(*)[GN][GK + GPAD]>(smem + 2 * GM * (GK + GPAD));
+  auto Cs = reinterpret_cast<float(*)[GN + 4]>(smem);
+  static_assert(64 * (GN + 4) * sizeof(float) <= sizeof(smem), "C half tile fits the operand tiles");
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int64_t m0 = (int64_t)blockIdx.y * GM, n0 = (int64_t)blockIdx.x * GN;
@@ -243,44 +249,94 @@ __global__ __launch_bounds__(256) void gemm_bf16_128(const uint16_t *__restrict_
     }
     __syncthreads();
   }
-  // lane holds C[row = 4 (lane / 16) + r][col = lane % 16] of each 16 x 16 tile
-  float *cz = E.c + (int64_t)blockIdx.z * m * n;
+  // lane holds C[row = 4 (lane / 16) + r][col = lane % 16] of each 16 x 16
+  // tile.  The tile leaves through LDS, 64 rows at a time, so that every
+  // global access of the epilogue is a 16-byte (float4) or 8-byte (four
+  // bf16) access along a row.  The k loop ended on a barrier: the operand
+  // tiles are free.
+  float *cz = E.c ? E.c + (int64_t)blockIdx.z * m * n : nullptr;
   const int64_t ld = gridDim.z > 1 ? n : E.ldc;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int h = 0; h < 2; ++h) {
+    if ((wave >> 1) == h) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t col = n0 + wn + 16 * j + (lane & 15);
-      float bias = 0.0f;
-      if (EPI == EPI_BIAS_SILU) bias = E.bias[col];
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
-        const int64_t at = row * ld + col;
-        float v = acc[i][j][r];
-        if (EPI == EPI_BIAS_SILU) {
-          v += bias;
-          cz[at] = v;
-          E.out16[at] = to_bf16(silu(v));
-        } else if (EPI == EPI_SILU_BACK) {
-          const float g = v * silu_grad(E.pre[at]);
-          cz[at] = g;
-          E.out16[at] = to_bf16(g);
-        } else {
-          cz[at] = v;
-        }
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) Cs[16 * i + 4 * (lane >> 4) + r][wn + 16 * j + (lane & 15)] = acc[i][j][r];
+    }
+    __syncthreads();
+    const int cl = 4 * (tid & 31);
+    const int64_t col = n0 + cl;
+    float4 bias = float4{0.0f, 0.0f, 0.0f, 0.0f};
+    if (EPI == EPI_BIAS_SILU) bias = *reinterpret_cast<const float4 *>(E.bias + col);
+#pragma unroll 2
+    for (int pass = 0; pass < 8; ++pass) {
+      const int rl = 8 * pass + (tid >> 5);
+      float4 v = *reinterpret_cast<const float4 *>(&Cs[rl][cl]);
+      const int64_t at = (m0 + 64 * h + rl) * ld + col;
+      if (EPI == EPI_BIAS_SILU) {
+        v.x += bias.x;
+        v.y += bias.y;
+        v.z += bias.z;
+        v.w += bias.w;
+        *reinterpret_cast<float4 *>(cz + at) = v;
+        *reinterpret_cast<uint2 *>(E.out16 + at) =
+            uint2{pack_bf16(silu(v.x), silu(v.y)), pack_bf16(silu(v.z), silu(v.w))};
+      } else if (EPI == EPI_SILU_BACK) {
+        const float4 p = *reinterpret_cast<const float4 *>(E.pre + at);
+        v.x *= silu_grad(p.x);
+        v.y *= silu_grad(p.y);
+        v.z *= silu_grad(p.z);
+        v.w *= silu_grad(p.w);
+        if (cz) *reinterpret_cast<float4 *>(cz + at) = v;
+        *reinterpret_cast<uint2 *>(E.out16 + at) = uint2{pack_bf16(v.x, v.y), pack_bf16(v.z, v.w)};
+      } else {
+        *reinterpret_cast<float4 *>(cz + at) = v;
       }
     }
+    __syncthreads();
+  }
 }
 
-// c[i] = sum over splits of partial[s][i] (m * n elements)
-__global__ __launch_bounds__(256) void gemm_reduce(const float *__restrict__ partial, int splits,
-                                                   int64_t count, float *__restrict__ c) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+// out[j] = sum over s < splits of partial[s * count + j].  A block owns 64
+// outputs (one per lane); its 16 waves each sum every 16th split with eight
+// loads in flight, and the 16 wave sums are added in a fixed order, so the
+// result does not depend on timing.  Grid: ceil(count / 64) blocks.
+constexpr int RED_WAVES = 16;
+
+__global__ __launch_bounds__(64 * RED_WAVES) void reduce_partials(const float *__restrict__ partial,
+                                                                  int splits, int64_t count,
+                                                                  float *__restrict__ out) {
+  __shared__ float red[RED_WAVES][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t j = (int64_t)blockIdx.x * 64 + lane;
   float s = 0.0f;
-  for (int z = 0; z < splits; ++z) s += partial[(int64_t)z * count + i];
-  c[i] = s;
+  if (j < count) {
+    const float *p = partial + j;
+    int z = w;
+    for (; z + 7 * RED_WAVES < splits; z += 8 * RED_WAVES) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = p[(int64_t)(z + u * RED_WAVES) * count];
+      s += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+    }
+    for (; z < splits; z += RED_WAVES) s += p[(int64_t)z * count];
+  }
+  red[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && j < count) {
+    float t = 0.0f;
+#pragma unroll
+    for (int u = 0; u < RED_WAVES; ++u) t += red[u][lane];
+    out[j] = t;
+  }
+}
+
+inline void launch_reduce(const float *partial, int splits, int64_t count, float *out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_partials, dim3((unsigned)((count + 63) / 64)), dim3(64 * RED_WAVES), 0, s,
+                     partial, splits, count, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -383,7 +439,7 @@ __global__ __launch_bounds__(256) void silu_back(const float *__restrict__ dact,
 // Partial outer products for the small layers' weight gradients: block b
 // sums rows [b R, (b + 1) R) of x[r][c] * g[r][j] for c < cols (one thread
 // per column, cols <= 256) and j < inner (<= 8); the partials are reduced
-// over blocks by gemm_reduce.  x is float32 (x32) or bf16 (x16).  Layout of
+// over blocks by reduce_partials.  x is float32 (x32) or bf16 (x16).  Layout of
 // a partial: [j][c] (transposed = 0) or [c][j] (transposed = 1).
 __global__ __launch_bounds__(256) void outer_grad(const float *__restrict__ x32,
                                                   const uint16_t *__restrict__ x16, int cols,
@@ -396,6 +452,7 @@ __global__ __launch_bounds__(256) void outer_grad(const float *__restrict__ x32,
   const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
   int64_t r1 = r0 + rows_per_block;
   if (r1 > rows) r1 = rows;
+#pragma unroll 4
   for (int64_t r = r0; r < r1; ++r) {
     float xv = x32 ? x32[r * cols + c] : from_bf16(x16[r * cols + c]);
     for (int j = 0; j < inner; ++j) acc[j] += xv * g[r * inner + j];
@@ -406,6 +463,156 @@ __global__ __launch_bounds__(256) void outer_grad(const float *__restrict__ x32,
       out[(int64_t)c * inner + j] = acc[j];
     else
       out[(int64_t)j * cols + c] = acc[j];
+  }
+}
+
+// ---- vectorised forms of the small layers (four consecutive features per
+// thread, 16-byte float and 8-byte bf16 accesses); the same arithmetic
+// order as the scalar kernels above, which serve the shapes these do not
+
+// dense_in for IN inputs, cols a multiple of 4
+template <int IN>
+__global__ __launch_bounds__(256) void dense_in4(const float *__restrict__ x, const float *__restrict__ w,
+                                                 const float *__restrict__ bias, float *__restrict__ pre,
+                                                 uint16_t *__restrict__ act, int64_t rows, int cols) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int cq = cols / 4;
+  if (q >= rows * cq) return;
+  const int64_t r = q / cq;
+  const int o = (int)(q % cq) * 4;
+  float xv[IN];
+#pragma unroll
+  for (int k = 0; k < IN; ++k) xv[k] = x[r * IN + k];
+  float v[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    v[u] = bias[o + u];
+#pragma unroll
+    for (int k = 0; k < IN; ++k) v[u] += xv[k] * w[(o + u) * IN + k];
+  }
+  const int64_t at = r * cols + o;
+  if (pre) *reinterpret_cast<float4 *>(pre + at) = float4{v[0], v[1], v[2], v[3]};
+  *reinterpret_cast<uint2 *>(act + at) =
+      uint2{pack_bf16(silu(v[0]), silu(v[1])), pack_bf16(silu(v[2]), silu(v[3]))};
+}
+
+// dense_out for hidden == 256 and OUT outputs: one wave per row, lane l
+// holding features 4 l .. 4 l + 3; the 64 lane sums meet in a butterfly
+template <int OUT>
+__global__ __launch_bounds__(256) void dense_out_wave(const uint16_t *__restrict__ act,
+                                                      const float *__restrict__ w,
+                                                      const float *__restrict__ bias,
+                                                      float *__restrict__ y, int64_t rows) {
+  const int lane = threadIdx.x & 63;
+  const int64_t first = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  float wl[OUT][4];
+#pragma unroll
+  for (int j = 0; j < OUT; ++j)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) wl[j][u] = w[j * 256 + 4 * lane + u];
+  for (int64_t r = first; r < rows; r += stride) {
+    const uint2 a = *reinterpret_cast<const uint2 *>(act + r * 256 + 4 * lane);
+    const float h[4] = {__uint_as_float(a.x << 16), __uint_as_float(a.x & 0xffff0000u),
+                        __uint_as_float(a.y << 16), __uint_as_float(a.y & 0xffff0000u)};
+    float s[OUT];
+#pragma unroll
+    for (int j = 0; j < OUT; ++j) {
+      s[j] = (h[0] * wl[j][0] + h[1] * wl[j][1]) + (h[2] * wl[j][2] + h[3] * wl[j][3]);
+#pragma unroll
+      for (int m = 32; m > 0; m >>= 1) s[j] += __shfl_xor(s[j], m, 64);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < OUT; ++j) y[r * OUT + j] = s[j] + bias[j];
+    }
+  }
+}
+
+// dense_out_back for OUT outputs, hidden a multiple of 4
+template <int OUT>
+__global__ __launch_bounds__(256) void dense_out_back4(const float *__restrict__ dy,
+                                                       const float *__restrict__ w,
+                                                       const float *__restrict__ pre,
+                                                       uint16_t *__restrict__ dpre,
+                                                       float *__restrict__ dpre32, int64_t rows,
+                                                       int hidden) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int hq = hidden / 4;
+  if (q >= rows * hq) return;
+  const int64_t r = q / hq;
+  const int h = (int)(q % hq) * 4;
+  float d[OUT];
+#pragma unroll
+  for (int j = 0; j < OUT; ++j) d[j] = dy[r * OUT + j];
+  const int64_t at = r * hidden + h;
+  const float4 p = *reinterpret_cast<const float4 *>(pre + at);
+  const float pv[4] = {p.x, p.y, p.z, p.w};
+  float g[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < OUT; ++j) s += d[j] * w[j * hidden + h + u];
+    g[u] = s * silu_grad(pv[u]);
+  }
+  *reinterpret_cast<uint2 *>(dpre + at) = uint2{pack_bf16(g[0], g[1]), pack_bf16(g[2], g[3])};
+  *reinterpret_cast<float4 *>(dpre32 + at) = float4{g[0], g[1], g[2], g[3]};
+}
+
+// outer_grad for cols a multiple of 4 (<= 256) and INNER columns of g:
+// thread (wave p, lane c) sums rows r0 + p, r0 + p + 4, ... of columns
+// 4 c .. 4 c + 3; the four row phases are added in LDS in a fixed order
+template <int INNER, bool X16>
+__global__ __launch_bounds__(256) void outer_grad4(const float *__restrict__ x32,
+                                                   const uint16_t *__restrict__ x16, int cols,
+                                                   const float *__restrict__ g, int64_t rows,
+                                                   int64_t rows_per_block, int transposed,
+                                                   float *__restrict__ partial) {
+  __shared__ float red[4][256 * INNER];
+  const int c4 = 4 * (threadIdx.x & 63), phase = threadIdx.x >> 6;
+  float acc[4][INNER];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int j = 0; j < INNER; ++j) acc[u][j] = 0.0f;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
+  if (c4 < cols) {
+#pragma unroll 2
+    for (int64_t r = r0 + phase; r < r1; r += 4) {
+      float xv[4];
+      if (X16) {
+        const uint2 a = *reinterpret_cast<const uint2 *>(x16 + r * cols + c4);
+        xv[0] = __uint_as_float(a.x << 16);
+        xv[1] = __uint_as_float(a.x & 0xffff0000u);
+        xv[2] = __uint_as_float(a.y << 16);
+        xv[3] = __uint_as_float(a.y & 0xffff0000u);
+      } else {
+        const float4 a = *reinterpret_cast<const float4 *>(x32 + r * cols + c4);
+        xv[0] = a.x;
+        xv[1] = a.y;
+        xv[2] = a.z;
+        xv[3] = a.w;
+      }
+#pragma unroll
+      for (int j = 0; j < INNER; ++j) {
+        const float gj = g[r * INNER + j];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u][j] += xv[u] * gj;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int j = 0; j < INNER; ++j) red[phase][(c4 + u) * INNER + j] = acc[u][j];
+  __syncthreads();
+  float *out = partial + (int64_t)blockIdx.x * cols * INNER;
+  for (int idx = threadIdx.x; idx < cols * INNER; idx += blockDim.x) {
+    const float v = (red[0][idx] + red[1][idx]) + (red[2][idx] + red[3][idx]);
+    const int c = idx / INNER, j = idx % INNER;
+    out[transposed ? idx : j * cols + c] = v;
   }
 }
 
@@ -578,45 +785,200 @@ __global__ __launch_bounds__(256) void poly_eval_kernel(const double *__restrict
 // aperture-edge distances, the error-map lookup and the compaction of the
 // rays that fall back to the trace -- one kernel, activations in LDS.
 //
-// Block: 256 threads (4 waves), a tile of 128 rays (64: four blocks); two
-// blocks share a CU, so one block's MFMA phase overlaps the other's VALU phase (SiLU, the
-// polynomial).  Every layer runs on MFMA 16x16x32 bf16 with float32
-// accumulation.  Layer 1 (5 -> 256): the normalised inputs and the weights
-// are split into bf16 high and low parts, k row [hi | lo | hi] against
-// weight row [w_hi | w_hi | w_lo], so the one 32-wide k step carries the
-// float32 product to ~2^-16.  Hidden layers: a 128 x 256 x 256 product
-// each.  Wave w computes all 128 rays x features [64 w, 64 w + 64) as
-// 8 x 4 tiles, A operands (activations act[ray][feature], bf16) from LDS,
-// B operands (weights, row-major [out][in], k contiguous) straight from
-// global memory (L2-resident), accumulators started at the bias.  The B
-// operand of n-tile t and lane column c is weight row 64 w + 4 c + t, so
+// Block: 256 threads (4 waves), a persistent loop over tiles of 128 rays;
+// two blocks share a CU, so one block's MFMA phase overlaps the other's
+// VALU phase (SiLU, the polynomial).  Every layer runs on MFMA 16x16x32
+// bf16 with float32 accumulation.  Layer 1 (5 -> 256): the normalised
+// inputs and the weights are split into bf16 high and low parts, k row
+// [hi | lo | hi] against weight row [w_hi | w_hi | w_lo], so the one
+// 32-wide k step carries the float32 product to ~2^-16.  Hidden layers: a
+// 128 x 256 x 256 product each.  Wave w computes all 128 rays x features
+// [64 w, 64 w + 64) as 8 x 4 tiles, A operands (activations
+// act[ray][feature], bf16) from LDS, B operands (weights, row-major
+// [out][in], k contiguous) straight from global memory (L2-resident).  The
+// B operand of n-tile t and lane column c is weight row 64 w + 4 c + t, so
 // each lane's four accumulator tiles hold four consecutive features: the
-// epilogue (SiLU, bf16) writes them as one 8-byte store per row.  The
-// output layer (3 rows, high and low bf16 parts) is one more MFMA tile per
-// 16 rays; the polynomial and edge tests run one thread per ray.
+// epilogue (bias, SiLU, bf16) writes them as one 8-byte store per row.
+// Weights stream through two register buffers, each chunk (32 k values)
+// requested one chunk ahead of its MFMAs -- across layer boundaries too --
+// so the L2 latency hides behind 32 MFMAs (and the other wave's on the SIMD).  The output
+// layer (3 rows, high and low bf16 parts) is one more MFMA tile per 16
+// rays.  The polynomial runs on all four waves (waves 0-1 and 2-3 each
+// take half of the monomials of the same 128 rays); the edge tests, the
+// error map and the stores then run one thread per ray.
 // ---------------------------------------------------------------------------
 
-#ifndef OPTK_INF_RAYS
-#define OPTK_INF_RAYS 128
-#endif
-constexpr int INF_RAYS = OPTK_INF_RAYS;  // rays per tile: 64 or 128
-constexpr int INF_MT = INF_RAYS / 16;    // 16-ray MFMA row tiles per tile
-constexpr int INF_OQ = INF_MT / 4;       // output-layer row tiles per wave
-constexpr int INF_BLOCKS_PER_CU = INF_RAYS == 64 ? 4 : 2;
-static_assert(INF_RAYS == 64 || INF_RAYS == 128, "tile of 64 or 128 rays");
+constexpr int INF_RAYS = 128;          // rays per tile
+constexpr int INF_MT = INF_RAYS / 16;  // 16-ray MFMA row tiles per tile
+constexpr int INF_OQ = INF_MT / 4;     // output-layer row tiles per wave
+constexpr int INF_BLOCKS_PER_CU = 2;
 constexpr int INF_HIDDEN = 256;
 constexpr int INF_PAD = 8;
+constexpr int INF_TARGETS = OPTK_SURROGATE_MAX_TARGETS;
+
+// The kernel's order of the 56 cubic monomials: depth first over
+// nondecreasing variable indices (1, u0, u0^2, u0^3, u0^2 u1, ..., u4^3),
+// so that each monomial is its parent times one input.  A subtree (the
+// monomials whose lowest variable is a) starts at POLY_SUBTREE[a].  The
+// launch maps every kernel-order monomial to its row of the caller's poly.
+struct PolyOrder {
+  uint8_t row[OPTK_SURROGATE_POLY];
+};
+constexpr int POLY_SUBTREE[5] = {1, 22, 37, 47, 53};
+
+static void kernel_poly_codes(uint32_t (&codes)[OPTK_SURROGATE_POLY]) {
+  int t = 0;
+  codes[t++] = 0;
+  for (int a = 0; a < 5; ++a) {
+    const uint32_t ea = 1u << (2 * a);
+    codes[t++] = ea;
+    for (int b = a; b < 5; ++b) {
+      const uint32_t eb = ea + (1u << (2 * b));
+      codes[t++] = eb;
+      for (int c = b; c < 5; ++c) codes[t++] = eb + (1u << (2 * c));
+    }
+  }
+}
+
+__device__ __forceinline__ void poly_term(float m, const float *__restrict__ c,
+                                          float (&tgt)[INF_TARGETS]) {
+#pragma unroll
+  for (int j = 0; j < INF_TARGETS; ++j) tgt[j] = fmaf(m, c[j], tgt[j]);
+}
+
+// the monomials whose lowest variable is A (kernel order from S); coef:
+// the coefficients in kernel order
+template <int A, int S>
+__device__ __forceinline__ void poly_subtree(const float (&u)[5], const float (*coef)[INF_TARGETS],
+                                             float (&tgt)[INF_TARGETS]) {
+  int t = S;
+  const float m1 = u[A];
+  poly_term(m1, coef[t++], tgt);
+#pragma unroll
+  for (int b = A; b < 5; ++b) {
+    const float m2 = m1 * u[b];
+    poly_term(m2, coef[t++], tgt);
+#pragma unroll
+    for (int c = b; c < 5; ++c) poly_term(m2 * u[c], coef[t++], tgt);
+  }
+}
+
+typedef uint16_t ActRow[INF_HIDDEN + INF_PAD];
+
+// this lane's B operands of one 32-wide k chunk: rows col .. col + 3 of W
+// ([out][K] bf16), k values k0 + fk .. k0 + fk + 7.  voff = (col K + fk) * 2
+// bytes: one 32-bit offset against the uniform base, so that no per-lane
+// 64-bit row addresses stay live across the tile loop
+template <int K>
+__device__ __forceinline__ void wload(uint4 (&w)[4], const uint16_t *__restrict__ W, uint32_t voff, int k0) {
+  const char *base = reinterpret_cast<const char *>(W) + 2 * k0;
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) w[nt] = *reinterpret_cast<const uint4 *>(base + voff + 2 * K * nt);
+}
+
+// acc (+)= act[:, k0 .. k0 + 32) x the chunk's weights (first: acc = product).
+// The A operand of row tile mt + 1 is read while tile mt's four MFMAs run;
+// the scheduling barriers keep the compiler from reading all eight ahead
+// (32 more live registers than the two blocks per CU leave room for)
+template <bool FIRST>
+__device__ __forceinline__ void mfma_chunk(floatx4 (&acc)[INF_MT][4], const uint4 (&w)[4],
+                                           const ActRow *act, int k0, int fr, int fk) {
+  bf16x8 b[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) b[nt] = __builtin_bit_cast(bf16x8, w[nt]);
+  uint4 a_cur = *reinterpret_cast<const uint4 *>(&act[fr][k0 + fk]);
+#pragma unroll
+  for (int mt = 0; mt < INF_MT; ++mt) {
+    uint4 a_next = a_cur;
+    if (mt + 1 < INF_MT) a_next = *reinterpret_cast<const uint4 *>(&act[16 * (mt + 1) + fr][k0 + fk]);
+    __builtin_amdgcn_sched_barrier(0);
+    const bf16x8 a = __builtin_bit_cast(bf16x8, a_cur);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+      acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+          a, b[nt], FIRST ? floatx4{0.0f, 0.0f, 0.0f, 0.0f} : acc[mt][nt], 0, 0, 0);
+    a_cur = a_next;
+  }
+}
+
+// one hidden layer (K = 256): chunk c's weights sit in ring slot c % 2;
+// chunk c requests chunk c + 1 into the other slot before its MFMAs (past
+// the end: the next layer's first chunk, into slot 0, where that layer
+// starts)
+template <bool NEXT>
+__device__ __forceinline__ void hidden_layer(floatx4 (&acc)[INF_MT][4], uint4 (&ring)[2][4],
+                                             const uint16_t *__restrict__ W,
+                                             const uint16_t *__restrict__ Wn, const ActRow *act,
+                                             uint32_t voff, int fr, int fk) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    if (c + 1 < 8)
+      wload<INF_HIDDEN>(ring[(c + 1) % 2], W, voff, 32 * (c + 1));
+    else if (NEXT)
+      wload<INF_HIDDEN>(ring[0], Wn, voff, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    if (c == 0)
+      mfma_chunk<true>(acc, ring[c % 2], act, 32 * c, fr, fk);
+    else
+      mfma_chunk<false>(acc, ring[c % 2], act, 32 * c, fr, fk);
+  }
+}
+
+// act[row][col .. col + 3] = bf16(silu(acc + bias)) for the lane's rows
+__device__ __forceinline__ void silu_store(const floatx4 (&acc)[INF_MT][4], const float (&bias)[4],
+                                           ActRow *act, int col, int lane) {
+#pragma unroll
+  for (int mt = 0; mt < INF_MT; ++mt)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int row = 16 * mt + 4 * (lane >> 4) + q;
+      uint2 v;
+      v.x = pack_bf16(silu(acc[mt][0][q] + bias[0]), silu(acc[mt][1][q] + bias[1]));
+      v.y = pack_bf16(silu(acc[mt][2][q] + bias[2]), silu(acc[mt][3][q] + bias[3]));
+      *reinterpret_cast<uint2 *>(&act[row][col]) = v;
+    }
+}
+
+__device__ __forceinline__ void load_bias(float (&b)[4], const float *__restrict__ bias, uint32_t col) {
+  const char *base = reinterpret_cast<const char *>(bias);
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) b[nt] = *reinterpret_cast<const float *>(base + 4 * col + 4 * nt);
+}
 
 __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const optk_surrogate_desc D,
-                                                          const optk_surrogate_io IO) {
-  __shared__ __attribute__((aligned(16))) uint16_t act[INF_RAYS][INF_HIDDEN + INF_PAD];
+                                                                          const optk_surrogate_io IO,
+                                                                          const PolyOrder P) {
+  __shared__ __attribute__((aligned(16))) ActRow act[INF_RAYS];
+  __shared__ float yout[INF_RAYS][8];
+  __shared__ float us[INF_RAYS][5];
+  __shared__ float coef[OPTK_SURROGATE_POLY][INF_TARGETS];  // kernel order
+  // after the output layer: the two halves' polynomial sums, in act's memory
+  auto part = reinterpret_cast<float(*)[INF_RAYS][INF_TARGETS]>(&act[0][0]);
+  static_assert(2 * INF_RAYS * INF_TARGETS * sizeof(float) <= sizeof(act), "partial sums fit act");
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  const int col = wave * 64 + 4 * fr;  // the lane's four output features
+  const uint32_t voff_h = 2u * (uint32_t)(col * INF_HIDDEN + fk), voff1_h = 2u * (uint32_t)(col * 32 + fk);
+  const uint32_t voff5_h = 2u * (uint32_t)(fr * INF_HIDDEN + fk), bcol_h = (uint32_t)col;
   const int64_t n = IO.n;
   const int64_t tiles = (n + INF_RAYS - 1) / INF_RAYS;
   const float *in[5] = {IO.field_x, IO.field_y, IO.pupil_x, IO.pupil_y, IO.wavelength};
+  for (int idx = tid; idx < OPTK_SURROGATE_POLY * INF_TARGETS; idx += blockDim.x)
+    coef[idx / INF_TARGETS][idx % INF_TARGETS] = D.poly[P.row[idx / INF_TARGETS] * INF_TARGETS + idx % INF_TARGETS];
   for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const int64_t base = tile * INF_RAYS;
+    // opaque per tile: the weight and bias addresses are formed here, not
+    // hoisted out of the loop as dozens of live 64-bit pointers
+    uint32_t voff = voff_h, voff1 = voff1_h, voff5 = voff5_h, bcol = bcol_h;
+    asm volatile("" : "+v"(voff), "+v"(voff1), "+v"(voff5), "+v"(bcol));
+    // weights of layer 1 and the first chunk of layer 2, requested before
+    // the inputs are staged
+    uint4 w1[4], ring[2][4];
+    wload<32>(w1, D.w1_16, voff1, 0);
+    wload<INF_HIDDEN>(ring[0], D.w16[0], voff, 0);
+    float bias[4];
+    load_bias(bias, D.b1, bcol);
     // ---- inputs: thread t < 128 -> ray t, normalised and split into bf16
     // high and low parts, as the 32-wide k row [hi | lo | hi | 0] that meets
     // the first layer's weight rows [w_hi | w_hi | w_lo | 0] ---------------
@@ -627,6 +989,7 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
       for (int k = 0; k < 5; ++k) {
         const float v = i < n ? in[k][i] : 0.0f;
         const float u = (v - D.in_mid[k]) * D.in_inv_half[k];
+        us[tid][k] = u;
         hi[k] = from_bf16(to_bf16(u));
         lo[k] = u - hi[k];
       }
@@ -645,81 +1008,47 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
       *reinterpret_cast<uint4 *>(&act[tid][24]) = uint4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
-    // ---- four layers on MFMA: wave w -> features [64 w, 64 w + 64) of all
-    // 128 rays; the first layer's k extent is one 32-wide step ------------
-    const int obase = wave * 64;
-    const int fr = lane & 15, fk = (lane >> 4) * 8;
-    for (int layer = 0; layer < 4; ++layer) {
-      const uint16_t *W = layer == 0 ? D.w1_16 : D.w16[layer - 1];
-      const float *bias = layer == 0 ? D.b1 : D.b[layer - 1];
-      const int K = layer == 0 ? 32 : INF_HIDDEN;
-      // lane (row group g, column c) accumulates features obase + 4 c + nt,
-      // started at the bias
-      const int col = obase + 4 * fr;
-      floatx4 acc[INF_MT][4];
+    floatx4 acc[INF_MT][4];
+    // ---- layer 1: one 32-wide k step ------------------------------------
+    mfma_chunk<true>(acc, w1, act, 0, fr, fk);
+    __syncthreads();  // every wave has read the input rows
+    silu_store(acc, bias, act, bcol, lane);
+    __syncthreads();
+    // ---- hidden layers: ring slot 0 holds each one's first chunk ---------
+    load_bias(bias, D.b[0], bcol);
+    hidden_layer<true>(acc, ring, D.w16[0], D.w16[1], act, voff, fr, fk);
+    __syncthreads();
+    silu_store(acc, bias, act, bcol, lane);
+    __syncthreads();
+    load_bias(bias, D.b[1], bcol);
+    hidden_layer<true>(acc, ring, D.w16[1], D.w16[2], act, voff, fr, fk);
+    __syncthreads();
+    silu_store(acc, bias, act, bcol, lane);
+    __syncthreads();
+    load_bias(bias, D.b[2], bcol);
+    hidden_layer<false>(acc, ring, D.w16[2], nullptr, act, voff, fr, fk);
+    // the output layer's weights, requested before the last SiLU
+    uint4 w5[INF_HIDDEN / 32];
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-        const float bn = bias[col + nt];
-#pragma unroll
-        for (int mt = 0; mt < INF_MT; ++mt) acc[mt][nt] = floatx4{bn, bn, bn, bn};
-      }
-      // software pipeline: the next k-chunk's weights are requested before
-      // this chunk's 32 MFMAs, so the L2 latency hides behind them
-      const uint16_t *wrow[4];
-#pragma unroll
-      for (int nt = 0; nt < 4; ++nt) wrow[nt] = W + (int64_t)(col + nt) * K + fk;
-      uint4 bnext[4];
-#pragma unroll
-      for (int nt = 0; nt < 4; ++nt) bnext[nt] = *reinterpret_cast<const uint4 *>(wrow[nt]);
-#pragma unroll
-      for (int kc = 0; kc < INF_HIDDEN; kc += 32) {
-        if (kc < K) {
-          bf16x8 bw[4];
-#pragma unroll
-          for (int nt = 0; nt < 4; ++nt) bw[nt] = __builtin_bit_cast(bf16x8, bnext[nt]);
-          if (kc + 32 < K) {
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt) bnext[nt] = *reinterpret_cast<const uint4 *>(wrow[nt] + kc + 32);
-          }
-#pragma unroll
-          for (int mt = 0; mt < INF_MT; ++mt) {
-            uint4 v = *reinterpret_cast<const uint4 *>(&act[16 * mt + fr][kc + fk]);
-            bf16x8 a = __builtin_bit_cast(bf16x8, v);
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt)
-              acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[nt], acc[mt][nt], 0, 0, 0);
-          }
-        }
-      }
-      __syncthreads();  // every wave has read this layer's inputs
-#pragma unroll
-      for (int mt = 0; mt < INF_MT; ++mt)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int row = 16 * mt + 4 * (lane >> 4) + q;
-          uint2 v;
-          v.x = pack_bf16(silu(acc[mt][0][q]), silu(acc[mt][1][q]));
-          v.y = pack_bf16(silu(acc[mt][2][q]), silu(acc[mt][3][q]));
-          *reinterpret_cast<uint2 *>(&act[row][col]) = v;
-        }
-      __syncthreads();
-    }
+    for (int kc = 0; kc < INF_HIDDEN / 32; ++kc)
+      w5[kc] = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(D.w5_16) + 64 * kc + voff5);
+    __syncthreads();
+    silu_store(acc, bias, act, bcol, lane);
+    __syncthreads();
     // ---- output layer on MFMA: wave w -> rays [16 INF_OQ w, +16 INF_OQ).
     // The three outputs' weights are split into bf16 high and low parts
     // (rows 0..2 and 3..5 of one 16-row tile, the rest zero), so the
     // products carry the float32 weights to ~2^-17 at no extra MFMA --------
-    __shared__ float yout[INF_RAYS][8];
     {
       floatx4 oacc[INF_OQ];
 #pragma unroll
       for (int q = 0; q < INF_OQ; ++q) oacc[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int kc = 0; kc < INF_HIDDEN; kc += 32) {
-        uint4 vb = *reinterpret_cast<const uint4 *>(D.w5_16 + fr * INF_HIDDEN + kc + fk);
-        bf16x8 b5 = __builtin_bit_cast(bf16x8, vb);
+      for (int kc = 0; kc < INF_HIDDEN / 32; ++kc) {
+        const bf16x8 b5 = __builtin_bit_cast(bf16x8, w5[kc]);
 #pragma unroll
         for (int q = 0; q < INF_OQ; ++q) {
-          uint4 va = *reinterpret_cast<const uint4 *>(&act[16 * (INF_OQ * wave + q) + fr][kc + fk]);
+          const uint4 va = *reinterpret_cast<const uint4 *>(&act[16 * (INF_OQ * wave + q) + fr][32 * kc + fk]);
           oacc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, va), b5, oacc[q], 0, 0, 0);
         }
       }
@@ -730,44 +1059,41 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
           for (int r = 0; r < 4; ++r) yout[16 * (INF_OQ * wave + q) + 4 * (lane >> 4) + r][fr] = oacc[q][r];
       }
     }
+    __syncthreads();  // act is free: the polynomial's partial sums go there
+    // ---- polynomial: waves 0-1 take the constant and subtrees 0 and 3,
+    // waves 2-3 subtrees 1, 2 and 4 (28 monomials each) of rays tid % 128 --
+    {
+      const int ray = tid & (INF_RAYS - 1);
+      float u[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) u[k] = us[ray][k];
+      float tgt[INF_TARGETS];
+#pragma unroll
+      for (int j = 0; j < INF_TARGETS; ++j) tgt[j] = 0.0f;
+      if (wave < 2) {
+        poly_term(1.0f, coef[0], tgt);
+        poly_subtree<0, POLY_SUBTREE[0]>(u, coef, tgt);
+        poly_subtree<3, POLY_SUBTREE[3]>(u, coef, tgt);
+      } else {
+        poly_subtree<1, POLY_SUBTREE[1]>(u, coef, tgt);
+        poly_subtree<2, POLY_SUBTREE[2]>(u, coef, tgt);
+        poly_subtree<4, POLY_SUBTREE[4]>(u, coef, tgt);
+      }
+#pragma unroll
+      for (int j = 0; j < INF_TARGETS; ++j) part[wave >> 1][ray][j] = tgt[j];
+    }
     __syncthreads();
-    // ---- polynomial, apertures, fallback: one thread per ray ------------
+    // ---- apertures, error map, stores: one thread per ray -----------------
     bool fallback = false;
     const int64_t i = base + tid;
     const bool live = tid < INF_RAYS && i < n;
     if (live) {
+      float tgt[INF_TARGETS];
+#pragma unroll
+      for (int j = 0; j < INF_TARGETS; ++j) tgt[j] = part[0][tid][j] + part[1][tid][j];
       const float y0 = yout[tid][0] + yout[tid][3] + D.b5[0];
       const float y1 = yout[tid][1] + yout[tid][4] + D.b5[1];
       const float y2 = yout[tid][2] + yout[tid][5] + D.b5[2];
-      float u[5];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) u[k] = (in[k][i] - D.in_mid[k]) * D.in_inv_half[k];
-      // cubic monomials, in the order of optk_surrogate_desc.poly_exp
-      float tgt[OPTK_SURROGATE_MAX_TARGETS];
-#pragma unroll
-      for (int j = 0; j < OPTK_SURROGATE_MAX_TARGETS; ++j) tgt[j] = 0.0f;
-      float pw[5][4];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        pw[k][0] = 1.0f;
-        pw[k][1] = u[k];
-        pw[k][2] = u[k] * u[k];
-        pw[k][3] = pw[k][2] * u[k];
-      }
-      const int nt = D.num_targets;
-      for (int t = 0; t < OPTK_SURROGATE_POLY; ++t) {
-        const uint32_t e = D.poly_exp[t];  // 2 bits per variable
-        float mono = 1.0f;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-          const uint32_t ek = (e >> (2 * k)) & 3u;
-          mono *= ek == 0 ? 1.0f : (ek == 1 ? pw[k][1] : (ek == 2 ? pw[k][2] : pw[k][3]));
-        }
-        const float *c = D.poly + t * OPTK_SURROGATE_MAX_TARGETS;
-#pragma unroll
-        for (int j = 0; j < OPTK_SURROGATE_MAX_TARGETS; ++j)
-          if (j < nt) tgt[j] += mono * c[j];
-      }
       const float x = tgt[0] + D.scale[0] * y0;
       const float yy = tgt[1] + D.scale[1] * y1;
       const float li = tgt[2] + D.scale[2] * y2;
@@ -804,7 +1130,8 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
         near = near || fabsf(d) < D.edge_margin;
       }
       // error-map cell: 16 wavelength x 8 field x x 8 field y bins of [-1, 1]
-      int cw = (int)((u[4] + 1.0f) * 8.0f), cx = (int)((u[0] + 1.0f) * 4.0f), cy = (int)((u[1] + 1.0f) * 4.0f);
+      int cw = (int)((us[tid][4] + 1.0f) * 8.0f), cx = (int)((us[tid][0] + 1.0f) * 4.0f),
+          cy = (int)((us[tid][1] + 1.0f) * 4.0f);
       cw = cw < 0 ? 0 : (cw > 15 ? 15 : cw);
       cx = cx < 0 ? 0 : (cx > 7 ? 7 : cx);
       cy = cy < 0 ? 0 : (cy > 7 ? 7 : cy);
@@ -831,7 +1158,7 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
         }
       }
     }
-    __syncthreads();  // act is rewritten by the next tile's first layer
+    __syncthreads();  // act, us and yout are rewritten by the next tile
   }
 }
 
@@ -858,8 +1185,11 @@ int32_t optk_gemm_bf16(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, i
   const uint16_t *A = (const uint16_t *)a, *B = (const uint16_t *)b;
   if (m % GM == 0 && n % GN == 0) {
     // 128 x 128 tiles; the transposed operands also need 16-byte aligned rows
+    // (the epilogue stores float4 along rows of C)
     const bool rows_ok = (trans_a ? (lda % 8 == 0 && (uintptr_t)a % 16 == 0) : true) &&
-                         (!trans_b ? (ldb % 8 == 0 && (uintptr_t)b % 16 == 0) : true);
+                         (!trans_b ? (ldb % 8 == 0 && (uintptr_t)b % 16 == 0) : true) &&
+                         (split_k > 1 || (ldc % 4 == 0 && (uintptr_t)c % 16 == 0)) &&
+                         (split_k == 1 || (uintptr_t)workspace % 16 == 0);
     if (rows_ok) {
       GemmEpilogue E;
       std::memset(&E, 0, sizeof(E));
@@ -877,8 +1207,7 @@ int32_t optk_gemm_bf16(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, i
       hipError_t e = hipGetLastError();
       if (e == hipSuccess && split_k > 1) {
         if (ldc != n) return fail(OPTK_EINVAL, "gemm: split_k > 1 needs ldc == n");
-        hipLaunchKernelGGL(gemm_reduce, dim3(blocks_for(m * n)), dim3(256), 0, stream, workspace,
-                           split_k, m * n, c);
+        launch_reduce(workspace, split_k, m * n, c, stream);
         e = hipGetLastError();
       }
       if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("gemm launch: ") + hipGetErrorString(e));
@@ -897,8 +1226,7 @@ int32_t optk_gemm_bf16(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, i
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && split_k > 1) {
     if (ldc != n) return fail(OPTK_EINVAL, "gemm: split_k > 1 needs ldc == n");
-    hipLaunchKernelGGL(gemm_reduce, dim3(blocks_for(m * n)), dim3(256), 0, stream, workspace,
-                       split_k, m * n, c);
+    launch_reduce(workspace, split_k, m * n, c, stream);
     e = hipGetLastError();
   }
   if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("gemm launch: ") + hipGetErrorString(e));
@@ -909,7 +1237,8 @@ int32_t optk_gemm_bf16_fused(int32_t trans_a, int32_t trans_b, int64_t m, int64_
                              const void *a, int64_t lda, const void *b, int64_t ldb, float *c,
                              int64_t ldc, int32_t epilogue, const float *bias, const float *pre,
                              void *out16, void *hip_stream) {
-  if (!a || !b || !c || !out16) return fail(OPTK_EINVAL, "gemm_fused: null operand");
+  if (!a || !b || !out16) return fail(OPTK_EINVAL, "gemm_fused: null operand");
+  if (!c && epilogue != OPTK_GEMM_SILU_BACK) return fail(OPTK_EINVAL, "gemm_fused: c missing");
   if (epilogue != OPTK_GEMM_BIAS_SILU && epilogue != OPTK_GEMM_SILU_BACK)
     return fail(OPTK_EINVAL, "gemm_fused: unknown epilogue");
   if (epilogue == OPTK_GEMM_BIAS_SILU && !bias) return fail(OPTK_EINVAL, "gemm_fused: bias missing");
@@ -920,6 +1249,8 @@ int32_t optk_gemm_bf16_fused(int32_t trans_a, int32_t trans_b, int64_t m, int64_
     return fail(OPTK_EINVAL, "gemm_fused: leading dimension too small");
   if (lda % 8 || ldb % 8 || (uintptr_t)a % 16 || (uintptr_t)b % 16)
     return fail(OPTK_EINVAL, "gemm_fused: operands need 16-byte aligned rows (leading dimensions multiples of 8)");
+  if (ldc % 4 || (uintptr_t)c % 16 || (uintptr_t)out16 % 8 || (uintptr_t)bias % 16 || (uintptr_t)pre % 16)
+    return fail(OPTK_EINVAL, "gemm_fused: c, pre and bias need 16-byte, out16 8-byte alignment and ldc a multiple of 4");
   GemmEpilogue E;
   E.c = c;
   E.ldc = ldc;
@@ -970,13 +1301,24 @@ int32_t optk_mlp_op(int32_t op, const optk_mlp_args *A, void *hip_stream) {
     break;
   case OPTK_MLP_DENSE_IN:
     NEED(A->f32[0] && A->f32[1] && A->f32[2] && A->u16[0], "dense_in: x, w, b, act");
-    hipLaunchKernelGGL(dense_in, dim3(blocks_for(count)), dim3(256), 0, s, A->f32[0], A->inner,
-                       A->f32[1], A->f32[2], A->f32[3], A->u16[0], rows, A->cols);
+    if (A->inner == 5 && A->cols % 4 == 0 && (uintptr_t)A->f32[3] % 16 == 0 && (uintptr_t)A->u16[0] % 8 == 0)
+      hipLaunchKernelGGL(dense_in4<5>, dim3(blocks_for(count / 4)), dim3(256), 0, s, A->f32[0], A->f32[1],
+                         A->f32[2], A->f32[3], A->u16[0], rows, A->cols);
+    else
+      hipLaunchKernelGGL(dense_in, dim3(blocks_for(count)), dim3(256), 0, s, A->f32[0], A->inner,
+                         A->f32[1], A->f32[2], A->f32[3], A->u16[0], rows, A->cols);
     break;
   case OPTK_MLP_DENSE_OUT:
     NEED(A->u16[0] && A->f32[0] && A->f32[1] && A->f32[2], "dense_out: act, w, b, y");
-    hipLaunchKernelGGL(dense_out, dim3(blocks_for(rows)), dim3(256), 0, s, A->u16[0], A->cols,
-                       A->f32[0], A->f32[1], A->f32[2], A->inner, rows);
+    if (A->cols == 256 && A->inner == 3 && (uintptr_t)A->u16[0] % 8 == 0) {
+      // four rows per wave
+      const int64_t nb = std::min<int64_t>((rows + 15) / 16, 8192);
+      hipLaunchKernelGGL(dense_out_wave<3>, dim3((unsigned)nb), dim3(256), 0, s, A->u16[0], A->f32[0],
+                         A->f32[1], A->f32[2], rows);
+    } else {
+      hipLaunchKernelGGL(dense_out, dim3(blocks_for(rows)), dim3(256), 0, s, A->u16[0], A->cols,
+                         A->f32[0], A->f32[1], A->f32[2], A->inner, rows);
+    }
     break;
   case OPTK_MLP_MSE_GRAD:
     NEED(A->f32[0] && A->f32[1] && A->f32[2] && A->f32[3] && A->f32[4], "mse: y, t, w, dy, loss");
@@ -985,8 +1327,13 @@ int32_t optk_mlp_op(int32_t op, const optk_mlp_args *A, void *hip_stream) {
     break;
   case OPTK_MLP_DENSE_OUT_BACK:
     NEED(A->f32[0] && A->f32[1] && A->f32[2] && A->u16[0] && A->f32[3], "dense_out_back: dy, w, pre, dpre, dpre32");
-    hipLaunchKernelGGL(dense_out_back, dim3(blocks_for(count)), dim3(256), 0, s, A->f32[0], A->inner,
-                       A->f32[1], A->f32[2], A->u16[0], A->f32[3], rows, A->cols);
+    if (A->inner == 3 && A->cols % 4 == 0 && (uintptr_t)A->f32[2] % 16 == 0 && (uintptr_t)A->f32[3] % 16 == 0 &&
+        (uintptr_t)A->u16[0] % 8 == 0)
+      hipLaunchKernelGGL(dense_out_back4<3>, dim3(blocks_for(count / 4)), dim3(256), 0, s, A->f32[0],
+                         A->f32[1], A->f32[2], A->u16[0], A->f32[3], rows, A->cols);
+    else
+      hipLaunchKernelGGL(dense_out_back, dim3(blocks_for(count)), dim3(256), 0, s, A->f32[0], A->inner,
+                         A->f32[1], A->f32[2], A->u16[0], A->f32[3], rows, A->cols);
     break;
   case OPTK_MLP_SILU_BACK:
     NEED(A->f32[0] && A->f32[1] && A->u16[0] && A->f32[2], "silu_back: dact, pre, dpre, dpre32");
@@ -998,18 +1345,36 @@ int32_t optk_mlp_op(int32_t op, const optk_mlp_args *A, void *hip_stream) {
     // f32[2]: out (cols * inner); f32[3]: workspace (blocks * cols * inner)
     NEED((A->f32[0] || A->u16[0]) && A->f32[1] && A->f32[2] && A->f32[3], "outer_grad: x, g, out, workspace");
     NEED(A->cols <= 256 && A->inner >= 1 && A->inner <= 8, "outer_grad: cols <= 256, 1 <= inner <= 8");
-    // 32 rows per block: enough blocks (2048 at 65536 rows) to keep every CU
-    // busy; each thread's loads of its column are independent
-    const int64_t rpb = 32;
+    // about 512 blocks (two per CU), at least 32 rows each: the partials
+    // stay few enough for one short reduction
+    const int64_t rpb = std::max<int64_t>(32, (rows + 511) / 512);
     const int64_t nblk = (rows + rpb - 1) / rpb;
     NEED(A->count >= nblk * A->cols * A->inner, "outer_grad: workspace too small");
-    hipLaunchKernelGGL(outer_grad, dim3((unsigned)nblk), dim3(256), 0, s, A->f32[0], A->u16[0],
-                       A->cols, A->f32[1], A->inner, rows, rpb, A->flag, A->f32[3]);
+    const bool x16 = A->f32[0] == nullptr;
+    const bool vec = A->cols % 4 == 0 &&
+                     (x16 ? (uintptr_t)A->u16[0] % 8 == 0 : (uintptr_t)A->f32[0] % 16 == 0);
+#define OPTK_OUTER4(IN)                                                                               \
+  if (x16)                                                                                         \
+    hipLaunchKernelGGL((outer_grad4<IN, true>), dim3((unsigned)nblk), dim3(256), 0, s, nullptr, A->u16[0], \
+                       A->cols, A->f32[1], rows, rpb, A->flag, A->f32[3]);                        \
+  else                                                                                             \
+    hipLaunchKernelGGL((outer_grad4<IN, false>), dim3((unsigned)nblk), dim3(256), 0, s, A->f32[0], nullptr, \
+                       A->cols, A->f32[1], rows, rpb, A->flag, A->f32[3]);
+    if (vec && A->inner == 1) {
+      OPTK_OUTER4(1)
+    } else if (vec && A->inner == 3) {
+      OPTK_OUTER4(3)
+    } else if (vec && A->inner == 5) {
+      OPTK_OUTER4(5)
+    } else {
+      hipLaunchKernelGGL(outer_grad, dim3((unsigned)nblk), dim3(256), 0, s, A->f32[0], A->u16[0],
+                         A->cols, A->f32[1], A->inner, rows, rpb, A->flag, A->f32[3]);
+    }
+#undef OPTK_OUTER4
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("outer_grad: ") + hipGetErrorString(e));
     const int64_t cnt = (int64_t)A->cols * A->inner;
-    hipLaunchKernelGGL(gemm_reduce, dim3(blocks_for(cnt)), dim3(256), 0, s, A->f32[3], (int)nblk,
-                       cnt, A->f32[2]);
+    launch_reduce(A->f32[3], (int)nblk, cnt, A->f32[2], s);
     break;
   }
   case OPTK_MLP_COLUMN_SUM:
@@ -1055,7 +1420,7 @@ extern "C" int32_t optk_surrogate_infer(const optk_surrogate_desc *desc, const o
     return fail(OPTK_EINVAL, "missing ray array");
   if (IO.fallback_list && (!IO.fallback_count || IO.fallback_capacity < 0))
     return fail(OPTK_EINVAL, "fallback list needs a counter and a capacity");
-  if (!D.w1_16 || !D.b1 || !D.w5 || !D.b5 || !D.w5_16) return fail(OPTK_EINVAL, "missing weights");
+  if (!D.w1_16 || !D.b1 || !D.w5 || !D.b5 || !D.w5_16 || !D.poly) return fail(OPTK_EINVAL, "missing weights");
   if ((uintptr_t)D.w1_16 % 16) return fail(OPTK_EINVAL, "bf16 weights must be 16-byte aligned");
   if ((uintptr_t)D.w5_16 % 16) return fail(OPTK_EINVAL, "bf16 weights must be 16-byte aligned");
   for (int l = 0; l < 3; ++l) {
@@ -1075,7 +1440,21 @@ extern "C" int32_t optk_surrogate_infer(const optk_surrogate_desc *desc, const o
   const int64_t tiles = (IO.n + INF_RAYS - 1) / INF_RAYS;
   int64_t blocks = cus > 0 ? (int64_t)cus * INF_BLOCKS_PER_CU : 1024;  // all resident
   if (blocks > tiles) blocks = tiles;
-  hipLaunchKernelGGL(surrogate_infer, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)hip_stream, D, IO);
+  // the caller's row of every monomial in the kernel's order
+  uint32_t codes[OPTK_SURROGATE_POLY];
+  kernel_poly_codes(codes);
+  PolyOrder P;
+  for (int t = 0; t < OPTK_SURROGATE_POLY; ++t) {
+    int row = -1;
+    for (int r = 0; r < OPTK_SURROGATE_POLY; ++r)
+      if (D.poly_exp[r] == codes[t]) {
+        if (row >= 0) return fail(OPTK_EINVAL, "poly_exp lists a monomial twice");
+        row = r;
+      }
+    if (row < 0) return fail(OPTK_EINVAL, "poly_exp must list every monomial of degree <= 3 in the five inputs");
+    P.row[t] = (uint8_t)row;
+  }
+  hipLaunchKernelGGL(surrogate_infer, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)hip_stream, D, IO, P);
   e = hipGetLastError();
   if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("surrogate launch: ") + hipGetErrorString(e));
   return OPTK_OK;

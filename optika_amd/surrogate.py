@@ -161,7 +161,9 @@ class MLP:
                 while rows % (32 * split):
                     split //= 2
                 self.split = split
-                self.ws = f(max(split * H * H, ((rows + 255) // 256) * H * 8))
+                # also the per-block partials of the small outer products
+                # (at most min(512, rows / 32) blocks of H x 8)
+                self.ws = f(max(split * H * H, min(512, (rows + 31) // 32) * H * 8))
                 self.loss_part = f((rows * 3 + 255) // 256)
             self._buf_key = key
 
@@ -202,7 +204,8 @@ class MLP:
         # output layer
         _op(_lib.MLP_OUTER_GRAD, f32=[None, self.dy, self.view(g, 4, 0), self.ws],
             u16=[self.act[3]], rows=rows, cols=H, inner=3, count=self.ws.numel(), flag=0)
-        _op(_lib.MLP_COLUMN_SUM, f32=[self.dy, self.view(g, 4, 1)], rows=rows, cols=3)
+        _op(_lib.MLP_OUTER_GRAD, f32=[self.dy, self.ones, self.view(g, 4, 1), self.ws],
+            rows=rows, cols=3, inner=1, count=self.ws.numel(), flag=1)
         _op(_lib.MLP_DENSE_OUT_BACK, f32=[self.dy, self.view(self.p, 4, 0), self.pre[3],
                                           self.dpre32], u16=[self.dpre16], rows=rows, cols=H,
             inner=3)
