@@ -785,85 +785,140 @@ __global__ __launch_bounds__(256) void poly_eval_kernel(const double *__restrict
 // aperture-edge distances, the error-map lookup and the compaction of the
 // rays that fall back to the trace -- one kernel, activations in LDS.
 //
-// Block: 256 threads (4 waves), a persistent loop over tiles of 128 rays;
-// two blocks share a CU, so one block's MFMA phase overlaps the other's
-// VALU phase (SiLU, the polynomial).  Every layer runs on MFMA 16x16x32
-// bf16 with float32 accumulation.  Layer 1 (5 -> 256): the normalised
-// inputs and the weights are split into bf16 high and low parts, k row
-// [hi | lo | hi] against weight row [w_hi | w_hi | w_lo], so the one
-// 32-wide k step carries the float32 product to ~2^-16.  Hidden layers: a
-// 128 x 256 x 256 product each.  Wave w computes all 128 rays x features
-// [64 w, 64 w + 64) as 8 x 4 tiles, A operands (activations
-// act[ray][feature], bf16) from LDS, B operands (weights, row-major
-// [out][in], k contiguous) straight from global memory (L2-resident).  The
-// B operand of n-tile t and lane column c is weight row 64 w + 4 c + t, so
-// each lane's four accumulator tiles hold four consecutive features: the
-// epilogue (bias, SiLU, bf16) writes them as one 8-byte store per row.
-// Weights stream through two register buffers, each chunk (32 k values)
-// requested one chunk ahead of its MFMAs -- across layer boundaries too --
-// so the L2 latency hides behind 32 MFMAs (and the other wave's on the SIMD).  The output
-// layer (3 rows, high and low bf16 parts) is one more MFMA tile per 16
-// rays.  The polynomial runs on all four waves (waves 0-1 and 2-3 each
-// take half of the monomials of the same 128 rays); the edge tests, the
-// error map and the stores then run one thread per ray.
+// Block: 256 threads (4 waves), one per CU, a persistent loop over pairs
+// of 128-ray tiles (A, B), each with its own LDS activation buffer.  The
+// two tiles' layers are staggered by half a layer: while a wave issues
+// tile A's MFMAs of layer L it applies bias + SiLU to tile B's layer L - 1
+// accumulators (row tile c during k chunk c), and the other way round in
+// the next stage, so each wave keeps its matrix core and its VALU busy at
+// once; one barrier separates the stages.  The two accumulator sets take
+// 256 registers, which the one wave per SIMD leaves room for.
+//
+// Every layer runs on MFMA 16x16x32 bf16 with float32 accumulation.
+// Layer 1 (5 -> 256): the normalised inputs and the weights are split into
+// bf16 high and low parts, k row [hi | lo | hi] against weight row
+// [w_hi | w_hi | w_lo], so the one 32-wide k step carries the float32
+// product to ~2^-16.  Hidden layers: a 128 x 256 x 256 product per tile.
+// Wave w computes all 128 rays x features [64 w, 64 w + 64) as 8 x 4
+// tiles, A operands (activations act[ray][feature], bf16) from LDS, B
+// operands (weights, row-major [out][in], k contiguous) straight from
+// global memory (L2-resident), streamed through three register buffers
+// two 32-wide k chunks ahead of their MFMAs (across stages too).  The B operand
+// of n-tile t and lane column c is weight row 64 w + 4 c + t, so each
+// lane's four accumulator tiles hold four consecutive features: the SiLU
+// writes them as one 8-byte store per row.  The output layer (3 rows, high
+// and low bf16 parts) is one more MFMA tile per 16 rays.  The polynomial,
+// the edge tests, the error map and the stores run one thread per ray.
 // ---------------------------------------------------------------------------
 
-constexpr int INF_RAYS = 128;          // rays per tile
+constexpr int INF_RAYS = 128;          // rays per tile; a block holds two tiles
 constexpr int INF_MT = INF_RAYS / 16;  // 16-ray MFMA row tiles per tile
 constexpr int INF_OQ = INF_MT / 4;     // output-layer row tiles per wave
-constexpr int INF_BLOCKS_PER_CU = 2;
+constexpr int INF_BLOCKS_PER_CU = 1;
 constexpr int INF_HIDDEN = 256;
 constexpr int INF_PAD = 8;
 constexpr int INF_TARGETS = OPTK_SURROGATE_MAX_TARGETS;
 
 // The kernel's order of the 56 cubic monomials: depth first over
-// nondecreasing variable indices (1, u0, u0^2, u0^3, u0^2 u1, ..., u4^3),
-// so that each monomial is its parent times one input.  A subtree (the
-// monomials whose lowest variable is a) starts at POLY_SUBTREE[a].  The
-// launch maps every kernel-order monomial to its row of the caller's poly.
+// nondecreasing variable indices (1, u0, u0^2, u0^3, u0^2 u1, ..., u4^3).
+// Monomial t is u[a] u[b] u[c] with u[5] = 1; make_poly_factors().f[t]
+// packs 4 a, 4 b and 4 c in its low three bytes.  The launch maps every
+// kernel-order monomial to its row of the caller's poly.
 struct PolyOrder {
   uint8_t row[OPTK_SURROGATE_POLY];
 };
-constexpr int POLY_SUBTREE[5] = {1, 22, 37, 47, 53};
+struct PolyFactors {
+  uint32_t f[OPTK_SURROGATE_POLY];
+};
 
-static void kernel_poly_codes(uint32_t (&codes)[OPTK_SURROGATE_POLY]) {
+constexpr uint32_t poly_factor(int a, int b, int c) {
+  return (uint32_t)(4 * a) | ((uint32_t)(4 * b) << 8) | ((uint32_t)(4 * c) << 16);
+}
+
+constexpr PolyFactors make_poly_factors() {
+  PolyFactors p{};
   int t = 0;
-  codes[t++] = 0;
+  p.f[t++] = poly_factor(5, 5, 5);
   for (int a = 0; a < 5; ++a) {
-    const uint32_t ea = 1u << (2 * a);
-    codes[t++] = ea;
+    p.f[t++] = poly_factor(a, 5, 5);
     for (int b = a; b < 5; ++b) {
-      const uint32_t eb = ea + (1u << (2 * b));
-      codes[t++] = eb;
-      for (int c = b; c < 5; ++c) codes[t++] = eb + (1u << (2 * c));
+      p.f[t++] = poly_factor(a, b, 5);
+      for (int c = b; c < 5; ++c) p.f[t++] = poly_factor(a, b, c);
     }
   }
+  return p;
 }
 
-__device__ __forceinline__ void poly_term(float m, const float *__restrict__ c,
-                                          float (&tgt)[INF_TARGETS]) {
-#pragma unroll
-  for (int j = 0; j < INF_TARGETS; ++j) tgt[j] = fmaf(m, c[j], tgt[j]);
-}
-
-// the monomials whose lowest variable is A (kernel order from S); coef:
-// the coefficients in kernel order
-template <int A, int S>
-__device__ __forceinline__ void poly_subtree(const float (&u)[5], const float (*coef)[INF_TARGETS],
-                                             float (&tgt)[INF_TARGETS]) {
-  int t = S;
-  const float m1 = u[A];
-  poly_term(m1, coef[t++], tgt);
-#pragma unroll
-  for (int b = A; b < 5; ++b) {
-    const float m2 = m1 * u[b];
-    poly_term(m2, coef[t++], tgt);
-#pragma unroll
-    for (int c = b; c < 5; ++c) poly_term(m2 * u[c], coef[t++], tgt);
+// exponent codes (2 bits per input) of the kernel-order monomials
+static void kernel_poly_codes(uint32_t (&codes)[OPTK_SURROGATE_POLY]) {
+  constexpr PolyFactors p = make_poly_factors();
+  for (int t = 0; t < OPTK_SURROGATE_POLY; ++t) {
+    uint32_t code = 0;
+    for (int k = 0; k < 3; ++k) {
+      const int v = (int)((p.f[t] >> (8 * k)) & 255u) / 4;
+      if (v < 5) code += 1u << (2 * v);
+    }
+    codes[t] = code;
   }
+}
+
+// The polynomial of a tile pair runs on the float32 matrix cores
+// (v_mfma_f32_16x16x4_f32): each thread writes its ray's 56 monomials to
+// LDS (rows of POLY_MONO_STRIDE floats, bank-conflict free for the A
+// operand reads), each wave multiplies its 64 rays' rows by the [56][16]
+// coefficient matrix (targets padded to 16 columns with zeros), and the
+// products go back to LDS, a row of 16 per ray.  A wave only reads the
+// rows of its own rays.
+constexpr int INF_COEF_STRIDE = 16;   // coefficient row: the targets, zero padded
+constexpr int POLY_MONO_STRIDE = 60;  // floats per monomial row (56 used)
+
+typedef float PolyRow[POLY_MONO_STRIDE];
+
+// the ray's monomials in kernel order into its LDS row
+__device__ __forceinline__ void poly_monomials(const float (&u)[5], float *row) {
+  constexpr PolyFactors PF = make_poly_factors();
+  const float U[6] = {u[0], u[1], u[2], u[3], u[4], 1.0f};
+  float m[OPTK_SURROGATE_POLY];
+#pragma unroll
+  for (int t = 0; t < OPTK_SURROGATE_POLY; ++t) {
+    const uint32_t f = PF.f[t];
+    m[t] = (U[(f & 255u) / 4] * U[((f >> 8) & 255u) / 4]) * U[(f >> 16) / 4];
+  }
+#pragma unroll
+  for (int q = 0; q < OPTK_SURROGATE_POLY / 4; ++q)
+    *reinterpret_cast<float4 *>(row + 4 * q) = float4{m[4 * q], m[4 * q + 1], m[4 * q + 2], m[4 * q + 3]};
+}
+
+// targets of rays [row0, row0 + 64) (four 16-row tiles): out[ray][j] =
+// sum_t mono[ray][t] coef[t][j]
+__device__ __forceinline__ void poly_mfma(const PolyRow *mono, int row0, const float (*coef)[INF_COEF_STRIDE],
+                                          float (*out)[INF_COEF_STRIDE], int lane) {
+  const int fr = lane & 15, fk = lane >> 4;
+  floatx4 acc[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) acc[mt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int ks = 0; ks < OPTK_SURROGATE_POLY / 4; ++ks) {
+    const float b = coef[4 * ks + fk][fr];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(mono[row0 + 16 * mt + fr][4 * ks + fk], b, acc[mt], 0, 0, 0);
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[row0 + 16 * mt + 4 * fk + r][fr] = acc[mt][r];
 }
 
 typedef uint16_t ActRow[INF_HIDDEN + INF_PAD];
+typedef floatx4 Acc[INF_MT][4];
+
+// two floats -> two bfloat16 (round to nearest even) in one word, a low
+__device__ __forceinline__ uint32_t pack2_bf16(float a, float b) {
+  typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+  typedef float float2_t __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((float2_t{a, b}), bf16x2_t));
+}
 
 // this lane's B operands of one 32-wide k chunk: rows col .. col + 3 of W
 // ([out][K] bf16), k values k0 + fk .. k0 + fk + 7.  voff = (col K + fk) * 2
@@ -876,85 +931,128 @@ __device__ __forceinline__ void wload(uint4 (&w)[4], const uint16_t *__restrict_
   for (int nt = 0; nt < 4; ++nt) w[nt] = *reinterpret_cast<const uint4 *>(base + voff + 2 * K * nt);
 }
 
-// acc (+)= act[:, k0 .. k0 + 32) x the chunk's weights (first: acc = product).
-// The A operand of row tile mt + 1 is read while tile mt's four MFMAs run;
-// the scheduling barriers keep the compiler from reading all eight ahead
-// (32 more live registers than the two blocks per CU leave room for)
+// acc (+)= act[:, k0 .. k0 + 32) x the chunk's weights (FIRST: acc = c0 +
+// product, c0[nt] the scaled bias of the lane's column in n-tile nt)
 template <bool FIRST>
-__device__ __forceinline__ void mfma_chunk(floatx4 (&acc)[INF_MT][4], const uint4 (&w)[4],
-                                           const ActRow *act, int k0, int fr, int fk) {
+__device__ __forceinline__ void mfma_chunk(Acc &acc, const uint4 (&w)[4], const ActRow *act, int k0,
+                                           int fr, int fk, const floatx4 (&c0)[4]) {
   bf16x8 b[4];
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) b[nt] = __builtin_bit_cast(bf16x8, w[nt]);
-  uint4 a_cur = *reinterpret_cast<const uint4 *>(&act[fr][k0 + fk]);
 #pragma unroll
   for (int mt = 0; mt < INF_MT; ++mt) {
-    uint4 a_next = a_cur;
-    if (mt + 1 < INF_MT) a_next = *reinterpret_cast<const uint4 *>(&act[16 * (mt + 1) + fr][k0 + fk]);
-    __builtin_amdgcn_sched_barrier(0);
-    const bf16x8 a = __builtin_bit_cast(bf16x8, a_cur);
+    const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&act[16 * mt + fr][k0 + fk]));
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
-      acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-          a, b[nt], FIRST ? floatx4{0.0f, 0.0f, 0.0f, 0.0f} : acc[mt][nt], 0, 0, 0);
-    a_cur = a_next;
+      acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], FIRST ? c0[nt] : acc[mt][nt], 0, 0, 0);
   }
 }
 
-// one hidden layer (K = 256): chunk c's weights sit in ring slot c % 2;
-// chunk c requests chunk c + 1 into the other slot before its MFMAs (past
-// the end: the next layer's first chunk, into slot 0, where that layer
-// starts)
-template <bool NEXT>
-__device__ __forceinline__ void hidden_layer(floatx4 (&acc)[INF_MT][4], uint4 (&ring)[2][4],
-                                             const uint16_t *__restrict__ W,
-                                             const uint16_t *__restrict__ Wn, const ActRow *act,
-                                             uint32_t voff, int fr, int fk) {
+// The kernel carries every pre-activation and activation scaled by
+// log2(e): v' = log2(e) v, s = log2(e) silu(v) = v' / (1 + 2^-v').  The
+// scale enters through the inputs (u' = log2(e) u) and the biases, passes
+// through the hidden layers unchanged (their inputs carry it), and leaves
+// at the output layer (times ln 2); the SiLU then needs no multiply before
+// its exponential and no bias add (the bias starts the accumulator).
+constexpr float INF_LOG2E = 1.44269504088896341f, INF_LN2 = 0.693147180559945309f;
+
+__device__ __forceinline__ float silu_scaled(float v) {
+  return v * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-v));
+}
+
+// act[row][col .. col + 3] = bf16(silu_scaled(acc)) for the lane's rows of
+// row tile mt
+__device__ __forceinline__ void silu_rows(const Acc &acc, ActRow *act, int col, int lane, int mt) {
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    if (c + 1 < 8)
-      wload<INF_HIDDEN>(ring[(c + 1) % 2], W, voff, 32 * (c + 1));
-    else if (NEXT)
-      wload<INF_HIDDEN>(ring[0], Wn, voff, 0);
-    __builtin_amdgcn_sched_barrier(0);
-    if (c == 0)
-      mfma_chunk<true>(acc, ring[c % 2], act, 32 * c, fr, fk);
-    else
-      mfma_chunk<false>(acc, ring[c % 2], act, 32 * c, fr, fk);
+  for (int q = 0; q < 4; ++q) {
+    const int row = 16 * mt + 4 * (lane >> 4) + q;
+    uint2 v;
+    v.x = pack2_bf16(silu_scaled(acc[mt][0][q]), silu_scaled(acc[mt][1][q]));
+    v.y = pack2_bf16(silu_scaled(acc[mt][2][q]), silu_scaled(acc[mt][3][q]));
+    *reinterpret_cast<uint2 *>(&act[row][col]) = v;
   }
 }
 
-// act[row][col .. col + 3] = bf16(silu(acc + bias)) for the lane's rows
-__device__ __forceinline__ void silu_store(const floatx4 (&acc)[INF_MT][4], const float (&bias)[4],
-                                           ActRow *act, int col, int lane) {
-#pragma unroll
-  for (int mt = 0; mt < INF_MT; ++mt)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int row = 16 * mt + 4 * (lane >> 4) + q;
-      uint2 v;
-      v.x = pack_bf16(silu(acc[mt][0][q] + bias[0]), silu(acc[mt][1][q] + bias[1]));
-      v.y = pack_bf16(silu(acc[mt][2][q] + bias[2]), silu(acc[mt][3][q] + bias[3]));
-      *reinterpret_cast<uint2 *>(&act[row][col]) = v;
-    }
-}
-
-__device__ __forceinline__ void load_bias(float (&b)[4], const float *__restrict__ bias, uint32_t col) {
+// the accumulator start of a layer: log2(e) times the bias of the lane's
+// four columns, one per n-tile
+__device__ __forceinline__ void bias_init(floatx4 (&c0)[4], const float *__restrict__ bias, uint32_t col) {
   const char *base = reinterpret_cast<const char *>(bias);
 #pragma unroll
-  for (int nt = 0; nt < 4; ++nt) b[nt] = *reinterpret_cast<const float *>(base + 4 * col + 4 * nt);
+  for (int nt = 0; nt < 4; ++nt) {
+    const float b = INF_LOG2E * *reinterpret_cast<const float *>(base + 4 * col + 4 * nt);
+    c0[nt] = floatx4{b, b, b, b};
+  }
+}
+
+// one stage: accx = c0 + actx . W (K = 256, eight 32-wide k chunks), and,
+// while the MFMAs run, row tile c of accy through the SiLU into acty.
+// Weights stream through a ring of three register buffers: chunk c sits in
+// slot (S + c) % 3 and requests chunk c + 2 before its MFMAs; past the end
+// the requests fetch the next stage's first two chunks from Wn, which
+// therefore starts at slot (S + 2) % 3.  Wn must be a valid weight matrix
+// (the last stage passes its own: those requests go unused).
+template <int S>
+__device__ __forceinline__ void stage(Acc &accx, uint4 (&ring)[3][4], const uint16_t *__restrict__ W,
+                                      const uint16_t *__restrict__ Wn, const ActRow *actx, uint32_t voff,
+                                      const floatx4 (&c0)[4], const Acc &accy, ActRow *acty, int col,
+                                      int lane, int fr, int fk) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    if (c + 2 < 8)
+      wload<INF_HIDDEN>(ring[(S + c + 2) % 3], W, voff, 32 * (c + 2));
+    else
+      wload<INF_HIDDEN>(ring[(S + c + 2) % 3], Wn, voff, 32 * (c + 2 - 8));
+    if (c == 0)
+      mfma_chunk<true>(accx, ring[(S + c) % 3], actx, 32 * c, fr, fk, c0);
+    else
+      mfma_chunk<false>(accx, ring[(S + c) % 3], actx, 32 * c, fr, fk, c0);
+    silu_rows(accy, acty, col, lane, c);
+    // issue order: each MFMA followed by four VALU instructions of the
+    // SiLU, which execute while the matrix core works on it
+#pragma unroll
+    for (int g = 0; g < 32; ++g) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x402, 4, 0);  // VALU, transcendental
+    }
+    // one scheduling region per chunk (the pattern search grows fast with
+    // the region)
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// the output layer of one tile: wave w -> rays [16 INF_OQ w, +16 INF_OQ).
+// The three outputs' weights are split into bf16 high and low parts (rows
+// 0..2 and 3..5 of one 16-row tile, the rest zero), so the products carry
+// the float32 weights to ~2^-17 at no extra MFMA
+__device__ __forceinline__ void output_layer(const uint4 (&w5)[INF_HIDDEN / 32], const ActRow *act,
+                                             float (*yout)[8], int wave, int lane, int fr, int fk) {
+  floatx4 oacc[INF_OQ];
+#pragma unroll
+  for (int q = 0; q < INF_OQ; ++q) oacc[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int kc = 0; kc < INF_HIDDEN / 32; ++kc) {
+    const bf16x8 b5 = __builtin_bit_cast(bf16x8, w5[kc]);
+#pragma unroll
+    for (int q = 0; q < INF_OQ; ++q) {
+      const uint4 va = *reinterpret_cast<const uint4 *>(&act[16 * (INF_OQ * wave + q) + fr][32 * kc + fk]);
+      oacc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, va), b5, oacc[q], 0, 0, 0);
+    }
+  }
+  if (fr < 6) {
+#pragma unroll
+    for (int q = 0; q < INF_OQ; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) yout[16 * (INF_OQ * wave + q) + 4 * (lane >> 4) + r][fr] = oacc[q][r];
+  }
 }
 
 __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const optk_surrogate_desc D,
                                                                           const optk_surrogate_io IO,
                                                                           const PolyOrder P) {
-  __shared__ __attribute__((aligned(16))) ActRow act[INF_RAYS];
-  __shared__ float yout[INF_RAYS][8];
-  __shared__ float us[INF_RAYS][5];
-  __shared__ float coef[OPTK_SURROGATE_POLY][INF_TARGETS];  // kernel order
-  // after the output layer: the two halves' polynomial sums, in act's memory
-  auto part = reinterpret_cast<float(*)[INF_RAYS][INF_TARGETS]>(&act[0][0]);
-  static_assert(2 * INF_RAYS * INF_TARGETS * sizeof(float) <= sizeof(act), "partial sums fit act");
+  __shared__ __attribute__((aligned(16))) ActRow act[2][INF_RAYS];
+  __shared__ float yout[2][INF_RAYS][8];
+  __shared__ float us[2][INF_RAYS][5];  // normalised inputs
+  __shared__ __attribute__((aligned(16))) float coef[OPTK_SURROGATE_POLY][INF_COEF_STRIDE];  // kernel order
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int fr = lane & 15, fk = (lane >> 4) * 8;
@@ -962,138 +1060,116 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
   const uint32_t voff_h = 2u * (uint32_t)(col * INF_HIDDEN + fk), voff1_h = 2u * (uint32_t)(col * 32 + fk);
   const uint32_t voff5_h = 2u * (uint32_t)(fr * INF_HIDDEN + fk), bcol_h = (uint32_t)col;
   const int64_t n = IO.n;
-  const int64_t tiles = (n + INF_RAYS - 1) / INF_RAYS;
+  const int64_t pairs = (n + 2 * INF_RAYS - 1) / (2 * INF_RAYS);
   const float *in[5] = {IO.field_x, IO.field_y, IO.pupil_x, IO.pupil_y, IO.wavelength};
-  for (int idx = tid; idx < OPTK_SURROGATE_POLY * INF_TARGETS; idx += blockDim.x)
-    coef[idx / INF_TARGETS][idx % INF_TARGETS] = D.poly[P.row[idx / INF_TARGETS] * INF_TARGETS + idx % INF_TARGETS];
-  for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    const int64_t base = tile * INF_RAYS;
-    // opaque per tile: the weight and bias addresses are formed here, not
+  for (int idx = tid; idx < OPTK_SURROGATE_POLY * INF_COEF_STRIDE; idx += blockDim.x) {
+    const int t = idx / INF_COEF_STRIDE, j = idx % INF_COEF_STRIDE;
+    coef[t][j] = j < INF_TARGETS ? D.poly[P.row[t] * INF_TARGETS + j] : 0.0f;
+  }
+  for (int64_t pair = blockIdx.x; pair < pairs; pair += gridDim.x) {
+    // rays [base, base + 128) form tile A, [base + 128, base + 256) tile B;
+    // thread t stages, and at the end finishes, ray base + t
+    const int64_t base = pair * 2 * INF_RAYS;
+    const int X = tid >> 7, r = tid & (INF_RAYS - 1);
+    const int64_t i = base + tid;
+    // opaque per pair: the weight and bias addresses are formed here, not
     // hoisted out of the loop as dozens of live 64-bit pointers
     uint32_t voff = voff_h, voff1 = voff1_h, voff5 = voff5_h, bcol = bcol_h;
     asm volatile("" : "+v"(voff), "+v"(voff1), "+v"(voff5), "+v"(bcol));
-    // weights of layer 1 and the first chunk of layer 2, requested before
-    // the inputs are staged
-    uint4 w1[4], ring[2][4];
+    uint4 w1[4], ring[3][4];
     wload<32>(w1, D.w1_16, voff1, 0);
     wload<INF_HIDDEN>(ring[0], D.w16[0], voff, 0);
-    float bias[4];
-    load_bias(bias, D.b1, bcol);
-    // ---- inputs: thread t < 128 -> ray t, normalised and split into bf16
-    // high and low parts, as the 32-wide k row [hi | lo | hi | 0] that meets
-    // the first layer's weight rows [w_hi | w_hi | w_lo | 0] ---------------
-    if (tid < INF_RAYS) {
-      const int64_t i = base + tid;
+    wload<INF_HIDDEN>(ring[1], D.w16[0], voff, 32);
+    floatx4 c0[4];
+    bias_init(c0, D.b1, bcol);
+    // ---- inputs, normalised and split into bf16 high and low parts, as
+    // the 32-wide k row [hi | lo | hi | 0] that meets the first layer's
+    // weight rows [w_hi | w_hi | w_lo | 0] ---------------------------------
+    {
       float hi[5], lo[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
         const float v = i < n ? in[k][i] : 0.0f;
         const float u = (v - D.in_mid[k]) * D.in_inv_half[k];
-        us[tid][k] = u;
-        hi[k] = from_bf16(to_bf16(u));
-        lo[k] = u - hi[k];
+        us[X][r][k] = u;
+        const float us2 = INF_LOG2E * u;
+        hi[k] = from_bf16(to_bf16(us2));
+        lo[k] = us2 - hi[k];
       }
       uint4 row0, row1;
-      row0.x = pack_bf16(hi[0], hi[1]);
-      row0.y = pack_bf16(hi[2], hi[3]);
-      row0.z = pack_bf16(hi[4], lo[0]);
-      row0.w = pack_bf16(lo[1], lo[2]);
-      row1.x = pack_bf16(lo[3], lo[4]);
-      row1.y = pack_bf16(hi[0], hi[1]);
-      row1.z = pack_bf16(hi[2], hi[3]);
-      row1.w = pack_bf16(hi[4], 0.0f);
-      *reinterpret_cast<uint4 *>(&act[tid][0]) = row0;
-      *reinterpret_cast<uint4 *>(&act[tid][8]) = row1;
-      *reinterpret_cast<uint4 *>(&act[tid][16]) = uint4{0u, 0u, 0u, 0u};
-      *reinterpret_cast<uint4 *>(&act[tid][24]) = uint4{0u, 0u, 0u, 0u};
+      row0.x = pack2_bf16(hi[0], hi[1]);
+      row0.y = pack2_bf16(hi[2], hi[3]);
+      row0.z = pack2_bf16(hi[4], lo[0]);
+      row0.w = pack2_bf16(lo[1], lo[2]);
+      row1.x = pack2_bf16(lo[3], lo[4]);
+      row1.y = pack2_bf16(hi[0], hi[1]);
+      row1.z = pack2_bf16(hi[2], hi[3]);
+      row1.w = pack2_bf16(hi[4], 0.0f);
+      *reinterpret_cast<uint4 *>(&act[X][r][0]) = row0;
+      *reinterpret_cast<uint4 *>(&act[X][r][8]) = row1;
+      *reinterpret_cast<uint4 *>(&act[X][r][16]) = uint4{0u, 0u, 0u, 0u};
+      *reinterpret_cast<uint4 *>(&act[X][r][24]) = uint4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
-    floatx4 acc[INF_MT][4];
-    // ---- layer 1: one 32-wide k step ------------------------------------
-    mfma_chunk<true>(acc, w1, act, 0, fr, fk);
-    __syncthreads();  // every wave has read the input rows
-    silu_store(acc, bias, act, bcol, lane);
+    Acc accA, accB;
+    // ---- the staggered layers ---------------------------------------------
+    mfma_chunk<true>(accA, w1, act[0], 0, fr, fk, c0);  // A layer 1
     __syncthreads();
-    // ---- hidden layers: ring slot 0 holds each one's first chunk ---------
-    load_bias(bias, D.b[0], bcol);
-    hidden_layer<true>(acc, ring, D.w16[0], D.w16[1], act, voff, fr, fk);
+    mfma_chunk<true>(accB, w1, act[1], 0, fr, fk, c0);  // B layer 1 | A SiLU 1
+#pragma unroll
+    for (int mt = 0; mt < INF_MT; ++mt) silu_rows(accA, act[0], bcol, lane, mt);
     __syncthreads();
-    silu_store(acc, bias, act, bcol, lane);
+    bias_init(c0, D.b[0], bcol);
+    stage<0>(accA, ring, D.w16[0], D.w16[0], act[0], voff, c0, accB, act[1], bcol, lane, fr, fk);  // A 2 | B 1
     __syncthreads();
-    load_bias(bias, D.b[1], bcol);
-    hidden_layer<true>(acc, ring, D.w16[1], D.w16[2], act, voff, fr, fk);
+    stage<2>(accB, ring, D.w16[0], D.w16[1], act[1], voff, c0, accA, act[0], bcol, lane, fr, fk);  // B 2 | A 2
     __syncthreads();
-    silu_store(acc, bias, act, bcol, lane);
+    bias_init(c0, D.b[1], bcol);
+    stage<1>(accA, ring, D.w16[1], D.w16[1], act[0], voff, c0, accB, act[1], bcol, lane, fr, fk);  // A 3 | B 2
     __syncthreads();
-    load_bias(bias, D.b[2], bcol);
-    hidden_layer<false>(acc, ring, D.w16[2], nullptr, act, voff, fr, fk);
-    // the output layer's weights, requested before the last SiLU
+    stage<0>(accB, ring, D.w16[1], D.w16[2], act[1], voff, c0, accA, act[0], bcol, lane, fr, fk);  // B 3 | A 3
+    __syncthreads();
+    bias_init(c0, D.b[2], bcol);
+    stage<2>(accA, ring, D.w16[2], D.w16[2], act[0], voff, c0, accB, act[1], bcol, lane, fr, fk);  // A 4 | B 3
+    __syncthreads();
+    stage<1>(accB, ring, D.w16[2], D.w16[2], act[1], voff, c0, accA, act[0], bcol, lane, fr, fk);  // B 4 | A 4
     uint4 w5[INF_HIDDEN / 32];
 #pragma unroll
     for (int kc = 0; kc < INF_HIDDEN / 32; ++kc)
       w5[kc] = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(D.w5_16) + 64 * kc + voff5);
     __syncthreads();
-    silu_store(acc, bias, act, bcol, lane);
+    output_layer(w5, act[0], yout[0], wave, lane, fr, fk);  // A output | B SiLU 4
+#pragma unroll
+    for (int mt = 0; mt < INF_MT; ++mt) silu_rows(accB, act[1], bcol, lane, mt);
     __syncthreads();
-    // ---- output layer on MFMA: wave w -> rays [16 INF_OQ w, +16 INF_OQ).
-    // The three outputs' weights are split into bf16 high and low parts
-    // (rows 0..2 and 3..5 of one 16-row tile, the rest zero), so the
-    // products carry the float32 weights to ~2^-17 at no extra MFMA --------
+    output_layer(w5, act[1], yout[1], wave, lane, fr, fk);  // B output
+    __syncthreads();
+    // ---- polynomial on the matrix cores (activations are dead: the
+    // monomials go to act[0]'s memory, the targets to act[1]'s) -------------
+    auto mono = reinterpret_cast<PolyRow *>(&act[0][0][0]);
+    auto ptgt = reinterpret_cast<float(*)[INF_COEF_STRIDE]>(&act[1][0][0]);
+    static_assert(2 * INF_RAYS * POLY_MONO_STRIDE * sizeof(float) <= sizeof(act[0]), "monomials fit");
+    static_assert(2 * INF_RAYS * INF_COEF_STRIDE * sizeof(float) <= sizeof(act[1]), "targets fit");
     {
-      floatx4 oacc[INF_OQ];
-#pragma unroll
-      for (int q = 0; q < INF_OQ; ++q) oacc[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int kc = 0; kc < INF_HIDDEN / 32; ++kc) {
-        const bf16x8 b5 = __builtin_bit_cast(bf16x8, w5[kc]);
-#pragma unroll
-        for (int q = 0; q < INF_OQ; ++q) {
-          const uint4 va = *reinterpret_cast<const uint4 *>(&act[16 * (INF_OQ * wave + q) + fr][32 * kc + fk]);
-          oacc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, va), b5, oacc[q], 0, 0, 0);
-        }
-      }
-      if (fr < 6) {
-#pragma unroll
-        for (int q = 0; q < INF_OQ; ++q)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) yout[16 * (INF_OQ * wave + q) + 4 * (lane >> 4) + r][fr] = oacc[q][r];
-      }
-    }
-    __syncthreads();  // act is free: the polynomial's partial sums go there
-    // ---- polynomial: waves 0-1 take the constant and subtrees 0 and 3,
-    // waves 2-3 subtrees 1, 2 and 4 (28 monomials each) of rays tid % 128 --
-    {
-      const int ray = tid & (INF_RAYS - 1);
       float u[5];
 #pragma unroll
-      for (int k = 0; k < 5; ++k) u[k] = us[ray][k];
-      float tgt[INF_TARGETS];
-#pragma unroll
-      for (int j = 0; j < INF_TARGETS; ++j) tgt[j] = 0.0f;
-      if (wave < 2) {
-        poly_term(1.0f, coef[0], tgt);
-        poly_subtree<0, POLY_SUBTREE[0]>(u, coef, tgt);
-        poly_subtree<3, POLY_SUBTREE[3]>(u, coef, tgt);
-      } else {
-        poly_subtree<1, POLY_SUBTREE[1]>(u, coef, tgt);
-        poly_subtree<2, POLY_SUBTREE[2]>(u, coef, tgt);
-        poly_subtree<4, POLY_SUBTREE[4]>(u, coef, tgt);
-      }
-#pragma unroll
-      for (int j = 0; j < INF_TARGETS; ++j) part[wave >> 1][ray][j] = tgt[j];
+      for (int k = 0; k < 5; ++k) u[k] = us[X][r][k];
+      poly_monomials(u, mono[tid]);
     }
     __syncthreads();
-    // ---- apertures, error map, stores: one thread per ray -----------------
+    poly_mfma(mono, 64 * wave, coef, ptgt, lane);
+    __syncthreads();
+    // ---- apertures, error map, stores: one thread per ray ------------------
     bool fallback = false;
-    const int64_t i = base + tid;
-    const bool live = tid < INF_RAYS && i < n;
-    if (live) {
-      float tgt[INF_TARGETS];
+    const bool live = i < n;
+    float tgt[INF_TARGETS];
 #pragma unroll
-      for (int j = 0; j < INF_TARGETS; ++j) tgt[j] = part[0][tid][j] + part[1][tid][j];
-      const float y0 = yout[tid][0] + yout[tid][3] + D.b5[0];
-      const float y1 = yout[tid][1] + yout[tid][4] + D.b5[1];
-      const float y2 = yout[tid][2] + yout[tid][5] + D.b5[2];
+    for (int j = 0; j < INF_TARGETS; ++j) tgt[j] = ptgt[tid][j];
+    if (live) {
+      // the activations carry log2(e): the output layer's sums times ln 2
+      const float y0 = (yout[X][r][0] + yout[X][r][3]) * INF_LN2 + D.b5[0];
+      const float y1 = (yout[X][r][1] + yout[X][r][4]) * INF_LN2 + D.b5[1];
+      const float y2 = (yout[X][r][2] + yout[X][r][5]) * INF_LN2 + D.b5[2];
       const float x = tgt[0] + D.scale[0] * y0;
       const float yy = tgt[1] + D.scale[1] * y1;
       const float li = tgt[2] + D.scale[2] * y2;
@@ -1130,8 +1206,8 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
         near = near || fabsf(d) < D.edge_margin;
       }
       // error-map cell: 16 wavelength x 8 field x x 8 field y bins of [-1, 1]
-      int cw = (int)((us[tid][4] + 1.0f) * 8.0f), cx = (int)((us[tid][0] + 1.0f) * 4.0f),
-          cy = (int)((us[tid][1] + 1.0f) * 4.0f);
+      int cw = (int)((us[X][r][4] + 1.0f) * 8.0f), cx = (int)((us[X][r][0] + 1.0f) * 4.0f),
+          cy = (int)((us[X][r][1] + 1.0f) * 4.0f);
       cw = cw < 0 ? 0 : (cw > 15 ? 15 : cw);
       cx = cx < 0 ? 0 : (cx > 7 ? 7 : cx);
       cy = cy < 0 ? 0 : (cy > 7 ? 7 : cy);
@@ -1144,7 +1220,7 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
       IO.fallback[i] = fallback ? 1 : 0;
     }
     // ---- compaction of the fallback rays: one atomic per wave ------------
-    if (IO.fallback_list && wave < INF_RAYS / 64) {
+    if (IO.fallback_list) {
       const uint64_t mask = __ballot(live && fallback);
       if (mask) {
         int32_t wbase = 0;
@@ -1158,7 +1234,7 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
         }
       }
     }
-    __syncthreads();  // act, us and yout are rewritten by the next tile
+    __syncthreads();  // act, us and yout are rewritten by the next pair
   }
 }
 
@@ -1437,9 +1513,9 @@ extern "C" int32_t optk_surrogate_infer(const optk_surrogate_desc *desc, const o
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("device query: ") + hipGetErrorString(e));
-  const int64_t tiles = (IO.n + INF_RAYS - 1) / INF_RAYS;
-  int64_t blocks = cus > 0 ? (int64_t)cus * INF_BLOCKS_PER_CU : 1024;  // all resident
-  if (blocks > tiles) blocks = tiles;
+  const int64_t pairs = (IO.n + 2 * INF_RAYS - 1) / (2 * INF_RAYS);
+  int64_t blocks = cus > 0 ? (int64_t)cus * INF_BLOCKS_PER_CU : 256;  // all resident
+  if (blocks > pairs) blocks = pairs;
   // the caller's row of every monomial in the kernel's order
   uint32_t codes[OPTK_SURROGATE_POLY];
   kernel_poly_codes(codes);
