@@ -984,39 +984,59 @@ __device__ __forceinline__ void bias_init(floatx4 (&c0)[4], const float *__restr
   }
 }
 
+// one row (q) of row tile mt of acc through the SiLU into act (four
+// features, one 8-byte store)
+__device__ __forceinline__ void silu_row(const Acc &acc, ActRow *act, int col, int lane, int mt, int q) {
+  const int row = 16 * mt + 4 * (lane >> 4) + q;
+  uint2 v;
+  v.x = pack2_bf16(silu_scaled(acc[mt][0][q]), silu_scaled(acc[mt][1][q]));
+  v.y = pack2_bf16(silu_scaled(acc[mt][2][q]), silu_scaled(acc[mt][3][q]));
+  *reinterpret_cast<uint2 *>(&act[row][col]) = v;
+}
+
 // one stage: accx = c0 + actx . W (K = 256, eight 32-wide k chunks), and,
 // while the MFMAs run, row tile c of accy through the SiLU into acty.
 // Weights stream through a ring of three register buffers: chunk c sits in
 // slot (S + c) % 3 and requests chunk c + 2 before its MFMAs; past the end
 // the requests fetch the next stage's first two chunks from Wn, which
 // therefore starts at slot (S + 2) % 3.  Wn must be a valid weight matrix
-// (the last stage passes its own: those requests go unused).
+// (the last stage passes its own: those requests go unused).  The A
+// operands (activation rows) are read one chunk ahead: row tile mt of
+// chunk c + 1 replaces that of chunk c as soon as its four MFMAs are
+// issued.  Each chunk is cut into four scheduling regions of two row tiles
+// (eight MFMAs) and one SiLU row (four values): a small region lets the
+// scheduler put the SiLU's VALU work between the MFMAs, which execute
+// meanwhile.
 template <int S>
 __device__ __forceinline__ void stage(Acc &accx, uint4 (&ring)[3][4], const uint16_t *__restrict__ W,
                                       const uint16_t *__restrict__ Wn, const ActRow *actx, uint32_t voff,
                                       const floatx4 (&c0)[4], const Acc &accy, ActRow *acty, int col,
                                       int lane, int fr, int fk) {
+  uint4 a[INF_MT];
+#pragma unroll
+  for (int mt = 0; mt < INF_MT; ++mt) a[mt] = *reinterpret_cast<const uint4 *>(&actx[16 * mt + fr][fk]);
+  __builtin_amdgcn_sched_barrier(0);  // chunk 0's operands first, in a region of their own
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     if (c + 2 < 8)
       wload<INF_HIDDEN>(ring[(S + c + 2) % 3], W, voff, 32 * (c + 2));
     else
       wload<INF_HIDDEN>(ring[(S + c + 2) % 3], Wn, voff, 32 * (c + 2 - 8));
-    if (c == 0)
-      mfma_chunk<true>(accx, ring[(S + c) % 3], actx, 32 * c, fr, fk, c0);
-    else
-      mfma_chunk<false>(accx, ring[(S + c) % 3], actx, 32 * c, fr, fk, c0);
-    silu_rows(accy, acty, col, lane, c);
-    // issue order: each MFMA followed by four VALU instructions of the
-    // SiLU, which execute while the matrix core works on it
+    bf16x8 b[4];
 #pragma unroll
-    for (int g = 0; g < 32; ++g) {
-      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-      __builtin_amdgcn_sched_group_barrier(0x402, 4, 0);  // VALU, transcendental
+    for (int nt = 0; nt < 4; ++nt) b[nt] = __builtin_bit_cast(bf16x8, ring[(S + c) % 3][nt]);
+#pragma unroll
+    for (int mt = 0; mt < INF_MT; ++mt) {
+      const bf16x8 am = __builtin_bit_cast(bf16x8, a[mt]);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt)
+        accx[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, b[nt], c == 0 ? c0[nt] : accx[mt][nt], 0, 0, 0);
+      if (c + 1 < 8) a[mt] = *reinterpret_cast<const uint4 *>(&actx[16 * mt + fr][32 * (c + 1) + fk]);
+      if (mt & 1) {
+        silu_row(accy, acty, col, lane, c, mt >> 1);
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
-    // one scheduling region per chunk (the pattern search grows fast with
-    // the region)
-    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
