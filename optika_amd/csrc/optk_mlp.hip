@@ -933,8 +933,8 @@ __device__ __forceinline__ void wload(uint4 (&w)[4], const uint16_t *__restrict_
 
 // acc (+)= act[:, k0 .. k0 + 32) x the chunk's weights (FIRST: acc = c0 +
 // product, c0[nt] the scaled bias of the lane's column in n-tile nt)
-template <bool FIRST>
-__device__ __forceinline__ void mfma_chunk(Acc &acc, const uint4 (&w)[4], const ActRow *act, int k0,
+template <bool FIRST, typename Row>
+__device__ __forceinline__ void mfma_chunk(Acc &acc, const uint4 (&w)[4], const Row *act, int k0,
                                            int fr, int fk, const floatx4 (&c0)[4]) {
   bf16x8 b[4];
 #pragma unroll
@@ -1070,7 +1070,13 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
                                                                           const optk_surrogate_io IO,
                                                                           const PolyOrder P) {
   __shared__ __attribute__((aligned(16))) ActRow act[2][INF_RAYS];
-  __shared__ float yout[2][INF_RAYS][8];
+  __shared__ __attribute__((aligned(16))) float yout[2][INF_RAYS][8];
+  // tile A's layer-1 input rows live in yout's memory (free until the
+  // output layer), so that tile A's layer-1 SiLU may write act[0] while
+  // other waves still read those rows
+  typedef uint16_t InRow[32];
+  auto ain = reinterpret_cast<InRow *>(&yout[0][0][0]);
+  static_assert(INF_RAYS * sizeof(InRow) <= sizeof(yout), "tile A's input rows fit yout");
   __shared__ float us[2][INF_RAYS][5];  // normalised inputs
   __shared__ __attribute__((aligned(16))) float coef[OPTK_SURROGATE_POLY][INF_COEF_STRIDE];  // kernel order
   const int tid = threadIdx.x;
@@ -1085,6 +1091,34 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
   for (int idx = tid; idx < OPTK_SURROGATE_POLY * INF_COEF_STRIDE; idx += blockDim.x) {
     const int t = idx / INF_COEF_STRIDE, j = idx % INF_COEF_STRIDE;
     coef[t][j] = j < INF_TARGETS ? D.poly[P.row[t] * INF_TARGETS + j] : 0.0f;
+  }
+  // the error map and, per polygon edge from vertex v - 1 to v, its start,
+  // direction, 1 / |direction|^2 and inverse slope dx / dy (0 for a
+  // horizontal edge, which the crossing rule never uses)
+  __shared__ uint8_t emap[16 * 8 * 8];
+  __shared__ float edge[OPTK_SURROGATE_MAX_APERTURES][OPTK_SURROGATE_MAX_VERTICES][6];
+  for (int idx = tid; idx < 16 * 8 * 8; idx += blockDim.x) emap[idx] = D.error_map ? D.error_map[idx] : 0;
+  if (tid < OPTK_SURROGATE_MAX_APERTURES * OPTK_SURROGATE_MAX_VERTICES) {
+    const int ap = tid / OPTK_SURROGATE_MAX_VERTICES, v = tid % OPTK_SURROGATE_MAX_VERTICES;
+    const int nv = D.apertures[ap].num_vertices;
+    if (ap < D.num_apertures && v < nv) {
+      const int w = v == 0 ? nv - 1 : v - 1;
+      const float x1 = D.apertures[ap].vx[w], y1 = D.apertures[ap].vy[w];
+      const float ex = D.apertures[ap].vx[v] - x1, ey = D.apertures[ap].vy[v] - y1;
+      edge[ap][v][0] = x1;
+      edge[ap][v][1] = y1;
+      edge[ap][v][2] = ex;
+      edge[ap][v][3] = ey;
+      edge[ap][v][4] = 1.0f / (ex * ex + ey * ey);
+      edge[ap][v][5] = ey != 0.0f ? ex / ey : 0.0f;
+    }
+  }
+  // the first pair's inputs; each pair requests the next one's in its tail
+  float nxt[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const int64_t i0 = (int64_t)blockIdx.x * 2 * INF_RAYS + tid;
+    nxt[k] = i0 < n ? in[k][i0] : 0.0f;
   }
   for (int64_t pair = blockIdx.x; pair < pairs; pair += gridDim.x) {
     // rays [base, base + 128) form tile A, [base + 128, base + 256) tile B;
@@ -1109,8 +1143,7 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
       float hi[5], lo[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
-        const float v = i < n ? in[k][i] : 0.0f;
-        const float u = (v - D.in_mid[k]) * D.in_inv_half[k];
+        const float u = (nxt[k] - D.in_mid[k]) * D.in_inv_half[k];
         us[X][r][k] = u;
         const float us2 = INF_LOG2E * u;
         hi[k] = from_bf16(to_bf16(us2));
@@ -1125,16 +1158,16 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
       row1.y = pack2_bf16(hi[0], hi[1]);
       row1.z = pack2_bf16(hi[2], hi[3]);
       row1.w = pack2_bf16(hi[4], 0.0f);
-      *reinterpret_cast<uint4 *>(&act[X][r][0]) = row0;
-      *reinterpret_cast<uint4 *>(&act[X][r][8]) = row1;
-      *reinterpret_cast<uint4 *>(&act[X][r][16]) = uint4{0u, 0u, 0u, 0u};
-      *reinterpret_cast<uint4 *>(&act[X][r][24]) = uint4{0u, 0u, 0u, 0u};
+      uint16_t *dst = X == 0 ? &ain[r][0] : &act[1][r][0];
+      *reinterpret_cast<uint4 *>(dst) = row0;
+      *reinterpret_cast<uint4 *>(dst + 8) = row1;
+      *reinterpret_cast<uint4 *>(dst + 16) = uint4{0u, 0u, 0u, 0u};
+      *reinterpret_cast<uint4 *>(dst + 24) = uint4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
     Acc accA, accB;
     // ---- the staggered layers ---------------------------------------------
-    mfma_chunk<true>(accA, w1, act[0], 0, fr, fk, c0);  // A layer 1
-    __syncthreads();
+    mfma_chunk<true>(accA, w1, ain, 0, fr, fk, c0);     // A layer 1
     mfma_chunk<true>(accB, w1, act[1], 0, fr, fk, c0);  // B layer 1 | A SiLU 1
 #pragma unroll
     for (int mt = 0; mt < INF_MT; ++mt) silu_rows(accA, act[0], bcol, lane, mt);
@@ -1163,7 +1196,12 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
     for (int mt = 0; mt < INF_MT; ++mt) silu_rows(accB, act[1], bcol, lane, mt);
     __syncthreads();
     output_layer(w5, act[1], yout[1], wave, lane, fr, fk);  // B output
-    __syncthreads();
+    // ---- the next pair's inputs, requested now: they arrive during the tail
+    {
+      const int64_t inext = (pair + gridDim.x) * 2 * INF_RAYS + tid;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) nxt[k] = inext < n ? in[k][inext] : 0.0f;
+    }
     // ---- polynomial on the matrix cores (activations are dead: the
     // monomials go to act[0]'s memory, the targets to act[1]'s) -------------
     auto mono = reinterpret_cast<PolyRow *>(&act[0][0][0]);
@@ -1174,11 +1212,12 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
       float u[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) u[k] = us[X][r][k];
-      poly_monomials(u, mono[tid]);
+      poly_monomials(u, mono[tid]);  // act[0] is free: tile A's output layer is done
     }
-    __syncthreads();
+    __syncthreads();  // every wave has read act[1] (tile B's output layer)
+    // a wave reads only its own rays' monomials and targets, and a wave's
+    // LDS accesses take effect in order: no barrier around the product
     poly_mfma(mono, 64 * wave, coef, ptgt, lane);
-    __syncthreads();
     // ---- apertures, error map, stores: one thread per ray ------------------
     bool fallback = false;
     const bool live = i < n;
@@ -1205,19 +1244,20 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
         } else if (ap.kind == OPTK_APERTURE_RECTANGLE) {
           d = fminf(ap.p[0] - fabsf(ax), ap.p[1] - fabsf(ay));
         } else if (ap.kind == OPTK_APERTURE_POLYGON) {
-          float dmin = INFINITY;
+          // distance to the nearest edge (the minimum of the squares, one
+          // square root) and the crossing rule, from the edge table
+          float dmin2 = INFINITY;
           bool in_poly = false;
           for (int v = 0; v < ap.num_vertices; ++v) {
-            const int w = v == 0 ? ap.num_vertices - 1 : v - 1;
-            const float x1 = ap.vx[w], y1 = ap.vy[w], x2 = ap.vx[v], y2 = ap.vy[v];
-            const float ex = x2 - x1, ey = y2 - y1;
-            float s = ((ax - x1) * ex + (ay - y1) * ey) / (ex * ex + ey * ey);
+            const float *e = edge[a][v];
+            const float x1 = e[0], y1 = e[1], ex = e[2], ey = e[3];
+            float s = ((ax - x1) * ex + (ay - y1) * ey) * e[4];
             s = fminf(fmaxf(s, 0.0f), 1.0f);
             const float dx = ax - (x1 + s * ex), dy = ay - (y1 + s * ey);
-            dmin = fminf(dmin, sqrtf(dx * dx + dy * dy));
-            if (((y1 > ay) != (y2 > ay)) && (ax < (x2 - x1) * (ay - y1) / (y2 - y1) + x1)) in_poly = !in_poly;
+            dmin2 = fminf(dmin2, dx * dx + dy * dy);
+            if (((y1 > ay) != (y1 + ey > ay)) && (ax < e[5] * (ay - y1) + x1)) in_poly = !in_poly;
           }
-          d = in_poly ? dmin : -dmin;
+          d = in_poly ? sqrtf(dmin2) : -sqrtf(dmin2);
         } else {
           d = 0.0f;  // no distance model: always near
         }
@@ -1231,7 +1271,7 @@ __global__ __launch_bounds__(256, INF_BLOCKS_PER_CU) void surrogate_infer(const 
       cw = cw < 0 ? 0 : (cw > 15 ? 15 : cw);
       cx = cx < 0 ? 0 : (cx > 7 ? 7 : cx);
       cy = cy < 0 ? 0 : (cy > 7 ? 7 : cy);
-      const bool flagged = D.error_map ? D.error_map[(cw * 8 + cx) * 8 + cy] != 0 : false;
+      const bool flagged = emap[(cw * 8 + cx) * 8 + cy] != 0;
       fallback = near || flagged || !(fabsf(x) < INFINITY && fabsf(yy) < INFINITY && fabsf(li) < INFINITY);
       IO.x[i] = x;
       IO.y[i] = yy;
