@@ -44,24 +44,28 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 # algorithmic bytes per ray of the grid-input rayfunction (float64):
-#   in:  field x, y, pupil x, y, wavelength           5 x 8 = 40 B
+#   in:  field x, y, pupil x, y                       4 x 8 = 32 B
+#        wavelength, when the trace depends on it       + 8 B
 #   out: position 3, direction 3, intensity           7 x 8 = 56 B
 #        unvignetted mask                                      1 B
 # (wavelength, attenuation and index are invariant through an all-mirror
-#  system and are returned as the input / constants, not rewritten)
-BYTES_PER_RAY_F64 = 97
+#  system and are returned as the input / constants, not rewritten; the
+#  Newtonian's specialised kernels do not read the wavelength at all: no
+#  surface of it depends on it)
+def bytes_per_ray(precision: str, reads_wavelength: bool) -> int:
+    w = 8 if precision == "f64" else 4
+    return w * (4 + int(reads_wavelength)) + 7 * w + 1
 
-# HBM bytes per launch of trace_grid_f64 at 2^24 rays, from rocprofv3 --pmc:
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 256 MiB device copy);
-# the runs and commands are in the profile summary named here
-TRAFFIC_BYTES_F64_2P24 = (327867 * 2.0 + 934303 * 1.0) * 1024
+
+# HBM bytes per launch of the Newtonian's specialised trace_grid kernels at
+# 2^24 rays, from rocprofv3 --pmc: reads = 128 B x TCC_EA0_RDREQ_128B +
+# 64 B x TCC_EA0_RDREQ_64B + 32 B x TCC_EA0_RDREQ_32B (a 256 MiB device
+# copy in the same run reads exactly 256 MiB by this count), writes =
+# WRITE_SIZE KiB (the copy writes 256 MiB by it); the runs and commands are
+# in the profile summaries named here
+TRAFFIC_BYTES_F64_2P24 = 4194304 * 128 + 1848 * 64 + 933888 * 1024
 TRAFFIC_SOURCE_F64 = "profiles/trace_grid_f64_newtonian_2p24.md (rocprofv3 --pmc)"
-
-# float32 form: in 5 x 4 B, out 7 x 4 B + 1 B mask
-BYTES_PER_RAY_F32 = 49
-# HBM bytes per launch of trace_grid_f32 at 2^24 rays (rocprofv3 --pmc, the
-# same scaling; profiles/trace_grid_f32_newtonian_2p24.md)
-TRAFFIC_BYTES_F32_2P24 = (163964 * 2.0 + 475159 * 1.0) * 1024
+TRAFFIC_BYTES_F32_2P24 = 2097152 * 128 + 1560 * 64 + 475136 * 1024
 TRAFFIC_SOURCE_F32 = "profiles/trace_grid_f32_newtonian_2p24.md (rocprofv3 --pmc)"
 
 PIXEL_MM = 0.015
@@ -338,8 +342,8 @@ def two_forms(args, system, grid, n, surfaces, paths, traffic32=None,
                 "kernel": kernel, "kernel_avg_ms": round(k_ms, 4),
                 "bytes_per_ray": bytes_per_ray}
 
-    roof64 = roof(BYTES_PER_RAY_F64, k64, kernel64, traffic64)
-    roof32 = roof(BYTES_PER_RAY_F32, k32, kernel32, traffic32)
+    roof64 = roof(bytes_per_ray("f64", plan64.reads_wavelength), k64, kernel64, traffic64)
+    roof32 = roof(bytes_per_ray("f32", plan32.reads_wavelength), k32, kernel32, traffic32)
     err = errs[0] if paths == 1 else {f"path{p}": e for p, e in enumerate(errs)}
     forms = {
         "f64": {"value": value64, "ms_per_step": ms64, "kernel_avg_ms": round(k64, 4),

@@ -21,8 +21,12 @@ from .vectors import Cartesian3dVectorArray, ObjectVectorArray
 __all__ = ["trace_sequence", "trace_system", "device", "compiled"]
 
 # the layouts the trace kernels are specialised for, in the order of the
-# library's signature table (optk_system_specialised returns the index)
+# library's signature table (optk_system_specialised returns the index),
+# and whether their trace depends on the wavelength (an all-mirror system
+# without gratings or coatings does not: its specialised kernels never
+# read the wavelength array)
 SIGNATURE_NAMES = ("newtonian", "spectrograph")
+SIGNATURE_READS_WAVELENGTH = (False, True)
 
 _PRECISIONS = {"f64": _lib.F64, "float64": _lib.F64, "f32": _lib.F32, "float32": _lib.F32}
 
@@ -386,6 +390,15 @@ class GridTracePlan:
         self.grid.intensity_default = 1.0
         self.num_surfaces = self.cs.num_surfaces
         self.generic = bool(generic)
+
+    @property
+    def reads_wavelength(self) -> bool:
+        """Whether a call reads the wavelength array (the generic kernels
+        always do)."""
+        sig = self.cs.signature
+        if sig >= 0 and not self.generic and sig < len(SIGNATURE_READS_WAVELENGTH):
+            return SIGNATURE_READS_WAVELENGTH[sig]
+        return True
 
     @property
     def kernel(self) -> str:
