@@ -132,6 +132,45 @@ template <int IO> __device__ __forceinline__ void store_ray(const FRaysOut &o, i
   store_f(o, j, r);
 }
 
+// p[i] through a 32-bit byte offset against the uniform base: global
+// addressing with the base in scalar registers and one VGPR offset, no
+// 64-bit address add per array.  Callers guarantee i * sizeof(T) < 2^32.
+template <typename T> __device__ __forceinline__ T &at32(T *p, uint32_t i) {
+  typedef typename std::conditional<std::is_const<T>::value, const char, char>::type Byte;
+  return *reinterpret_cast<T *>(reinterpret_cast<Byte *>(p) + i * (uint32_t)sizeof(T));
+}
+
+// the sensor-frame layout's loads and stores at 32-bit offsets (the
+// specialised grid kernels; other layouts take the 64-bit forms)
+template <int IO> __device__ __forceinline__ FRay load_ray32(const FGridIn &g, uint32_t i) {
+  if (IO == IO_SENSOR)
+    return fast_grid_ray(at32(g.field[0], i), at32(g.field[1], i), at32(g.pupil[0], i), at32(g.pupil[1], i),
+                         at32(g.wavelength, i), g.intensity_default, g.object_at_infinity);
+  return load_ray<IO>(g, (int64_t)i);
+}
+template <int IO> __device__ __forceinline__ FRay load_ray32(const FRaysIn &in, uint32_t i) {
+  return load_ray<IO>(in, (int64_t)i);
+}
+
+template <int IO> __device__ __forceinline__ void store_ray32(const FRaysOut &o, uint32_t j, const FRay &r) {
+  if (IO == IO_SENSOR) {
+    // the offset is formed here (an empty asm stops it being hoisted into
+    // an earlier block as a 64-bit address, which the stores could then not
+    // take as a scalar base plus a 32-bit offset)
+    asm volatile("" : "+v"(j));
+    at32(o.p[0], j) = (float)r.px;
+    at32(o.p[1], j) = (float)r.py;
+    at32(o.p[2], j) = (float)r.pz;
+    at32(o.d[0], j) = (float)r.dx;
+    at32(o.d[1], j) = (float)r.dy;
+    at32(o.d[2], j) = (float)r.dz;
+    at32(o.intensity, j) = r.intensity;
+    at32(o.unvignetted, j) = r.unvignetted ? 1 : 0;
+    return;
+  }
+  store_ray<IO>(o, (int64_t)j, r);
+}
+
 size_t fast_staged_bytes(int num_surfaces) {
   return sizeof(FastSpec) + (sizeof(FastSurface) + sizeof(optk_surface)) * (size_t)num_surfaces;
 }
@@ -264,8 +303,10 @@ __device__ __forceinline__ void fast_spec_step(int k, int k0, ConstFastSurface *
 
 template <typename In, int F, int IO, uint32_t... K>
 __device__ __forceinline__ void fast_spec_body(const FastArgs &A, const In &in) {
-  const int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
-  if (i >= A.n) return;
+  // the host launches these kernels with n * paths < 2^29: indices and
+  // every element offset fit 32 bits
+  const uint32_t i32 = blockIdx.x * (uint32_t)TRACE_BLOCK + threadIdx.x, n32 = (uint32_t)A.n;
+  if (i32 >= n32) return;
   ConstFastSpec *c_spec = (ConstFastSpec *)A.spec;
   ConstFastSurface *c_fast = (ConstFastSurface *)A.fast;
   ConstSurface *c_surf = (ConstSurface *)A.surfaces;
@@ -277,7 +318,7 @@ __device__ __forceinline__ void fast_spec_body(const FastArgs &A, const In &in) 
   const int num_paths = (F & FEAT_RULINGS) ? A.num_paths : 1;
   const bool skip = A.entry == 1 && c_spec->skip_first != 0;
   const int k0 = skip ? 1 : 0;
-  FRay r0 = load_ray<IO>(in, i);
+  FRay r0 = load_ray32<IO>(in, i32);
   if (skip) {
     const bool parallel = r0.dz == 0.0;
     fast_xf(*(const optk_affine *)&c_spec->first_skip, r0);
@@ -294,7 +335,7 @@ __device__ __forceinline__ void fast_spec_body(const FastArgs &A, const In &in) 
     (fast_spec_step<F, K>(k++, k0, c_fast, c_surf, v, path, r), ...);
     const optk_affine &last = *(const optk_affine *)&c_spec->last[A.apply_post];
     fast_xf(last, r);
-    store_ray<IO>(A.out, (int64_t)path * A.n + i, r);
+    store_ray32<IO>(A.out, (uint32_t)path * n32 + i32, r);
   }
 }
 
@@ -336,7 +377,8 @@ int32_t launch(const KernelSet<In> &ks, bool special, const optk_system *sys, co
                int32_t apply_post, void (*spec)(FastArgs, In), hipStream_t stream) {
   if (n == 0) return OPTK_OK;
   FastArgs A = make_args(sys, out, n, entry, apply_post);
-  if (!accum && special && spec && n / TRACE_BLOCK < INT32_MAX) {
+  // (the specialised kernels address their arrays with 32-bit offsets)
+  if (!accum && special && spec && n * (int64_t)sys->num_paths < ((int64_t)1 << 29)) {
     hipLaunchKernelGGL(spec, dim3((unsigned)((n + TRACE_BLOCK - 1) / TRACE_BLOCK)),
                        dim3(TRACE_BLOCK), 0, stream, A, in);
     hipError_t e = hipGetLastError();

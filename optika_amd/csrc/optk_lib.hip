@@ -220,6 +220,45 @@ template <int IO> __device__ __forceinline__ void store_io(const RaysOut &o, int
   store(o, j, r);
 }
 
+// p[i] through a 32-bit byte offset against the uniform base: global
+// addressing with the base in scalar registers and one VGPR offset, no
+// 64-bit address add per array.  Callers guarantee i * sizeof(T) < 2^32.
+template <typename T> __device__ __forceinline__ T &at32(T *p, uint32_t i) {
+  typedef typename std::conditional<std::is_const<T>::value, const char, char>::type Byte;
+  return *reinterpret_cast<T *>(reinterpret_cast<Byte *>(p) + i * (uint32_t)sizeof(T));
+}
+
+// the sensor-frame layout's loads and stores at 32-bit offsets (the
+// specialised grid kernels; other layouts take the 64-bit forms)
+template <int IO> __device__ __forceinline__ Ray<double> load_io32(const GridIn &g, uint32_t i) {
+  if (IO == IO_SENSOR)
+    return grid_ray(at32(g.field[0], i), at32(g.field[1], i), at32(g.pupil[0], i), at32(g.pupil[1], i),
+                    at32(g.wavelength, i), g.intensity_default, g.object_at_infinity);
+  return load_io<IO>(g, (int64_t)i);
+}
+template <int IO> __device__ __forceinline__ Ray<double> load_io32(const RaysIn &in, uint32_t i) {
+  return load_io<IO>(in, (int64_t)i);
+}
+
+template <int IO> __device__ __forceinline__ void store_io32(const RaysOut &o, uint32_t j, const Ray<double> &r) {
+  if (IO == IO_SENSOR) {
+    // the offset is formed here (an empty asm stops it being hoisted into
+    // an earlier block as a 64-bit address, which the stores could then not
+    // take as a scalar base plus a 32-bit offset)
+    asm volatile("" : "+v"(j));
+    at32(o.p[0], j) = r.px;
+    at32(o.p[1], j) = r.py;
+    at32(o.p[2], j) = r.pz;
+    at32(o.d[0], j) = r.dx;
+    at32(o.d[1], j) = r.dy;
+    at32(o.d[2], j) = r.dz;
+    at32(o.intensity, j) = r.intensity;
+    at32(o.unvignetted, j) = r.unvignetted ? 1 : 0;
+    return;
+  }
+  store_io<IO>(o, (int64_t)j, r);
+}
+
 // The system's records are read through the constant address space: every
 // lane of a wave reads the same record at the same time, so the loads are
 // scalar-cache reads whose values land in SGPRs, which the VALU
@@ -293,22 +332,24 @@ static void launch_trace(void (*k0)(TraceArgs, In, RaysOut), void (*k3)(TraceArg
 // lanes.)  The arithmetic is that of the generic kernels.
 template <typename In, int F, int IO, uint32_t... K>
 __device__ __forceinline__ void spec_body(const TraceArgs &A, const In &in) {
-  const int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
-  if (i >= A.n) return;
+  // the host launches these kernels with n * paths < 2^29: indices and
+  // every element offset fit 32 bits
+  const uint32_t i32 = blockIdx.x * (uint32_t)TRACE_BLOCK + threadIdx.x, n32 = (uint32_t)A.n;
+  if (i32 >= n32) return;
   ConstTraceSpec *c_spec = (ConstTraceSpec *)A.spec;
   ConstSurface *c_surf = (ConstSurface *)A.surfaces;
   const TraceSpec &spec = *(const TraceSpec *)c_spec;
   SystemView sys = spec.sys;
   sys.surfaces = A.surfaces;
   const int num_paths = (F & FEAT_RULINGS) ? sys.num_paths : 1;
-  Ray<double> r0 = load_io<IO>(in, i);
+  Ray<double> r0 = load_io32<IO>(in, i32);
   apply_pre(spec, r0);
   for (int path = 0; path < num_paths; ++path) {
     Ray<double> r = r0;
     int k = 0;
     (propagate_surface<F, K>(*(const optk_surface *)&c_surf[k++], sys, path, r), ...);
     if (A.apply_post) xf_inverse_ray(spec.post, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
-    store_io<IO>(A.out, (int64_t)path * A.n + i, r);
+    store_io32<IO>(A.out, (uint32_t)path * n32 + i32, r);
   }
 }
 
@@ -1526,7 +1567,8 @@ int32_t optk_trace(const optk_system *sys, int32_t precision, const optk_rays *i
   ri.attenuation_default = in->attenuation_default;
   ri.index_default = in->index_default;
   ri.unvignetted_default = in->unvignetted_default;
-  const bool spec_ok = sys->signature >= 0 && !(flags & OPTK_TRACE_GENERIC) && n / TRACE_BLOCK < INT32_MAX;
+  const bool spec_ok = sys->signature >= 0 && !(flags & OPTK_TRACE_GENERIC) &&
+                       n * (int64_t)sys->num_paths < ((int64_t)1 << 29);
   if (!accum && full_output(A.out) && spec_ok)
     hipLaunchKernelGGL(SPEC_RAYS64[sys->signature], dim3(spec_blocks(n)), dim3(TRACE_BLOCK), 0,
                        (hipStream_t)hip_stream, A, ri);
@@ -1583,7 +1625,8 @@ int32_t optk_trace_grid(const optk_system *sys, int32_t precision, const optk_gr
   g.wavelength_default = in->wavelength_default;
   g.intensity_default = in->intensity_default;
   g.object_at_infinity = in->object_at_infinity;
-  const bool spec_ok = sys->signature >= 0 && !(flags & OPTK_TRACE_GENERIC) && n / TRACE_BLOCK < INT32_MAX;
+  const bool spec_ok = sys->signature >= 0 && !(flags & OPTK_TRACE_GENERIC) &&
+                       n * (int64_t)sys->num_paths < ((int64_t)1 << 29);
   if (!accum && sensor_layout(g, A.out) && spec_ok)
     hipLaunchKernelGGL(SPEC_GRID64[sys->signature], dim3(spec_blocks(n)), dim3(TRACE_BLOCK), 0,
                        (hipStream_t)hip_stream, A, g);

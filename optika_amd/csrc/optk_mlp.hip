@@ -816,7 +816,7 @@ constexpr int INF_MT = INF_RAYS / 16;  // 16-ray MFMA row tiles per tile
 constexpr int INF_OQ = INF_MT / 4;     // output-layer row tiles per wave
 constexpr int INF_BLOCKS_PER_CU = 1;
 constexpr int INF_HIDDEN = 256;
-constexpr int INF_PAD = 8;
+constexpr int INF_PAD = 16;
 constexpr int INF_TARGETS = OPTK_SURROGATE_MAX_TARGETS;
 
 // The kernel's order of the 56 cubic monomials: depth first over
