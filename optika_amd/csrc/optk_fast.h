@@ -147,6 +147,10 @@ struct FastSurface {
   float surf_row[3], surf_tz;  // z row of sag_to_surface (toroid residual)
   int32_t sag_kind;
   int32_t polish;              // float64 Newton steps after the float32 solve
+  // the aperture's parameters as its float32 test uses them: circle
+  // {r^2}; sector {r^2, start, stop}; ellipse and rectangle {a, b} (r^2
+  // rounded from its float64 square, each value rounded once)
+  float ap32[4];
 };
 
 struct FastSpec {
@@ -342,6 +346,14 @@ inline void fast_build(const optk_surface *surfaces, int num_surfaces, const opt
     f.surf_tz = (float)S.t[2];
     f.sag_kind = s.sag_kind;
     f.polish = sag_steepness(s) > OPTK_STEEP_SLOPE ? 2 : 0;
+    if (s.aperture_kind == OPTK_APERTURE_CIRCLE || s.aperture_kind == OPTK_APERTURE_SECTOR) {
+      f.ap32[0] = (float)(s.aperture_p[0] * s.aperture_p[0]);
+      f.ap32[1] = (float)s.aperture_p[1];
+      f.ap32[2] = (float)s.aperture_p[2];
+    } else {
+      f.ap32[0] = (float)s.aperture_p[0];
+      f.ap32[1] = (float)s.aperture_p[1];
+    }
     prev_to_global = T;
   }
   if (num_surfaces > 0) {
@@ -995,33 +1007,33 @@ OPTK_HD float fast_material_efficiency(const optk_surface &s, const StackView &s
 }
 
 template <uint32_t K = 0>
-OPTK_HD bool fast_aperture(const optk_surface &s, const Affine32 &to_aperture, const float *tables32,
-                           float x, float y, float z) {
+OPTK_HD bool fast_aperture(const optk_surface &s, const float (&ap)[4], const Affine32 &to_aperture,
+                           const float *tables32, float x, float y, float z) {
   affine32_apply(to_aperture, x, y, z);
   bool mask = true;
   switch (k_aperture<K>(s)) {
   case OPTK_APERTURE_CIRCLE:
-    mask = x * x + y * y <= (float)(s.aperture_p[0] * s.aperture_p[0]);
+    mask = x * x + y * y <= ap[0];
     break;
   case OPTK_APERTURE_SECTOR: {
-    bool mr = x * x + y * y <= (float)(s.aperture_p[0] * s.aperture_p[0]);
+    bool mr = x * x + y * y <= ap[0];
     float ang = atan2f(y, x);
     const float two_pi = 6.283185307179586f;
     float apos = fmodf(ang, two_pi);
     if (apos < 0.0f) apos += two_pi;
     float aneg = fmodf(ang, -two_pi);
     if (aneg > 0.0f) aneg -= two_pi;
-    float a0 = (float)s.aperture_p[1], a1 = (float)s.aperture_p[2];
+    float a0 = ap[1], a1 = ap[2];
     mask = mr && (((a0 < apos) && (apos < a1)) || ((a0 < aneg) && (aneg < a1)));
     break;
   }
   case OPTK_APERTURE_ELLIPSE: {
-    float u = x / (float)s.aperture_p[0], v = y / (float)s.aperture_p[1];
+    float u = x / ap[0], v = y / ap[1];
     mask = u * u + v * v <= 1.0f;
     break;
   }
   case OPTK_APERTURE_RECTANGLE: {
-    float hx = (float)s.aperture_p[0], hy = (float)s.aperture_p[1];
+    float hx = ap[0], hy = ap[1];
     mask = ((-hx <= x) && (x <= hx)) && ((-hy <= y) && (y <= hy));
     break;
   }
@@ -1110,9 +1122,9 @@ OPTK_HD void fast_propagate(const FastSurface &f, const optk_surface &s, const F
 
   if (k_aperture<K>(s) != OPTK_APERTURE_NONE) {
     bool m = (k_aperture_flags<K>(s) & OPTK_AP_DIRECTION)
-                 ? fast_aperture<K>(s, f.aperture_from_surface, v.tables32, (float)r.dx,
+                 ? fast_aperture<K>(s, f.ap32, f.aperture_from_surface, v.tables32, (float)r.dx,
                                     (float)r.dy, (float)r.dz)
-                 : fast_aperture<K>(s, f.aperture_from_sag, v.tables32, sx, sy, sz);
+                 : fast_aperture<K>(s, f.ap32, f.aperture_from_sag, v.tables32, sx, sy, sz);
     r.unvignetted = r.unvignetted && m;
   }
 }
