@@ -506,7 +506,10 @@ int32_t optk_gemm_bf16(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, i
  *                        out16[i][j] = bf16((A B)[i][j] * silu'(pre[i][j]));
  * silu(x) = x / (1 + e^-x).  c, pre and out16 are [m][ldc].  m and n
  * multiples of 128, k a multiple of 32, lda and ldb multiples of 8, ldc a
- * multiple of 4; a, b, c, pre and bias 16-byte aligned, out16 8-byte. */
+ * multiple of 4; a, b, c, pre and bias 16-byte aligned, out16 8-byte.
+ * With trans_a = 0, trans_b = 1 and n = k = 256 (the surrogate's hidden
+ * layers) the product runs as row panels that read each row of A once;
+ * the results are the same bits. */
 #define OPTK_GEMM_BIAS_SILU 1
 #define OPTK_GEMM_SILU_BACK 2
 int32_t optk_gemm_bf16_fused(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k,
@@ -534,7 +537,8 @@ enum {
   OPTK_MLP_ADAM = 8,          /* f32[0] params, f32[1] grads, f32[2] m, f32[3] v (count each);
                                  u16[0] bf16 copy of params (or NULL);
                                  scalars: lr, beta1, beta2, eps, 1 - beta1^t, 1 - beta2^t */
-  OPTK_MLP_TO_BF16 = 9        /* f32[0] x -> u16[0] (rows * cols) */
+  OPTK_MLP_TO_BF16 = 9,       /* f32[0] x -> u16[0] (rows * cols) */
+  OPTK_MLP_TRANSPOSE16 = 10   /* u16[0] x [rows][cols] -> u16[1] y [cols][rows] (distinct) */
 };
 typedef struct optk_mlp_args {
   float *f32[8];

@@ -277,6 +277,7 @@ MLP_OUTER_GRAD = 6
 MLP_COLUMN_SUM = 7
 MLP_ADAM = 8
 MLP_TO_BF16 = 9
+MLP_TRANSPOSE16 = 10
 
 GEMM_BIAS_SILU = 1
 GEMM_SILU_BACK = 2

@@ -134,9 +134,9 @@ __global__ __launch_bounds__(256) void gemm_bf16(const uint16_t *__restrict__ a,
 // 65536 with bf16 operands) are bandwidth-bound on MI355X: 128 to 256 flops
 // per byte against a ridge near 300.  Block tile 128 x 128 x 32, 256 threads,
 // each wave a 64 x 64 quarter as 4 x 4 MFMA 16x16x32 tiles (16 MFMAs per
-// k step).  Operands are staged in LDS with k contiguous, double-buffered:
-// the next k step's 16-byte global loads are issued before this step's
-// MFMAs and land in the other buffer after them, one barrier per step.  An
+// k step).  Operands are staged in LDS with k contiguous, double-buffered,
+// and requested from global memory GPF k steps ahead (16-byte loads into
+// registers), one barrier per step.  An
 // operand stored with k as its row index (TA = 1, TB = 0) is loaded 16 bytes
 // along m (or n) and transposed on its way into LDS.  The epilogue can fuse
 // the layer's elementwise step (bias + SiLU forward, SiLU derivative
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const uint16_t *__restrict__ a,
 // Requires m, n multiples of 128 and k a multiple of 32 * split.
 // ---------------------------------------------------------------------------
 
-constexpr int GM = 128, GN = 128, GK = 32, GPAD = 8;
+constexpr int GM = 128, GN = 128, GK = 32, GPAD = 8, GPF = 4;
 
 enum { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_SILU_BACK = 2 };
 
@@ -156,39 +156,44 @@ struct GemmEpilogue {
   uint16_t *out16;   // BIAS_SILU: bf16 activations; SILU_BACK: bf16 dpre
 };
 
+// 16 bytes as a vector register value (a plain vector type, which the
+// compiler keeps in registers where an array of structs may go to scratch)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // one 128 x 32 operand tile into LDS ([row][k], k contiguous); KC: stored
 // with k contiguous (x[row * ld + k]), else k as the row index (x[k * ld + row])
 template <bool KC> struct TileLoad {
-  uint4 v[2];
-  __device__ __forceinline__ void load(const uint16_t *__restrict__ x, int64_t ld, int64_t row0,
-                                       int64_t k0, int tid) {
+  static __device__ __forceinline__ void load(u32x4 (&v)[2], const uint16_t *__restrict__ x, int64_t ld,
+                                              int64_t row0, int64_t k0, int tid) {
     if (KC) {
       const int r = tid >> 2, kk = (tid & 3) * 8;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
-        v[h] = *reinterpret_cast<const uint4 *>(x + (row0 + r + 64 * h) * ld + k0 + kk);
+        v[h] = *reinterpret_cast<const u32x4 *>(x + (row0 + r + 64 * h) * ld + k0 + kk);
     } else {
-      const int kr = tid >> 4, rr = (tid & 15) * 8;
+      // k rows 2 kp and 2 kp + 1, tile rows 8 r .. 8 r + 7
+      const int kp = tid & 15, r = tid >> 4;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
-        v[h] = *reinterpret_cast<const uint4 *>(x + (k0 + kr + 16 * h) * ld + row0 + rr);
+        v[h] = *reinterpret_cast<const u32x4 *>(x + (k0 + 2 * kp + h) * ld + row0 + 8 * r);
     }
   }
-  __device__ __forceinline__ void store(uint16_t (*t)[GK + GPAD], int tid) const {
+  static __device__ __forceinline__ void store(const u32x4 (&v)[2], uint16_t (*t)[GK + GPAD], int tid) {
     if (KC) {
       const int r = tid >> 2, kk = (tid & 3) * 8;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) *reinterpret_cast<uint4 *>(&t[r + 64 * h][kk]) = v[h];
+      for (int h = 0; h < 2; ++h) *reinterpret_cast<u32x4 *>(&t[r + 64 * h][kk]) = v[h];
     } else {
-      const int kr = tid >> 4, rr = (tid & 15) * 8;
+      // tile row 8 r + j gets the word (k = 2 kp, k = 2 kp + 1): one 32-bit
+      // store per row.  Within a wave kp runs over 16 and r over 4 values,
+      // so a store's 64 lanes hit at most two lanes per bank (rows of 20
+      // words, 8 rows apart: 32 banks apart)
+      const int kp = tid & 15, r = tid >> 4;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const uint32_t w[4] = {v[h].x, v[h].y, v[h].z, v[h].w};
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          t[rr + 2 * j][kr + 16 * h] = (uint16_t)(w[j] & 0xffffu);
-          t[rr + 2 * j + 1][kr + 16 * h] = (uint16_t)(w[j] >> 16);
-        }
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t lo = v[0][q], hi = v[1][q];
+        *reinterpret_cast<uint32_t *>(&t[8 * r + 2 * q][2 * kp]) = (lo & 0xffffu) | (hi << 16);
+        *reinterpret_cast<uint32_t *>(&t[8 * r + 2 * q + 1][2 * kp]) = (lo >> 16) | (hi & 0xffff0000u);
       }
     }
   }
@@ -217,37 +222,59 @@ __global__ __launch_bounds__(256) void gemm_bf16_128(const uint16_t *__restrict_
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-  TileLoad<TA == 0> la;
-  TileLoad<TB == 1> lb;
-  la.load(a, lda, m0, k_begin, tid);
-  lb.load(b, ldb, n0, k_begin, tid);
-  la.store(As[0], tid);
-  lb.store(Bs[0], tid);
-  __syncthreads();
+  // Operand tiles are requested GPF k steps ahead into registers (slot j
+  // holds the steps s with s % GPF == j) and moved into the LDS double
+  // buffer one step ahead of their use: a block keeps GPF x 16 KB of loads
+  // in flight, which HBM's latency needs at these tile sizes.
+  typedef TileLoad<TA == 0> LA;
+  typedef TileLoad<TB == 1> LB;
+  u32x4 ra[GPF][2], rb[GPF][2];
   const int steps = (int)(kc / GK);
-  for (int step = 0; step < steps; ++step) {
-    const int buf = step & 1;
-    const bool more = step + 1 < steps;
-    if (more) {
-      la.load(a, lda, m0, k_begin + (int64_t)(step + 1) * GK, tid);
-      lb.load(b, ldb, n0, k_begin + (int64_t)(step + 1) * GK, tid);
-    }
-    bf16x8 af[4], bfr[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&As[buf][wm + 16 * i + fr][fk]));
-      bfr[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&Bs[buf][wn + 16 * i + fr][fk]));
+  for (int u = 0; u < GPF; ++u)
+    if (u < steps) {
+      LA::load(ra[u], a, lda, m0, k_begin + (int64_t)u * GK, tid);
+      LB::load(rb[u], b, ldb, n0, k_begin + (int64_t)u * GK, tid);
     }
+  LA::store(ra[0], As[0], tid);
+  LB::store(rb[0], Bs[0], tid);
+  if (GPF < steps) {
+    LA::load(ra[0], a, lda, m0, k_begin + (int64_t)GPF * GK, tid);
+    LB::load(rb[0], b, ldb, n0, k_begin + (int64_t)GPF * GK, tid);
+  }
+  __syncthreads();
+  for (int s0 = 0; s0 < steps; s0 += GPF) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int u = 0; u < GPF; ++u) {
+      const int step = s0 + u;
+      if (step < steps) {
+        const int buf = u & 1;  // GPF is even: step & 1
+        bf16x8 af[4], bfr[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
-    if (more) {
-      la.store(As[buf ^ 1], tid);
-      lb.store(Bs[buf ^ 1], tid);
+        for (int i = 0; i < 4; ++i) {
+          af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&As[buf][wm + 16 * i + fr][fk]));
+          bfr[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&Bs[buf][wn + 16 * i + fr][fk]));
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        // step + 1 (slot (u + 1) % GPF) into the other buffer, whose last
+        // reads (step - 1) ended at the previous barrier; the slot then
+        // requests step + 1 + GPF
+        const int nxt = step + 1, slot = (u + 1) % GPF;
+        if (nxt < steps) {
+          LA::store(ra[slot], As[buf ^ 1], tid);
+          LB::store(rb[slot], Bs[buf ^ 1], tid);
+          if (nxt + GPF < steps) {
+            LA::load(ra[slot], a, lda, m0, k_begin + (int64_t)(nxt + GPF) * GK, tid);
+            LB::load(rb[slot], b, ldb, n0, k_begin + (int64_t)(nxt + GPF) * GK, tid);
+          }
+        }
+        __syncthreads();
+      }
     }
-    __syncthreads();
   }
   // lane holds C[row = 4 (lane / 16) + r][col = lane % 16] of each 16 x 16
   // tile.  The tile leaves through LDS, 64 rows at a time, so that every
@@ -337,6 +364,141 @@ __global__ __launch_bounds__(64 * RED_WAVES) void reduce_partials(const float *_
 inline void launch_reduce(const float *partial, int splits, int64_t count, float *out, hipStream_t s) {
   hipLaunchKernelGGL(reduce_partials, dim3((unsigned)((count + 63) / 64)), dim3(64 * RED_WAVES), 0, s,
                      partial, splits, count, out);
+}
+
+// ---------------------------------------------------------------------------
+// Row-panel GEMM of the network's 256-wide layers: out[m][0, 256) =
+// A[m][0, 256) . Bt^T, row j of Bt holding output column j's 256 weights
+// (k contiguous), with the epilogues of gemm_bf16_128.  At k = n = 256 a
+// product moves more bytes than the matrix cores need time for (per row:
+// 512 B of A in, 1.5 KB of pre-activation and activation out, against
+// 131,072 flops), so the kernel is laid out for HBM traffic:
+//  - a block owns a 64-row tile and all 256 output columns, so every A row
+//    is read from HBM once, into LDS (rows padded by 16 halves);
+//  - each of the four waves owns 64 columns; its weights (L2-resident)
+//    stream through a register ring two 32-wide k chunks ahead;
+//  - the epilogue stores straight from the accumulators: the B operand's
+//    columns are permuted so that lane (fr, g) holds columns col .. col + 3
+//    (col = 64 wave + 4 fr) of rows 4 g .. 4 g + 3 of each 16-row tile, and
+//    one store instruction writes four 256-byte (float) or 128-byte (bf16)
+//    row segments;
+//  - the SiLU-derivative epilogue requests its pre-activations before the k
+//    loop, so they arrive while the matrix cores run.
+// Every accumulator sums the same 32-wide k chunks in the same order with
+// the same MFMA as gemm_bf16_128, and the epilogue arithmetic is the same:
+// the results are bit-identical.  Requires m a multiple of 64, k = n = 256.
+// ---------------------------------------------------------------------------
+
+constexpr int PM = 64, PK = 256, PPAD = 16;
+typedef uint16_t PanelRow[PK + PPAD];
+
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_panel(const uint16_t *__restrict__ a, int64_t lda,
+                                                  const uint16_t *__restrict__ bt, int64_t ldb,
+                                                  const GemmEpilogue E) {
+  __shared__ __attribute__((aligned(16))) PanelRow As[PM];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  const int col = wave * 64 + 4 * fr;
+  const int64_t row0 = (int64_t)blockIdx.x * PM;
+  // the tile's A rows: thread t moves 16-byte segments t + 256 j, j < 8
+  // (row (t + 256 j) / 32, segment (t + 256 j) % 32 of it)
+  uint4 st[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int seg = tid + 256 * j, r = seg >> 5, s = seg & 31;
+    st[j] = *reinterpret_cast<const uint4 *>(a + (row0 + r) * lda + 8 * s);
+  }
+  // the lane's B operands of chunk c: rows col + nt of Bt, k = 32 c + fk ..
+  const uint16_t *brow = bt + (int64_t)col * ldb + fk;
+  uint4 ring[3][4];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) ring[c][nt] = *reinterpret_cast<const uint4 *>(brow + nt * ldb + 32 * c);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int seg = tid + 256 * j, r = seg >> 5, s = seg & 31;
+    *reinterpret_cast<uint4 *>(&As[r][8 * s]) = st[j];
+  }
+  float4 bias = float4{0.0f, 0.0f, 0.0f, 0.0f};
+  if (EPI == EPI_BIAS_SILU) bias = *reinterpret_cast<const float4 *>(E.bias + col);
+  __syncthreads();
+  // row of element q of row tile mt in this lane
+  auto row_of = [&](int mt, int q) { return row0 + 16 * mt + 4 * (lane >> 4) + q; };
+  float4 pre[4][4];
+  if (EPI == EPI_SILU_BACK) {
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) pre[mt][q] = *reinterpret_cast<const float4 *>(E.pre + row_of(mt, q) * E.ldc + col);
+  }
+  floatx4 acc[4][4];
+#pragma unroll
+  for (int c = 0; c < PK / 32; ++c) {
+    if (c + 2 < PK / 32) {
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt)
+        ring[(c + 2) % 3][nt] = *reinterpret_cast<const uint4 *>(brow + nt * ldb + 32 * (c + 2));
+    }
+    bf16x8 b[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) b[nt] = __builtin_bit_cast(bf16x8, ring[c % 3][nt]);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      const bf16x8 av = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&As[16 * mt + fr][32 * c + fk]));
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt)
+        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[nt], c == 0 ? floatx4{0.0f, 0.0f, 0.0f, 0.0f} : acc[mt][nt], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t at = row_of(mt, q) * E.ldc + col;
+      float4 v = float4{acc[mt][0][q], acc[mt][1][q], acc[mt][2][q], acc[mt][3][q]};
+      if (EPI == EPI_BIAS_SILU) {
+        v.x += bias.x;
+        v.y += bias.y;
+        v.z += bias.z;
+        v.w += bias.w;
+        *reinterpret_cast<float4 *>(E.c + at) = v;
+        *reinterpret_cast<uint2 *>(E.out16 + at) =
+            uint2{pack_bf16(silu(v.x), silu(v.y)), pack_bf16(silu(v.z), silu(v.w))};
+      } else {
+        const float4 p = pre[mt][q];
+        v.x *= silu_grad(p.x);
+        v.y *= silu_grad(p.y);
+        v.z *= silu_grad(p.z);
+        v.w *= silu_grad(p.w);
+        if (E.c) *reinterpret_cast<float4 *>(E.c + at) = v;
+        *reinterpret_cast<uint2 *>(E.out16 + at) = uint2{pack_bf16(v.x, v.y), pack_bf16(v.z, v.w)};
+      }
+    }
+}
+
+// y[c][r] = x[r][c] for a rows x cols array of 16-bit values: 32 x 32
+// tiles through LDS (row padded by one to spread the banks)
+__global__ __launch_bounds__(256) void transpose16(const uint16_t *__restrict__ x, uint16_t *__restrict__ y,
+                                                   int64_t rows, int cols) {
+  __shared__ uint16_t t[32][33];
+  const int64_t r0 = (int64_t)blockIdx.y * 32;
+  const int c0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+  for (int j = 0; j < 32; j += 8) {
+    const int64_t r = r0 + ty + j;
+    const int c = c0 + tx;
+    if (r < rows && c < cols) t[ty + j][tx] = x[r * cols + c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 32; j += 8) {
+    const int c = c0 + ty + j;
+    const int64_t r = r0 + tx;
+    if (r < rows && c < cols) y[(int64_t)c * rows + r] = t[tx][ty + j];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1395,8 +1557,18 @@ int32_t optk_gemm_bf16_fused(int32_t trans_a, int32_t trans_b, int64_t m, int64_
   E.out16 = (uint16_t *)out16;
   const uint16_t *A = (const uint16_t *)a, *B = (const uint16_t *)b;
   hipStream_t stream = (hipStream_t)hip_stream;
+  if (!trans_a && trans_b && n == PK && k == PK && m % PM == 0) {
+    // the network's 256-wide layers: row panels (every A row read once)
+    if (epilogue == OPTK_GEMM_BIAS_SILU)
+      hipLaunchKernelGGL(gemm_panel<EPI_BIAS_SILU>, dim3((unsigned)(m / PM)), dim3(256), 0, stream, A, lda, B, ldb, E);
+    else
+      hipLaunchKernelGGL(gemm_panel<EPI_SILU_BACK>, dim3((unsigned)(m / PM)), dim3(256), 0, stream, A, lda, B, ldb, E);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("gemm_fused launch: ") + hipGetErrorString(e));
+    return OPTK_OK;
+  }
   dim3 grid((unsigned)(n / GN), (unsigned)(m / GM), 1);
-#define OPTK_GEMM_FUSED(TA, TB)                                                                    \
+#define OPTK_GEMM_FUSED(TA, TB)                                                                  \
   if (epilogue == OPTK_GEMM_BIAS_SILU)                                                             \
     hipLaunchKernelGGL((gemm_bf16_128<TA, TB, EPI_BIAS_SILU>), grid, dim3(256), 0, stream, A, lda, B, ldb, E, m, n, k); \
   else                                                                                             \
@@ -1531,6 +1703,11 @@ int32_t optk_mlp_op(int32_t op, const optk_mlp_args *A, void *hip_stream) {
     NEED(A->f32[0] && A->u16[0], "to_bf16: x, y");
     hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(count)), dim3(256), 0, s, A->f32[0],
                        A->u16[0], count);
+    break;
+  case OPTK_MLP_TRANSPOSE16:
+    NEED(A->u16[0] && A->u16[1] && A->u16[0] != A->u16[1], "transpose16: x, y (distinct)");
+    hipLaunchKernelGGL(transpose16, dim3((unsigned)((A->cols + 31) / 32), (unsigned)((rows + 31) / 32)), dim3(256),
+                       0, s, A->u16[0], A->u16[1], rows, A->cols);
     break;
   default:
     return fail(OPTK_EINVAL, "unknown mlp op");

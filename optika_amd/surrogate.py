@@ -154,6 +154,9 @@ class MLP:
                 self.dpre16 = h(rows, H)
                 self.dpre16_b = h(rows, H)
                 self.dpre32 = f(rows, H)
+                # the hidden layers' bf16 weights transposed ([in][out]): the
+                # backward product's B operand with k contiguous
+                self.wt16 = [h(H, H) for _ in range(3)]
                 self.ones = torch.ones(rows, 1, dtype=torch.float32, device=self.dev)
                 # split the weight-gradient GEMM's k = rows so that its four
                 # 128 x 128 output tiles make ~2 blocks per CU
@@ -209,6 +212,9 @@ class MLP:
         _op(_lib.MLP_DENSE_OUT_BACK, f32=[self.dy, self.view(self.p, 4, 0), self.pre[3],
                                           self.dpre32], u16=[self.dpre16], rows=rows, cols=H,
             inner=3)
+        for layer in (1, 2, 3):
+            _op(_lib.MLP_TRANSPOSE16, u16=[self.view(self.p16, layer, 0), self.wt16[layer - 1]],
+                rows=H, cols=H)
         d16, d16_next = self.dpre16, self.dpre16_b
         for layer in (3, 2, 1):
             # weight gradient: gW[o][i] = sum_r dpre[r][o] act[r][i]
@@ -218,7 +224,7 @@ class MLP:
                 rows=rows, cols=H, inner=1, count=self.ws.numel(), flag=1)
             # back to the previous layer: dpre = (dpre W) * silu'(pre), the
             # derivative in the GEMM's epilogue
-            _gemm_fused(0, 0, rows, H, H, d16, H, self.view(self.p16, layer, 0), H, self.dpre32, H,
+            _gemm_fused(0, 1, rows, H, H, d16, H, self.wt16[layer - 1], H, self.dpre32, H,
                         _lib.GEMM_SILU_BACK, d16_next, pre=self.pre[layer - 1])
             d16, d16_next = d16_next, d16
         # first layer (float32)

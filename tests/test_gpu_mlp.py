@@ -38,6 +38,51 @@ def test_gemm_bf16_matches_float64(ta, tb, split):
     assert np.all(np.abs(got - want) <= bound), np.max(np.abs(got - want) / bound)
 
 
+@pytest.mark.parametrize("back", [False, True])
+def test_row_panel_gemm_matches_tiled_path(back):
+    """The hidden layers' row-panel product (ta = 0, tb = 1, n = k = 256)
+    gives the same bits as the 128 x 128 tiled kernel on the same operands
+    (b stored k-major through the transpose op, tb = 0), and the float64
+    product of the same bf16 values within accumulation rounding, for both
+    fused epilogues."""
+    rng = np.random.default_rng(70 + back)
+    m, n, k = 384, 256, 256
+    A, a = _bf16(rng, (m, k), 0.5)
+    Bt, bt = _bf16(rng, (n, k), 0.1)  # row j: output column j's weights
+    b = torch.empty(k, n, dtype=torch.int16, device="cuda")
+    surrogate._op(surrogate._lib.MLP_TRANSPOSE16, u16=[bt, b], rows=n, cols=k)
+    assert np.array_equal(b.cpu().numpy(), bt.cpu().numpy().T)
+    bias64 = rng.normal(0, 0.1, n).astype(np.float32).astype(np.float64)
+    pre64 = rng.normal(0, 2, (m, n)).astype(np.float32).astype(np.float64)
+    bias = torch.as_tensor(bias64, dtype=torch.float32, device="cuda")
+    pre = torch.as_tensor(pre64, dtype=torch.float32, device="cuda")
+    epi = surrogate._lib.GEMM_SILU_BACK if back else surrogate._lib.GEMM_BIAS_SILU
+    print(f"case: row-panel gemm epilogue {epi}", flush=True)
+    outs = []
+    for tb, bm in ((1, bt), (0, b)):
+        c = torch.empty(m, n, dtype=torch.float32, device="cuda")
+        o16 = torch.empty(m, n, dtype=torch.int16, device="cuda")
+        surrogate._gemm_fused(0, tb, m, n, k, a, k, bm, k if tb else n, c, n, epi, o16,
+                              bias=None if back else bias, pre=pre if back else None)
+        outs.append((c.cpu().numpy(), o16.cpu().numpy()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    got = outs[0][0].astype(np.float64)
+    mag = np.abs(A) @ np.abs(Bt).T
+    if back:
+        # silu' through the hardware exp and reciprocal: ~1e-7 absolute
+        # (it crosses zero at x = -1.28, so no relative bound holds there)
+        g = ref_mlp.silu_grad(pre64)
+        want, bound = (A @ Bt.T) * g, (1e-5 * mag + 1e-6) * np.abs(g) + 1e-6 * np.abs(A @ Bt.T)
+        act = got
+    else:
+        want, bound = A @ Bt.T + bias64, 1e-5 * mag + 1e-6
+        act = ref_mlp.silu(got)
+    assert np.all(np.abs(got - want) <= bound), np.max(np.abs(got - want) / bound)
+    o = (outs[0][1].astype(np.uint16).astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    assert np.all(np.abs(o - act) <= 2.0 ** -8 * np.abs(act) + 1e-30)
+
+
 def test_training_gradients_match_cpu_network():
     rng = np.random.default_rng(3)
     dev = torch.device("cuda")
