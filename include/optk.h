@@ -517,6 +517,21 @@ int32_t optk_gemm_bf16_fused(int32_t trans_a, int32_t trans_b, int64_t m, int64_
                              int64_t ldc, int32_t epilogue, const float *bias, const float *pre,
                              void *out16, void *hip_stream);
 
+/* optk_gemm_bf16_fused's OPTK_GEMM_SILU_BACK for the surrogate's 256-wide
+ * layers (trans_a = 0, trans_b = 1, n = k = 256) that also sums the float32
+ * result over the rows -- the previous layer's bias gradient:
+ *   d[i][j] = (A B)[i][j] * silu'(pre[i][j]), A(i, k) = a[i * lda + k],
+ *   B(k, j) = b[j * ldb + k];
+ *   out16[i][j] = bf16(d[i][j]); c[i][j] = d[i][j] (c may be NULL: not
+ *   stored); sums[j] = sum_i d[i][j] (float32, added in a fixed order).
+ * m a positive multiple of 64; pre, c and out16 are [m][ldc]; workspace
+ * holds (m / 64) * 256 floats of per-tile partial sums.  lda, ldb >= 256
+ * and multiples of 8, ldc >= 256 and a multiple of 4; a, b, pre, c and
+ * workspace 16-byte aligned, out16 8-byte. */
+int32_t optk_gemm_bf16_silu_back_sums(int64_t m, const void *a, int64_t lda, const void *b, int64_t ldb,
+                                      const float *pre, int64_t ldc, float *c, void *out16, float *sums,
+                                      float *workspace, void *hip_stream);
+
 /* elementwise and small-layer pieces of the forward and backward passes
  * (argument use per op below; rows x cols row-major arrays) */
 enum {
@@ -528,7 +543,10 @@ enum {
   OPTK_MLP_MSE_GRAD = 3,      /* f32[0] y, f32[1] target, f32[2] weight per column, f32[3] dy,
                                  f32[4] loss partial per 256 elements */
   OPTK_MLP_DENSE_OUT_BACK = 4,/* f32[0] dy [rows][inner], f32[1] w [inner][cols], f32[2] pre;
-                                 u16[0] dpre (bf16), f32[3] dpre (float32) */
+                                 u16[0] dpre (bf16), f32[3] dpre (float32, or NULL), f32[4]
+                                 column sums of dpre (the bias gradient, or NULL; cols = 256,
+                                 inner = 3), f32[5] their workspace of count floats (at least
+                                 min(1024, ceil(rows / 4)) * 256) */
   OPTK_MLP_SILU_BACK = 5,     /* f32[0] dact, f32[1] pre; u16[0] dpre, f32[2] dpre (float32) */
   OPTK_MLP_OUTER_GRAD = 6,    /* out = sum_r x[r][c] g[r][j]: f32[0] x (or u16[0] bf16 x),
                                  f32[1] g [rows][inner], f32[2] out ([inner][cols], or

@@ -347,6 +347,9 @@ def lib() -> ctypes.CDLL:
     L.optk_gemm_bf16_fused.argtypes = [c_int32, c_int32, c_int64, c_int64, c_int64, c_void_p, c_int64,
                                        c_void_p, c_int64, c_void_p, c_int64, c_int32, c_void_p,
                                        c_void_p, c_void_p, c_void_p]
+    L.optk_gemm_bf16_silu_back_sums.restype = c_int32
+    L.optk_gemm_bf16_silu_back_sums.argtypes = [c_int64, c_void_p, c_int64, c_void_p, c_int64, c_void_p,
+                                                c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.optk_mlp_op.restype = c_int32
     L.optk_mlp_op.argtypes = [c_int32, ctypes.POINTER(MlpArgs), c_void_p]
     L.optk_surrogate_infer.restype = c_int32

@@ -154,6 +154,7 @@ struct GemmEpilogue {
   const float *bias; // BIAS_SILU
   const float *pre;  // SILU_BACK: pre-activation in, [m][ldc]
   uint16_t *out16;   // BIAS_SILU: bf16 activations; SILU_BACK: bf16 dpre
+  float *sums = nullptr;  // row-panel SILU_BACK: per-tile column sums of dpre ([tile][256]), or null
 };
 
 // 16 bytes as a vector register value (a plain vector type, which the
@@ -383,17 +384,19 @@ inline void launch_reduce(const float *partial, int splits, int64_t count, float
 //    one store instruction writes four 256-byte (float) or 128-byte (bf16)
 //    row segments;
 //  - the SiLU-derivative epilogue requests its pre-activations before the k
-//    loop, so they arrive while the matrix cores run.
+//    loop, so they arrive while the matrix cores run, and can sum its
+//    float32 results over the tile's rows (the previous layer's bias
+//    gradient, one partial per tile, added up by reduce_partials).
 // Every accumulator sums the same 32-wide k chunks in the same order with
 // the same MFMA as gemm_bf16_128, and the epilogue arithmetic is the same:
 // the results are bit-identical.  Requires m a multiple of 64, k = n = 256.
 // ---------------------------------------------------------------------------
 
-constexpr int PM = 64, PK = 256, PPAD = 16;
+constexpr int PMT = 4, PM = 16 * PMT, PK = 256, PPAD = 16;  // PMT 16-row tiles per block
 typedef uint16_t PanelRow[PK + PPAD];
 
 template <int EPI>
-__global__ __launch_bounds__(256) void gemm_panel(const uint16_t *__restrict__ a, int64_t lda,
+__global__ __launch_bounds__(256, 2) void gemm_panel(const uint16_t *__restrict__ a, int64_t lda,
                                                   const uint16_t *__restrict__ bt, int64_t ldb,
                                                   const GemmEpilogue E) {
   __shared__ __attribute__((aligned(16))) PanelRow As[PM];
@@ -401,11 +404,11 @@ __global__ __launch_bounds__(256) void gemm_panel(const uint16_t *__restrict__ a
   const int fr = lane & 15, fk = (lane >> 4) * 8;
   const int col = wave * 64 + 4 * fr;
   const int64_t row0 = (int64_t)blockIdx.x * PM;
-  // the tile's A rows: thread t moves 16-byte segments t + 256 j, j < 8
+  // the tile's A rows: thread t moves 16-byte segments t + 256 j, j < 2 PMT
   // (row (t + 256 j) / 32, segment (t + 256 j) % 32 of it)
-  uint4 st[8];
+  uint4 st[2 * PMT];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
+  for (int j = 0; j < 2 * PMT; ++j) {
     const int seg = tid + 256 * j, r = seg >> 5, s = seg & 31;
     st[j] = *reinterpret_cast<const uint4 *>(a + (row0 + r) * lda + 8 * s);
   }
@@ -417,7 +420,7 @@ __global__ __launch_bounds__(256) void gemm_panel(const uint16_t *__restrict__ a
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) ring[c][nt] = *reinterpret_cast<const uint4 *>(brow + nt * ldb + 32 * c);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
+  for (int j = 0; j < 2 * PMT; ++j) {
     const int seg = tid + 256 * j, r = seg >> 5, s = seg & 31;
     *reinterpret_cast<uint4 *>(&As[r][8 * s]) = st[j];
   }
@@ -426,14 +429,15 @@ __global__ __launch_bounds__(256) void gemm_panel(const uint16_t *__restrict__ a
   __syncthreads();
   // row of element q of row tile mt in this lane
   auto row_of = [&](int mt, int q) { return row0 + 16 * mt + 4 * (lane >> 4) + q; };
-  float4 pre[4][4];
+  float4 pre[PMT][4];
   if (EPI == EPI_SILU_BACK) {
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+    for (int mt = 0; mt < PMT; ++mt)
 #pragma unroll
       for (int q = 0; q < 4; ++q) pre[mt][q] = *reinterpret_cast<const float4 *>(E.pre + row_of(mt, q) * E.ldc + col);
   }
-  floatx4 acc[4][4];
+  floatx4 acc[PMT][4];
+  float4 csum = float4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int c = 0; c < PK / 32; ++c) {
     if (c + 2 < PK / 32) {
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(256) void gemm_panel(const uint16_t *__restrict__ a
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) b[nt] = __builtin_bit_cast(bf16x8, ring[c % 3][nt]);
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
+    for (int mt = 0; mt < PMT; ++mt) {
       const bf16x8 av = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&As[16 * mt + fr][32 * c + fk]));
 #pragma unroll
       for (int nt = 0; nt < 4; ++nt)
@@ -453,7 +457,7 @@ __global__ __launch_bounds__(256) void gemm_panel(const uint16_t *__restrict__ a
     }
   }
 #pragma unroll
-  for (int mt = 0; mt < 4; ++mt)
+  for (int mt = 0; mt < PMT; ++mt)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int64_t at = row_of(mt, q) * E.ldc + col;
@@ -474,8 +478,24 @@ __global__ __launch_bounds__(256) void gemm_panel(const uint16_t *__restrict__ a
         v.w *= silu_grad(p.w);
         if (E.c) *reinterpret_cast<float4 *>(E.c + at) = v;
         *reinterpret_cast<uint2 *>(E.out16 + at) = uint2{pack_bf16(v.x, v.y), pack_bf16(v.z, v.w)};
+        csum.x += v.x;
+        csum.y += v.y;
+        csum.z += v.z;
+        csum.w += v.w;
       }
     }
+  if (EPI == EPI_SILU_BACK && E.sums) {
+    // the lane's 4 PMT rows, then the four row groups (lanes fr, fr + 16,
+    // fr + 32, fr + 48) in a fixed order
+#pragma unroll
+    for (int o = 16; o <= 32; o *= 2) {
+      csum.x += __shfl_xor(csum.x, o, 64);
+      csum.y += __shfl_xor(csum.y, o, 64);
+      csum.z += __shfl_xor(csum.z, o, 64);
+      csum.w += __shfl_xor(csum.w, o, 64);
+    }
+    if (lane < 16) *reinterpret_cast<float4 *>(E.sums + (int64_t)blockIdx.x * PK + col) = csum;
+  }
 }
 
 // y[c][r] = x[r][c] for a rows x cols array of 16-bit values: 32 x 32
@@ -691,35 +711,98 @@ __global__ __launch_bounds__(256) void dense_out_wave(const uint16_t *__restrict
   }
 }
 
-// dense_out_back for OUT outputs, hidden a multiple of 4
+// dense_out_back for OUT outputs and 256 hidden features: a wave takes rows
+// in turn (lane l: features 4 l .. 4 l + 3), writes the bf16 dpre and, if
+// asked, the float32 one, and sums its rows' float32 dpre per feature (the
+// layer's bias gradient); the block's four waves add theirs in a fixed
+// order into partial[block][256]
+constexpr int DOB_BLOCKS = 1024;
+
 template <int OUT>
 __global__ __launch_bounds__(256) void dense_out_back4(const float *__restrict__ dy,
                                                        const float *__restrict__ w,
                                                        const float *__restrict__ pre,
                                                        uint16_t *__restrict__ dpre,
                                                        float *__restrict__ dpre32, int64_t rows,
-                                                       int hidden) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int hq = hidden / 4;
-  if (q >= rows * hq) return;
-  const int64_t r = q / hq;
-  const int h = (int)(q % hq) * 4;
-  float d[OUT];
+                                                       float *__restrict__ partial) {
+  __shared__ float4 red[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = 4 * lane;
+  float wl[OUT][4];
 #pragma unroll
-  for (int j = 0; j < OUT; ++j) d[j] = dy[r * OUT + j];
-  const int64_t at = r * hidden + h;
-  const float4 p = *reinterpret_cast<const float4 *>(pre + at);
-  const float pv[4] = {p.x, p.y, p.z, p.w};
-  float g[4];
+  for (int j = 0; j < OUT; ++j)
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    float s = 0.0f;
+    for (int u = 0; u < 4; ++u) wl[j][u] = w[j * 256 + h + u];
+  float4 cs = float4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < rows; r += nw) {
+    float d[OUT];
 #pragma unroll
-    for (int j = 0; j < OUT; ++j) s += d[j] * w[j * hidden + h + u];
-    g[u] = s * silu_grad(pv[u]);
+    for (int j = 0; j < OUT; ++j) d[j] = dy[r * OUT + j];
+    const int64_t at = r * 256 + h;
+    const float4 p = *reinterpret_cast<const float4 *>(pre + at);
+    const float pv[4] = {p.x, p.y, p.z, p.w};
+    float g[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      float sum = 0.0f;
+#pragma unroll
+      for (int j = 0; j < OUT; ++j) sum += d[j] * wl[j][u];
+      g[u] = sum * silu_grad(pv[u]);
+    }
+    *reinterpret_cast<uint2 *>(dpre + at) = uint2{pack_bf16(g[0], g[1]), pack_bf16(g[2], g[3])};
+    if (dpre32) *reinterpret_cast<float4 *>(dpre32 + at) = float4{g[0], g[1], g[2], g[3]};
+    cs.x += g[0];
+    cs.y += g[1];
+    cs.z += g[2];
+    cs.w += g[3];
   }
-  *reinterpret_cast<uint2 *>(dpre + at) = uint2{pack_bf16(g[0], g[1]), pack_bf16(g[2], g[3])};
-  *reinterpret_cast<float4 *>(dpre32 + at) = float4{g[0], g[1], g[2], g[3]};
+  if (!partial) return;
+  red[wave][lane] = cs;
+  __syncthreads();
+  if (wave == 0) {
+    float4 t = red[0][lane];
+#pragma unroll
+    for (int v = 1; v < 4; ++v) {
+      t.x += red[v][lane].x;
+      t.y += red[v][lane].y;
+      t.z += red[v][lane].z;
+      t.w += red[v][lane].w;
+    }
+    *reinterpret_cast<float4 *>(partial + (int64_t)blockIdx.x * 256 + h) = t;
+  }
+}
+
+// column sums of a narrow float32 array (COLS <= 8 columns) times one
+// column of g: out[c] = sum_r x[r][c] g[r], per block over its rows, the
+// block's waves added in a fixed order into partial[block][COLS]
+template <int COLS>
+__global__ __launch_bounds__(256) void narrow_outer1(const float *__restrict__ x, const float *__restrict__ g,
+                                                     int64_t rows, int64_t rows_per_block,
+                                                     float *__restrict__ partial) {
+  __shared__ float red[4][COLS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float acc[COLS];
+#pragma unroll
+  for (int c = 0; c < COLS; ++c) acc[c] = 0.0f;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) {
+    const float gv = g[r];
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) acc[c] += x[r * COLS + c] * gv;
+  }
+#pragma unroll
+  for (int c = 0; c < COLS; ++c)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) red[wave][c] = acc[c];
+  __syncthreads();
+  if (threadIdx.x < COLS)
+    partial[(int64_t)blockIdx.x * COLS + threadIdx.x] =
+        (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 // outer_grad for cols a multiple of 4 (<= 256) and INNER columns of g:
@@ -1588,6 +1671,36 @@ int32_t optk_gemm_bf16_fused(int32_t trans_a, int32_t trans_b, int64_t m, int64_
   return OPTK_OK;
 }
 
+int32_t optk_gemm_bf16_silu_back_sums(int64_t m, const void *a, int64_t lda, const void *b, int64_t ldb,
+                                      const float *pre, int64_t ldc, float *c, void *out16, float *sums,
+                                      float *workspace, void *hip_stream) {
+  if (!a || !b || !pre || !out16 || !sums || !workspace)
+    return fail(OPTK_EINVAL, "gemm_silu_back_sums: null operand");
+  if (m <= 0 || m % PM) return fail(OPTK_EINVAL, "gemm_silu_back_sums: m must be a positive multiple of 64");
+  if (lda < PK || ldb < PK || ldc < PK || lda % 8 || ldb % 8 || ldc % 4)
+    return fail(OPTK_EINVAL, "gemm_silu_back_sums: leading dimensions below 256 or misaligned");
+  if ((uintptr_t)a % 16 || (uintptr_t)b % 16 || (uintptr_t)pre % 16 || (uintptr_t)c % 16 ||
+      (uintptr_t)out16 % 8 || (uintptr_t)workspace % 16)
+    return fail(OPTK_EINVAL, "gemm_silu_back_sums: a, b, pre, c and workspace need 16-byte, out16 8-byte alignment");
+  GemmEpilogue E;
+  E.c = c;
+  E.ldc = ldc;
+  E.bias = nullptr;
+  E.pre = pre;
+  E.out16 = (uint16_t *)out16;
+  E.sums = workspace;
+  hipStream_t stream = (hipStream_t)hip_stream;
+  hipLaunchKernelGGL(gemm_panel<EPI_SILU_BACK>, dim3((unsigned)(m / PM)), dim3(256), 0, stream,
+                     (const uint16_t *)a, lda, (const uint16_t *)b, ldb, E);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) {
+    launch_reduce(workspace, (int)(m / PM), PK, sums, stream);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("gemm_silu_back_sums launch: ") + hipGetErrorString(e));
+  return OPTK_OK;
+}
+
 int32_t optk_mlp_op(int32_t op, const optk_mlp_args *A, void *hip_stream) {
   if (!A) return fail(OPTK_EINVAL, "null argument");
   hipStream_t s = (hipStream_t)hip_stream;
@@ -1634,14 +1747,21 @@ int32_t optk_mlp_op(int32_t op, const optk_mlp_args *A, void *hip_stream) {
                        A->f32[2], A->f32[3], A->cols, rows, A->f32[4]);
     break;
   case OPTK_MLP_DENSE_OUT_BACK:
-    NEED(A->f32[0] && A->f32[1] && A->f32[2] && A->u16[0] && A->f32[3], "dense_out_back: dy, w, pre, dpre, dpre32");
-    if (A->inner == 3 && A->cols % 4 == 0 && (uintptr_t)A->f32[2] % 16 == 0 && (uintptr_t)A->f32[3] % 16 == 0 &&
-        (uintptr_t)A->u16[0] % 8 == 0)
-      hipLaunchKernelGGL(dense_out_back4<3>, dim3(blocks_for(count / 4)), dim3(256), 0, s, A->f32[0],
-                         A->f32[1], A->f32[2], A->u16[0], A->f32[3], rows, A->cols);
-    else
+    NEED(A->f32[0] && A->f32[1] && A->f32[2] && A->u16[0] && (A->f32[3] || A->f32[4]),
+         "dense_out_back: dy, w, pre, dpre, and dpre32 or sums");
+    NEED(!A->f32[4] || A->f32[5], "dense_out_back: sums need a workspace");
+    if (A->inner == 3 && A->cols == 256 && (uintptr_t)A->f32[2] % 16 == 0 && (uintptr_t)A->f32[3] % 16 == 0 &&
+        (uintptr_t)A->u16[0] % 8 == 0 && (uintptr_t)A->f32[5] % 16 == 0) {
+      const int64_t nblk = std::min<int64_t>(DOB_BLOCKS, (rows + 3) / 4);
+      NEED(!A->f32[4] || A->count >= nblk * 256, "dense_out_back: workspace too small");
+      hipLaunchKernelGGL(dense_out_back4<3>, dim3((unsigned)nblk), dim3(256), 0, s, A->f32[0], A->f32[1],
+                         A->f32[2], A->u16[0], A->f32[3], rows, A->f32[4] ? A->f32[5] : nullptr);
+      if (A->f32[4] && hipGetLastError() == hipSuccess) launch_reduce(A->f32[5], (int)nblk, 256, A->f32[4], s);
+    } else {
+      NEED(A->f32[3] && !A->f32[4], "dense_out_back: this shape needs dpre32 and no sums");
       hipLaunchKernelGGL(dense_out_back, dim3(blocks_for(count)), dim3(256), 0, s, A->f32[0], A->inner,
                          A->f32[1], A->f32[2], A->u16[0], A->f32[3], rows, A->cols);
+    }
     break;
   case OPTK_MLP_SILU_BACK:
     NEED(A->f32[0] && A->f32[1] && A->u16[0] && A->f32[2], "silu_back: dact, pre, dpre, dpre32");
@@ -1668,7 +1788,10 @@ int32_t optk_mlp_op(int32_t op, const optk_mlp_args *A, void *hip_stream) {
   else                                                                                             \
     hipLaunchKernelGGL((outer_grad4<IN, false>), dim3((unsigned)nblk), dim3(256), 0, s, A->f32[0], nullptr, \
                        A->cols, A->f32[1], rows, rpb, A->flag, A->f32[3]);
-    if (vec && A->inner == 1) {
+    if (!x16 && A->cols == 3 && A->inner == 1) {
+      hipLaunchKernelGGL(narrow_outer1<3>, dim3((unsigned)nblk), dim3(256), 0, s, A->f32[0], A->f32[1], rows,
+                         rpb, A->f32[3]);
+    } else if (vec && A->inner == 1) {
       OPTK_OUTER4(1)
     } else if (vec && A->inner == 3) {
       OPTK_OUTER4(3)

@@ -107,6 +107,15 @@ def _gemm_fused(ta, tb, m, n, k, a, lda, b, ldb, c, ldc, epilogue, out16, bias=N
                                                _stream()))
 
 
+def _gemm_silu_back_sums(m, a, lda, b, ldb, pre, ldc, c, out16, sums, ws):
+    """Backward product of a 256-wide layer with the SiLU derivative, the
+    bf16 (and optionally float32) result and its column sums -- the
+    previous layer's bias gradient (optk_gemm_bf16_silu_back_sums)."""
+    _lib.check(_lib.lib().optk_gemm_bf16_silu_back_sums(m, _ptr(a), lda, _ptr(b), ldb, _ptr(pre), ldc,
+                                                        _ptr(c), _ptr(out16), _ptr(sums), _ptr(ws),
+                                                        _stream()))
+
+
 class MLP:
     """Parameters (float32 master + bf16 copy) and Adam state of the
     network, with forward/backward passes on the library's kernels."""
@@ -209,9 +218,12 @@ class MLP:
             u16=[self.act[3]], rows=rows, cols=H, inner=3, count=self.ws.numel(), flag=0)
         _op(_lib.MLP_OUTER_GRAD, f32=[self.dy, self.ones, self.view(g, 4, 1), self.ws],
             rows=rows, cols=3, inner=1, count=self.ws.numel(), flag=1)
-        _op(_lib.MLP_DENSE_OUT_BACK, f32=[self.dy, self.view(self.p, 4, 0), self.pre[3],
-                                          self.dpre32], u16=[self.dpre16], rows=rows, cols=H,
-            inner=3)
+        # the last hidden layer's dpre (bf16) and its column sums, that
+        # layer's bias gradient; the other bias gradients come with the
+        # backward products below
+        _op(_lib.MLP_DENSE_OUT_BACK, f32=[self.dy, self.view(self.p, 4, 0), self.pre[3], None,
+                                          self.view(g, 3, 1), self.ws], u16=[self.dpre16],
+            rows=rows, cols=H, inner=3, count=self.ws.numel())
         for layer in (1, 2, 3):
             _op(_lib.MLP_TRANSPOSE16, u16=[self.view(self.p16, layer, 0), self.wt16[layer - 1]],
                 rows=H, cols=H)
@@ -220,18 +232,17 @@ class MLP:
             # weight gradient: gW[o][i] = sum_r dpre[r][o] act[r][i]
             _gemm(1, 0, H, H, rows, d16, H, self.act[layer - 1], H, self.view(g, layer, 0),
                   H, self.split, self.ws)
-            _op(_lib.MLP_OUTER_GRAD, f32=[self.dpre32, self.ones, self.view(g, layer, 1), self.ws],
-                rows=rows, cols=H, inner=1, count=self.ws.numel(), flag=1)
             # back to the previous layer: dpre = (dpre W) * silu'(pre), the
-            # derivative in the GEMM's epilogue
-            _gemm_fused(0, 1, rows, H, H, d16, H, self.wt16[layer - 1], H, self.dpre32, H,
-                        _lib.GEMM_SILU_BACK, d16_next, pre=self.pre[layer - 1])
+            # derivative in the GEMM's epilogue with the column sums (that
+            # layer's bias gradient); the float32 dpre is kept only for the
+            # first layer's weight gradient
+            _gemm_silu_back_sums(rows, d16, H, self.wt16[layer - 1], H, self.pre[layer - 1], H,
+                                 self.dpre32 if layer == 1 else None, d16_next,
+                                 self.view(g, layer - 1, 1), self.ws)
             d16, d16_next = d16_next, d16
         # first layer (float32)
         _op(_lib.MLP_OUTER_GRAD, f32=[self.dpre32, x, self.view(g, 0, 0), self.ws], rows=rows,
             cols=H, inner=5, count=self.ws.numel(), flag=1)
-        _op(_lib.MLP_OUTER_GRAD, f32=[self.dpre32, self.ones, self.view(g, 0, 1), self.ws],
-            rows=rows, cols=H, inner=1, count=self.ws.numel(), flag=1)
         return self.loss_part.sum()
 
     def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):

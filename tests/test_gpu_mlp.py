@@ -67,6 +67,23 @@ def test_row_panel_gemm_matches_tiled_path(back):
         outs.append((c.cpu().numpy(), o16.cpu().numpy()))
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
+    if back:
+        # the same product with the column sums of its float32 result (the
+        # previous layer's bias gradient), with and without that result
+        ws = torch.empty(m // 64 * 256, dtype=torch.float32, device="cuda")
+        for keep in (True, False):
+            c = torch.full((m, n), 7.0, dtype=torch.float32, device="cuda")
+            o16 = torch.empty(m, n, dtype=torch.int16, device="cuda")
+            sums = torch.empty(n, dtype=torch.float32, device="cuda")
+            surrogate._gemm_silu_back_sums(m, a, k, bt, k, pre, n, c if keep else None, o16, sums, ws)
+            assert np.array_equal(o16.cpu().numpy(), outs[0][1])
+            if keep:
+                assert np.array_equal(c.cpu().numpy(), outs[0][0])
+            else:
+                assert torch.all(c == 7.0).item()
+            col = outs[0][0].astype(np.float64)
+            err = np.abs(sums.cpu().numpy() - col.sum(0))
+            assert np.all(err <= 1e-6 * np.abs(col).sum(0) + 1e-30), err.max()
     got = outs[0][0].astype(np.float64)
     mag = np.abs(A) @ np.abs(Bt).T
     if back:
