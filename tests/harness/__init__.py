@@ -105,6 +105,14 @@ def trace_grid(system, grid, sensor_frame: bool = True):
     return res
 
 
+def small_angle_sincos(x):
+    """The float64 kernels' series sin and cos for |x| <= 0.01 rad."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    s, c = np.empty_like(x), np.empty_like(x)
+    lib().host_small_angle_sincos(_dp(x), _dp(s), _dp(c), ctypes.c_int64(x.size))
+    return s, c
+
+
 def fast_skip_first(system, sensor_frame: bool = True) -> bool:
     """Whether the float32 form's grid kernels elide the system's surface 0
     (a pass-through plane the grid rays start on)."""

@@ -175,6 +175,11 @@ int host_trace_grid_f32(const optk_system_desc *d, const float *fx, const float 
   return 0;
 }
 
+// the float64 kernels' small-angle sin and cos (direction from field angles)
+void host_small_angle_sincos(const double *x, double *s, double *c, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) small_angle_sincos(x[i], s[i], c[i]);
+}
+
 // 1 when the float32 form's grid kernels elide surface 0 of the system
 // (FastSpec::skip_first), else 0.
 int host_fast_skip_first(const optk_system_desc *d) {
