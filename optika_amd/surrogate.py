@@ -435,7 +435,9 @@ class Surrogate:
     def prepare(self, grid_fn, n_train: int, chunk: int = 1 << 22, poly_rays: int = 1 << 22):
         """Trace ``n_train`` rays from ``grid_fn(count, seed)`` with the
         float64 trace, fit the polynomial, and keep the network's inputs
-        and normalised residual targets on the device."""
+        and normalised residual targets on the device.  ``grid_fn`` returns
+        a grid of host arrays, or the five input columns (field x, y, pupil
+        x, y, wavelength) as device tensors already."""
         dev = self.dev
         us, ts, oks = [], [], []
         done, seed = 0, 1000
@@ -443,8 +445,11 @@ class Surrogate:
             m = min(chunk, n_train - done)
             g = grid_fn(m, seed)
             seed += 1
-            t = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=dev).double()  # noqa: E731
-            fx, fy, px, py, wl = t(g.field_x), t(g.field_y), t(g.pupil_x), t(g.pupil_y), t(g.wavelength)
+            if isinstance(g, (list, tuple)):
+                fx, fy, px, py, wl = [torch.as_tensor(a, device=dev).double() for a in g]
+            else:
+                t = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=dev).double()  # noqa: E731
+                fx, fy, px, py, wl = t(g.field_x), t(g.field_y), t(g.pupil_x), t(g.pupil_y), t(g.wavelength)
             d = self.trace64(fx, fy, px, py, wl)
             u = self.normalised(fx, fy, px, py, wl)
             tg = self._targets(d)
