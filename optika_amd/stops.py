@@ -18,7 +18,10 @@ The Newton iteration runs on the device in one launch per stop pair
 (``optk_stop_newton``): one thread per outline ray iterates its own root,
 tracing the residual and both Jacobian columns through the surfaces in
 registers; the walk back to the object is one ``optk_trace``.  The host
-keeps the outlines and the seeds (float64), as the reference does.
+keeps the outlines and the seeds (float64), as the reference does; they do
+not depend on the wavelength, so they are made once per stop pair and kept
+under the content keys of the pair's records.  The solved rays stay on the
+device through the walk back and come to the host in one copy.
 
 A grating with several diffraction orders on the way (between the stops,
 or between the first stop and the object) gives the reference's stop rays
@@ -42,6 +45,7 @@ outline's extent).
 
 from __future__ import annotations
 
+import collections
 import ctypes
 import dataclasses
 
@@ -112,31 +116,28 @@ def anchor_surface(subsystem):
     return subsystem[-1]
 
 
-def _propagate(surfaces, position, direction, wavelength, system=None):
-    """Rays [3][N] through ``surfaces`` on the device (system frame in and
-    out); returns position, direction [3][N] on the host.  One copy each
-    way."""
+def _propagate_device(surfaces, position, direction, wavelength, iterations):
+    """Device rays [3][N] through ``surfaces`` (system frame in and out);
+    returns position, direction [3][N] and ``iterations`` on the host, in
+    one copy."""
     dev = _trace.device()
-    n = position.shape[-1]
-    host = np.empty((7, n))
-    host[0:3] = position
-    host[3:6] = direction
-    host[6] = wavelength
-    d = torch.as_tensor(host).to(dev)
+    wave = torch.as_tensor(np.ascontiguousarray(wavelength, np.float64)).to(dev)
     rays = RayVectorArray(
-        wavelength=d[6],
-        position=Cartesian3dVectorArray(d[0], d[1], d[2]),
-        direction=Cartesian3dVectorArray(d[3], d[4], d[5]),
+        wavelength=wave,
+        position=Cartesian3dVectorArray(position[0], position[1], position[2]),
+        direction=Cartesian3dVectorArray(direction[0], direction[1], direction[2]),
     )
-    out = _trace.trace_sequence(list(surfaces), rays, system=system)
+    out = _trace.trace_sequence(list(surfaces), rays)
     if out.position.x.ndim != 1:
         raise NotImplementedError(
             "the stop solve needs a single ray path between the stops "
             "(one diffraction order per grating)"
         )
-    back = torch.stack([out.position.x, out.position.y, out.position.z,
-                        out.direction.x, out.direction.y, out.direction.z]).cpu().numpy()
-    return back[0:3], back[3:6]
+    back = torch.cat([torch.stack([out.position.x, out.position.y, out.position.z,
+                                   out.direction.x, out.direction.y, out.direction.z]).reshape(-1),
+                      iterations.to(torch.float64).reshape(1)]).cpu().numpy()
+    n = out.position.x.numel()
+    return back[0:3 * n].reshape(3, n), back[3 * n:6 * n].reshape(3, n), back[-1]
 
 
 def single_path(surfaces, path: int) -> list:
@@ -180,8 +181,97 @@ def _wire(surface, num) -> np.ndarray:
                      np.asarray(w.z, np.float64)]), ap.clips_direction
 
 
+# The seeds of a stop pair (outlines, starting rays, targets, step sizes)
+# do not depend on the wavelength: they are kept per pair under the content
+# keys of the records they derive from (the surfaces after the first stop,
+# the first stop itself) and the sample counts, so an edited surface
+# misses.  Most-recently used first out; a few pairs suffice.
+_SEEDS: "collections.OrderedDict" = collections.OrderedDict()
+
+
+def _pair_seeds(sub, samples_pupil_stop, samples_field_stop):
+    """Seeds of the stop pair sub[0] .. sub[-1] at one wavelength, host
+    float64 with the outline axes [S1][S2], and the compiled records the
+    Newton solve traces through."""
+    from ._compile import build_records
+    from .surfaces import Surface
+
+    s_first, s_last = sub[0], sub[-1]
+    forward = _trace.compiled(list(sub[1:]), post=s_last.transformation)
+    if forward.num_paths != 1:
+        raise NotImplementedError(
+            "the stop solve needs a single ray path between the stops "
+            "(one diffraction order per grating)"
+        )
+    key = (forward._key, build_records([s_first]).key(), samples_pupil_stop, samples_field_stop,
+           bool(s_first.is_pupil_stop), bool(s_first.is_field_stop))
+    hit = _SEEDS.get(key)
+    if hit is not None:
+        _SEEDS.move_to_end(key)
+        return hit
+    g1, first_angular = _wire(s_first, samples_pupil_stop)
+    g2, last_angular = _wire(s_last, samples_field_stop)
+    S1, S2 = g1.shape[1], g2.shape[1]
+    shape = (S1, S2)
+    gf = np.broadcast_to(g1[:2, :, None], (2,) + shape)  # first stop outline
+    gl = np.broadcast_to(g2[:2, None, :], (2,) + shape)  # last stop outline
+    if not first_angular:
+        # outline in millimetres: the ray starts on it, free direction
+        pos = np.zeros((3,) + shape)
+        pos[:2] = gf
+        pos[2] = _sag(s_first, pos)
+        dirs = np.zeros((3,) + shape)
+        dirs[2] = 1.0
+        variable = "direction"
+    else:
+        # angular outline: the direction is fixed, free position
+        dirs = np.zeros((3,) + shape)
+        dirs[:2] = gf
+        dirs[2] = np.sqrt(1 - (gf[0] ** 2 + gf[1] ** 2))
+        pos = np.zeros((3,) + shape)
+        variable = "position"
+
+    # seed: aim at the target point on the last stop when no powered
+    # surface lies between the stops, else at the first powered surface
+    anchor = anchor_surface(sub)
+    if anchor is s_last and not last_angular:
+        aim = np.zeros((3,) + shape)
+        aim[:2] = gl
+        aim[2] = _sag(s_last, aim)
+        aim = _apply(s_last.transformation, aim)
+    else:
+        aim = np.zeros((3,) + shape)
+        aim = _apply(anchor.transformation, aim)
+    aim = _apply(s_first.transformation, aim, inverse=True)
+    if variable == "direction":
+        d = aim - pos
+        d = d / np.sqrt((d * d).sum(0))
+        flip = np.sign(d[2])
+        nz = d[2] != 0
+        dirs = np.stack([np.where(nz, flip * d[0], 0.0), np.where(nz, flip * d[1], 0.0),
+                         np.where(nz, flip * d[2], 1.0)])
+    else:
+        t = aim[2] / dirs[2]
+        pos = np.stack([aim[0] - dirs[0] * t, aim[1] - dirs[1] * t, np.zeros(shape)])
+        pos[2] = _sag(s_first, pos)
+    pos = _apply(s_first.transformation, pos)
+    dirs = _apply(s_first.transformation, dirs, direction=True)
+
+    scale = max(np.ptp(g2[0]), np.ptp(g2[1]))
+    seeds = dict(
+        pos=pos, dirs=dirs, gf=np.array(gf), gl=np.array(gl), shape=shape, variable=variable,
+        target="direction" if last_angular else "position",
+        tol=1e-9 * max(scale, 1.0), dx=1e-6 if variable == "direction" else 1e-6 * max(scale, 1.0),
+        forward=forward,
+        first=_trace.compiled([Surface(sag=s_first.sag)]) if variable == "position" else None)
+    _SEEDS[key] = seeds
+    while len(_SEEDS) > 16:
+        _SEEDS.popitem(last=False)
+    return seeds
+
+
 def solve_stop_rays(system, wavelength, samples_pupil_stop: int = 21,
-                    samples_field_stop: int = 21, surfaces=None) -> dict:
+                    samples_field_stop: int = 21, surfaces=None, on_device: bool = False) -> dict:
     """Rays on the first stop joining the outlines of the field and pupil
     stops (``_sequential.py:396-662``), in the system frame.
 
@@ -190,7 +280,10 @@ def solve_stop_rays(system, wavelength, samples_pupil_stop: int = 21,
     with S1 the first stop's outline samples (``samples_pupil_stop``) and S2
     the last stop's (``samples_field_stop``).  ``surfaces``: the system's
     surfaces (default ``surfaces_all``), one diffraction-order path of them
-    (``single_path``) for a system with several."""
+    (``single_path``) for a system with several.  ``on_device``: position
+    and direction stay on the device as [3][S1 S2 W] tensors, with the
+    solve's largest iteration count (``iterations``, a device scalar) for
+    the caller to check; otherwise they are checked and come to the host."""
     if surfaces is None:
         surfaces = system.surfaces_all
     ip = [i for i, s in enumerate(surfaces) if s.is_pupil_stop]
@@ -206,12 +299,12 @@ def solve_stop_rays(system, wavelength, samples_pupil_stop: int = 21,
         first, last = min(i_p, i_f), max(i_p, i_f)
         sub = surfaces[first:last + 1]
         s_first, s_last = sub[0], sub[-1]
-        g1, first_angular = _wire(s_first, samples_pupil_stop)
-        g2, last_angular = _wire(s_last, samples_field_stop)
-        S1, S2, W = g1.shape[1], g2.shape[1], wl.size
+        sd = _pair_seeds(sub, samples_pupil_stop, samples_field_stop)
+        S1, S2 = sd["shape"]
+        W = wl.size
         shape = (S1, S2, W)
-        gf = np.broadcast_to(g1[:2, :, None, None], (2,) + shape)  # first stop outline
-        gl = np.broadcast_to(g2[:2, None, :, None], (2,) + shape)  # last stop outline
+        gf = np.broadcast_to(sd["gf"][..., None], (2,) + shape)
+        gl = np.broadcast_to(sd["gl"][..., None], (2,) + shape)
         if s_first.is_pupil_stop:
             ip.pop(0)
             pupil_grid, field_grid = gf, gl
@@ -219,84 +312,51 @@ def solve_stop_rays(system, wavelength, samples_pupil_stop: int = 21,
             ifs.pop(0)
             pupil_grid, field_grid = gl, gf
         wave = np.broadcast_to(wl[None, None, :], shape)
-
-        if not first_angular:
-            # outline in millimetres: the ray starts on it, free direction
-            pos = np.zeros((3,) + shape)
-            pos[:2] = gf
-            pos[2] = _sag(s_first, pos)
-            dirs = np.zeros((3,) + shape)
-            dirs[2] = 1.0
-            variable = "direction"
-        else:
-            # angular outline: the direction is fixed, free position
-            dirs = np.zeros((3,) + shape)
-            dirs[:2] = gf
-            dirs[2] = np.sqrt(1 - (gf[0] ** 2 + gf[1] ** 2))
-            pos = np.zeros((3,) + shape)
-            variable = "position"
-
-        # seed: aim at the target point on the last stop when no powered
-        # surface lies between the stops, else at the first powered surface
-        anchor = anchor_surface(sub)
-        if anchor is s_last and not last_angular:
-            aim = np.zeros((3,) + shape)
-            aim[:2] = gl
-            aim[2] = _sag(s_last, aim)
-            aim = _apply(s_last.transformation, aim)
-        else:
-            aim = np.zeros((3,) + shape)
-            aim = _apply(anchor.transformation, aim)
-        aim = _apply(s_first.transformation, aim, inverse=True)
-        if variable == "direction":
-            d = aim - pos
-            d = d / np.sqrt((d * d).sum(0))
-            flip = np.sign(d[2])
-            nz = d[2] != 0
-            dirs = np.stack([np.where(nz, flip * d[0], 0.0), np.where(nz, flip * d[1], 0.0),
-                             np.where(nz, flip * d[2], 1.0)])
-        else:
-            t = aim[2] / dirs[2]
-            pos = np.stack([aim[0] - dirs[0] * t, aim[1] - dirs[1] * t, np.zeros(shape)])
-            pos[2] = _sag(s_first, pos)
-        pos = _apply(s_first.transformation, pos)
-        dirs = _apply(s_first.transformation, dirs, direction=True)
-
-        target = "direction" if last_angular else "position"
-        scale = max(np.ptp(g2[0]), np.ptp(g2[1]))
-        max_abs_error = 1e-9 * max(scale, 1.0)
-        dx = 1e-6 if variable == "direction" else 1e-6 * max(scale, 1.0)
-
+        pos = np.broadcast_to(sd["pos"][..., None], (3,) + shape)
+        dirs = np.broadcast_to(sd["dirs"][..., None], (3,) + shape)
+        variable = sd["variable"]
         x0 = (dirs if variable == "direction" else pos)[:2].reshape(2, -1)
         # the root and its z (sqrt(1 - x^2 - y^2) for a direction, the first
-        # surface's sag for a position) from the device
-        root = _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, x0,
-                              dx, max_abs_error)
-        v = root.reshape((3,) + shape)
+        # surface's sag for a position), on the device
+        d, iters = _newton_device(sd, pos, dirs, wave, gl, x0)
+        root = d[9:12]
         if variable == "direction":
-            dirs = v
+            p_dev, d_dev = d[0:3], root
         else:
-            pos = v
-        result = dict(position=pos, direction=dirs, field=np.array(field_grid),
-                      pupil=np.array(pupil_grid), wavelength=wave, first=first)
+            p_dev, d_dev = root, d[3:6]
+        result = dict(field=np.array(field_grid), pupil=np.array(pupil_grid), wavelength=wave,
+                      first=first, s_first=s_first, s_last=s_last, shape=shape)
+        if on_device:
+            result.update(position=p_dev, direction=d_dev, iterations=iters.max())
+        else:
+            back = torch.cat([p_dev.reshape(-1), d_dev.reshape(-1),
+                              iters.max().to(torch.float64).reshape(1)]).cpu().numpy()
+            _check_iterations(back[-1], s_first, s_last)
+            result.update(position=back[:-1][:p_dev.numel()].reshape((3,) + shape),
+                          direction=back[:-1][p_dev.numel():].reshape((3,) + shape))
     return result
 
 
-def _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, x0, dx, tol):
+def _check_iterations(count, s_first, s_last):
+    if count > MAX_ITERATIONS:
+        raise ValueError(
+            f"Could not solve for the rays connecting the stop surfaces {s_first.name!r} and "
+            f"{s_last.name!r}. If the field stop is only partially reachable at a single "
+            f"wavelength (e.g. a spectrograph sensor), consider marking the object surface, "
+            f"with an angular (dimensionless, sine of the half-angle) aperture, as the field "
+            f"stop instead."
+        )
+
+
+def _newton_device(sd, pos, dirs, wave, gl, x0):
     """Newton's method for every outline ray in one device launch
     (``optk_stop_newton``): each thread iterates its own ray, tracing the
-    residual and both forward-difference Jacobian columns through
-    ``sub[1:]`` into the last stop's frame."""
-    from .surfaces import Surface
-
+    residual and both forward-difference Jacobian columns through the
+    surfaces after the first stop into the last stop's frame.  Returns the
+    [12][n] device array (rows 0-5 the starting rays; rows 9-11 the root
+    and its z) and the per-ray iteration counts (device)."""
     dev = _trace.device()
-    forward = _trace.compiled(list(sub[1:]), post=s_last.transformation)
-    if forward.num_paths != 1:
-        raise NotImplementedError(
-            "the stop solve needs a single ray path between the stops "
-            "(one diffraction order per grating)"
-        )
-    first = _trace.compiled([Surface(sag=s_first.sag)]) if variable == "position" else None
+    forward, first = sd["forward"], sd["first"]
     n = x0.shape[1]
     # rays, targets and the initial roots in one copy; the roots are solved
     # in place (rows 9 and 10), the free component's z at the root goes to
@@ -319,24 +379,14 @@ def _newton_device(sub, s_first, s_last, variable, target, pos, dirs, wave, gl, 
     a.x[0], a.x[1] = x[0].data_ptr(), x[1].data_ptr()
     a.iterations = iters.data_ptr()
     a.z = d[11].data_ptr()
-    a.tol, a.dx = float(tol), float(dx)
-    a.free_component = _lib.STOP_FREE_DIRECTION if variable == "direction" else _lib.STOP_FREE_POSITION
-    a.target_component = (_lib.STOP_TARGET_POSITION if target == "position"
+    a.tol, a.dx = float(sd["tol"]), float(sd["dx"])
+    a.free_component = _lib.STOP_FREE_DIRECTION if sd["variable"] == "direction" else _lib.STOP_FREE_POSITION
+    a.target_component = (_lib.STOP_TARGET_POSITION if sd["target"] == "position"
                           else _lib.STOP_TARGET_DIRECTION)
     a.max_iter = MAX_ITERATIONS
     _lib.check(_lib.lib().optk_stop_newton(forward.handle, first.handle if first else None,
                                            ctypes.byref(a), n, _trace.stream_handle()))
-    # the roots, their z and the largest iteration count in one copy back
-    back = torch.cat([d[9:12].reshape(-1), iters.max().to(torch.float64).reshape(1)]).cpu().numpy()
-    if back[-1] > MAX_ITERATIONS:
-        raise ValueError(
-            f"Could not solve for the rays connecting the stop surfaces {s_first.name!r} and "
-            f"{s_last.name!r}. If the field stop is only partially reachable at a single "
-            f"wavelength (e.g. a spectrograph sensor), consider marking the object surface, "
-            f"with an angular (dimensionless, sine of the half-angle) aperture, as the field "
-            f"stop instead."
-        )
-    return back[:-1].reshape(3, n)
+    return d, iters
 
 
 def rayfunction_stops(system, wavelength, samples_pupil_stop: int = 21,
@@ -369,10 +419,14 @@ def _rayfunction_stops_path(system, wavelength, samples_pupil_stop, samples_fiel
         raise ValueError("Field stop is not defined for this system.")
     index_stop = min(ip[-1], ifs[-1])
     back = surfaces[index_stop::-1]
-    r = solve_stop_rays(system, wavelength, samples_pupil_stop, samples_field_stop, surfaces)
-    shape = r["position"].shape[1:]
-    p, d = _propagate(back, r["position"].reshape(3, -1), r["direction"].reshape(3, -1),
-                      r["wavelength"].reshape(-1))
+    r = solve_stop_rays(system, wavelength, samples_pupil_stop, samples_field_stop, surfaces,
+                        on_device=True)
+    shape = r["shape"]
+    # the solved rays walk back to the object without leaving the device;
+    # they and the solve's iteration count come to the host in one copy
+    p, d, count = _propagate_device(back, r["position"], r["direction"],
+                                    r["wavelength"].reshape(-1), r["iterations"])
+    _check_iterations(count, r["s_first"], r["s_last"])
     obj = back[-1]
     pl = _apply(obj.transformation, p, inverse=True)
     dl = _apply(obj.transformation, d, direction=True, inverse=True)
