@@ -469,8 +469,10 @@ int32_t optk_batch_destroy(optk_batch *batch);
  * offsets[0] = 0).  out (device rays, num_paths * n, path-major) may be
  * NULL; stats (device, num_instances * num_paths * OPTK_BATCH_STATS
  * doubles) is overwritten.  tune NULL: the last autotuned shape, else a
- * default.  flags: OPTK_TRACE_APPLY_POST, OPTK_BATCH_GRAPH.  One call at a
- * time per batch (the work-queue counter is the batch's). */
+ * default.  flags: OPTK_TRACE_APPLY_POST, OPTK_BATCH_GRAPH,
+ * OPTK_TRACE_GENERIC (the generic kernels even where every instance matches
+ * a specialised layout).  One call at a time per batch (the work-queue
+ * counter is the batch's). */
 int32_t optk_batch_trace(optk_batch *batch, int32_t precision, const optk_grid *in,
                          const int64_t *offsets, const optk_rays *out, double *stats,
                          const optk_tune *tune, int32_t flags, void *hip_stream);

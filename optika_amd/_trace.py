@@ -25,8 +25,8 @@ __all__ = ["trace_sequence", "trace_system", "device", "compiled"]
 # and whether their trace depends on the wavelength (an all-mirror system
 # without gratings or coatings does not: its specialised kernels never
 # read the wavelength array)
-SIGNATURE_NAMES = ("newtonian", "spectrograph")
-SIGNATURE_READS_WAVELENGTH = (False, True)
+SIGNATURE_NAMES = ("newtonian", "spectrograph", "tilted spectrograph")
+SIGNATURE_READS_WAVELENGTH = (False, True, True)
 
 _PRECISIONS = {"f64": _lib.F64, "float64": _lib.F64, "f32": _lib.F32, "float32": _lib.F32}
 

@@ -134,11 +134,13 @@ class BatchedSystem:
     # -- API -------------------------------------------------------------
     def trace(self, field_x, field_y, pupil_x, pupil_y, wavelength, counts,
               precision: str = "f32", tune: _lib.Tune | None = None, graph: bool = True,
-              outputs: bool = False, stats: torch.Tensor | None = None):
+              outputs: bool = False, stats: torch.Tensor | None = None, generic: bool = False):
         """Trace every instance's rays.  Returns the statistics tensor
         (instances, paths, 8) with columns ``STAT_FIELDS``, and the output
         rays (a ``_trace._Out``: path-major, all instances back to back)
-        when ``outputs`` is set."""
+        when ``outputs`` is set.  ``generic``: the generic kernels even
+        where every instance matches a specialised layout (same results in
+        float64; float32 to its rounding)."""
         prec = _trace._precision(precision)
         dtype = torch.float64 if prec == _lib.F64 else torch.float32
         g, n = self._grid(field_x, field_y, pupil_x, pupil_y, wavelength, dtype)
@@ -149,7 +151,8 @@ class BatchedSystem:
         if outputs:
             out = _trace._Out(self.num_paths * n, self.dev, dtype,
                               {"wavelength", "intensity", "attenuation", "index_refraction"})
-        flags = (_lib.TRACE_APPLY_POST if self.sensor_frame else 0) | (_lib.BATCH_GRAPH if graph else 0)
+        flags = ((_lib.TRACE_APPLY_POST if self.sensor_frame else 0) | (_lib.BATCH_GRAPH if graph else 0)
+                 | (_lib.TRACE_GENERIC if generic else 0))
         L = _lib.lib()
         _lib.check(L.optk_batch_trace(
             self.handle, prec, ctypes.byref(g), offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),

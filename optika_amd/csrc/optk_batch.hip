@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "optk_internal.h"
+#include "optk_signatures.h"
 
 using namespace optk;
 
@@ -45,6 +46,7 @@ struct optk_batch {
   int32_t num_surfaces = 0;
   int32_t num_paths = 1;
   int32_t features = 0;
+  int32_t signature = -1;  // the instances' common surface-kind signature, or -1
   int32_t num_cus = 0;
   // shared by every instance
   optk_layer *layers = nullptr;
@@ -214,10 +216,16 @@ void fill_inputs(Inputs &io, int32_t precision, const optk_grid *in, const optk_
   (void)precision;
 }
 
-int resident_blocks(const optk_batch *b, int32_t precision, int block) {
+// the signature whose specialised kernels a call runs (-1: the generic ones)
+int call_signature(const optk_batch *b, int32_t flags) {
+  return (flags & OPTK_TRACE_GENERIC) ? -1 : b->signature;
+}
+
+int resident_blocks(const optk_batch *b, int32_t precision, int block, int32_t flags) {
+  const int sig = call_signature(b, flags);
   if (precision == OPTK_F64)
-    return batch_occupancy_f64(b->features, block, batch_lds_f64(b->num_surfaces));
-  return batch_occupancy_f32(b->features, block, batch_lds_f32(b->num_surfaces));
+    return batch_occupancy_f64(b->features, sig, block, batch_lds_f64(b->num_surfaces));
+  return batch_occupancy_f32(b->features, sig, block, batch_lds_f32(b->num_surfaces));
 }
 
 // zero the statistics and the queue head, run the persistent kernel, reduce
@@ -251,9 +259,10 @@ int32_t enqueue(optk_batch *b, int32_t precision, const Inputs &io, double *stat
   A.stats = stats;
   A.n = n;
   if (b->num_tiles > 0) {
+    const int sig = call_signature(b, flags);
     e = precision == OPTK_F64
-            ? batch_launch_f64(b->features, shape.blocks, shape.block, A, io.g64, io.o64, stream)
-            : batch_launch_f32(b->features, shape.blocks, shape.block, A, io.g32, io.o32, stream);
+            ? batch_launch_f64(b->features, sig, shape.blocks, shape.block, A, io.g64, io.o64, stream)
+            : batch_launch_f32(b->features, sig, shape.blocks, shape.block, A, io.g32, io.o32, stream);
     if (e != hipSuccess) return fail(OPTK_EDEVICE, std::string("batch trace: ") + hipGetErrorString(e));
   }
   const int64_t pairs = (int64_t)b->num_instances * b->num_paths;
@@ -275,10 +284,11 @@ int32_t check_call(const optk_batch *b, int32_t precision, const optk_grid *in,
   return OPTK_OK;
 }
 
-int32_t shape_for(optk_batch *b, int32_t precision, const optk_tune &t, LaunchShape &shape) {
+int32_t shape_for(optk_batch *b, int32_t precision, const optk_tune &t, int32_t flags,
+                  LaunchShape &shape) {
   int32_t rc = check_tune(t);
   if (rc) return rc;
-  int fit = resident_blocks(b, precision, t.block);
+  int fit = resident_blocks(b, precision, t.block, flags);
   if (fit <= 0) return fail(OPTK_EDEVICE, "the batched kernel does not fit on a CU at this block size");
   int per_cu = (t.blocks_per_cu > 0 && t.blocks_per_cu < fit) ? t.blocks_per_cu : fit;
   shape.block = t.block;
@@ -329,12 +339,18 @@ int32_t optk_batch_create(const optk_batch_desc *desc, optk_batch **out) {
     features = f;
   }
   for (auto &s : surfaces) derive_constants(s);
+  // the specialised kernels serve a batch whose instances all match one
+  // signature (perturbations keep the kinds of every record)
+  int32_t signature = find_signature(surfaces.data(), S, features);
+  for (int i = 1; i < N && signature >= 0; ++i)
+    if (find_signature(surfaces.data() + (size_t)i * S, S, features) != signature) signature = -1;
 
   optk_batch *b = new optk_batch();
   b->num_instances = N;
   b->num_surfaces = S;
   b->num_paths = base.num_paths;
   b->features = features;
+  b->signature = signature;
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -414,7 +430,7 @@ int32_t optk_batch_trace(optk_batch *b, int32_t precision, const optk_grid *in,
   else
     t = optk_tune{256, 2, 0, 0, 0.0};
   LaunchShape shape;
-  if ((rc = shape_for(b, precision, t, shape))) return rc;
+  if ((rc = shape_for(b, precision, t, flags, shape))) return rc;
   if ((rc = prepare_tiles(b, offsets, (int64_t)shape.block * shape.rays_per_thread))) return rc;
   const int64_t n = offsets[b->num_instances];
   Inputs io;
@@ -481,7 +497,7 @@ int32_t optk_batch_autotune(optk_batch *b, int32_t precision, const optk_grid *i
     for (int ri = 0; ri < 4 && rc == OPTK_OK; ++ri) {
       optk_tune t = {blocks[bi], rpts[ri], 0, 0, 0.0};
       LaunchShape shape;
-      if (shape_for(b, precision, t, shape) != OPTK_OK) continue;  // does not fit: skip
+      if (shape_for(b, precision, t, flags, shape) != OPTK_OK) continue;  // does not fit: skip
       if ((rc = prepare_tiles(b, offsets, (int64_t)shape.block * shape.rays_per_thread))) break;
       if ((rc = enqueue(b, precision, io, stats, shape, flags, n, stream))) break;  // warm-up
       float best_ms = INFINITY;

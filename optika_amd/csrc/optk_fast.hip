@@ -181,6 +181,12 @@ size_t fast_staged_bytes(int num_surfaces) {
 template <int F> struct FastWaves {
   static constexpr int value = F == 0 ? 6 : ((F & FEAT_MULTILAYER) ? 2 : 4);
 };
+// the specialised kernels: 3 waves for the multilayer layout (168 VGPRs)
+// measured 2 % faster on the spectrograph than 2 (256 VGPRs), 4 (128
+// VGPRs) 16 % slower
+template <int F> struct FastSpecWaves {
+  static constexpr int value = F == 0 ? 6 : ((F & FEAT_MULTILAYER) ? 3 : 4);
+};
 // the batched kernels keep 4 / 2 (at 6 the geometric one spills)
 template <int F> struct BatchWaves {
   static constexpr int value = (F & FEAT_MULTILAYER) ? 2 : 4;
@@ -340,13 +346,13 @@ __device__ __forceinline__ void fast_spec_body(const FastArgs &A, const In &in) 
 }
 
 template <int F, int IO, uint32_t... K>
-__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(FastSpecWaves<F>::value)))
 void trace_grid_f32_spec(const FastArgs A, const FGridIn g) {
   fast_spec_body<FGridIn, F, IO, K...>(A, g);
 }
 
 template <int F, int IO, uint32_t... K>
-__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(FastWaves<F>::value)))
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(FastSpecWaves<F>::value)))
 void trace_rays_f32_spec(const FastArgs A, const FRaysIn in) {
   fast_spec_body<FRaysIn, F, IO, K...>(A, in);
 }
@@ -356,10 +362,14 @@ typedef void (*SpecRays32)(FastArgs, FRaysIn);
 const SpecGrid32 SPEC_GRID32[NUM_SIGNATURES] = {
     trace_grid_f32_spec<SIGNATURES[SIG_NEWTONIAN].features, IO_SENSOR, OPTK_SIGNATURE_NEWTONIAN>,
     trace_grid_f32_spec<SIGNATURES[SIG_SPECTROGRAPH].features, IO_SENSOR, OPTK_SIGNATURE_SPECTROGRAPH>,
+    trace_grid_f32_spec<SIGNATURES[SIG_SPECTROGRAPH_TILTED].features, IO_SENSOR,
+                        OPTK_SIGNATURE_SPECTROGRAPH_TILTED>,
 };
 const SpecRays32 SPEC_RAYS32[NUM_SIGNATURES] = {
     trace_rays_f32_spec<SIGNATURES[SIG_NEWTONIAN].features, IO_FULL, OPTK_SIGNATURE_NEWTONIAN>,
     trace_rays_f32_spec<SIGNATURES[SIG_SPECTROGRAPH].features, IO_FULL, OPTK_SIGNATURE_SPECTROGRAPH>,
+    trace_rays_f32_spec<SIGNATURES[SIG_SPECTROGRAPH_TILTED].features, IO_FULL,
+                        OPTK_SIGNATURE_SPECTROGRAPH_TILTED>,
 };
 
 // kernels of one entry point: by feature set (geometric, toroid + rulings,
@@ -412,7 +422,7 @@ template <int F> struct Tracer32 {
 
   __device__ int num_paths() const { return (F & FEAT_RULINGS) ? A.num_paths : 1; }
 
-  __device__ void stage(int instance) {
+  __device__ __forceinline__ void stage(int instance) {
     stage_words(s_spec, A.fast_specs + instance, sizeof(FastSpec));
     stage_words(s_fast, A.fast + (int64_t)instance * A.num_surfaces,
                 sizeof(FastSurface) * (size_t)A.num_surfaces);
@@ -420,7 +430,8 @@ template <int F> struct Tracer32 {
                 sizeof(optk_surface) * (size_t)A.num_surfaces);
   }
 
-  __device__ void trace(int64_t i, int path, double &x, double &y, double &inten, bool &unv) {
+  __device__ __forceinline__ void trace(int64_t i, int path, double &x, double &y, double &inten,
+                                        bool &unv) {
     FastView v;
     v.surfaces = s_surf;
     v.fast = s_fast;
@@ -457,6 +468,50 @@ template <int F> struct Tracer32 {
   }
 };
 
+// the same with the instances' surface kinds fixed at compile time (every
+// instance of a batch matching one signature), the surfaces unrolled
+template <int F, uint32_t K>
+__device__ __forceinline__ void batch_fast_step(int k, int k0, const FastSurface *s_fast,
+                                                const optk_surface *s_surf, const FastView &v, int path,
+                                                FRay &r) {
+  if (k < k0) return;
+  if (k > k0) fast_xf(s_fast[k].hop, r);
+  fast_propagate<F, K>(s_fast[k], s_surf[k], v, path, r);
+}
+
+template <int F, uint32_t... K> struct Tracer32Spec : Tracer32<F> {
+  __device__ __forceinline__ void trace(int64_t i, int path, double &x, double &y, double &inten,
+                                        bool &unv) {
+    FastView v;
+    v.surfaces = this->s_surf;
+    v.fast = this->s_fast;
+    v.sv = this->A.sv;
+    v.tables32 = this->A.tables32;
+    FRay r = load_fray(this->in, i);
+    const bool skip = this->s_spec->skip_first != 0;
+    const int k0 = skip ? 1 : 0;
+    if (skip) {
+      const bool parallel = r.dz == 0.0;
+      fast_xf(this->s_spec->first_skip, r);
+      if (parallel) {
+        r.px = r.py = r.pz = NAN;
+        r.intensity = NAN;
+      }
+    } else {
+      fast_xf(this->s_spec->first_hop[1], r);
+    }
+    int k = 0;
+    (batch_fast_step<F, K>(k++, k0, this->s_fast, this->s_surf, v, path, r), ...);
+    const optk_affine &last = this->s_spec->last[this->A.apply_post];
+    fast_xf(last, r);
+    store_f(this->out, (int64_t)path * this->A.n + i, r);
+    x = (double)(float)r.px;
+    y = (double)(float)r.py;
+    inten = r.intensity;
+    unv = r.unvignetted;
+  }
+};
+
 template <int F>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BatchWaves<F>::value)))
 void batch_grid_f32(const BatchArgs A, const FGridIn in, const FRaysOut out) {
@@ -468,7 +523,28 @@ void batch_grid_f32(const BatchArgs A, const FGridIn in, const FRaysOut out) {
   batch_loop(A, tr);
 }
 
-void *batch_kernel_f32(int features) {
+template <int F, uint32_t... K>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(FastSpecWaves<F>::value)))
+void batch_grid_f32_spec(const BatchArgs A, const FGridIn in, const FRaysOut out) {
+  extern __shared__ double staged[];
+  FastSpec *spec = reinterpret_cast<FastSpec *>(staged);
+  FastSurface *fast = reinterpret_cast<FastSurface *>(spec + 1);
+  optk_surface *surf = reinterpret_cast<optk_surface *>(fast + A.num_surfaces);
+  Tracer32Spec<F, K...> tr{{A, in, out, spec, fast, surf}};
+  batch_loop(A, tr);
+}
+
+typedef void (*BatchSpec32)(BatchArgs, FGridIn, FRaysOut);
+// by signature; the geometric Newtonian layout keeps the generic kernel
+const BatchSpec32 BATCH_SPEC32[NUM_SIGNATURES] = {
+    nullptr,
+    batch_grid_f32_spec<SIGNATURES[SIG_SPECTROGRAPH].features, OPTK_SIGNATURE_SPECTROGRAPH>,
+    batch_grid_f32_spec<SIGNATURES[SIG_SPECTROGRAPH_TILTED].features, OPTK_SIGNATURE_SPECTROGRAPH_TILTED>,
+};
+
+void *batch_kernel_f32(int features, int signature) {
+  if (signature >= 0 && signature < NUM_SIGNATURES && BATCH_SPEC32[signature])
+    return (void *)BATCH_SPEC32[signature];
   if (features == 0) return (void *)batch_grid_f32<0>;
   if (features == (FEAT_TOROID | FEAT_RULINGS)) return (void *)batch_grid_f32<FEAT_TOROID | FEAT_RULINGS>;
   return (void *)batch_grid_f32<FEAT_ALL>;
@@ -505,18 +581,20 @@ int32_t fast_trace_grid(const optk_system *sys, const FGridIn &in, const FRaysOu
 
 size_t batch_lds_f32(int num_surfaces) { return fast_staged_bytes(num_surfaces); }
 
-int batch_occupancy_f32(int features, int block, size_t lds) {
+int batch_occupancy_f32(int features, int signature, int block, size_t lds) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel_f32(features), block, lds) !=
-      hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel_f32(features, signature), block,
+                                                   lds) != hipSuccess)
     return 0;
   return nb;
 }
 
-hipError_t batch_launch_f32(int features, int blocks, int block, const BatchArgs &A,
+hipError_t batch_launch_f32(int features, int signature, int blocks, int block, const BatchArgs &A,
                             const FGridIn &in, const FRaysOut &out, hipStream_t stream) {
   size_t lds = fast_staged_bytes(A.num_surfaces);
-  if (features == 0)
+  if (signature >= 0 && signature < NUM_SIGNATURES && BATCH_SPEC32[signature])
+    hipLaunchKernelGGL(BATCH_SPEC32[signature], dim3(blocks), dim3(block), lds, stream, A, in, out);
+  else if (features == 0)
     hipLaunchKernelGGL(batch_grid_f32<0>, dim3(blocks), dim3(block), lds, stream, A, in, out);
   else if (features == (FEAT_TOROID | FEAT_RULINGS))
     hipLaunchKernelGGL(batch_grid_f32<FEAT_TOROID | FEAT_RULINGS>, dim3(blocks), dim3(block), lds,

@@ -155,12 +155,14 @@ struct BatchArgs {
 
 size_t batch_lds_f64(int num_surfaces);
 size_t batch_lds_f32(int num_surfaces);
-// resident blocks per CU of the kernel covering `features` (0 on error)
-int batch_occupancy_f64(int features, int block, size_t lds);
-int batch_occupancy_f32(int features, int block, size_t lds);
-hipError_t batch_launch_f64(int features, int blocks, int block, const BatchArgs &A,
+// resident blocks per CU of the kernel covering `features` (0 on error);
+// signature >= 0: the instances' common surface-kind signature
+// (optk_signatures.h), whose specialised kernel runs where one exists
+int batch_occupancy_f64(int features, int signature, int block, size_t lds);
+int batch_occupancy_f32(int features, int signature, int block, size_t lds);
+hipError_t batch_launch_f64(int features, int signature, int blocks, int block, const BatchArgs &A,
                             const GridIn &in, const RaysOut &out, hipStream_t stream);
-hipError_t batch_launch_f32(int features, int blocks, int block, const BatchArgs &A,
+hipError_t batch_launch_f32(int features, int signature, int blocks, int block, const BatchArgs &A,
                             const FGridIn &in, const FRaysOut &out, hipStream_t stream);
 
 }  // namespace optk

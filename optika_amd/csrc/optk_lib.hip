@@ -156,6 +156,12 @@ static size_t staged_bytes(int num_surfaces) {
 template <int F> struct Waves {
   static constexpr int value = (F & FEAT_MULTILAYER) ? 2 : 4;
 };
+// the specialised kernels: 4 waves also for the multilayer layout (128
+// VGPRs, more of the multilayer's temporaries spilled), measured 2.5 %
+// faster on the spectrograph than 2 waves with 256 VGPRs
+template <int F> struct SpecWaves {
+  static constexpr int value = 4;
+};
 
 // I/O layouts of the kernels.  IO_ANY honours every null pointer of the
 // descriptors (a value default in, the field not written out).  IO_SENSOR
@@ -354,13 +360,13 @@ __device__ __forceinline__ void spec_body(const TraceArgs &A, const In &in) {
 }
 
 template <int F, int IO, uint32_t... K>
-__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(SpecWaves<F>::value)))
 void trace_grid_f64_spec(const TraceArgs A, const GridIn g) {
   spec_body<GridIn, F, IO, K...>(A, g);
 }
 
 template <int F, int IO, uint32_t... K>
-__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(SpecWaves<F>::value)))
 void trace_rays_f64_spec(const TraceArgs A, const RaysIn in) {
   spec_body<RaysIn, F, IO, K...>(A, in);
 }
@@ -372,10 +378,14 @@ typedef void (*SpecRays64)(TraceArgs, RaysIn);
 static const SpecGrid64 SPEC_GRID64[NUM_SIGNATURES] = {
     trace_grid_f64_spec<SIGNATURES[SIG_NEWTONIAN].features, IO_SENSOR, OPTK_SIGNATURE_NEWTONIAN>,
     trace_grid_f64_spec<SIGNATURES[SIG_SPECTROGRAPH].features, IO_SENSOR, OPTK_SIGNATURE_SPECTROGRAPH>,
+    trace_grid_f64_spec<SIGNATURES[SIG_SPECTROGRAPH_TILTED].features, IO_SENSOR,
+                        OPTK_SIGNATURE_SPECTROGRAPH_TILTED>,
 };
 static const SpecRays64 SPEC_RAYS64[NUM_SIGNATURES] = {
     trace_rays_f64_spec<SIGNATURES[SIG_NEWTONIAN].features, IO_FULL, OPTK_SIGNATURE_NEWTONIAN>,
     trace_rays_f64_spec<SIGNATURES[SIG_SPECTROGRAPH].features, IO_FULL, OPTK_SIGNATURE_SPECTROGRAPH>,
+    trace_rays_f64_spec<SIGNATURES[SIG_SPECTROGRAPH_TILTED].features, IO_FULL,
+                        OPTK_SIGNATURE_SPECTROGRAPH_TILTED>,
 };
 
 static unsigned spec_blocks(int64_t n) { return (unsigned)((n + TRACE_BLOCK - 1) / TRACE_BLOCK); }
@@ -549,13 +559,14 @@ template <int F> struct Tracer64 {
 
   __device__ int num_paths() const { return (F & FEAT_RULINGS) ? A.num_paths : 1; }
 
-  __device__ void stage(int instance) {
+  __device__ __forceinline__ void stage(int instance) {
     stage_words(s_spec, A.specs64 + instance, sizeof(TraceSpec));
     stage_words(s_surf, A.surfaces + (int64_t)instance * A.num_surfaces,
                 sizeof(optk_surface) * (size_t)A.num_surfaces);
   }
 
-  __device__ void trace(int64_t i, int path, double &x, double &y, double &inten, bool &unv) {
+  __device__ __forceinline__ void trace(int64_t i, int path, double &x, double &y, double &inten,
+                                        bool &unv) {
     Ray<double> r = load_ray(in, i);
     apply_pre(*s_spec, r);
     SystemView sys = s_spec->sys;
@@ -572,6 +583,29 @@ template <int F> struct Tracer64 {
   }
 };
 
+// the same with the instances' surface kinds fixed at compile time (every
+// instance of a batch matching one signature): the surfaces unroll as in
+// the specialised single-system kernels, the records are read from LDS
+template <int F, uint32_t... K> struct Tracer64Spec : Tracer64<F> {
+  __device__ __forceinline__ void trace(int64_t i, int path, double &x, double &y, double &inten,
+                                        bool &unv) {
+    Ray<double> r = load_ray(this->in, i);
+    apply_pre(*this->s_spec, r);
+    SystemView sys = this->s_spec->sys;
+    sys.surfaces = this->s_surf;
+    int k = 0;
+    (propagate_surface<F, K>(this->s_surf[k++], sys, path, r), ...);
+    if (this->A.apply_post) {
+      xf_inverse_ray(this->s_spec->post, r.px, r.py, r.pz, r.dx, r.dy, r.dz);
+    }
+    store(this->out, (int64_t)path * this->A.n + i, r);
+    x = r.px;
+    y = r.py;
+    inten = r.intensity;
+    unv = r.unvignetted;
+  }
+};
+
 template <int F>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
 void batch_grid_f64(const BatchArgs A, const GridIn in, const RaysOut out) {
@@ -581,28 +615,54 @@ void batch_grid_f64(const BatchArgs A, const GridIn in, const RaysOut out) {
   batch_loop(A, tr);
 }
 
+// 2 waves per SIMD (256 VGPRs) for the multilayer layouts, as the generic
+// batch kernel: built for 3 or 4 waves (168 / 128 VGPRs, about twice the
+// spill slots) this kernel returned every ray vignetted on the tolerance
+// batch, while the same source at 256 VGPRs, and with the trace left as a
+// call, matches the single-system traces; the narrower builds are not used
+template <int F, uint32_t... K>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(Waves<F>::value)))
+void batch_grid_f64_spec(const BatchArgs A, const GridIn in, const RaysOut out) {
+  extern __shared__ double staged[];
+  Tracer64Spec<F, K...> tr{{A, in, out, reinterpret_cast<TraceSpec *>(staged),
+                            reinterpret_cast<optk_surface *>(reinterpret_cast<TraceSpec *>(staged) + 1)}};
+  batch_loop(A, tr);
+}
+
+typedef void (*BatchSpec64)(BatchArgs, GridIn, RaysOut);
+// by signature; the geometric Newtonian layout keeps the generic kernel
+static const BatchSpec64 BATCH_SPEC64[NUM_SIGNATURES] = {
+    nullptr,
+    batch_grid_f64_spec<SIGNATURES[SIG_SPECTROGRAPH].features, OPTK_SIGNATURE_SPECTROGRAPH>,
+    batch_grid_f64_spec<SIGNATURES[SIG_SPECTROGRAPH_TILTED].features, OPTK_SIGNATURE_SPECTROGRAPH_TILTED>,
+};
+
 namespace optk {
 
 size_t batch_lds_f64(int num_surfaces) { return staged_bytes(num_surfaces); }
 
-static void *batch_kernel_f64(int features) {
+static void *batch_kernel_f64(int features, int signature) {
+  if (signature >= 0 && signature < NUM_SIGNATURES && BATCH_SPEC64[signature])
+    return (void *)BATCH_SPEC64[signature];
   if (features == 0) return (void *)batch_grid_f64<0>;
   if (features == (FEAT_TOROID | FEAT_RULINGS)) return (void *)batch_grid_f64<FEAT_TOROID | FEAT_RULINGS>;
   return (void *)batch_grid_f64<FEAT_ALL>;
 }
 
-int batch_occupancy_f64(int features, int block, size_t lds) {
+int batch_occupancy_f64(int features, int signature, int block, size_t lds) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel_f64(features), block, lds) !=
-      hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel_f64(features, signature), block,
+                                                   lds) != hipSuccess)
     return 0;
   return nb;
 }
 
-hipError_t batch_launch_f64(int features, int blocks, int block, const BatchArgs &A,
+hipError_t batch_launch_f64(int features, int signature, int blocks, int block, const BatchArgs &A,
                             const GridIn &in, const RaysOut &out, hipStream_t stream) {
   size_t lds = staged_bytes(A.num_surfaces);
-  if (features == 0)
+  if (signature >= 0 && signature < NUM_SIGNATURES && BATCH_SPEC64[signature])
+    hipLaunchKernelGGL(BATCH_SPEC64[signature], dim3(blocks), dim3(block), lds, stream, A, in, out);
+  else if (features == 0)
     hipLaunchKernelGGL(batch_grid_f64<0>, dim3(blocks), dim3(block), lds, stream, A, in, out);
   else if (features == (FEAT_TOROID | FEAT_RULINGS))
     hipLaunchKernelGGL(batch_grid_f64<FEAT_TOROID | FEAT_RULINGS>, dim3(blocks), dim3(block), lds,

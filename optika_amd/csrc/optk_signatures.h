@@ -54,6 +54,24 @@ namespace optk {
                  OPTK_RULINGS_NONE, OPTK_SPACING_CONSTANT, OPTK_APERTURE_RECTANGLE,              \
                  OPTK_XF_IDENTITY, OPTK_AP_ACTIVE)
 
+// the EUV spectrograph with a tilted primary (its transformation a full
+// affine map): every instance of the tolerance batch (decentred and tilted
+// surfaces) has these kinds
+#define OPTK_SIGNATURE_SPECTROGRAPH_TILTED                                                       \
+  OPTK_K_PLAIN_OBJECT,                                                                          \
+      kinds_pack(OPTK_XF_AFFINE, OPTK_SAG_CONIC, OPTK_XF_TRANSLATE,                              \
+                 OPTK_MATERIAL_MULTILAYER_MIRROR, 1, OPTK_RULINGS_NONE, OPTK_SPACING_CONSTANT,   \
+                 OPTK_APERTURE_CIRCLE, OPTK_XF_IDENTITY, OPTK_AP_ACTIVE),                       \
+      kinds_pack(OPTK_XF_AFFINE, OPTK_SAG_TOROID, OPTK_XF_IDENTITY,                              \
+                 OPTK_MATERIAL_MULTILAYER_MIRROR, 1, OPTK_RULINGS_IDEAL, OPTK_SPACING_POLY1D,    \
+                 OPTK_APERTURE_RECTANGLE, OPTK_XF_IDENTITY, OPTK_AP_ACTIVE),                    \
+      kinds_pack(OPTK_XF_AFFINE, OPTK_SAG_NONE, OPTK_XF_IDENTITY, OPTK_MATERIAL_VACUUM, 0,       \
+                 OPTK_RULINGS_NONE, OPTK_SPACING_CONSTANT, OPTK_APERTURE_POLYGON,                \
+                 OPTK_XF_IDENTITY, OPTK_AP_ACTIVE),                                             \
+      kinds_pack(OPTK_XF_AFFINE, OPTK_SAG_NONE, OPTK_XF_IDENTITY, OPTK_MATERIAL_VACUUM, 0,       \
+                 OPTK_RULINGS_NONE, OPTK_SPACING_CONSTANT, OPTK_APERTURE_RECTANGLE,              \
+                 OPTK_XF_IDENTITY, OPTK_AP_ACTIVE)
+
 constexpr int SIGNATURE_MAX_SURFACES = 8;
 
 struct Signature {
@@ -63,11 +81,12 @@ struct Signature {
 };
 
 // index into the kernels' signature tables; the order of SIGNATURES
-enum { SIG_NEWTONIAN = 0, SIG_SPECTROGRAPH = 1, NUM_SIGNATURES = 2 };
+enum { SIG_NEWTONIAN = 0, SIG_SPECTROGRAPH = 1, SIG_SPECTROGRAPH_TILTED = 2, NUM_SIGNATURES = 3 };
 
 constexpr Signature SIGNATURES[NUM_SIGNATURES] = {
     {0, 4, {OPTK_SIGNATURE_NEWTONIAN}},
     {FEAT_ALL, 5, {OPTK_SIGNATURE_SPECTROGRAPH}},
+    {FEAT_ALL, 5, {OPTK_SIGNATURE_SPECTROGRAPH_TILTED}},
 };
 
 // the signature a system's records match (features: its kernel variant),

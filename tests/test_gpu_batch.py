@@ -140,3 +140,40 @@ def test_batch_rejects_bad_counts(batch_case):
     b = BatchedSystem(systems[:4])
     with pytest.raises(ValueError):
         b.trace(*ins, counts=counts[:4], precision="f32")
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_batch_specialised_kernel_equals_generic(batch_case, precision):
+    """Every instance of the tolerance batch keeps the tilted-spectrograph
+    kinds, so the batch runs the specialised persistent kernel; the generic
+    one (OPTK_TRACE_GENERIC) gives the same rays: identical in float64, to
+    float32 rounding in float32."""
+    from optika_amd import _trace
+
+    systems, counts, grid = batch_case
+    dtype = torch.float64 if precision == "f64" else torch.float32
+    print(f"case: batch specialised vs generic {precision}", flush=True)
+    sigs = {_trace.GridTracePlan(s, 1, precision="f64").cs.signature for s in systems[:4]}
+    assert sigs == {_trace.SIGNATURE_NAMES.index("tilted spectrograph")}
+    ins = _inputs(grid, dtype)
+    b = BatchedSystem(systems)
+    s_spec, spec = b.trace(*ins, counts=counts, precision=precision, outputs=True, graph=False)
+    s_gen, gen = b.trace(*ins, counts=counts, precision=precision, outputs=True, graph=False,
+                         generic=True)
+    torch.cuda.synchronize()
+    assert torch.equal(spec.t["unvignetted"], gen.t["unvignetted"])
+    for name in ("position", "direction"):
+        for k in range(3):
+            a, e = spec.t[name][k].double(), gen.t[name][k].double()
+            fin = torch.isfinite(e)
+            assert torch.equal(torch.isfinite(a), fin), f"{name}[{k}] finiteness"
+            if precision == "f64":
+                assert torch.equal(a[fin], e[fin]), f"{name}[{k}]"
+            else:
+                tol = (1e-5 if name == "position" else 1e-7) + 1e-7 * torch.abs(e[fin])
+                assert torch.all(torch.abs(a[fin] - e[fin]) <= tol), f"{name}[{k}]"
+    a, e = s_spec.cpu().numpy(), s_gen.cpu().numpy()
+    assert np.array_equal(a[..., 0], e[..., 0]), "ray counts"
+    if precision == "f64":  # the sums are accumulated with atomics: order may differ
+        ok = np.isclose(a, e, rtol=1e-9, atol=1e-9, equal_nan=True)
+        assert ok.all(), f"stats differ at {np.argwhere(~ok)[:5]}"

@@ -62,6 +62,15 @@ CASES = {
                                                               grating_radii=(0.001, -0.001),
                                                               spacing=(0.004, -0.003, 0.002))),
         lambda n: workloads.spectrograph_grid(n, seed=43)),
+    # primary and grating decentred and tilted: the primary's transformation
+    # is a full affine map, the tilted-spectrograph signature
+    "spectrograph_tilted": (
+        lambda: workloads.spectrograph(workloads.Perturbation(primary_decenter=(0.08, -0.05),
+                                                              primary_tilt=(0.02, -0.015),
+                                                              grating_decenter=(-0.04, 0.06),
+                                                              grating_tilt=(0.01, 0.012),
+                                                              primary_radius=-0.001)),
+        lambda n: workloads.spectrograph_grid(n, seed=45)),
 }
 
 
