@@ -683,6 +683,47 @@ class Surrogate:
                                                    _stream()))
         return out
 
+    def _fallback_plan(self, n: int, precision: str):
+        """A prepared sensor-frame trace of ``n`` rays through the system
+        with every multi-order grating restricted to the surrogate's order
+        path (one path: the other order is not part of the emulated rays).
+        The last plan is kept, so repeated calls with the same fallback
+        count allocate nothing."""
+        key = (n, precision)
+        if getattr(self, "_fb_key", None) != key:
+            from .stops import single_path
+
+            one = dataclasses.replace(self.system, surfaces=single_path(list(self.system.surfaces),
+                                                                         self.path))
+            self._fb_plan = None  # release the old buffers first
+            self._fb_plan = _trace.GridTracePlan(one, n, sensor_frame=True, precision=precision)
+            self._fb_key = key
+        return self._fb_plan
+
+    def infer_traced(self, fx, fy, px, py, wl, out=None, precision: str = "f32"):
+        """Fused inference with the fallback rays traced: the surrogate's
+        outputs for every ray, where the rays it flags carry the trace's
+        position, intensity and mask (``precision`` "f32" or "f64") instead
+        of the prediction.  One host read of the fallback count per call;
+        the flagged rays' inputs are gathered on the device, traced in one
+        launch and scattered back."""
+        out = self.infer(fx, fy, px, py, wl, out=out)
+        count = int(out["count"].item())
+        if count > out["list"].numel():  # the list overflowed: take the mask
+            idx = torch.nonzero(out["fallback"]).reshape(-1)
+        else:
+            idx = out["list"][:count].long()
+        if idx.numel() == 0:
+            return out
+        plan = self._fallback_plan(idx.numel(), precision)
+        sel = [t.index_select(0, idx).to(plan.dtype) for t in (fx, fy, px, py, wl)]
+        r = plan(*sel)
+        out["x"].index_copy_(0, idx, r.position.x.to(out["x"].dtype))
+        out["y"].index_copy_(0, idx, r.position.y.to(out["y"].dtype))
+        out["intensity"].index_copy_(0, idx, r.intensity.to(out["intensity"].dtype))
+        out["unvignetted"].index_copy_(0, idx, r.unvignetted)
+        return out
+
     def invalidate(self):
         """Forget the fused descriptor (after retraining or recalibrating)."""
         if hasattr(self, "_desc_cache"):

@@ -10,6 +10,7 @@ import torch
 
 from oracle import mlp as ref_mlp
 from optika_amd import surrogate, workloads
+from optika_amd.vectors import Cartesian2dVectorArray
 
 pytestmark = pytest.mark.gpu
 
@@ -221,6 +222,39 @@ def test_fused_inference_matches_layered_path(trained):
     assert count == int(out["fallback"].sum().item())
     lst = out["list"][:count].long()
     assert torch.all(out["fallback"][lst]) and torch.unique(lst).numel() == count
+
+
+def test_fused_inference_with_fallback_traced(trained):
+    """infer_traced: the rays the fused kernel flags carry the float32
+    trace of their own order (the same as order -1 of the two-order trace),
+    the others the fused prediction unchanged; twice in a row (the cached
+    fallback plan), the second time with fewer rays."""
+    sg = trained
+    for n, seed in ((100_000, 81), (60_001, 82)):
+        ins = [t.float().contiguous() for t in _eval_grid(n, seed=seed)]
+        print(f"case: infer_traced n={n}", flush=True)
+        plain = {k: v.clone() for k, v in sg.infer(*ins).items()}
+        out = sg.infer_traced(*ins, precision="f32")
+        torch.cuda.synchronize()
+        fb = plain["fallback"]
+        assert int(fb.sum().item()) > 0, "the case should flag some rays"
+        for k in ("x", "y", "intensity", "unvignetted"):
+            assert torch.equal(out[k][~fb], plain[k][~fb]), k
+        idx = torch.nonzero(fb).reshape(-1)
+        sel = [t.index_select(0, idx).contiguous() for t in ins]
+        r = sg.system.rayfunction(
+            wavelength=sel[4], field=Cartesian2dVectorArray(sel[0], sel[1]),
+            pupil=Cartesian2dVectorArray(sel[2], sel[3]), normalized_field=False,
+            normalized_pupil=False, precision="f32").outputs
+        r = r[sg.path] if r.position.x.ndim == 2 else r
+        assert torch.equal(out["unvignetted"][idx], r.unvignetted)
+        m = r.unvignetted
+        for k, e in (("x", r.position.x), ("y", r.position.y), ("intensity", r.intensity)):
+            a = out[k][idx][m].double()
+            e = e[m].double()
+            # the same float32 arithmetic; the bound only absorbs a different
+            # kernel variant's contraction (a wrong ray mapping is off by mm)
+            assert torch.all(torch.abs(a - e) <= 1e-5 + 1e-6 * torch.abs(e)), k
 
 
 @pytest.mark.parametrize("n", [1, 6400, 100_037])
