@@ -103,30 +103,37 @@ template <int IO> __device__ __forceinline__ FRay load_ray(const FGridIn &g, int
   return load_fray(g, i);
 }
 
+// a streaming store: the trace writes each output element once and never
+// reads it back, so the store is marked nontemporal and does not keep the
+// line in the caches (the Newtonian sensor-frame kernels measured 8 %
+// faster in float32 and 2-3 % in float64 with it; nontemporal loads of the
+// inputs gained nothing)
+template <typename T> __device__ __forceinline__ void put(T &dst, T v) { __builtin_nontemporal_store(v, &dst); }
+
 template <int IO> __device__ __forceinline__ void store_ray(const FRaysOut &o, int64_t j, const FRay &r) {
   if (IO == IO_FULL) {
-    o.p[0][j] = (float)r.px;
-    o.p[1][j] = (float)r.py;
-    o.p[2][j] = (float)r.pz;
-    o.d[0][j] = (float)r.dx;
-    o.d[1][j] = (float)r.dy;
-    o.d[2][j] = (float)r.dz;
-    o.wavelength[j] = r.wavelength;
-    o.intensity[j] = r.intensity;
-    o.attenuation[j] = r.attenuation;
-    o.index[j] = r.index;
-    o.unvignetted[j] = r.unvignetted ? 1 : 0;
+    put<float>(o.p[0][j], r.px);
+    put<float>(o.p[1][j], r.py);
+    put<float>(o.p[2][j], r.pz);
+    put<float>(o.d[0][j], r.dx);
+    put<float>(o.d[1][j], r.dy);
+    put<float>(o.d[2][j], r.dz);
+    put<float>(o.wavelength[j], r.wavelength);
+    put<float>(o.intensity[j], r.intensity);
+    put<float>(o.attenuation[j], r.attenuation);
+    put<float>(o.index[j], r.index);
+    put<uint8_t>(o.unvignetted[j], r.unvignetted ? 1 : 0);
     return;
   }
   if (IO == IO_SENSOR) {
-    o.p[0][j] = (float)r.px;
-    o.p[1][j] = (float)r.py;
-    o.p[2][j] = (float)r.pz;
-    o.d[0][j] = (float)r.dx;
-    o.d[1][j] = (float)r.dy;
-    o.d[2][j] = (float)r.dz;
-    o.intensity[j] = r.intensity;
-    o.unvignetted[j] = r.unvignetted ? 1 : 0;
+    put<float>(o.p[0][j], r.px);
+    put<float>(o.p[1][j], r.py);
+    put<float>(o.p[2][j], r.pz);
+    put<float>(o.d[0][j], r.dx);
+    put<float>(o.d[1][j], r.dy);
+    put<float>(o.d[2][j], r.dz);
+    put<float>(o.intensity[j], r.intensity);
+    put<uint8_t>(o.unvignetted[j], r.unvignetted ? 1 : 0);
     return;
   }
   store_f(o, j, r);
@@ -139,6 +146,7 @@ template <typename T> __device__ __forceinline__ T &at32(T *p, uint32_t i) {
   typedef typename std::conditional<std::is_const<T>::value, const char, char>::type Byte;
   return *reinterpret_cast<T *>(reinterpret_cast<Byte *>(p) + i * (uint32_t)sizeof(T));
 }
+
 
 // the sensor-frame layout's loads and stores at 32-bit offsets (the
 // specialised grid kernels; other layouts take the 64-bit forms)
@@ -158,14 +166,14 @@ template <int IO> __device__ __forceinline__ void store_ray32(const FRaysOut &o,
     // an earlier block as a 64-bit address, which the stores could then not
     // take as a scalar base plus a 32-bit offset)
     asm volatile("" : "+v"(j));
-    at32(o.p[0], j) = (float)r.px;
-    at32(o.p[1], j) = (float)r.py;
-    at32(o.p[2], j) = (float)r.pz;
-    at32(o.d[0], j) = (float)r.dx;
-    at32(o.d[1], j) = (float)r.dy;
-    at32(o.d[2], j) = (float)r.dz;
-    at32(o.intensity, j) = r.intensity;
-    at32(o.unvignetted, j) = r.unvignetted ? 1 : 0;
+    put<float>(at32(o.p[0], j), r.px);
+    put<float>(at32(o.p[1], j), r.py);
+    put<float>(at32(o.p[2], j), r.pz);
+    put<float>(at32(o.d[0], j), r.dx);
+    put<float>(at32(o.d[1], j), r.dy);
+    put<float>(at32(o.d[2], j), r.dz);
+    put<float>(at32(o.intensity, j), r.intensity);
+    put<uint8_t>(at32(o.unvignetted, j), r.unvignetted ? 1 : 0);
     return;
   }
   store_ray<IO>(o, (int64_t)j, r);

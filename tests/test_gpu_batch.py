@@ -177,3 +177,37 @@ def test_batch_specialised_kernel_equals_generic(batch_case, precision):
     if precision == "f64":  # the sums are accumulated with atomics: order may differ
         ok = np.isclose(a, e, rtol=1e-9, atol=1e-9, equal_nan=True)
         assert ok.all(), f"stats differ at {np.argwhere(~ok)[:5]}"
+
+
+def test_mixed_layout_batch_runs_generic_and_matches_single_traces():
+    """A batch whose instances do not share one surface-kind signature (the
+    nominal spectrograph, whose primary is only decentred, beside tilted
+    ones, whose primaries take a full affine map) runs the generic kernel:
+    every instance still equals its single-system float64 trace."""
+    from optika_amd import _trace
+
+    tilted = workloads.Perturbation(primary_tilt=(0.02, -0.01), grating_tilt=(0.01, 0.0),
+                                    primary_decenter=(0.05, 0.0))
+    systems = [workloads.spectrograph(), workloads.spectrograph(tilted), workloads.spectrograph()]
+    sigs = [_trace.GridTracePlan(s, 1, precision="f64").cs.signature for s in systems]
+    assert len(set(sigs)) == 2, sigs
+    counts = np.array([3000, 1500, 2200])
+    n = int(counts.sum())
+    grid = workloads.spectrograph_grid(n, seed=12)
+    print("case: mixed-layout batch f64", flush=True)
+    ins = _inputs(grid, torch.float64)
+    b = BatchedSystem(systems)
+    _, out = b.trace(*ins, counts=counts, precision="f64", outputs=True, graph=False)
+    torch.cuda.synchronize()
+    offsets = np.concatenate([[0], np.cumsum(counts)])
+    for i in range(len(systems)):
+        lo, hi = int(offsets[i]), int(offsets[i + 1])
+        ref = _single(systems[i], ins, lo, hi, "f64")
+        for p in range(b.num_paths):
+            r = ref[p]
+            assert torch.equal(out.t["unvignetted"][p * n + lo:p * n + hi], r.unvignetted)
+            for k, e in enumerate(r.position):
+                a = out.t["position"][k][p * n + lo:p * n + hi]
+                fin = torch.isfinite(e)
+                tol = 1e-12 * torch.clamp(torch.abs(e[fin]), min=1.0)
+                assert torch.all(torch.abs(a[fin] - e[fin]) <= tol), f"instance {i} path {p} pos {k}"
