@@ -63,9 +63,9 @@ def bytes_per_ray(precision: str, reads_wavelength: bool) -> int:
 # copy in the same run reads exactly 256 MiB by this count), writes =
 # WRITE_SIZE KiB (the copy writes 256 MiB by it); the runs and commands are
 # in the profile summaries named here
-TRAFFIC_BYTES_F64_2P24 = 4194304 * 128 + 1848 * 64 + 933888 * 1024
+TRAFFIC_BYTES_F64_2P24 = 4194304 * 128 + 1833 * 64 + 939217 * 1024
 TRAFFIC_SOURCE_F64 = "profiles/trace_grid_f64_newtonian_2p24.md (rocprofv3 --pmc)"
-TRAFFIC_BYTES_F32_2P24 = 2097152 * 128 + 1560 * 64 + 475136 * 1024
+TRAFFIC_BYTES_F32_2P24 = 2097152 * 128 + 1576 * 64 + 479621 * 1024
 TRAFFIC_SOURCE_F32 = "profiles/trace_grid_f32_newtonian_2p24.md (rocprofv3 --pmc)"
 
 PIXEL_MM = 0.015
