@@ -78,3 +78,54 @@ def test_wires_follow_the_reference_sampling():
         o = S.wire(ap, n)
         p = ap.wire(n)
         np.testing.assert_allclose(np.stack([p.x, p.y, p.z]), o, atol=1e-14)
+
+
+def test_angles_invert_the_direction_map():
+    """``angles`` (optika/_util.py:76-97) undoes ``optika.direction``
+    (d = (-cos y sin x, -sin y, cos y cos x)): round trips of random field
+    angles, and a hand value (10 deg about y alone gives (-sin 10, 0, cos 10))."""
+    rng = np.random.default_rng(5)
+    ax, ay = rng.uniform(-30, 30, 200), rng.uniform(-30, 30, 200)
+    d = np.stack(T.direction_from_angles(ax, ay))
+    back = S.angles(d)
+    np.testing.assert_allclose(back[0], ax, atol=1e-12)
+    np.testing.assert_allclose(back[1], ay, atol=1e-12)
+    t = np.deg2rad(10.0)
+    np.testing.assert_allclose(S.angles(np.array([[-np.sin(t)], [0.0], [np.cos(t)]]))[:, 0], [10.0, 0.0],
+                               atol=1e-12)
+    # a non-unit direction: only the y angle's arcsine divides by the length
+    np.testing.assert_allclose(S.angles(3.0 * d)[1], ay, atol=1e-12)
+
+
+def test_num_paths_counts_the_orders_between_the_stops():
+    from optika_amd import workloads
+
+    assert S.num_paths(newtonian()) == 1
+    # the spectrograph's grating (orders -1, +1) sits between its pupil stop
+    # (the primary) and its field stop
+    assert S.num_paths(workloads.spectrograph()) == 2
+
+
+def test_rayfunction_stops_walks_back_to_the_object_plane():
+    """The stop rays walked back to the object: on the object plane, unit
+    directions turned into the system (+z for the Newtonian, whose object
+    sits in front of the primary), every outline point's ray present."""
+    r = S.rayfunction_stops(newtonian(), [500.0], 21, 21)
+    p, d = r["position"], r["direction"]
+    assert p.shape[0] == 3 and d.shape == p.shape
+    assert np.all(np.abs(p[2]) < 1e-9)
+    np.testing.assert_allclose(np.sqrt((d * d).sum(0)), 1.0, atol=1e-12)
+    assert np.all(d[2] > 0)
+
+
+def test_denormalize_maps_the_unit_square_onto_the_extent():
+    """``denormalize`` (_sequential.py:777-785): -1 and +1 land on the
+    extent's ends, 0 on its middle, and the map is linear between."""
+    sys_ = newtonian()
+    (flo, fhi), (plo, phi) = S.field_pupil_extent(sys_, [500.0])
+    u = np.array([-1.0, 0.0, 0.5, 1.0])
+    fx, fy, px, py = S.denormalize(sys_, 500.0, u, u, u, u)
+    for got, lo, hi in ((fx, flo[0, 0], fhi[0, 0]), (fy, flo[1, 0], fhi[1, 0]),
+                        (px, plo[0, 0], phi[0, 0]), (py, plo[1, 0], phi[1, 0])):
+        np.testing.assert_allclose(got, [lo, (lo + hi) / 2, lo + 0.75 * (hi - lo), hi], rtol=1e-14,
+                                   atol=1e-14)
