@@ -55,18 +55,25 @@ __device__ __forceinline__ FRay load_fray(const FGridIn &g, int64_t i) {
                        ldf(g.intensity, i, g.intensity_default), g.object_at_infinity);
 }
 
+// a streaming store: the trace writes each output element once and never
+// reads it back, so the store is marked nontemporal and does not keep the
+// line in the caches (the Newtonian sensor-frame kernels measured 8 %
+// faster in float32 and 2-3 % in float64 with it; nontemporal loads of the
+// inputs gained nothing)
+template <typename T> __device__ __forceinline__ void put(T &dst, T v) { __builtin_nontemporal_store(v, &dst); }
+
 __device__ __forceinline__ void store_f(const FRaysOut &o, int64_t j, const FRay &r) {
-  if (o.p[0]) o.p[0][j] = (float)r.px;
-  if (o.p[1]) o.p[1][j] = (float)r.py;
-  if (o.p[2]) o.p[2][j] = (float)r.pz;
-  if (o.d[0]) o.d[0][j] = (float)r.dx;
-  if (o.d[1]) o.d[1][j] = (float)r.dy;
-  if (o.d[2]) o.d[2][j] = (float)r.dz;
-  if (o.wavelength) o.wavelength[j] = r.wavelength;
-  if (o.intensity) o.intensity[j] = r.intensity;
-  if (o.attenuation) o.attenuation[j] = r.attenuation;
-  if (o.index) o.index[j] = r.index;
-  if (o.unvignetted) o.unvignetted[j] = r.unvignetted ? 1 : 0;
+  if (o.p[0]) put<float>(o.p[0][j], r.px);
+  if (o.p[1]) put<float>(o.p[1][j], r.py);
+  if (o.p[2]) put<float>(o.p[2][j], r.pz);
+  if (o.d[0]) put<float>(o.d[0][j], r.dx);
+  if (o.d[1]) put<float>(o.d[1][j], r.dy);
+  if (o.d[2]) put<float>(o.d[2][j], r.dz);
+  if (o.wavelength) put<float>(o.wavelength[j], r.wavelength);
+  if (o.intensity) put<float>(o.intensity[j], r.intensity);
+  if (o.attenuation) put<float>(o.attenuation[j], r.attenuation);
+  if (o.index) put<float>(o.index[j], r.index);
+  if (o.unvignetted) put<uint8_t>(o.unvignetted[j], r.unvignetted ? 1 : 0);
 }
 
 // I/O layouts of the kernels.  IO_ANY honours every null pointer of the
@@ -102,13 +109,6 @@ template <int IO> __device__ __forceinline__ FRay load_ray(const FGridIn &g, int
                          g.wavelength[i], g.intensity_default, g.object_at_infinity);
   return load_fray(g, i);
 }
-
-// a streaming store: the trace writes each output element once and never
-// reads it back, so the store is marked nontemporal and does not keep the
-// line in the caches (the Newtonian sensor-frame kernels measured 8 %
-// faster in float32 and 2-3 % in float64 with it; nontemporal loads of the
-// inputs gained nothing)
-template <typename T> __device__ __forceinline__ void put(T &dst, T v) { __builtin_nontemporal_store(v, &dst); }
 
 template <int IO> __device__ __forceinline__ void store_ray(const FRaysOut &o, int64_t j, const FRay &r) {
   if (IO == IO_FULL) {

@@ -108,18 +108,25 @@ __device__ __forceinline__ double ld(const double *p, int64_t i, double dflt) {
   return p ? p[i] : dflt;
 }
 
+// a streaming store: the trace writes each output element once and never
+// reads it back, so the store is marked nontemporal and does not keep the
+// line in the caches (the Newtonian sensor-frame kernels measured 8 %
+// faster in float32 and 2-3 % in float64 with it; nontemporal loads of the
+// inputs gained nothing)
+template <typename T> __device__ __forceinline__ void put(T &dst, T v) { __builtin_nontemporal_store(v, &dst); }
+
 __device__ __forceinline__ void store(const RaysOut &o, int64_t j, const Ray<double> &r) {
-  if (o.p[0]) o.p[0][j] = r.px;
-  if (o.p[1]) o.p[1][j] = r.py;
-  if (o.p[2]) o.p[2][j] = r.pz;
-  if (o.d[0]) o.d[0][j] = r.dx;
-  if (o.d[1]) o.d[1][j] = r.dy;
-  if (o.d[2]) o.d[2][j] = r.dz;
-  if (o.wavelength) o.wavelength[j] = r.wavelength;
-  if (o.intensity) o.intensity[j] = r.intensity;
-  if (o.attenuation) o.attenuation[j] = r.attenuation;
-  if (o.index) o.index[j] = r.index;
-  if (o.unvignetted) o.unvignetted[j] = r.unvignetted ? 1 : 0;
+  if (o.p[0]) put<double>(o.p[0][j], r.px);
+  if (o.p[1]) put<double>(o.p[1][j], r.py);
+  if (o.p[2]) put<double>(o.p[2][j], r.pz);
+  if (o.d[0]) put<double>(o.d[0][j], r.dx);
+  if (o.d[1]) put<double>(o.d[1][j], r.dy);
+  if (o.d[2]) put<double>(o.d[2][j], r.dz);
+  if (o.wavelength) put<double>(o.wavelength[j], r.wavelength);
+  if (o.intensity) put<double>(o.intensity[j], r.intensity);
+  if (o.attenuation) put<double>(o.attenuation[j], r.attenuation);
+  if (o.index) put<double>(o.index[j], r.index);
+  if (o.unvignetted) put<uint8_t>(o.unvignetted[j], r.unvignetted ? 1 : 0);
 }
 
 __device__ __forceinline__ Ray<double> load_ray(const RaysIn &in, int64_t i) {
@@ -196,13 +203,6 @@ template <int IO> __device__ __forceinline__ Ray<double> load_io(const GridIn &g
                     g.intensity_default, g.object_at_infinity);
   return load_ray(g, i);
 }
-
-// a streaming store: the trace writes each output element once and never
-// reads it back, so the store is marked nontemporal and does not keep the
-// line in the caches (the Newtonian sensor-frame kernels measured 8 %
-// faster in float32 and 2-3 % in float64 with it; nontemporal loads of the
-// inputs gained nothing)
-template <typename T> __device__ __forceinline__ void put(T &dst, T v) { __builtin_nontemporal_store(v, &dst); }
 
 template <int IO> __device__ __forceinline__ void store_io(const RaysOut &o, int64_t j, const Ray<double> &r) {
   if (IO == IO_FULL) {
