@@ -255,6 +255,19 @@ def test_fused_inference_with_fallback_traced(trained):
             # the same float32 arithmetic; the bound only absorbs a different
             # kernel variant's contraction (a wrong ray mapping is off by mm)
             assert torch.all(torch.abs(a - e) <= 1e-5 + 1e-6 * torch.abs(e)), k
+    # a fallback list too short for the flagged rays: the mask is used instead
+    n = ins[0].numel()
+    small = {"x": torch.empty(n, dtype=torch.float32, device="cuda"),
+             "y": torch.empty(n, dtype=torch.float32, device="cuda"),
+             "intensity": torch.empty(n, dtype=torch.float32, device="cuda"),
+             "unvignetted": torch.empty(n, dtype=torch.bool, device="cuda"),
+             "fallback": torch.empty(n, dtype=torch.bool, device="cuda"),
+             "list": torch.empty(16, dtype=torch.int32, device="cuda"),
+             "count": torch.zeros(1, dtype=torch.int32, device="cuda")}
+    got = sg.infer_traced(*ins, out=small, precision="f32")
+    assert int(got["count"].item()) > 16
+    for k in ("x", "y", "intensity", "unvignetted"):
+        assert torch.equal(got[k], out[k]), f"overflowed list: {k}"
 
 
 @pytest.mark.parametrize("n", [1, 6400, 100_037])
