@@ -599,8 +599,7 @@ def surrogate_leg(args, steps: int):
     # and their results written over the predictions
     ms_sur, _ = time_calls(lambda: sg.infer_traced(*big, out=res, precision="f32"), steps, 1, stream)
     log(f"surrogate with the fallback rays traced: {ms_sur:.2f} ms for {n} rays")
-    sg._fb_plan = None  # the fallback plan's buffers
-    sg._fb_key = None
+    sg.release_fallback_plan()
     # float64 and float32 traces of the first 2^24 of the same rays
     m = min(n, 1 << 24)
     sub32 = [t[:m].contiguous() for t in big]

@@ -724,6 +724,11 @@ class Surrogate:
         out["unvignetted"].index_copy_(0, idx, r.unvignetted)
         return out
 
+    def release_fallback_plan(self):
+        """Free the buffers of the cached fallback trace plan."""
+        self._fb_plan = None
+        self._fb_key = None
+
     def invalidate(self):
         """Forget the fused descriptor (after retraining or recalibrating)."""
         if hasattr(self, "_desc_cache"):
