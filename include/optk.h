@@ -350,8 +350,10 @@ int32_t optk_eval(const optk_system *sys, int32_t op, const optk_eval_args *args
  * Rays with unvignetted == 0 or outside every bin are skipped.
  * image and moment are float64 device arrays of n_wavelength * nx * ny,
  * accumulated into (not cleared).  moment gets intensity * d_z.
- * A pixel range must satisfy num * (|lo| + |hi|) / (hi - lo) < 2^48 (the
- * device bin search is exact there); narrower ones are rejected (EINVAL).
+ * Any pixel range is binned exactly by numpy's edges: where num * (|lo| +
+ * |hi|) / (hi - lo) reaches 2^48 (a range narrow for its distance from the
+ * origin) the bins come from a binary search over the edges, in the one-pass
+ * kernel (no workspace used).
  * The workspace size depends on precision (set it before the query). */
 typedef struct optk_collect_args {
   const void *position[2];

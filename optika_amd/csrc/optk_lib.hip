@@ -818,8 +818,8 @@ struct CollectParams {
 // numpy.histogramdd's rule (right edge of the last bin inclusive); -1 if out.
 // The first guess multiplies by 1/step.  Its error, and the rounding of the
 // edges e_j = j * step + lo (e_num = hi) themselves, are a few ulp of
-// num * (|lo| + |hi|) / (hi - lo) bins, below one bin for the ranges
-// optk_collect accepts (that ratio < 2^48); two bounded steps each way then
+// num * (|lo| + |hi|) / (hi - lo) bins, below one bin while that ratio is
+// below 2^48 (the ranges this function serves); two bounded steps each way then
 // compare x with the exact edges, so the result is the bin the edges
 // define.  The steps compile to selects: no data-dependent loops (which
 // had cost the binning kernels their scalar registers).
@@ -836,6 +836,22 @@ __device__ __forceinline__ int linspace_bin(double x, double lo, double hi, int 
 #pragma unroll
   for (int r = 0; r < 2; ++r)
     if (k > 0 && edge(k) > x) --k;
+  return k;
+}
+
+// The same bin by binary search over the edges formed as numpy forms them
+// (e_j = j * step + lo for j < num, e_num = hi; non-decreasing, since each
+// of the two roundings is monotone): the largest j < num with e_j <= x.
+// Exact for every range, also those far from the origin for their width,
+// where the first guess above can miss by more than two bins (the edges may
+// even repeat there: zero-width bins, which numpy's rule skips the same way).
+__device__ __forceinline__ int linspace_bin_search(double x, double lo, double hi, int num,
+                                                   double step) {
+  if (!(x >= lo) || !(x <= hi)) return -1;
+  if (x == hi) return num - 1;
+  int k = 0;
+  for (int s = 1 << (31 - __clz(num)); s > 0; s >>= 1)
+    if (k + s < num && (double)(k + s) * step + lo <= x) k += s;
   return k;
 }
 
@@ -1247,6 +1263,7 @@ __global__ __launch_bounds__(256) void collect_tiles(const TiledParams T) {
 // then cost one atomic per pixel per wave instead of one per ray; when a
 // round finds fewer than 4 lanes in a bin the remaining lanes (scattered
 // rays) use their own atomics.
+template <bool SEARCH>
 __global__ __launch_bounds__(256) void collect_kernel(const CollectParams C) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -1268,8 +1285,10 @@ __global__ __launch_bounds__(256) void collect_kernel(const CollectParams C) {
       inten = C.intensityf[i];
     }
     int bw = edges_bin(w, C.wedges, C.nw);
-    int bx = linspace_bin(x, C.x_lo, C.x_hi, C.nx, C.x_step, C.x_inv);
-    int by = linspace_bin(y, C.y_lo, C.y_hi, C.ny, C.y_step, C.y_inv);
+    int bx = SEARCH ? linspace_bin_search(x, C.x_lo, C.x_hi, C.nx, C.x_step)
+                    : linspace_bin(x, C.x_lo, C.x_hi, C.nx, C.x_step, C.x_inv);
+    int by = SEARCH ? linspace_bin_search(y, C.y_lo, C.y_hi, C.ny, C.y_step)
+                    : linspace_bin(y, C.y_lo, C.y_hi, C.ny, C.y_step, C.y_inv);
     if (bw >= 0 && bx >= 0 && by >= 0) idx = ((int64_t)bw * C.nx + bx) * C.ny + by;
   }
   uint64_t pending = __ballot(idx >= 0);
@@ -1885,13 +1904,6 @@ static int32_t collect_validate(const optk_collect_args *a, int64_t n) {
       !a->wavelength_edges || !a->image || !a->moment)
     return fail(OPTK_EINVAL, "missing array");
   if (!(a->x_hi > a->x_lo) || !(a->y_hi > a->y_lo)) return fail(OPTK_EINVAL, "empty pixel range");
-  // the device bin search is exact while num * (|lo| + |hi|) / (hi - lo) < 2^48
-  auto narrow = [](double lo, double hi, int num) {
-    return !((double)num * (fabs(lo) + fabs(hi)) < 281474976710656.0 * (hi - lo));
-  };
-  if (narrow(a->x_lo, a->x_hi, a->nx) || narrow(a->y_lo, a->y_hi, a->ny))
-    return fail(OPTK_EINVAL, "pixel range too narrow for its distance from the origin "
-                             "(num * (|lo| + |hi|) / (hi - lo) must be below 2^48)");
   if (a->precision != OPTK_F64 && a->precision != OPTK_F32) return fail(OPTK_EINVAL, "bad precision");
   return OPTK_OK;
 }
@@ -1934,11 +1946,22 @@ static int32_t collect_launch(const optk_collect_args *a, int64_t n, void *ws, u
   C.x_inv = 1.0 / C.x_step;
   C.y_inv = 1.0 / C.y_step;
   const bool f32 = a->precision == OPTK_F32;
+  hipStream_t s = (hipStream_t)hip_stream;
+  // the first-guess bin search is exact while num * (|lo| + |hi|) / (hi - lo)
+  // < 2^48; a range beyond (narrow for its distance from the origin) is
+  // binned by the one-pass kernel with a binary search over the edges
+  auto narrow = [](double lo, double hi, int num) {
+    return !((double)num * (fabs(lo) + fabs(hi)) < 281474976710656.0 * (hi - lo));
+  };
+  if (narrow(C.x_lo, C.x_hi, C.nx) || narrow(C.y_lo, C.y_hi, C.ny)) {
+    hipLaunchKernelGGL(collect_kernel<true>, grid_for(n), dim3(256), 0, s, C);
+    OPTK_HIP_CHECK(hipGetLastError());
+    return OPTK_OK;
+  }
   TiledParams T;
   const uint64_t need = tiled_layout(a->n_wavelength, a->nx, a->ny, n, f32, ws, &T);
-  hipStream_t s = (hipStream_t)hip_stream;
   if (need == 0 || !ws || ws_bytes < need) {
-    hipLaunchKernelGGL(collect_kernel, grid_for(n), dim3(256), 0, s, C);
+    hipLaunchKernelGGL(collect_kernel<false>, grid_for(n), dim3(256), 0, s, C);
     OPTK_HIP_CHECK(hipGetLastError());
     return OPTK_OK;
   }

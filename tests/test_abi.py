@@ -150,14 +150,11 @@ def test_collect_arguments_are_rejected_without_a_device():
     a.image = a.moment = p
     a.n_wavelength, a.nx, a.ny, a.precision = 1, 2048, 2048, _lib.F64
     a.x_lo, a.x_hi, a.y_lo, a.y_hi = -15.36, 15.36, -15.36, 15.36
-    # an empty range, then a range so narrow for its offset from the origin
-    # that the device bin search could not guarantee numpy's bins
+    # an empty range (a range merely narrow for its offset from the origin is
+    # accepted: its bins come from a search over the edges)
     a.x_hi = a.x_lo
     assert L.optk_collect(ctypes.byref(a), 4, None) == -1
     assert b"empty pixel range" in L.optk_last_error()
-    a.x_lo, a.x_hi = 1e6, 1e6 + 1e-6
-    assert L.optk_collect(ctypes.byref(a), 4, None) == -1
-    assert b"too narrow" in L.optk_last_error()
     # zero rays: accepted without touching a device
     a.x_lo, a.x_hi = -15.36, 15.36
     assert L.optk_collect(ctypes.byref(a), 0, None) == 0

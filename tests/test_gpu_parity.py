@@ -410,3 +410,49 @@ def test_rays_that_miss_and_total_internal_reflection():
         assert_parity(rays_to_dict(out, n), ref, "edge")
     assert np.isinf(oracle.propagate([miss], r)[0]["position"]).any()
     assert np.isnan(oracle.propagate([exit_face], r)[0]["direction"]).any()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_collect_narrow_offset_range_matches_histogramdd(precision):
+    """Pixel ranges narrow for their distance from the origin (num * (|lo| +
+    |hi|) / (hi - lo) far above 2^48, where a first-guess bin can be off by
+    many bins and numpy's edges repeat) bin exactly as numpy.histogramdd with
+    numpy.linspace edges does: every ray in the same pixel (unit weights, so
+    the sums are exact counts), including rays on edges and on the last edge."""
+    from optika_amd import _collect
+
+    rng = np.random.default_rng(17)
+    nx, ny = 2048, 64
+    # ~1.4e17 by the ratio above: 2048 bins over ~260 distinct float64
+    # values, so most edges repeat; one multiply and two correction steps
+    # would put ~90 % of these rays in the wrong bin
+    x_lo, x_hi = 1e6, 1e6 + 3e-8
+    y_lo, y_hi = -3e7, -3e7 + 1.0  # ~4e9: the first guess would still do
+    n = 200_000
+    x_edges = np.linspace(x_lo, x_hi, nx + 1)
+    y_edges = np.linspace(y_lo, y_hi, ny + 1)
+    dt = np.float64 if precision == "f64" else np.float32
+    # points inside the ranges, plus exact edge values (representable in the
+    # rays' precision, so cast the edges first)
+    x = rng.uniform(x_lo - 1.5e-9, x_hi + 1.5e-9, n)
+    y = rng.uniform(y_lo - 0.01, y_hi + 0.01, n)
+    x[:4096] = rng.choice(x_edges, 4096)
+    y[:4096] = rng.choice(y_edges, 4096)
+    x, y = x.astype(dt).astype(np.float64), y.astype(dt).astype(np.float64)
+    wl = np.full(n, 550.0)
+    edges = np.array([400.0, 700.0])
+    t = lambda a: torch.as_tensor(a.astype(dt), device="cuda")  # noqa: E731
+    print(f"case: collect narrow offset range {precision}", flush=True)
+    rays = RayVectorArray(wavelength=t(wl), position=Cartesian3dVectorArray(t(x), t(y), t(np.zeros(n))),
+                          direction=Cartesian3dVectorArray(t(np.zeros(n)), t(np.zeros(n)), t(np.ones(n))),
+                          intensity=t(np.ones(n)))
+    for tiled in (True, False):
+        img, mom = _collect.histogram(rays, edges, (x_lo, x_hi), (y_lo, y_hi), nx, ny, tiled=tiled)
+        want, _ = np.histogramdd(np.stack([wl, x, y], 1), bins=(edges, x_edges, y_edges))
+        got = img.cpu().numpy()
+        if precision == "f64":
+            # most rays inside, over many pixels (float32 coordinates cannot
+            # resolve such a range: they collapse onto a few values, binned
+            # the same way all the same)
+            assert want.sum() > 0.8 * n and (want > 0).sum() > 5000
+        assert np.array_equal(got, want), f"tiled={tiled}: {np.abs(got - want).sum()} rays misbinned"
